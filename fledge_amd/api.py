@@ -1,0 +1,272 @@
+"""Python front end over the C ABI, with torch tensors as device buffers.
+
+Context        one HIP stream (torch's current stream by default)
+LBVH           flg_lbvh_build / flg_lbvh_build_batched / flg_lbvh_export
+trace_closest  flg_trace_closest (rays [n, 8] float32, hits [n, 2] int32/float32)
+camera_rays    flg_camera_rays
+Volume         flg_volume_create / flg_volpath_render / differentiable pair
+Scene helpers  numpy generators from libfledge_scene.so
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    def __init__(self, device=0, stream=None):
+        self.lib = N.hip()
+        torch.cuda.set_device(device)
+        if stream is None:
+            stream = torch.cuda.current_stream(device).cuda_stream
+        self.device = device
+        self.h = ctypes.c_void_p()
+        N.check(self.lib.flg_ctx_create(device, ctypes.c_void_p(stream), ctypes.byref(self.h)))
+
+    def check(self, st):
+        N.check(st, self.h)
+
+    def sync(self):
+        self.check(self.lib.flg_ctx_synchronize(self.h))
+
+    def profiling(self, on=True):
+        self.check(self.lib.flg_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def reset_profiling(self):
+        self.check(self.lib.flg_ctx_reset_profiling(self.h))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        st = self.lib.flg_ctx_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n))
+        if st != 0:
+            return 0.0, 0
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            self.lib.flg_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LBVH:
+    """A built radix-tree BVH (drop-in for RadixBVHBuilder / NaiveBVHAccel)."""
+
+    def __init__(self, ctx, handle, ntris, keep=()):
+        self.ctx = ctx
+        self.h = handle
+        self.ntris = ntris
+        self._keep = keep
+
+    @classmethod
+    def build(cls, ctx, verts, ind, compat=False):
+        verts = verts.contiguous()
+        ind = ind.contiguous()
+        assert verts.dtype == torch.float32 and ind.dtype == torch.int32 and verts.is_cuda
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.flg_lbvh_build(ctx.h, _ptr(verts), _ptr(ind), ind.numel() // 3,
+                                         verts.numel() // 3, 1 if compat else 0, ctypes.byref(h)))
+        return cls(ctx, h, ind.numel() // 3, keep=(verts, ind))
+
+    @classmethod
+    def build_batched(cls, ctx, meshes, compat=False):
+        n = len(meshes)
+        descs = (N.MeshDesc * n)()
+        keep = []
+        for i, (v, ind) in enumerate(meshes):
+            v = v.contiguous()
+            ind = ind.contiguous()
+            keep += [v, ind]
+            descs[i].d_verts = v.data_ptr()
+            descs[i].d_ind = ind.data_ptr()
+            descs[i].ntris = ind.numel() // 3
+            descs[i].nverts = v.numel() // 3
+        out = (ctypes.c_void_p * n)()
+        ctx.check(ctx.lib.flg_lbvh_build_batched(ctx.h, descs, n, 1 if compat else 0, out))
+        return [cls(ctx, ctypes.c_void_p(out[i]), descs[i].ntris, keep=tuple(keep)) for i in range(n)]
+
+    def info(self):
+        inf = N.LbvhInfo()
+        self.ctx.check(self.ctx.lib.flg_lbvh_get_info(self.h, ctypes.byref(inf)))
+        return inf
+
+    def export(self):
+        n = self.ntris
+        ni = max(n - 1, 0)
+        arrs = {
+            "morton": np.zeros(n, np.uint32), "morton_sorted": np.zeros(n, np.uint32),
+            "order": np.zeros(n, np.uint32), "split": np.zeros(ni, np.int32),
+            "range_lo": np.zeros(ni, np.int32), "range_hi": np.zeros(ni, np.int32),
+            "parent": np.zeros(ni, np.int32), "left_leaf": np.zeros(ni, np.uint8),
+            "right_leaf": np.zeros(ni, np.uint8), "node_bound": np.zeros((ni, 6), np.float32),
+            "leaf_parent": np.zeros(n, np.int32),
+        }
+        v = N.LbvhView()
+        for k, a in arrs.items():
+            setattr(v, "h_" + k, a.ctypes.data if a.size else None)
+        self.ctx.check(self.ctx.lib.flg_lbvh_export(self.ctx.h, self.h, ctypes.byref(v)))
+        arrs["bound"] = np.array(self.info().bound, np.float32)
+        return arrs
+
+    def trace(self, rays, hits=None):
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 2), dtype=torch.int32, device=rays.device)
+        self.ctx.check(self.ctx.lib.flg_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n, None))
+        return hits
+
+    def count(self, rays):
+        c = torch.zeros(2, dtype=torch.int64, device=rays.device)
+        self.ctx.check(self.ctx.lib.flg_trace_count(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(c)))
+        return c
+
+    def pathtrace(self, cam, width, height, spp, max_depth, record=False, trace_block=0,
+                  count_visits=False, use_graph=False):
+        cfg = N.PtCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
+        cfg.trace_block = trace_block
+        cfg.count_visits = 1 if count_visits else 0
+        cfg.use_graph = 1 if use_graph else 0
+        npaths = width * height * spp
+        dev = torch.device("cuda", self.ctx.device)
+        nseg = seg_id = seg_t = None
+        if record:
+            nseg = torch.zeros(npaths, dtype=torch.int32, device=dev)
+            seg_id = torch.full((npaths, max_depth + 1), -2, dtype=torch.int32, device=dev)
+            seg_t = torch.zeros((npaths, max_depth + 1), dtype=torch.float32, device=dev)
+        stats = (ctypes.c_uint64 * 4)()
+        self.ctx.check(self.ctx.lib.flg_pathtrace_diffuse(self.ctx.h, self.h, ctypes.byref(cam),
+                                                          ctypes.byref(cfg), _ptr(nseg), _ptr(seg_id),
+                                                          _ptr(seg_t), stats))
+        return {"segments": int(stats[0]), "paths": int(stats[1]), "node_visits": int(stats[2]),
+                "tri_tests": int(stats[3]), "nseg": nseg, "seg_id": seg_id, "seg_t": seg_t}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.flg_lbvh_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def camera_rays(ctx, cam, width, height, spp=1, sample=0, out=None):
+    if out is None:
+        out = torch.empty((width * height, 8), dtype=torch.float32, device=torch.device("cuda", ctx.device))
+    ctx.check(ctx.lib.flg_camera_rays(ctx.h, ctypes.byref(cam), width, height, spp, sample, _ptr(out)))
+    return out
+
+
+def hits_as_numpy(hits):
+    h = hits.cpu().numpy()
+    return h[:, 0].copy(), h[:, 1].view(np.float32).copy()
+
+
+class Volume:
+    def __init__(self, ctx, density, bounds, sigma_s, sigma_a, g):
+        self.ctx = ctx
+        density = density.contiguous()
+        assert density.dtype == torch.float32 and density.dim() == 3 and density.is_cuda
+        nz, ny, nx = density.shape
+        self.shape = (nz, ny, nx)
+        med = N.Medium(sigma_s, sigma_a, g)
+        b = (ctypes.c_float * 6)(*[float(x) for x in bounds])
+        self.h = ctypes.c_void_p()
+        ctx.check(ctx.lib.flg_volume_create(ctx.h, _ptr(density), nx, ny, nz, b, ctypes.byref(med),
+                                            ctypes.byref(self.h)))
+
+    def update(self, density):
+        density = density.contiguous()
+        self.ctx.check(self.ctx.lib.flg_volume_update(self.ctx.h, self.h, _ptr(density)))
+
+    def render(self, scene, width, height, spp, max_depth, out=None, stats=False):
+        cfg = N.RenderCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
+        st = (ctypes.c_uint64 * 2)()
+        self.ctx.check(self.ctx.lib.flg_volpath_render(self.ctx.h, self.h, ctypes.byref(scene),
+                                                       ctypes.byref(cfg), _ptr(out), st if stats else None))
+        if stats:
+            return out, {"paths": int(st[0]), "lookups": int(st[1])}
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.flg_volume_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def vol_scene(cam, point=(2.0, 2.0, -2.0), intensity=50.0, env=0.5, has_point=True, has_env=True):
+    s = N.VolScene()
+    s.camera = cam
+    s.has_point = 1 if has_point else 0
+    s.point_pos[:] = [float(x) for x in point]
+    s.point_intensity = float(intensity)
+    s.has_env = 1 if has_env else 0
+    s.env_radiance = float(env)
+    return s
+
+
+# ------------------------------------------------------------ generators ---
+def gen_soup(ntris, initstate=42, initseq=54):
+    L = N.scene()
+    v = np.zeros(ntris * 9, np.float32)
+    i = np.zeros(ntris * 3, np.int32)
+    L.flg_scene_soup(initstate, initseq, ntris, v.ctypes.data, i.ctypes.data)
+    return v, i
+
+
+def gen_octasphere(level, amplitude, seed=7, base_freq=2.0):
+    L = N.scene()
+    nv = ctypes.c_int64()
+    nt = L.flg_scene_octasphere_counts(level, ctypes.byref(nv))
+    v = np.zeros(nv.value * 3, np.float32)
+    i = np.zeros(nt * 3, np.int32)
+    got = L.flg_scene_octasphere(level, amplitude, seed, base_freq, v.ctypes.data, i.ctypes.data)
+    assert got == nv.value, (got, nv.value)
+    return v, i
+
+
+def gen_uv_sphere(n_theta, n_phi, radius):
+    L = N.scene()
+    nt = L.flg_scene_uv_sphere(n_theta, n_phi, radius, None, None)
+    v = np.zeros(n_theta * n_phi * 3, np.float32)
+    i = np.zeros(nt * 3, np.int32)
+    L.flg_scene_uv_sphere(n_theta, n_phi, radius, v.ctypes.data, i.ctypes.data)
+    return v, i
+
+
+def gen_fbm_grid(n, seed=1, base_freq=4.0, octaves=5, bias=0.5, scale=1.0):
+    L = N.scene()
+    g = np.zeros((n, n, n), np.float32)
+    L.flg_scene_fbm_grid(n, n, n, seed, base_freq, octaves, bias, scale, g.ctypes.data)
+    return g
+
+
+def gen_batch_sizes(nmeshes, lo=1024, hi=262144, initstate=42, initseq=56):
+    L = N.scene()
+    s = np.zeros(nmeshes, np.int32)
+    L.flg_scene_batch_sizes(initstate, initseq, nmeshes, lo, hi, s.ctypes.data)
+    return s

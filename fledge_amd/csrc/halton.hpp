@@ -1,0 +1,101 @@
+// Device Halton sampler: a pure function (pixel, sample, dimension) -> float
+// with the results of fledge's HaltonSampler (src/common/sampler.h:143-252)
+// under the per-sample schedule "sample s of pixel p uses global index
+// offset(p) + s * stride and dimensions 0, 1, 2, ...".
+//
+// Tables (primes of the linear sieve below 262144, sampler.h:39-57, and their
+// float reciprocals 1.0f / base) live in device memory, one copy per context.
+// The digit loop of RadicalInverse_ (sampler.h:59-72) divides by the prime
+// with a float reciprocal estimate and one exact integer correction, valid
+// for indices below 2^24 (checked by the host before launch).
+#pragma once
+#include "common.hpp"
+
+namespace flg {
+
+struct HaltonTables {
+  const int32_t* prime = nullptr;
+  const float* inv = nullptr;
+  int32_t nprimes = 0;
+  int32_t* overflow = nullptr;  // set to 1 when a dimension beyond the table is asked for
+};
+
+// Per-resolution constants of HaltonSampler's ctor (sampler.h:145-163).
+struct HaltonParams {
+  int32_t bs0, bs1, exp0, exp1;
+  uint32_t stride;
+  uint32_t mulinv0, mulinv1;
+};
+
+int get_halton_tables(flg_ctx* ctx, HaltonTables* out);
+void release_halton_tables(flg_ctx* ctx);
+HaltonParams make_halton_params(int resx, int resy);
+constexpr uint32_t kMaxHaltonIndex = 1u << 24;
+
+FLG_D float radical_inverse(int32_t base, float inv_base, uint32_t a) {
+  uint64_t reversed = 0;
+  float inv_base_n = 1.0f;
+  while (a) {
+    uint32_t q = (uint32_t)((float)a * inv_base);
+    int32_t r = (int32_t)(a - q * (uint32_t)base);
+    if (r < 0) {
+      q -= 1;
+      r += base;
+    } else if (r >= base) {
+      q += 1;
+      r -= base;
+    }
+    reversed = reversed * (uint64_t)base + (uint64_t)r;
+    inv_base_n *= inv_base;
+    a = q;
+  }
+  float rf = (reversed >> 32) ? (float)reversed : (float)(uint32_t)reversed;
+  return rf * inv_base_n;
+}
+
+// HaltonSampler::get1D (sampler.h:171-174) for global index `index` and
+// dimension `dim`.
+FLG_D float halton_get1d(const HaltonTables& t, uint32_t index, uint32_t dim, int32_t bs0,
+                         int32_t bs1) {
+  if (dim >= (uint32_t)t.nprimes) {
+    if (t.overflow) *t.overflow = 1;
+    return 0.5f;
+  }
+  uint32_t a = dim == 0 ? index / (uint32_t)bs0 : (dim == 1 ? index / (uint32_t)bs1 : index);
+  float v = radical_inverse(t.prime[dim], t.inv[dim], a);
+  return smin(v, ONE_MINUS_EPSILON);
+}
+
+// InverseRadicalInverse (sampler.h:111-120)
+FLG_HD uint64_t inverse_radical_inverse(uint32_t base, uint64_t inverse, int n_digits) {
+  uint64_t index = 0;
+  for (int i = 0; i < n_digits; ++i) {
+    uint64_t digit = inverse % base;
+    inverse /= base;
+    index = index * base + digit;
+  }
+  return index;
+}
+
+// getIndexForSample's pixel offset (sampler.h:206-227), Mod(p, 128).
+FLG_HD uint32_t halton_pixel_offset(const HaltonParams& h, int px, int py) {
+  if (h.stride <= 1) return 0;
+  int r0 = px - (px / 128) * 128, r1 = py - (py / 128) * 128;
+  uint64_t pm0 = (uint64_t)(r0 < 0 ? r0 + 128 : r0), pm1 = (uint64_t)(r1 < 0 ? r1 + 128 : r1);
+  uint64_t off = inverse_radical_inverse(2, pm0, h.exp0) * (uint64_t)(h.stride / (uint32_t)h.bs0) *
+                 (uint64_t)h.mulinv0;
+  off += inverse_radical_inverse(3, pm1, h.exp1) * (uint64_t)(h.stride / (uint32_t)h.bs1) *
+         (uint64_t)h.mulinv1;
+  return (uint32_t)(off % h.stride);
+}
+
+// A path's sampler state: global index plus the next dimension.
+struct PathSampler {
+  uint32_t index;
+  uint32_t dim;
+  FLG_D float next(const HaltonTables& t, const HaltonParams& h) {
+    return halton_get1d(t, index, dim++, h.bs0, h.bs1);
+  }
+};
+
+}  // namespace flg

@@ -1,0 +1,539 @@
+// Closest-hit traversal, primary rays and the wavefront diffuse path tracer.
+//
+// Traversal order and tie rule are those of RadixBVHBuilder::recursiveIntersect
+// (src/experimental/radix_bvh.cpp:202-245): depth first from the root, a
+// node's leaf children are tested first (left then right, an equal t
+// replaces the current hit), then the left internal subtree, then the right
+// one; a node is skipped when its box is missed or best < tnear.  Child boxes
+// are stored in the parent (64-byte node), so a child's box test runs when its
+// parent is visited; a child that would be skipped at its visit is not pushed
+// (best only decreases, so it would be skipped later too) and pushed entries
+// are re-checked against best when popped.  The result is therefore
+// identical to the recursive traversal.
+#include <algorithm>
+#include <cmath>
+
+#include "bvh.hpp"
+#include "halton.hpp"
+
+namespace flg {
+
+namespace {
+
+constexpr int kStack = 64;  // > tree depth: delta strictly grows per level and is < 64
+
+struct TraceGeom {
+  const BNode* nodes;
+  const BTri* tris;
+  int32_t n;
+  float root[6];
+};
+
+TraceGeom geom_of(const flg_bvh* b) {
+  TraceGeom g;
+  g.nodes = b->nodes;
+  g.tris = b->tris;
+  g.n = b->n;
+  for (int k = 0; k < 6; ++k) g.root[k] = b->bound[k];
+  return g;
+}
+
+struct Counters {
+  uint32_t nodes, tris;
+};
+
+template <bool COMPAT, bool COUNT>
+FLG_D void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, int32_t& hit,
+                    Counters& c) {
+  const BTri t = g.tris[p];
+  if (COUNT) ++c.tris;
+  float th;
+  V3 ng;
+  if (!pluecker(v3(t.v0.x, t.v0.y, t.v0.z), v3(t.v1.x, t.v1.y, t.v1.z), v3(t.v2.x, t.v2.y, t.v2.z),
+                o, d, COMPAT, &th, &ng))
+    return;
+  if (th > best) return;
+  best = th;
+  hit = (int32_t)p;  // sorted position; the caller maps it to the original id
+}
+
+// Returns the sorted position of the hit (-1 on a miss); best holds t.
+template <bool COMPAT, bool COUNT>
+FLG_D int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c) {
+  int32_t hit = -1;
+  if (g.n <= 0) return -1;
+  const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  float tn;
+  if (!slab(g.root[0], g.root[1], g.root[2], g.root[3], g.root[4], g.root[5], o, inv, &tn)) return -1;
+  if (best < tn) return -1;
+  if (g.n == 1) {
+    test_tri<COMPAT, COUNT>(g, 0, o, d, best, hit, c);
+    return hit;
+  }
+  int32_t stack[kStack];
+  float stackt[kStack];
+  int sp = 0;
+  int32_t node = 0;
+  // every node is visited at most once; the bound only guards a corrupt tree
+  for (int64_t visits = 0; visits < (int64_t)g.n; ++visits) {
+    if (COUNT) ++c.nodes;
+    const BNode nd = g.nodes[node];
+    const int32_t refL = nd.d.x, refR = nd.d.y;
+    if (refL < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refL & 0x7fffffffu, o, d, best, hit, c);
+    if (refR < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refR & 0x7fffffffu, o, d, best, hit, c);
+    float tL = 0.0f, tR = 0.0f;
+    bool goL = refL >= 0 && slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, &tL) &&
+               !(best < tL);
+    bool goR = refR >= 0 && slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, &tR) &&
+               !(best < tR);
+    if (goL) {
+      if (goR && sp < kStack) {
+        stack[sp] = refR;
+        stackt[sp] = tR;
+        ++sp;
+      }
+      node = refL;
+      continue;
+    }
+    if (goR) {
+      node = refR;
+      continue;
+    }
+    bool found = false;
+    while (sp > 0) {
+      --sp;
+      if (!(best < stackt[sp])) {
+        node = stack[sp];
+        found = true;
+        break;
+      }
+    }
+    if (!found) break;
+  }
+  return hit;
+}
+
+template <bool COMPAT>
+__global__ __launch_bounds__(256) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
+                                               int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4* r = reinterpret_cast<const float4*>(rays + i);
+  float4 a = r[0], b = r[1];
+  float best = a.w;
+  Counters c{0, 0};
+  int32_t p = trace_one<COMPAT, false>(g, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), best, c);
+  flg_hit h;
+  h.id = p >= 0 ? (int32_t)f2u(g.tris[p].v0.w) : -1;
+  h.t = p >= 0 ? best : a.w;
+  hits[i] = h;
+}
+
+template <bool COMPAT>
+__global__ __launch_bounds__(256) void k_trace_count(TraceGeom g, const flg_ray* rays, int64_t n,
+                                                     unsigned long long* counts) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  Counters c{0, 0};
+  if (i < n) {
+    const float4* r = reinterpret_cast<const float4*>(rays + i);
+    float4 a = r[0], b = r[1];
+    float best = a.w;
+    (void)trace_one<COMPAT, true>(g, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), best, c);
+  }
+  unsigned long long cn = c.nodes, ct = c.tris;
+  for (int o = 32; o >= 1; o >>= 1) {
+    cn += __shfl_xor(cn, o, 64);
+    ct += __shfl_xor(ct, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(counts, cn);
+    atomicAdd(counts + 1, ct);
+  }
+}
+
+// ----------------------------------------------------------------- camera --
+struct CamDev {
+  V3 pos, forward, right, up2;
+  float fov_ratio, aspect_fov;  // fov_ratio, fov_ratio * aspect
+  int32_t w, h;
+};
+
+// Camera ctor (src/common/camera.h:12-21) evaluated on the host in float.
+CamDev make_cam(const flg_camera* c, int w, int h) {
+  CamDev cd;
+  cd.pos = v3(c->pos[0], c->pos[1], c->pos[2]);
+  V3 look = v3(c->look_at[0], c->look_at[1], c->look_at[2]);
+  V3 up = v3(c->up[0], c->up[1], c->up[2]);
+  cd.forward = normalize(look - cd.pos);
+  cd.right = normalize(cross(up, cd.forward));
+  cd.up2 = normalize(cross(cd.forward, cd.right));
+  cd.fov_ratio = tanf(PI * c->fov_deg / 360.0f);
+  float aspect = (float)w / (float)h;
+  cd.aspect_fov = cd.fov_ratio * aspect;
+  cd.w = w;
+  cd.h = h;
+  return cd;
+}
+
+// Camera::generateRay (camera.h:21-31): the offsets are products with the
+// double literal 1.0, rounded back to float.
+FLG_D void gen_ray(const CamDev& c, float x, float y, V3* o, V3* d) {
+  float y_offset = (float)((double)c.fov_ratio * ((double)((y / (float)c.h) * 2.0f) - 1.0));
+  float x_offset = (float)((double)c.aspect_fov * ((double)((x / (float)c.w) * 2.0f) - 1.0));
+  V3 dir = (c.forward + c.right * x_offset) + c.up2 * y_offset;
+  *o = c.pos;
+  *d = normalize(dir);
+}
+
+__global__ __launch_bounds__(256) void k_camera(CamDev cam, HaltonTables tab, HaltonParams hp,
+                                                uint32_t sample, flg_ray* rays) {
+  int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= (int64_t)cam.w * cam.h) return;
+  int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
+  PathSampler ps;
+  ps.index = halton_pixel_offset(hp, x, y) + sample * hp.stride;
+  ps.dim = 0;
+  float ux = ps.next(tab, hp);
+  float uy = ps.next(tab, hp);
+  V3 o, d;
+  gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
+  float4* r = reinterpret_cast<float4*>(rays + pix);
+  r[0] = make_float4(o.x, o.y, o.z, __builtin_inff());
+  r[1] = make_float4(d.x, d.y, d.z, 0.0f);
+}
+
+// ------------------------------------------------------------ path tracer --
+// OffsetRayOrigin (src/common/ray.h:64-93)
+FLG_D V3 offset_ray_origin(V3 p, V3 n, V3 dir) {
+  const float origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f, int_scale = 256.0f;
+  V3 offset = n;
+  if (dot(n, dir) <= 0) offset = -n;
+  int oi0 = (int)(offset.x * int_scale), oi1 = (int)(offset.y * int_scale),
+      oi2 = (int)(offset.z * int_scale);
+  float px = u2f((uint32_t)((int32_t)f2u(p.x) + ((p.x < 0) ? -oi0 : oi0)));
+  float py = u2f((uint32_t)((int32_t)f2u(p.y) + ((p.y < 0) ? -oi1 : oi1)));
+  float pz = u2f((uint32_t)((int32_t)f2u(p.z) + ((p.z < 0) ? -oi2 : oi2)));
+  return v3(fabsf(p.x) < origin ? p.x + float_scale * offset.x : px,
+            fabsf(p.y) < origin ? p.y + float_scale * offset.y : py,
+            fabsf(p.z) < origin ? p.z + float_scale * offset.z : pz);
+}
+
+// ConcentricSampleDisk / CosineSampleHemisphere (math_utils.h:69-88)
+FLG_D V3 cosine_hemisphere(float u0, float u1) {
+  float ux = (float)((double)u0 * 2.0 - 1.0), uy = (float)((double)u1 * 2.0 - 1.0);
+  float dx = 0.0f, dy = 0.0f;
+  if (!(ux == 0 && uy == 0)) {
+    float theta, r;
+    if (fabsf(ux) > fabsf(uy)) {
+      r = ux;
+      theta = PI_OVER4 * (uy / ux);
+    } else {
+      r = uy;
+      theta = PI_OVER2 - (PI_OVER4 * (ux / uy));
+    }
+    float s, c;
+    flg_sincosf(theta, &s, &c);
+    dx = c * r;
+    dy = s * r;
+  }
+  float z = sqrtf(smax(0.0f, 1.0f - dx * dx - dy * dy));
+  return v3(dx, dy, z);
+}
+
+struct QRay {  // 32 bytes: origin + path id, direction + Halton index
+  float4 o;    // w: path id bits
+  float4 d;    // w: Halton index bits
+};
+
+struct PtState {
+  QRay* q[2];
+  int32_t* hit;   // per queue slot: sorted position or -1
+  float* hit_t;
+  uint32_t* count;  // [max_depth + 2] queue sizes
+  int32_t* nseg;
+  int32_t* seg_id;
+  float* seg_t;
+  int32_t stride_seg;
+};
+
+__global__ __launch_bounds__(256) void k_pt_zero(uint32_t* count, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) count[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_pt_gen(CamDev cam, HaltonTables tab, HaltonParams hp,
+                                                int32_t spp, QRay* q, uint32_t* count) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  if (p == 0) count[0] = (uint32_t)npaths;
+  if (p >= npaths) return;
+  int64_t pix = p / spp;
+  uint32_t s = (uint32_t)(p % spp);
+  int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
+  PathSampler ps;
+  ps.index = halton_pixel_offset(hp, x, y) + s * hp.stride;
+  ps.dim = 0;
+  float ux = ps.next(tab, hp);
+  float uy = ps.next(tab, hp);
+  V3 o, d;
+  gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
+  QRay r;
+  r.o = make_float4(o.x, o.y, o.z, u2f((uint32_t)p));
+  r.d = make_float4(d.x, d.y, d.z, u2f(ps.index));
+  q[p] = r;
+}
+
+template <bool COMPAT, bool COUNT>
+__global__ __launch_bounds__(256) void k_pt_trace(TraceGeom g, const QRay* q,
+                                                  const uint32_t* count, int bounce, int32_t* hit,
+                                                  float* hit_t, unsigned long long* visits) {
+  const uint32_t n = count[bounce];
+  Counters c{0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const QRay r = q[i];
+    float best = __builtin_inff();
+    int32_t p = trace_one<COMPAT, COUNT>(g, v3(r.o.x, r.o.y, r.o.z), v3(r.d.x, r.d.y, r.d.z), best, c);
+    hit[i] = p;
+    hit_t[i] = best;
+  }
+  if (COUNT) {
+    unsigned long long cn = c.nodes, ct = c.tris;
+    for (int o = 32; o >= 1; o >>= 1) {
+      cn += __shfl_xor(cn, o, 64);
+      ct += __shfl_xor(ct, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(visits, cn);
+      atomicAdd(visits + 1, ct);
+    }
+  }
+}
+
+template <bool COMPAT>
+void launch_pt_trace(bool count, int grid, int block, hipStream_t st, const TraceGeom& g,
+                     const QRay* q, const uint32_t* cnt, int b, int32_t* hit, float* hit_t,
+                     unsigned long long* visits) {
+  if (count)
+    k_pt_trace<COMPAT, true><<<grid, block, 0, st>>>(g, q, cnt, b, hit, hit_t, visits);
+  else
+    k_pt_trace<COMPAT, false><<<grid, block, 0, st>>>(g, q, cnt, b, hit, hit_t, visits);
+}
+
+// CoordinateTransition (src/materials/material_base.hpp:14-41)
+struct Frame {
+  V3 n, t, b;
+  FLG_D explicit Frame(V3 normal) {
+    n = normal / sqrtf(sqnorm(normal));
+    if (fabsf(n.x) > fabsf(n.z)) b = v3(-n.y, n.x, 0.0f);
+    else b = v3(0.0f, -n.z, n.y);
+    b = b / sqrtf(sqnorm(b));
+    t = cross(b, n);
+  }
+  FLG_D V3 to_local(V3 p) const { return v3(dot(p, t), dot(p, b), dot(p, n)); }
+  FLG_D V3 to_world(V3 p) const { return (t * p.x + b * p.y) + n * p.z; }
+};
+
+template <bool COMPAT>
+__global__ __launch_bounds__(256) void k_pt_shade(TraceGeom g, PtState s, HaltonTables tab,
+                                                  HaltonParams hp, int bounce, int max_depth) {
+  const uint32_t n = s.count[bounce];
+  const QRay* qin = s.q[bounce & 1];
+  QRay* qout = s.q[(bounce + 1) & 1];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool spawn = false;
+    QRay nr;
+    if (i < n) {
+      const QRay r = qin[i];
+      const uint32_t path = f2u(r.o.w);
+      const int32_t p = s.hit[i];
+      const float t = s.hit_t[i];
+      if (s.seg_id) {
+        int64_t k = (int64_t)path * s.stride_seg + bounce;
+        s.seg_id[k] = p >= 0 ? (int32_t)f2u(g.tris[p].v0.w) : -1;
+        s.seg_t[k] = t;
+      }
+      bool end = true;
+      if (p >= 0 && bounce < max_depth) {
+        const V3 o = v3(r.o.x, r.o.y, r.o.z), d = v3(r.d.x, r.d.y, r.d.z);
+        const BTri tri = g.tris[p];
+        const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
+                 v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
+        const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
+        const V3 hp3 = o + d * t;
+        Frame fr(ng);
+        PathSampler ps;
+        ps.index = f2u(r.d.w);
+        ps.dim = 2 + 2 * (uint32_t)bounce;
+        float u0 = ps.next(tab, hp);
+        float u1 = ps.next(tab, hp);
+        V3 wo_l = fr.to_local(-d);
+        V3 wi_l = cosine_hemisphere(u0, u1);
+        float pdf = wo_l.z * wi_l.z > 0 ? fabsf(wi_l.z) * INV_PI : 0.0f;
+        V3 wi = fr.to_world(wi_l);
+        if (pdf != 0.0f) {
+          V3 no = offset_ray_origin(hp3, ng, wi);
+          V3 nd = normalize(wi);
+          nr.o = make_float4(no.x, no.y, no.z, r.o.w);
+          nr.d = make_float4(nd.x, nd.y, nd.z, r.d.w);
+          spawn = true;
+          end = false;
+        }
+      }
+      if (end && s.nseg) s.nseg[path] = bounce + 1;
+    }
+    // wave-aggregated append to the next queue
+    uint64_t m = __ballot(spawn);
+    if (m) {
+      const int lane = threadIdx.x & 63;
+      uint32_t wbase = 0;
+      if (lane == 0) wbase = atomicAdd(&s.count[bounce + 1], (uint32_t)__popcll(m));
+      wbase = __shfl(wbase, 0, 64);
+      if (spawn) {
+        uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        qout[wbase + (uint32_t)__popcll(m & lt)] = nr;
+      }
+    }
+  }
+}
+
+}  // namespace
+
+}  // namespace flg
+
+using namespace flg;
+
+extern "C" {
+
+int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, flg_hit* d_hits,
+                      int64_t n, const flg_trace_cfg* cfg) {
+  if (!ctx || !bvh || n < 0 || (n > 0 && (!d_rays || !d_hits)))
+    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: bad arguments");
+  if (n == 0) return FLG_OK;
+  (void)cfg;
+  TraceGeom g = geom_of(bvh);
+  int grid = div_up(n, 256);
+  {
+    ScopedTimer tm(ctx, "trace_closest");
+    if (bvh->flags & FLG_NORMAL_COMPAT)
+      k_trace<true><<<grid, 256, 0, ctx->stream>>>(g, d_rays, d_hits, n);
+    else
+      k_trace<false><<<grid, 256, 0, ctx->stream>>>(g, d_rays, d_hits, n);
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  return FLG_OK;
+}
+
+int flg_trace_count(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, int64_t n,
+                    uint64_t* d_counts) {
+  if (!ctx || !bvh || n < 0 || !d_counts || (n > 0 && !d_rays))
+    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_count: bad arguments");
+  TraceGeom g = geom_of(bvh);
+  k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)d_counts, 4);
+  if (n == 0) return FLG_OK;
+  int grid = div_up(n, 256);
+  if (bvh->flags & FLG_NORMAL_COMPAT)
+    k_trace_count<true><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
+  else
+    k_trace_count<false><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
+  FLG_CHECK_LAUNCH(ctx);
+  return FLG_OK;
+}
+
+int flg_camera_rays(flg_ctx* ctx, const flg_camera* cam, int32_t width, int32_t height, int32_t spp,
+                    int32_t sample, flg_ray* d_rays) {
+  if (!ctx || !cam || width <= 0 || height <= 0 || spp <= 0 || sample < 0 || sample >= spp || !d_rays)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_camera_rays: bad arguments");
+  HaltonParams hp = make_halton_params(width, height);
+  if ((uint64_t)hp.stride * (uint64_t)spp > kMaxHaltonIndex)
+    return set_error(ctx, FLG_ERR_RANGE, "flg_camera_rays: spp * Halton stride must stay below 2^24");
+  HaltonTables tab;
+  int st = get_halton_tables(ctx, &tab);
+  if (st) return st;
+  CamDev cd = make_cam(cam, width, height);
+  int64_t npix = (int64_t)width * height;
+  {
+    ScopedTimer tm(ctx, "camera_rays");
+    k_camera<<<div_up(npix, 256), 256, 0, ctx->stream>>>(cd, tab, hp, (uint32_t)sample, d_rays);
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  return FLG_OK;
+}
+
+int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
+                          const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
+                          float* d_seg_t, uint64_t* h_stats) {
+  if (!ctx || !bvh || !cam || !cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
+      cfg->max_depth < 0 || cfg->max_depth > 63 || ((d_seg_id == nullptr) != (d_seg_t == nullptr)))
+    return set_error(ctx, FLG_ERR_INVALID, "flg_pathtrace_diffuse: bad arguments");
+  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+    return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
+  int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+  if (npaths >= (int64_t)1 << 31) return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: too many paths");
+  HaltonTables tab;
+  int st = get_halton_tables(ctx, &tab);
+  if (st) return st;
+  CamDev cd = make_cam(cam, cfg->width, cfg->height);
+  const int D = cfg->max_depth;
+  size_t qbytes = (size_t)npaths * sizeof(QRay);
+  char* sc = (char*)scratch(ctx, "pt_queues", 2 * qbytes + (size_t)npaths * 8 + 4096, &st);
+  if (!sc) return st;
+  PtState s;
+  s.q[0] = (QRay*)sc;
+  s.q[1] = (QRay*)(sc + qbytes);
+  s.hit = (int32_t*)(sc + 2 * qbytes);
+  s.hit_t = (float*)(sc + 2 * qbytes + (size_t)npaths * 4);
+  s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
+  unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
+  const bool count_visits = cfg->count_visits != 0;
+  s.nseg = d_nseg;
+  s.seg_id = d_seg_id;
+  s.seg_t = d_seg_t;
+  s.stride_seg = D + 1;
+  TraceGeom g = geom_of(bvh);
+  const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
+  const int tb = cfg->trace_block > 0 ? cfg->trace_block : 256;
+  const int pgrid = ctx->num_cus * 8;
+  {
+    ScopedTimer tm(ctx, "pathtrace_total");
+    k_pt_zero<<<1, 256, 0, ctx->stream>>>(s.count, D + 2);
+    if (count_visits) k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)visits, 4);
+    k_pt_gen<<<div_up(npaths, 256), 256, 0, ctx->stream>>>(cd, tab, hp, cfg->spp, s.q[0], s.count);
+    for (int b = 0; b <= D; ++b) {
+      {
+        ScopedTimer tt(ctx, "pathtrace_trace");
+        if (compat)
+          launch_pt_trace<true>(count_visits, pgrid, tb, ctx->stream, g, s.q[b & 1], s.count, b,
+                                s.hit, s.hit_t, visits);
+        else
+          launch_pt_trace<false>(count_visits, pgrid, tb, ctx->stream, g, s.q[b & 1], s.count, b,
+                                 s.hit, s.hit_t, visits);
+      }
+      if (compat)
+        k_pt_shade<true><<<pgrid, 256, 0, ctx->stream>>>(g, s, tab, hp, b, D);
+      else
+        k_pt_shade<false><<<pgrid, 256, 0, ctx->stream>>>(g, s, tab, hp, b, D);
+    }
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  if (h_stats) {
+    std::vector<uint32_t> cnt(D + 2);
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(cnt.data(), s.count, (D + 2) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t tot = 0;
+    for (int b = 0; b <= D; ++b) tot += cnt[b];
+    h_stats[0] = tot;
+    h_stats[1] = (uint64_t)npaths;
+    if (count_visits) {
+      unsigned long long v[2] = {0, 0};
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(v, visits, 16, hipMemcpyDeviceToHost, ctx->stream));
+      FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      h_stats[2] = v[0];
+      h_stats[3] = v[1];
+    }
+  }
+  return FLG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,507 @@
+// Volume-only path integrator on gfx950: SVolIntegrator::Li
+// (src/integrator.cpp:328-405) with delta tracking and ratio tracking
+// (src/volume.cpp:75-148), next-event estimation (integrator.cpp:41-120),
+// HG phase sampling, a constant environment light (light.cpp:93-159) and a
+// point light.  One thread per path sample (pixel, s); the per-pixel value is
+// the in-order float sum over s divided by spp, as in eval_pixel
+// (integrator.cpp:136-159) under the per-sample Halton schedule.
+#include <algorithm>
+#include <cmath>
+
+#include "volume.hpp"
+
+struct flg_volume {
+  flg_ctx* ctx = nullptr;
+  int32_t nx = 0, ny = 0, nz = 0;
+  float bounds[6];
+  flg_medium medium;
+  float* grid = nullptr;  // padded
+  float max_density = 0;
+  // environment-light tables, rebuilt when the radiance changes
+  float* env = nullptr;
+  float env_radiance = -1.0f;
+  float env_marg_int = 0;
+  float env_center[3] = {0, 0, 0};
+  float env_radius = 0;
+};
+
+namespace flg {
+
+namespace {
+
+constexpr int kEnvN = 512;
+// Free-flight steps per tracking call.  A Halton index of 0 (pixel offset 0,
+// sample 0) makes every dimension 0, so t never advances and the reference's
+// loops never end; the cap ends such a call (ratio tracking returns the
+// current transmittance, delta tracking reports no collision).
+constexpr int kMaxTrackSteps = 4096;
+
+__global__ __launch_bounds__(256) void k_pad(const float* src, float* dst, int nx, int ny, int nz) {
+  const int px = nx + 2, py = ny + 2, pz = nz + 2;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t tot = (int64_t)px * py * pz;
+  if (i >= tot) return;
+  int x = (int)(i % px), y = (int)((i / px) % py), z = (int)(i / ((int64_t)px * py));
+  float v = 0.0f;
+  if (x >= 1 && x <= nx && y >= 1 && y <= ny && z >= 1 && z <= nz)
+    v = src[((int64_t)(z - 1) * ny + (y - 1)) * nx + (x - 1)];
+  dst[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_max(const float* src, int64_t n, float* out) {
+  __shared__ float red[4];
+  float m = -3.402823466e+38f;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = smax(m, src[i]);
+  for (int o = 32; o >= 1; o >>= 1) m = smax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float r = smax(smax(red[0], red[1]), smax(red[2], red[3]));
+    out[blockIdx.x] = r;
+  }
+}
+
+// Dist1D ctor (distribution.hpp:14-22)
+void dist_init(const float* f, int n, float* cdf, float* integral) {
+  cdf[0] = 0.0f;
+  for (int i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + f[i - 1] / (float)n;
+  *integral = cdf[n];
+  if (*integral == 0.0f)
+    for (int i = 1; i < n + 1; ++i) cdf[i] = (float)i / (float)n;
+  else
+    for (int i = 1; i < n + 1; ++i) cdf[i] /= *integral;
+}
+
+// InfiniteAreaLight(texture) with a ConstTexture (light.cpp:93-108) and
+// preprocess (light.cpp:110-117) over the scene bound (= the volume box).
+int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
+  if (vol->env && vol->env_radiance == radiance) return FLG_OK;
+  const int NU = kEnvN, NV = kEnvN;
+  std::vector<float> host((size_t)513 + 512 + 512 * 513 + 512 * 512 + 512, 0.0f);
+  float* marg_cdf = host.data();
+  float* marg_f = marg_cdf + 513;
+  float* row_cdf = marg_f + 512;
+  float* row_f = row_cdf + 512 * 513;
+  float* row_int = row_f + 512 * 512;
+  float nrm = sqrtf(radiance * radiance + radiance * radiance + radiance * radiance);
+  for (int v = 0; v < NV; ++v) {
+    float sin_theta = sinf(PI * (float)(v + 0.5f) / (float)NV);
+    for (int u = 0; u < NU; ++u) {
+      float f = nrm;
+      f *= sin_theta;
+      row_f[v * NU + u] = f;
+    }
+    dist_init(row_f + v * NU, NU, row_cdf + v * 513, &row_int[v]);
+    marg_f[v] = row_int[v];
+  }
+  float marg_int;
+  dist_init(marg_f, NV, marg_cdf, &marg_int);
+  if (!vol->env) FLG_CHECK_HIP(ctx, hipMalloc(&vol->env, host.size() * 4));
+  FLG_CHECK_HIP(ctx, hipMemcpyAsync(vol->env, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  vol->env_radiance = radiance;
+  vol->env_marg_int = marg_int;
+  // AABB::boundSphere (aabb.h:67-70), radius * 1000 capped at FLT_MAX / 2
+  V3 lo = v3(vol->bounds[0], vol->bounds[1], vol->bounds[2]);
+  V3 hi = v3(vol->bounds[3], vol->bounds[4], vol->bounds[5]);
+  V3 c = (lo + hi) / 2.0f;
+  V3 dd = hi - c;
+  float r = sqrtf(sqnorm(dd));
+  r *= 1000.0f;
+  r = smin(3.402823466e+38f / 2, r);
+  vol->env_center[0] = c.x;
+  vol->env_center[1] = c.y;
+  vol->env_center[2] = c.z;
+  vol->env_radius = r;
+  return FLG_OK;
+}
+
+// Ratio tracking with Russian roulette (OpenVDBVolume::tr, volume.cpp:75-107)
+template <bool COUNT>
+FLG_D float vol_tr(const VolDev& v, const HaltonTables& tab, const HaltonParams& hp,
+                   PathSampler& ps, V3 o, V3 d, float ray_tmax, uint32_t& nlook) {
+  float t_min, t_max;
+  if (!box_pbrt(v.bmin, v.bmax, o, d, ray_tmax, &t_min, &t_max)) return 1.0f;
+  float tr = 1.0f, t = smax(t_min, 0.0f);
+  t_max = smin(t_max, ray_tmax);
+  const float max_mu = v.max_mu;
+  if (!(max_mu > 0.0f)) return 1.0f;  // empty medium: no interaction
+  for (int step = 0; step < kMaxTrackSteps; ++step) {
+    t += -flg_logf(1.0f - ps.next(tab, hp)) / max_mu;
+    if (t >= t_max) break;
+    V3 pos = o + d * t;
+    float density = lookup(v, pos).rho;
+    if (COUNT) ++nlook;
+    tr *= 1.0f - smax(0.0f, density * v.inv_max_density);
+    if (tr < 0.1f) {
+      float q = smax(0.05f, 1.0f - tr);
+      if (ps.next(tab, hp) < q) return 0.0f;
+      tr /= 1.0f - q;
+    }
+  }
+  return tr;
+}
+
+// Delta tracking (OpenVDBVolume::sample, volume.cpp:109-148)
+template <bool COUNT>
+FLG_D float vol_sample(const VolDev& v, const HaltonTables& tab, const HaltonParams& hp,
+                       PathSampler& ps, V3 o, V3 d, bool* success, V3* pos_out,
+                       uint32_t& nlook) {
+  float t_min, t_max;
+  if (!box_pbrt(v.bmin, v.bmax, o, d, __builtin_inff(), &t_min, &t_max)) {
+    *success = false;
+    return 1.0f;
+  }
+  float t = smax(t_min, 0.0f);
+  const float max_mu = v.max_mu;
+  t_max = smin(t_max, __builtin_inff());
+  if (!(max_mu > 0.0f)) {
+    *success = false;
+    return 1.0f;
+  }
+  for (int step = 0; step < kMaxTrackSteps; ++step) {
+    t += -flg_logf(1.0f - ps.next(tab, hp)) / max_mu;
+    if (t >= t_max) break;
+    V3 pos = o + d * t;
+    float density = lookup(v, pos).rho;
+    if (COUNT) ++nlook;
+    float p_s = v.sigma_s * density / max_mu;
+    float p_a = v.sigma_a * density / max_mu;
+    float p_n = 1.0f - p_s - p_a;
+    if (ps.next(tab, hp) > p_n) {
+      *success = true;
+      *pos_out = pos;
+      return v.sigma_s / v.sigma_t;
+    }
+  }
+  *success = false;
+  return 1.0f;
+}
+
+// UniformSampleOneLight + EstimateDirect for a volume vertex
+// (integrator.cpp:69-120); lights ordered [point, environment], p = 1/n.
+template <bool COUNT>
+FLG_D float sample_one_light(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                             const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo,
+                             uint32_t& nlook) {
+  const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
+  if (n_lights == 0) return 0.0f;
+  int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
+  if (light_num > n_lights - 1) light_num = n_lights - 1;
+  const float light_pdf = (float)(1.0 / (double)n_lights);
+  const bool use_point = L.has_point && light_num == 0;
+  float u0 = ps.next(tab, hp);
+  float u1 = ps.next(tab, hp);
+  V3 wi, p_light;
+  float pdf, Li;
+  if (use_point) {
+    V3 diff = L.point_pos - p;
+    float d2 = sqnorm(diff);
+    wi = normalize(diff);
+    Li = L.point_intensity / d2;
+    pdf = 1.0f;
+    p_light = L.point_pos;
+  } else {
+    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, &p_light);
+  }
+  float out = 0.0f;
+  if (pdf == 0.0f || Li == 0.0f) return out / light_pdf;
+  float f = hg_p(wi, wo, v.g);
+  if (f != 0.0f) {
+    V3 diff = p_light - p;
+    float nrm = sqrtf(sqnorm(diff));
+    V3 d = diff / nrm;
+    float tr = vol_tr<COUNT>(v, tab, hp, ps, p, d, nrm - SHADOW_EPS, nlook);
+    Li *= tr;
+    out += f * Li / pdf;
+  }
+  return out / light_pdf;
+}
+
+// SVolIntegrator::Li (integrator.cpp:328-405), m_rrThreshold = 0.01.
+template <bool COUNT>
+FLG_D float svol_li(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                    const HaltonParams& hp, PathSampler& ps, V3 o, V3 d, int max_depth,
+                    uint32_t& nlook) {
+  float Lr = 0.0f, beta = 1.0f;
+  const float rr = 0.01f;
+  float t_min, t_max;
+  if (!box_enter(v.bmin, v.bmax, o, d, &t_min, &t_max)) {
+    if (L.env.present) Lr += beta * L.env.radiance;
+    return Lr;
+  }
+  {
+    float tr = vol_tr<COUNT>(v, tab, hp, ps, o, d, __builtin_inff(), nlook);
+    if (L.env.present) Lr += tr * L.env.radiance;
+    o = o + d * t_min;
+  }
+  for (int bounces = 1;; ++bounces) {
+    bool success = false;
+    V3 pos;
+    float f = vol_sample<COUNT>(v, tab, hp, ps, o, d, &success, &pos, nlook);
+    if (!success) break;
+    beta = beta * f;
+    V3 wo = -d;
+    float u = ps.next(tab, hp);
+    float vv = ps.next(tab, hp);
+    V3 wi = hg_sample(wo, u, vv, v.g);
+    Lr += beta * sample_one_light<COUNT>(v, L, tab, hp, ps, pos, wo, nlook);
+    if (bounces >= max_depth || beta * beta + beta * beta + beta * beta == 0.0f) break;
+    o = pos;
+    d = normalize(wi);
+    if (bounces > 10) {
+      if (ps.next(tab, hp) < rr) break;
+      beta /= (1.0f - rr);
+    }
+  }
+  return Lr;
+}
+
+struct CamDevV {
+  V3 pos, forward, right, up2;
+  float fov_ratio, aspect_fov;
+  int32_t w, h;
+};
+
+FLG_D void gen_ray_v(const CamDevV& c, float x, float y, V3* o, V3* d) {
+  float y_offset = (float)((double)c.fov_ratio * ((double)((y / (float)c.h) * 2.0f) - 1.0));
+  float x_offset = (float)((double)c.aspect_fov * ((double)((x / (float)c.w) * 2.0f) - 1.0));
+  V3 dir = (c.forward + c.right * x_offset) + c.up2 * y_offset;
+  *o = c.pos;
+  *d = normalize(dir);
+}
+
+CamDevV make_cam_v(const flg_camera* c, int w, int h) {
+  CamDevV cd;
+  cd.pos = v3(c->pos[0], c->pos[1], c->pos[2]);
+  V3 look = v3(c->look_at[0], c->look_at[1], c->look_at[2]);
+  V3 up = v3(c->up[0], c->up[1], c->up[2]);
+  cd.forward = normalize(look - cd.pos);
+  cd.right = normalize(cross(up, cd.forward));
+  cd.up2 = normalize(cross(cd.forward, cd.right));
+  cd.fov_ratio = tanf(PI * c->fov_deg / 360.0f);
+  float aspect = (float)w / (float)h;
+  cd.aspect_fov = cd.fov_ratio * aspect;
+  cd.w = w;
+  cd.h = h;
+  return cd;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab,
+                                              HaltonParams hp, CamDevV cam, int32_t spp,
+                                              int32_t max_depth, float* lpath,
+                                              unsigned long long* nlookups) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  uint32_t nlook = 0;
+  if (p < npaths) {
+    int64_t pix = p / spp;
+    uint32_t s = (uint32_t)(p % spp);
+    int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
+    PathSampler ps;
+    ps.index = halton_pixel_offset(hp, x, y) + s * hp.stride;
+    ps.dim = 0;
+    float ux = ps.next(tab, hp);
+    float uy = ps.next(tab, hp);
+    V3 o, d;
+    gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
+    lpath[p] = svol_li<COUNT>(v, L, tab, hp, ps, o, d, max_depth, nlook);
+  }
+  if (COUNT) {
+    unsigned long long c = nlook;
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(nlookups, c);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pixel_mean(const float* lpath, int64_t npix, int32_t spp,
+                                                    float* rgb) {
+  int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= npix) return;
+  float c = 0.0f;
+  const float* l = lpath + pix * spp;
+  for (int s = 0; s < spp; ++s) c += l[s];
+  c = c / (float)spp;
+  rgb[3 * pix] = c;
+  rgb[3 * pix + 1] = c;
+  rgb[3 * pix + 2] = c;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace
+
+VolDev make_voldev(const flg_volume* vol) {
+  VolDev v;
+  v.grid = vol->grid;
+  v.nx = vol->nx;
+  v.ny = vol->ny;
+  v.nz = vol->nz;
+  v.px = vol->nx + 2;
+  v.pxy = (vol->nx + 2) * (vol->ny + 2);
+  v.bmin = v3(vol->bounds[0], vol->bounds[1], vol->bounds[2]);
+  v.bmax = v3(vol->bounds[3], vol->bounds[4], vol->bounds[5]);
+  v.scale = v3((float)vol->nx / (v.bmax.x - v.bmin.x), (float)vol->ny / (v.bmax.y - v.bmin.y),
+               (float)vol->nz / (v.bmax.z - v.bmin.z));
+  // OpenVDBVolume ctor (volume.cpp:33-59): sigma_t = sigma_a + sigma_s, the
+  // inverse max density through a double division.
+  v.sigma_s = vol->medium.sigma_s;
+  v.sigma_a = vol->medium.sigma_a;
+  v.sigma_t = v.sigma_a + v.sigma_s;
+  v.g = vol->medium.g;
+  v.inv_max_density = (float)(1.0 / (double)vol->max_density);
+  v.max_mu = v.sigma_t * vol->max_density;
+  return v;
+}
+
+int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L) {
+  *L = LightsDev{};
+  L->has_point = sc->has_point != 0;
+  L->point_pos = v3(sc->point_pos[0], sc->point_pos[1], sc->point_pos[2]);
+  L->point_intensity = sc->point_intensity;
+  L->env.present = sc->has_env != 0;
+  if (L->env.present) {
+    int st = build_env(ctx, vol, sc->env_radiance);
+    if (st) return st;
+    const float* e = vol->env;
+    L->env.marg_cdf = e;
+    L->env.marg_f = e + 513;
+    L->env.row_cdf = e + 513 + 512;
+    L->env.row_f = e + 513 + 512 + 512 * 513;
+    L->env.row_int = e + 513 + 512 + 512 * 513 + 512 * 512;
+    L->env.marg_int = vol->env_marg_int;
+    L->env.radiance = sc->env_radiance;
+    L->env.center = v3(vol->env_center[0], vol->env_center[1], vol->env_center[2]);
+    L->env.radius = vol->env_radius;
+  }
+  return FLG_OK;
+}
+
+int refresh_max(flg_ctx* ctx, flg_volume* vol, const float* d_density) {
+  int st = FLG_OK;
+  const int64_t tot = (int64_t)vol->nx * vol->ny * vol->nz;
+  const int blocks = 512;
+  float* part = (float*)scratch(ctx, "vol_max", blocks * 4, &st);
+  if (!part) return st;
+  k_max<<<blocks, 256, 0, ctx->stream>>>(d_density, tot, part);
+  FLG_CHECK_LAUNCH(ctx);
+  std::vector<float> h(blocks);
+  FLG_CHECK_HIP(ctx, hipMemcpyAsync(h.data(), part, blocks * 4, hipMemcpyDeviceToHost, ctx->stream));
+  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  float m = -3.402823466e+38f;
+  for (float x : h) m = smax(m, x);
+  vol->max_density = m;
+  return FLG_OK;
+}
+
+}  // namespace flg
+
+using namespace flg;
+
+extern "C" {
+
+int flg_volume_create(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz,
+                      const float* h_bounds, const flg_medium* medium, flg_volume** out) {
+  if (!ctx || !d_density || !h_bounds || !medium || !out || nx <= 0 || ny <= 0 || nz <= 0)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: bad arguments");
+  *out = nullptr;
+  if ((int64_t)(nx + 2) * (ny + 2) * (nz + 2) >= ((int64_t)1 << 31))
+    return set_error(ctx, FLG_ERR_RANGE, "flg_volume_create: grid too large");
+  for (int k = 0; k < 3; ++k)
+    if (!(h_bounds[3 + k] > h_bounds[k])) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: empty bounds");
+  FLG_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+  flg_volume* vol = new flg_volume;
+  vol->ctx = ctx;
+  vol->nx = nx;
+  vol->ny = ny;
+  vol->nz = nz;
+  for (int k = 0; k < 6; ++k) vol->bounds[k] = h_bounds[k];
+  vol->medium = *medium;
+  const int64_t tot = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+  hipError_t e = hipMalloc(&vol->grid, tot * 4);
+  if (e != hipSuccess) {
+    delete vol;
+    return check_hip(ctx, e, "hipMalloc(volume grid)");
+  }
+  *out = vol;
+  int st = flg_volume_update(ctx, vol, d_density);
+  if (st) {
+    flg_volume_destroy(vol);
+    *out = nullptr;
+  }
+  return st;
+}
+
+int flg_volume_update(flg_ctx* ctx, flg_volume* vol, const float* d_density) {
+  if (!ctx || !vol || !d_density) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_update: bad arguments");
+  const int64_t tot = (int64_t)(vol->nx + 2) * (vol->ny + 2) * (vol->nz + 2);
+  k_pad<<<div_up(tot, 256), 256, 0, ctx->stream>>>(d_density, vol->grid, vol->nx, vol->ny, vol->nz);
+  FLG_CHECK_LAUNCH(ctx);
+  return refresh_max(ctx, vol, d_density);
+}
+
+int flg_volume_destroy(flg_volume* vol) {
+  if (!vol) return FLG_ERR_INVALID;
+  (void)hipSetDevice(vol->ctx->device);
+  (void)hipStreamSynchronize(vol->ctx->stream);
+  (void)hipFree(vol->grid);
+  if (vol->env) (void)hipFree(vol->env);
+  delete vol;
+  return FLG_OK;
+}
+
+int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                       const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats) {
+  if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 ||
+      cfg->spp <= 0 || cfg->max_depth < 1)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
+  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
+  const int64_t npix = (int64_t)cfg->width * cfg->height;
+  const int64_t npaths = npix * cfg->spp;
+  HaltonTables tab;
+  int st = get_halton_tables(ctx, &tab);
+  if (st) return st;
+  LightsDev L;
+  if ((st = make_lights(ctx, vol, scene, &L))) return st;
+  VolDev v = make_voldev(vol);
+  CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
+  char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * 4 + 256, &st);
+  if (!sc) return st;
+  float* lpath = (float*)(sc + 256);
+  unsigned long long* nl = (unsigned long long*)sc;
+  const bool count = h_stats != nullptr;
+  {
+    ScopedTimer tm(ctx, "volpath_paths");
+    if (count) {
+      k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 1);
+      k_svol<true><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
+                                                                 cfg->max_depth, lpath, nl);
+    } else {
+      k_svol<false><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
+                                                                  cfg->max_depth, lpath, nl);
+    }
+  }
+  {
+    ScopedTimer tm(ctx, "volpath_mean");
+    k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  if (h_stats) {
+    unsigned long long c = 0;
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    h_stats[0] = (uint64_t)npaths;
+    h_stats[1] = c;
+  }
+  return FLG_OK;
+}
+
+}  // extern "C"

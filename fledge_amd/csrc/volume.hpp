@@ -1,0 +1,193 @@
+// Device description of the dense density volume and the per-sample helpers
+// of the volume-only integrator (shared by the forward and adjoint kernels).
+#pragma once
+#include "common.hpp"
+#include "halton.hpp"
+
+namespace flg {
+
+// Dense grid in HBM with a one-voxel zero border: (nx+2)(ny+2)(nz+2) floats,
+// x fastest; voxel (i,j,k) of the caller's grid is at ((k+1)*py + j+1)*px + i+1.
+struct VolDev {
+  const float* grid;
+  int32_t nx, ny, nz;
+  int32_t px, pxy;  // padded row and slice strides
+  V3 bmin, bmax, scale;
+  float sigma_s, sigma_a, sigma_t, g;
+  float inv_max_density, max_mu;
+};
+
+struct Tri8 {  // one trilinear lookup: padded base index and weights
+  int32_t base;  // -1 when the lookup is outside the grid (density 0)
+  float fx, fy, fz;
+  float rho;
+};
+
+// Trilinear lookup: index space g = (p - bmin) * scale - 0.5, zero outside the
+// grid, lerps along x, then y, then z (the library's definition of the dense
+// grid's interpolation; OpenVDB's BoxSampler is not in the reference tree).
+FLG_D Tri8 lookup(const VolDev& v, V3 p) {
+  float gx = (p.x - v.bmin.x) * v.scale.x - 0.5f;
+  float gy = (p.y - v.bmin.y) * v.scale.y - 0.5f;
+  float gz = (p.z - v.bmin.z) * v.scale.z - 0.5f;
+  float flx = floorf(gx), fly = floorf(gy), flz = floorf(gz);
+  Tri8 r;
+  r.fx = gx - flx;
+  r.fy = gy - fly;
+  r.fz = gz - flz;
+  int i = (int)smin(smax(flx, -2.0f), (float)v.nx + 1.0f);
+  int j = (int)smin(smax(fly, -2.0f), (float)v.ny + 1.0f);
+  int k = (int)smin(smax(flz, -2.0f), (float)v.nz + 1.0f);
+  if (i < -1 || j < -1 || k < -1 || i > v.nx - 1 || j > v.ny - 1 || k > v.nz - 1) {
+    r.base = -1;
+    r.rho = 0.0f;
+    return r;
+  }
+  r.base = ((k + 1) * (v.ny + 2) + (j + 1)) * v.px + (i + 1);
+  const float* g0 = v.grid + r.base;
+  const float* g1 = g0 + v.px;
+  const float* g2 = g0 + v.pxy;
+  const float* g3 = g2 + v.px;
+  float wx0 = 1.0f - r.fx, wx1 = r.fx;
+  float wy0 = 1.0f - r.fy, wy1 = r.fy;
+  float wz0 = 1.0f - r.fz, wz1 = r.fz;
+  float c00 = g0[0] * wx0 + g0[1] * wx1;
+  float c10 = g1[0] * wx0 + g1[1] * wx1;
+  float c01 = g2[0] * wx0 + g2[1] * wx1;
+  float c11 = g3[0] * wx0 + g3[1] * wx1;
+  float c0 = c00 * wy0 + c10 * wy1;
+  float c1 = c01 * wy0 + c11 * wy1;
+  r.rho = c0 * wz0 + c1 * wz1;
+  return r;
+}
+
+// AABB::intersect_pbrt (src/common/aabb.h:38-59)
+FLG_D bool box_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max) {
+  float t0 = 0.0f, t1 = tmax;
+  float los[3] = {lo.x, lo.y, lo.z}, his[3] = {hi.x, hi.y, hi.z};
+  float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float inv = 1.0f / ds[i];
+    float tn = (los[i] - os[i]) * inv;
+    float tf = (his[i] - os[i]) * inv;
+    if (tn > tf) { float t = tn; tn = tf; tf = t; }
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (t0 > t1) return false;
+  }
+  *t_min = t0;
+  *t_max = t1;
+  return true;
+}
+
+// AABB::intersect (aabb.h:20-36) with the inverse through the double literal
+FLG_D bool box_enter(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
+  V3 inv = v3((float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z));
+  V3 t1 = (lo - o) * inv;
+  V3 t2 = (hi - o) * inv;
+  V3 vmn = vmin(t1, t2), vmx = vmax(t1, t2);
+  float lmin = smax(smax(vmn.x, vmn.y), vmn.z);
+  float lmax = smin(smin(vmx.x, vmx.y), vmx.z);
+  if (lmin > lmax) { float t = lmin; lmin = lmax; lmax = t; }
+  if (lmin < lmax && lmax >= 0) {
+    *t_min = lmin;
+    *t_max = lmax;
+    return true;
+  }
+  return false;
+}
+
+// HGP (math_utils.h:103-107)
+FLG_D float hg_p(V3 wi, V3 wo, float g) {
+  float cos_theta = dot(wi, wo);
+  float denom = 1.0f + g * g + 2.0f * g * cos_theta;
+  return (1.0f - g * g) / (denom * sqrtf(denom)) * INV_4PI;
+}
+
+// CoordinateSystem + HGSampleP (math_utils.h:57-66, 112-128); u is the first
+// draw (left-to-right argument evaluation at integrator.cpp:378).
+FLG_D V3 hg_sample(V3 wo, float u, float vv, float g) {
+  float cos_theta;
+  if ((double)fabsf(g) < 1e-3) {
+    cos_theta = 1.0f - 2.0f * u;
+  } else {
+    float tmp = (1.0f - g * g) / (1.0f + g - 2.0f * g * u);
+    cos_theta = -(1.0f + g * g - tmp * tmp) / (2.0f * g);
+  }
+  float phi = 2.0f * PI * vv;
+  float x = 1.0f - cos_theta * cos_theta;
+  // sqrt(fmax(0, x)) in double rounded to float equals sqrtf of the clamped value
+  float sin_theta = sqrtf(x > 0.0f ? x : 0.0f);
+  V3 v1, v2;
+  if (fabsf(wo.x) > fabsf(wo.y))
+    v1 = v3(-wo.z, 0.0f, wo.x) / sqrtf(wo.x * wo.x + wo.z * wo.z);
+  else
+    v1 = v3(0.0f, wo.z, -wo.y) / sqrtf(wo.y * wo.y + wo.z * wo.z);
+  v2 = cross(wo, v1);
+  float sp, cp;
+  flg_sincosf(phi, &sp, &cp);
+  return (v1 * (sin_theta * cp) + v2 * (sin_theta * sp)) + wo * cos_theta;
+}
+
+// Environment light tables (Dist2D of InfiniteAreaLight, light.cpp:93-108,
+// distribution.hpp:12-92), 512 x 512.
+struct EnvDev {
+  const float* marg_cdf;  // 513
+  const float* marg_f;    // 512
+  const float* row_cdf;   // 512 x 513
+  const float* row_f;     // 512 x 512
+  const float* row_int;   // 512
+  float marg_int;
+  float radiance;
+  V3 center;
+  float radius;
+  int32_t present;
+};
+
+// Dist1D::inverseCDF / sampleC (distribution.hpp:26-50), r = mid - 1 kept.
+FLG_D float dist_sample(const float* cdf, const float* f, float integral, int n, float u,
+                        float* pdf, int* off) {
+  int l = 0, r = n;
+  while (r - l > 1) {
+    int mid = (l + r) / 2;
+    if (cdf[mid] <= u) l = mid;
+    else r = mid - 1;
+  }
+  *off = l;
+  float du = u - cdf[l];
+  if ((cdf[l + 1] - cdf[l]) > 0) du /= (cdf[l + 1] - cdf[l]);
+  *pdf = f[l] / integral;
+  return ((float)l + du) / (float)n;
+}
+
+// InfiniteAreaLight::sampleLi (light.cpp:119-145) with a constant texture.
+FLG_D float env_sample_li(const EnvDev& e, V3 ref, float u0, float u1, V3* wi, float* pdf,
+                          V3* p_light) {
+  float p1, p0;
+  int o1, o0;
+  float d1 = dist_sample(e.marg_cdf, e.marg_f, e.marg_int, 512, u1, &p1, &o1);
+  float d0 = dist_sample(e.row_cdf + 513 * o1, e.row_f + 512 * o1, e.row_int[o1], 512, u0, &p0, &o0);
+  *pdf = p0 * p1;
+  float phi = d0 * 2.0f * PI;
+  float theta = d1 * PI;
+  float st, ct, sp, cp;
+  flg_sincosf(theta, &st, &ct);
+  flg_sincosf(phi, &sp, &cp);
+  V3 dir = v3(st * cp, st * sp, ct);
+  *pdf /= (2.0f * PI * PI * smax(st, 1e-4f));
+  dir = v3(-dir.y, dir.z, -dir.x);
+  V3 p = e.center + (dir * 2.0f) * e.radius;
+  *p_light = p;
+  *wi = normalize(p - ref);
+  return e.radiance;
+}
+
+struct LightsDev {
+  EnvDev env;
+  int32_t has_point;
+  V3 point_pos;
+  float point_intensity;
+};
+
+}  // namespace flg

@@ -1,0 +1,266 @@
+/*
+ * fledge_hip.h — C ABI of the MI355X (gfx950) ray-tracing hot path:
+ * radix-tree LBVH build, closest-hit traversal with the watertight Pluecker
+ * test, primary-ray generation with the Halton sampler, the volume-only path
+ * integrator (delta + ratio tracking, HG phase, next-event estimation) and
+ * its path-replay adjoint.
+ *
+ * Conventions
+ *  - Every function returns an int status (FLG_OK == 0); the message of the
+ *    last failure on a context is returned by flg_last_error().  No C++
+ *    exception crosses this boundary.
+ *  - Buffers named d_* are device pointers (hipMalloc / torch CUDA tensors /
+ *    flg_malloc); the caller owns them.  Buffers named h_* are host pointers.
+ *    Handles (flg_bvh, flg_volume) own the device memory they allocate and
+ *    free it in *_destroy.
+ *  - All work of a context is issued on its stream, in call order; calls on
+ *    one context are not thread-safe, different contexts may run
+ *    concurrently.  Functions returning results into host memory synchronise
+ *    the stream; the others are asynchronous.
+ *  - Floating point follows the reference's operation order with no
+ *    contraction; hit ids, Morton codes, sort order and tree topology are
+ *    bit-exact with the CPU restatement of fledge's builder and traversal.
+ *
+ * Reference interfaces replaced (paths relative to the fledge source tree):
+ *  - flg_lbvh_build / flg_lbvh_export: RadixBVHBuilder::build
+ *    (src/experimental/radix_bvh.cpp:18-24, prestage :32-115,
+ *    parallelBuilder :117-200), BVHBuilderBase::build
+ *    (src/experimental/base_bvh.hpp:112); the C++ LBVHAccel ctor
+ *    (drop-in for NaiveBVHAccel, src/accel.cpp:35-86).
+ *  - flg_trace_closest: RadixBVHBuilder::intersect / recursiveIntersect
+ *    (radix_bvh.cpp:26-30, 202-245) with PlueckerTriangleIntersect
+ *    (src/experimental/intersector.hpp:63-101); Accel::intersect
+ *    (src/accel.hpp:12-21, src/primitive.hpp:31-42) batched.
+ *  - flg_camera_rays: Camera::generateRay (src/common/camera.h:21-31) with
+ *    HaltonSampler::getPixelSample (src/common/sampler.h:183-186).
+ *  - flg_volume_create / flg_volpath_render: OpenVDBVolume ctor, tr, sample
+ *    (src/volume.cpp:33-148) and SVolIntegrator::Li driven per pixel sample
+ *    (src/integrator.cpp:123-235, 328-405).
+ *  - flg_pathtrace_diffuse: PathIntegrator::Li's segment loop
+ *    (src/integrator.cpp:252-326) over a diffuse mesh.
+ */
+#ifndef FLEDGE_HIP_H_
+#define FLEDGE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define FLG_API __declspec(dllexport)
+#else
+#define FLG_API __attribute__((visibility("default")))
+#endif
+
+/* ------------------------------------------------------------- status --- */
+enum {
+  FLG_OK = 0,
+  FLG_ERR_INVALID = 1,   /* bad argument (null pointer, negative size, ...) */
+  FLG_ERR_HIP = 2,       /* a HIP runtime call failed */
+  FLG_ERR_NO_DEVICE = 3, /* no usable GPU */
+  FLG_ERR_OOM = 4,       /* device allocation failed */
+  FLG_ERR_RANGE = 5      /* input beyond a documented limit */
+};
+
+typedef struct flg_ctx flg_ctx;
+typedef struct flg_bvh flg_bvh;
+typedef struct flg_volume flg_volume;
+
+/* ------------------------------------------------------------ context --- */
+/* Creates a context on HIP device `device` issuing all work on `stream`
+ * (a hipStream_t; NULL selects the device's default stream, which orders the
+ * work with other default-stream users such as torch's default stream). */
+FLG_API int flg_ctx_create(int device, void* stream, flg_ctx** out);
+FLG_API int flg_ctx_destroy(flg_ctx* ctx);
+FLG_API int flg_ctx_synchronize(flg_ctx* ctx);
+FLG_API void* flg_ctx_stream(flg_ctx* ctx);
+FLG_API const char* flg_last_error(flg_ctx* ctx);
+FLG_API const char* flg_status_string(int status);
+
+/* Per-kernel timing with HIP events on the context stream (off by default). */
+FLG_API int flg_ctx_set_profiling(flg_ctx* ctx, int enable);
+/* Total milliseconds and launch count recorded for kernel `name` since the
+ * last reset; returns FLG_ERR_INVALID if the name is unknown. */
+FLG_API int flg_ctx_kernel_time(flg_ctx* ctx, const char* name, double* ms, int64_t* launches);
+FLG_API int flg_ctx_reset_profiling(flg_ctx* ctx);
+
+/* Device memory helpers for callers without HIP headers. */
+FLG_API int flg_malloc(flg_ctx* ctx, size_t bytes, void** d_out);
+FLG_API int flg_free(flg_ctx* ctx, void* d_ptr);
+FLG_API int flg_memcpy_htod(flg_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+FLG_API int flg_memcpy_dtoh(flg_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+/* --------------------------------------------------------------- LBVH --- */
+/* Normal policy of the watertight test (StableNormalize, vector.h:495-504). */
+#define FLG_NORMAL_FIXED 0u  /* x^2 + y^2 + z^2 */
+#define FLG_NORMAL_COMPAT 1u /* the reference's x^2 + y^2 + z + z */
+
+/* Builds a Karras radix-tree BVH over an indexed triangle mesh.
+ * d_verts: nverts * 3 floats (x,y,z); d_ind: ntris * 3 int32.
+ * flags: FLG_NORMAL_* used by later traces of this BVH.
+ * Limits: 0 <= ntris < 2^30; ntris == 0 gives an empty BVH (all misses). */
+FLG_API int flg_lbvh_build(flg_ctx* ctx, const float* d_verts, const int32_t* d_ind, int32_t ntris,
+                           int32_t nverts, uint32_t flags, flg_bvh** out);
+FLG_API int flg_lbvh_destroy(flg_bvh* bvh);
+
+typedef struct flg_mesh_desc {
+  const float* d_verts;
+  const int32_t* d_ind;
+  int32_t ntris;
+  int32_t nverts;
+} flg_mesh_desc;
+
+/* Builds n independent BVHs in one batch (one launch per build stage for all
+ * meshes).  Each out[i] equals flg_lbvh_build on mesh i alone, bit for bit. */
+FLG_API int flg_lbvh_build_batched(flg_ctx* ctx, const flg_mesh_desc* h_meshes, int32_t n,
+                                   uint32_t flags, flg_bvh** out);
+
+typedef struct flg_lbvh_info {
+  int32_t ntris;
+  float bound[6]; /* global bound lo.xyz, hi.xyz */
+  uint64_t device_bytes;
+} flg_lbvh_info;
+FLG_API int flg_lbvh_get_info(flg_bvh* bvh, flg_lbvh_info* out);
+
+/* Copies the build to host arrays for parity checks; any pointer may be NULL.
+ * Node arrays hold ntris-1 entries (Karras node i), leaf arrays ntris
+ * (sorted position). */
+typedef struct flg_lbvh_view {
+  uint32_t* h_morton;      /* codes in original triangle order */
+  uint32_t* h_morton_sorted;
+  uint32_t* h_order;       /* sorted position -> original triangle */
+  int32_t* h_split;        /* gamma */
+  int32_t* h_range_lo;     /* min(i, j) */
+  int32_t* h_range_hi;     /* max(i, j) */
+  int32_t* h_parent;       /* parent of internal node (root: 0) */
+  uint8_t* h_left_leaf;
+  uint8_t* h_right_leaf;
+  float* h_node_bound;     /* 6 per node: lo.xyz, hi.xyz */
+  int32_t* h_leaf_parent;
+} flg_lbvh_view;
+FLG_API int flg_lbvh_export(flg_ctx* ctx, flg_bvh* bvh, const flg_lbvh_view* view);
+
+/* -------------------------------------------------------------- trace --- */
+typedef struct flg_ray {
+  float ox, oy, oz, tmax; /* tmax: hits with t <= tmax are accepted */
+  float dx, dy, dz, pad;
+} flg_ray; /* 32 bytes */
+
+typedef struct flg_hit {
+  int32_t id; /* original triangle index, -1 on miss */
+  float t;    /* hit distance, tmax on miss */
+} flg_hit;    /* 8 bytes */
+
+typedef struct flg_trace_cfg {
+  int32_t block_size; /* 0: tuned default */
+  int32_t reserved[7];
+} flg_trace_cfg;
+
+/* Closest hit of n rays; the visit order and tie rule are those of
+ * RadixBVHBuilder::recursiveIntersect. cfg may be NULL. */
+FLG_API int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
+                              flg_hit* d_hits, int64_t n, const flg_trace_cfg* cfg);
+
+/* Counts node visits and triangle tests of the same traversal (for the
+ * roofline's algorithmic bytes); d_counts: 2 uint64 (nodes, triangles). */
+FLG_API int flg_trace_count(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, int64_t n,
+                            uint64_t* d_counts);
+
+typedef struct flg_camera {
+  float pos[3], look_at[3], up[3];
+  float fov_deg;
+} flg_camera;
+
+/* Primary rays of sample `sample` (Halton index offset(pixel) + sample *
+ * stride, dimensions 0 and 1 for the pixel jitter) for a W x H image,
+ * pixel-major: ray[y * W + x].  spp sets the sampler's sample count. */
+FLG_API int flg_camera_rays(flg_ctx* ctx, const flg_camera* cam, int32_t width, int32_t height,
+                            int32_t spp, int32_t sample, flg_ray* d_rays);
+
+/* Wavefront path tracer over a diffuse mesh with no lights: every path
+ * (pixel, sample) traces its camera ray and up to max_depth cosine-weighted
+ * bounces.  Optional per-path outputs (path index = pixel * spp + s):
+ * d_nseg[paths], d_seg_id[paths * (max_depth+1)], d_seg_t[same].
+ * h_stats (may be NULL) receives {total segments, paths}, followed by
+ * {node visits, triangle tests} when count_visits is set (4 entries). */
+typedef struct flg_pt_cfg {
+  int32_t width, height, spp, max_depth;
+  int32_t trace_block;  /* 0: default */
+  int32_t use_graph;    /* capture the bounce loop in a hipGraph */
+  int32_t count_visits; /* count traversal work (slower; for the roofline) */
+  int32_t reserved[5];
+} flg_pt_cfg;
+FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
+                                  const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
+                                  float* d_seg_t, uint64_t* h_stats);
+
+/* ------------------------------------------------------------- volume --- */
+typedef struct flg_medium {
+  float sigma_s, sigma_a; /* sigma_t = sigma_a + sigma_s */
+  float g;                /* Henyey-Greenstein asymmetry */
+} flg_medium;
+
+/* Dense nx*ny*nz density grid (x fastest) over the box bounds[6]
+ * (lo.xyz, hi.xyz).  The majorant is sigma_t * max(density). */
+FLG_API int flg_volume_create(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny,
+                              int32_t nz, const float* h_bounds, const flg_medium* medium,
+                              flg_volume** out);
+FLG_API int flg_volume_destroy(flg_volume* vol);
+/* Replaces the density values (same dimensions); recomputes the majorant. */
+FLG_API int flg_volume_update(flg_ctx* ctx, flg_volume* vol, const float* d_density);
+
+typedef struct flg_vol_scene {
+  flg_camera camera;
+  int32_t has_point;
+  float point_pos[3];
+  float point_intensity;
+  int32_t has_env;
+  float env_radiance; /* constant environment */
+} flg_vol_scene;
+
+typedef struct flg_render_cfg {
+  int32_t width, height, spp, max_depth;
+  int32_t reserved[8];
+} flg_render_cfg;
+
+/* Renders width*height RGB float pixels (d_rgb, 3 floats per pixel) with the
+ * volume-only integrator.  h_stats (may be NULL): {paths, density lookups}. */
+FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                               const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
+
+/* Differentiable forward (differential delta tracking, ratio tracking on the
+ * shadow and camera rays without roulette, fixed majorant) of the same
+ * estimator with albedo alpha and density scale sigma_t_scale: sigma_t =
+ * scale * density, sigma_s = alpha * sigma_t.  The proposal grid (decisions)
+ * is the volume's density at the time of the call to flg_volpath_diff_begin;
+ * gradients are exact for that grid. */
+typedef struct flg_diff_params {
+  float sigma_t_scale; /* sigma_t per unit density */
+  float albedo;        /* single-scatter albedo alpha */
+  float g;
+  float majorant;      /* fixed majorant (per unit length) */
+} flg_diff_params;
+
+FLG_API int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                                     const flg_render_cfg* cfg, const flg_diff_params* params,
+                                     float* d_rgb);
+/* Path-replay adjoint of flg_volpath_diff_forward: given dLoss/dImage (RGB,
+ * d_dl_drgb), accumulates dLoss/dDensity into d_grad (nx*ny*nz floats, added
+ * to) and dLoss/dAlbedo into d_grad_albedo (1 float, added to). */
+FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                                const flg_render_cfg* cfg, const flg_diff_params* params,
+                                const float* d_dl_drgb, float* d_grad, float* d_grad_albedo);
+
+/* ------------------------------------------------------------- checks --- */
+/* Device copies of the path's scalar helpers, for bit-exactness tests:
+ * kind 0: flg_logf(x); 1: sin; 2: cos; 3: Halton get1D with x = (index, dim)
+ * packed as two uint32 in the input. */
+FLG_API int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLEDGE_HIP_H_ */

@@ -1,0 +1,165 @@
+// TEST INFRASTRUCTURE — NOT PART OF THE PRODUCT.
+//
+// CPU restatement of fledge's ray-tracing hot path (LBVH build, closest-hit
+// traversal, Halton sampler, volumetric path integrator), used only as the
+// parity oracle by tests/, by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg.  Nothing under fledge_amd/ may include, link or call it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the fledge source tree).  The reference cannot be compiled here (fmt, TBB,
+// Embree, OpenVDB are absent), so this file re-states its arithmetic with the
+// floating-point policy fixed explicitly:
+//   * no FP contraction anywhere (built with -ffp-contract=off), except the
+//     explicit fused multiply-subtract in msub (embree's fmsub intent,
+//     intersector.hpp:41-43);
+//   * std::min / std::max are restated as their exact ternaries so NaN
+//     behaviour is reproduced;
+//   * logf / sinf / cosf on the per-sample path use the project's own
+//     implementations (flg_logf, flg_sincosf below, after fdlibm/musl) so that
+//     the GPU kernels, which carry their own copy of the same method, agree
+//     bit for bit; host-only constants (camera tanf, environment-light
+//     sinf table) use the C library, as the product's host code does.
+#pragma once
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------- scalar ---
+static inline float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+static inline float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+static inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// ---------------------------------------------------------------- vector ---
+// fledge Vector3f (src/common/vector.h:37-340, vector.inc): component-wise
+// operators in x, y, z order; Dot = (x*x' + y*y') + z*z' (Sum, vector.h:300-302).
+struct V3 {
+  float x, y, z;
+};
+static inline V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+static inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+static inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+static inline V3 operator/(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+static inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+static inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Cross, vector.h:399-404
+static inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// SquaredNorm<T,3> vector.h:311-317 ; Norm :320-322 ; Normalize :325-327 / :490-493
+static inline float sqnorm(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static inline float norm(V3 a) { return std::sqrt(sqnorm(a)); }
+static inline V3 normalize(V3 a) { return a / std::sqrt(sqnorm(a)); }
+// Min / Max vector.h:435-447 (std::min / std::max per component)
+static inline V3 vmin(V3 a, V3 b) { return {smin(a.x, b.x), smin(a.y, b.y), smin(a.z, b.z)}; }
+static inline V3 vmax(V3 a, V3 b) { return {smax(a.x, b.x), smax(a.y, b.y), smax(a.z, b.z)}; }
+static inline float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// ------------------------------------------------------ transcendentals ---
+// flg_logf: natural log, fdlibm e_logf.c method (reduce to m in
+// [sqrt(2)/2, sqrt(2)), s = f/(2+f), degree-4 minimax in s^2), evaluated with
+// plain IEEE float operations in a fixed order.  < 1 ulp from the true value.
+static inline float flg_logf(float x) {
+  const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f;
+  const float Lg1 = 0.66666662693f, Lg2 = 0.40000972152f, Lg3 = 0.28498786688f,
+              Lg4 = 0.24279078841f;
+  uint32_t ix = f2u(x);
+  int k = 0;
+  if (ix < 0x00800000u || (ix >> 31)) {
+    if ((ix << 1) == 0) return -INFINITY;
+    if (ix >> 31) return NAN;
+    k -= 25;  // subnormal: scale up
+    x *= 33554432.0f;
+    ix = f2u(x);
+  } else if (ix >= 0x7f800000u) {
+    return x;
+  }
+  ix += 0x3f800000u - 0x3f3504f3u;
+  k += (int)(ix >> 23) - 0x7f;
+  ix = (ix & 0x007fffffu) + 0x3f3504f3u;
+  float f = u2f(ix) - 1.0f;
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float w = z * z;
+  float t1 = w * (Lg2 + w * Lg4);
+  float t2 = z * (Lg1 + w * Lg3);
+  float R = t2 + t1;
+  float hfsq = 0.5f * f * f;
+  float dk = (float)k;
+  return s * (hfsq + R) + dk * ln2_lo - hfsq + f + dk * ln2_hi;
+}
+
+// flg_sincosf: sine and cosine of a float, FreeBSD msun / musl method:
+// double-precision reduction by pi/2 (two-part constant) and the __sindf /
+// __cosdf minimax kernels on [-pi/4, pi/4], rounded once to float.
+// Domain used on the path: |x| < 2^20.
+static inline double flg_sindf(double x) {
+  const double S1 = -0.166666666416265235595, S2 = 0.0083333293858894631756,
+               S3 = -0.000198393348360966317347, S4 = 0.0000027183114939898219064;
+  double z = x * x, w = z * z, r = S3 + z * S4, s = z * x;
+  return (x + s * (S1 + z * S2)) + s * w * r;
+}
+static inline double flg_cosdf(double x) {
+  const double C0 = -0.499999997251031003120, C1 = 0.0416666233237390631894,
+               C2 = -0.00138867637746099294692, C3 = 0.0000243904487962774090654;
+  double z = x * x, w = z * z, r = C2 + z * C3;
+  return ((1.0 + z * C0) + w * C1) + (w * z) * r;
+}
+static inline void flg_sincosf(float xf, float* s, float* c) {
+  const double invpio2 = 6.36619772367581382433e-01, pio2_1 = 1.57079631090164184570e+00,
+               pio2_1t = 1.58932547735281966916e-08;
+  double x = (double)xf;
+  double fn = std::rint(x * invpio2);
+  int n = (int)fn;
+  double y = (x - fn * pio2_1) - fn * pio2_1t;
+  double sy = flg_sindf(y), cy = flg_cosdf(y);
+  switch (n & 3) {
+    case 0: *s = (float)sy; *c = (float)cy; break;
+    case 1: *s = (float)cy; *c = (float)(-sy); break;
+    case 2: *s = (float)(-sy); *c = (float)(-cy); break;
+    default: *s = (float)(-cy); *c = (float)sy; break;
+  }
+}
+static inline float flg_sinf(float x) { float s, c; flg_sincosf(x, &s, &c); return s; }
+static inline float flg_cosf(float x) { float s, c; flg_sincosf(x, &s, &c); return c; }
+
+// fledge constants, src/fledge.h:27-36
+static constexpr float PI = 3.14159265358979323846f;
+static constexpr float PI_OVER2 = PI / 2;
+static constexpr float PI_OVER4 = PI / 4;
+static constexpr float INV_PI = 0.31830988618379067154f;
+static constexpr float INV_4PI = 0.07957747154594766788f;
+static constexpr float SHADOW_EPS = 1e-4f;
+static constexpr float ONE_MINUS_EPSILON = 0.99999994f;  // src/common/sampler.h:15
+
+// ------------------------------------------------------------------ PCG32 ---
+// Workload generator used by the synthetic configurations (O'Neill's PCG32,
+// pcg32_srandom / pcg32_random); restated here so the oracle can regenerate
+// inputs independently of the product's generator.
+struct Pcg32 {
+  uint64_t state = 0, inc = 0;
+  uint32_t next() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+  }
+  void seed(uint64_t initstate, uint64_t initseq) {
+    state = 0;
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+}  // namespace orc

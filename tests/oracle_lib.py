@@ -1,0 +1,175 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so) — test
+infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "build", "liboracle.so")
+
+P = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+f32 = ctypes.c_float
+
+_SIGS = {
+    "orc_logf": (None, [P, P, i64]),
+    "orc_sincosf": (None, [P, P, P, i64]),
+    "orc_pcg32": (None, [u64, u64, i64, P]),
+    "orc_gen_soup": (None, [u64, u64, i32, P, P]),
+    "orc_left_shift3": (u32, [u32]),
+    "orc_encode_morton3": (u32, [f32, f32, f32]),
+    "orc_lbvh_build": (P, [P, P, i32, i32]),
+    "orc_lbvh_free": (None, [P]),
+    "orc_lbvh_ntris": (i32, [P]),
+    "orc_lbvh_export": (None, [P] + [P] * 12),
+    "orc_trace": (None, [P, P, i64, i32, P, P, P, P, P, ctypes.c_int]),
+    "orc_brute_trace": (None, [P, P, i32, P, i64, i32, P, P, P, ctypes.c_int]),
+    "orc_num_primes": (i32, []),
+    "orc_prime": (i32, [i32]),
+    "orc_radical_inverse": (f32, [i32, u64]),
+    "orc_halton_offset": (u64, [i32, i32, i32, i32]),
+    "orc_halton_get": (None, [i32, i32, i32, i32, i32, i64, i32, P]),
+    "orc_halton_sequence": (None, [i32, i32, i32, i32, i32, i32, P]),
+    "orc_camera_rays": (None, [P, i32, i32, i32, i32, P, ctypes.c_int]),
+    "orc_volpath_render": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P,
+                                  ctypes.c_int, P, P]),
+    "orc_diffuse_render": (None, [P, i32, P, i32, i32, i32, i32, i64, i64, P, P, P, ctypes.c_int, P]),
+    "orc_offset_ray_origin": (None, [P, P, P, P]),
+}
+
+_lib = None
+NTHREADS = max(1, min(8, os.cpu_count() or 1))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle not built (make -C oracle)")
+        L = ctypes.CDLL(LIB)
+        for n, (r, a) in _SIGS.items():
+            f = getattr(L, n, None)
+            if f is None:
+                continue
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Lbvh:
+    def __init__(self, verts, ind):
+        self.verts = np.ascontiguousarray(verts, np.float32).ravel()
+        self.ind = np.ascontiguousarray(ind, np.int32).ravel()
+        self.n = self.ind.size // 3
+        self.h = lib().orc_lbvh_build(_p(self.verts), _p(self.ind), self.n, self.verts.size // 3)
+
+    def export(self):
+        n, ni = self.n, max(self.n - 1, 0)
+        out = {
+            "bound": np.zeros(6, np.float32), "morton": np.zeros(n, np.uint32),
+            "morton_sorted": np.zeros(n, np.uint32), "order": np.zeros(n, np.uint32),
+            "split": np.zeros(ni, np.int32), "range_lo": np.zeros(ni, np.int32),
+            "range_hi": np.zeros(ni, np.int32), "parent": np.zeros(ni, np.int32),
+            "left_leaf": np.zeros(ni, np.uint8), "right_leaf": np.zeros(ni, np.uint8),
+            "node_bound": np.zeros((ni, 6), np.float32), "leaf_parent": np.zeros(n, np.int32),
+        }
+        lib().orc_lbvh_export(self.h, *[_p(out[k]) for k in (
+            "bound", "morton", "morton_sorted", "order", "split", "range_lo", "range_hi", "parent",
+            "left_leaf", "right_leaf", "node_bound", "leaf_parent")])
+        return out
+
+    def trace(self, rays, compat=False, normals=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        ids = np.zeros(n, np.int32)
+        ts = np.zeros(n, np.float32)
+        ng = np.zeros((n, 3), np.float32) if normals else None
+        nodes = np.zeros(1, np.uint64)
+        tris = np.zeros(1, np.uint64)
+        lib().orc_trace(self.h, _p(rays), n, 1 if compat else 0, _p(ids), _p(ts), _p(ng), _p(nodes),
+                        _p(tris), NTHREADS)
+        self.last_counts = (int(nodes[0]), int(tris[0]))
+        if normals:
+            return ids, ts, ng
+        return ids, ts
+
+    def diffuse_render(self, cam10, W, H, spp, max_depth, p0, p1, compat=False, record=True,
+                       nthreads=None):
+        n = p1 - p0
+        S = max_depth + 1
+        nseg = np.zeros(n, np.int32)
+        ids = np.full((n, S), -2, np.int32) if record else None
+        ts = np.zeros((n, S), np.float32) if record else None
+        rays = np.zeros(1, np.uint64)
+        cam = np.asarray(cam10, np.float32)
+        lib().orc_diffuse_render(self.h, 1 if compat else 0, _p(cam), W, H, spp, max_depth, p0, p1,
+                                 _p(nseg), _p(ids), _p(ts), nthreads or NTHREADS, _p(rays))
+        return nseg, ids, ts, int(rays[0])
+
+    def __del__(self):
+        try:
+            lib().orc_lbvh_free(self.h)
+        except Exception:
+            pass
+
+
+def brute_trace(verts, ind, rays, compat=False):
+    verts = np.ascontiguousarray(verts, np.float32).ravel()
+    ind = np.ascontiguousarray(ind, np.int32).ravel()
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    ids = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.float32)
+    ties = np.zeros(n, np.int32)
+    lib().orc_brute_trace(_p(verts), _p(ind), ind.size // 3, _p(rays), n, 1 if compat else 0,
+                          _p(ids), _p(ts), _p(ties), NTHREADS)
+    return ids, ts, ties
+
+
+def camera_rays(cam10, W, H, spp=1, sample=0):
+    out = np.zeros((W * H, 8), np.float32)
+    lib().orc_camera_rays(_p(np.asarray(cam10, np.float32)), W, H, spp, sample, _p(out), NTHREADS)
+    return out
+
+
+def volpath_render(grid, bounds, medium, lights, cam10, W, H, spp, max_depth, nthreads=None):
+    grid = np.ascontiguousarray(grid, np.float32)
+    nz, ny, nx = grid.shape
+    rgb = np.zeros((H, W, 3), np.float32)
+    lk = np.zeros(1, np.uint64)
+    pa = np.zeros(1, np.uint64)
+    lib().orc_volpath_render(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                             _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                             _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
+                             nthreads or NTHREADS, _p(lk), _p(pa))
+    return rgb, int(lk[0]), int(pa[0])
+
+
+def logf(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros_like(x)
+    lib().orc_logf(_p(x), _p(y), x.size)
+    return y
+
+
+def sincosf(x):
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    lib().orc_sincosf(_p(x), _p(s), _p(c), x.size)
+    return s, c
+
+
+def halton_get(resx, resy, spp, px, py, sample, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_halton_get(resx, resy, spp, px, py, sample, n, _p(out))
+    return out

@@ -1,0 +1,169 @@
+"""GPU parity of the LBVH build, closest-hit traversal, primary rays and the
+diffuse path tracer against the CPU oracle (bit-exact)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as orc
+from fledge_amd import _native as N
+from fledge_amd import api
+
+pytestmark = pytest.mark.gpu
+
+CAM0 = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _build_both(ctx, v, i, compat=False):
+    g = api.LBVH.build(ctx, _dev(v), _dev(i), compat=compat)
+    o = orc.Lbvh(v, i)
+    return g, o
+
+
+def _compare_export(ge, oe):
+    for k in ("morton", "morton_sorted", "order", "split", "range_lo", "range_hi", "parent",
+              "left_leaf", "right_leaf", "leaf_parent"):
+        assert np.array_equal(ge[k], oe[k]), k
+    assert np.array_equal(ge["node_bound"].view(np.uint32), oe["node_bound"].view(np.uint32))
+    assert np.array_equal(ge["bound"].view(np.uint32), oe["bound"].view(np.uint32))
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 1000, 4096, 65536])
+def test_build_parity_soup(ctx, n):
+    print("build parity n=%d" % n, flush=True)
+    v, i = api.gen_soup(n)
+    g, o = _build_both(ctx, v, i)
+    _compare_export(g.export(), o.export())
+
+
+def test_build_parity_duplicate_codes(ctx):
+    # many equal Morton codes: ties must keep the original index order
+    rng = np.random.default_rng(3)
+    n = 5000
+    c = rng.integers(0, 8, size=(n, 3)).astype(np.float32) / 8.0
+    v = np.repeat(c, 3, axis=0) + np.tile(np.array([[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0]], np.float32), (n, 1))
+    i = np.arange(3 * n, dtype=np.int32)
+    g, o = _build_both(ctx, v.ravel(), i)
+    ge, oe = g.export(), o.export()
+    _compare_export(ge, oe)
+    assert len(np.unique(oe["morton_sorted"])) < n // 4
+
+
+def test_build_single_and_empty(ctx):
+    v, i = api.gen_soup(1)
+    g, o = _build_both(ctx, v, i)
+    rays = orc.camera_rays(CAM0, 32, 32)
+    gh = api.hits_as_numpy(g.trace(_dev(rays)))
+    oi, ot = o.trace(rays)
+    assert np.array_equal(gh[0], oi)
+    assert np.array_equal(gh[1].view(np.uint32), ot.view(np.uint32))
+    e = api.LBVH.build(ctx, torch.zeros(3, dtype=torch.float32, device="cuda"),
+                       torch.zeros(0, dtype=torch.int32, device="cuda"))
+    hits = api.hits_as_numpy(e.trace(_dev(rays)))
+    assert (hits[0] == -1).all()
+
+
+def test_camera_rays_parity(ctx):
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    for (W, H, spp, s) in [(1024, 1024, 1, 0), (300, 200, 16, 5), (97, 61, 4, 3)]:
+        print("camera W=%d H=%d spp=%d s=%d" % (W, H, spp, s), flush=True)
+        gr = api.camera_rays(ctx, cam, W, H, spp, s).cpu().numpy()
+        orr = orc.camera_rays(CAM0, W, H, spp, s)
+        assert np.array_equal(gr.view(np.uint32), orr.view(np.uint32))
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_trace_parity_config0(ctx, compat):
+    """Config 0: 65,536-triangle soup, 1024x1024 jittered primary rays."""
+    print("trace config0 compat=%s" % compat, flush=True)
+    v, i = api.gen_soup(65536)
+    g, o = _build_both(ctx, v, i, compat=compat)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    rays = api.camera_rays(ctx, cam, 1024, 1024)
+    gid, gt = api.hits_as_numpy(g.trace(rays))
+    oid, ot = o.trace(rays.cpu().numpy(), compat=compat)
+    assert (oid >= 0).sum() > 10000
+    assert np.array_equal(gid, oid)
+    assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+
+
+def test_trace_random_rays_vs_oracle_and_brute(ctx):
+    v, i = api.gen_soup(3000, initseq=77)
+    g, o = _build_both(ctx, v, i)
+    rng = np.random.default_rng(5)
+    n = 20000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.random((n, 3), dtype=np.float32) * 1.4 - 0.2
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 4:7] = d
+    rays[:, 3] = np.float32(3.4028234663852886e38)
+    rays[: n // 4, 3] = 0.3  # finite tmax on a quarter
+    gid, gt = api.hits_as_numpy(g.trace(_dev(rays)))
+    oid, ot = o.trace(rays)
+    assert np.array_equal(gid, oid)
+    assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    bid, bt, ties = orc.brute_trace(v, i, rays)
+    ok = ties == 0
+    assert np.array_equal(gid[ok], bid[ok])
+
+
+def test_watertight_octasphere(ctx):
+    """Rays from inside a closed octahedral sphere always hit (shape_tests.cpp:22-48 pattern)."""
+    v, i = api.gen_octasphere(5, 0.0)
+    g, o = _build_both(ctx, v, i)
+    rng = np.random.default_rng(11)
+    n = 50000
+    p = rng.normal(size=(n, 3)).astype(np.float32)
+    p = p / np.linalg.norm(p, axis=1, keepdims=True) * 0.5 * rng.random((n, 1), dtype=np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = p
+    rays[:, 4:7] = d
+    rays[:, 3] = np.inf
+    gid, gt = api.hits_as_numpy(g.trace(_dev(rays)))
+    assert (gid >= 0).all()
+    oid, ot = o.trace(rays)
+    assert np.array_equal(gid, oid)
+
+
+def test_batched_equals_single(ctx):
+    sizes = [1, 2, 1000, 5000, 333, 70000, 4096]
+    meshes = []
+    for k, n in enumerate(sizes):
+        v, i = api.gen_soup(n, initseq=100 + k)
+        meshes.append((_dev(v), _dev(i)))
+    batch = api.LBVH.build_batched(ctx, meshes)
+    rays = _dev(orc.camera_rays(CAM0, 128, 128))
+    for k, (v, i) in enumerate(meshes):
+        single = api.LBVH.build(ctx, v, i)
+        a, b = batch[k].export(), single.export()
+        for key in a:
+            assert np.array_equal(np.asarray(a[key]).view(np.uint8), np.asarray(b[key]).view(np.uint8)), (k, key)
+        ha = batch[k].trace(rays).cpu().numpy()
+        hb = single.trace(rays).cpu().numpy()
+        assert np.array_equal(ha, hb)
+
+
+def test_pathtrace_parity_small(ctx):
+    v, i = api.gen_octasphere(6, 0.1)
+    g, o = _build_both(ctx, v, i)
+    cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
+    cam = N.Camera.make(cam10[0:3], cam10[3:6], cam10[6:9], cam10[9])
+    W, H, spp, D = 160, 90, 4, 8
+    r = g.pathtrace(cam, W, H, spp, D, record=True)
+    npaths = W * H * spp
+    nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, npaths)
+    assert r["segments"] == nrays
+    gn = r["nseg"].cpu().numpy()
+    assert np.array_equal(gn, nseg)
+    gi = r["seg_id"].cpu().numpy()
+    gt = r["seg_t"].cpu().numpy()
+    mask = np.arange(D + 1)[None, :] < nseg[:, None]
+    assert np.array_equal(gi[mask], ids[mask])
+    assert np.array_equal(gt[mask].view(np.uint32), ts[mask].view(np.uint32))
+    assert nrays > npaths  # some bounces happened

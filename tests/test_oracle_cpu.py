@@ -1,0 +1,463 @@
+"""CPU tests: the oracle pinned against the reference's own known answers and
+fixtures, closed forms and brute force (no GPU needed)."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from fledge_amd import api
+
+L = orc.lib()
+P = ctypes.c_void_p
+f32 = ctypes.c_float
+i32 = ctypes.c_int32
+L.orc_karras.argtypes = [P, i32, P, P, P, P, P]
+L.orc_dist1d_sample.restype = f32
+L.orc_dist1d_sample.argtypes = [P, i32, f32, P, P]
+L.orc_hg_sample.restype = f32
+L.orc_hg_sample.argtypes = [P, f32, f32, f32, P]
+L.orc_hg_p.restype = f32
+L.orc_hg_p.argtypes = [P, P, f32]
+L.orc_pluecker.restype = i32
+L.orc_pluecker.argtypes = [P, P, P, P, P, i32, P, P]
+L.orc_bvh_test_rays.argtypes = [ctypes.c_int64, P]
+L.orc_watertight_rays.argtypes = [ctypes.c_int64, P]
+L.orc_offset_ray_origin.argtypes = [P, P, P, P]
+
+
+def f32a(*x):
+    return np.array(x, np.float32)
+
+
+# ---------------------------------------------------------------- PCG32 ---
+def test_pcg32_demo_kat():
+    # pcg32-demo output for pcg32_srandom(42, 54)
+    want = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+    out = np.zeros(6, np.uint32)
+    L.orc_pcg32(42, 54, 6, out.ctypes.data)
+    assert out.tolist() == want
+    from fledge_amd import _native as N
+    out2 = np.zeros(6, np.uint32)
+    N.scene().flg_scene_pcg32(42, 54, 6, out2.ctypes.data)
+    assert out2.tolist() == want
+
+
+def test_soup_generator_matches_oracle():
+    v, i = api.gen_soup(5000)
+    v2 = np.zeros_like(v)
+    i2 = np.zeros_like(i)
+    L.orc_gen_soup(42, 54, 5000, v2.ctypes.data, i2.ctypes.data)
+    assert np.array_equal(v.view(np.uint32), v2.view(np.uint32)) and np.array_equal(i, i2)
+    c = v.reshape(-1, 3, 3).mean(axis=1)
+    assert c.min() > -0.02 and c.max() < 1.02
+
+
+# --------------------------------------------------------------- Morton ---
+def _interleave(x, y, z):
+    r = 0
+    for b in range(10):
+        r |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return r
+
+
+def test_morton_kats():
+    assert L.orc_left_shift3(1023) == 0x09249249
+    assert L.orc_left_shift3(1024) == 0x09249249  # 1024 clamps to 1023
+    assert L.orc_encode_morton3(1023.0, 0.0, 0.0) == 0x09249249
+    assert L.orc_encode_morton3(0.0, 1023.0, 0.0) == 0x12492492
+    assert L.orc_encode_morton3(0.0, 0.0, 1023.0) == 0x24924924
+    assert L.orc_encode_morton3(1024.0, 1024.0, 1024.0) == 0x3FFFFFFF
+    assert L.orc_encode_morton3(1023.9, 0.99, 2.5) == _interleave(1023, 0, 2)  # truncation
+    rng = np.random.default_rng(0)
+    for x, y, z in rng.integers(0, 1024, size=(2000, 3)):
+        assert L.orc_encode_morton3(float(x), float(y), float(z)) == _interleave(int(x), int(y), int(z))
+
+
+# ---------------------------------------------------------------- Karras ---
+def _karras(codes):
+    codes = np.ascontiguousarray(codes, np.uint32)
+    n = codes.size
+    out = [np.zeros(n - 1, np.int32) for _ in range(3)] + [np.zeros(n - 1, np.uint8) for _ in range(2)]
+    L.orc_karras(codes.ctypes.data, n, *[a.ctypes.data for a in out])
+    return out
+
+
+def test_karras_paper_figure():
+    # Karras, HPG 2012, Fig. 3: keys 00001 00010 00100 00101 10011 11000 11001 11110
+    codes = [0b00001, 0b00010, 0b00100, 0b00101, 0b10011, 0b11000, 0b11001, 0b11110]
+    split, lo, hi, ll, rl = _karras(codes)
+    assert split.tolist() == [3, 0, 2, 1, 4, 6, 5]
+    assert list(zip(lo.tolist(), hi.tolist())) == [(0, 7), (0, 1), (2, 3), (0, 3), (4, 7), (5, 7), (5, 6)]
+    assert ll.tolist() == [0, 1, 1, 0, 1, 0, 1]
+    assert rl.tolist() == [0, 1, 1, 0, 0, 1, 1]
+
+
+def _brute_tree(keys):
+    """Binary radix tree by recursive splitting at the highest differing bit of
+    the 62-bit key (code << 32 | index); returns {(lo, hi): split}."""
+    n = len(keys)
+    aug = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    out = {}
+
+    def rec(lo, hi):
+        if lo == hi:
+            return
+        x = aug[lo] ^ aug[hi]
+        bit = x.bit_length() - 1
+        s = lo
+        while s < hi and not ((aug[s + 1] >> bit) & 1):
+            s += 1
+        out[(lo, hi)] = s
+        rec(lo, s)
+        rec(s + 1, hi)
+
+    rec(0, n - 1)
+    return out
+
+
+@pytest.mark.parametrize("seed,n,bits", [(1, 50, 30), (2, 300, 30), (3, 200, 4), (4, 64, 1), (5, 2, 30)])
+def test_karras_vs_brute_radix_tree(seed, n, bits):
+    rng = np.random.default_rng(seed)
+    codes = np.sort(rng.integers(0, 1 << bits, n).astype(np.uint32))
+    split, lo, hi, ll, rl = _karras(codes)
+    want = _brute_tree(codes)
+    got = {(int(a), int(b)): int(s) for a, b, s in zip(lo, hi, split)}
+    assert got == want
+    assert (ll == (lo == split)).all() and (rl == (hi == split + 1)).all()
+
+
+@pytest.mark.parametrize("n", [2, 3, 100, 5000])
+def test_lbvh_invariants(n):
+    v, i = api.gen_soup(n, initseq=9)
+    b = orc.Lbvh(v, i).export()
+    # sorted codes non-decreasing, order a permutation, ties in index order
+    ms, order = b["morton_sorted"], b["order"]
+    assert np.all(np.diff(ms.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    eq = ms[1:] == ms[:-1]
+    assert np.all(order[1:][eq] > order[:-1][eq])
+    assert np.array_equal(b["morton"][order], ms)
+    # topology: every internal node except the root has exactly one parent,
+    # every leaf exactly one, DFS covers every leaf once
+    ni = n - 1
+    nchild = np.zeros(ni, np.int64)
+    lchild = np.zeros(n, np.int64)
+    for k in range(ni):
+        g = b["split"][k]
+        for side, leaf in ((0, b["left_leaf"][k]), (1, b["right_leaf"][k])):
+            if leaf:
+                lchild[g + side] += 1
+                assert b["leaf_parent"][g + side] == k
+            else:
+                nchild[g + side] += 1
+                assert b["parent"][g + side] == k
+    assert nchild[0] == 0 and (nchild[1:] == 1).all() and (lchild == 1).all()
+    # bounds: node = union of children; root = union of all vertices
+    tri = v.reshape(-1, 3, 3)
+    tlo, thi = tri.min(axis=1), tri.max(axis=1)
+    nb = b["node_bound"]
+    for k in range(ni):
+        g = b["split"][k]
+        parts = []
+        for side, leaf in ((0, b["left_leaf"][k]), (1, b["right_leaf"][k])):
+            if leaf:
+                t = order[g + side]
+                parts.append(np.concatenate([tlo[t], thi[t]]))
+            else:
+                parts.append(nb[g + side])
+        want = np.concatenate([np.minimum(parts[0][:3], parts[1][:3]), np.maximum(parts[0][3:], parts[1][3:])])
+        assert np.array_equal(nb[k], want)
+    allv = v.reshape(-1, 3)
+    assert np.array_equal(nb[0], np.concatenate([allv.min(axis=0), allv.max(axis=0)]))
+    assert np.array_equal(b["bound"], nb[0])
+
+
+# ------------------------------------------------------------- Pluecker ---
+def _pl(a, b, c, o, d, compat=0):
+    t = np.zeros(1, np.float32)
+    ng = np.zeros(3, np.float32)
+    h = L.orc_pluecker(*[np.asarray(x, np.float32).ctypes.data for x in (a, b, c, o, d)], compat,
+                       t.ctypes.data, ng.ctypes.data)
+    return bool(h), float(t[0]), ng
+
+
+def test_pluecker_kats():
+    a, b, c = f32a(-1, -1, 2), f32a(1, -1, 2), f32a(0, 1, 2)
+    h, t, ng = _pl(a, b, c, f32a(0, 0, 0), f32a(0, 0, 1))
+    assert h and t == 2.0 and np.array_equal(ng, f32a(0, 0, 1))
+    h, t, ng = _pl(a, c, b, f32a(0, 0, 0), f32a(0, 0, 1))  # two-sided
+    assert h and t == 2.0 and np.array_equal(ng, f32a(0, 0, -1))
+    h, _, _ = _pl(a, b, c, f32a(0, 0, 0), f32a(0, 0, -1))  # behind the origin
+    assert not h
+    h, _, _ = _pl(a, b, c, f32a(5, 0, 0), f32a(0, 0, 1))  # outside
+    assert not h
+    h, t, _ = _pl(a, b, c, f32a(0, 0, 2), f32a(0, 0, 1))  # t == 0 accepted
+    assert h and t == 0.0
+    # compat: StableNormalize's x^2 + y^2 + z + z is negative for a -z normal
+    h, _, _ = _pl(a, c, b, f32a(0, 0, 0), f32a(0, 0, 1), compat=1)
+    assert not h
+    h, t, _ = _pl(a, b, c, f32a(0, 0, 0), f32a(0, 0, 1), compat=1)
+    assert h and t == 2.0
+
+
+def test_pluecker_shared_edge_watertight():
+    # two triangles sharing the edge (0,-1)-(0,1) at z = 1; rays through the edge hit one of them
+    a, b, c = f32a(0, -1, 1), f32a(0, 1, 1), f32a(-1, 0, 1)
+    e = f32a(1, 0, 1)
+    for y in np.linspace(-0.9, 0.9, 181, dtype=np.float32):
+        o = f32a(0, y, 0)
+        d = f32a(0, 0, 1)
+        h1, _, _ = _pl(a, b, c, o, d)
+        h2, _, _ = _pl(b, a, e, o, d)
+        assert h1 or h2
+
+
+def test_trace_matches_brute_force():
+    v, i = api.gen_soup(20000, initseq=31)
+    o = orc.Lbvh(v, i)
+    rng = np.random.default_rng(7)
+    n = 2000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.random((n, 3), dtype=np.float32) * 1.2 - 0.1
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3] = np.inf
+    ids, ts = o.trace(rays)
+    bid, bts, ties = orc.brute_trace(v, i, rays)
+    ok = ties == 0
+    assert (ids >= 0).sum() > 100
+    assert np.array_equal(ids[ok], bid[ok]) and np.array_equal(ts[ok], bts[ok])
+
+
+def test_bvh_test_fixed_ray_fixture():
+    """bvh_test.hpp:143-168: the fixed ray must hit MakeMeshedSphere(1000, 1000, 5)."""
+    v, i = api.gen_uv_sphere(1000, 1000, 5.0)
+    assert i.size // 3 == 999 * 2 * 998
+    o = orc.Lbvh(v, i)
+    ray = np.array([[0.345564, 0.422557, 0.259793, np.finfo(np.float32).max,
+                     0.524748325347900390625, 0.714325249195098876953125, 0.46301043033599853515625, 0]],
+                   np.float32)
+    ids, ts = o.trace(ray)
+    assert ids[0] >= 0
+    p = ray[0, :3] + ray[0, 4:7] * ts[0]
+    assert abs(np.linalg.norm(p) - 5.0) < 1e-2
+    # the 500k-ray differential harness (bvh_test.hpp:107-141) rays: all start
+    # inside the sphere, so all must hit with |p| close to 5
+    n = 20000
+    rays = np.zeros((n, 8), np.float32)
+    L.orc_bvh_test_rays(n, rays.ctypes.data)
+    ids, ts = o.trace(rays)
+    assert (ids >= 0).all()
+    pts = rays[:, :3] + rays[:, 4:7] * ts[:, None]
+    assert np.abs(np.linalg.norm(pts, axis=1) - 5.0).max() < 2e-2
+
+
+def test_watertight_fixture():
+    """shape_tests.cpp:22-48: rays from inside MakeMeshedSphere(16, 16, 1) always hit."""
+    v, i = api.gen_uv_sphere(16, 16, 1.0)
+    o = orc.Lbvh(v, i)
+    n = 1000
+    rays = np.zeros((n, 8), np.float32)
+    L.orc_watertight_rays(n, rays.ctypes.data)
+    ids, _ = o.trace(rays)
+    assert (ids >= 0).all()
+
+
+def test_octasphere_closed_and_outward():
+    for level in (0, 1, 4):
+        v, i = api.gen_octasphere(level, 0.0)
+        t = i.reshape(-1, 3)
+        assert t.shape[0] == 8 * 4 ** level and v.size // 3 == 4 * 4 ** level + 2
+        edges = {}
+        for a, b, c in t:
+            for x, y in ((a, b), (b, c), (c, a)):
+                edges[(x, y)] = edges.get((x, y), 0) + 1
+        assert all(n == 1 for n in edges.values())
+        assert all((y, x) in edges for (x, y) in edges)  # every directed edge has its twin
+        p = v.reshape(-1, 3)
+        nrm = np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]])
+        assert (np.einsum("ij,ij->i", nrm, p[t].mean(axis=1)) > 0).all()
+
+
+# ---------------------------------------------------------------- Halton ---
+def test_primes_kat():
+    assert [L.orc_prime(k) for k in range(10)] == [2, 3, 5, 7, 11, 13, 17, 19, 23, 29]
+    assert L.orc_num_primes() == 23000  # pi(262144)
+
+
+def test_radical_inverse_closed_forms():
+    ri = L.orc_radical_inverse
+    ri.restype = f32
+    assert [ri(2, a) for a in (1, 2, 3)] == [0.5, 0.25, 0.75]
+    assert [ri(3, a) for a in (1, 2, 3)] == pytest.approx([1 / 3, 2 / 3, 1 / 9], rel=1e-7)
+    for base in (2, 3, 5, 7, 101, 7919):
+        for a in (1, 17, 1234, 99991, 1990655):
+            digits, x, scale, nd = 0.0, a, 1.0, 0
+            while x:
+                scale /= base
+                digits += (x % base) * scale
+                x //= base
+                nd += 1
+            # float product chain of 1/base per digit (sampler.h:66-71): ~1 ulp per digit
+            assert abs(ri(base, a) - digits) <= (nd + 2) * 2.0 ** -24 * digits
+
+
+def test_halton_crt_offsets():
+    for px, py in [(0, 0), (1, 1), (5, 200), (127, 127), (128, 3), (1919, 1079)]:
+        off = L.orc_halton_offset(1920, 1080, px, py)
+        assert off < 31104
+        rev2 = int(format(px % 128, "07b")[::-1], 2)
+        m, rev3 = py % 128, 0
+        for _ in range(5):
+            rev3 = rev3 * 3 + m % 3
+            m //= 3
+        assert off % 128 == rev2 and off % 243 == rev3
+
+
+def test_halton_histogram_reference():
+    """sampler_tests.cpp:24-45: HaltonSampler(128, {5,5}), pixel (1,1), 128 dims x 128 spp."""
+    out = np.zeros(128 * 128, np.float32)
+    L.orc_halton_sequence(5, 5, 128, 1, 1, 128, out.ctypes.data)
+    hist = np.bincount(np.minimum((out * 10).astype(int), 9), minlength=10) / out.size
+    assert np.abs(hist - 0.1).max() < 1e-2
+
+
+# -------------------------------------------------- sampling & lights -----
+def test_dist1d_inverse_cdf_quirk():
+    f = np.ones(4, np.float32)
+    pdf = np.zeros(1, np.float32)
+    off = np.zeros(1, np.int32)
+    # cdf = [0, .25, .5, .75, 1]; r = mid - 1 sends u = 0.3 to bin 0 (correct: 1)
+    r = L.orc_dist1d_sample(f.ctypes.data, 4, 0.3, pdf.ctypes.data, off.ctypes.data)
+    assert off[0] == 0 and pdf[0] == 1.0 and abs(r - 0.3) < 1e-7
+    L.orc_dist1d_sample(f.ctypes.data, 4, 0.6, pdf.ctypes.data, off.ctypes.data)
+    assert off[0] == 2
+    L.orc_dist1d_sample(f.ctypes.data, 4, 0.8, pdf.ctypes.data, off.ctypes.data)
+    assert off[0] == 3
+
+
+def test_hg_phase_normalised_and_sampled():
+    g = 0.3
+    wo = f32a(0.0, 0.0, 1.0)
+    # pdf integrates to 1 over the sphere
+    cos = np.linspace(-1, 1, 20001)
+    vals = []
+    for c in cos[::100]:
+        wi = f32a(math.sqrt(max(0.0, 1 - c * c)), 0.0, c)
+        vals.append(L.orc_hg_p(wi.ctypes.data, wo.ctypes.data, g))
+    integ = np.trapezoid(np.array(vals), cos[::100]) * 2 * math.pi
+    assert abs(integ - 1.0) < 1e-3
+    # sampled cos(theta) follows the HG distribution (CDF check)
+    rng = np.random.default_rng(2)
+    us = rng.random((20000, 2)).astype(np.float32)
+    wi = np.zeros(3, np.float32)
+    cs = []
+    for u, v in us:
+        p = L.orc_hg_sample(wo.ctypes.data, float(u), float(v), g, wi.ctypes.data)
+        assert p > 0
+        cs.append(float(np.dot(wi, wo)))
+    cs = np.sort(np.array(cs))
+    # HG CDF in mu = dot(wi, wo) for the convention 1 + g^2 + 2 g mu
+    mu = np.linspace(-1, 1, 41)
+    pdf = lambda m: (1 - g * g) / (1 + g * g + 2 * g * m) ** 1.5 / 2
+    grid = np.linspace(-1, 1, 40001)
+    cdf = np.concatenate([[0], np.cumsum((pdf(grid[1:]) + pdf(grid[:-1])) / 2 * np.diff(grid))])
+    emp = np.searchsorted(cs, mu) / cs.size
+    assert np.abs(emp - np.interp(mu, grid, cdf)).max() < 0.015
+
+
+def test_offset_ray_origin_kat():
+    out = np.zeros(3, np.float32)
+    p, n, d = f32a(1.0, 2.0, 3.0), f32a(0.0, 0.0, 1.0), f32a(0.0, 0.0, 1.0)
+    L.orc_offset_ray_origin(p.ctypes.data, n.ctypes.data, d.ctypes.data, out.ctypes.data)
+    want = (np.array([3.0], np.float32).view(np.int32) + 256).view(np.float32)[0]
+    assert out[0] == 1.0 and out[1] == 2.0 and out[2] == want
+    p = f32a(0.001, -0.5, 0.0)
+    L.orc_offset_ray_origin(p.ctypes.data, n.ctypes.data, f32a(0.0, 0.0, -1.0).ctypes.data, out.ctypes.data)
+    assert out[0] == np.float32(0.001) and out[2] == np.float32(0.0) - np.float32(1 / 65536)
+
+
+# ---------------------------------------------------------------- volume ---
+CAM0 = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+
+
+def _chords(W, H, spp=1, s=0):
+    rays = orc.camera_rays(CAM0, W, H, spp, s)
+    o, d = rays[:, :3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+    with np.errstate(divide="ignore"):
+        t0 = (0.0 - o) / d
+        t1 = (1.0 - o) / d
+    tn = np.minimum(t0, t1).max(axis=1)
+    tf = np.maximum(t0, t1).min(axis=1)
+    return np.maximum(tf - np.maximum(tn, 0), 0)
+
+
+def _trilinear(grid, p):
+    """Density at points p [m, 3] in the unit box: index space p * n - 0.5, zero outside."""
+    n = grid.shape[0]
+    g = p * n - 0.5
+    f = np.floor(g)
+    w = g - f
+    i = f.astype(int)
+    out = np.zeros(len(p))
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                ii, jj, kk = i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz
+                ok = (ii >= 0) & (ii < n) & (jj >= 0) & (jj < n) & (kk >= 0) & (kk < n)
+                v = np.zeros(len(p))
+                v[ok] = grid[kk[ok], jj[ok], ii[ok]]
+                wt = (w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) * \
+                     (w[:, 2] if dz else 1 - w[:, 2])
+                out += wt * v
+    return out
+
+
+def test_volume_homogeneous_transmittance_closed_form():
+    """Albedo 0, environment only: pixel = Le * E[ratio-tracking T] = Le exp(-sigma_t tau), with
+    tau the optical depth of the trilinear density along each camera ray (quadrature)."""
+    W = H = 12
+    spp = 256
+    grid = np.ones((8, 8, 8), np.float32)
+    rgb, _, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [0.0, 2.0, 0.0], [0, 0, 0, 0, 0, 1, 1.0],
+                                   CAM0, W, H, spp, 4)
+    want = np.zeros(W * H)
+    u = (np.arange(400) + 0.5) / 400
+    for s in range(spp):
+        rays = orc.camera_rays(CAM0, W, H, spp, s).astype(np.float64)
+        o, d = rays[:, :3], rays[:, 4:7]
+        with np.errstate(divide="ignore"):
+            t0, t1 = (0.0 - o) / d, (1.0 - o) / d
+        tn = np.maximum(np.minimum(t0, t1).max(axis=1), 0)
+        tf = np.maximum(t0, t1).min(axis=1)
+        ln = np.maximum(tf - tn, 0)
+        pts = o[:, None, :] + d[:, None, :] * (tn[:, None] + ln[:, None] * u[None, :])[:, :, None]
+        rho = _trilinear(grid.astype(np.float64), pts.reshape(-1, 3)).reshape(W * H, -1)
+        want += np.exp(-2.0 * rho.mean(axis=1) * ln)
+    want = (want / spp).reshape(H, W)
+    got = rgb[:, :, 0]
+    # average over pixels of equal expectation: the Monte Carlo error shrinks
+    assert abs(got.mean() - want.mean()) < 5e-3
+    assert np.abs(got - want).max() < 0.08
+
+
+def test_volume_white_furnace_oracle():
+    """Albedo 1, constant environment 0.5, no point light: every pixel ~ 0.5."""
+    grid = np.full((8, 8, 8), 0.5, np.float32)
+    rgb, _, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [4.0, 0.0, 0.3], [0, 0, 0, 0, 0, 1, 0.5],
+                                   CAM0, 24, 24, 128, 512)
+    assert abs(rgb.mean() - 0.5) < 0.01
+
+
+def test_volume_point_light_single_scatter_scale():
+    """Doubling the point-light intensity (no environment) doubles every pixel exactly
+    (the light enters the estimator linearly; same Halton numbers)."""
+    grid = api.gen_fbm_grid(16, seed=4)
+    a, _, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], [1, 2, 2, -2, 50.0, 0, 0],
+                                 CAM0, 16, 16, 4, 8)
+    b, _, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], [1, 2, 2, -2, 100.0, 0, 0],
+                                 CAM0, 16, 16, 4, 8)
+    assert a.max() > 0
+    assert np.allclose(b, 2 * a, rtol=1e-6, atol=0)
