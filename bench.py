@@ -103,7 +103,7 @@ def main():
         frame()
     sync()
     dt = (time.perf_counter() - t0) / args.steps
-    trace_ms, trace_launches = ctx.kernel_time("pathtrace_trace")
+    trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
     total_ms, _ = ctx.kernel_time("pathtrace_total")
     ctx.profiling(False)
     value = segments / dt / 1e6
@@ -120,7 +120,7 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_pt_trace (closest-hit traversal)",
+                "kernel": "k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)",
                 "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                 "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
                 "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,

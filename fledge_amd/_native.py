@@ -100,6 +100,7 @@ _SIGS = {
     "flg_lbvh_destroy": (c_int, [c_vp]),
     "flg_lbvh_get_info": (c_int, [c_vp, P(LbvhInfo)]),
     "flg_lbvh_export": (c_int, [c_vp, c_vp, P(LbvhView)]),
+    "flg_lbvh_download": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp, c_size]),
     "flg_trace_closest": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, P(TraceCfg)]),
     "flg_trace_count": (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "flg_camera_rays": (c_int, [c_vp, P(Camera), c_i32, c_i32, c_i32, c_i32, c_vp]),
@@ -108,9 +109,10 @@ _SIGS = {
     "flg_volume_destroy": (c_int, [c_vp]),
     "flg_volume_update": (c_int, [c_vp, c_vp, c_vp]),
     "flg_volpath_render": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp, P(c_u64)]),
-    "flg_volpath_diff_forward": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp]),
-    "flg_volpath_adjoint": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
-                                    c_vp, c_vp]),
+    "flg_volpath_diff_forward": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams),
+                                         c_vp, c_vp]),
+    "flg_volpath_adjoint": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
+                                    c_vp, c_vp, c_vp]),
     "flg_debug_eval": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i64]),
 }
 

@@ -82,9 +82,12 @@ def build_host():
         return None
     out = os.path.join(HERE, "libfledge_host.so")
     tmp = out + ".tmp"
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off",
-          "-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "host"),
-          "-o", tmp, src, "-L", HERE, "-lfledge_hip", "-Wl,-rpath,$ORIGIN"])
+    # hipcc, as HIP: the host mirrors compile the device headers' traversal and
+    # trackers with the kernels' floating-point flags
+    flags = [f for f in HIP_FLAGS if f != "-fvisibility=hidden"]
+    _run([_hipcc(), *flags, "-x", "hip", "-shared", "-I", os.path.join(ROOT, "include"),
+          "-I", os.path.join(CSRC, "host"), "-o", tmp, src, "-L", HERE, "-lfledge_hip",
+          "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
     return out
 

@@ -1,0 +1,531 @@
+// C++ drop-ins over the C ABI (declarations and contracts in fledge_gpu.hpp).
+// Built with hipcc so that the host mirrors compile the traversal and the
+// trackers from the same headers as the GPU kernels (traverse.hpp,
+// volume.hpp), with the same floating-point flags.
+#include "fledge_gpu.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "../traverse.hpp"
+#include "../volume.hpp"
+
+namespace fledge {
+
+// ------------------------------------------------------ memory resources
+namespace {
+void* aligned_or_throw(void* p, std::size_t alignment) {
+  if (p && alignment && (reinterpret_cast<uintptr_t>(p) % alignment) != 0) throw std::bad_alloc();
+  return p;
+}
+}  // namespace
+
+void* managed_memory_resource::do_allocate(std::size_t bytes, std::size_t alignment) {
+  void* p = nullptr;
+  if (hipMallocManaged(&p, bytes ? bytes : 1) != hipSuccess) throw std::bad_alloc();
+  return aligned_or_throw(p, alignment);
+}
+void managed_memory_resource::do_deallocate(void* p, std::size_t, std::size_t) { (void)hipFree(p); }
+bool managed_memory_resource::do_is_equal(const std::pmr::memory_resource& o) const noexcept {
+  return this == &o;
+}
+
+void* gpu_memory_resource::do_allocate(std::size_t bytes, std::size_t alignment) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) throw std::bad_alloc();
+  return aligned_or_throw(p, alignment);
+}
+void gpu_memory_resource::do_deallocate(void* p, std::size_t, std::size_t) { (void)hipFree(p); }
+bool gpu_memory_resource::do_is_equal(const std::pmr::memory_resource& o) const noexcept {
+  return this == &o;
+}
+
+void* async_memory_resource::do_allocate(std::size_t bytes, std::size_t alignment) {
+  void* p = nullptr;
+  if (hipMallocAsync(&p, bytes ? bytes : 1, static_cast<hipStream_t>(m_stream)) != hipSuccess)
+    throw std::bad_alloc();
+  return aligned_or_throw(p, alignment);
+}
+void async_memory_resource::do_deallocate(void* p, std::size_t, std::size_t) {
+  (void)hipFreeAsync(p, static_cast<hipStream_t>(m_stream));
+}
+bool async_memory_resource::do_is_equal(const std::pmr::memory_resource& o) const noexcept {
+  auto* a = dynamic_cast<const async_memory_resource*>(&o);
+  return a && a->m_stream == m_stream;
+}
+
+std::pmr::memory_resource* GlobalManagedMemoryResource() {
+  static managed_memory_resource resource;
+  return &resource;
+}
+
+namespace gpu {
+
+// ---------------------------------------------------------------- Context
+Context::Context(int device, void* stream) : m_device(device) {
+  int st = flg_ctx_create(device, stream, &m_ctx);
+  if (st != FLG_OK)
+    throw Error(st, std::string("flg_ctx_create: ") + flg_status_string(st));
+}
+
+Context::~Context() {
+  if (m_ctx) flg_ctx_destroy(m_ctx);
+}
+
+void Context::check(int status, const char* what) const {
+  if (status == FLG_OK) return;
+  const char* msg = m_ctx ? flg_last_error(m_ctx) : "";
+  throw Error(status, std::string(what) + ": " + flg_status_string(status) + ": " + (msg ? msg : ""));
+}
+
+void Context::synchronize() const { check(flg_ctx_synchronize(m_ctx), "flg_ctx_synchronize"); }
+
+Context& Context::global() {
+  // never destroyed: a static destructor would run after the HIP runtime's
+  static Context* ctx = new Context(0, nullptr);
+  return *ctx;
+}
+
+// ----------------------------------------------------------- DeviceBuffer
+DeviceBuffer::DeviceBuffer(const Context& ctx, size_t bytes) : m_ctx(&ctx), m_bytes(bytes) {
+  if (bytes) ctx.check(flg_malloc(ctx.get(), bytes, &m_ptr), "flg_malloc");
+}
+
+DeviceBuffer::DeviceBuffer(const Context& ctx, const void* host, size_t bytes) : DeviceBuffer(ctx, bytes) {
+  upload(host, bytes);
+}
+
+DeviceBuffer::~DeviceBuffer() {
+  if (m_ptr) flg_free(m_ctx->get(), m_ptr);
+}
+
+DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : m_ctx(o.m_ctx), m_ptr(o.m_ptr), m_bytes(o.m_bytes) {
+  o.m_ptr = nullptr;
+  o.m_bytes = 0;
+}
+
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+  if (this != &o) {
+    if (m_ptr) flg_free(m_ctx->get(), m_ptr);
+    m_ctx = o.m_ctx;
+    m_ptr = o.m_ptr;
+    m_bytes = o.m_bytes;
+    o.m_ptr = nullptr;
+    o.m_bytes = 0;
+  }
+  return *this;
+}
+
+void DeviceBuffer::upload(const void* host, size_t bytes) {
+  if (bytes > m_bytes) throw Error(FLG_ERR_RANGE, "DeviceBuffer::upload: larger than the buffer");
+  if (bytes) m_ctx->check(flg_memcpy_htod(m_ctx->get(), m_ptr, host, bytes), "flg_memcpy_htod");
+}
+
+void DeviceBuffer::download(void* host, size_t bytes) const {
+  if (bytes > m_bytes) throw Error(FLG_ERR_RANGE, "DeviceBuffer::download: larger than the buffer");
+  if (bytes) m_ctx->check(flg_memcpy_dtoh(m_ctx->get(), host, m_ptr, bytes), "flg_memcpy_dtoh");
+}
+
+// ------------------------------------------------------------- HostMirror
+void HostMirror::load(const Context& ctx, flg_bvh* bvh) {
+  flg_lbvh_info info;
+  ctx.check(flg_lbvh_get_info(bvh, &info), "flg_lbvh_get_info");
+  m_n = info.ntris;
+  std::memcpy(m_root, info.bound, sizeof(m_root));
+  const size_t ni = m_n > 0 ? size_t(m_n) - 1 : 0;
+  m_nodes.assign(ni * sizeof(flg::BNode), 0);
+  m_tris.assign(size_t(m_n) * sizeof(flg::BTri), 0);
+  if (reinterpret_cast<uintptr_t>(m_nodes.data()) % 16 || reinterpret_cast<uintptr_t>(m_tris.data()) % 16)
+    throw Error(FLG_ERR_INVALID, "HostMirror: host allocation is not 16-byte aligned");
+  ctx.check(flg_lbvh_download(ctx.get(), bvh, ni ? m_nodes.data() : nullptr, m_nodes.size(),
+                              m_n ? m_tris.data() : nullptr, m_tris.size()),
+            "flg_lbvh_download");
+  m_by_id.assign(size_t(m_n), -1);
+  const flg::BTri* tris = reinterpret_cast<const flg::BTri*>(m_tris.data());
+  for (int32_t p = 0; p < m_n; ++p) {
+    const uint32_t id = flg::f2u(tris[p].v0.w);
+    if (id >= uint32_t(m_n)) throw Error(FLG_ERR_INVALID, "HostMirror: corrupt triangle ids");
+    m_by_id[id] = p;
+  }
+}
+
+void HostMirror::setCompat(bool compat) { m_compat = compat; }
+
+int32_t HostMirror::intersect(const Vector3f& o, const Vector3f& d, float tmax, float* t,
+                              Vector3f* ng) const {
+  flg::TraceGeom g;
+  g.nodes = reinterpret_cast<const flg::BNode*>(m_nodes.data());
+  g.tris = reinterpret_cast<const flg::BTri*>(m_tris.data());
+  g.n = m_n;
+  for (int k = 0; k < 6; ++k) g.root[k] = m_root[k];
+  const flg::V3 O = flg::v3(o[0], o[1], o[2]), D = flg::v3(d[0], d[1], d[2]);
+  float best = tmax;
+  flg::Counters c{0, 0};
+  const int32_t p = m_compat ? flg::trace_one<true, false>(g, O, D, best, c)
+                             : flg::trace_one<false, false>(g, O, D, best, c);
+  if (p < 0) return -1;
+  const flg::BTri& tri = g.tris[p];
+  // the winning test once more for its normal (same inputs, same bits)
+  float th;
+  flg::V3 n;
+  flg::pluecker(flg::v3(tri.v0.x, tri.v0.y, tri.v0.z), flg::v3(tri.v1.x, tri.v1.y, tri.v1.z),
+                flg::v3(tri.v2.x, tri.v2.y, tri.v2.z), O, D, m_compat, &th, &n);
+  *t = best;
+  if (ng) *ng = Vector3f(n.x, n.y, n.z);
+  return (int32_t)flg::f2u(tri.v0.w);
+}
+
+Vector3f HostMirror::normal(int32_t tri_id, const Vector3f& o, const Vector3f& d) const {
+  if (tri_id < 0 || tri_id >= m_n) return Vector3f(0.0f);
+  const flg::BTri& tri = reinterpret_cast<const flg::BTri*>(m_tris.data())[m_by_id[tri_id]];
+  float th;
+  flg::V3 n{0, 0, 0};
+  flg::pluecker(flg::v3(tri.v0.x, tri.v0.y, tri.v0.z), flg::v3(tri.v1.x, tri.v1.y, tri.v1.z),
+                flg::v3(tri.v2.x, tri.v2.y, tri.v2.z), flg::v3(o[0], o[1], o[2]),
+                flg::v3(d[0], d[1], d[2]), m_compat, &th, &n);
+  return Vector3f(n.x, n.y, n.z);
+}
+
+}  // namespace gpu
+
+// -------------------------------------------------------------- LBVHAccel
+namespace {
+std::vector<float> flatten(const Vector3f* p, size_t n) {
+  std::vector<float> v(3 * n);
+  for (size_t i = 0; i < n; ++i) {
+    v[3 * i] = p[i][0];
+    v[3 * i + 1] = p[i][1];
+    v[3 * i + 2] = p[i][2];
+  }
+  return v;
+}
+
+flg_ray to_flg_ray(const Vector3f& o, const Vector3f& d, float tmax) {
+  flg_ray r;
+  r.ox = o[0];
+  r.oy = o[1];
+  r.oz = o[2];
+  r.tmax = tmax;
+  r.dx = d[0];
+  r.dy = d[1];
+  r.dz = d[2];
+  r.pad = 0.0f;
+  return r;
+}
+
+flg_bvh* build_bvh(gpu::Context& ctx, const float* verts, int32_t nverts, const int32_t* ind,
+                   int32_t ntris, uint32_t flags) {
+  gpu::DeviceBuffer dv(ctx, verts, size_t(nverts) * 12);
+  gpu::DeviceBuffer di(ctx, ind, size_t(ntris) * 12);
+  flg_bvh* bvh = nullptr;
+  ctx.check(flg_lbvh_build(ctx.get(), static_cast<const float*>(dv.data()),
+                           static_cast<const int32_t*>(di.data()), ntris, nverts, flags, &bvh),
+            "flg_lbvh_build");
+  return bvh;  // the build synchronises: the staging buffers may go
+}
+}  // namespace
+
+LBVHAccel::LBVHAccel(const std::vector<Vector3f>& p, const std::vector<int>& ind, gpu::Context& ctx,
+                     uint32_t flags)
+    : m_ctx(&ctx) {
+  if (ind.size() % 3) throw gpu::Error(FLG_ERR_INVALID, "LBVHAccel: index count is not a multiple of 3");
+  std::vector<float> v = flatten(p.data(), p.size());
+  build(v.data(), (int32_t)p.size(), reinterpret_cast<const int32_t*>(ind.data()),
+        (int32_t)(ind.size() / 3), flags);
+}
+
+LBVHAccel::LBVHAccel(const experimental::InternalTriangleMesh& mesh, gpu::Context& ctx, uint32_t flags)
+    : m_ctx(&ctx) {
+  if (mesh.nInd % 3) throw gpu::Error(FLG_ERR_INVALID, "LBVHAccel: nInd is not a multiple of 3");
+  std::vector<float> v = flatten(mesh.p, size_t(mesh.nVert));
+  build(v.data(), mesh.nVert, reinterpret_cast<const int32_t*>(mesh.ind), mesh.nInd / 3, flags);
+}
+
+void LBVHAccel::build(const float* verts, int32_t nverts, const int32_t* ind, int32_t ntris,
+                      uint32_t flags) {
+  m_bvh = build_bvh(*m_ctx, verts, nverts, ind, ntris, flags);
+  m_mirror.setCompat((flags & FLG_NORMAL_COMPAT) != 0);
+  m_mirror.load(*m_ctx, m_bvh);
+  flg_lbvh_info info;
+  m_ctx->check(flg_lbvh_get_info(m_bvh, &info), "flg_lbvh_get_info");
+  m_bound = AABB(Vector3f(info.bound[0], info.bound[1], info.bound[2]),
+                 Vector3f(info.bound[3], info.bound[4], info.bound[5]));
+}
+
+LBVHAccel::~LBVHAccel() {
+  if (m_bvh) flg_lbvh_destroy(m_bvh);
+}
+
+bool LBVHAccel::intersect(const Ray& ray, SInteraction& isect) const {
+  return intersect(ray, isect, nullptr);
+}
+
+bool LBVHAccel::intersect(const Ray& ray, SInteraction& isect, int32_t* tri_id) const {
+  float t;
+  Vector3f ng;
+  const int32_t id = m_mirror.intersect(ray.m_o, ray.m_d, ray.m_tMax, &t, &ng);
+  if (tri_id) *tri_id = id;
+  if (id < 0) return false;
+  ray.m_tMax = t;
+  isect.m_p = ray(t);
+  isect.m_ng = ng;
+  isect.m_ns = ng;
+  isect.m_wo = -ray.m_d;
+  isect.m_ray = ray;
+  isect.m_primitive = this;
+  return true;
+}
+
+AABB LBVHAccel::getBound() const { return m_bound; }
+
+void LBVHAccel::intersectBatch(const flg_ray* d_rays, flg_hit* d_hits, int64_t n) const {
+  m_ctx->check(flg_trace_closest(m_ctx->get(), m_bvh, d_rays, d_hits, n, nullptr), "flg_trace_closest");
+}
+
+std::vector<flg_hit> LBVHAccel::intersectBatch(const std::vector<Ray>& rays) const {
+  std::vector<flg_ray> r(rays.size());
+  for (size_t i = 0; i < rays.size(); ++i) r[i] = to_flg_ray(rays[i].m_o, rays[i].m_d, rays[i].m_tMax);
+  std::vector<flg_hit> h(rays.size());
+  if (rays.empty()) return h;
+  gpu::DeviceBuffer dr(*m_ctx, r.data(), r.size() * sizeof(flg_ray));
+  gpu::DeviceBuffer dh(*m_ctx, h.size() * sizeof(flg_hit));
+  intersectBatch(static_cast<const flg_ray*>(dr.data()), static_cast<flg_hit*>(dh.data()),
+                 (int64_t)rays.size());
+  dh.download(h.data(), h.size() * sizeof(flg_hit));
+  return h;
+}
+
+// ------------------------------------------------------------ LBVHBuilder
+namespace experimental {
+
+LBVHBuilder::LBVHBuilder(InternalTriangleMesh* mesh, std::pmr::memory_resource* mem_resource,
+                         gpu::Context* ctx, uint32_t flags)
+    : BVHBuilderBase(mesh, mem_resource), m_ctx(ctx ? ctx : &gpu::Context::global()), m_flags(flags) {}
+
+LBVHBuilder::~LBVHBuilder() {
+  if (m_bvh) flg_lbvh_destroy(m_bvh);
+}
+
+void LBVHBuilder::build() {
+  if (m_bvh) {
+    flg_lbvh_destroy(m_bvh);
+    m_bvh = nullptr;
+  }
+  if (m_mesh->nInd % 3) throw gpu::Error(FLG_ERR_INVALID, "LBVHBuilder: nInd is not a multiple of 3");
+  std::vector<float> v = flatten(m_mesh->p, size_t(m_mesh->nVert));
+  m_bvh = build_bvh(*m_ctx, v.data(), m_mesh->nVert, reinterpret_cast<const int32_t*>(m_mesh->ind),
+                    m_mesh->nInd / 3, m_flags);
+  m_mirror.setCompat((m_flags & FLG_NORMAL_COMPAT) != 0);
+  m_mirror.load(*m_ctx, m_bvh);
+  flg_lbvh_info info;
+  m_ctx->check(flg_lbvh_get_info(m_bvh, &info), "flg_lbvh_get_info");
+  m_bound.lower = Vector3f(info.bound[0], info.bound[1], info.bound[2]);
+  m_bound.upper = Vector3f(info.bound[3], info.bound[4], info.bound[5]);
+}
+
+BVHBuilderBase::BVHBound LBVHBuilder::getBound() const { return m_bound; }
+
+bool LBVHBuilder::intersect(BVHRayHit& rayhit) { return intersect(rayhit, nullptr); }
+
+bool LBVHBuilder::intersect(BVHRayHit& rayhit, int32_t* hit_id) {
+  if (!m_bvh) throw gpu::Error(FLG_ERR_INVALID, "LBVHBuilder::intersect before build()");
+  float t;
+  Vector3f ng;
+  const int32_t id = m_mirror.intersect(rayhit.ray_o, rayhit.ray_d, rayhit.tfar, &t, &ng);
+  if (hit_id) *hit_id = id;
+  if (id < 0) return false;
+  rayhit.hit = true;
+  rayhit.tfar = t;
+  rayhit.hit_ng = rayhit.hit_ns = ng;
+  return true;
+}
+
+void LBVHBuilder::intersectBatch(BVHRayHit* rayhits, int32_t* hit_ids, size_t n) {
+  if (!m_bvh) throw gpu::Error(FLG_ERR_INVALID, "LBVHBuilder::intersectBatch before build()");
+  if (n == 0) return;
+  std::vector<flg_ray> r(n);
+  for (size_t i = 0; i < n; ++i) r[i] = to_flg_ray(rayhits[i].ray_o, rayhits[i].ray_d, rayhits[i].tfar);
+  std::vector<flg_hit> h(n);
+  gpu::DeviceBuffer dr(*m_ctx, r.data(), n * sizeof(flg_ray));
+  gpu::DeviceBuffer dh(*m_ctx, n * sizeof(flg_hit));
+  m_ctx->check(flg_trace_closest(m_ctx->get(), m_bvh, static_cast<const flg_ray*>(dr.data()),
+                                 static_cast<flg_hit*>(dh.data()), (int64_t)n, nullptr),
+               "flg_trace_closest");
+  dh.download(h.data(), n * sizeof(flg_hit));
+  for (size_t i = 0; i < n; ++i) {
+    if (hit_ids) hit_ids[i] = h[i].id;
+    if (h[i].id < 0) continue;
+    rayhits[i].hit = true;
+    rayhits[i].tfar = h[i].t;
+    rayhits[i].hit_ng = rayhits[i].hit_ns = m_mirror.normal(h[i].id, rayhits[i].ray_o, rayhits[i].ray_d);
+  }
+}
+
+}  // namespace experimental
+
+// ------------------------------------------------------------- GridVolume
+namespace {
+struct SamplerDraw {  // the trackers' draw source over a fledge Sampler
+  Sampler& s;
+  float next() { return s.get1D(); }
+};
+}  // namespace
+
+GridVolume::GridVolume(const float* density, int nx, int ny, int nz, const AABB& bound,
+                       const Vector3f& sigma_s, const Vector3f& sigma_a, Float g, gpu::Context* ctx)
+    : m_ctx(ctx), m_nx(nx), m_ny(ny), m_nz(nz), m_sigma_s(sigma_s), m_sigma_a(sigma_a), m_g(g) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !density)
+    throw gpu::Error(FLG_ERR_INVALID, "GridVolume: bad grid");
+  if ((int64_t)(nx + 2) * (ny + 2) * (nz + 2) >= ((int64_t)1 << 31))
+    throw gpu::Error(FLG_ERR_RANGE, "GridVolume: grid too large");
+  for (int k = 0; k < 3; ++k)
+    if (!(bound.m_max[k] > bound.m_min[k])) throw gpu::Error(FLG_ERR_INVALID, "GridVolume: empty bounds");
+  m_aabb = bound;
+  pad(density);
+}
+
+gpu::Context& GridVolume::context() const {
+  if (!m_ctx) m_ctx = &gpu::Context::global();
+  return *m_ctx;
+}
+
+GridVolume::~GridVolume() {
+  if (m_vol) flg_volume_destroy(m_vol);
+}
+
+void GridVolume::pad(const float* density) {
+  const size_t n = size_t(m_nx) * m_ny * m_nz;
+  m_density.assign(density, density + n);
+  const int px = m_nx + 2, py = m_ny + 2, pz = m_nz + 2;
+  m_padded.assign(size_t(px) * py * pz, 0.0f);
+  float mx = 0.0f;
+  for (int k = 0; k < m_nz; ++k)
+    for (int j = 0; j < m_ny; ++j)
+      for (int i = 0; i < m_nx; ++i) {
+        const float v = density[(size_t(k) * m_ny + j) * m_nx + i];
+        m_padded[(size_t(k + 1) * py + (j + 1)) * px + (i + 1)] = v;
+        mx = std::max(mx, v);
+      }
+  m_max_density = mx;
+}
+
+void GridVolume::update(const float* density) {
+  pad(density);
+  if (m_vol) {
+    m_dev.upload(m_density.data(), m_density.size() * 4);
+    context().check(flg_volume_update(context().get(), m_vol, static_cast<const float*>(m_dev.data())),
+                 "flg_volume_update");
+  }
+}
+
+void GridVolume::setBound(const AABB& aabb) {
+  for (int k = 0; k < 3; ++k)
+    if (!(aabb.m_max[k] > aabb.m_min[k])) throw gpu::Error(FLG_ERR_INVALID, "GridVolume: empty bounds");
+  m_aabb = aabb;
+  if (m_vol) {  // the device copy is rebuilt with the new box on next use
+    flg_volume_destroy(m_vol);
+    m_vol = nullptr;
+  }
+}
+
+flg_volume* GridVolume::handle() const {
+  if (!m_vol) {
+    if (!m_dev.data()) m_dev = gpu::DeviceBuffer(context(), m_density.data(), m_density.size() * 4);
+    const float b[6] = {m_aabb.m_min[0], m_aabb.m_min[1], m_aabb.m_min[2],
+                        m_aabb.m_max[0], m_aabb.m_max[1], m_aabb.m_max[2]};
+    flg_medium med{m_sigma_s[0], m_sigma_a[0], m_g};
+    context().check(flg_volume_create(context().get(), static_cast<const float*>(m_dev.data()), m_nx, m_ny,
+                                   m_nz, b, &med, &m_vol),
+                 "flg_volume_create");
+  }
+  return m_vol;
+}
+
+namespace {
+flg::VolDev host_voldev(const std::vector<float>& padded, int nx, int ny, int nz, const AABB& box,
+                        const Vector3f& ss, const Vector3f& sa, Float g, float max_density) {
+  const float b[6] = {box.m_min[0], box.m_min[1], box.m_min[2], box.m_max[0], box.m_max[1], box.m_max[2]};
+  return flg::voldev_of(padded.data(), nx, ny, nz, b, flg_medium{ss[0], sa[0], g}, max_density);
+}
+}  // namespace
+
+Vector3f GridVolume::tr(const Ray& ray, Sampler& rng) const {
+  const flg::VolDev v = host_voldev(m_padded, m_nx, m_ny, m_nz, m_aabb, m_sigma_s, m_sigma_a, m_g,
+                                    m_max_density);
+  SamplerDraw draw{rng};
+  uint32_t nlook = 0;
+  const float t = flg::vol_tr<false>(v, draw, flg::v3(ray.m_o[0], ray.m_o[1], ray.m_o[2]),
+                                     flg::v3(ray.m_d[0], ray.m_d[1], ray.m_d[2]), ray.m_tMax, nlook);
+  return Vector3f(t);
+}
+
+Vector3f GridVolume::sample(const Ray& ray, Sampler& rng, VInteraction& vi, bool& success) const {
+  const flg::VolDev v = host_voldev(m_padded, m_nx, m_ny, m_nz, m_aabb, m_sigma_s, m_sigma_a, m_g,
+                                    m_max_density);
+  SamplerDraw draw{rng};
+  uint32_t nlook = 0;
+  bool ok = false;
+  flg::V3 pos{0, 0, 0};
+  const float f = flg::vol_sample<false>(v, draw, flg::v3(ray.m_o[0], ray.m_o[1], ray.m_o[2]),
+                                         flg::v3(ray.m_d[0], ray.m_d[1], ray.m_d[2]), ray.m_tMax, &ok,
+                                         &pos, nlook);
+  success = ok;
+  if (ok) vi = VInteraction(Vector3f(pos.x, pos.y, pos.z), -ray.m_d, m_g);
+  return Vector3f(f);
+}
+
+// ------------------------------------------------------- GPUVolIntegrator
+GPUVolIntegrator::GPUVolIntegrator(int max_depth, const GPUVolLights& lights)
+    : m_maxDepth(max_depth), m_lights(lights) {}
+
+void GPUVolIntegrator::render(const GridVolume& volume, const Vector3f& origin, const Vector3f& target,
+                              const Vector3f& up, Float fov, int width, int height, int spp,
+                              std::vector<float>& rgb) {
+  if (width <= 0 || height <= 0 || spp <= 0)
+    throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator::render: bad resolution or spp");
+  gpu::Context& ctx = volume.context();
+  flg_vol_scene sc;
+  std::memset(&sc, 0, sizeof(sc));
+  for (int k = 0; k < 3; ++k) {
+    sc.camera.pos[k] = origin[k];
+    sc.camera.look_at[k] = target[k];
+    sc.camera.up[k] = up[k];
+    sc.point_pos[k] = m_lights.point_pos[k];
+  }
+  sc.camera.fov_deg = fov;
+  sc.has_point = m_lights.has_point ? 1 : 0;
+  sc.point_intensity = m_lights.point_intensity;
+  sc.has_env = m_lights.has_env ? 1 : 0;
+  sc.env_radiance = m_lights.env_radiance;
+  flg_render_cfg cfg;
+  std::memset(&cfg, 0, sizeof(cfg));
+  cfg.width = width;
+  cfg.height = height;
+  cfg.spp = spp;
+  cfg.max_depth = m_maxDepth;
+  const size_t n = size_t(width) * height * 3;
+  gpu::DeviceBuffer d(ctx, n * sizeof(float));
+  ctx.check(flg_volpath_render(ctx.get(), volume.handle(), &sc, &cfg, static_cast<float*>(d.data()), m_stats),
+            "flg_volpath_render");
+  rgb.resize(n);
+  d.download(rgb.data(), n * sizeof(float));
+}
+
+void GPUVolIntegrator::render(const Scene& scene) {
+  const GridVolume* vol = dynamic_cast<const GridVolume*>(scene.m_volume);
+  if (!vol) throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator: scene.m_volume is not a GridVolume");
+  if (!scene.m_film) throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator: scene has no film");
+  std::vector<float> rgb;
+  render(*vol, scene.m_origin, scene.m_target, scene.m_up, scene.m_FoV, scene.m_resX, scene.m_resY,
+         scene.m_SPP, rgb);
+  for (int y = 0; y < scene.m_resY; ++y)
+    for (int x = 0; x < scene.m_resX; ++x) {
+      const float* c = &rgb[(size_t(y) * scene.m_resX + x) * 3];
+      scene.m_film->getBuffer(x, y, EFilmBufferType::EColor) = Vector3f(c[0], c[1], c[2]);
+    }
+}
+
+}  // namespace fledge

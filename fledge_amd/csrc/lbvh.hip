@@ -317,7 +317,9 @@ struct BuildOut {
   int32_t* parent;
   uint8_t* leafflags;
   int32_t* leaf_parent;
-  uint32_t* counter;
+  uint8_t* level;      // per global node: delta of its key range
+  int32_t* pslot;      // per global node: 2 * global parent + side, -1 for a root
+  int32_t* leaf_slot;  // per global leaf: 2 * global parent + side
 };
 
 __global__ __launch_bounds__(kBlock) void k_emit(const SegDesc* segs, BlockMap map, BuildOut o) {
@@ -348,10 +350,26 @@ __global__ __launch_bounds__(kBlock) void k_emit(const SegDesc* segs, BlockMap m
   o.rlo[nb + i] = lo;
   o.rhi[nb + i] = hi;
   o.leafflags[nb + i] = (uint8_t)((lleaf ? 1 : 0) | (rleaf ? 2 : 0));
-  o.counter[nb + i] = 0;
-  if (i == 0) o.parent[nb] = 0;
-  if (lleaf) o.leaf_parent[tb + gamma] = i; else o.parent[nb + gamma] = i;
-  if (rleaf) o.leaf_parent[tb + gamma + 1] = i; else o.parent[nb + gamma + 1] = i;
+  o.level[nb + i] = (uint8_t)dnode;  // common prefix length of the node's keys
+  if (i == 0) {
+    o.parent[nb] = 0;
+    o.pslot[nb] = -1;
+  }
+  const int slot0 = 2 * (nb + i);
+  if (lleaf) {
+    o.leaf_parent[tb + gamma] = i;
+    o.leaf_slot[tb + gamma] = slot0;
+  } else {
+    o.parent[nb + gamma] = i;
+    o.pslot[nb + gamma] = slot0;
+  }
+  if (rleaf) {
+    o.leaf_parent[tb + gamma + 1] = i;
+    o.leaf_slot[tb + gamma + 1] = slot0 + 1;
+  } else {
+    o.parent[nb + gamma + 1] = i;
+    o.pslot[nb + gamma + 1] = slot0 + 1;
+  }
   int4 refs;
   refs.x = lleaf ? (int)(kLeafBit | (uint32_t)gamma) : gamma;
   refs.y = rleaf ? (int)(kLeafBit | (uint32_t)(gamma + 1)) : gamma + 1;
@@ -366,36 +384,86 @@ FLG_D void store_slot(BNode* node, int side, V3 lo, V3 hi) {
   f[3] = hi.x; f[4] = hi.y; f[5] = hi.z;
 }
 
-// Bottom-up refit (radix_bvh.cpp:163-199): the first thread to reach a node
-// stops, the second merges both child boxes and climbs on.
-__global__ __launch_bounds__(kBlock) void k_refit(const SegDesc* segs, BlockMap map, BuildOut o) {
+// Bottom-up refit (the reference climbs leaf-to-root with fetch-and-add flags,
+// radix_bvh.cpp:163-199; min/max are exact, so any children-before-parent
+// order gives the same bounds).  Here the order comes from the node's common
+// prefix length delta, which is strictly larger in a child than in its
+// parent: leaves first, then the nodes of each delta level from 63 down to
+// 0, one launch per level, no atomics on the tree.
+__global__ __launch_bounds__(kBlock) void k_refit_leaves(const SegDesc* segs, BlockMap map,
+                                                         BuildOut o) {
   const int b = blockIdx.x;
   const SegDesc s = segs[map.seg[b]];
-  const int n = s.ntris;
   const int p = map.start[b] + (int)threadIdx.x;
-  if (p >= n || n < 2) return;
-  const int nb = s.node_base, tb = s.tri_base;
+  if (p >= s.ntris || s.ntris < 2) return;
   V3 lo, hi;
-  tri_bound(s, (int32_t)o.order[tb + p], &lo, &hi);
-  int child = p;
-  int cur = o.leaf_parent[tb + p];
-  // the climb is at most the tree depth (< 64 levels); the bound only guards
-  // against a corrupt parent chain
-  for (int level = 0; level < 128; ++level) {
-    BNode* node = o.nodes + nb + cur;
-    int side = (o.split[nb + cur] == child) ? 0 : 1;
-    store_slot(node, side, lo, hi);
-    uint32_t prev = __hip_atomic_fetch_add(o.counter + nb + cur, 1u, __ATOMIC_ACQ_REL,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    if (prev != 1) return;  // first arrival (0) stops; more than two arrivals cannot happen
-    const float* f = reinterpret_cast<const float*>(node);
+  tri_bound(s, (int32_t)o.order[s.tri_base + p], &lo, &hi);
+  const int slot = o.leaf_slot[s.tri_base + p];
+  store_slot(o.nodes + (slot >> 1), slot & 1, lo, hi);
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+// Counting sort of the internal nodes by delta level (64 buckets).
+__global__ __launch_bounds__(kBlock) void k_level_hist(const uint8_t* level, int64_t nnodes,
+                                                       uint32_t* hist) {
+  __shared__ uint32_t h[64];
+  if (threadIdx.x < 64) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nnodes; i += (int64_t)gridDim.x * kBlock)
+    atomicAdd(&h[level[i] & 63], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void k_level_scan(const uint32_t* hist, uint32_t* offs, uint32_t* fill) {
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int d = 0; d < 64; ++d) {
+      offs[d] = run;
+      fill[d] = run;
+      run += hist[d];
+    }
+    offs[64] = run;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_level_scatter(const uint8_t* level, int64_t nnodes,
+                                                          uint32_t* fill, uint32_t* list) {
+  __shared__ uint32_t h[64], base[64];
+  for (int64_t c0 = (int64_t)blockIdx.x * kBlock; c0 < nnodes; c0 += (int64_t)gridDim.x * kBlock) {
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t i = c0 + threadIdx.x;
+    uint32_t d = 0, r = 0;
+    if (i < nnodes) {
+      d = level[i] & 63;
+      r = atomicAdd(&h[d], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&fill[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (i < nnodes) list[base[d] + r] = (uint32_t)i;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refit_level(BuildOut o, const uint32_t* offs,
+                                                        const uint32_t* list, int d) {
+  const uint32_t b0 = offs[d], b1 = offs[d + 1];
+  for (uint32_t k = b0 + blockIdx.x * kBlock + threadIdx.x; k < b1; k += gridDim.x * kBlock) {
+    const uint32_t x = list[k];
+    const int slot = o.pslot[x];
+    if (slot < 0) continue;  // a segment root
+    const float* f = reinterpret_cast<const float*>(o.nodes + x);
     V3 alo = v3(f[0], f[1], f[2]), ahi = v3(f[3], f[4], f[5]);
     V3 clo = v3(f[6], f[7], f[8]), chi = v3(f[9], f[10], f[11]);
-    lo = vmin(vmin(v3(3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f), clo), alo);
-    hi = vmax(vmax(v3(-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f), chi), ahi);
-    if (cur == 0) return;
-    child = cur;
-    cur = o.parent[nb + cur];
+    // BVHBound{} merged with the other child, then with this one (radix_bvh.cpp:190-192)
+    V3 lo = vmin(vmin(v3(3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f), clo), alo);
+    V3 hi = vmax(vmax(v3(-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f), chi), ahi);
+    store_slot(o.nodes + (slot >> 1), slot & 1, lo, hi);
   }
 }
 
@@ -479,7 +547,9 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
   int64_t hist_n = 256 * tile_total;
   int64_t scan_blocks = (hist_n + kTile - 1) / kTile;
   size_t so_hist = stake(hist_n * 4), so_offs = stake(hist_n * 4), so_tot = stake(scan_blocks * 4 + 4);
-  size_t so_cnt = stake(NI * 4);
+  size_t so_level = stake(NI), so_pslot = stake(NI * 4), so_lslot = stake(N * 4);
+  size_t so_lhist = stake(64 * 4), so_loffs = stake(65 * 4), so_lfill = stake(64 * 4);
+  size_t so_llist = stake(NI * 4);
   char* sc = (char*)scratch(ctx, "lbvh_build", s_off ? s_off : 256, &st);
   if (!sc) {
     (void)hipFree(blockp);
@@ -519,7 +589,13 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
   o.parent = (int32_t*)(base + o_parent);
   o.leafflags = (uint8_t*)(base + o_lf);
   o.leaf_parent = (int32_t*)(base + o_lp);
-  o.counter = (uint32_t*)(sc + so_cnt);
+  o.level = (uint8_t*)(sc + so_level);
+  o.pslot = (int32_t*)(sc + so_pslot);
+  o.leaf_slot = (int32_t*)(sc + so_lslot);
+  uint32_t* lhist = (uint32_t*)(sc + so_lhist);
+  uint32_t* loffs = (uint32_t*)(sc + so_loffs);
+  uint32_t* lfill = (uint32_t*)(sc + so_lfill);
+  uint32_t* llist = (uint32_t*)(sc + so_llist);
   float* gb = (float*)(base + o_gb);
 
   if (blk_total > 0) {
@@ -554,7 +630,16 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
     }
     {
       ScopedTimer tm(ctx, "lbvh_refit");
-      k_refit<<<(int)blk_total, kBlock, 0, ctx->stream>>>(d_segs, bmap, o);
+      k_refit_leaves<<<(int)blk_total, kBlock, 0, ctx->stream>>>(d_segs, bmap, o);
+      if (NI > 0) {
+        const int g = std::max(1, std::min(div_up((int64_t)NI, kBlock), ctx->num_cus * 4));
+        k_zero_u32<<<1, 64, 0, ctx->stream>>>(lhist, 64);
+        k_level_hist<<<g, kBlock, 0, ctx->stream>>>(o.level, (int64_t)NI, lhist);
+        k_level_scan<<<1, 64, 0, ctx->stream>>>(lhist, loffs, lfill);
+        k_level_scatter<<<g, kBlock, 0, ctx->stream>>>(o.level, (int64_t)NI, lfill, llist);
+        for (int d = 63; d >= 0; --d)
+          k_refit_level<<<g, kBlock, 0, ctx->stream>>>(o, loffs, llist, d);
+      }
     }
     {
       ScopedTimer tm(ctx, "lbvh_soup");
@@ -682,6 +767,24 @@ int flg_lbvh_export(flg_ctx* ctx, flg_bvh* bvh, const flg_lbvh_view* v) {
       }
     }
   }
+  return FLG_OK;
+}
+
+int flg_lbvh_download(flg_ctx* ctx, const flg_bvh* bvh, void* h_nodes, size_t nodes_bytes,
+                      void* h_tris, size_t tris_bytes) {
+  if (!ctx || !bvh) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_download: bad arguments");
+  const size_t n = (size_t)bvh->n, ni = n > 0 ? n - 1 : 0;
+  if ((h_nodes && nodes_bytes != ni * sizeof(BNode)) || (h_tris && tris_bytes != n * sizeof(BTri)))
+    return set_error(ctx, FLG_ERR_INVALID,
+                     "flg_lbvh_download: nodes need " + std::to_string(ni * sizeof(BNode)) +
+                         " bytes and tris " + std::to_string(n * sizeof(BTri)) + " bytes");
+  if (h_nodes && ni)
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_nodes, bvh->nodes, nodes_bytes, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+  if (h_tris && n)
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_tris, bvh->tris, tris_bytes, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return FLG_OK;
 }
 

@@ -15,103 +15,11 @@
 
 #include "bvh.hpp"
 #include "halton.hpp"
+#include "traverse.hpp"
 
 namespace flg {
 
 namespace {
-
-constexpr int kStack = 64;  // > tree depth: delta strictly grows per level and is < 64
-
-struct TraceGeom {
-  const BNode* nodes;
-  const BTri* tris;
-  int32_t n;
-  float root[6];
-};
-
-TraceGeom geom_of(const flg_bvh* b) {
-  TraceGeom g;
-  g.nodes = b->nodes;
-  g.tris = b->tris;
-  g.n = b->n;
-  for (int k = 0; k < 6; ++k) g.root[k] = b->bound[k];
-  return g;
-}
-
-struct Counters {
-  uint32_t nodes, tris;
-};
-
-template <bool COMPAT, bool COUNT>
-FLG_D void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, int32_t& hit,
-                    Counters& c) {
-  const BTri t = g.tris[p];
-  if (COUNT) ++c.tris;
-  float th;
-  V3 ng;
-  if (!pluecker(v3(t.v0.x, t.v0.y, t.v0.z), v3(t.v1.x, t.v1.y, t.v1.z), v3(t.v2.x, t.v2.y, t.v2.z),
-                o, d, COMPAT, &th, &ng))
-    return;
-  if (th > best) return;
-  best = th;
-  hit = (int32_t)p;  // sorted position; the caller maps it to the original id
-}
-
-// Returns the sorted position of the hit (-1 on a miss); best holds t.
-template <bool COMPAT, bool COUNT>
-FLG_D int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c) {
-  int32_t hit = -1;
-  if (g.n <= 0) return -1;
-  const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  float tn;
-  if (!slab(g.root[0], g.root[1], g.root[2], g.root[3], g.root[4], g.root[5], o, inv, &tn)) return -1;
-  if (best < tn) return -1;
-  if (g.n == 1) {
-    test_tri<COMPAT, COUNT>(g, 0, o, d, best, hit, c);
-    return hit;
-  }
-  int32_t stack[kStack];
-  float stackt[kStack];
-  int sp = 0;
-  int32_t node = 0;
-  // every node is visited at most once; the bound only guards a corrupt tree
-  for (int64_t visits = 0; visits < (int64_t)g.n; ++visits) {
-    if (COUNT) ++c.nodes;
-    const BNode nd = g.nodes[node];
-    const int32_t refL = nd.d.x, refR = nd.d.y;
-    if (refL < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refL & 0x7fffffffu, o, d, best, hit, c);
-    if (refR < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refR & 0x7fffffffu, o, d, best, hit, c);
-    float tL = 0.0f, tR = 0.0f;
-    bool goL = refL >= 0 && slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, &tL) &&
-               !(best < tL);
-    bool goR = refR >= 0 && slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, &tR) &&
-               !(best < tR);
-    if (goL) {
-      if (goR && sp < kStack) {
-        stack[sp] = refR;
-        stackt[sp] = tR;
-        ++sp;
-      }
-      node = refL;
-      continue;
-    }
-    if (goR) {
-      node = refR;
-      continue;
-    }
-    bool found = false;
-    while (sp > 0) {
-      --sp;
-      if (!(best < stackt[sp])) {
-        node = stack[sp];
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
-  }
-  return hit;
-}
 
 template <bool COMPAT>
 __global__ __launch_bounds__(256) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
@@ -247,8 +155,6 @@ struct QRay {  // 32 bytes: origin + path id, direction + Halton index
 
 struct PtState {
   QRay* q[2];
-  int32_t* hit;   // per queue slot: sorted position or -1
-  float* hit_t;
   uint32_t* count;  // [max_depth + 2] queue sizes
   int32_t* nseg;
   int32_t* seg_id;
@@ -259,64 +165,6 @@ struct PtState {
 __global__ __launch_bounds__(256) void k_pt_zero(uint32_t* count, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) count[i] = 0;
-}
-
-__global__ __launch_bounds__(256) void k_pt_gen(CamDev cam, HaltonTables tab, HaltonParams hp,
-                                                int32_t spp, QRay* q, uint32_t* count) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t npaths = (int64_t)cam.w * cam.h * spp;
-  if (p == 0) count[0] = (uint32_t)npaths;
-  if (p >= npaths) return;
-  int64_t pix = p / spp;
-  uint32_t s = (uint32_t)(p % spp);
-  int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
-  PathSampler ps;
-  ps.index = halton_pixel_offset(hp, x, y) + s * hp.stride;
-  ps.dim = 0;
-  float ux = ps.next(tab, hp);
-  float uy = ps.next(tab, hp);
-  V3 o, d;
-  gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
-  QRay r;
-  r.o = make_float4(o.x, o.y, o.z, u2f((uint32_t)p));
-  r.d = make_float4(d.x, d.y, d.z, u2f(ps.index));
-  q[p] = r;
-}
-
-template <bool COMPAT, bool COUNT>
-__global__ __launch_bounds__(256) void k_pt_trace(TraceGeom g, const QRay* q,
-                                                  const uint32_t* count, int bounce, int32_t* hit,
-                                                  float* hit_t, unsigned long long* visits) {
-  const uint32_t n = count[bounce];
-  Counters c{0, 0};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const QRay r = q[i];
-    float best = __builtin_inff();
-    int32_t p = trace_one<COMPAT, COUNT>(g, v3(r.o.x, r.o.y, r.o.z), v3(r.d.x, r.d.y, r.d.z), best, c);
-    hit[i] = p;
-    hit_t[i] = best;
-  }
-  if (COUNT) {
-    unsigned long long cn = c.nodes, ct = c.tris;
-    for (int o = 32; o >= 1; o >>= 1) {
-      cn += __shfl_xor(cn, o, 64);
-      ct += __shfl_xor(ct, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(visits, cn);
-      atomicAdd(visits + 1, ct);
-    }
-  }
-}
-
-template <bool COMPAT>
-void launch_pt_trace(bool count, int grid, int block, hipStream_t st, const TraceGeom& g,
-                     const QRay* q, const uint32_t* cnt, int b, int32_t* hit, float* hit_t,
-                     unsigned long long* visits) {
-  if (count)
-    k_pt_trace<COMPAT, true><<<grid, block, 0, st>>>(g, q, cnt, b, hit, hit_t, visits);
-  else
-    k_pt_trace<COMPAT, false><<<grid, block, 0, st>>>(g, q, cnt, b, hit, hit_t, visits);
 }
 
 // CoordinateTransition (src/materials/material_base.hpp:14-41)
@@ -333,37 +181,63 @@ struct Frame {
   FLG_D V3 to_world(V3 p) const { return (t * p.x + b * p.y) + n * p.z; }
 };
 
-template <bool COMPAT>
-__global__ __launch_bounds__(256) void k_pt_shade(TraceGeom g, PtState s, HaltonTables tab,
-                                                  HaltonParams hp, int bounce, int max_depth) {
-  const uint32_t n = s.count[bounce];
+// One bounce of every live path: (bounce 0) camera ray, closest hit, segment
+// record, diffuse sampling in the shading frame and the spawn of the next
+// segment into the other queue (wave-aggregated append).  Persistent grid:
+// the queue length is read from device memory, so the host never waits.
+template <bool COMPAT, bool FIRST, bool COUNT>
+__global__ __launch_bounds__(256) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
+                                                   HaltonTables tab, HaltonParams hp, int32_t spp,
+                                                   int bounce, int max_depth,
+                                                   unsigned long long* visits) {
+  const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
+  if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
   const QRay* qin = s.q[bounce & 1];
   QRay* qout = s.q[(bounce + 1) & 1];
+  Counters c{0, 0};
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
+    const uint32_t i = base + threadIdx.x;
     bool spawn = false;
     QRay nr;
     if (i < n) {
-      const QRay r = qin[i];
-      const uint32_t path = f2u(r.o.w);
-      const int32_t p = s.hit[i];
-      const float t = s.hit_t[i];
+      V3 o, d;
+      uint32_t path, hidx;
+      if (FIRST) {
+        path = i;
+        const uint32_t pix = i / (uint32_t)spp, smp = i - pix * (uint32_t)spp;
+        const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+        PathSampler ps;
+        ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+        ps.dim = 0;
+        float ux = ps.next(tab, hp);
+        float uy = ps.next(tab, hp);
+        gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
+        hidx = ps.index;
+      } else {
+        const QRay r = qin[i];
+        o = v3(r.o.x, r.o.y, r.o.z);
+        d = v3(r.d.x, r.d.y, r.d.z);
+        path = f2u(r.o.w);
+        hidx = f2u(r.d.w);
+      }
+      float t = __builtin_inff();
+      const int32_t p = trace_one<COMPAT, COUNT>(g, o, d, t, c);
+      BTri tri;
+      if (p >= 0) tri = g.tris[p];
       if (s.seg_id) {
         int64_t k = (int64_t)path * s.stride_seg + bounce;
-        s.seg_id[k] = p >= 0 ? (int32_t)f2u(g.tris[p].v0.w) : -1;
+        s.seg_id[k] = p >= 0 ? (int32_t)f2u(tri.v0.w) : -1;
         s.seg_t[k] = t;
       }
       bool end = true;
       if (p >= 0 && bounce < max_depth) {
-        const V3 o = v3(r.o.x, r.o.y, r.o.z), d = v3(r.d.x, r.d.y, r.d.z);
-        const BTri tri = g.tris[p];
         const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
                  v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
         const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
         const V3 hp3 = o + d * t;
         Frame fr(ng);
         PathSampler ps;
-        ps.index = f2u(r.d.w);
+        ps.index = hidx;
         ps.dim = 2 + 2 * (uint32_t)bounce;
         float u0 = ps.next(tab, hp);
         float u1 = ps.next(tab, hp);
@@ -374,15 +248,14 @@ __global__ __launch_bounds__(256) void k_pt_shade(TraceGeom g, PtState s, Halton
         if (pdf != 0.0f) {
           V3 no = offset_ray_origin(hp3, ng, wi);
           V3 nd = normalize(wi);
-          nr.o = make_float4(no.x, no.y, no.z, r.o.w);
-          nr.d = make_float4(nd.x, nd.y, nd.z, r.d.w);
+          nr.o = make_float4(no.x, no.y, no.z, u2f(path));
+          nr.d = make_float4(nd.x, nd.y, nd.z, u2f(hidx));
           spawn = true;
           end = false;
         }
       }
       if (end && s.nseg) s.nseg[path] = bounce + 1;
     }
-    // wave-aggregated append to the next queue
     uint64_t m = __ballot(spawn);
     if (m) {
       const int lane = threadIdx.x & 63;
@@ -394,6 +267,34 @@ __global__ __launch_bounds__(256) void k_pt_shade(TraceGeom g, PtState s, Halton
         qout[wbase + (uint32_t)__popcll(m & lt)] = nr;
       }
     }
+  }
+  if (COUNT) {
+    unsigned long long cn = c.nodes, ct = c.tris;
+    for (int o = 32; o >= 1; o >>= 1) {
+      cn += __shfl_xor(cn, o, 64);
+      ct += __shfl_xor(ct, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(visits, cn);
+      atomicAdd(visits + 1, ct);
+    }
+  }
+}
+
+template <bool COMPAT>
+void launch_bounce(bool first, bool count, int grid, int block, hipStream_t st, const TraceGeom& g,
+                   const PtState& s, const CamDev& cam, const HaltonTables& tab,
+                   const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
+  if (first) {
+    if (count)
+      k_pt_bounce<COMPAT, true, true><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+    else
+      k_pt_bounce<COMPAT, true, false><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+  } else {
+    if (count)
+      k_pt_bounce<COMPAT, false, true><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+    else
+      k_pt_bounce<COMPAT, false, false><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
   }
 }
 
@@ -482,8 +383,6 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   PtState s;
   s.q[0] = (QRay*)sc;
   s.q[1] = (QRay*)(sc + qbytes);
-  s.hit = (int32_t*)(sc + 2 * qbytes);
-  s.hit_t = (float*)(sc + 2 * qbytes + (size_t)npaths * 4);
   s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
   unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
   const bool count_visits = cfg->count_visits != 0;
@@ -499,21 +398,14 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     ScopedTimer tm(ctx, "pathtrace_total");
     k_pt_zero<<<1, 256, 0, ctx->stream>>>(s.count, D + 2);
     if (count_visits) k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)visits, 4);
-    k_pt_gen<<<div_up(npaths, 256), 256, 0, ctx->stream>>>(cd, tab, hp, cfg->spp, s.q[0], s.count);
     for (int b = 0; b <= D; ++b) {
-      {
-        ScopedTimer tt(ctx, "pathtrace_trace");
-        if (compat)
-          launch_pt_trace<true>(count_visits, pgrid, tb, ctx->stream, g, s.q[b & 1], s.count, b,
-                                s.hit, s.hit_t, visits);
-        else
-          launch_pt_trace<false>(count_visits, pgrid, tb, ctx->stream, g, s.q[b & 1], s.count, b,
-                                 s.hit, s.hit_t, visits);
-      }
+      ScopedTimer tt(ctx, "pathtrace_bounce");
       if (compat)
-        k_pt_shade<true><<<pgrid, 256, 0, ctx->stream>>>(g, s, tab, hp, b, D);
+        launch_bounce<true>(b == 0, count_visits, pgrid, tb, ctx->stream, g, s, cd, tab, hp,
+                            cfg->spp, b, D, visits);
       else
-        k_pt_shade<false><<<pgrid, 256, 0, ctx->stream>>>(g, s, tab, hp, b, D);
+        launch_bounce<false>(b == 0, count_visits, pgrid, tb, ctx->stream, g, s, cd, tab, hp,
+                             cfg->spp, b, D, visits);
     }
   }
   FLG_CHECK_LAUNCH(ctx);

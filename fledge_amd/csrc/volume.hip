@@ -10,32 +10,11 @@
 
 #include "volume.hpp"
 
-struct flg_volume {
-  flg_ctx* ctx = nullptr;
-  int32_t nx = 0, ny = 0, nz = 0;
-  float bounds[6];
-  flg_medium medium;
-  float* grid = nullptr;  // padded
-  float max_density = 0;
-  // environment-light tables, rebuilt when the radiance changes
-  float* env = nullptr;
-  float env_radiance = -1.0f;
-  float env_marg_int = 0;
-  float env_center[3] = {0, 0, 0};
-  float env_radius = 0;
-};
-
 namespace flg {
 
 namespace {
 
 constexpr int kEnvN = 512;
-// Free-flight steps per tracking call.  A Halton index of 0 (pixel offset 0,
-// sample 0) makes every dimension 0, so t never advances and the reference's
-// loops never end; the cap ends such a call (ratio tracking returns the
-// current transmittance, delta tracking reports no collision).
-constexpr int kMaxTrackSteps = 4096;
-
 __global__ __launch_bounds__(256) void k_pad(const float* src, float* dst, int nx, int ny, int nz) {
   const int px = nx + 2, py = ny + 2, pz = nz + 2;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,74 +97,13 @@ int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
   return FLG_OK;
 }
 
-// Ratio tracking with Russian roulette (OpenVDBVolume::tr, volume.cpp:75-107)
-template <bool COUNT>
-FLG_D float vol_tr(const VolDev& v, const HaltonTables& tab, const HaltonParams& hp,
-                   PathSampler& ps, V3 o, V3 d, float ray_tmax, uint32_t& nlook) {
-  float t_min, t_max;
-  if (!box_pbrt(v.bmin, v.bmax, o, d, ray_tmax, &t_min, &t_max)) return 1.0f;
-  float tr = 1.0f, t = smax(t_min, 0.0f);
-  t_max = smin(t_max, ray_tmax);
-  const float max_mu = v.max_mu;
-  if (!(max_mu > 0.0f)) return 1.0f;  // empty medium: no interaction
-  for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - ps.next(tab, hp)) / max_mu;
-    if (t >= t_max) break;
-    V3 pos = o + d * t;
-    float density = lookup(v, pos).rho;
-    if (COUNT) ++nlook;
-    tr *= 1.0f - smax(0.0f, density * v.inv_max_density);
-    if (tr < 0.1f) {
-      float q = smax(0.05f, 1.0f - tr);
-      if (ps.next(tab, hp) < q) return 0.0f;
-      tr /= 1.0f - q;
-    }
-  }
-  return tr;
-}
-
-// Delta tracking (OpenVDBVolume::sample, volume.cpp:109-148)
-template <bool COUNT>
-FLG_D float vol_sample(const VolDev& v, const HaltonTables& tab, const HaltonParams& hp,
-                       PathSampler& ps, V3 o, V3 d, bool* success, V3* pos_out,
-                       uint32_t& nlook) {
-  float t_min, t_max;
-  if (!box_pbrt(v.bmin, v.bmax, o, d, __builtin_inff(), &t_min, &t_max)) {
-    *success = false;
-    return 1.0f;
-  }
-  float t = smax(t_min, 0.0f);
-  const float max_mu = v.max_mu;
-  t_max = smin(t_max, __builtin_inff());
-  if (!(max_mu > 0.0f)) {
-    *success = false;
-    return 1.0f;
-  }
-  for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - ps.next(tab, hp)) / max_mu;
-    if (t >= t_max) break;
-    V3 pos = o + d * t;
-    float density = lookup(v, pos).rho;
-    if (COUNT) ++nlook;
-    float p_s = v.sigma_s * density / max_mu;
-    float p_a = v.sigma_a * density / max_mu;
-    float p_n = 1.0f - p_s - p_a;
-    if (ps.next(tab, hp) > p_n) {
-      *success = true;
-      *pos_out = pos;
-      return v.sigma_s / v.sigma_t;
-    }
-  }
-  *success = false;
-  return 1.0f;
-}
-
 // UniformSampleOneLight + EstimateDirect for a volume vertex
 // (integrator.cpp:69-120); lights ordered [point, environment], p = 1/n.
 template <bool COUNT>
 FLG_D float sample_one_light(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
                              const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo,
                              uint32_t& nlook) {
+  HaltonDraw rng{ps, tab, hp};
   const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
   if (n_lights == 0) return 0.0f;
   int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
@@ -213,7 +131,7 @@ FLG_D float sample_one_light(const VolDev& v, const LightsDev& L, const HaltonTa
     V3 diff = p_light - p;
     float nrm = sqrtf(sqnorm(diff));
     V3 d = diff / nrm;
-    float tr = vol_tr<COUNT>(v, tab, hp, ps, p, d, nrm - SHADOW_EPS, nlook);
+    float tr = vol_tr<COUNT>(v, rng, p, d, nrm - SHADOW_EPS, nlook);
     Li *= tr;
     out += f * Li / pdf;
   }
@@ -225,6 +143,7 @@ template <bool COUNT>
 FLG_D float svol_li(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
                     const HaltonParams& hp, PathSampler& ps, V3 o, V3 d, int max_depth,
                     uint32_t& nlook) {
+  HaltonDraw rng{ps, tab, hp};
   float Lr = 0.0f, beta = 1.0f;
   const float rr = 0.01f;
   float t_min, t_max;
@@ -233,14 +152,14 @@ FLG_D float svol_li(const VolDev& v, const LightsDev& L, const HaltonTables& tab
     return Lr;
   }
   {
-    float tr = vol_tr<COUNT>(v, tab, hp, ps, o, d, __builtin_inff(), nlook);
+    float tr = vol_tr<COUNT>(v, rng, o, d, __builtin_inff(), nlook);
     if (L.env.present) Lr += tr * L.env.radiance;
     o = o + d * t_min;
   }
   for (int bounces = 1;; ++bounces) {
     bool success = false;
     V3 pos;
-    float f = vol_sample<COUNT>(v, tab, hp, ps, o, d, &success, &pos, nlook);
+    float f = vol_sample<COUNT>(v, rng, o, d, kInf, &success, &pos, nlook);
     if (!success) break;
     beta = beta * f;
     V3 wo = -d;
@@ -257,36 +176,6 @@ FLG_D float svol_li(const VolDev& v, const LightsDev& L, const HaltonTables& tab
     }
   }
   return Lr;
-}
-
-struct CamDevV {
-  V3 pos, forward, right, up2;
-  float fov_ratio, aspect_fov;
-  int32_t w, h;
-};
-
-FLG_D void gen_ray_v(const CamDevV& c, float x, float y, V3* o, V3* d) {
-  float y_offset = (float)((double)c.fov_ratio * ((double)((y / (float)c.h) * 2.0f) - 1.0));
-  float x_offset = (float)((double)c.aspect_fov * ((double)((x / (float)c.w) * 2.0f) - 1.0));
-  V3 dir = (c.forward + c.right * x_offset) + c.up2 * y_offset;
-  *o = c.pos;
-  *d = normalize(dir);
-}
-
-CamDevV make_cam_v(const flg_camera* c, int w, int h) {
-  CamDevV cd;
-  cd.pos = v3(c->pos[0], c->pos[1], c->pos[2]);
-  V3 look = v3(c->look_at[0], c->look_at[1], c->look_at[2]);
-  V3 up = v3(c->up[0], c->up[1], c->up[2]);
-  cd.forward = normalize(look - cd.pos);
-  cd.right = normalize(cross(up, cd.forward));
-  cd.up2 = normalize(cross(cd.forward, cd.right));
-  cd.fov_ratio = tanf(PI * c->fov_deg / 360.0f);
-  float aspect = (float)w / (float)h;
-  cd.aspect_fov = cd.fov_ratio * aspect;
-  cd.w = w;
-  cd.h = h;
-  return cd;
 }
 
 template <bool COUNT>
@@ -338,26 +227,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 }  // namespace
 
 VolDev make_voldev(const flg_volume* vol) {
-  VolDev v;
-  v.grid = vol->grid;
-  v.nx = vol->nx;
-  v.ny = vol->ny;
-  v.nz = vol->nz;
-  v.px = vol->nx + 2;
-  v.pxy = (vol->nx + 2) * (vol->ny + 2);
-  v.bmin = v3(vol->bounds[0], vol->bounds[1], vol->bounds[2]);
-  v.bmax = v3(vol->bounds[3], vol->bounds[4], vol->bounds[5]);
-  v.scale = v3((float)vol->nx / (v.bmax.x - v.bmin.x), (float)vol->ny / (v.bmax.y - v.bmin.y),
-               (float)vol->nz / (v.bmax.z - v.bmin.z));
-  // OpenVDBVolume ctor (volume.cpp:33-59): sigma_t = sigma_a + sigma_s, the
-  // inverse max density through a double division.
-  v.sigma_s = vol->medium.sigma_s;
-  v.sigma_a = vol->medium.sigma_a;
-  v.sigma_t = v.sigma_a + v.sigma_s;
-  v.g = vol->medium.g;
-  v.inv_max_density = (float)(1.0 / (double)vol->max_density);
-  v.max_mu = v.sigma_t * vol->max_density;
-  return v;
+  return voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, vol->bounds, vol->medium, vol->max_density);
 }
 
 int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L) {

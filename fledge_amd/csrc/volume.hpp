@@ -4,6 +4,21 @@
 #include "common.hpp"
 #include "halton.hpp"
 
+struct flg_volume {
+  flg_ctx* ctx = nullptr;
+  int32_t nx = 0, ny = 0, nz = 0;
+  float bounds[6];
+  flg_medium medium;
+  float* grid = nullptr;  // padded
+  float max_density = 0;
+  // environment-light tables, rebuilt when the radiance changes
+  float* env = nullptr;
+  float env_radiance = -1.0f;
+  float env_marg_int = 0;
+  float env_center[3] = {0, 0, 0};
+  float env_radius = 0;
+};
+
 namespace flg {
 
 // Dense grid in HBM with a one-voxel zero border: (nx+2)(ny+2)(nz+2) floats,
@@ -17,6 +32,31 @@ struct VolDev {
   float inv_max_density, max_mu;
 };
 
+// Device (or host-mirror) description of a padded grid.  OpenVDBVolume ctor
+// (volume.cpp:33-59): sigma_t = sigma_a + sigma_s, the inverse max density
+// through a double division.
+inline VolDev voldev_of(const float* padded, int32_t nx, int32_t ny, int32_t nz,
+                        const float* bounds, flg_medium m, float max_density) {
+  VolDev v;
+  v.grid = padded;
+  v.nx = nx;
+  v.ny = ny;
+  v.nz = nz;
+  v.px = nx + 2;
+  v.pxy = (nx + 2) * (ny + 2);
+  v.bmin = v3(bounds[0], bounds[1], bounds[2]);
+  v.bmax = v3(bounds[3], bounds[4], bounds[5]);
+  v.scale = v3((float)nx / (v.bmax.x - v.bmin.x), (float)ny / (v.bmax.y - v.bmin.y),
+               (float)nz / (v.bmax.z - v.bmin.z));
+  v.sigma_s = m.sigma_s;
+  v.sigma_a = m.sigma_a;
+  v.sigma_t = v.sigma_a + v.sigma_s;
+  v.g = m.g;
+  v.inv_max_density = (float)(1.0 / (double)max_density);
+  v.max_mu = v.sigma_t * max_density;
+  return v;
+}
+
 struct Tri8 {  // one trilinear lookup: padded base index and weights
   int32_t base;  // -1 when the lookup is outside the grid (density 0)
   float fx, fy, fz;
@@ -26,7 +66,7 @@ struct Tri8 {  // one trilinear lookup: padded base index and weights
 // Trilinear lookup: index space g = (p - bmin) * scale - 0.5, zero outside the
 // grid, lerps along x, then y, then z (the library's definition of the dense
 // grid's interpolation; OpenVDB's BoxSampler is not in the reference tree).
-FLG_D Tri8 lookup(const VolDev& v, V3 p) {
+FLG_HD Tri8 lookup(const VolDev& v, V3 p) {
   float gx = (p.x - v.bmin.x) * v.scale.x - 0.5f;
   float gy = (p.y - v.bmin.y) * v.scale.y - 0.5f;
   float gz = (p.z - v.bmin.z) * v.scale.z - 0.5f;
@@ -62,7 +102,7 @@ FLG_D Tri8 lookup(const VolDev& v, V3 p) {
 }
 
 // AABB::intersect_pbrt (src/common/aabb.h:38-59)
-FLG_D bool box_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max) {
+FLG_HD bool box_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max) {
   float t0 = 0.0f, t1 = tmax;
   float los[3] = {lo.x, lo.y, lo.z}, his[3] = {hi.x, hi.y, hi.z};
   float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
@@ -82,7 +122,7 @@ FLG_D bool box_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t
 }
 
 // AABB::intersect (aabb.h:20-36) with the inverse through the double literal
-FLG_D bool box_enter(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
+FLG_HD bool box_enter(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
   V3 inv = v3((float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z));
   V3 t1 = (lo - o) * inv;
   V3 t2 = (hi - o) * inv;
@@ -99,7 +139,7 @@ FLG_D bool box_enter(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
 }
 
 // HGP (math_utils.h:103-107)
-FLG_D float hg_p(V3 wi, V3 wo, float g) {
+FLG_HD float hg_p(V3 wi, V3 wo, float g) {
   float cos_theta = dot(wi, wo);
   float denom = 1.0f + g * g + 2.0f * g * cos_theta;
   return (1.0f - g * g) / (denom * sqrtf(denom)) * INV_4PI;
@@ -107,7 +147,7 @@ FLG_D float hg_p(V3 wi, V3 wo, float g) {
 
 // CoordinateSystem + HGSampleP (math_utils.h:57-66, 112-128); u is the first
 // draw (left-to-right argument evaluation at integrator.cpp:378).
-FLG_D V3 hg_sample(V3 wo, float u, float vv, float g) {
+FLG_HD V3 hg_sample(V3 wo, float u, float vv, float g) {
   float cos_theta;
   if ((double)fabsf(g) < 1e-3) {
     cos_theta = 1.0f - 2.0f * u;
@@ -132,6 +172,82 @@ FLG_D V3 hg_sample(V3 wo, float u, float vv, float g) {
 
 // Environment light tables (Dist2D of InfiniteAreaLight, light.cpp:93-108,
 // distribution.hpp:12-92), 512 x 512.
+// Free-flight steps per tracking call.  A Halton index of 0 (pixel offset 0,
+// sample 0) makes every dimension 0, so t never advances and the reference's
+// loops never end; the cap ends such a call (ratio tracking returns the
+// current transmittance, delta tracking reports no collision).
+constexpr int kMaxTrackSteps = 4096;
+constexpr float kInf = __builtin_huge_valf();
+
+// Draw source of the trackers on the device: the path's Halton stream.  The
+// host drop-ins instantiate the same trackers over a host-side sampler.
+struct HaltonDraw {
+  PathSampler& ps;
+  const HaltonTables& tab;
+  const HaltonParams& hp;
+  FLG_D float next() { return ps.next(tab, hp); }
+};
+
+// Ratio tracking with Russian roulette (OpenVDBVolume::tr, volume.cpp:75-107)
+template <bool COUNT, class Draw>
+FLG_HD float vol_tr(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, uint32_t& nlook) {
+  float t_min, t_max;
+  if (!box_pbrt(v.bmin, v.bmax, o, d, ray_tmax, &t_min, &t_max)) return 1.0f;
+  float tr = 1.0f, t = smax(t_min, 0.0f);
+  t_max = smin(t_max, ray_tmax);
+  const float max_mu = v.max_mu;
+  if (!(max_mu > 0.0f)) return 1.0f;  // empty medium: no interaction
+  for (int step = 0; step < kMaxTrackSteps; ++step) {
+    t += -flg_logf(1.0f - rng.next()) / max_mu;
+    if (t >= t_max) break;
+    V3 pos = o + d * t;
+    float density = lookup(v, pos).rho;
+    if (COUNT) ++nlook;
+    tr *= 1.0f - smax(0.0f, density * v.inv_max_density);
+    if (tr < 0.1f) {
+      float q = smax(0.05f, 1.0f - tr);
+      if (rng.next() < q) return 0.0f;
+      tr /= 1.0f - q;
+    }
+  }
+  return tr;
+}
+
+// Delta tracking (OpenVDBVolume::sample, volume.cpp:109-148)
+template <bool COUNT, class Draw>
+FLG_HD float vol_sample(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, bool* success,
+                        V3* pos_out, uint32_t& nlook) {
+  float t_min, t_max;
+  if (!box_pbrt(v.bmin, v.bmax, o, d, ray_tmax, &t_min, &t_max)) {
+    *success = false;
+    return 1.0f;
+  }
+  float t = smax(t_min, 0.0f);
+  const float max_mu = v.max_mu;
+  t_max = smin(t_max, ray_tmax);
+  if (!(max_mu > 0.0f)) {
+    *success = false;
+    return 1.0f;
+  }
+  for (int step = 0; step < kMaxTrackSteps; ++step) {
+    t += -flg_logf(1.0f - rng.next()) / max_mu;
+    if (t >= t_max) break;
+    V3 pos = o + d * t;
+    float density = lookup(v, pos).rho;
+    if (COUNT) ++nlook;
+    float p_s = v.sigma_s * density / max_mu;
+    float p_a = v.sigma_a * density / max_mu;
+    float p_n = 1.0f - p_s - p_a;
+    if (rng.next() > p_n) {
+      *success = true;
+      *pos_out = pos;
+      return v.sigma_s / v.sigma_t;
+    }
+  }
+  *success = false;
+  return 1.0f;
+}
+
 struct EnvDev {
   const float* marg_cdf;  // 513
   const float* marg_f;    // 512
@@ -189,5 +305,40 @@ struct LightsDev {
   V3 point_pos;
   float point_intensity;
 };
+
+// Camera (src/common/camera.h:11-31): ctor on the host in float, per-ray
+// offsets through the double literal as in generateRay.
+struct CamDevV {
+  V3 pos, forward, right, up2;
+  float fov_ratio, aspect_fov;
+  int32_t w, h;
+};
+
+FLG_D void gen_ray_v(const CamDevV& c, float x, float y, V3* o, V3* d) {
+  float y_offset = (float)((double)c.fov_ratio * ((double)((y / (float)c.h) * 2.0f) - 1.0));
+  float x_offset = (float)((double)c.aspect_fov * ((double)((x / (float)c.w) * 2.0f) - 1.0));
+  V3 dir = (c.forward + c.right * x_offset) + c.up2 * y_offset;
+  *o = c.pos;
+  *d = normalize(dir);
+}
+
+inline CamDevV make_cam_v(const flg_camera* c, int w, int h) {
+  CamDevV cd;
+  cd.pos = v3(c->pos[0], c->pos[1], c->pos[2]);
+  V3 look = v3(c->look_at[0], c->look_at[1], c->look_at[2]);
+  V3 up = v3(c->up[0], c->up[1], c->up[2]);
+  cd.forward = normalize(look - cd.pos);
+  cd.right = normalize(cross(up, cd.forward));
+  cd.up2 = normalize(cross(cd.forward, cd.right));
+  cd.fov_ratio = tanf(PI * c->fov_deg / 360.0f);
+  float aspect = (float)w / (float)h;
+  cd.aspect_fov = cd.fov_ratio * aspect;
+  cd.w = w;
+  cd.h = h;
+  return cd;
+}
+
+VolDev make_voldev(const flg_volume* vol);
+int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L);
 
 }  // namespace flg

@@ -1,0 +1,156 @@
+"""Differentiable volume rendering on the GPU through the C ABI.
+
+`render(vol_ctx, density, albedo, ...)` is a torch.autograd.Function whose
+forward calls flg_volpath_diff_forward and whose backward calls
+flg_volpath_adjoint (path-replay adjoint): gradients with respect to the
+density grid and the single-scatter albedo.
+"""
+import ctypes
+
+import torch
+
+from . import _native as N
+from .api import Volume, _ptr
+
+
+class DiffVolume:
+    """Holds the native volume (evaluated grid) and, optionally, a separate
+    proposal volume for the collision decisions."""
+
+    def __init__(self, ctx, density, bounds=(0, 0, 0, 1, 1, 1), g=0.3):
+        self.ctx = ctx
+        self.bounds = bounds
+        self.g = g
+        self.vol = Volume(ctx, density.detach(), bounds, 1.0, 0.0, g)
+        self.proposal = None
+
+    def set_density(self, density):
+        self.vol.update(density.detach())
+
+    def set_proposal(self, density):
+        if density is None:
+            self.proposal = None
+        else:
+            self.proposal = Volume(self.ctx, density.detach(), self.bounds, 1.0, 0.0, self.g)
+
+
+def _params(s, albedo, g, majorant):
+    p = N.DiffParams()
+    p.sigma_t_scale = float(s)
+    p.albedo = float(albedo)
+    p.g = float(g)
+    p.majorant = float(majorant)
+    return p
+
+
+def _cfg(w, h, spp, depth):
+    c = N.RenderCfg()
+    c.width, c.height, c.spp, c.max_depth = w, h, spp, depth
+    return c
+
+
+def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None):
+    ctx = dv.ctx
+    dev = torch.device("cuda", ctx.device)
+    if out is None:
+        out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
+    if lpath is None:
+        lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
+    prop = dv.proposal.h if dv.proposal is not None else None
+    ctx.check(ctx.lib.flg_volpath_diff_forward(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
+                                               ctypes.byref(_cfg(w, h, spp, depth)),
+                                               ctypes.byref(_params(s, albedo, dv.g, majorant)),
+                                               _ptr(out), _ptr(lpath)))
+    return out, lpath
+
+
+def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None):
+    ctx = dv.ctx
+    dev = torch.device("cuda", ctx.device)
+    nz, ny, nx = dv.vol.shape
+    if grad is None:
+        grad = torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev)
+    if galpha is None:
+        galpha = torch.zeros(1, dtype=torch.float32, device=dev)
+    prop = dv.proposal.h if dv.proposal is not None else None
+    dl = dl_drgb.contiguous()
+    ctx.check(ctx.lib.flg_volpath_adjoint(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
+                                          ctypes.byref(_cfg(w, h, spp, depth)),
+                                          ctypes.byref(_params(s, albedo, dv.g, majorant)),
+                                          _ptr(lpath), _ptr(dl), _ptr(grad), _ptr(galpha)))
+    return grad, galpha
+
+
+class VolumeRender(torch.autograd.Function):
+    """image = VolumeRender.apply(density, albedo, dv, scene, w, h, spp, depth, s, majorant)
+
+    density: (nz, ny, nx) float32 CUDA tensor (the grid evaluated and used as
+    the proposal); albedo: 0-d float32 tensor."""
+
+    @staticmethod
+    def forward(ctx_, density, albedo, dv, scene, w, h, spp, depth, s, majorant):
+        dv.set_density(density)
+        a = float(albedo.item())
+        img, lpath = forward(dv, scene, w, h, spp, depth, s, a, majorant)
+        ctx_.save_for_backward(lpath)
+        ctx_.args = (dv, scene, w, h, spp, depth, s, a, majorant)
+        return img
+
+    @staticmethod
+    def backward(ctx_, grad_img):
+        (lpath,) = ctx_.saved_tensors
+        dv, scene, w, h, spp, depth, s, a, majorant = ctx_.args
+        grad, galpha = adjoint(dv, scene, w, h, spp, depth, s, a, majorant, lpath,
+                               grad_img.float().contiguous())
+        return grad, galpha.reshape(()), None, None, None, None, None, None, None, None
+
+
+def bench_adjoint(ctx, timed, sync):
+    """Config 3: 128^3 grid initialised to 0.5, 256x256, 16 spp, L2 loss against
+    a target rendered from a different fBm grid; point light and environment
+    as in the volumetric config, albedo 0.8, g 0.3, sigma_t = 8 density."""
+    from . import api
+
+    dev = torch.device("cuda", ctx.device)
+    W = H = 256
+    SPP, DEPTH, S = 16, 32, 8.0
+    cam = N.Camera.make((0.5, 0.5, -1.5), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0), 45.0)
+    scene = api.vol_scene(cam)
+    target_grid = torch.from_numpy(api.gen_fbm_grid(128, seed=2)).to(dev)
+    tv = DiffVolume(ctx, target_grid)
+    target, _ = forward(tv, scene, W, H, SPP, DEPTH, S, 0.8, S * float(target_grid.max()))
+    theta = torch.full((128, 128, 128), 0.5, dtype=torch.float32, device=dev)
+    dv = DiffVolume(ctx, theta)
+    mu = S * 0.5
+    plain = Volume(ctx, theta, (0, 0, 0, 1, 1, 1), 0.8 * S, 0.2 * S, 0.3)
+    img_plain = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    lpath = torch.empty(W * H * SPP, dtype=torch.float32, device=dev)
+    img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    grad = torch.zeros((128, 128, 128), dtype=torch.float32, device=dev)
+    galpha = torch.zeros(1, dtype=torch.float32, device=dev)
+
+    t_plain = timed(lambda: plain.render(scene, W, H, SPP, DEPTH, out=img_plain), 5, 2, sync)
+    t_fwd = timed(lambda: forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath), 5, 2, sync)
+    dl = 2.0 * (img - target) / img.numel()
+
+    def adj():
+        grad.zero_()
+        galpha.zero_()
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, dl, grad, galpha)
+
+    t_adj = timed(adj, 5, 2, sync)
+
+    def iteration():
+        forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath)
+        d = 2.0 * (img - target) / img.numel()
+        grad.zero_()
+        galpha.zero_()
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, d, grad, galpha)
+
+    t_iter = timed(iteration, 5, 2, sync)
+    return {"adjoint_config": "128^3 grid, 256x256, 16 spp, depth 32",
+            "plain_forward_ms": round(t_plain * 1e3, 3),
+            "diff_forward_ms": round(t_fwd * 1e3, 3),
+            "adjoint_ms": round(t_adj * 1e3, 3),
+            "fwd_adjoint_ms_per_iter": round(t_iter * 1e3, 3),
+            "fwd_adjoint_vs_plain": round(t_iter / t_plain, 3)}

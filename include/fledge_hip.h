@@ -143,6 +143,20 @@ typedef struct flg_lbvh_view {
 } flg_lbvh_view;
 FLG_API int flg_lbvh_export(flg_ctx* ctx, flg_bvh* bvh, const flg_lbvh_view* view);
 
+/* Copies the traversal image of a build to host memory, for host mirrors that
+ * run the same closest-hit traversal on the CPU (the C++ Accel drop-in's
+ * single-ray intersect; replaces the CPU RadixBVHBuilder tree,
+ * src/experimental/radix_bvh.hpp:35-57).
+ *   h_nodes: (ntris-1) x 64 bytes, node i = Karras node i:
+ *            float L.lo[3], L.hi[3], R.lo[3], R.hi[3]; int32 refL, refR, 0, 0
+ *            (ref >= 0: internal node; bit 31 set: leaf at sorted position
+ *            ref & 0x7fffffff)
+ *   h_tris:  ntris x 48 bytes in sorted order: float v0[3], uint32 original id,
+ *            float v1[3], 0, float v2[3], 0
+ * The byte counts must equal these sizes; NULL skips an array. */
+FLG_API int flg_lbvh_download(flg_ctx* ctx, const flg_bvh* bvh, void* h_nodes, size_t nodes_bytes,
+                              void* h_tris, size_t tris_bytes);
+
 /* -------------------------------------------------------------- trace --- */
 typedef struct flg_ray {
   float ox, oy, oz, tmax; /* tmax: hits with t <= tmax are accepted */
@@ -231,27 +245,33 @@ typedef struct flg_render_cfg {
 FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
 
-/* Differentiable forward (differential delta tracking, ratio tracking on the
- * shadow and camera rays without roulette, fixed majorant) of the same
- * estimator with albedo alpha and density scale sigma_t_scale: sigma_t =
- * scale * density, sigma_s = alpha * sigma_t.  The proposal grid (decisions)
- * is the volume's density at the time of the call to flg_volpath_diff_begin;
- * gradients are exact for that grid. */
+/* Differentiable forward of the same path structure (differential delta
+ * tracking on the proposal grid, ratio tracking without roulette on camera
+ * and shadow rays, fixed majorant): sigma_t = sigma_t_scale * density,
+ * sigma_s = albedo * sigma_t.  Every branch depends only on the Halton
+ * numbers and the proposal grid, so the image is a smooth function of `vol`'s
+ * densities.  `proposal` (NULL: `vol` itself) supplies the collision
+ * decisions.  d_rgb: width*height RGB (may be NULL); d_lpath: per-path
+ * radiance, width*height*spp floats, path = pixel * spp + s (may be NULL;
+ * the adjoint needs it). */
 typedef struct flg_diff_params {
   float sigma_t_scale; /* sigma_t per unit density */
-  float albedo;        /* single-scatter albedo alpha */
-  float g;
-  float majorant;      /* fixed majorant (per unit length) */
+  float albedo;        /* single-scatter albedo alpha, > 0 */
+  float g;             /* Henyey-Greenstein asymmetry */
+  float majorant;      /* fixed majorant per unit length, > 0 */
 } flg_diff_params;
 
-FLG_API int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
-                                     const flg_render_cfg* cfg, const flg_diff_params* params,
-                                     float* d_rgb);
-/* Path-replay adjoint of flg_volpath_diff_forward: given dLoss/dImage (RGB,
- * d_dl_drgb), accumulates dLoss/dDensity into d_grad (nx*ny*nz floats, added
- * to) and dLoss/dAlbedo into d_grad_albedo (1 float, added to). */
-FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
-                                const flg_render_cfg* cfg, const flg_diff_params* params,
+FLG_API int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
+                                     const flg_vol_scene* scene, const flg_render_cfg* cfg,
+                                     const flg_diff_params* params, float* d_rgb, float* d_lpath);
+/* Path-replay adjoint of flg_volpath_diff_forward (same arguments, d_lpath
+ * from that call): given dLoss/dImage (RGB, d_dl_drgb) adds dLoss/dDensity
+ * into d_grad (nx*ny*nz floats, x fastest) and dLoss/dAlbedo into
+ * d_grad_albedo (1 float).  Accumulation uses float atomics, so the
+ * summation order (not the terms) varies between runs. */
+FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
+                                const flg_vol_scene* scene, const flg_render_cfg* cfg,
+                                const flg_diff_params* params, const float* d_lpath,
                                 const float* d_dl_drgb, float* d_grad, float* d_grad_albedo);
 
 /* ------------------------------------------------------------- checks --- */

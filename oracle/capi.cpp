@@ -44,6 +44,28 @@ Camera make_camera(const float* c) {  // pos3, look_at3, up3, fov
 }
 }  // namespace
 
+// Grid and medium of a VolScene: OpenVDBVolume ctor volume.cpp:33-59
+// (sigma_t = sigma_a + sigma_s, max density from the grid, inverse in double).
+void orc::init_vol_grid(VolScene& sc, const float* grid, int32_t nx, int32_t ny, int32_t nz,
+                   const float* bounds6, const float* medium3) {
+  sc.grid.nx = nx; sc.grid.ny = ny; sc.grid.nz = nz;
+  sc.grid.data = grid;
+  sc.grid.bmin = v3(bounds6[0], bounds6[1], bounds6[2]);
+  sc.grid.bmax = v3(bounds6[3], bounds6[4], bounds6[5]);
+  sc.grid.scale = v3((float)nx / (sc.grid.bmax.x - sc.grid.bmin.x),
+                     (float)ny / (sc.grid.bmax.y - sc.grid.bmin.y),
+                     (float)nz / (sc.grid.bmax.z - sc.grid.bmin.z));
+  sc.med.sigma_s = medium3[0];
+  sc.med.sigma_a = medium3[1];
+  sc.med.g = medium3[2];
+  sc.med.sigma_t = sc.med.sigma_a + sc.med.sigma_s;
+  float mx = -FLT_MAX;
+  for (int64_t i = 0; i < (int64_t)nx * ny * nz; ++i) mx = smax(mx, grid[i]);
+  sc.med.max_density = mx;
+  sc.med.inv_max_density = (float)(1.0 / (double)mx);
+  sc.med.max_mu = sc.med.sigma_t * mx;
+}
+
 extern "C" {
 
 // ---------------------------------------------------------------- math ----
@@ -203,24 +225,7 @@ void orc_volpath_render(const float* grid, int32_t nx, int32_t ny, int32_t nz, c
                         int32_t H, int32_t spp, int32_t max_depth, float* rgb, int nthreads,
                         uint64_t* lookups, uint64_t* paths) {
   VolScene sc;
-  sc.grid.nx = nx; sc.grid.ny = ny; sc.grid.nz = nz;
-  sc.grid.data = grid;
-  sc.grid.bmin = v3(bounds6[0], bounds6[1], bounds6[2]);
-  sc.grid.bmax = v3(bounds6[3], bounds6[4], bounds6[5]);
-  sc.grid.scale = v3((float)nx / (sc.grid.bmax.x - sc.grid.bmin.x),
-                     (float)ny / (sc.grid.bmax.y - sc.grid.bmin.y),
-                     (float)nz / (sc.grid.bmax.z - sc.grid.bmin.z));
-  // OpenVDBVolume ctor volume.cpp:33-59: sigma_t = sigma_a + sigma_s,
-  // max density from the grid, inverse in double.
-  sc.med.sigma_s = medium3[0];
-  sc.med.sigma_a = medium3[1];
-  sc.med.g = medium3[2];
-  sc.med.sigma_t = sc.med.sigma_a + sc.med.sigma_s;
-  float mx = -FLT_MAX;
-  for (int64_t i = 0; i < (int64_t)nx * ny * nz; ++i) mx = smax(mx, grid[i]);
-  sc.med.max_density = mx;
-  sc.med.inv_max_density = (float)(1.0 / (double)mx);
-  sc.med.max_mu = sc.med.sigma_t * mx;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
   sc.point.present = lights7[0] != 0.0f;
   sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
   sc.point.intensity = lights7[4];

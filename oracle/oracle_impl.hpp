@@ -117,9 +117,20 @@ struct VolScene {
   int width = 0, height = 0, spp = 1, max_depth = 1;
 };
 
+bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max);
+bool box_intersect(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max);
+float hg_p(V3 wi, V3 wo, float g);
+V3 hg_sample(V3 wo, float u, float v, float g);
+float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf, V3* p_light);
 V3 hg_sample_pub(V3 wo, float u, float v, float g);
 float hg_p_pub(V3 wi, V3 wo, float g);
 double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups);
+void init_vol_grid(VolScene& sc, const float* grid, int32_t nx, int32_t ny, int32_t nz,
+                   const float* bounds6, const float* medium3);
+// OpenVDBVolume::tr / ::sample for one ray, drawing from hs
+float vol_tr_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax);
+float vol_sample_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success,
+                     V3* pos);
 void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups, uint64_t* paths);
 
 }  // namespace orc

@@ -66,7 +66,7 @@ Lookup grid_lookup(const Grid& g, V3 p) {
 }
 
 // AABB::intersect_pbrt aabb.h:38-59
-static bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max) {
+bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max) {
   float t0 = 0.0f, t1 = tmax;
   for (int i = 0; i < 3; ++i) {
     float inv = 1.0f / comp(d, i);
@@ -84,7 +84,7 @@ static bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_mi
 
 // AABB::intersect aabb.h:20-36: inverse via the double literal 1.0
 // (cwiseInverse, vector.h:248-250 and operator/(T_, Vector) :291-294).
-static bool box_intersect(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
+bool box_intersect(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max) {
   V3 inv = v3((float)(1.0 / (double)d.x), (float)(1.0 / (double)d.y), (float)(1.0 / (double)d.z));
   V3 t1 = (lo - o) * inv;
   V3 t2 = (hi - o) * inv;
@@ -159,15 +159,26 @@ static float vol_sample(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tm
   return 1.0f;
 }
 
+float vol_tr_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax) {
+  uint64_t lk = 0;
+  return vol_tr(sc, hs, o, d, ray_tmax, &lk);
+}
+
+float vol_sample_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success,
+                     V3* pos) {
+  uint64_t lk = 0;
+  return vol_sample(sc, hs, o, d, ray_tmax, success, pos, &lk);
+}
+
 // HGP math_utils.h:103-107
-static inline float hg_p(V3 wi, V3 wo, float g) {
+float hg_p(V3 wi, V3 wo, float g) {
   float cos_theta = dot(wi, wo);
   float denom = 1.0f + g * g + 2.0f * g * cos_theta;
   return (1.0f - g * g) / (denom * std::sqrt(denom)) * INV_4PI;
 }
 
 // CoordinateSystem math_utils.h:57-66
-static inline void coordinate_system(V3 v1, V3* v2, V3* v3o) {
+void coordinate_system(V3 v1, V3* v2, V3* v3o) {
   if (std::fabs(v1.x) > std::fabs(v1.y))
     *v2 = v3(-v1.z, 0.0f, v1.x) / std::sqrt(v1.x * v1.x + v1.z * v1.z);
   else
@@ -177,7 +188,7 @@ static inline void coordinate_system(V3 v1, V3* v2, V3* v3o) {
 
 // HGSampleP math_utils.h:112-128 (u = first draw, v = second: clang's
 // left-to-right argument evaluation at integrator.cpp:378)
-static inline V3 hg_sample(V3 wo, float u, float v, float g) {
+V3 hg_sample(V3 wo, float u, float v, float g) {
   float cos_theta;
   if ((double)std::fabs(g) < 1e-3) {
     cos_theta = 1.0f - 2.0f * u;
@@ -259,7 +270,7 @@ void EnvLight::build(float rad, V3 lo, V3 hi) {
 }
 
 // Light::sampleLi for the environment, light.cpp:119-145
-static float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf,
+float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf,
                            V3* p_light) {
   float pdfs[2];
   int offs[2];

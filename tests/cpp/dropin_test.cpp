@@ -1,0 +1,379 @@
+// Tests of the C++ drop-ins (fledge_gpu.hpp) against the CPU oracle, in the
+// style of the reference's bvh_test harness (src/experimental/bvh_test.hpp:
+// 107-168) and its shape/volume tests.
+//   dropin_test --cpu   host-only checks (no GPU call): GridVolume::tr/sample
+//                       against the oracle's OpenVDBVolume::tr/sample
+//   dropin_test         everything, GPU included
+// Prints one line per check and exits non-zero on the first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "fledge_gpu.hpp"
+#include "fledge_scene.h"
+#include "oracle_impl.hpp"
+
+extern "C" void orc_bvh_test_rays(int64_t n, float* rays);
+
+using fledge::Vector3f;
+
+namespace {
+
+int g_failures = 0;
+
+#define CHECK(cond, ...)                                   \
+  do {                                                     \
+    if (!(cond)) {                                         \
+      std::printf("FAIL %s:%d: ", __FILE__, __LINE__);     \
+      std::printf(__VA_ARGS__);                            \
+      std::printf("\n");                                   \
+      std::fflush(stdout);                                 \
+      ++g_failures;                                        \
+      return;                                              \
+    }                                                      \
+  } while (0)
+
+uint32_t bits(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+bool same_bits(const Vector3f& a, orc::V3 b) {
+  return bits(a[0]) == bits(b.x) && bits(a[1]) == bits(b.y) && bits(a[2]) == bits(b.z);
+}
+
+// A fledge::Sampler over the oracle's Halton stream of one pixel sample.
+struct HaltonStream : fledge::Sampler {
+  orc::Halton h;
+  HaltonStream(int res, int x, int y, uint64_t s) : h(1, res, res) { h.start(x, y, s); }
+  fledge::Float get1D() override { return h.get1D(); }
+};
+
+// ------------------------------------------------------------ host only
+void test_grid_volume_host() {
+  const int n = 24;
+  std::vector<float> grid(size_t(n) * n * n);
+  flg_scene_fbm_grid(n, n, n, 5, 4.0f, 5, 0.0f, 1.0f, grid.data());
+  const float bounds[6] = {0, 0, 0, 1, 1, 1};
+  const float medium[3] = {6.4f, 1.6f, 0.3f};
+  fledge::GridVolume vol(grid.data(), n, n, n, fledge::AABB(Vector3f(0.0f), Vector3f(1.0f)),
+                         Vector3f(medium[0]), Vector3f(medium[1]), medium[2]);
+  orc::VolScene sc;
+  orc::init_vol_grid(sc, grid.data(), n, n, n, bounds, medium);
+  CHECK(bits(vol.maxDensity()) == bits(sc.med.max_density), "max density");
+  std::mt19937 gen(7);
+  std::uniform_real_distribution<float> U(0.0f, 1.0f);
+  int ntr = 0, nsample_hits = 0;
+  for (int i = 0; i < 4000; ++i) {
+    orc::V3 o = orc::v3(U(gen) * 3 - 1, U(gen) * 3 - 1, -1.0f + U(gen));
+    orc::V3 d = orc::normalize(orc::v3(0.5f + U(gen) * 0.2f, 0.5f + U(gen) * 0.2f, 1.0f) - o);
+    const float tmax = (i % 3 == 0) ? 1.5f : INFINITY;
+    fledge::Ray ray(Vector3f(o.x, o.y, o.z), Vector3f(d.x, d.y, d.z), tmax);
+    HaltonStream a(128, i % 128, (i / 128) % 128, 1 + i / 16384), b = a;
+    const Vector3f tr = vol.tr(ray, a);
+    const float tr_ref = orc::vol_tr_pub(sc, b.h, o, d, tmax);
+    CHECK(bits(tr[0]) == bits(tr_ref), "tr ray %d: %.9g vs %.9g", i, tr[0], tr_ref);
+    CHECK(a.h.cnt == b.h.cnt, "tr ray %d: draw count", i);
+    ntr += tr[0] < 1.0f;
+    fledge::VInteraction vi;
+    bool ok = false;
+    const Vector3f f = vol.sample(ray, a, vi, ok);
+    bool ok_ref = false;
+    orc::V3 pos{0, 0, 0};
+    const float f_ref = orc::vol_sample_pub(sc, b.h, o, d, tmax, &ok_ref, &pos);
+    CHECK(ok == ok_ref && bits(f[0]) == bits(f_ref), "sample ray %d", i);
+    if (ok) {
+      CHECK(same_bits(vi.m_p, pos), "sample ray %d: position", i);
+      CHECK(vi.m_g == medium[2] && bits(vi.m_wo[0]) == bits(-d.x), "sample ray %d: interaction", i);
+      ++nsample_hits;
+    }
+  }
+  CHECK(ntr > 100 && nsample_hits > 100, "too few interactions (%d, %d)", ntr, nsample_hits);
+  std::printf("PASS grid_volume_host (%d attenuated, %d collisions)\n", ntr, nsample_hits);
+}
+
+void test_host_errors() {
+  float one = 1.0f;
+  bool threw = false;
+  try {
+    fledge::GridVolume v(&one, 0, 1, 1, fledge::AABB(Vector3f(0.0f), Vector3f(1.0f)), Vector3f(1.0f),
+                         Vector3f(0.0f), 0.0f);
+  } catch (const fledge::gpu::Error& e) {
+    threw = e.status() == FLG_ERR_INVALID;
+  }
+  CHECK(threw, "GridVolume with an empty grid must throw");
+  threw = false;
+  try {
+    fledge::GridVolume v(&one, 1, 1, 1, fledge::AABB(Vector3f(0.0f), Vector3f(0.0f)), Vector3f(1.0f),
+                         Vector3f(0.0f), 0.0f);
+  } catch (const fledge::gpu::Error&) {
+    threw = true;
+  }
+  CHECK(threw, "GridVolume with empty bounds must throw");
+  std::printf("PASS host_errors\n");
+}
+
+// ------------------------------------------------------------------ GPU
+struct Mesh {
+  std::vector<float> verts;
+  std::vector<int32_t> ind;
+  std::vector<Vector3f> p;
+  fledge::experimental::InternalTriangleMesh im{};
+  void finish() {
+    p.resize(verts.size() / 3);
+    for (size_t i = 0; i < p.size(); ++i) p[i] = Vector3f(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    im.nInd = (int)ind.size();
+    im.nVert = (int)p.size();
+    im.ind = ind.data();
+    im.p = p.data();
+    im.n = nullptr;
+    im.uv = nullptr;
+  }
+};
+
+Mesh uv_sphere(int nt, int np, float r) {
+  Mesh m;
+  // MakeMeshedSphere layout: nt*np vertices; the triangle count is queried
+  const int32_t ntris = flg_scene_uv_sphere(nt, np, r, nullptr, nullptr);
+  m.verts.resize(size_t(nt) * np * 3);
+  m.ind.resize(size_t(ntris) * 3);
+  flg_scene_uv_sphere(nt, np, r, m.verts.data(), m.ind.data());
+  m.finish();
+  return m;
+}
+
+void test_lbvh_builder(fledge::gpu::Context& ctx) {
+  // bvh_test: MakeMeshedSphere, rays from [0, 0.5)^3 with normalised
+  // directions in the positive octant; every ray starts inside the sphere
+  Mesh m = uv_sphere(200, 200, 5.0f);
+  fledge::gpu_memory_resource mr;
+  fledge::experimental::LBVHBuilder bvh(&m.im, &mr, &ctx);
+  bvh.build();
+  CHECK(bvh.getMemoryResource() == &mr, "memory resource");
+  orc::Lbvh ref;
+  orc::lbvh_build(ref, m.verts.data(), m.ind.data(), (int32_t)m.ind.size() / 3, (int32_t)m.p.size());
+  auto bd = bvh.getBound();
+  CHECK(same_bits(bd.lower, ref.glo) && same_bits(bd.upper, ref.ghi), "global bound");
+
+  const int N = 20000;
+  std::vector<float> rays(size_t(N) * 8);
+  orc_bvh_test_rays(N, rays.data());
+  std::vector<fledge::experimental::BVHBuilderBase::BVHRayHit> batch(N);
+  for (int i = 0; i < N; ++i) {
+    batch[i].ray_o = Vector3f(rays[8 * i], rays[8 * i + 1], rays[8 * i + 2]);
+    batch[i].ray_d = Vector3f(rays[8 * i + 4], rays[8 * i + 5], rays[8 * i + 6]);
+  }
+  std::vector<int32_t> ids(N);
+  bvh.intersectBatch(batch.data(), ids.data(), N);
+  for (int i = 0; i < N; ++i) {
+    fledge::experimental::BVHBuilderBase::BVHRayHit h;
+    h.ray_o = batch[i].ray_o;
+    h.ray_d = batch[i].ray_d;
+    int32_t id = -2;
+    const bool hit = bvh.intersect(h, &id);
+    orc::V3 o = orc::v3(rays[8 * i], rays[8 * i + 1], rays[8 * i + 2]);
+    orc::V3 d = orc::v3(rays[8 * i + 4], rays[8 * i + 5], rays[8 * i + 6]);
+    float t_ref = 0;
+    orc::V3 ng_ref{0, 0, 0};
+    const int32_t id_ref = orc::lbvh_trace(ref, o, d, rays[8 * i + 3], false, &t_ref, &ng_ref, nullptr);
+    CHECK(hit && id == id_ref, "ray %d: host mirror id %d vs oracle %d", i, id, id_ref);
+    CHECK(bits(h.tfar) == bits(t_ref) && same_bits(h.hit_ng, ng_ref), "ray %d: host mirror t/ng", i);
+    CHECK(batch[i].hit && ids[i] == id && bits(batch[i].tfar) == bits(h.tfar) &&
+              same_bits(batch[i].hit_ng, ng_ref) && same_bits(batch[i].hit_ns, ng_ref),
+          "ray %d: GPU batch differs from the host mirror", i);
+  }
+  // bvh_test correctness2: the fixed ray hits
+  fledge::experimental::BVHBuilderBase::BVHRayHit fixed;
+  fixed.ray_o = Vector3f(0.345564f, 0.422557f, 0.259793f);
+  fixed.ray_d = Vector3f(0.524748325347900390625f, 0.714325249195098876953125f, 0.46301043033599853515625f);
+  CHECK(bvh.intersect(fixed), "fixed ray must hit");
+  // a ray leaving the sphere's bound misses
+  fledge::experimental::BVHBuilderBase::BVHRayHit away;
+  away.ray_o = Vector3f(20.0f, 0.0f, 0.0f);
+  away.ray_d = Vector3f(1.0f, 0.0f, 0.0f);
+  CHECK(!bvh.intersect(away) && !away.hit, "ray away from the mesh must miss");
+  std::printf("PASS lbvh_builder (%d rays: host mirror, GPU batch and oracle agree bit for bit)\n", N);
+}
+
+void test_lbvh_accel(fledge::gpu::Context& ctx) {
+  const int ntris = 4096;
+  Mesh m;
+  m.verts.resize(size_t(ntris) * 9);
+  m.ind.resize(size_t(ntris) * 3);
+  flg_scene_soup(42, 54, ntris, m.verts.data(), m.ind.data());
+  m.finish();
+  std::vector<int> ind(m.ind.begin(), m.ind.end());
+  fledge::LBVHAccel accel(m.p, ind, ctx);
+  CHECK(accel.getMaterial() == nullptr && accel.getAreaLight() == nullptr, "Accel has no material");
+  orc::Lbvh ref;
+  orc::lbvh_build(ref, m.verts.data(), m.ind.data(), ntris, ntris * 3);
+  const fledge::AABB b = accel.getBound();
+  CHECK(same_bits(b.m_min, ref.glo) && same_bits(b.m_max, ref.ghi), "bound");
+
+  orc::Camera cam(orc::v3(0.5f, 0.5f, -1.5f), orc::v3(0.5f, 0.5f, 0.0f), orc::v3(0, 1, 0), 45.0f);
+  const int W = 96, H = 96;
+  std::vector<fledge::Ray> rays;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      orc::V3 o, d;
+      cam.generate(x + 0.5f, y + 0.5f, W, H, &o, &d);
+      rays.emplace_back(Vector3f(o.x, o.y, o.z), Vector3f(d.x, d.y, d.z));
+    }
+  const std::vector<flg_hit> hits = accel.intersectBatch(rays);
+  int nhit = 0;
+  for (size_t i = 0; i < rays.size(); ++i) {
+    fledge::Ray r = rays[i];
+    fledge::SInteraction isect;
+    int32_t id = -2;
+    const bool hit = accel.intersect(r, isect, &id);
+    float t_ref = 0;
+    orc::V3 ng_ref{0, 0, 0};
+    orc::V3 o = orc::v3(r.m_o[0], r.m_o[1], r.m_o[2]), d = orc::v3(r.m_d[0], r.m_d[1], r.m_d[2]);
+    const int32_t id_ref = orc::lbvh_trace(ref, o, d, INFINITY, false, &t_ref, &ng_ref, nullptr);
+    CHECK(id == id_ref && hits[i].id == id_ref, "ray %zu: ids %d / %d vs oracle %d", i, id, hits[i].id, id_ref);
+    if (!hit) {
+      CHECK(std::isinf(r.m_tMax) && isect.m_primitive == nullptr, "ray %zu: a miss must not touch ray/isect", i);
+      continue;
+    }
+    ++nhit;
+    CHECK(bits(r.m_tMax) == bits(t_ref) && bits(hits[i].t) == bits(t_ref), "ray %zu: t", i);
+    CHECK(same_bits(isect.m_ng, ng_ref) && same_bits(isect.m_ns, ng_ref), "ray %zu: normal", i);
+    CHECK(isect.m_primitive == &accel, "ray %zu: primitive", i);
+    const Vector3f p = rays[i](t_ref);
+    CHECK(bits(isect.m_p[0]) == bits(p[0]) && bits(isect.m_wo[2]) == bits(-rays[i].m_d[2]), "ray %zu: p/wo", i);
+    // the shortened ray finds the same hit (t <= tMax accepted) and nothing nearer
+    fledge::SInteraction again;
+    CHECK(accel.intersect(r, again, &id) && id == id_ref, "ray %zu: re-intersect", i);
+  }
+  CHECK(nhit > 1000, "too few hits (%d)", nhit);
+  bool threw = false;
+  try {
+    std::vector<int> bad(ind.begin(), ind.begin() + 4);
+    fledge::LBVHAccel broken(m.p, bad, ctx);
+  } catch (const fledge::gpu::Error& e) {
+    threw = e.status() == FLG_ERR_INVALID;
+  }
+  CHECK(threw, "index count not a multiple of 3 must throw");
+  std::printf("PASS lbvh_accel (%d hits of %zu rays)\n", nhit, rays.size());
+}
+
+void test_vol_integrator(fledge::gpu::Context& ctx) {
+  const int n = 32, W = 40, H = 32, spp = 4, depth = 16;
+  std::vector<float> grid(size_t(n) * n * n);
+  flg_scene_fbm_grid(n, n, n, 9, 4.0f, 5, 0.0f, 1.0f, grid.data());
+  fledge::GridVolume vol(grid.data(), n, n, n, fledge::AABB(Vector3f(0.0f), Vector3f(1.0f)),
+                         Vector3f(6.4f), Vector3f(1.6f), 0.3f, &ctx);
+  fledge::GPUVolLights lights;
+  lights.has_point = true;
+  lights.point_pos = Vector3f(2.0f, 2.0f, -2.0f);
+  lights.point_intensity = 50.0f;
+  lights.has_env = true;
+  lights.env_radiance = 0.5f;
+  fledge::GPUVolIntegrator integ(depth, lights);
+  fledge::Film film(W, H);
+  fledge::Scene scene;
+  scene.m_resX = W;
+  scene.m_resY = H;
+  scene.m_SPP = spp;
+  scene.m_maxDepth = depth;
+  scene.m_FoV = 45.0f;
+  scene.m_origin = Vector3f(0.5f, 0.5f, -1.5f);
+  scene.m_target = Vector3f(0.5f, 0.5f, 0.0f);
+  scene.m_up = Vector3f(0.0f, 1.0f, 0.0f);
+  scene.m_volume = &vol;
+  scene.m_film = &film;
+  integ.render(scene);
+  CHECK(integ.lastLookups() > 0, "lookup count");
+
+  const float bounds[6] = {0, 0, 0, 1, 1, 1};
+  const float medium[3] = {6.4f, 1.6f, 0.3f};
+  orc::VolScene sc;
+  orc::init_vol_grid(sc, grid.data(), n, n, n, bounds, medium);
+  sc.point.present = true;
+  sc.point.pos = orc::v3(2, 2, -2);
+  sc.point.intensity = 50.0f;
+  sc.env.build(0.5f, sc.grid.bmin, sc.grid.bmax);
+  orc::Camera cam(orc::v3(0.5f, 0.5f, -1.5f), orc::v3(0.5f, 0.5f, 0.0f), orc::v3(0, 1, 0), 45.0f);
+  sc.cam = &cam;
+  sc.width = W;
+  sc.height = H;
+  sc.spp = spp;
+  sc.max_depth = depth;
+  std::vector<float> rgb(size_t(W) * H * 3);
+  uint64_t lk = 0, paths = 0;
+  orc::svol_render(sc, rgb.data(), 8, &lk, &paths);
+  double worst = 0;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const Vector3f& c = film.getBuffer(x, y, fledge::EFilmBufferType::EColor);
+      const float r = rgb[(size_t(y) * W + x) * 3];
+      worst = std::max(worst, std::fabs(double(c[0]) - r) / std::max(std::fabs(double(r)), 1e-6));
+    }
+  CHECK(worst <= 1e-4, "film vs oracle: worst relative error %.3g", worst);
+  CHECK(integ.lastLookups() == lk, "lookup counts %llu vs oracle %llu",
+        (unsigned long long)integ.lastLookups(), (unsigned long long)lk);
+  // update() and setBound() reach the device copy
+  std::vector<float> half(grid);
+  for (float& v : half) v *= 0.5f;
+  vol.update(half.data());
+  std::vector<float> out;
+  integ.render(vol, scene.m_origin, scene.m_target, scene.m_up, 45.0f, W, H, spp, out);
+  bool changed = false;
+  for (int i = 0; i < W * H; ++i) changed |= out[3 * i] != film.m_color[i][0];
+  CHECK(changed, "update() must change the image");
+  std::printf("PASS vol_integrator (film vs oracle worst rel %.2g)\n", worst);
+}
+
+void test_memory_resources(fledge::gpu::Context& ctx) {
+  fledge::gpu_memory_resource gmr;
+  void* p = gmr.allocate(1 << 20, 256);
+  std::vector<uint8_t> a(1 << 20), b(1 << 20);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = uint8_t(i * 131 + 7);
+  ctx.check(flg_memcpy_htod(ctx.get(), p, a.data(), a.size()), "htod");
+  ctx.check(flg_memcpy_dtoh(ctx.get(), b.data(), p, b.size()), "dtoh");
+  gmr.deallocate(p, 1 << 20, 256);
+  CHECK(a == b, "gpu_memory_resource round trip");
+  CHECK(gmr.is_equal(gmr) && !gmr.is_equal(*std::pmr::new_delete_resource()), "is_equal");
+  fledge::async_memory_resource amr(nullptr);
+  void* q = amr.allocate(4096, 256);
+  CHECK(q != nullptr, "async allocation");
+  amr.deallocate(q, 4096, 256);
+  ctx.synchronize();
+  bool managed = true;
+  try {
+    void* m = fledge::GlobalManagedMemoryResource()->allocate(4096);
+    fledge::GlobalManagedMemoryResource()->deallocate(m, 4096);
+  } catch (const std::bad_alloc&) {
+    managed = false;
+  }
+  std::printf("PASS memory_resources (managed memory %s)\n", managed ? "available" : "unavailable");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const bool cpu_only = argc > 1 && std::string(argv[1]) == "--cpu";
+  test_grid_volume_host();
+  test_host_errors();
+  if (!cpu_only && g_failures == 0) {
+    try {
+      fledge::gpu::Context ctx(0, nullptr);
+      std::printf("context on device 0\n");
+      std::fflush(stdout);
+      if (!g_failures) test_memory_resources(ctx);
+      if (!g_failures) test_lbvh_builder(ctx);
+      if (!g_failures) test_lbvh_accel(ctx);
+      if (!g_failures) test_vol_integrator(ctx);
+      ctx.synchronize();
+    } catch (const std::exception& e) {
+      std::printf("FAIL exception: %s\n", e.what());
+      ++g_failures;
+    }
+  }
+  std::printf("%s\n", g_failures ? "FAILED" : "ALL PASSED");
+  return g_failures ? 1 : 0;
+}

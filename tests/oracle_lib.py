@@ -154,6 +154,50 @@ def volpath_render(grid, bounds, medium, lights, cam10, W, H, spp, max_depth, nt
     return rgb, int(lk[0]), int(pa[0])
 
 
+def diff_render(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, bounds=(0, 0, 0, 1, 1, 1),
+                nthreads=None):
+    """FP64 differentiable forward: (per-path radiance, per-pixel mean)."""
+    L = lib()
+    _diff_sig(L)
+    theta = np.ascontiguousarray(theta, np.float64)
+    theta_p = np.ascontiguousarray(theta_p, np.float32)
+    nz, ny, nx = theta_p.shape
+    lp = np.zeros(W * H * spp, np.float64)
+    img = np.zeros(W * H, np.float64)
+    L.orc_diff_render(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                      _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                      _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), _p(img), None,
+                      None, None, nthreads or NTHREADS, None)
+    return lp, img.reshape(H, W)
+
+
+def diff_grad(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, lpath, dldpix,
+              bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
+    """FP64 analytic gradient of sum(dldpix * image) w.r.t. theta and albedo."""
+    L = lib()
+    _diff_sig(L)
+    theta = np.ascontiguousarray(theta, np.float64)
+    theta_p = np.ascontiguousarray(theta_p, np.float32)
+    nz, ny, nx = theta_p.shape
+    grad = np.zeros(nx * ny * nz, np.float64)
+    ga = np.zeros(1, np.float64)
+    dl = np.ascontiguousarray(dldpix, np.float64).ravel()
+    lp = np.ascontiguousarray(lpath, np.float64)
+    L.orc_diff_render(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                      _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                      _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), None, _p(dl),
+                      _p(grad), _p(ga), nthreads or NTHREADS, None)
+    return grad.reshape(nz, ny, nx), float(ga[0])
+
+
+def _diff_sig(L):
+    f = L.orc_diff_render
+    if f.argtypes is None:
+        f.restype = None
+        f.argtypes = [P, P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
+                      ctypes.c_int, P]
+
+
 def logf(x):
     x = np.ascontiguousarray(x, np.float32)
     y = np.zeros_like(x)

@@ -1,0 +1,167 @@
+"""GPU differentiable forward and path-replay adjoint against the FP64 CPU
+twin of the same estimator (analytic gradient) and central finite
+differences (eps 1e-3) of that twin; gradients within 1e-3 relative."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as orc
+from fledge_amd import _native as N
+from fledge_amd import api, diff
+
+pytestmark = pytest.mark.gpu
+
+CAM0 = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+LIGHTS = [1.0, 2.0, 2.0, -2.0, 50.0, 1.0, 0.5]
+S, ALBEDO, G = 8.0, 0.8, 0.3
+
+
+def _setup(ctx, grid, W, H, spp, depth, mu):
+    dv = diff.DiffVolume(ctx, torch.from_numpy(grid).cuda(), g=G)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    scene = api.vol_scene(cam)
+    img, lpath = diff.forward(dv, scene, W, H, spp, depth, S, ALBEDO, mu)
+    return dv, scene, img, lpath
+
+
+def _grid(n, seed):
+    g = api.gen_fbm_grid(n, seed=seed)
+    return (0.1 + 0.8 * g).astype(np.float32)  # keeps 1 - s rho / mu away from 0
+
+
+def test_diff_forward_matches_fp64_twin(ctx):
+    grid = _grid(24, 5)
+    W = H = 48
+    spp, depth = 4, 32
+    mu = S * float(grid.max())
+    dv, scene, img, lpath = _setup(ctx, grid, W, H, spp, depth, mu)
+    lp64, img64 = orc.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0,
+                                  W, H, spp, depth)
+    got = img.cpu().numpy()[:, :, 0]
+    rel = np.abs(got - img64) / np.maximum(np.abs(img64), 1e-6)
+    assert rel.max() < 1e-4
+    rel_p = np.abs(lpath.cpu().numpy() - lp64) / np.maximum(np.abs(lp64), 1e-6)
+    assert np.quantile(rel_p, 0.999) < 1e-4
+
+
+def test_diff_forward_estimates_the_plain_integrator(ctx):
+    """Same scene through the reference delta-tracking integrator and the
+    differentiable one: equal in expectation (image means within noise)."""
+    grid = _grid(32, 6)
+    W = H = 64
+    spp = 64
+    mu = S * float(grid.max())
+    dv, scene, img, _ = _setup(ctx, grid, W, H, spp, 32, mu)
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], ALBEDO * S,
+                     (1 - ALBEDO) * S, G)
+    ref = vol.render(scene, W, H, spp, 32)
+    a, b = float(img.mean()), float(ref.mean())
+    assert abs(a - b) / b < 0.02
+
+
+def _adjoint_case(ctx, n, W, H, spp, depth, seed):
+    grid = _grid(n, seed)
+    mu = S * float(grid.max())
+    dv, scene, img, lpath = _setup(ctx, grid, W, H, spp, depth, mu)
+    rng = np.random.default_rng(seed)
+    dl = rng.normal(size=(H, W, 3)).astype(np.float32)
+    grad, galpha = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lpath,
+                                torch.from_numpy(dl).cuda())
+    lp64, _ = orc.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0,
+                              W, H, spp, depth)
+    g64, ga64 = orc.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0,
+                              W, H, spp, depth, lp64, dl.astype(np.float64).sum(axis=2))
+    return grid, mu, grad.cpu().numpy().astype(np.float64), float(galpha.item()), g64, ga64, dl
+
+
+def test_adjoint_matches_fp64_gradient(ctx):
+    grid, mu, g, ga, g64, ga64, dl = _adjoint_case(ctx, 24, 48, 48, 4, 32, 7)
+    assert np.abs(g64).max() > 0
+    rel = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    assert rel < 1e-3
+    big = np.abs(g64) > 1e-2 * np.abs(g64).max()
+    assert (np.abs(g - g64)[big] / np.abs(g64)[big]).max() < 1e-3
+    assert abs(ga - ga64) / abs(ga64) < 1e-3
+
+
+def test_adjoint_central_finite_differences(ctx):
+    """dF/dtheta_v for 64 random voxels with a non-negligible gradient, and
+    dF/dalbedo, against central differences (eps 1e-3) of the FP64 forward with
+    the proposal grid held fixed; F = sum(dl * image)."""
+    W = H = 32
+    spp, depth = 4, 32
+    grid, mu, g, ga, g64, ga64, dl = _adjoint_case(ctx, 16, W, H, spp, depth, 11)
+    dlp = dl.astype(np.float64).sum(axis=2)
+    theta = grid.astype(np.float64)
+
+    def F(th, alb=ALBEDO):
+        _, img = orc.diff_render(th, grid, [S, alb, G, mu], LIGHTS, CAM0, W, H, spp, depth)
+        return float((dlp * img).sum())
+
+    cand = np.flatnonzero(np.abs(g64.ravel()) > 0.05 * np.abs(g64).max())
+    rng = np.random.default_rng(0)
+    pick = rng.choice(cand, size=min(64, cand.size), replace=False)
+    eps = 1e-3
+    for v in pick:
+        tp = theta.copy()
+        tp.flat[v] += eps
+        tm = theta.copy()
+        tm.flat[v] -= eps
+        fd = (F(tp) - F(tm)) / (2 * eps)
+        assert abs(g.flat[v] - fd) <= 1e-3 * abs(fd) + 1e-6, (int(v), float(g.flat[v]), fd)
+    # albedo: the estimator is polynomial in alpha; central difference in float32
+    # parameters of the FP64 twin (alpha +- 1e-3 represented exactly enough)
+    fa = (F(theta, ALBEDO + 1e-3) - F(theta, ALBEDO - 1e-3)) / 2e-3
+    assert abs(ga - fa) <= 1e-3 * abs(fa) + 1e-6
+
+
+def test_adjoint_config3_size(ctx):
+    """Config 3 sizes: 128^3 grid at 0.5, 256x256, 16 spp; L2 loss against a
+    target from a different fBm grid.  GPU gradient vs the FP64 gradient."""
+    W = H = 256
+    spp, depth = 16, 32
+    theta = np.full((128, 128, 128), 0.5, np.float32)
+    target_grid = api.gen_fbm_grid(128, seed=2)
+    mu = S * 0.5
+    tv = diff.DiffVolume(ctx, torch.from_numpy(target_grid).cuda(), g=G)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    scene = api.vol_scene(cam)
+    target, _ = diff.forward(tv, scene, W, H, spp, depth, S, ALBEDO, S * float(target_grid.max()))
+    dv, scene, img, lpath = _setup(ctx, theta, W, H, spp, depth, mu)
+    dl = 2.0 * (img - target) / img.numel()
+    grad, galpha = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lpath, dl)
+    lp64, _ = orc.diff_render(theta.astype(np.float64), theta, [S, ALBEDO, G, mu], LIGHTS, CAM0,
+                              W, H, spp, depth)
+    g64, ga64 = orc.diff_grad(theta.astype(np.float64), theta, [S, ALBEDO, G, mu], LIGHTS, CAM0,
+                              W, H, spp, depth, lp64, dl.cpu().numpy().astype(np.float64).sum(axis=2))
+    g = grad.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 1e-3
+    assert abs(float(galpha.item()) - ga64) / abs(ga64) < 1e-3
+
+
+def test_autograd_function_and_descent(ctx):
+    W = H = 64
+    spp, depth = 4, 16
+    target_grid = torch.from_numpy(_grid(32, 3)).cuda()
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    scene = api.vol_scene(cam)
+    tv = diff.DiffVolume(ctx, target_grid, g=G)
+    mu = S
+    target, _ = diff.forward(tv, scene, W, H, spp, depth, S, ALBEDO, mu)
+    theta = torch.full((32, 32, 32), 0.5, dtype=torch.float32, device="cuda", requires_grad=True)
+    albedo = torch.tensor(0.6, dtype=torch.float32, device="cuda", requires_grad=True)
+    dv = diff.DiffVolume(ctx, theta.detach(), g=G)
+    opt = torch.optim.Adam([theta, albedo], lr=0.02)
+    losses = []
+    for it in range(8):
+        opt.zero_grad()
+        img = diff.VolumeRender.apply(theta, albedo, dv, scene, W, H, spp, depth, S, mu)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        assert torch.isfinite(theta.grad).all() and theta.grad.abs().sum() > 0
+        losses.append(float(loss))
+        opt.step()
+        with torch.no_grad():
+            theta.clamp_(0.0, 1.0)
+            albedo.clamp_(0.05, 1.0)
+    assert losses[-1] < losses[0]
