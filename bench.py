@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true", help="headline only (no secondary workloads)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="one calibration copy + one headline frame, no JSON (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path has no multi-GPU exchange to time "
@@ -86,6 +88,20 @@ def main():
     info = bvh.pathtrace(cam2, W, H, SPP, DEPTH)  # also warms the queues
     segments = info["segments"]
     log("frame: %d paths, %d ray segments" % (info["paths"], segments))
+
+    if args.profile_pass:
+        # for hardware-counter runs: a plain device copy of known size (the
+        # calibration of FETCH_SIZE / WRITE_SIZE), then one headline frame
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        sync()
+        log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 4, dst.numel() * 4))
+        del src, dst
+        bvh.pathtrace(cam2, W, H, SPP, DEPTH)
+        sync()
+        log("profile pass done: one frame of %d ray segments" % segments)
+        return
 
     def frame():
         cfg = N.PtCfg()

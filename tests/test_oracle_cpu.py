@@ -31,6 +31,47 @@ def f32a(*x):
     return np.array(x, np.float32)
 
 
+def _ulp_error(got, exact):
+    """|got - exact| in units of the float spacing at exact (float64 math)."""
+    e32 = exact.astype(np.float32)
+    spacing = np.spacing(np.abs(e32)).astype(np.float64)
+    return np.abs(got.astype(np.float64) - exact) / spacing
+
+
+# ------------------------------------------------------- transcendentals ---
+def test_logf_within_one_ulp_of_log():
+    """The path's logf (fdlibm e_logf method) against float64 log: < 1 ulp over
+    the whole positive range, exact special values."""
+    bits = np.concatenate([np.arange(1, 0x7F800000, 4099, dtype=np.uint32),
+                           np.arange(0x3F700000, 0x3F880000, 13, dtype=np.uint32)])
+    x = bits.view(np.float32)
+    got = orc.logf(x)
+    err = _ulp_error(got, np.log(x.astype(np.float64)))
+    assert err.max() < 1.0, (float(x[err.argmax()]), float(err.max()))
+    sp = orc.logf(np.array([1.0, 0.0, -0.0, np.inf, -1.0, np.nan], np.float32))
+    assert sp[0] == 0.0 and sp[1] == -np.inf and sp[2] == -np.inf and sp[3] == np.inf
+    assert np.isnan(sp[4]) and np.isnan(sp[5])
+
+
+def test_sincosf_within_one_ulp():
+    """sin/cos (msun double kernels, pi/2 reduction) against float64 sin/cos:
+    < 1 ulp on the arguments the path uses (|x| < 8) and beyond."""
+    x = np.concatenate([np.linspace(-8.0, 8.0, 400_001, dtype=np.float32),
+                        np.linspace(-1e4, 1e4, 100_001, dtype=np.float32),
+                        np.array([0.0, -0.0, 1e-30, np.float32(np.pi / 2)], np.float32)])
+    s, c = orc.sincosf(x)
+    xd = x.astype(np.float64)
+    es, ec = _ulp_error(s, np.sin(xd)), _ulp_error(c, np.cos(xd))
+    # near a zero of sin/cos the float spacing of the result collapses; bound
+    # the absolute error there instead
+    tiny_s, tiny_c = np.abs(np.sin(xd)) < 1e-6, np.abs(np.cos(xd)) < 1e-6
+    assert es[~tiny_s].max() < 1.0 and ec[~tiny_c].max() < 1.0
+    assert np.abs(s[tiny_s] - np.sin(xd[tiny_s])).max(initial=0) < 6e-8
+    assert np.abs(c[tiny_c] - np.cos(xd[tiny_c])).max(initial=0) < 6e-8
+    s0, c0 = orc.sincosf(np.array([0.0, -0.0], np.float32))
+    assert s0[0] == 0.0 and s0[1] == 0.0 and c0[0] == 1.0 and c0[1] == 1.0
+
+
 # ---------------------------------------------------------------- PCG32 ---
 def test_pcg32_demo_kat():
     # pcg32-demo output for pcg32_srandom(42, 54)
