@@ -60,7 +60,12 @@ class Camera(ctypes.Structure):
 class PtCfg(ctypes.Structure):
     _fields_ = [("width", c_i32), ("height", c_i32), ("spp", c_i32), ("max_depth", c_i32),
                 ("trace_block", c_i32), ("use_graph", c_i32), ("count_visits", c_i32),
-                ("reserved", c_i32 * 5)]
+                ("blocks_per_cu", c_i32), ("reserved", c_i32 * 4)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("block_size", c_i32), ("blocks_per_cu", c_i32), ("ms", c_f), ("ms_default", c_f),
+                ("candidates", c_i32), ("reserved", c_i32 * 3)]
 
 
 class Medium(ctypes.Structure):
@@ -103,6 +108,8 @@ _SIGS = {
     "flg_lbvh_download": (c_int, [c_vp, c_vp, c_vp, c_size, c_vp, c_size]),
     "flg_trace_closest": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, P(TraceCfg)]),
     "flg_trace_count": (c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "flg_autotune_trace": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, P(Tuning)]),
+    "flg_autotune_pathtrace": (c_int, [c_vp, c_vp, P(Camera), P(PtCfg), P(Tuning)]),
     "flg_camera_rays": (c_int, [c_vp, P(Camera), c_i32, c_i32, c_i32, c_i32, c_vp]),
     "flg_pathtrace_diffuse": (c_int, [c_vp, c_vp, P(Camera), P(PtCfg), c_vp, c_vp, c_vp, P(c_u64)]),
     "flg_volume_create": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, P(c_f), P(Medium), P(c_vp)]),

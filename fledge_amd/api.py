@@ -120,11 +120,14 @@ class LBVH:
         arrs["bound"] = np.array(self.info().bound, np.float32)
         return arrs
 
-    def trace(self, rays, hits=None):
+    def trace(self, rays, hits=None, block_size=0):
         n = rays.shape[0]
         if hits is None:
             hits = torch.empty((n, 2), dtype=torch.int32, device=rays.device)
-        self.ctx.check(self.ctx.lib.flg_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n, None))
+        cfg = N.TraceCfg()
+        cfg.block_size = block_size
+        self.ctx.check(self.ctx.lib.flg_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n,
+                                                      ctypes.byref(cfg)))
         return hits
 
     def count(self, rays):
@@ -133,16 +136,22 @@ class LBVH:
         return c
 
     def pathtrace(self, cam, width, height, spp, max_depth, record=False, trace_block=0,
-                  count_visits=False, use_graph=False):
+                  count_visits=False, use_graph=False, blocks_per_cu=0, out=None):
+        """One frame of the diffuse path tracer.  record: also return per-path
+        segment counts, ids and t (into `out` = (nseg, seg_id, seg_t) when
+        given, so that a replayed graph writes the same buffers)."""
         cfg = N.PtCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
         cfg.trace_block = trace_block
+        cfg.blocks_per_cu = blocks_per_cu
         cfg.count_visits = 1 if count_visits else 0
         cfg.use_graph = 1 if use_graph else 0
         npaths = width * height * spp
         dev = torch.device("cuda", self.ctx.device)
         nseg = seg_id = seg_t = None
-        if record:
+        if out is not None:
+            nseg, seg_id, seg_t = out
+        elif record:
             nseg = torch.zeros(npaths, dtype=torch.int32, device=dev)
             seg_id = torch.full((npaths, max_depth + 1), -2, dtype=torch.int32, device=dev)
             seg_t = torch.zeros((npaths, max_depth + 1), dtype=torch.float32, device=dev)
@@ -152,6 +161,23 @@ class LBVH:
                                                           _ptr(seg_t), stats))
         return {"segments": int(stats[0]), "paths": int(stats[1]), "node_visits": int(stats[2]),
                 "tri_tests": int(stats[3]), "nseg": nseg, "seg_id": seg_id, "seg_t": seg_t}
+
+    def autotune(self, cam, width, height, spp, max_depth):
+        """Times the path tracer's launch shapes on this frame; later frames of
+        this context with trace_block / blocks_per_cu 0 use the fastest."""
+        cfg = N.PtCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
+        t = N.Tuning()
+        self.ctx.check(self.ctx.lib.flg_autotune_pathtrace(self.ctx.h, self.h, ctypes.byref(cam),
+                                                           ctypes.byref(cfg), ctypes.byref(t)))
+        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms,
+                "ms_default": t.ms_default, "candidates": t.candidates}
+
+    def autotune_trace(self, rays, hits):
+        t = N.Tuning()
+        self.ctx.check(self.ctx.lib.flg_autotune_trace(self.ctx.h, self.h, _ptr(rays), _ptr(hits),
+                                                       rays.shape[0], ctypes.byref(t)))
+        return {"block_size": t.block_size, "ms": t.ms, "ms_default": t.ms_default}
 
     def close(self):
         if self.h:

@@ -65,7 +65,7 @@ void resolve_timers(flg_ctx* ctx) {
 }
 
 ScopedTimer::ScopedTimer(flg_ctx* c, const char* n) : ctx(c), name(n) {
-  if (!ctx->profiling) return;
+  if (!ctx || !ctx->profiling) return;  // a null context times nothing
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
     a = b = nullptr;
     return;
@@ -137,6 +137,8 @@ int flg_ctx_destroy(flg_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   resolve_timers(ctx);
+  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   for (auto& kv : ctx->scratch)
     if (kv.second.first) (void)hipFree(kv.second.first);
   release_halton_tables(ctx);

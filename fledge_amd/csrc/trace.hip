@@ -21,9 +21,9 @@ namespace flg {
 
 namespace {
 
-template <bool COMPAT>
-__global__ __launch_bounds__(256) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
-                                               int64_t n) {
+template <bool COMPAT, int BS>
+__global__ __launch_bounds__(BS) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
+                                              int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* r = reinterpret_cast<const float4*>(rays + i);
@@ -185,10 +185,10 @@ struct Frame {
 // record, diffuse sampling in the shading frame and the spawn of the next
 // segment into the other queue (wave-aggregated append).  Persistent grid:
 // the queue length is read from device memory, so the host never waits.
-template <bool COMPAT, bool FIRST, bool COUNT>
-__global__ __launch_bounds__(256) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
-                                                   HaltonTables tab, HaltonParams hp, int32_t spp,
-                                                   int bounce, int max_depth,
+template <bool COMPAT, bool FIRST, bool COUNT, int BS>
+__global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
+                                                  HaltonTables tab, HaltonParams hp, int32_t spp,
+                                                  int bounce, int max_depth,
                                                    unsigned long long* visits) {
   const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
   if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
@@ -281,21 +281,81 @@ __global__ __launch_bounds__(256) void k_pt_bounce(TraceGeom g, PtState s, CamDe
   }
 }
 
+// Launch shapes: threads per block in {64, 128, 256, 512}; the counting
+// variants (instrumentation only) exist at 256.
+bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+
+template <bool COMPAT>
+void launch_trace(int block, hipStream_t st, const TraceGeom& g, const flg_ray* rays, flg_hit* hits,
+                  int64_t n) {
+  const int grid = (int)div_up(n, block);
+  switch (block) {
+    case 64: k_trace<COMPAT, 64><<<grid, 64, 0, st>>>(g, rays, hits, n); break;
+    case 128: k_trace<COMPAT, 128><<<grid, 128, 0, st>>>(g, rays, hits, n); break;
+    case 512: k_trace<COMPAT, 512><<<grid, 512, 0, st>>>(g, rays, hits, n); break;
+    default: k_trace<COMPAT, 256><<<grid, 256, 0, st>>>(g, rays, hits, n); break;
+  }
+}
+
+template <bool COMPAT, bool FIRST>
+void launch_bounce_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s,
+                      const CamDev& cam, const HaltonTables& tab, const HaltonParams& hp, int spp,
+                      int b, int D, unsigned long long* visits) {
+  switch (block) {
+    case 64:
+      k_pt_bounce<COMPAT, FIRST, false, 64><<<grid, 64, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      break;
+    case 128:
+      k_pt_bounce<COMPAT, FIRST, false, 128><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      break;
+    case 512:
+      k_pt_bounce<COMPAT, FIRST, false, 512><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      break;
+    default:
+      k_pt_bounce<COMPAT, FIRST, false, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      break;
+  }
+}
+
 template <bool COMPAT>
 void launch_bounce(bool first, bool count, int grid, int block, hipStream_t st, const TraceGeom& g,
                    const PtState& s, const CamDev& cam, const HaltonTables& tab,
                    const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
-  if (first) {
-    if (count)
-      k_pt_bounce<COMPAT, true, true><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+  if (count) {
+    if (first)
+      k_pt_bounce<COMPAT, true, true, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
     else
-      k_pt_bounce<COMPAT, true, false><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, false, true, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+  } else if (first) {
+    launch_bounce_bs<COMPAT, true>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
   } else {
-    if (count)
-      k_pt_bounce<COMPAT, false, true><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
-    else
-      k_pt_bounce<COMPAT, false, false><<<grid, block, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+    launch_bounce_bs<COMPAT, false>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
   }
+}
+
+
+// Milliseconds of one call of fn on the context stream: one warm-up, then the
+// best of `reps` timed calls.
+template <class F>
+int time_call(flg_ctx* ctx, F fn, int reps, float* ms) {
+  hipEvent_t a, b;
+  FLG_CHECK_HIP(ctx, hipEventCreate(&a));
+  FLG_CHECK_HIP(ctx, hipEventCreate(&b));
+  int st = fn();
+  float best = 1e30f;
+  for (int r = 0; r < reps && st == FLG_OK; ++r) {
+    (void)hipEventRecord(a, ctx->stream);
+    st = fn();
+    (void)hipEventRecord(b, ctx->stream);
+    if (hipEventSynchronize(b) != hipSuccess) st = set_error(ctx, FLG_ERR_HIP, "autotune: event wait failed");
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, a, b);
+    best = std::min(best, t);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = best;
+  return st;
 }
 
 }  // namespace
@@ -310,16 +370,19 @@ int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, f
                       int64_t n, const flg_trace_cfg* cfg) {
   if (!ctx || !bvh || n < 0 || (n > 0 && (!d_rays || !d_hits)))
     return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: bad arguments");
+  int block = cfg && cfg->block_size ? cfg->block_size : ctx->tuned_trace_block;
+  if (block == 0) block = 256;
+  if (!valid_block(block))
+    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: block_size must be 64, 128, 256 or 512");
   if (n == 0) return FLG_OK;
-  (void)cfg;
+  if (div_up(n, 64) > (int64_t)INT32_MAX) return set_error(ctx, FLG_ERR_RANGE, "flg_trace_closest: too many rays");
   TraceGeom g = geom_of(bvh);
-  int grid = div_up(n, 256);
   {
     ScopedTimer tm(ctx, "trace_closest");
     if (bvh->flags & FLG_NORMAL_COMPAT)
-      k_trace<true><<<grid, 256, 0, ctx->stream>>>(g, d_rays, d_hits, n);
+      launch_trace<true>(block, ctx->stream, g, d_rays, d_hits, n);
     else
-      k_trace<false><<<grid, 256, 0, ctx->stream>>>(g, d_rays, d_hits, n);
+      launch_trace<false>(block, ctx->stream, g, d_rays, d_hits, n);
   }
   FLG_CHECK_LAUNCH(ctx);
   return FLG_OK;
@@ -367,6 +430,13 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   if (!ctx || !bvh || !cam || !cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
       cfg->max_depth < 0 || cfg->max_depth > 63 || ((d_seg_id == nullptr) != (d_seg_t == nullptr)))
     return set_error(ctx, FLG_ERR_INVALID, "flg_pathtrace_diffuse: bad arguments");
+  int tb = cfg->trace_block ? cfg->trace_block : ctx->tuned_pt_block;
+  if (tb == 0) tb = 256;
+  int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
+  if (bpc == 0) bpc = 8;
+  if (!valid_block(tb) || bpc < 1 || bpc > 64)
+    return set_error(ctx, FLG_ERR_INVALID,
+                     "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
@@ -392,21 +462,62 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   s.stride_seg = D + 1;
   TraceGeom g = geom_of(bvh);
   const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
-  const int tb = cfg->trace_block > 0 ? cfg->trace_block : 256;
-  const int pgrid = ctx->num_cus * 8;
-  {
-    ScopedTimer tm(ctx, "pathtrace_total");
-    k_pt_zero<<<1, 256, 0, ctx->stream>>>(s.count, D + 2);
-    if (count_visits) k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)visits, 4);
+  const int pgrid = ctx->num_cus * bpc;
+  // the whole frame: clear the queue counts, then one fused launch per bounce
+  auto enqueue = [&](hipStream_t stream, bool timed) {
+    k_pt_zero<<<1, 256, 0, stream>>>(s.count, D + 2);
+    if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
     for (int b = 0; b <= D; ++b) {
-      ScopedTimer tt(ctx, "pathtrace_bounce");
+      ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
       if (compat)
-        launch_bounce<true>(b == 0, count_visits, pgrid, tb, ctx->stream, g, s, cd, tab, hp,
-                            cfg->spp, b, D, visits);
+        launch_bounce<true>(b == 0, count_visits, pgrid, tb, stream, g, s, cd, tab, hp, cfg->spp, b, D,
+                            visits);
       else
-        launch_bounce<false>(b == 0, count_visits, pgrid, tb, ctx->stream, g, s, cd, tab, hp,
-                             cfg->spp, b, D, visits);
+        launch_bounce<false>(b == 0, count_visits, pgrid, tb, stream, g, s, cd, tab, hp, cfg->spp, b,
+                             D, visits);
     }
+  };
+  if (cfg->use_graph) {
+    // every kernel argument of the frame is in the key: a cached graph is
+    // replayed only for an identical launch sequence
+    std::string key;
+    auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+    put(&g, sizeof g);
+    put(&s, sizeof s);
+    put(&cd, sizeof cd);
+    put(&tab, sizeof tab);
+    put(&hp, sizeof hp);
+    const int32_t misc[6] = {cfg->spp, D, pgrid, tb, compat ? 1 : 0, count_visits ? 1 : 0};
+    put(misc, sizeof misc);
+    put(&visits, sizeof visits);
+    hipGraphExec_t exec = nullptr;
+    for (auto& kv : ctx->graphs)
+      if (kv.first == key) exec = kv.second;
+    if (!exec) {
+      // captured on a private stream (the default stream cannot be captured)
+      // and replayed on the context's stream
+      if (!ctx->capture_stream)
+        FLG_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      FLG_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+      enqueue(ctx->capture_stream, false);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->capture_stream, &graph);
+      if (e != hipSuccess) return check_hip(ctx, e, "hipStreamEndCapture (path tracer)");
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return check_hip(ctx, e, "hipGraphInstantiate (path tracer)");
+      if (ctx->graphs.size() >= 8) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(ctx->graphs.front().second);
+        ctx->graphs.erase(ctx->graphs.begin());
+      }
+      ctx->graphs.emplace_back(std::move(key), exec);
+    }
+    ScopedTimer tm(ctx, "pathtrace_total");
+    FLG_CHECK_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  } else {
+    ScopedTimer tm(ctx, "pathtrace_total");
+    enqueue(ctx->stream, true);
   }
   FLG_CHECK_LAUNCH(ctx);
   if (h_stats) {
@@ -425,6 +536,75 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
       h_stats[3] = v[1];
     }
   }
+  return FLG_OK;
+}
+
+int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, flg_hit* d_hits,
+                       int64_t n, flg_tuning* out) {
+  if (!ctx || !bvh || n <= 0 || !d_rays || !d_hits || !out)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_trace: bad arguments");
+  const int blocks[4] = {64, 128, 256, 512};
+  int best_b = 256;
+  float best = 1e30f, def = 0.0f;
+  for (int b : blocks) {
+    flg_trace_cfg c{};
+    c.block_size = b;
+    float ms = 0.0f;
+    int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
+    if (st) return st;
+    if (b == 256) def = ms;
+    if (ms < best) {
+      best = ms;
+      best_b = b;
+    }
+  }
+  ctx->tuned_trace_block = best_b;
+  *out = flg_tuning{};
+  out->block_size = best_b;
+  out->ms = best;
+  out->ms_default = def;
+  out->candidates = 4;
+  return FLG_OK;
+}
+
+int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
+                           const flg_pt_cfg* cfg, flg_tuning* out) {
+  if (!ctx || !bvh || !cam || !cfg || !out)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_pathtrace: bad arguments");
+  // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
+  const int blocks[4] = {64, 128, 256, 512};
+  const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
+  int best_b = 256, best_bpc = 8, n = 0;
+  float best = 1e30f, def = 0.0f;
+  for (int b : blocks)
+    for (int t : threads_per_cu) {
+      flg_pt_cfg c = *cfg;
+      c.trace_block = b;
+      c.blocks_per_cu = t / b;
+      c.count_visits = 0;
+      c.use_graph = 0;
+      if (c.blocks_per_cu > 64) continue;
+      float ms = 0.0f;
+      int st = time_call(
+          ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); }, 2,
+          &ms);
+      if (st) return st;
+      ++n;
+      if (b == 256 && c.blocks_per_cu == 8) def = ms;
+      if (ms < best) {
+        best = ms;
+        best_b = b;
+        best_bpc = c.blocks_per_cu;
+      }
+    }
+  ctx->tuned_pt_block = best_b;
+  ctx->tuned_pt_bpc = best_bpc;
+  *out = flg_tuning{};
+  out->block_size = best_b;
+  out->blocks_per_cu = best_bpc;
+  out->ms = best;
+  out->ms_default = def;
+  out->candidates = n;
   return FLG_OK;
 }
 

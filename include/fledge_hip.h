@@ -169,7 +169,8 @@ typedef struct flg_hit {
 } flg_hit;    /* 8 bytes */
 
 typedef struct flg_trace_cfg {
-  int32_t block_size; /* 0: tuned default */
+  int32_t block_size; /* 64, 128, 256 or 512 threads; 0: the context's tuned
+                         value (flg_autotune_trace), else 256 */
   int32_t reserved[7];
 } flg_trace_cfg;
 
@@ -202,14 +203,38 @@ FLG_API int flg_camera_rays(flg_ctx* ctx, const flg_camera* cam, int32_t width, 
  * {node visits, triangle tests} when count_visits is set (4 entries). */
 typedef struct flg_pt_cfg {
   int32_t width, height, spp, max_depth;
-  int32_t trace_block;  /* 0: default */
-  int32_t use_graph;    /* capture the bounce loop in a hipGraph */
-  int32_t count_visits; /* count traversal work (slower; for the roofline) */
-  int32_t reserved[5];
+  int32_t trace_block;   /* 64, 128, 256 or 512 threads per block; 0: the
+                            context's tuned value (flg_autotune_pathtrace), else 256 */
+  int32_t use_graph;     /* replay the bounce loop from a captured hipGraph
+                            (captured on first use, cached per argument set) */
+  int32_t count_visits;  /* count traversal work (slower; for the roofline) */
+  int32_t blocks_per_cu; /* persistent grid of each bounce = CUs x this (1..64);
+                            0: tuned value, else 8 */
+  int32_t reserved[4];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                                   const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
                                   float* d_seg_t, uint64_t* h_stats);
+
+/* Autotuner (no counterpart in the reference, which traces one ray per call):
+ * times the traversal launch shapes on the caller's own workload and keeps
+ * the fastest in the context, where calls whose block_size / trace_block /
+ * blocks_per_cu are 0 pick it up.  The tuned calls run the real work: their
+ * outputs are written as by a normal call.  Stack depth (64, the bound on the
+ * tree depth) and node width (binary, which the reference's visiting order
+ * needs) are fixed, not tuned. */
+typedef struct flg_tuning {
+  int32_t block_size;    /* chosen threads per block */
+  int32_t blocks_per_cu; /* chosen persistent blocks per CU (path tracer; 0 for trace) */
+  float ms;              /* one call with the chosen shape */
+  float ms_default;      /* one call with the untuned shape (256 threads, 8 blocks per CU) */
+  int32_t candidates;    /* shapes timed */
+  int32_t reserved[3];
+} flg_tuning;
+FLG_API int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
+                               flg_hit* d_hits, int64_t n, flg_tuning* out);
+FLG_API int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
+                                   const flg_pt_cfg* cfg, flg_tuning* out);
 
 /* ------------------------------------------------------------- volume --- */
 typedef struct flg_medium {

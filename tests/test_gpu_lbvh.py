@@ -167,3 +167,71 @@ def test_pathtrace_parity_small(ctx):
     assert np.array_equal(gi[mask], ids[mask])
     assert np.array_equal(gt[mask].view(np.uint32), ts[mask].view(np.uint32))
     assert nrays > npaths  # some bounces happened
+
+
+def _octa_case(ctx):
+    v, i = api.gen_octasphere(6, 0.1)
+    g, o = _build_both(ctx, v, i)
+    cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
+    cam = N.Camera.make(cam10[0:3], cam10[3:6], cam10[6:9], cam10[9])
+    return g, o, cam10, cam
+
+
+def _same_frame(a, b):
+    assert a["segments"] == b["segments"]
+    assert torch.equal(a["nseg"], b["nseg"])
+    D1 = a["seg_id"].shape[1]
+    mask = torch.arange(D1, device=a["nseg"].device)[None, :] < a["nseg"][:, None]
+    assert torch.equal(a["seg_id"][mask], b["seg_id"][mask])
+    assert torch.equal(a["seg_t"][mask].view(torch.int32), b["seg_t"][mask].view(torch.int32))
+
+
+@pytest.mark.parametrize("block,bpc", [(64, 16), (128, 4), (512, 2), (256, 32)])
+def test_pathtrace_launch_shapes_identical(ctx, block, bpc):
+    """Every launch shape the autotuner may pick gives the same frame."""
+    g, o, cam10, cam = _octa_case(ctx)
+    W, H, spp, D = 96, 64, 2, 6
+    ref = g.pathtrace(cam, W, H, spp, D, record=True)
+    got = g.pathtrace(cam, W, H, spp, D, record=True, trace_block=block, blocks_per_cu=bpc)
+    _same_frame(ref, got)
+
+
+def test_pathtrace_graph_replays_match_eager(ctx):
+    """The captured bounce loop, replayed several times with other eager work and
+    synchronisations in between, reproduces the eager frame every time."""
+    g, o, cam10, cam = _octa_case(ctx)
+    W, H, spp, D = 96, 64, 2, 6
+    ref = g.pathtrace(cam, W, H, spp, D, record=True)
+    npaths = W * H * spp
+    bufs = (torch.zeros(npaths, dtype=torch.int32, device="cuda"),
+            torch.full((npaths, D + 1), -2, dtype=torch.int32, device="cuda"),
+            torch.zeros((npaths, D + 1), dtype=torch.float32, device="cuda"))
+    rays = api.camera_rays(ctx, cam, 64, 64)
+    for rep in range(4):
+        for b in bufs:
+            b.fill_(-7)
+        got = g.pathtrace(cam, W, H, spp, D, use_graph=True, out=bufs)
+        torch.cuda.synchronize()
+        _same_frame(ref, got)
+        g.trace(rays)  # unrelated eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_autotuner_choices_give_identical_results(ctx):
+    g, o, cam10, cam = _octa_case(ctx)
+    W, H, spp, D = 96, 64, 2, 6
+    ref = g.pathtrace(cam, W, H, spp, D, record=True)
+    t = g.autotune(cam, W, H, spp, D)
+    assert t["block_size"] in (64, 128, 256, 512) and 1 <= t["blocks_per_cu"] <= 64
+    assert t["candidates"] >= 12 and t["ms"] <= t["ms_default"] * 1.0001
+    _same_frame(ref, g.pathtrace(cam, W, H, spp, D, record=True))  # uses the tuned shape
+    rays = api.camera_rays(ctx, cam, 128, 128)
+    hits = torch.empty((128 * 128, 2), dtype=torch.int32, device="cuda")
+    tt = g.autotune_trace(rays, hits)
+    assert tt["block_size"] in (64, 128, 256, 512)
+    gh = api.hits_as_numpy(g.trace(rays))
+    oi, ot = o.trace(rays.cpu().numpy())
+    assert np.array_equal(gh[0], oi) and np.array_equal(gh[1].view(np.uint32), ot.view(np.uint32))
+    for bs in (64, 128, 512):
+        gb = api.hits_as_numpy(g.trace(rays, block_size=bs))
+        assert np.array_equal(gb[0], oi) and np.array_equal(gb[1].view(np.uint32), ot.view(np.uint32))
