@@ -120,12 +120,13 @@ class LBVH:
         arrs["bound"] = np.array(self.info().bound, np.float32)
         return arrs
 
-    def trace(self, rays, hits=None, block_size=0):
+    def trace(self, rays, hits=None, block_size=0, stack_mode=0):
         n = rays.shape[0]
         if hits is None:
             hits = torch.empty((n, 2), dtype=torch.int32, device=rays.device)
         cfg = N.TraceCfg()
         cfg.block_size = block_size
+        cfg.stack_mode = stack_mode
         self.ctx.check(self.ctx.lib.flg_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n,
                                                       ctypes.byref(cfg)))
         return hits
@@ -136,7 +137,7 @@ class LBVH:
         return c
 
     def pathtrace(self, cam, width, height, spp, max_depth, record=False, trace_block=0,
-                  count_visits=False, use_graph=False, blocks_per_cu=0, out=None):
+                  count_visits=False, use_graph=False, blocks_per_cu=0, stack_mode=0, out=None):
         """One frame of the diffuse path tracer.  record: also return per-path
         segment counts, ids and t (into `out` = (nseg, seg_id, seg_t) when
         given, so that a replayed graph writes the same buffers)."""
@@ -144,6 +145,7 @@ class LBVH:
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
         cfg.trace_block = trace_block
         cfg.blocks_per_cu = blocks_per_cu
+        cfg.stack_mode = stack_mode
         cfg.count_visits = 1 if count_visits else 0
         cfg.use_graph = 1 if use_graph else 0
         npaths = width * height * spp
@@ -170,14 +172,16 @@ class LBVH:
         t = N.Tuning()
         self.ctx.check(self.ctx.lib.flg_autotune_pathtrace(self.ctx.h, self.h, ctypes.byref(cam),
                                                            ctypes.byref(cfg), ctypes.byref(t)))
-        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu, "ms": t.ms,
-                "ms_default": t.ms_default, "candidates": t.candidates}
+        return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu,
+                "stack_mode": t.stack_mode, "ms": t.ms, "ms_default": t.ms_default,
+                "candidates": t.candidates}
 
     def autotune_trace(self, rays, hits):
         t = N.Tuning()
         self.ctx.check(self.ctx.lib.flg_autotune_trace(self.ctx.h, self.h, _ptr(rays), _ptr(hits),
                                                        rays.shape[0], ctypes.byref(t)))
-        return {"block_size": t.block_size, "ms": t.ms, "ms_default": t.ms_default}
+        return {"block_size": t.block_size, "stack_mode": t.stack_mode, "ms": t.ms,
+                "ms_default": t.ms_default}
 
     def close(self):
         if self.h:

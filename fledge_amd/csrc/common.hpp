@@ -147,7 +147,8 @@ struct flg_ctx {
   std::map<std::string, std::pair<void*, size_t>> scratch;
   int num_cus = 256;
   // launch shapes chosen by the autotuner (0: not tuned)
-  int tuned_trace_block = 0, tuned_pt_block = 0, tuned_pt_bpc = 0;
+  int tuned_trace_block = 0, tuned_trace_stack = 0, tuned_pt_block = 0, tuned_pt_bpc = 0,
+      tuned_pt_stack = 0;
   // captured bounce loops, keyed by the bytes of their launch arguments
   hipStream_t capture_stream = nullptr;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;

@@ -21,16 +21,30 @@ namespace flg {
 
 namespace {
 
-template <bool COMPAT, int BS>
+// Traversal with the stack policy of RING: 0 keeps the whole stack in private
+// memory, K > 0 keeps the K most recent entries of each lane in LDS.
+template <bool COMPAT, bool COUNT, int BS, int RING>
+FLG_D int32_t trace_lane(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, uint2* ring) {
+  if constexpr (RING > 0) {
+    RingStack<RING> st(ring + threadIdx.x, BS);
+    return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
+  } else {
+    (void)ring;
+    return trace_one<COMPAT, COUNT>(g, o, d, best, c);
+  }
+}
+
+template <bool COMPAT, int BS, int RING>
 __global__ __launch_bounds__(BS) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
                                               int64_t n) {
+  __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* r = reinterpret_cast<const float4*>(rays + i);
   float4 a = r[0], b = r[1];
   float best = a.w;
   Counters c{0, 0};
-  int32_t p = trace_one<COMPAT, false>(g, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), best, c);
+  int32_t p = trace_lane<COMPAT, false, BS, RING>(g, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), best, c, ring);
   flg_hit h;
   h.id = p >= 0 ? (int32_t)f2u(g.tris[p].v0.w) : -1;
   h.t = p >= 0 ? best : a.w;
@@ -185,11 +199,12 @@ struct Frame {
 // record, diffuse sampling in the shading frame and the spawn of the next
 // segment into the other queue (wave-aggregated append).  Persistent grid:
 // the queue length is read from device memory, so the host never waits.
-template <bool COMPAT, bool FIRST, bool COUNT, int BS>
+template <bool COMPAT, bool FIRST, bool COUNT, int BS, int RING>
 __global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
                                                   HaltonTables tab, HaltonParams hp, int32_t spp,
                                                   int bounce, int max_depth,
-                                                   unsigned long long* visits) {
+                                                  unsigned long long* visits) {
+  __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
   const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
   if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
   const QRay* qin = s.q[bounce & 1];
@@ -221,7 +236,7 @@ __global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev
         hidx = f2u(r.d.w);
       }
       float t = __builtin_inff();
-      const int32_t p = trace_one<COMPAT, COUNT>(g, o, d, t, c);
+      const int32_t p = trace_lane<COMPAT, COUNT, BS, RING>(g, o, d, t, c, ring);
       BTri tri;
       if (p >= 0) tri = g.tris[p];
       if (s.seg_id) {
@@ -285,54 +300,73 @@ __global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev
 // variants (instrumentation only) exist at 256.
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 
-template <bool COMPAT>
-void launch_trace(int block, hipStream_t st, const TraceGeom& g, const flg_ray* rays, flg_hit* hits,
-                  int64_t n) {
+// Stack policy of a launch: stack_mode 2 keeps the 8 most recent entries of
+// each lane in LDS (RingStack), anything else the whole stack in scratch.
+constexpr int kRing = 8;
+int ring_of(int mode) { return mode == 2 ? kRing : 0; }
+
+template <bool COMPAT, int RING>
+void launch_trace_r(int block, hipStream_t st, const TraceGeom& g, const flg_ray* rays, flg_hit* hits,
+                    int64_t n) {
   const int grid = (int)div_up(n, block);
   switch (block) {
-    case 64: k_trace<COMPAT, 64><<<grid, 64, 0, st>>>(g, rays, hits, n); break;
-    case 128: k_trace<COMPAT, 128><<<grid, 128, 0, st>>>(g, rays, hits, n); break;
-    case 512: k_trace<COMPAT, 512><<<grid, 512, 0, st>>>(g, rays, hits, n); break;
-    default: k_trace<COMPAT, 256><<<grid, 256, 0, st>>>(g, rays, hits, n); break;
+    case 64: k_trace<COMPAT, 64, RING><<<grid, 64, 0, st>>>(g, rays, hits, n); break;
+    case 128: k_trace<COMPAT, 128, RING><<<grid, 128, 0, st>>>(g, rays, hits, n); break;
+    case 512: k_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, rays, hits, n); break;
+    default: k_trace<COMPAT, 256, RING><<<grid, 256, 0, st>>>(g, rays, hits, n); break;
   }
 }
 
-template <bool COMPAT, bool FIRST>
+template <bool COMPAT>
+void launch_trace(int block, int stack_mode, hipStream_t st, const TraceGeom& g, const flg_ray* rays,
+                  flg_hit* hits, int64_t n) {
+  if (ring_of(stack_mode))
+    launch_trace_r<COMPAT, kRing>(block, st, g, rays, hits, n);
+  else
+    launch_trace_r<COMPAT, 0>(block, st, g, rays, hits, n);
+}
+
+template <bool COMPAT, bool FIRST, int RING>
 void launch_bounce_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s,
                       const CamDev& cam, const HaltonTables& tab, const HaltonParams& hp, int spp,
                       int b, int D, unsigned long long* visits) {
   switch (block) {
     case 64:
-      k_pt_bounce<COMPAT, FIRST, false, 64><<<grid, 64, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, FIRST, false, 64, RING><<<grid, 64, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
     case 128:
-      k_pt_bounce<COMPAT, FIRST, false, 128><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, FIRST, false, 128, RING><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
     case 512:
-      k_pt_bounce<COMPAT, FIRST, false, 512><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, FIRST, false, 512, RING><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
     default:
-      k_pt_bounce<COMPAT, FIRST, false, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, FIRST, false, 256, RING><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
   }
 }
 
 template <bool COMPAT>
-void launch_bounce(bool first, bool count, int grid, int block, hipStream_t st, const TraceGeom& g,
-                   const PtState& s, const CamDev& cam, const HaltonTables& tab,
+void launch_bounce(bool first, bool count, int grid, int block, int stack_mode, hipStream_t st,
+                   const TraceGeom& g, const PtState& s, const CamDev& cam, const HaltonTables& tab,
                    const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
-  if (count) {
+  if (count) {  // instrumentation: one shape
     if (first)
-      k_pt_bounce<COMPAT, true, true, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, true, true, 256, 0><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
     else
-      k_pt_bounce<COMPAT, false, true, 256><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
-  } else if (first) {
-    launch_bounce_bs<COMPAT, true>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+      k_pt_bounce<COMPAT, false, true, 256, 0><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+  } else if (ring_of(stack_mode)) {
+    if (first)
+      launch_bounce_bs<COMPAT, true, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+    else
+      launch_bounce_bs<COMPAT, false, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
   } else {
-    launch_bounce_bs<COMPAT, false>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+    if (first)
+      launch_bounce_bs<COMPAT, true, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+    else
+      launch_bounce_bs<COMPAT, false, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
   }
 }
-
 
 // Milliseconds of one call of fn on the context stream: one warm-up, then the
 // best of `reps` timed calls.
@@ -372,17 +406,19 @@ int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, f
     return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: bad arguments");
   int block = cfg && cfg->block_size ? cfg->block_size : ctx->tuned_trace_block;
   if (block == 0) block = 256;
-  if (!valid_block(block))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: block_size must be 64, 128, 256 or 512");
+  int smode = cfg && cfg->stack_mode ? cfg->stack_mode : ctx->tuned_trace_stack;
+  if (!valid_block(block) || smode < 0 || smode > 2)
+    return set_error(ctx, FLG_ERR_INVALID,
+                     "flg_trace_closest: block_size must be 64, 128, 256 or 512, stack_mode 0..2");
   if (n == 0) return FLG_OK;
   if (div_up(n, 64) > (int64_t)INT32_MAX) return set_error(ctx, FLG_ERR_RANGE, "flg_trace_closest: too many rays");
   TraceGeom g = geom_of(bvh);
   {
     ScopedTimer tm(ctx, "trace_closest");
     if (bvh->flags & FLG_NORMAL_COMPAT)
-      launch_trace<true>(block, ctx->stream, g, d_rays, d_hits, n);
+      launch_trace<true>(block, smode, ctx->stream, g, d_rays, d_hits, n);
     else
-      launch_trace<false>(block, ctx->stream, g, d_rays, d_hits, n);
+      launch_trace<false>(block, smode, ctx->stream, g, d_rays, d_hits, n);
   }
   FLG_CHECK_LAUNCH(ctx);
   return FLG_OK;
@@ -434,7 +470,8 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   if (tb == 0) tb = 256;
   int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
   if (bpc == 0) bpc = 8;
-  if (!valid_block(tb) || bpc < 1 || bpc > 64)
+  const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
+  if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2)
     return set_error(ctx, FLG_ERR_INVALID,
                      "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
@@ -470,10 +507,10 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     for (int b = 0; b <= D; ++b) {
       ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
       if (compat)
-        launch_bounce<true>(b == 0, count_visits, pgrid, tb, stream, g, s, cd, tab, hp, cfg->spp, b, D,
+        launch_bounce<true>(b == 0, count_visits, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp, b, D,
                             visits);
       else
-        launch_bounce<false>(b == 0, count_visits, pgrid, tb, stream, g, s, cd, tab, hp, cfg->spp, b,
+        launch_bounce<false>(b == 0, count_visits, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp, b,
                              D, visits);
     }
   };
@@ -487,7 +524,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     put(&cd, sizeof cd);
     put(&tab, sizeof tab);
     put(&hp, sizeof hp);
-    const int32_t misc[6] = {cfg->spp, D, pgrid, tb, compat ? 1 : 0, count_visits ? 1 : 0};
+    const int32_t misc[7] = {cfg->spp, D, pgrid, tb, smode, compat ? 1 : 0, count_visits ? 1 : 0};
     put(misc, sizeof misc);
     put(&visits, sizeof visits);
     hipGraphExec_t exec = nullptr;
@@ -544,26 +581,32 @@ int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, 
   if (!ctx || !bvh || n <= 0 || !d_rays || !d_hits || !out)
     return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_trace: bad arguments");
   const int blocks[4] = {64, 128, 256, 512};
-  int best_b = 256;
+  int best_b = 256, best_s = 1, ncand = 0;
   float best = 1e30f, def = 0.0f;
-  for (int b : blocks) {
-    flg_trace_cfg c{};
-    c.block_size = b;
-    float ms = 0.0f;
-    int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
-    if (st) return st;
-    if (b == 256) def = ms;
-    if (ms < best) {
-      best = ms;
-      best_b = b;
+  for (int sm = 1; sm <= 2; ++sm)
+    for (int b : blocks) {
+      flg_trace_cfg c{};
+      c.block_size = b;
+      c.stack_mode = sm;
+      float ms = 0.0f;
+      int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
+      if (st) return st;
+      ++ncand;
+      if (b == 256 && sm == 1) def = ms;
+      if (ms < best) {
+        best = ms;
+        best_b = b;
+        best_s = sm;
+      }
     }
-  }
   ctx->tuned_trace_block = best_b;
+  ctx->tuned_trace_stack = best_s;
   *out = flg_tuning{};
   out->block_size = best_b;
+  out->stack_mode = best_s;
   out->ms = best;
   out->ms_default = def;
-  out->candidates = 4;
+  out->candidates = ncand;
   return FLG_OK;
 }
 
@@ -574,34 +617,39 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
   // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
   const int blocks[4] = {64, 128, 256, 512};
   const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
-  int best_b = 256, best_bpc = 8, n = 0;
+  int best_b = 256, best_bpc = 8, best_s = 1, n = 0;
   float best = 1e30f, def = 0.0f;
-  for (int b : blocks)
-    for (int t : threads_per_cu) {
-      flg_pt_cfg c = *cfg;
-      c.trace_block = b;
-      c.blocks_per_cu = t / b;
-      c.count_visits = 0;
-      c.use_graph = 0;
-      if (c.blocks_per_cu > 64) continue;
-      float ms = 0.0f;
-      int st = time_call(
-          ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); }, 2,
-          &ms);
-      if (st) return st;
-      ++n;
-      if (b == 256 && c.blocks_per_cu == 8) def = ms;
-      if (ms < best) {
-        best = ms;
-        best_b = b;
-        best_bpc = c.blocks_per_cu;
+  for (int sm = 1; sm <= 2; ++sm)
+    for (int b : blocks)
+      for (int t : threads_per_cu) {
+        flg_pt_cfg c = *cfg;
+        c.trace_block = b;
+        c.blocks_per_cu = t / b;
+        c.stack_mode = sm;
+        c.count_visits = 0;
+        c.use_graph = 0;
+        if (c.blocks_per_cu > 64) continue;
+        float ms = 0.0f;
+        int st = time_call(
+            ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); },
+            2, &ms);
+        if (st) return st;
+        ++n;
+        if (b == 256 && c.blocks_per_cu == 8 && sm == 1) def = ms;
+        if (ms < best) {
+          best = ms;
+          best_b = b;
+          best_bpc = c.blocks_per_cu;
+          best_s = sm;
+        }
       }
-    }
   ctx->tuned_pt_block = best_b;
   ctx->tuned_pt_bpc = best_bpc;
+  ctx->tuned_pt_stack = best_s;
   *out = flg_tuning{};
   out->block_size = best_b;
   out->blocks_per_cu = best_bpc;
+  out->stack_mode = best_s;
   out->ms = best;
   out->ms_default = def;
   out->candidates = n;

@@ -43,9 +43,74 @@ FLG_HD void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, in
   hit = (int32_t)p;  // sorted position; the caller maps it to the original id
 }
 
+// Traversal stacks of (node, tnear) entries.  The depth of the tree is below
+// 64 (the common-prefix length grows strictly from a node to its children and
+// is below 64), so a 64-entry stack never overflows.
+
+// All entries in private memory (scratch on the device).
+struct PrivStack {
+  int32_t node[kStack];
+  float t[kStack];
+  int sp = 0;
+  FLG_HD bool empty() const { return sp == 0; }
+  FLG_HD void push(int32_t n, float tn) {
+    if (sp < kStack) {
+      node[sp] = n;
+      t[sp] = tn;
+      ++sp;
+    }
+  }
+  FLG_HD void pop(int32_t* n, float* tn) {
+    --sp;
+    *n = node[sp];
+    *tn = t[sp];
+  }
+};
+
+// The K most recent entries in a fast per-lane ring (LDS on the device; entry
+// k of this lane at fast[k * stride]), older ones moved to a private spill
+// stack when the ring is full.  Pops take the ring first, then the spill
+// stack, so the order is exactly that of PrivStack.
+template <int K>
+struct RingStack {
+  uint2* fast;
+  int stride;
+  int nfast = 0, top = 0, nspill = 0;  // top: next ring slot to write
+  int32_t node[kStack - K];
+  float t[kStack - K];
+  FLG_HD RingStack(uint2* f, int s) : fast(f), stride(s) {}
+  FLG_HD bool empty() const { return nfast == 0 && nspill == 0; }
+  FLG_HD void push(int32_t n, float tn) {
+    if (nfast == K) {  // full: the oldest ring entry sits in the slot about to be written
+      if (nspill >= kStack - K) return;  // unreachable (depth < 64), as PrivStack
+      const uint2 e = fast[top * stride];
+      node[nspill] = (int32_t)e.x;
+      t[nspill] = u2f(e.y);
+      ++nspill;
+      --nfast;
+    }
+    fast[top * stride] = make_uint2((uint32_t)n, f2u(tn));
+    top = top + 1 == K ? 0 : top + 1;
+    ++nfast;
+  }
+  FLG_HD void pop(int32_t* n, float* tn) {
+    if (nfast > 0) {
+      top = top == 0 ? K - 1 : top - 1;
+      const uint2 e = fast[top * stride];
+      *n = (int32_t)e.x;
+      *tn = u2f(e.y);
+      --nfast;
+    } else {
+      --nspill;
+      *n = node[nspill];
+      *tn = t[nspill];
+    }
+  }
+};
+
 // Returns the sorted position of the hit (-1 on a miss); best holds t.
-template <bool COMPAT, bool COUNT>
-FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c) {
+template <bool COMPAT, bool COUNT, class Stack>
+FLG_HD int32_t trace_one_s(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, Stack& st) {
   int32_t hit = -1;
   if (g.n <= 0) return -1;
   const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -56,9 +121,6 @@ FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
     test_tri<COMPAT, COUNT>(g, 0, o, d, best, hit, c);
     return hit;
   }
-  int32_t stack[kStack];
-  float stackt[kStack];
-  int sp = 0;
   int32_t node = 0;
   // every node is visited at most once; the bound only guards a corrupt tree
   for (int64_t visits = 0; visits < (int64_t)g.n; ++visits) {
@@ -73,11 +135,7 @@ FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
     bool goR = refR >= 0 && slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, &tR) &&
                !(best < tR);
     if (goL) {
-      if (goR && sp < kStack) {
-        stack[sp] = refR;
-        stackt[sp] = tR;
-        ++sp;
-      }
+      if (goR) st.push(refR, tR);
       node = refL;
       continue;
     }
@@ -86,10 +144,12 @@ FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
       continue;
     }
     bool found = false;
-    while (sp > 0) {
-      --sp;
-      if (!(best < stackt[sp])) {
-        node = stack[sp];
+    while (!st.empty()) {
+      int32_t n;
+      float t;
+      st.pop(&n, &t);
+      if (!(best < t)) {
+        node = n;
         found = true;
         break;
       }
@@ -97,6 +157,12 @@ FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
     if (!found) break;
   }
   return hit;
+}
+
+template <bool COMPAT, bool COUNT>
+FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c) {
+  PrivStack st;
+  return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
 }
 
 }  // namespace flg

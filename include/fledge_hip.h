@@ -171,7 +171,9 @@ typedef struct flg_hit {
 typedef struct flg_trace_cfg {
   int32_t block_size; /* 64, 128, 256 or 512 threads; 0: the context's tuned
                          value (flg_autotune_trace), else 256 */
-  int32_t reserved[7];
+  int32_t stack_mode; /* traversal stack: 1 private memory, 2 the 8 most recent
+                         entries per ray in LDS; 0: tuned value, else 1 */
+  int32_t reserved[6];
 } flg_trace_cfg;
 
 /* Closest hit of n rays; the visit order and tie rule are those of
@@ -210,7 +212,8 @@ typedef struct flg_pt_cfg {
   int32_t count_visits;  /* count traversal work (slower; for the roofline) */
   int32_t blocks_per_cu; /* persistent grid of each bounce = CUs x this (1..64);
                             0: tuned value, else 8 */
-  int32_t reserved[4];
+  int32_t stack_mode;    /* as flg_trace_cfg.stack_mode */
+  int32_t reserved[3];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                                   const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
@@ -220,16 +223,18 @@ FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_ca
  * times the traversal launch shapes on the caller's own workload and keeps
  * the fastest in the context, where calls whose block_size / trace_block /
  * blocks_per_cu are 0 pick it up.  The tuned calls run the real work: their
- * outputs are written as by a normal call.  Stack depth (64, the bound on the
- * tree depth) and node width (binary, which the reference's visiting order
- * needs) are fixed, not tuned. */
+ * outputs are written as by a normal call.  Tuned: threads per block, resident
+ * blocks per CU (path tracer) and where the traversal stack lives.  Stack
+ * depth (64, the bound on the tree depth) and node width (binary, which the
+ * reference's visiting order needs) are fixed. */
 typedef struct flg_tuning {
   int32_t block_size;    /* chosen threads per block */
   int32_t blocks_per_cu; /* chosen persistent blocks per CU (path tracer; 0 for trace) */
   float ms;              /* one call with the chosen shape */
   float ms_default;      /* one call with the untuned shape (256 threads, 8 blocks per CU) */
   int32_t candidates;    /* shapes timed */
-  int32_t reserved[3];
+  int32_t stack_mode;    /* chosen traversal stack (1 private, 2 LDS ring) */
+  int32_t reserved[2];
 } flg_tuning;
 FLG_API int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
                                flg_hit* d_hits, int64_t n, flg_tuning* out);

@@ -1,0 +1,169 @@
+// Host check of the traversal's stack policies (test infrastructure): the
+// device kernels' closest-hit traversal (traverse.hpp) run on the CPU with the
+// private stack and with the LDS ring stack at ring sizes 1, 2 and 8 (host
+// memory standing in for LDS; small rings force the spill path), on a tree
+// packed from the oracle's build, against the oracle's traversal.
+// Built with hipcc as host code; runs without a GPU.
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "../../fledge_amd/csrc/traverse.hpp"
+#include "fledge_scene.h"
+#include "oracle_impl.hpp"
+
+extern "C" void orc_bvh_test_rays(int64_t n, float* rays);
+
+namespace {
+
+int failures = 0;
+
+void pack(const orc::Lbvh& b, const float* verts, const int32_t* ind, std::vector<flg::BNode>& nodes,
+          std::vector<flg::BTri>& tris) {
+  const int n = b.n;
+  tris.resize(n);
+  auto vert = [&](int v) { return flg::v3(verts[3 * v], verts[3 * v + 1], verts[3 * v + 2]); };
+  std::vector<flg::V3> tlo(n), thi(n);
+  for (int p = 0; p < n; ++p) {
+    const int t = (int)b.order[p];
+    flg::V3 a = vert(ind[3 * t]), c = vert(ind[3 * t + 1]), d = vert(ind[3 * t + 2]);
+    tris[p].v0 = make_float4(a.x, a.y, a.z, flg::u2f((uint32_t)t));
+    tris[p].v1 = make_float4(c.x, c.y, c.z, 0.0f);
+    tris[p].v2 = make_float4(d.x, d.y, d.z, 0.0f);
+    tlo[p] = flg::vmin(flg::vmin(a, c), d);
+    thi[p] = flg::vmax(flg::vmax(a, c), d);
+  }
+  nodes.resize(n > 1 ? n - 1 : 0);
+  for (int i = 0; i + 1 < n; ++i) {
+    const int s = b.split[i];
+    float box[12];
+    int32_t ref[2];
+    for (int side = 0; side < 2; ++side) {
+      const int c = s + side;
+      const bool leaf = side ? b.rleaf[i] : b.lleaf[i];
+      flg::V3 lo = leaf ? tlo[c] : flg::v3(b.nlo[c].x, b.nlo[c].y, b.nlo[c].z);
+      flg::V3 hi = leaf ? thi[c] : flg::v3(b.nhi[c].x, b.nhi[c].y, b.nhi[c].z);
+      float* q = box + 6 * side;
+      q[0] = lo.x; q[1] = lo.y; q[2] = lo.z; q[3] = hi.x; q[4] = hi.y; q[5] = hi.z;
+      ref[side] = leaf ? (int32_t)(0x80000000u | (uint32_t)c) : c;
+    }
+    nodes[i].a = make_float4(box[0], box[1], box[2], box[3]);
+    nodes[i].b = make_float4(box[4], box[5], box[6], box[7]);
+    nodes[i].c = make_float4(box[8], box[9], box[10], box[11]);
+    nodes[i].d = make_int4(ref[0], ref[1], 0, 0);
+  }
+}
+
+template <int K>
+int32_t trace_ring(const flg::TraceGeom& g, flg::V3 o, flg::V3 d, float& best, flg::Counters& c) {
+  uint2 fast[K];
+  flg::RingStack<K> st(fast, 1);
+  return flg::trace_one_s<false, true>(g, o, d, best, c, st);
+}
+
+void run_case(const char* name, const std::vector<float>& verts, const std::vector<int32_t>& ind,
+              const std::vector<float>& rays) {
+  orc::Lbvh b;
+  const int n = (int)ind.size() / 3;
+  orc::lbvh_build(b, verts.data(), ind.data(), n, (int)verts.size() / 3);
+  std::vector<flg::BNode> nodes;
+  std::vector<flg::BTri> tris;
+  pack(b, verts.data(), ind.data(), nodes, tris);
+  flg::TraceGeom g;
+  g.nodes = nodes.data();
+  g.tris = tris.data();
+  g.n = n;
+  const float root[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
+  std::memcpy(g.root, root, sizeof root);
+  const size_t nr = rays.size() / 8;
+  size_t hits = 0;
+  uint64_t visits = 0;
+  for (size_t i = 0; i < nr; ++i) {
+    const float* r = &rays[8 * i];
+    const flg::V3 o = flg::v3(r[0], r[1], r[2]), d = flg::v3(r[4], r[5], r[6]);
+    float t_ref = 0;
+    orc::V3 ng;
+    const int32_t id_ref = orc::lbvh_trace(b, orc::v3(r[0], r[1], r[2]), orc::v3(r[4], r[5], r[6]), r[3],
+                                           false, &t_ref, &ng, nullptr);
+    float t0 = r[3], t1 = r[3], t2 = r[3], t8 = r[3];
+    flg::Counters c0{0, 0}, c1{0, 0}, c2{0, 0}, c8{0, 0};
+    const int32_t p0 = flg::trace_one<false, true>(g, o, d, t0, c0);
+    const int32_t p1 = trace_ring<1>(g, o, d, t1, c1);
+    const int32_t p2 = trace_ring<2>(g, o, d, t2, c2);
+    const int32_t p8 = trace_ring<8>(g, o, d, t8, c8);
+    const int32_t id0 = p0 >= 0 ? (int32_t)flg::f2u(tris[p0].v0.w) : -1;
+    bool ok = id0 == id_ref && p0 == p1 && p0 == p2 && p0 == p8;
+    ok = ok && c0.nodes == c1.nodes && c0.nodes == c2.nodes && c0.nodes == c8.nodes && c0.tris == c8.tris;
+    if (p0 >= 0) {
+      uint32_t a, b1, b2, b8;
+      std::memcpy(&a, &t0, 4);
+      std::memcpy(&b1, &t1, 4);
+      std::memcpy(&b2, &t2, 4);
+      std::memcpy(&b8, &t8, 4);
+      uint32_t r0;
+      std::memcpy(&r0, &t_ref, 4);
+      ok = ok && a == r0 && a == b1 && a == b2 && a == b8;
+      ++hits;
+    }
+    visits += c0.nodes;
+    if (!ok) {
+      std::printf("FAIL %s ray %zu: ids %d/%d/%d/%d oracle %d, visits %u/%u/%u/%u\n", name, i, p0, p1, p2, p8,
+                  id_ref, c0.nodes, c1.nodes, c2.nodes, c8.nodes);
+      ++failures;
+      return;
+    }
+  }
+  std::printf("PASS %s: %zu rays, %zu hits, %.2f node visits per ray\n", name, nr, hits,
+              (double)visits / (double)nr);
+}
+
+}  // namespace
+
+int main() {
+  {  // bvh_test: MakeMeshedSphere(200, 200, 5), rays from inside
+    const int nt = 200, np = 200;
+    const int32_t ntris = flg_scene_uv_sphere(nt, np, 5.0f, nullptr, nullptr);
+    std::vector<float> v(size_t(nt) * np * 3);
+    std::vector<int32_t> ind(size_t(ntris) * 3);
+    flg_scene_uv_sphere(nt, np, 5.0f, v.data(), ind.data());
+    std::vector<float> rays(20000 * 8);
+    orc_bvh_test_rays(20000, rays.data());
+    run_case("uv_sphere", v, ind, rays);
+  }
+  {  // soup of 20000 triangles, rays from outside through the cube
+    const int n = 20000;
+    std::vector<float> v(size_t(n) * 9);
+    std::vector<int32_t> ind(size_t(n) * 3);
+    flg_scene_soup(42, 60, n, v.data(), ind.data());
+    std::mt19937 gen(3);
+    std::uniform_real_distribution<float> U(0.0f, 1.0f);
+    std::vector<float> rays;
+    for (int i = 0; i < 20000; ++i) {
+      orc::V3 o = orc::v3(U(gen) * 2 - 0.5f, U(gen) * 2 - 0.5f, -1.0f);
+      orc::V3 tgt = orc::v3(U(gen), U(gen), U(gen));
+      orc::V3 d = orc::normalize(tgt - o);
+      rays.insert(rays.end(), {o.x, o.y, o.z, INFINITY, d.x, d.y, d.z, 0.0f});
+    }
+    run_case("soup", v, ind, rays);
+  }
+  {  // displaced octahedral sphere, diffuse-like rays from points near the surface
+    const int level = 6;
+    int64_t nv = 0;
+    const int64_t ntris = flg_scene_octasphere_counts(level, &nv);
+    std::vector<float> v(size_t(nv) * 3);
+    std::vector<int32_t> ind(size_t(ntris) * 3);
+    flg_scene_octasphere(level, 0.1f, 7, 2.0f, v.data(), ind.data());
+    std::mt19937 gen(5);
+    std::uniform_real_distribution<float> U(-1.0f, 1.0f);
+    std::vector<float> rays;
+    for (int i = 0; i < 20000; ++i) {
+      orc::V3 o = orc::v3(U(gen) * 0.5f, U(gen) * 0.5f, U(gen) * 0.5f);
+      orc::V3 d = orc::normalize(orc::v3(U(gen), U(gen), U(gen)));
+      rays.insert(rays.end(), {o.x, o.y, o.z, INFINITY, d.x, d.y, d.z, 0.0f});
+    }
+    run_case("octasphere", v, ind, rays);
+  }
+  std::printf("%s\n", failures ? "FAILED" : "ALL PASSED");
+  return failures ? 1 : 0;
+}

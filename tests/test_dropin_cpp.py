@@ -29,6 +29,18 @@ def test_dropins_host_side():
     _run(["--cpu"], 120)
 
 
+def test_traversal_stack_policies_host():
+    """The device traversal with its private stack and with the LDS ring stack
+    (rings of 1, 2 and 8 entries: the spill path is exercised) gives the same
+    hits, t and node visits as the oracle, run on the CPU."""
+    exe = os.path.join(HERE, "cpp", "build", "stack_test")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpp")], check=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stdout
+
+
 def test_host_library_exports_dropins():
     out = subprocess.run(["nm", "-DC", os.path.join(os.path.dirname(HERE), "fledge_amd",
                                                     "libfledge_host.so")],

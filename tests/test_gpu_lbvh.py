@@ -186,14 +186,41 @@ def _same_frame(a, b):
     assert torch.equal(a["seg_t"][mask].view(torch.int32), b["seg_t"][mask].view(torch.int32))
 
 
-@pytest.mark.parametrize("block,bpc", [(64, 16), (128, 4), (512, 2), (256, 32)])
-def test_pathtrace_launch_shapes_identical(ctx, block, bpc):
-    """Every launch shape the autotuner may pick gives the same frame."""
+@pytest.mark.parametrize("block,bpc,smode", [(64, 16, 1), (128, 4, 2), (512, 2, 1), (256, 32, 2),
+                                              (512, 16, 2), (64, 64, 2)])
+def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode):
+    """Every launch shape and stack policy the autotuner may pick gives the same
+    frame (segment ids and t bit for bit)."""
     g, o, cam10, cam = _octa_case(ctx)
     W, H, spp, D = 96, 64, 2, 6
     ref = g.pathtrace(cam, W, H, spp, D, record=True)
-    got = g.pathtrace(cam, W, H, spp, D, record=True, trace_block=block, blocks_per_cu=bpc)
+    got = g.pathtrace(cam, W, H, spp, D, record=True, trace_block=block, blocks_per_cu=bpc,
+                      stack_mode=smode)
     _same_frame(ref, got)
+
+
+@pytest.mark.parametrize("block", [64, 256, 512])
+def test_trace_lds_stack_matches_oracle(ctx, block):
+    """Closest hit with the LDS ring stack on the config-0 soup and on random
+    rays inside a closed mesh: ids and t bit-equal to the oracle."""
+    v, i = api.gen_soup(65536)
+    g, o = _build_both(ctx, v, i)
+    rays = orc.camera_rays(CAM0, 512, 512)
+    gid, gt = api.hits_as_numpy(g.trace(_dev(rays), block_size=block, stack_mode=2))
+    oid, ot = o.trace(rays)
+    assert np.array_equal(gid, oid) and np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    vs, ins = api.gen_octasphere(7, 0.1)
+    gs, os_ = _build_both(ctx, vs, ins)
+    rng = np.random.default_rng(9)
+    r = np.zeros((100000, 8), np.float32)
+    r[:, 0:3] = rng.uniform(-0.5, 0.5, (100000, 3))
+    d = rng.normal(size=(100000, 3))
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 3] = np.inf
+    gid, gt = api.hits_as_numpy(gs.trace(_dev(r), block_size=block, stack_mode=2))
+    oid, ot = os_.trace(r)
+    assert (oid >= 0).all()
+    assert np.array_equal(gid, oid) and np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
 
 
 def test_pathtrace_graph_replays_match_eager(ctx):
@@ -223,7 +250,7 @@ def test_autotuner_choices_give_identical_results(ctx):
     ref = g.pathtrace(cam, W, H, spp, D, record=True)
     t = g.autotune(cam, W, H, spp, D)
     assert t["block_size"] in (64, 128, 256, 512) and 1 <= t["blocks_per_cu"] <= 64
-    assert t["candidates"] >= 12 and t["ms"] <= t["ms_default"] * 1.0001
+    assert t["candidates"] >= 24 and t["stack_mode"] in (1, 2) and t["ms"] <= t["ms_default"] * 1.0001
     _same_frame(ref, g.pathtrace(cam, W, H, spp, D, record=True))  # uses the tuned shape
     rays = api.camera_rays(ctx, cam, 128, 128)
     hits = torch.empty((128 * 128, 2), dtype=torch.int32, device="cuda")
@@ -233,5 +260,6 @@ def test_autotuner_choices_give_identical_results(ctx):
     oi, ot = o.trace(rays.cpu().numpy())
     assert np.array_equal(gh[0], oi) and np.array_equal(gh[1].view(np.uint32), ot.view(np.uint32))
     for bs in (64, 128, 512):
-        gb = api.hits_as_numpy(g.trace(rays, block_size=bs))
-        assert np.array_equal(gb[0], oi) and np.array_equal(gb[1].view(np.uint32), ot.view(np.uint32))
+        for sm in (1, 2):
+            gb = api.hits_as_numpy(g.trace(rays, block_size=bs, stack_mode=sm))
+            assert np.array_equal(gb[0], oi) and np.array_equal(gb[1].view(np.uint32), ot.view(np.uint32))
