@@ -6,7 +6,7 @@ on the 2,097,152-triangle fBm-displaced octahedral sphere (amplitude 0.1),
 bounces) is one closest-hit query of the radix-tree LBVH.  One step = one full
 frame; the BVH build and scene generation are outside the timed region.
 
-Also reported (`secondary`): LBVH build time of that mesh, config-0
+Also reported (`secondary`): LBVH build time of that mesh, primary-ray
 closest-hit Mrays/s (65,536-triangle soup, 1024^2 primary rays), the batched
 build of 256 meshes, volumetric path samples/s (256^3 fBm grid, 512^2, 64 spp,
 depth 32) and the differentiable forward + adjoint cost against the plain
@@ -53,12 +53,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--fast", action="store_true", help="headline only (no secondary workloads)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--graph", action="store_true", help="replay each frame from a captured hipGraph")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one calibration copy + one headline frame, no JSON (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path has no multi-GPU exchange to time "
-              "(rays and paths shard by image tiles; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "(rays and paths would shard by image tiles with a replicated BVH). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -103,9 +106,16 @@ def main():
         log("profile pass done: one frame of %d ray segments" % segments)
         return
 
+    # launch shape picked by the library's autotuner on this frame (untimed)
+    tuning = None
+    if not args.no_tune:
+        tuning = bvh.autotune(cam2, W, H, SPP, DEPTH)
+        log("autotune: %s" % tuning)
+
     def frame():
         cfg = N.PtCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = W, H, SPP, DEPTH
+        cfg.use_graph = 1 if args.graph else 0
         ctx.check(ctx.lib.flg_pathtrace_diffuse(ctx.h, bvh.h, cam2, cfg, None, None, None, None))
 
     # timed region: K frames, HIP-event kernel timers recorded live
@@ -121,6 +131,16 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
     total_ms, _ = ctx.kernel_time("pathtrace_total")
+    if args.graph:  # a replayed graph has no per-launch timers: one timed eager frame
+        ctx.reset_profiling()
+        cfg = N.PtCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.max_depth = W, H, SPP, DEPTH
+        ctx.check(ctx.lib.flg_pathtrace_diffuse(ctx.h, bvh.h, cam2, cfg, None, None, None, None))
+        sync()
+        trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
+        total_ms, _ = ctx.kernel_time("pathtrace_total")
+        trace_ms, trace_launches = trace_ms * args.steps, trace_launches * args.steps
+        total_ms *= args.steps
     ctx.profiling(False)
     value = segments / dt / 1e6
 
@@ -140,25 +160,26 @@ def main():
                 "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                 "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
                 "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,
-                "trace_kernel_share": round(trace_ms / max(total_ms, 1e-9), 4)}
+                "trace_kernel_share": round(trace_ms / max(total_ms, 1e-9), 4),
+                "launch_shape": tuning, "graph": bool(args.graph)}
 
     secondary = {"lbvh_build_2M_ms": round(build_ms, 3), "lbvh_build_2M_first_ms": round(build_ms_first, 3),
                  "lbvh_build_2M_mtris_per_s": round(ntris / build_ms / 1e3, 2)}
 
     # -------------------------------------------------- secondary workloads
     if not args.fast:
-        # config 0: 65,536-triangle soup, 1024^2 jittered primary rays
+        # 65,536-triangle soup, 1024^2 jittered primary rays
         v0, i0 = api.gen_soup(65536)
         b0 = api.LBVH.build(ctx, torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev))
         cam0 = cam_of(N, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
         rays0 = api.camera_rays(ctx, cam0, 1024, 1024)
         hits0 = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
         t = timed(lambda: b0.trace(rays0, hits0), 20, 5, sync)
-        secondary["config0_closest_hit_mrays_per_s"] = round(1024 * 1024 / t / 1e6, 2)
-        secondary["config0_build_ms"] = round(timed(
+        secondary["soup65k_primary_closest_hit_mrays_per_s"] = round(1024 * 1024 / t / 1e6, 2)
+        secondary["soup65k_build_ms"] = round(timed(
             lambda: api.LBVH.build(ctx, torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev)).close(),
             5, 2, sync) * 1e3, 3)
-        # config 2 builds: 256 meshes, log-uniform 1K..256K triangles, one batch
+        # batched build: 256 meshes, log-uniform 1K..256K triangles, one batch
         sizes = api.gen_batch_sizes(256)
         meshes = []
         for k, n in enumerate(sizes):
@@ -174,7 +195,7 @@ def main():
         secondary["batched_build_total_tris"] = int(sizes.sum())
         secondary["batched_build_mtris_per_s"] = round(int(sizes.sum()) / tb / 1e6, 2)
         del meshes
-        # config 1: volumetric path tracing
+        # volumetric path tracing: 256^3 fBm grid, 512x512, 64 spp, depth 32
         g1 = api.gen_fbm_grid(256, seed=1)
         vol = api.Volume(ctx, torch.from_numpy(g1).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
         sc1 = api.vol_scene(cam0)

@@ -106,9 +106,9 @@ class VolumeRender(torch.autograd.Function):
 
 
 def bench_adjoint(ctx, timed, sync):
-    """Config 3: 128^3 grid initialised to 0.5, 256x256, 16 spp, L2 loss against
+    """128^3 grid initialised to 0.5, 256x256, 16 spp, L2 loss against
     a target rendered from a different fBm grid; point light and environment
-    as in the volumetric config, albedo 0.8, g 0.3, sigma_t = 8 density."""
+    as in the 256^3 volume render, albedo 0.8, g 0.3, sigma_t = 8 density."""
     from . import api
 
     dev = torch.device("cuda", ctx.device)

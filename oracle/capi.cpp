@@ -80,7 +80,7 @@ void orc_pcg32(uint64_t initstate, uint64_t initseq, int64_t n, uint32_t* out) {
   r.seed(initstate, initseq);
   for (int64_t i = 0; i < n; ++i) out[i] = r.next();
 }
-// Config-0 triangle soup: per triangle 3 centroid draws then 9 vertex offsets
+// Triangle soup: per triangle 3 centroid draws then 9 vertex offsets
 // u * 0.02 - 0.01, from PCG32(initstate, initseq); vertex i*3+k, index = identity.
 void orc_gen_soup(uint64_t initstate, uint64_t initseq, int32_t ntris, float* verts, int32_t* ind) {
   Pcg32 r;
@@ -199,7 +199,7 @@ void orc_halton_sequence(int32_t resx, int32_t resy, int32_t spp, int32_t px, in
 }
 
 // ---------------------------------------------------------------- camera ---
-// Primary rays of config 0 (1 spp jittered by Halton dims 0,1) or sample s.
+// Primary rays (1 spp jittered by Halton dims 0,1) or sample s of spp.
 void orc_camera_rays(const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t sample,
                      float* rays, int nthreads) {
   Camera cam = make_camera(cam10);

@@ -115,8 +115,8 @@ def test_adjoint_central_finite_differences(ctx):
     assert abs(ga - fa) <= 1e-3 * abs(fa) + 1e-6
 
 
-def test_adjoint_config3_size(ctx):
-    """Config 3 sizes: 128^3 grid at 0.5, 256x256, 16 spp; L2 loss against a
+def test_adjoint_grid128_full_size(ctx):
+    """Full size: 128^3 grid at 0.5, 256x256, 16 spp; L2 loss against a
     target from a different fBm grid.  GPU gradient vs the FP64 gradient."""
     W = H = 256
     spp, depth = 16, 32

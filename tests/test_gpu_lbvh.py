@@ -76,9 +76,9 @@ def test_camera_rays_parity(ctx):
 
 
 @pytest.mark.parametrize("compat", [False, True])
-def test_trace_parity_config0(ctx, compat):
-    """Config 0: 65,536-triangle soup, 1024x1024 jittered primary rays."""
-    print("trace config0 compat=%s" % compat, flush=True)
+def test_trace_parity_soup65k_primary(ctx, compat):
+    """65,536-triangle soup, 1024x1024 jittered primary rays."""
+    print("trace soup65k compat=%s" % compat, flush=True)
     v, i = api.gen_soup(65536)
     g, o = _build_both(ctx, v, i, compat=compat)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
@@ -201,7 +201,7 @@ def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode):
 
 @pytest.mark.parametrize("block", [64, 256, 512])
 def test_trace_lds_stack_matches_oracle(ctx, block):
-    """Closest hit with the LDS ring stack on the config-0 soup and on random
+    """Closest hit with the LDS ring stack on the 65,536-triangle soup and on random
     rays inside a closed mesh: ids and t bit-equal to the oracle."""
     v, i = api.gen_soup(65536)
     g, o = _build_both(ctx, v, i)

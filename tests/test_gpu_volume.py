@@ -84,8 +84,8 @@ def test_volpath_parity_small(ctx, point, env):
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
 
 
-def test_volpath_parity_config1_crop(ctx):
-    """Config 1 volume (256^3 fBm, sigma_t_max 8, albedo 0.8, g 0.3) at 128x128, 8 spp."""
+def test_volpath_parity_fbm256_crop(ctx):
+    """The 256^3 fBm volume (sigma_t_max 8, albedo 0.8, g 0.3) at 128x128, 8 spp."""
     grid = api.gen_fbm_grid(256, seed=1)
     out, ref, st, lk = _scene(ctx, grid, 8, 128, 128, 32)
     rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-6)
