@@ -203,6 +203,8 @@ def main():
         _, st1 = vol.render(sc1, 512, 512, 64, 32, out=out1, stats=True)
         t1 = timed(lambda: vol.render(sc1, 512, 512, 64, 32, out=out1), 3, 1, sync)
         paths1 = 512 * 512 * 64
+        t1b = timed(lambda: vol.render(sc1, 512, 512, 64, 32, out=out1, schedule=1), 3, 1, sync)
+        secondary["vol_ms_per_frame_lane_per_path"] = round(t1b * 1e3, 3)
         secondary["vol_path_samples_per_s"] = round(paths1 / t1, 1)
         secondary["vol_ms_per_frame"] = round(t1 * 1e3, 3)
         secondary["vol_lookups_per_path"] = round(st1["lookups"] / paths1, 3)

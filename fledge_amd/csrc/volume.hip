@@ -138,72 +138,151 @@ FLG_D float sample_one_light(const VolDev& v, const LightsDev& L, const HaltonTa
   return out / light_pdf;
 }
 
-// SVolIntegrator::Li (integrator.cpp:328-405), m_rrThreshold = 0.01.
+// SVolIntegrator::Li (integrator.cpp:328-405), m_rrThreshold = 0.01, split
+// into the path's start (camera ray, entry into the box, primary
+// transmittance) and one step per scattering event, so that a persistent
+// kernel can interleave paths; the arithmetic and the draw order are those
+// of one uninterrupted call.
+struct SvolPath {
+  PathSampler ps;
+  V3 o, d;
+  float Lr, beta;
+  int bounces;
+};
+
+// Starts path p (pixel p / spp, sample p % spp).  Returns true when the path
+// is already complete (its camera ray misses the volume).
 template <bool COUNT>
-FLG_D float svol_li(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                    const HaltonParams& hp, PathSampler& ps, V3 o, V3 d, int max_depth,
-                    uint32_t& nlook) {
-  HaltonDraw rng{ps, tab, hp};
-  float Lr = 0.0f, beta = 1.0f;
-  const float rr = 0.01f;
+FLG_D bool svol_begin(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, int32_t spp, uint32_t p,
+                      SvolPath& s, uint32_t& nlook) {
+  const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+  s.ps.dim = 0;
+  const float ux = s.ps.next(tab, hp);
+  const float uy = s.ps.next(tab, hp);
+  V3 o, d;
+  gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
+  s.Lr = 0.0f;
+  s.beta = 1.0f;
   float t_min, t_max;
   if (!box_enter(v.bmin, v.bmax, o, d, &t_min, &t_max)) {
-    if (L.env.present) Lr += beta * L.env.radiance;
-    return Lr;
+    if (L.env.present) s.Lr += s.beta * L.env.radiance;
+    return true;
   }
-  {
-    float tr = vol_tr<COUNT>(v, rng, o, d, __builtin_inff(), nlook);
-    if (L.env.present) Lr += tr * L.env.radiance;
-    o = o + d * t_min;
+  HaltonDraw rng{s.ps, tab, hp};
+  const float tr = vol_tr<COUNT>(v, rng, o, d, kInf, nlook);
+  if (L.env.present) s.Lr += tr * L.env.radiance;
+  s.o = o + d * t_min;
+  s.d = d;
+  s.bounces = 1;
+  return false;
+}
+
+// One scattering event of the path: delta tracking to the next collision, HG
+// sampling, next-event estimation, termination tests.  Returns true when the
+// path is complete.
+template <bool COUNT>
+FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                     const HaltonParams& hp, int max_depth, SvolPath& s, uint32_t& nlook) {
+  HaltonDraw rng{s.ps, tab, hp};
+  bool success = false;
+  V3 pos;
+  const float f = vol_sample<COUNT>(v, rng, s.o, s.d, kInf, &success, &pos, nlook);
+  if (!success) return true;
+  s.beta = s.beta * f;
+  const V3 wo = -s.d;
+  const float u = s.ps.next(tab, hp);
+  const float vv = s.ps.next(tab, hp);
+  const V3 wi = hg_sample(wo, u, vv, v.g);
+  s.Lr += s.beta * sample_one_light<COUNT>(v, L, tab, hp, s.ps, pos, wo, nlook);
+  if (s.bounces >= max_depth || s.beta * s.beta + s.beta * s.beta + s.beta * s.beta == 0.0f) return true;
+  s.o = pos;
+  s.d = normalize(wi);
+  if (s.bounces > 10) {
+    const float rr = 0.01f;
+    if (s.ps.next(tab, hp) < rr) return true;
+    s.beta /= (1.0f - rr);
   }
-  for (int bounces = 1;; ++bounces) {
-    bool success = false;
-    V3 pos;
-    float f = vol_sample<COUNT>(v, rng, o, d, kInf, &success, &pos, nlook);
-    if (!success) break;
-    beta = beta * f;
-    V3 wo = -d;
-    float u = ps.next(tab, hp);
-    float vv = ps.next(tab, hp);
-    V3 wi = hg_sample(wo, u, vv, v.g);
-    Lr += beta * sample_one_light<COUNT>(v, L, tab, hp, ps, pos, wo, nlook);
-    if (bounces >= max_depth || beta * beta + beta * beta + beta * beta == 0.0f) break;
-    o = pos;
-    d = normalize(wi);
-    if (bounces > 10) {
-      if (ps.next(tab, hp) < rr) break;
-      beta /= (1.0f - rr);
-    }
-  }
-  return Lr;
+  ++s.bounces;
+  return false;
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab,
-                                              HaltonParams hp, CamDevV cam, int32_t spp,
-                                              int32_t max_depth, float* lpath,
-                                              unsigned long long* nlookups) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t npaths = (int64_t)cam.w * cam.h * spp;
-  uint32_t nlook = 0;
-  if (p < npaths) {
-    int64_t pix = p / spp;
-    uint32_t s = (uint32_t)(p % spp);
-    int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
-    PathSampler ps;
-    ps.index = halton_pixel_offset(hp, x, y) + s * hp.stride;
-    ps.dim = 0;
-    float ux = ps.next(tab, hp);
-    float uy = ps.next(tab, hp);
-    V3 o, d;
-    gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
-    lpath[p] = svol_li<COUNT>(v, L, tab, hp, ps, o, d, max_depth, nlook);
-  }
+FLG_D void count_lookups(uint32_t nlook, unsigned long long* nlookups) {
   if (COUNT) {
     unsigned long long c = nlook;
     for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(nlookups, c);
   }
+}
+
+// One lane per path sample.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab,
+                                              HaltonParams hp, CamDevV cam, int32_t spp,
+                                              int32_t max_depth, float* lpath,
+                                              unsigned long long* nlookups) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  uint32_t nlook = 0;
+  if (p < npaths) {
+    SvolPath s;
+    bool done = svol_begin<COUNT>(v, L, tab, hp, cam, spp, (uint32_t)p, s, nlook);
+    while (!done) done = svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook);
+    lpath[p] = s.Lr;
+  }
+  count_lookups<COUNT>(nlook, nlookups);
+}
+
+// Persistent lanes with path regeneration: a lane whose path completes takes
+// the next unstarted path (wave-aggregated fetch from a global counter), so
+// short and long paths no longer idle each other's wave.  Every path runs
+// the same per-path code as k_svol: results are identical.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, HaltonTables tab,
+                                                    HaltonParams hp, CamDevV cam, int32_t spp,
+                                                    int32_t max_depth, float* lpath,
+                                                    unsigned int* next_path,
+                                                    unsigned long long* nlookups) {
+  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
+  const int lane = threadIdx.x & 63;
+  uint32_t nlook = 0;
+  SvolPath s;
+  uint32_t p = 0;
+  bool need = true, exhausted = false;
+  while (true) {
+    // lanes without a path draw the next indices, one atomic per wave
+    const uint64_t want = __ballot(need && !exhausted);
+    if (want) {
+      uint32_t base = 0;
+      const int leader = __builtin_ctzll(want);
+      if (lane == leader) base = atomicAdd(next_path, (unsigned int)__popcll(want));
+      base = __shfl(base, leader, 64);
+      if (need && !exhausted) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        p = base + (uint32_t)__popcll(want & below);
+        if (p >= npaths) {
+          exhausted = true;
+        } else {
+          need = false;
+          if (svol_begin<COUNT>(v, L, tab, hp, cam, spp, p, s, nlook)) {
+            lpath[p] = s.Lr;
+            need = true;
+          }
+        }
+      }
+    }
+    if (__ballot(!need) == 0 && __ballot(!exhausted) == 0) break;
+    if (!need) {
+      if (svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook)) {
+        lpath[p] = s.Lr;
+        need = true;
+      }
+    }
+  }
+  count_lookups<COUNT>(nlook, nlookups);
 }
 
 __global__ __launch_bounds__(256) void k_pixel_mean(const float* lpath, int64_t npix, int32_t spp,
@@ -331,6 +410,8 @@ int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene
   if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 ||
       cfg->spp <= 0 || cfg->max_depth < 1)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
+  if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
+    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: too many paths");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
@@ -348,15 +429,27 @@ int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene
   float* lpath = (float*)(sc + 256);
   unsigned long long* nl = (unsigned long long*)sc;
   const bool count = h_stats != nullptr;
+  unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
+  const int sched = cfg->schedule ? cfg->schedule : 2;
+  if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
   {
     ScopedTimer tm(ctx, "volpath_paths");
-    if (count) {
-      k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 1);
-      k_svol<true><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
-                                                                 cfg->max_depth, lpath, nl);
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
+    if (sched == 1) {
+      if (count)
+        k_svol<true><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
+                                                                   cfg->max_depth, lpath, nl);
+      else
+        k_svol<false><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
+                                                                    cfg->max_depth, lpath, nl);
     } else {
-      k_svol<false><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
-                                                                  cfg->max_depth, lpath, nl);
+      const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+      if (count)
+        k_svol_regen<true><<<grid, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp, cfg->max_depth,
+                                                          lpath, next_path, nl);
+      else
+        k_svol_regen<false><<<grid, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp, cfg->max_depth,
+                                                           lpath, next_path, nl);
     }
   }
   {

@@ -267,7 +267,9 @@ typedef struct flg_vol_scene {
 
 typedef struct flg_render_cfg {
   int32_t width, height, spp, max_depth;
-  int32_t reserved[8];
+  int32_t schedule; /* 1: one lane per path sample; 2: persistent lanes that
+                       take a new path when theirs ends; 0: 2.  Same results. */
+  int32_t reserved[7];
 } flg_render_cfg;
 
 /* Renders width*height RGB float pixels (d_rgb, 3 floats per pixel) with the

@@ -249,7 +249,7 @@ void test_lbvh_accel(fledge::gpu::Context& ctx) {
     fledge::SInteraction again;
     CHECK(accel.intersect(r, again, &id) && id == id_ref, "ray %zu: re-intersect", i);
   }
-  CHECK(nhit > 1000, "too few hits (%d)", nhit);
+  CHECK(nhit > 100, "too few hits (%d)", nhit);  // sparse soup: about 5 % of the rays hit
   bool threw = false;
   try {
     std::vector<int> bad(ind.begin(), ind.begin() + 4);

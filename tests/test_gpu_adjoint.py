@@ -40,8 +40,15 @@ def test_diff_forward_matches_fp64_twin(ctx):
     got = img.cpu().numpy()[:, :, 0]
     rel = np.abs(got - img64) / np.maximum(np.abs(img64), 1e-6)
     assert rel.max() < 1e-4
-    rel_p = np.abs(lpath.cpu().numpy() - lp64) / np.maximum(np.abs(lp64), 1e-6)
-    assert np.quantile(rel_p, 0.999) < 1e-4
+    # per path: float (GPU) against the FP64 twin, relative to the path's value
+    # or, for paths far darker than average (products of near-zero ratio
+    # tracking factors, where float and double round apart), to the mean path
+    lp = lpath.cpu().numpy().astype(np.float64)
+    scale = max(float(np.abs(lp64).mean()), 1e-6)
+    rel_p = np.abs(lp - lp64) / np.maximum(np.abs(lp64), scale)
+    print("per-path: max %.3g at %d (gpu %.9g, fp64 %.9g), mean path %.4g" % (
+        rel_p.max(), int(rel_p.argmax()), lp[rel_p.argmax()], lp64[rel_p.argmax()], scale), flush=True)
+    assert rel_p.max() < 1e-4
 
 
 def test_diff_forward_estimates_the_plain_integrator(ctx):
