@@ -130,100 +130,144 @@ FLG_D float nee_sample(const DiffDev& D, const LightsDev& L, const HaltonTables&
   return f * Li / pdf / light_pdf;
 }
 
-// One path.  ADJ = false: returns the radiance.  ADJ = true: replays with
-// the path's total L_total and adjoint weight delta, depositing gradients.
+// One path, as a start (camera ray, box entry, primary transmittance) and one
+// step per scattering event, so that persistent lanes can interleave paths.
+// ADJ = false: accumulates the radiance.  ADJ = true: replays with the path's
+// total L_total and adjoint weight delta, depositing gradients.
+struct DiffPath {
+  PathSampler ps;
+  V3 o, d;
+  float beta, acc, delta, L_total;
+  int bounces;
+};
+
+// Returns true when the path is complete (or, in the adjoint, has no weight).
 template <bool ADJ>
-FLG_D float diff_path(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
-                      const HaltonParams& hp, PathSampler& ps, V3 o, V3 d, int max_depth,
-                      float delta, float L_total, float* grad, float* galpha) {
-  float acc = 0.0f;
+FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, int32_t spp, uint32_t p,
+                      const float* lpath, const float* dl_drgb, float* grad, DiffPath& s) {
+  const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.acc = 0.0f;
+  if (ADJ) {
+    const float* g = dl_drgb + 3 * (int64_t)pix;
+    s.delta = (g[0] + g[1] + g[2]) / (float)spp;
+    if (s.delta == 0.0f) return true;
+    s.L_total = lpath[p];
+  }
+  s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+  s.ps.dim = 0;
+  const float ux = s.ps.next(tab, hp);
+  const float uy = s.ps.next(tab, hp);
+  V3 o, d;
+  gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
   const float Le = L.env.present ? L.env.radiance : 0.0f;
   float t_min, t_max;
-  if (!box_enter(D.v.bmin, D.v.bmax, o, d, &t_min, &t_max)) return Le;
+  if (!box_enter(D.v.bmin, D.v.bmax, o, d, &t_min, &t_max)) {
+    s.acc = Le;
+    return true;
+  }
   float T;
   if (ADJ) {
-    const uint32_t dim0 = ps.dim;
+    const uint32_t dim0 = s.ps.dim;
     int nz = 0;
-    T = ratio_track<1>(D, tab, hp, ps, o, d, __builtin_inff(), &nz, 0.0f, 0.0f, grad);
-    ps.dim = dim0;
-    (void)ratio_track<2>(D, tab, hp, ps, o, d, __builtin_inff(), &nz, T, delta * Le, grad);
+    T = ratio_track<1>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, 0.0f, 0.0f, grad);
+    s.ps.dim = dim0;
+    (void)ratio_track<2>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, T, s.delta * Le, grad);
     if (nz > 0) T = 0.0f;
   } else {
-    T = ratio_track<0>(D, tab, hp, ps, o, d, __builtin_inff(), nullptr, 0.0f, 0.0f, grad);
+    T = ratio_track<0>(D, tab, hp, s.ps, o, d, __builtin_inff(), nullptr, 0.0f, 0.0f, grad);
   }
-  acc += T * Le;
-  o = o + d * t_min;
-  float beta = 1.0f;
-  const float rr = 0.01f;
-  const float k = -D.s / D.mu;
-  for (int bounces = 1;; ++bounces) {
-    // differential delta tracking
-    float tb0, tb1;
-    bool hit = false;
-    V3 pos = o;
-    if (box_pbrt(D.v.bmin, D.v.bmax, o, d, __builtin_inff(), &tb0, &tb1) && D.mu > 0.0f) {
-      float t = smax(tb0, 0.0f);
-      tb1 = smin(tb1, __builtin_inff());
-      for (int step = 0; step < kMaxSteps; ++step) {
-        t += -flg_logf(1.0f - ps.next(tab, hp)) / D.mu;
-        if (t >= tb1) break;
-        const V3 p = o + d * t;
-        const Look2 q = look2(D, p);
-        const float p_n = 1.0f - D.s * q.rho_p / D.mu;
-        const bool real = ps.next(tab, hp) > p_n;
-        if (real) {
-          if (ADJ) {
-            const float S = L_total - acc;
-            if (q.t.rho != 0.0f) deposit(D, grad, q.t, delta * S / q.t.rho);
-            *galpha += delta * S / D.alpha;
-          }
-          beta *= D.alpha * (q.t.rho / q.rho_p);
-          pos = p;
-          hit = true;
-          break;
-        }
-        const float fnull = 1.0f - D.s * q.t.rho / D.mu;
-        if (ADJ && fnull != 0.0f) deposit(D, grad, q.t, delta * (L_total - acc) * k / fnull);
-        // a null collision has probability p_n > 0 except for u == 0
-        // exactly; that measure-zero case keeps weight 1
-        if (p_n != 0.0f) beta *= fnull / p_n;
-      }
-    }
-    if (!hit) break;
-    const V3 wo = -d;
-    float u = ps.next(tab, hp);
-    float vv = ps.next(tab, hp);
-    const V3 wi = hg_sample(wo, u, vv, D.g);
-    V3 p_light;
-    const float K = nee_sample(D, L, tab, hp, ps, pos, wo, &p_light);
-    if (K != 0.0f) {
-      V3 diff = p_light - pos;
-      float nrm = sqrtf(sqnorm(diff));
-      V3 dd = diff / nrm;
-      float Tsh;
-      if (ADJ) {
-        const uint32_t dim0 = ps.dim;
-        int nz = 0;
-        Tsh = ratio_track<1>(D, tab, hp, ps, pos, dd, nrm - SHADOW_EPS, &nz, 0.0f, 0.0f, grad);
-        ps.dim = dim0;
-        (void)ratio_track<2>(D, tab, hp, ps, pos, dd, nrm - SHADOW_EPS, &nz, Tsh, delta * beta * K, grad);
-        if (nz > 0) Tsh = 0.0f;
-      } else {
-        Tsh = ratio_track<0>(D, tab, hp, ps, pos, dd, nrm - SHADOW_EPS, nullptr, 0.0f, 0.0f, grad);
-      }
-      acc += beta * K * Tsh;
-    }
-    if (bounces >= max_depth || beta == 0.0f) break;
-    o = pos;
-    d = normalize(wi);
-    if (bounces > 10) {
-      if (ps.next(tab, hp) < rr) break;
-      beta /= (1.0f - rr);
-    }
-  }
-  return acc;
+  s.acc += T * Le;
+  s.o = o + d * t_min;
+  s.d = d;
+  s.beta = 1.0f;
+  s.bounces = 1;
+  return false;
 }
 
+// One scattering event; returns true when the path is complete.
+template <bool ADJ>
+FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
+                     const HaltonParams& hp, int max_depth, float* grad, float* galpha, DiffPath& s) {
+  const float rr = 0.01f;
+  const float k = -D.s / D.mu;
+  // differential delta tracking
+  float tb0, tb1;
+  bool hit = false;
+  V3 pos = s.o;
+  if (box_pbrt(D.v.bmin, D.v.bmax, s.o, s.d, __builtin_inff(), &tb0, &tb1) && D.mu > 0.0f) {
+    float t = smax(tb0, 0.0f);
+    tb1 = smin(tb1, __builtin_inff());
+    for (int step = 0; step < kMaxSteps; ++step) {
+      t += -flg_logf(1.0f - s.ps.next(tab, hp)) / D.mu;
+      if (t >= tb1) break;
+      const V3 p = s.o + s.d * t;
+      const Look2 q = look2(D, p);
+      const float p_n = 1.0f - D.s * q.rho_p / D.mu;
+      const bool real = s.ps.next(tab, hp) > p_n;
+      if (real) {
+        if (ADJ) {
+          const float S = s.L_total - s.acc;
+          if (q.t.rho != 0.0f) deposit(D, grad, q.t, s.delta * S / q.t.rho);
+          *galpha += s.delta * S / D.alpha;
+        }
+        s.beta *= D.alpha * (q.t.rho / q.rho_p);
+        pos = p;
+        hit = true;
+        break;
+      }
+      const float fnull = 1.0f - D.s * q.t.rho / D.mu;
+      if (ADJ && fnull != 0.0f) deposit(D, grad, q.t, s.delta * (s.L_total - s.acc) * k / fnull);
+      // a null collision has probability p_n > 0 except for u == 0
+      // exactly; that measure-zero case keeps weight 1
+      if (p_n != 0.0f) s.beta *= fnull / p_n;
+    }
+  }
+  if (!hit) return true;
+  const V3 wo = -s.d;
+  const float u = s.ps.next(tab, hp);
+  const float vv = s.ps.next(tab, hp);
+  const V3 wi = hg_sample(wo, u, vv, D.g);
+  V3 p_light;
+  const float K = nee_sample(D, L, tab, hp, s.ps, pos, wo, &p_light);
+  if (K != 0.0f) {
+    const V3 diff = p_light - pos;
+    const float nrm = sqrtf(sqnorm(diff));
+    const V3 dd = diff / nrm;
+    float Tsh;
+    if (ADJ) {
+      const uint32_t dim0 = s.ps.dim;
+      int nz = 0;
+      Tsh = ratio_track<1>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, 0.0f, 0.0f, grad);
+      s.ps.dim = dim0;
+      (void)ratio_track<2>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, Tsh, s.delta * s.beta * K, grad);
+      if (nz > 0) Tsh = 0.0f;
+    } else {
+      Tsh = ratio_track<0>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, nullptr, 0.0f, 0.0f, grad);
+    }
+    s.acc += s.beta * K * Tsh;
+  }
+  if (s.bounces >= max_depth || s.beta == 0.0f) return true;
+  s.o = pos;
+  s.d = normalize(wi);
+  if (s.bounces > 10) {
+    if (s.ps.next(tab, hp) < rr) return true;
+    s.beta /= (1.0f - rr);
+  }
+  ++s.bounces;
+  return false;
+}
+
+template <bool ADJ>
+FLG_D void reduce_galpha(float ga, float* galpha) {
+  if (ADJ) {
+    for (int off = 32; off >= 1; off >>= 1) ga += __shfl_xor(ga, off, 64);
+    if ((threadIdx.x & 63) == 0 && ga != 0.0f) unsafeAtomicAdd(galpha, ga);
+  }
+}
+
+// One lane per path sample.
 template <bool ADJ>
 __global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTables tab,
                                               HaltonParams hp, CamDevV cam, int32_t spp,
@@ -233,28 +277,57 @@ __global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTabl
   const int64_t npaths = (int64_t)cam.w * cam.h * spp;
   float ga = 0.0f;
   if (p < npaths) {
-    const int64_t pix = p / spp;
-    const uint32_t s = (uint32_t)(p % spp);
-    const int x = (int)(pix % cam.w), y = (int)(pix / cam.w);
-    PathSampler ps;
-    ps.index = halton_pixel_offset(hp, x, y) + s * hp.stride;
-    ps.dim = 0;
-    float ux = ps.next(tab, hp);
-    float uy = ps.next(tab, hp);
-    V3 o, d;
-    gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
-    if (ADJ) {
-      const float* g = dl_drgb + 3 * pix;
-      const float delta = (g[0] + g[1] + g[2]) / (float)spp;
-      if (delta != 0.0f) diff_path<true>(D, L, tab, hp, ps, o, d, max_depth, delta, lpath[p], grad, &ga);
-    } else {
-      lpath[p] = diff_path<false>(D, L, tab, hp, ps, o, d, max_depth, 0.0f, 0.0f, grad, &ga);
-    }
+    DiffPath s;
+    bool done = diff_begin<ADJ>(D, L, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, grad, s);
+    while (!done) done = diff_step<ADJ>(D, L, tab, hp, max_depth, grad, &ga, s);
+    if (!ADJ) lpath[p] = s.acc;
   }
-  if (ADJ) {
-    for (int off = 32; off >= 1; off >>= 1) ga += __shfl_xor(ga, off, 64);
-    if ((threadIdx.x & 63) == 0 && ga != 0.0f) unsafeAtomicAdd(galpha, ga);
+  reduce_galpha<ADJ>(ga, galpha);
+}
+
+// Persistent lanes with path regeneration (regen_loop); same per-path code.
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_diff_regen(DiffDev D, LightsDev L, HaltonTables tab,
+                                                    HaltonParams hp, CamDevV cam, int32_t spp,
+                                                    int32_t max_depth, float* lpath,
+                                                    const float* dl_drgb, float* grad, float* galpha,
+                                                    unsigned int* next_path) {
+  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
+  float ga = 0.0f;
+  regen_loop<DiffPath>(
+      npaths, next_path,
+      [&](uint32_t p, DiffPath& s) {
+        return diff_begin<ADJ>(D, L, tab, hp, cam, spp, p, lpath, dl_drgb, grad, s);
+      },
+      [&](DiffPath& s) { return diff_step<ADJ>(D, L, tab, hp, max_depth, grad, &ga, s); },
+      [&](uint32_t p, const DiffPath& s) {
+        if (!ADJ) lpath[p] = s.acc;
+      });
+  reduce_galpha<ADJ>(ga, galpha);
+}
+
+__global__ void k_zero_counter(unsigned int* c) { *c = 0u; }
+
+// Launches one pass (forward or adjoint) with the schedule of cfg.
+template <bool ADJ>
+int launch_diff(flg_ctx* ctx, const flg_render_cfg* cfg, const DiffDev& D, const LightsDev& L,
+                const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam, float* lpath,
+                const float* dl, float* grad, float* galpha) {
+  const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+  const int sched = cfg->schedule ? cfg->schedule : 2;
+  if (sched == 1) {
+    k_diff<ADJ><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(D, L, tab, hp, cam, cfg->spp, cfg->max_depth,
+                                                               lpath, dl, grad, galpha);
+    return FLG_OK;
   }
+  int st = FLG_OK;
+  unsigned int* counter = (unsigned int*)scratch(ctx, ADJ ? "diff_counter_adj" : "diff_counter_fwd", 256, &st);
+  if (!counter) return st;
+  k_zero_counter<<<1, 1, 0, ctx->stream>>>(counter);
+  const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+  k_diff_regen<ADJ><<<grid, 256, 0, ctx->stream>>>(D, L, tab, hp, cam, cfg->spp, cfg->max_depth, lpath, dl,
+                                                   grad, galpha, counter);
+  return FLG_OK;
 }
 
 __global__ __launch_bounds__(256) void k_mean(const float* lpath, int64_t npix, int32_t spp,
@@ -275,6 +348,10 @@ int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg
   if (!ctx || !vol || !scene || !cfg || !prm || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
       cfg->max_depth < 1)
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: bad arguments");
+  if (cfg->schedule < 0 || cfg->schedule > 2)
+    return set_error(ctx, FLG_ERR_INVALID, "differentiable render: schedule must be 0, 1 or 2");
+  if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
+    return set_error(ctx, FLG_ERR_RANGE, "differentiable render: too many paths");
   if (proposal && (proposal->nx != vol->nx || proposal->ny != vol->ny || proposal->nz != vol->nz))
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: proposal grid dimensions differ");
   if (!(prm->majorant > 0.0f) || !(prm->albedo > 0.0f))
@@ -323,9 +400,7 @@ int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* pr
   }
   {
     ScopedTimer tm(ctx, "diff_forward");
-    k_diff<false><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(D, L, tab, hp, cam, cfg->spp,
-                                                                cfg->max_depth, lpath, nullptr,
-                                                                nullptr, nullptr);
+    if ((st = launch_diff<false>(ctx, cfg, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr))) return st;
     if (d_rgb) k_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
   }
   FLG_CHECK_LAUNCH(ctx);
@@ -345,12 +420,11 @@ int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposa
   if (st) return st;
   if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_adjoint: null buffer");
-  const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
   {
     ScopedTimer tm(ctx, "diff_adjoint");
-    k_diff<true><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(D, L, tab, hp, cam, cfg->spp,
-                                                               cfg->max_depth, (float*)d_lpath,
-                                                               d_dl_drgb, d_grad, d_grad_albedo);
+    if ((st = launch_diff<true>(ctx, cfg, D, L, tab, hp, cam, (float*)d_lpath, d_dl_drgb, d_grad,
+                                d_grad_albedo)))
+      return st;
   }
   FLG_CHECK_LAUNCH(ctx);
   return FLG_OK;

@@ -247,41 +247,12 @@ __global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, Halto
                                                     unsigned int* next_path,
                                                     unsigned long long* nlookups) {
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
-  const int lane = threadIdx.x & 63;
   uint32_t nlook = 0;
-  SvolPath s;
-  uint32_t p = 0;
-  bool need = true, exhausted = false;
-  while (true) {
-    // lanes without a path draw the next indices, one atomic per wave
-    const uint64_t want = __ballot(need && !exhausted);
-    if (want) {
-      uint32_t base = 0;
-      const int leader = __builtin_ctzll(want);
-      if (lane == leader) base = atomicAdd(next_path, (unsigned int)__popcll(want));
-      base = __shfl(base, leader, 64);
-      if (need && !exhausted) {
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        p = base + (uint32_t)__popcll(want & below);
-        if (p >= npaths) {
-          exhausted = true;
-        } else {
-          need = false;
-          if (svol_begin<COUNT>(v, L, tab, hp, cam, spp, p, s, nlook)) {
-            lpath[p] = s.Lr;
-            need = true;
-          }
-        }
-      }
-    }
-    if (__ballot(!need) == 0 && __ballot(!exhausted) == 0) break;
-    if (!need) {
-      if (svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook)) {
-        lpath[p] = s.Lr;
-        need = true;
-      }
-    }
-  }
+  regen_loop<SvolPath>(
+      npaths, next_path,
+      [&](uint32_t p, SvolPath& s) { return svol_begin<COUNT>(v, L, tab, hp, cam, spp, p, s, nlook); },
+      [&](SvolPath& s) { return svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook); },
+      [&](uint32_t p, const SvolPath& s) { lpath[p] = s.Lr; });
   count_lookups<COUNT>(nlook, nlookups);
 }
 

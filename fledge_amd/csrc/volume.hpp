@@ -248,6 +248,48 @@ FLG_HD float vol_sample(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, 
   return 1.0f;
 }
 
+// Persistent lanes with path regeneration.  Each lane runs one path at a
+// time as begin(p, state) followed by step(state) until either returns true
+// (path complete), then finish(p, state) and the next unstarted path.  Path
+// indices come from *next_path, one atomic per wave for all lanes that need
+// work.  Every wave leaves the loop once the counter passes npaths and its
+// lanes' paths are complete; each path runs exactly the code of a one lane
+// per path launch, so results do not depend on the schedule.
+template <class State, class Begin, class Step, class Finish>
+FLG_D void regen_loop(uint32_t npaths, unsigned int* next_path, Begin begin, Step step, Finish finish) {
+  const int lane = threadIdx.x & 63;
+  State s;
+  uint32_t p = 0;
+  bool need = true, exhausted = false;
+  while (true) {
+    const uint64_t want = __ballot(need && !exhausted);
+    if (want) {
+      uint32_t base = 0;
+      const int leader = __builtin_ctzll(want);
+      if (lane == leader) base = atomicAdd(next_path, (unsigned int)__popcll(want));
+      base = __shfl(base, leader, 64);
+      if (need && !exhausted) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        p = base + (uint32_t)__popcll(want & below);
+        if (p >= npaths) {
+          exhausted = true;
+        } else {
+          need = false;
+          if (begin(p, s)) {
+            finish(p, s);
+            need = true;
+          }
+        }
+      }
+    }
+    if (__ballot(!need) == 0 && __ballot(!exhausted) == 0) break;
+    if (!need && step(s)) {
+      finish(p, s);
+      need = true;
+    }
+  }
+}
+
 struct EnvDev {
   const float* marg_cdf;  // 513
   const float* marg_f;    // 512

@@ -43,13 +43,14 @@ def _params(s, albedo, g, majorant):
     return p
 
 
-def _cfg(w, h, spp, depth):
+def _cfg(w, h, spp, depth, schedule=0):
     c = N.RenderCfg()
     c.width, c.height, c.spp, c.max_depth = w, h, spp, depth
+    c.schedule = schedule
     return c
 
 
-def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None):
+def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0):
     ctx = dv.ctx
     dev = torch.device("cuda", ctx.device)
     if out is None:
@@ -58,13 +59,14 @@ def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=No
         lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_diff_forward(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
-                                               ctypes.byref(_cfg(w, h, spp, depth)),
+                                               ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                                ctypes.byref(_params(s, albedo, dv.g, majorant)),
                                                _ptr(out), _ptr(lpath)))
     return out, lpath
 
 
-def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None):
+def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None,
+            schedule=0):
     ctx = dv.ctx
     dev = torch.device("cuda", ctx.device)
     nz, ny, nx = dv.vol.shape
@@ -75,7 +77,7 @@ def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, gr
     prop = dv.proposal.h if dv.proposal is not None else None
     dl = dl_drgb.contiguous()
     ctx.check(ctx.lib.flg_volpath_adjoint(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
-                                          ctypes.byref(_cfg(w, h, spp, depth)),
+                                          ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                           ctypes.byref(_params(s, albedo, dv.g, majorant)),
                                           _ptr(lpath), _ptr(dl), _ptr(grad), _ptr(galpha)))
     return grad, galpha

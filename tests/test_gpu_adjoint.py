@@ -172,3 +172,25 @@ def test_autograd_function_and_descent(ctx):
             theta.clamp_(0.0, 1.0)
             albedo.clamp_(0.05, 1.0)
     assert losses[-1] < losses[0]
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_diff_schedules_agree(ctx, schedule):
+    """One lane per path and persistent path regeneration: identical forward
+    (bit for bit) and gradients equal up to the float-atomic summation order."""
+    grid = _grid(20, 13)
+    W = H = 40
+    spp, depth = 4, 32
+    mu = S * float(grid.max())
+    dv, scene, img1, lp1 = _setup(ctx, grid, W, H, spp, depth, mu)  # default schedule
+    cfg_img = torch.empty_like(img1)
+    lp = torch.empty_like(lp1)
+    img, lp = diff.forward(dv, scene, W, H, spp, depth, S, ALBEDO, mu, out=cfg_img, lpath=lp,
+                           schedule=schedule)
+    assert torch.equal(img.view(torch.int32), img1.view(torch.int32))
+    assert torch.equal(lp.view(torch.int32), lp1.view(torch.int32))
+    dl = torch.from_numpy(np.random.default_rng(2).normal(size=(H, W, 3)).astype(np.float32)).cuda()
+    g_ref, ga_ref = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lp1, dl, schedule=1)
+    g, ga = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lp, dl, schedule=schedule)
+    rel = float((g - g_ref).norm() / g_ref.norm())
+    assert rel < 1e-5 and abs(float(ga) - float(ga_ref)) <= 1e-5 * abs(float(ga_ref))
