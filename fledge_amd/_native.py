@@ -60,12 +60,14 @@ class Camera(ctypes.Structure):
 class PtCfg(ctypes.Structure):
     _fields_ = [("width", c_i32), ("height", c_i32), ("spp", c_i32), ("max_depth", c_i32),
                 ("trace_block", c_i32), ("use_graph", c_i32), ("count_visits", c_i32),
-                ("blocks_per_cu", c_i32), ("stack_mode", c_i32), ("reserved", c_i32 * 3)]
+                ("blocks_per_cu", c_i32), ("stack_mode", c_i32), ("ray_fetch", c_i32),
+                ("reserved", c_i32 * 2)]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block_size", c_i32), ("blocks_per_cu", c_i32), ("ms", c_f), ("ms_default", c_f),
-                ("candidates", c_i32), ("stack_mode", c_i32), ("reserved", c_i32 * 2)]
+                ("candidates", c_i32), ("stack_mode", c_i32), ("ray_fetch", c_i32),
+                ("reserved", c_i32 * 1)]
 
 
 class Medium(ctypes.Structure):

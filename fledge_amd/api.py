@@ -137,7 +137,8 @@ class LBVH:
         return c
 
     def pathtrace(self, cam, width, height, spp, max_depth, record=False, trace_block=0,
-                  count_visits=False, use_graph=False, blocks_per_cu=0, stack_mode=0, out=None):
+                  count_visits=False, use_graph=False, blocks_per_cu=0, stack_mode=0, ray_fetch=0,
+                  out=None):
         """One frame of the diffuse path tracer.  record: also return per-path
         segment counts, ids and t (into `out` = (nseg, seg_id, seg_t) when
         given, so that a replayed graph writes the same buffers)."""
@@ -146,6 +147,7 @@ class LBVH:
         cfg.trace_block = trace_block
         cfg.blocks_per_cu = blocks_per_cu
         cfg.stack_mode = stack_mode
+        cfg.ray_fetch = ray_fetch
         cfg.count_visits = 1 if count_visits else 0
         cfg.use_graph = 1 if use_graph else 0
         npaths = width * height * spp
@@ -173,8 +175,8 @@ class LBVH:
         self.ctx.check(self.ctx.lib.flg_autotune_pathtrace(self.ctx.h, self.h, ctypes.byref(cam),
                                                            ctypes.byref(cfg), ctypes.byref(t)))
         return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu,
-                "stack_mode": t.stack_mode, "ms": t.ms, "ms_default": t.ms_default,
-                "candidates": t.candidates}
+                "stack_mode": t.stack_mode, "ray_fetch": t.ray_fetch, "ms": t.ms,
+                "ms_default": t.ms_default, "candidates": t.candidates}
 
     def autotune_trace(self, rays, hits):
         t = N.Tuning()

@@ -82,6 +82,10 @@ FLG_D float ratio_track(const DiffDev& D, const HaltonTables& tab, const HaltonP
   for (int step = 0; step < kMaxSteps; ++step) {
     t += -flg_logf(1.0f - ps.next(tab, hp)) / D.mu;
     if (t >= t_max) break;
+    // once the result can no longer change (the product is 0; two zero factors
+    // zero every derivative) the walk only consumes its Halton dimensions
+    if (MODE == 0 && T == 0.0f) continue;
+    if (MODE == 1 && *nzero >= 2) continue;
     const V3 pos = o + d * t;
     const Tri8 q = lookup(D.v, pos);
     const float f = 1.0f - D.s * q.rho / D.mu;
@@ -172,8 +176,10 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
     const uint32_t dim0 = s.ps.dim;
     int nz = 0;
     T = ratio_track<1>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, 0.0f, 0.0f, grad);
-    s.ps.dim = dim0;
-    (void)ratio_track<2>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, T, s.delta * Le, grad);
+    if (nz < 2 && s.delta * Le != 0.0f) {  // otherwise every deposit is 0
+      s.ps.dim = dim0;
+      (void)ratio_track<2>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, T, s.delta * Le, grad);
+    }
     if (nz > 0) T = 0.0f;
   } else {
     T = ratio_track<0>(D, tab, hp, s.ps, o, d, __builtin_inff(), nullptr, 0.0f, 0.0f, grad);
@@ -240,8 +246,10 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
       const uint32_t dim0 = s.ps.dim;
       int nz = 0;
       Tsh = ratio_track<1>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, 0.0f, 0.0f, grad);
-      s.ps.dim = dim0;
-      (void)ratio_track<2>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, Tsh, s.delta * s.beta * K, grad);
+      if (nz < 2 && s.delta * s.beta * K != 0.0f) {
+        s.ps.dim = dim0;
+        (void)ratio_track<2>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, Tsh, s.delta * s.beta * K, grad);
+      }
       if (nz > 0) Tsh = 0.0f;
     } else {
       Tsh = ratio_track<0>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, nullptr, 0.0f, 0.0f, grad);

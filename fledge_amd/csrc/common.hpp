@@ -148,7 +148,7 @@ struct flg_ctx {
   int num_cus = 256;
   // launch shapes chosen by the autotuner (0: not tuned)
   int tuned_trace_block = 0, tuned_trace_stack = 0, tuned_pt_block = 0, tuned_pt_bpc = 0,
-      tuned_pt_stack = 0;
+      tuned_pt_stack = 0, tuned_pt_fetch = 0;
   // captured bounce loops, keyed by the bytes of their launch arguments
   hipStream_t capture_stream = nullptr;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;

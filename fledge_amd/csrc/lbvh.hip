@@ -327,6 +327,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(const SegDesc* segs, BlockMap m
   const SegDesc s = segs[map.seg[b]];
   const int n = s.ntris;
   const int i = map.start[b] + (int)threadIdx.x;
+  // a single triangle has no internal node: its leaf's parent field is 0, the
+  // zero-initialised value the other builds leave in unused fields
+  if (n == 1 && i == 0) o.leaf_parent[s.tri_base] = 0;
   if (i >= n - 1) return;
   const uint32_t* code = o.morton_sorted + s.tri_base;
   int dl = delta(code, n, i, i + 1) - delta(code, n, i, i - 1);

@@ -12,6 +12,7 @@
 // identical to the recursive traversal.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "bvh.hpp"
 #include "halton.hpp"
@@ -20,6 +21,21 @@
 namespace flg {
 
 namespace {
+
+template <class Stack>
+FLG_D Stack make_stack(uint2* ring, int stride);
+template <>
+FLG_D PrivStack make_stack<PrivStack>(uint2*, int) {
+  return PrivStack{};
+}
+template <>
+FLG_D RingStack<8> make_stack<RingStack<8>>(uint2* ring, int stride) {
+  return RingStack<8>(ring, stride);
+}
+template <>
+FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride) {
+  return RingStack<1>(ring, stride);
+}
 
 // Traversal with the stack policy of RING: 0 keeps the whole stack in private
 // memory, K > 0 keeps the K most recent entries of each lane in LDS.
@@ -34,8 +50,13 @@ FLG_D int32_t trace_lane(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
   }
 }
 
+// Traversal is latency-bound: the kernels ask for 8 waves per SIMD (at most 64
+// VGPRs); the few values that no longer fit spill to scratch, which measured
+// faster than the 4-5 waves the unconstrained allocation allows.
+constexpr int kTraceWaves = 8;
+
 template <bool COMPAT, int BS, int RING>
-__global__ __launch_bounds__(BS) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
+__global__ __launch_bounds__(BS, kTraceWaves) void k_trace(TraceGeom g, const flg_ray* rays, flg_hit* hits,
                                               int64_t n) {
   __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,7 +221,7 @@ struct Frame {
 // segment into the other queue (wave-aggregated append).  Persistent grid:
 // the queue length is read from device memory, so the host never waits.
 template <bool COMPAT, bool FIRST, bool COUNT, int BS, int RING>
-__global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
+__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtState s, CamDev cam,
                                                   HaltonTables tab, HaltonParams hp, int32_t spp,
                                                   int bounce, int max_depth,
                                                   unsigned long long* visits) {
@@ -296,6 +317,126 @@ __global__ __launch_bounds__(BS) void k_pt_bounce(TraceGeom g, PtState s, CamDev
   }
 }
 
+// The same bounce with dynamic ray fetch (persistent threads, "while-while"):
+// bounce rays are incoherent and their traversal lengths heavy-tailed, so a
+// wave that takes one ray per lane idles most lanes behind its longest ray.
+// Here a lane whose traversal has ended shades its ray, appends the
+// continuation and takes the next ray from the queue (one atomic per wave for
+// all idle lanes); traversals advance kVisitsPerRound nodes between fetches.
+// Every ray runs exactly the traversal and shading of k_pt_bounce.
+constexpr int kVisitsPerRound = 8;
+
+template <bool COMPAT, bool FIRST, int BS, int RING>
+__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_dyn(TraceGeom g, PtState s, CamDev cam,
+                                                            HaltonTables tab, HaltonParams hp,
+                                                            int32_t spp, int bounce, int max_depth,
+                                                            uint32_t* fetch) {
+  __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
+  const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
+  if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
+  const QRay* qin = s.q[bounce & 1];
+  QRay* qout = s.q[(bounce + 1) & 1];
+  const int lane = threadIdx.x & 63;
+  using Stack = typename std::conditional<(RING > 0), RingStack<(RING > 0 ? RING : 1)>, PrivStack>::type;
+  Stack st = make_stack<Stack>(ring + threadIdx.x, BS);
+  Counters c{0, 0};
+  TraceRay r;
+  uint32_t path = 0, hidx = 0;
+  bool have = false, active = false, exhausted = false;
+  while (true) {
+    const uint64_t want = __ballot(!have && !exhausted);
+    if (want) {
+      const int leader = __builtin_ctzll(want);
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(want));
+      base = __shfl(base, leader, 64);
+      if (!have && !exhausted) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t i = base + (uint32_t)__popcll(want & below);
+        if (i >= n) {
+          exhausted = true;
+        } else {
+          V3 o, d;
+          if (FIRST) {
+            path = i;
+            const uint32_t pix = i / (uint32_t)spp, smp = i - pix * (uint32_t)spp;
+            const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+            PathSampler ps;
+            ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+            ps.dim = 0;
+            const float ux = ps.next(tab, hp);
+            const float uy = ps.next(tab, hp);
+            gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
+            hidx = ps.index;
+          } else {
+            const QRay q = qin[i];
+            o = v3(q.o.x, q.o.y, q.o.z);
+            d = v3(q.d.x, q.d.y, q.d.z);
+            path = f2u(q.o.w);
+            hidx = f2u(q.d.w);
+          }
+          st.reset();
+          have = true;
+          active = trace_begin<COMPAT, false>(g, o, d, __builtin_inff(), r, c);
+        }
+      }
+    }
+    if (__ballot(have) == 0 && __ballot(!exhausted) == 0) break;
+    for (int k = 0; k < kVisitsPerRound && active; ++k) active = trace_visit<COMPAT, false>(g, r, st, c);
+    bool spawn = false;
+    QRay nr;
+    if (have && !active) {
+      have = false;
+      const int32_t p = r.hit;
+      const float t = r.best;
+      const V3 o = r.o, d = r.d;
+      BTri tri;
+      if (p >= 0) tri = g.tris[p];
+      if (s.seg_id) {
+        const int64_t k = (int64_t)path * s.stride_seg + bounce;
+        s.seg_id[k] = p >= 0 ? (int32_t)f2u(tri.v0.w) : -1;
+        s.seg_t[k] = t;
+      }
+      bool end = true;
+      if (p >= 0 && bounce < max_depth) {
+        const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
+                 v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
+        const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
+        const V3 hp3 = o + d * t;
+        Frame fr(ng);
+        PathSampler ps;
+        ps.index = hidx;
+        ps.dim = 2 + 2 * (uint32_t)bounce;
+        const float u0 = ps.next(tab, hp);
+        const float u1 = ps.next(tab, hp);
+        const V3 wo_l = fr.to_local(-d);
+        const V3 wi_l = cosine_hemisphere(u0, u1);
+        const float pdf = wo_l.z * wi_l.z > 0 ? fabsf(wi_l.z) * INV_PI : 0.0f;
+        const V3 wi = fr.to_world(wi_l);
+        if (pdf != 0.0f) {
+          const V3 no = offset_ray_origin(hp3, ng, wi);
+          const V3 ndir = normalize(wi);
+          nr.o = make_float4(no.x, no.y, no.z, u2f(path));
+          nr.d = make_float4(ndir.x, ndir.y, ndir.z, u2f(hidx));
+          spawn = true;
+          end = false;
+        }
+      }
+      if (end && s.nseg) s.nseg[path] = bounce + 1;
+    }
+    const uint64_t m = __ballot(spawn);
+    if (m) {
+      uint32_t wbase = 0;
+      if (lane == 0) wbase = atomicAdd(&s.count[bounce + 1], (uint32_t)__popcll(m));
+      wbase = __shfl(wbase, 0, 64);
+      if (spawn) {
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        qout[wbase + (uint32_t)__popcll(m & lt)] = nr;
+      }
+    }
+  }
+}
+
 // Launch shapes: threads per block in {64, 128, 256, 512}; the counting
 // variants (instrumentation only) exist at 256.
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
@@ -346,10 +487,33 @@ void launch_bounce_bs(int grid, int block, hipStream_t st, const TraceGeom& g, c
   }
 }
 
+template <bool COMPAT, bool FIRST, int RING>
+void launch_dyn_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s,
+                   const CamDev& cam, const HaltonTables& tab, const HaltonParams& hp, int spp, int b,
+                   int D, uint32_t* fetch) {
+  switch (block) {
+    case 64: k_pt_dyn<COMPAT, FIRST, 64, RING><<<grid, 64, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
+    case 128: k_pt_dyn<COMPAT, FIRST, 128, RING><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
+    case 512: k_pt_dyn<COMPAT, FIRST, 512, RING><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
+    default: k_pt_dyn<COMPAT, FIRST, 256, RING><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
+  }
+}
+
 template <bool COMPAT>
-void launch_bounce(bool first, bool count, int grid, int block, int stack_mode, hipStream_t st,
+void launch_bounce(bool first, bool count, bool dyn, int grid, int block, int stack_mode, hipStream_t st,
                    const TraceGeom& g, const PtState& s, const CamDev& cam, const HaltonTables& tab,
-                   const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
+                   const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits,
+                   uint32_t* fetch) {
+  if (dyn && !count) {
+    if (ring_of(stack_mode)) {
+      if (first) launch_dyn_bs<COMPAT, true, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
+      else launch_dyn_bs<COMPAT, false, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
+    } else {
+      if (first) launch_dyn_bs<COMPAT, true, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
+      else launch_dyn_bs<COMPAT, false, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
+    }
+    return;
+  }
   if (count) {  // instrumentation: one shape
     if (first)
       k_pt_bounce<COMPAT, true, true, 256, 0><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
@@ -471,9 +635,12 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
   if (bpc == 0) bpc = 8;
   const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
-  if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2)
+  int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
+  if (fmode == 0) fmode = 2;
+  if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 2)
     return set_error(ctx, FLG_ERR_INVALID,
-                     "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64");
+                     "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
+                     "stack_mode and ray_fetch 0..2");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
@@ -501,17 +668,20 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
   const int pgrid = ctx->num_cus * bpc;
   // the whole frame: clear the queue counts, then one fused launch per bounce
+  // s.count[0 .. D+1]: queue sizes; s.count[128 + b]: fetch counter of bounce b
+  uint32_t* fetch = s.count + 128;
+  const bool dyn = fmode == 2;
   auto enqueue = [&](hipStream_t stream, bool timed) {
-    k_pt_zero<<<1, 256, 0, stream>>>(s.count, D + 2);
+    k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
     if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
     for (int b = 0; b <= D; ++b) {
       ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
       if (compat)
-        launch_bounce<true>(b == 0, count_visits, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp, b, D,
-                            visits);
+        launch_bounce<true>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+                            b, D, visits, fetch + b);
       else
-        launch_bounce<false>(b == 0, count_visits, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp, b,
-                             D, visits);
+        launch_bounce<false>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+                             b, D, visits, fetch + b);
     }
   };
   if (cfg->use_graph) {
@@ -524,7 +694,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     put(&cd, sizeof cd);
     put(&tab, sizeof tab);
     put(&hp, sizeof hp);
-    const int32_t misc[7] = {cfg->spp, D, pgrid, tb, smode, compat ? 1 : 0, count_visits ? 1 : 0};
+    const int32_t misc[8] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0};
     put(misc, sizeof misc);
     put(&visits, sizeof visits);
     hipGraphExec_t exec = nullptr;
@@ -617,8 +787,9 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
   // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
   const int blocks[4] = {64, 128, 256, 512};
   const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
-  int best_b = 256, best_bpc = 8, best_s = 1, n = 0;
+  int best_b = 256, best_bpc = 8, best_s = 1, best_f = 2, n = 0;
   float best = 1e30f, def = 0.0f;
+  for (int fm = 1; fm <= 2; ++fm)
   for (int sm = 1; sm <= 2; ++sm)
     for (int b : blocks)
       for (int t : threads_per_cu) {
@@ -626,6 +797,7 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
         c.trace_block = b;
         c.blocks_per_cu = t / b;
         c.stack_mode = sm;
+        c.ray_fetch = fm;
         c.count_visits = 0;
         c.use_graph = 0;
         if (c.blocks_per_cu > 64) continue;
@@ -635,21 +807,24 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
             2, &ms);
         if (st) return st;
         ++n;
-        if (b == 256 && c.blocks_per_cu == 8 && sm == 1) def = ms;
+        if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 2) def = ms;
         if (ms < best) {
           best = ms;
           best_b = b;
           best_bpc = c.blocks_per_cu;
           best_s = sm;
+          best_f = fm;
         }
       }
   ctx->tuned_pt_block = best_b;
   ctx->tuned_pt_bpc = best_bpc;
   ctx->tuned_pt_stack = best_s;
+  ctx->tuned_pt_fetch = best_f;
   *out = flg_tuning{};
   out->block_size = best_b;
   out->blocks_per_cu = best_bpc;
   out->stack_mode = best_s;
+  out->ray_fetch = best_f;
   out->ms = best;
   out->ms_default = def;
   out->candidates = n;

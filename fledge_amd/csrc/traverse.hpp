@@ -52,6 +52,7 @@ struct PrivStack {
   int32_t node[kStack];
   float t[kStack];
   int sp = 0;
+  FLG_HD void reset() { sp = 0; }
   FLG_HD bool empty() const { return sp == 0; }
   FLG_HD void push(int32_t n, float tn) {
     if (sp < kStack) {
@@ -79,6 +80,7 @@ struct RingStack {
   int32_t node[kStack - K];
   float t[kStack - K];
   FLG_HD RingStack(uint2* f, int s) : fast(f), stride(s) {}
+  FLG_HD void reset() { nfast = top = nspill = 0; }
   FLG_HD bool empty() const { return nfast == 0 && nspill == 0; }
   FLG_HD void push(int32_t n, float tn) {
     if (nfast == K) {  // full: the oldest ring entry sits in the slot about to be written
@@ -108,55 +110,84 @@ struct RingStack {
   }
 };
 
+// Resumable traversal of one ray: trace_begin tests the root (and a
+// one-triangle tree) and returns false when nothing is left to visit;
+// trace_visit visits one node and returns false when the traversal has
+// ended.  begin + visits until false is exactly the recursive order of the
+// reference; persistent kernels interleave visits of different rays with it.
+struct TraceRay {
+  V3 o, d, inv;
+  float best;
+  int32_t hit;   // sorted position of the closest hit so far, -1 if none
+  int32_t node;  // next node to visit
+  int64_t visits;
+};
+
+template <bool COMPAT, bool COUNT>
+FLG_HD bool trace_begin(const TraceGeom& g, V3 o, V3 d, float tmax, TraceRay& r, Counters& c) {
+  r.o = o;
+  r.d = d;
+  r.best = tmax;
+  r.hit = -1;
+  r.node = 0;
+  r.visits = 0;
+  if (g.n <= 0) return false;
+  r.inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  float tn;
+  if (!slab(g.root[0], g.root[1], g.root[2], g.root[3], g.root[4], g.root[5], o, r.inv, &tn)) return false;
+  if (r.best < tn) return false;
+  if (g.n == 1) {
+    test_tri<COMPAT, COUNT>(g, 0, o, d, r.best, r.hit, c);
+    return false;
+  }
+  return true;
+}
+
+template <bool COMPAT, bool COUNT, class Stack>
+FLG_HD bool trace_visit(const TraceGeom& g, TraceRay& r, Stack& st, Counters& c) {
+  // every node is visited at most once; the bound only guards a corrupt tree
+  if (r.visits >= (int64_t)g.n) return false;
+  ++r.visits;
+  if (COUNT) ++c.nodes;
+  const V3 o = r.o, d = r.d, inv = r.inv;
+  const BNode nd = g.nodes[r.node];
+  const int32_t refL = nd.d.x, refR = nd.d.y;
+  if (refL < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refL & 0x7fffffffu, o, d, r.best, r.hit, c);
+  if (refR < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refR & 0x7fffffffu, o, d, r.best, r.hit, c);
+  float tL = 0.0f, tR = 0.0f;
+  const bool goL = refL >= 0 && slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, &tL) &&
+                   !(r.best < tL);
+  const bool goR = refR >= 0 && slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, &tR) &&
+                   !(r.best < tR);
+  if (goL) {
+    if (goR) st.push(refR, tR);
+    r.node = refL;
+    return true;
+  }
+  if (goR) {
+    r.node = refR;
+    return true;
+  }
+  while (!st.empty()) {
+    int32_t n;
+    float t;
+    st.pop(&n, &t);
+    if (!(r.best < t)) {
+      r.node = n;
+      return true;
+    }
+  }
+  return false;
+}
+
 // Returns the sorted position of the hit (-1 on a miss); best holds t.
 template <bool COMPAT, bool COUNT, class Stack>
 FLG_HD int32_t trace_one_s(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, Stack& st) {
-  int32_t hit = -1;
-  if (g.n <= 0) return -1;
-  const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  float tn;
-  if (!slab(g.root[0], g.root[1], g.root[2], g.root[3], g.root[4], g.root[5], o, inv, &tn)) return -1;
-  if (best < tn) return -1;
-  if (g.n == 1) {
-    test_tri<COMPAT, COUNT>(g, 0, o, d, best, hit, c);
-    return hit;
-  }
-  int32_t node = 0;
-  // every node is visited at most once; the bound only guards a corrupt tree
-  for (int64_t visits = 0; visits < (int64_t)g.n; ++visits) {
-    if (COUNT) ++c.nodes;
-    const BNode nd = g.nodes[node];
-    const int32_t refL = nd.d.x, refR = nd.d.y;
-    if (refL < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refL & 0x7fffffffu, o, d, best, hit, c);
-    if (refR < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refR & 0x7fffffffu, o, d, best, hit, c);
-    float tL = 0.0f, tR = 0.0f;
-    bool goL = refL >= 0 && slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, &tL) &&
-               !(best < tL);
-    bool goR = refR >= 0 && slab(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, &tR) &&
-               !(best < tR);
-    if (goL) {
-      if (goR) st.push(refR, tR);
-      node = refL;
-      continue;
-    }
-    if (goR) {
-      node = refR;
-      continue;
-    }
-    bool found = false;
-    while (!st.empty()) {
-      int32_t n;
-      float t;
-      st.pop(&n, &t);
-      if (!(best < t)) {
-        node = n;
-        found = true;
-        break;
-      }
-    }
-    if (!found) break;
-  }
-  return hit;
+  TraceRay r;
+  bool more = trace_begin<COMPAT, COUNT>(g, o, d, best, r, c);
+  while (more) more = trace_visit<COMPAT, COUNT>(g, r, st, c);
+  best = r.best;
+  return r.hit;
 }
 
 template <bool COMPAT, bool COUNT>

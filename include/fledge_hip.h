@@ -213,7 +213,11 @@ typedef struct flg_pt_cfg {
   int32_t blocks_per_cu; /* persistent grid of each bounce = CUs x this (1..64);
                             0: tuned value, else 8 */
   int32_t stack_mode;    /* as flg_trace_cfg.stack_mode */
-  int32_t reserved[3];
+  int32_t ray_fetch;     /* 1: each lane traces one queue entry per loop trip;
+                            2: lanes whose ray has ended take the next entry
+                            while the others traverse (persistent threads);
+                            0: tuned value, else 2.  Same results. */
+  int32_t reserved[2];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                                   const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
@@ -224,7 +228,8 @@ FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_ca
  * the fastest in the context, where calls whose block_size / trace_block /
  * blocks_per_cu are 0 pick it up.  The tuned calls run the real work: their
  * outputs are written as by a normal call.  Tuned: threads per block, resident
- * blocks per CU (path tracer) and where the traversal stack lives.  Stack
+ * blocks per CU and ray fetch (path tracer) and where the traversal stack
+ * lives.  Stack
  * depth (64, the bound on the tree depth) and node width (binary, which the
  * reference's visiting order needs) are fixed. */
 typedef struct flg_tuning {
@@ -234,7 +239,8 @@ typedef struct flg_tuning {
   float ms_default;      /* one call with the untuned shape (256 threads, 8 blocks per CU) */
   int32_t candidates;    /* shapes timed */
   int32_t stack_mode;    /* chosen traversal stack (1 private, 2 LDS ring) */
-  int32_t reserved[2];
+  int32_t ray_fetch;     /* chosen ray fetch of the path tracer (1 or 2; 0 for trace) */
+  int32_t reserved[1];
 } flg_tuning;
 FLG_API int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
                                flg_hit* d_hits, int64_t n, flg_tuning* out);

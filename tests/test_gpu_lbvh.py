@@ -186,17 +186,24 @@ def _same_frame(a, b):
     assert torch.equal(a["seg_t"][mask].view(torch.int32), b["seg_t"][mask].view(torch.int32))
 
 
-@pytest.mark.parametrize("block,bpc,smode", [(64, 16, 1), (128, 4, 2), (512, 2, 1), (256, 32, 2),
-                                              (512, 16, 2), (64, 64, 2)])
-def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode):
-    """Every launch shape and stack policy the autotuner may pick gives the same
-    frame (segment ids and t bit for bit)."""
+@pytest.mark.parametrize("block,bpc,smode,fetch", [(64, 16, 1, 1), (128, 4, 2, 1), (512, 2, 1, 2),
+                                                    (256, 32, 2, 2), (512, 16, 2, 1), (64, 64, 2, 2),
+                                                    (256, 8, 1, 1), (128, 16, 1, 2)])
+def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode, fetch):
+    """Every launch shape, stack policy and ray fetch the autotuner may pick
+    gives the same frame (segment ids and t bit for bit), equal to the oracle."""
     g, o, cam10, cam = _octa_case(ctx)
     W, H, spp, D = 96, 64, 2, 6
-    ref = g.pathtrace(cam, W, H, spp, D, record=True)
+    ref = g.pathtrace(cam, W, H, spp, D, record=True, ray_fetch=1)
     got = g.pathtrace(cam, W, H, spp, D, record=True, trace_block=block, blocks_per_cu=bpc,
-                      stack_mode=smode)
+                      stack_mode=smode, ray_fetch=fetch)
     _same_frame(ref, got)
+    nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, W * H * spp)
+    assert got["segments"] == nrays
+    assert np.array_equal(got["nseg"].cpu().numpy(), nseg)
+    mask = np.arange(D + 1)[None, :] < nseg[:, None]
+    assert np.array_equal(got["seg_id"].cpu().numpy()[mask], ids[mask])
+    assert np.array_equal(got["seg_t"].cpu().numpy()[mask].view(np.uint32), ts[mask].view(np.uint32))
 
 
 @pytest.mark.parametrize("block", [64, 256, 512])
@@ -250,7 +257,8 @@ def test_autotuner_choices_give_identical_results(ctx):
     ref = g.pathtrace(cam, W, H, spp, D, record=True)
     t = g.autotune(cam, W, H, spp, D)
     assert t["block_size"] in (64, 128, 256, 512) and 1 <= t["blocks_per_cu"] <= 64
-    assert t["candidates"] >= 24 and t["stack_mode"] in (1, 2) and t["ms"] <= t["ms_default"] * 1.0001
+    assert t["candidates"] >= 48 and t["stack_mode"] in (1, 2) and t["ray_fetch"] in (1, 2)
+    assert t["ms"] <= t["ms_default"] * 1.0001
     _same_frame(ref, g.pathtrace(cam, W, H, spp, D, record=True))  # uses the tuned shape
     rays = api.camera_rays(ctx, cam, 128, 128)
     hits = torch.empty((128 * 128, 2), dtype=torch.int32, device="cuda")
