@@ -80,7 +80,9 @@ FLG_D float ratio_track(const DiffDev& D, const HaltonTables& tab, const HaltonP
   float T = 1.0f;
   const float k = -D.s / D.mu;
   for (int step = 0; step < kMaxSteps; ++step) {
-    t += -flg_logf(1.0f - ps.next(tab, hp)) / D.mu;
+    const float u = ps.next(tab, hp);
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / D.mu;
     if (t >= t_max) break;
     // once the result can no longer change (the product is 0; two zero factors
     // zero every derivative) the walk only consumes its Halton dimensions
@@ -206,7 +208,9 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
     float t = smax(tb0, 0.0f);
     tb1 = smin(tb1, __builtin_inff());
     for (int step = 0; step < kMaxSteps; ++step) {
-      t += -flg_logf(1.0f - s.ps.next(tab, hp)) / D.mu;
+      const float u = s.ps.next(tab, hp);
+      if (u == 0.0f) break;  // as in ratio_track
+      t += -flg_logf(1.0f - u) / D.mu;
       if (t >= tb1) break;
       const V3 p = s.o + s.d * t;
       const Look2 q = look2(D, p);
@@ -277,10 +281,12 @@ FLG_D void reduce_galpha(float ga, float* galpha) {
 
 // One lane per path sample.
 template <bool ADJ>
-__global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTables tab,
+__global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTables tab_g,
                                               HaltonParams hp, CamDevV cam, int32_t spp,
                                               int32_t max_depth, float* lpath,
                                               const float* dl_drgb, float* grad, float* galpha) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t npaths = (int64_t)cam.w * cam.h * spp;
   float ga = 0.0f;
@@ -295,11 +301,13 @@ __global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTabl
 
 // Persistent lanes with path regeneration (regen_loop); same per-path code.
 template <bool ADJ>
-__global__ __launch_bounds__(256) void k_diff_regen(DiffDev D, LightsDev L, HaltonTables tab,
+__global__ __launch_bounds__(256) void k_diff_regen(DiffDev D, LightsDev L, HaltonTables tab_g,
                                                     HaltonParams hp, CamDevV cam, int32_t spp,
                                                     int32_t max_depth, float* lpath,
                                                     const float* dl_drgb, float* grad, float* galpha,
                                                     unsigned int* next_path) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
   float ga = 0.0f;
   regen_loop<DiffPath>(

@@ -18,7 +18,14 @@ struct HaltonTables {
   const float* inv = nullptr;
   int32_t nprimes = 0;
   int32_t* overflow = nullptr;  // set to 1 when a dimension beyond the table is asked for
+  // optional copy of the first nlocal (prime, 1/prime bits) pairs in LDS: the
+  // draws of a path hit it instead of the global table's latency
+  const int2* local = nullptr;
+  int32_t nlocal = 0;
 };
+
+// Entries of the tables a kernel stages in LDS (8 KB per block).
+constexpr int kLocalPrimes = 1024;
 
 // Per-resolution constants of HaltonSampler's ctor (sampler.h:145-163).
 struct HaltonParams {
@@ -62,8 +69,31 @@ FLG_D float halton_get1d(const HaltonTables& t, uint32_t index, uint32_t dim, in
     return 0.5f;
   }
   uint32_t a = dim == 0 ? index / (uint32_t)bs0 : (dim == 1 ? index / (uint32_t)bs1 : index);
-  float v = radical_inverse(t.prime[dim], t.inv[dim], a);
+  int32_t p;
+  float ib;
+  if (dim < (uint32_t)t.nlocal) {
+    const int2 e = t.local[dim];
+    p = e.x;
+    ib = u2f((uint32_t)e.y);
+  } else {
+    p = t.prime[dim];
+    ib = t.inv[dim];
+  }
+  float v = radical_inverse(p, ib, a);
   return smin(v, ONE_MINUS_EPSILON);
+}
+
+// Copies the first kLocalPrimes table entries into the block's LDS and returns
+// tables that use them.  Every thread of the block must call it (it
+// synchronises the block).
+FLG_D HaltonTables stage_halton(const HaltonTables& t, int2* lds) {
+  const int n = t.nprimes < kLocalPrimes ? t.nprimes : kLocalPrimes;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = make_int2(t.prime[i], (int)f2u(t.inv[i]));
+  __syncthreads();
+  HaltonTables r = t;
+  r.local = lds;
+  r.nlocal = n;
+  return r;
 }
 
 // InverseRadicalInverse (sampler.h:111-120)

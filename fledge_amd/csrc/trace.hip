@@ -23,18 +23,18 @@ namespace flg {
 namespace {
 
 template <class Stack>
-FLG_D Stack make_stack(uint2* ring, int stride);
+FLG_D Stack make_stack(uint2* ring, int stride, uint2* priv);
 template <>
-FLG_D PrivStack make_stack<PrivStack>(uint2*, int) {
-  return PrivStack{};
+FLG_D PrivStack make_stack<PrivStack>(uint2*, int, uint2* priv) {
+  return PrivStack(priv);
 }
 template <>
-FLG_D RingStack<8> make_stack<RingStack<8>>(uint2* ring, int stride) {
-  return RingStack<8>(ring, stride);
+FLG_D RingStack<8> make_stack<RingStack<8>>(uint2* ring, int stride, uint2* priv) {
+  return RingStack<8>(ring, stride, priv);
 }
 template <>
-FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride) {
-  return RingStack<1>(ring, stride);
+FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride, uint2* priv) {
+  return RingStack<1>(ring, stride, priv);
 }
 
 // Traversal with the stack policy of RING: 0 keeps the whole stack in private
@@ -42,7 +42,8 @@ FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride) {
 template <bool COMPAT, bool COUNT, int BS, int RING>
 FLG_D int32_t trace_lane(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, uint2* ring) {
   if constexpr (RING > 0) {
-    RingStack<RING> st(ring + threadIdx.x, BS);
+    uint2 spill[kStack - RING];
+    RingStack<RING> st(ring + threadIdx.x, BS, spill);
     return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
   } else {
     (void)ring;
@@ -338,7 +339,8 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_dyn(TraceGeom g, PtState
   QRay* qout = s.q[(bounce + 1) & 1];
   const int lane = threadIdx.x & 63;
   using Stack = typename std::conditional<(RING > 0), RingStack<(RING > 0 ? RING : 1)>, PrivStack>::type;
-  Stack st = make_stack<Stack>(ring + threadIdx.x, BS);
+  uint2 priv[kStack];
+  Stack st = make_stack<Stack>(ring + threadIdx.x, BS, priv);
   Counters c{0, 0};
   TraceRay r;
   uint32_t path = 0, hidx = 0;
@@ -636,7 +638,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   if (bpc == 0) bpc = 8;
   const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
   int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
-  if (fmode == 0) fmode = 2;
+  if (fmode == 0) fmode = 1;
   if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 2)
     return set_error(ctx, FLG_ERR_INVALID,
                      "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
@@ -787,7 +789,7 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
   // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
   const int blocks[4] = {64, 128, 256, 512};
   const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
-  int best_b = 256, best_bpc = 8, best_s = 1, best_f = 2, n = 0;
+  int best_b = 256, best_bpc = 8, best_s = 1, best_f = 1, n = 0;
   float best = 1e30f, def = 0.0f;
   for (int fm = 1; fm <= 2; ++fm)
   for (int sm = 1; sm <= 2; ++sm)
@@ -807,7 +809,7 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
             2, &ms);
         if (st) return st;
         ++n;
-        if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 2) def = ms;
+        if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 1) def = ms;
         if (ms < best) {
           best = ms;
           best_b = b;

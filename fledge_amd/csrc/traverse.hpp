@@ -47,47 +47,48 @@ FLG_HD void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, in
 // 64 (the common-prefix length grows strictly from a node to its children and
 // is below 64), so a 64-entry stack never overflows.
 
-// All entries in private memory (scratch on the device).
+// All entries in private memory (scratch on the device).  The entries are an
+// array owned by the caller, so that the stack pointer stays in a register
+// (a counter stored beside a dynamically indexed array goes to scratch with it).
 struct PrivStack {
-  int32_t node[kStack];
-  float t[kStack];
+  uint2* e;  // kStack entries: node, tnear bits
   int sp = 0;
+  FLG_HD explicit PrivStack(uint2* entries) : e(entries) {}
   FLG_HD void reset() { sp = 0; }
   FLG_HD bool empty() const { return sp == 0; }
   FLG_HD void push(int32_t n, float tn) {
     if (sp < kStack) {
-      node[sp] = n;
-      t[sp] = tn;
+      e[sp] = make_uint2((uint32_t)n, f2u(tn));
       ++sp;
     }
   }
   FLG_HD void pop(int32_t* n, float* tn) {
     --sp;
-    *n = node[sp];
-    *tn = t[sp];
+    const uint2 x = e[sp];
+    *n = (int32_t)x.x;
+    *tn = u2f(x.y);
   }
 };
 
 // The K most recent entries in a fast per-lane ring (LDS on the device; entry
 // k of this lane at fast[k * stride]), older ones moved to a private spill
-// stack when the ring is full.  Pops take the ring first, then the spill
-// stack, so the order is exactly that of PrivStack.
+// array (kStack - K entries, owned by the caller) when the ring is full.  When
+// the ring runs empty, the most recent spilled entries are copied back into it
+// before the pop, so pops always read the ring (no load that could come from
+// either memory) and the order is exactly that of PrivStack.
 template <int K>
 struct RingStack {
   uint2* fast;
   int stride;
+  uint2* spill;
   int nfast = 0, top = 0, nspill = 0;  // top: next ring slot to write
-  int32_t node[kStack - K];
-  float t[kStack - K];
-  FLG_HD RingStack(uint2* f, int s) : fast(f), stride(s) {}
+  FLG_HD RingStack(uint2* f, int s, uint2* sp) : fast(f), stride(s), spill(sp) {}
   FLG_HD void reset() { nfast = top = nspill = 0; }
   FLG_HD bool empty() const { return nfast == 0 && nspill == 0; }
   FLG_HD void push(int32_t n, float tn) {
     if (nfast == K) {  // full: the oldest ring entry sits in the slot about to be written
       if (nspill >= kStack - K) return;  // unreachable (depth < 64), as PrivStack
-      const uint2 e = fast[top * stride];
-      node[nspill] = (int32_t)e.x;
-      t[nspill] = u2f(e.y);
+      spill[nspill] = fast[top * stride];
       ++nspill;
       --nfast;
     }
@@ -96,17 +97,22 @@ struct RingStack {
     ++nfast;
   }
   FLG_HD void pop(int32_t* n, float* tn) {
-    if (nfast > 0) {
-      top = top == 0 ? K - 1 : top - 1;
-      const uint2 e = fast[top * stride];
-      *n = (int32_t)e.x;
-      *tn = u2f(e.y);
-      --nfast;
-    } else {
-      --nspill;
-      *n = node[nspill];
-      *tn = t[nspill];
+    if (nfast == 0) {  // refill from the spill array, oldest first
+      const int m = nspill < K ? nspill : K;
+      top = 0;
+      for (int j = nspill - m; j < nspill; ++j) {
+        fast[top * stride] = spill[j];
+        ++top;
+      }
+      if (top == K) top = 0;
+      nfast = m;
+      nspill -= m;
     }
+    top = top == 0 ? K - 1 : top - 1;
+    const uint2 x = fast[top * stride];
+    --nfast;
+    *n = (int32_t)x.x;
+    *tn = u2f(x.y);
   }
 };
 
@@ -192,7 +198,8 @@ FLG_HD int32_t trace_one_s(const TraceGeom& g, V3 o, V3 d, float& best, Counters
 
 template <bool COMPAT, bool COUNT>
 FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c) {
-  PrivStack st;
+  uint2 entries[kStack];
+  PrivStack st(entries);
   return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
 }
 

@@ -220,10 +220,12 @@ FLG_D void count_lookups(uint32_t nlook, unsigned long long* nlookups) {
 
 // One lane per path sample.
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab,
+__global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab_g,
                                               HaltonParams hp, CamDevV cam, int32_t spp,
                                               int32_t max_depth, float* lpath,
                                               unsigned long long* nlookups) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t npaths = (int64_t)cam.w * cam.h * spp;
   uint32_t nlook = 0;
@@ -241,11 +243,13 @@ __global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTable
 // short and long paths no longer idle each other's wave.  Every path runs
 // the same per-path code as k_svol: results are identical.
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, HaltonTables tab,
+__global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, HaltonTables tab_g,
                                                     HaltonParams hp, CamDevV cam, int32_t spp,
                                                     int32_t max_depth, float* lpath,
                                                     unsigned int* next_path,
                                                     unsigned long long* nlookups) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
   uint32_t nlook = 0;
   regen_loop<SvolPath>(

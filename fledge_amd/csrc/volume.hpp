@@ -172,10 +172,11 @@ FLG_HD V3 hg_sample(V3 wo, float u, float vv, float g) {
 
 // Environment light tables (Dist2D of InfiniteAreaLight, light.cpp:93-108,
 // distribution.hpp:12-92), 512 x 512.
-// Free-flight steps per tracking call.  A Halton index of 0 (pixel offset 0,
-// sample 0) makes every dimension 0, so t never advances and the reference's
-// loops never end; the cap ends such a call (ratio tracking returns the
-// current transmittance, delta tracking reports no collision).
+// Free-flight walks.  A Halton index of 0 (pixel offset 0, sample 0) makes
+// every dimension >= 2 equal to 0, so t never advances and the reference's
+// loops never end.  A walk therefore ends at a zero draw (only index 0 gives
+// one), and in any case after kMaxTrackSteps steps: ratio tracking returns the
+// current transmittance, delta tracking reports no collision.
 constexpr int kMaxTrackSteps = 4096;
 constexpr float kInf = __builtin_huge_valf();
 
@@ -198,7 +199,9 @@ FLG_HD float vol_tr(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, uint
   const float max_mu = v.max_mu;
   if (!(max_mu > 0.0f)) return 1.0f;  // empty medium: no interaction
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - rng.next()) / max_mu;
+    const float u = rng.next();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / max_mu;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     float density = lookup(v, pos).rho;
@@ -230,7 +233,9 @@ FLG_HD float vol_sample(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, 
     return 1.0f;
   }
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - rng.next()) / max_mu;
+    const float u = rng.next();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / max_mu;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     float density = lookup(v, pos).rho;

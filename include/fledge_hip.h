@@ -216,7 +216,7 @@ typedef struct flg_pt_cfg {
   int32_t ray_fetch;     /* 1: each lane traces one queue entry per loop trip;
                             2: lanes whose ray has ended take the next entry
                             while the others traverse (persistent threads);
-                            0: tuned value, else 2.  Same results. */
+                            0: tuned value, else 1.  Same results. */
   int32_t reserved[2];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,

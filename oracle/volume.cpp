@@ -26,11 +26,11 @@
 
 namespace orc {
 
-// Free-flight steps per tracking call.  Halton index 0 (pixel offset 0,
-// sample 0) gives 0 in every dimension, so t never advances and the
-// reference's loops (volume.cpp:86, :123) never end; the cap ends the call
-// (ratio tracking returns the current transmittance, delta tracking reports
-// no collision).
+// Free-flight walks.  Halton index 0 (pixel offset 0, sample 0) gives 0 in
+// every dimension >= 2, so t never advances and the reference's loops
+// (volume.cpp:86, :123) never end.  A walk ends at a zero draw (only index 0
+// gives one) and in any case after kMaxTrackSteps steps: ratio tracking
+// returns the current transmittance, delta tracking reports no collision.
 static constexpr int kMaxTrackSteps = 4096;
 
 // Trilinear lookup on the dense grid: index space g = (p - bmin) * scale - 0.5,
@@ -109,7 +109,9 @@ static float vol_tr(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, 
   float max_mu = sc.med.max_mu;
   if (!(max_mu > 0.0f)) return 1.0f;  // empty medium (the reference divides by 0)
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - hs.get1D()) / max_mu;
+    const float u = hs.get1D();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / max_mu;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     float density = grid_lookup(sc.grid, pos).rho;
@@ -141,7 +143,9 @@ static float vol_sample(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tm
     return 1.0f;
   }
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - hs.get1D()) / max_mu;
+    const float u = hs.get1D();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / max_mu;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     float density = grid_lookup(sc.grid, pos).rho;

@@ -80,7 +80,9 @@ double ratio_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, float tmax, std:
   if (!(sc.mu_f > 0.0f)) return 1.0;
   double T = 1.0;
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - hs.get1D()) / sc.mu_f;
+    const float u = hs.get1D();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / sc.mu_f;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     Tri q = lookup2(sc, pos);
@@ -104,7 +106,9 @@ bool delta_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, V3* pos, std::vect
   t_max = smin(t_max, INFINITY);
   if (!(sc.mu_f > 0.0f)) return false;
   for (int step = 0; step < kMaxTrackSteps; ++step) {
-    t += -flg_logf(1.0f - hs.get1D()) / sc.mu_f;
+    const float u = hs.get1D();
+    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
+    t += -flg_logf(1.0f - u) / sc.mu_f;
     if (t >= t_max) break;
     V3 p = o + d * t;
     Tri q = lookup2(sc, p);

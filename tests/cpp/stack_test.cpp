@@ -58,7 +58,8 @@ void pack(const orc::Lbvh& b, const float* verts, const int32_t* ind, std::vecto
 template <int K>
 int32_t trace_ring(const flg::TraceGeom& g, flg::V3 o, flg::V3 d, float& best, flg::Counters& c) {
   uint2 fast[K];
-  flg::RingStack<K> st(fast, 1);
+  uint2 spill[flg::kStack - K];
+  flg::RingStack<K> st(fast, 1, spill);
   return flg::trace_one_s<false, true>(g, o, d, best, c, st);
 }
 
