@@ -155,7 +155,7 @@ def main():
     bytes_per_launch = bytes_per_ray * segments / launches_per_frame
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
                 "kernel": "k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)",
                 "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                 "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
@@ -244,6 +244,20 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(out))
+
+
+def measured_traffic():
+    """HBM bytes per k_pt_bounce launch from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs of `bench.py --profile-pass`, each scaled
+    by what the counter reads for a 1 GiB device copy); None if absent."""
+    path = os.path.join(ROOT, "profiles", "pt_bounce_octasphere2M_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(v, ind, W, H, SPP, DEPTH, seconds):

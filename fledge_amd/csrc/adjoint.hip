@@ -28,6 +28,16 @@ namespace {
 
 constexpr int kMaxSteps = 4096;
 
+// Steps a walk of this estimator may take over [t, t_max]: 8 times the
+// expected count (mu * length) plus 64, at most kMaxSteps.  A walk of ordinary
+// draws never comes near it; it trims the walks of the first Halton indices,
+// whose high dimensions are tiny (index / prime) and give minute free flights
+// (without roulette such a walk would run thousands of steps).
+FLG_D int walk_cap(float mu, float t, float t_max) {
+  const float e = 8.0f * mu * (t_max - t) + 64.0f;
+  return e < (float)kMaxSteps ? (int)e : kMaxSteps;
+}
+
 struct DiffDev {
   VolDev v;   // evaluated grid theta (densities rho)
   VolDev vp;  // proposal grid theta_p (decisions)
@@ -79,7 +89,8 @@ FLG_D float ratio_track(const DiffDev& D, const HaltonTables& tab, const HaltonP
   if (!(D.mu > 0.0f)) return 1.0f;
   float T = 1.0f;
   const float k = -D.s / D.mu;
-  for (int step = 0; step < kMaxSteps; ++step) {
+  const int cap = walk_cap(D.mu, t, t_max);
+  for (int step = 0; step < cap; ++step) {
     const float u = ps.next(tab, hp);
     if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
     t += -flg_logf(1.0f - u) / D.mu;
@@ -207,7 +218,8 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
   if (box_pbrt(D.v.bmin, D.v.bmax, s.o, s.d, __builtin_inff(), &tb0, &tb1) && D.mu > 0.0f) {
     float t = smax(tb0, 0.0f);
     tb1 = smin(tb1, __builtin_inff());
-    for (int step = 0; step < kMaxSteps; ++step) {
+    const int cap = walk_cap(D.mu, t, tb1);
+    for (int step = 0; step < cap; ++step) {
       const float u = s.ps.next(tab, hp);
       if (u == 0.0f) break;  // as in ratio_track
       t += -flg_logf(1.0f - u) / D.mu;

@@ -32,6 +32,14 @@ namespace {
 
 constexpr int kMaxTrackSteps = 4096;
 
+// Steps a walk may take over [t, t_max]: 8 times the expected count
+// (mu * length) plus 64, at most kMaxTrackSteps (the estimator's rule for
+// the minute free flights of the first Halton indices).
+int walk_cap(float mu, float t, float t_max) {
+  const float e = 8.0f * mu * (t_max - t) + 64.0f;
+  return e < (float)kMaxTrackSteps ? (int)e : kMaxTrackSteps;
+}
+
 struct DiffScene {
   Grid gp;                 // proposal grid (float), also gives lookup positions/weights
   const double* theta;     // evaluated grid
@@ -79,7 +87,8 @@ double ratio_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, float tmax, std:
   t_max = smin(t_max, tmax);
   if (!(sc.mu_f > 0.0f)) return 1.0;
   double T = 1.0;
-  for (int step = 0; step < kMaxTrackSteps; ++step) {
+  const int cap = walk_cap(sc.mu_f, t, t_max);
+  for (int step = 0; step < cap; ++step) {
     const float u = hs.get1D();
     if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
     t += -flg_logf(1.0f - u) / sc.mu_f;
@@ -105,7 +114,8 @@ bool delta_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, V3* pos, std::vect
   float t = smax(t_min, 0.0f);
   t_max = smin(t_max, INFINITY);
   if (!(sc.mu_f > 0.0f)) return false;
-  for (int step = 0; step < kMaxTrackSteps; ++step) {
+  const int cap = walk_cap(sc.mu_f, t, t_max);
+  for (int step = 0; step < cap; ++step) {
     const float u = hs.get1D();
     if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
     t += -flg_logf(1.0f - u) / sc.mu_f;
