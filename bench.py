@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--graph", action="store_true", help="replay each frame from a captured hipGraph")
+    ap.add_argument("--shape", default="",
+                    help="fixed launch shape block,blocks_per_cu,stack_mode,ray_fetch for every frame "
+                         "(skips the autotuner; profiling runs use the shape it picked)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one calibration copy + one headline frame, no JSON (for counter runs)")
     args = ap.parse_args()
@@ -88,7 +91,11 @@ def main():
     build_ms = timed(lambda: api.LBVH.build(ctx, dv, di).close(), 3, 1, sync) * 1e3
     cam2 = cam_of(N, (0.0, 0.0, -3.0), (0.0, 0.0, 0.0))
 
-    info = bvh.pathtrace(cam2, W, H, SPP, DEPTH)  # also warms the queues
+    shape = {}
+    if args.shape:
+        b, bpc, sm, rf = (int(x) for x in args.shape.split(","))
+        shape = {"trace_block": b, "blocks_per_cu": bpc, "stack_mode": sm, "ray_fetch": rf}
+    info = bvh.pathtrace(cam2, W, H, SPP, DEPTH, **shape)  # also warms the queues
     segments = info["segments"]
     log("frame: %d paths, %d ray segments" % (info["paths"], segments))
 
@@ -101,21 +108,27 @@ def main():
         sync()
         log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 4, dst.numel() * 4))
         del src, dst
-        bvh.pathtrace(cam2, W, H, SPP, DEPTH)
+        bvh.pathtrace(cam2, W, H, SPP, DEPTH, **shape)
         sync()
         log("profile pass done: one frame of %d ray segments" % segments)
         return
 
     # launch shape picked by the library's autotuner on this frame (untimed)
     tuning = None
-    if not args.no_tune:
+    if shape:
+        tuning = {"block_size": shape["trace_block"], "blocks_per_cu": shape["blocks_per_cu"],
+                  "stack_mode": shape["stack_mode"], "ray_fetch": shape["ray_fetch"], "fixed": True}
+    elif not args.no_tune:
         tuning = bvh.autotune(cam2, W, H, SPP, DEPTH)
         log("autotune: %s" % tuning)
 
-    def frame():
+    def frame(use_graph=args.graph):
         cfg = N.PtCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = W, H, SPP, DEPTH
-        cfg.use_graph = 1 if args.graph else 0
+        cfg.use_graph = 1 if use_graph else 0
+        if shape:
+            cfg.trace_block, cfg.blocks_per_cu = shape["trace_block"], shape["blocks_per_cu"]
+            cfg.stack_mode, cfg.ray_fetch = shape["stack_mode"], shape["ray_fetch"]
         ctx.check(ctx.lib.flg_pathtrace_diffuse(ctx.h, bvh.h, cam2, cfg, None, None, None, None))
 
     # timed region: K frames, HIP-event kernel timers recorded live
@@ -133,9 +146,7 @@ def main():
     total_ms, _ = ctx.kernel_time("pathtrace_total")
     if args.graph:  # a replayed graph has no per-launch timers: one timed eager frame
         ctx.reset_profiling()
-        cfg = N.PtCfg()
-        cfg.width, cfg.height, cfg.spp, cfg.max_depth = W, H, SPP, DEPTH
-        ctx.check(ctx.lib.flg_pathtrace_diffuse(ctx.h, bvh.h, cam2, cfg, None, None, None, None))
+        frame(use_graph=False)
         sync()
         trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
         total_ms, _ = ctx.kernel_time("pathtrace_total")
@@ -146,7 +157,7 @@ def main():
 
     # algorithmic bytes per ray segment: 32 (ray) + 8 (hit) + 64 per node
     # visit + 36 per triangle test, visits counted by the instrumented kernel
-    cnt = bvh.pathtrace(cam2, W, H, SPP, DEPTH, count_visits=True)
+    cnt = bvh.pathtrace(cam2, W, H, SPP, DEPTH, count_visits=True, **shape)
     vn = cnt["node_visits"] / segments
     vt = cnt["tri_tests"] / segments
     bytes_per_ray = 32 + 8 + 64 * vn + 36 * vt

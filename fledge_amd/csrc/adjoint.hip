@@ -3,20 +3,28 @@
 // No counterpart in the reference (the closest code is OpenVDBVolume::tr and
 // ::sample, src/volume.cpp:75-148, inside SVolIntegrator::Li,
 // src/integrator.cpp:328-405).  The estimator keeps SVolIntegrator's path
-// structure and Halton dimensions and makes every branch independent of the
-// evaluated density grid theta (differential delta tracking, Nimier-David et
-// al. 2022; path replay backpropagation, Vicini et al. 2021):
+// structure and makes every branch independent of the evaluated density grid
+// theta (differential delta tracking, Nimier-David et al. 2022; path replay
+// backpropagation, Vicini et al. 2021):
 //   * fixed majorant mu; collision decisions on the proposal grid theta_p
 //     (real iff u > 1 - s rho_p / mu) with weight alpha rho / rho_p on a real
 //     collision and (1 - s rho / mu) / (1 - s rho_p / mu) on a null one;
 //   * ratio tracking without roulette on camera and shadow rays, factor
 //     1 - s rho / mu per tentative collision;
 //   * the path roulette after 10 bounces is kept (it depends on u only).
-// The adjoint replays each path with the same Halton dimensions.  For a
-// main-path event j the contributions that carry its weight sum to
-// L_total - (radiance emitted before j), so the gradient needs no division
-// by the throughput; a ratio-tracking product is walked twice (product and
-// zero count, then per-lookup deposits), which stays exact when a factor is 0.
+// The main path (delta tracking, phase, light choice, roulette) draws the
+// path's Halton dimensions in SVolIntegrator's order; every ratio-tracking
+// walk draws from its own counter-based stream (walk_draw: walk 0 is the
+// camera ray, walk b the shadow ray of bounce b).  A walk therefore stops as
+// soon as its product and every derivative are fixed (two zero factors, or a
+// product that reached 0) without shifting any later draw.
+// The adjoint replays each path with the same draws.  For a main-path event j
+// the contributions that carry its weight sum to L_total - (radiance emitted
+// before j), so the gradient needs no division by the throughput.  A
+// ratio-tracking product is measured (product of the non-zero factors, count
+// of zero factors), then walked again to deposit d(product)/d(rho) per
+// lookup, which stays exact when a factor is 0.  The forward can record every
+// walk's measurement on a tape, which spares the adjoint the first walk.
 #include <algorithm>
 #include <cmath>
 
@@ -30,9 +38,9 @@ constexpr int kMaxSteps = 4096;
 
 // Steps a walk of this estimator may take over [t, t_max]: 8 times the
 // expected count (mu * length) plus 64, at most kMaxSteps.  A walk of ordinary
-// draws never comes near it; it trims the walks of the first Halton indices,
-// whose high dimensions are tiny (index / prime) and give minute free flights
-// (without roulette such a walk would run thousands of steps).
+// draws never comes near it; on the main path it trims the delta-tracking
+// walks of the first Halton indices, whose high dimensions are tiny
+// (index / prime) and give minute free flights, and it bounds every loop.
 FLG_D int walk_cap(float mu, float t, float t_max) {
   const float e = 8.0f * mu * (t_max - t) + 64.0f;
   return e < (float)kMaxSteps ? (int)e : kMaxSteps;
@@ -43,6 +51,13 @@ struct DiffDev {
   VolDev vp;  // proposal grid theta_p (decisions)
   bool same;  // theta_p is theta
   float s, alpha, g, mu;
+  // walk tape (NULL: none).  Walk w of path p: tape[w * npaths + p] = product
+  // of its non-zero factors (0 once two factors are 0); bit w of
+  // mask_hi[p]:mask_lo[p] set iff exactly one factor is 0.
+  float* tape;
+  uint32_t* mask_lo;
+  uint32_t* mask_hi;
+  uint32_t npaths;
 };
 
 struct Look2 {
@@ -74,46 +89,82 @@ FLG_D void deposit(const DiffDev& D, float* grad, const Tri8& t, float g) {
   }
 }
 
-// Ratio tracking without roulette (the camera and shadow rays).  MODE 0:
-// returns the product.  MODE 1: returns the product of the non-zero factors
-// and counts zero factors.  MODE 2: deposits scale * dT/drho for every lookup
-// given (tnz, nzero) from MODE 1.
-template <int MODE>
-FLG_D float ratio_track(const DiffDev& D, const HaltonTables& tab, const HaltonParams& hp,
-                        PathSampler& ps, V3 o, V3 d, float tmax, int* nzero, float tnz,
-                        float scale, float* grad) {
+// A ratio-tracking product: nonzero = product of the non-zero factors,
+// zeros = number of factors equal to 0.
+struct Track {
+  float nonzero;
+  int zeros;
+};
+
+// First walk of a ratio track without roulette (the camera and shadow rays):
+// factor 1 - s rho / mu per tentative collision.  Stops once the product and
+// every derivative are fixed: two zero factors (every derivative is 0), or a
+// product of the non-zero factors that reached 0.
+FLG_D Track track_measure(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
+  Track r{1.0f, 0};
   float t_min, t_max;
-  if (!box_pbrt(D.v.bmin, D.v.bmax, o, d, tmax, &t_min, &t_max)) return 1.0f;
+  if (!box_pbrt(D.v.bmin, D.v.bmax, o, d, tmax, &t_min, &t_max)) return r;
   float t = smax(t_min, 0.0f);
   t_max = smin(t_max, tmax);
-  if (!(D.mu > 0.0f)) return 1.0f;
-  float T = 1.0f;
+  const int cap = walk_cap(D.mu, t, t_max);
+  for (int step = 0; step < cap; ++step) {
+    t += -flg_logf(walk_draw(key, (uint32_t)step)) / D.mu;
+    if (t >= t_max) break;
+    const float f = 1.0f - D.s * lookup(D.v, o + d * t).rho / D.mu;
+    if (f == 0.0f) {
+      if (++r.zeros >= 2) break;
+    } else {
+      r.nonzero *= f;
+      if (r.nonzero == 0.0f) break;
+    }
+  }
+  return r;
+}
+
+// The product's value.
+FLG_D float track_value(const Track& r) { return r.zeros > 0 ? 0.0f : r.nonzero; }
+
+// Whether a measured track has a non-zero derivative.
+FLG_D bool track_live(const Track& r) { return r.zeros < 2 && r.nonzero != 0.0f; }
+
+// Second walk (same draws): deposits scale * d(product)/d(rho_i) for every
+// lookup i, d/d(rho_i) = k prod_{j != i} f_j with k = -s / mu.
+FLG_D void track_deposit(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax, const Track& r,
+                         float scale, float* grad) {
+  float t_min, t_max;
+  if (!box_pbrt(D.v.bmin, D.v.bmax, o, d, tmax, &t_min, &t_max)) return;
+  float t = smax(t_min, 0.0f);
+  t_max = smin(t_max, tmax);
   const float k = -D.s / D.mu;
   const int cap = walk_cap(D.mu, t, t_max);
   for (int step = 0; step < cap; ++step) {
-    const float u = ps.next(tab, hp);
-    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
-    t += -flg_logf(1.0f - u) / D.mu;
+    t += -flg_logf(walk_draw(key, (uint32_t)step)) / D.mu;
     if (t >= t_max) break;
-    // once the result can no longer change (the product is 0; two zero factors
-    // zero every derivative) the walk only consumes its Halton dimensions
-    if (MODE == 0 && T == 0.0f) continue;
-    if (MODE == 1 && *nzero >= 2) continue;
-    const V3 pos = o + d * t;
-    const Tri8 q = lookup(D.v, pos);
+    const Tri8 q = lookup(D.v, o + d * t);
     const float f = 1.0f - D.s * q.rho / D.mu;
-    if (MODE == 0) {
-      T *= f;
-    } else if (MODE == 1) {
-      if (f == 0.0f) ++*nzero;
-      else T *= f;
-    } else {
-      // d(prod)/d(rho_i) = k * prod_{j != i} f_j
-      float others = (*nzero == 0) ? tnz / f : ((*nzero == 1 && f == 0.0f) ? tnz : 0.0f);
-      deposit(D, grad, q, scale * k * others);
+    if (r.zeros == 0) {
+      deposit(D, grad, q, scale * k * (r.nonzero / f));
+    } else if (f == 0.0f) {  // the one zero factor: the others' product
+      deposit(D, grad, q, scale * k * r.nonzero);
+      break;
     }
   }
-  return T;
+}
+
+// Walk w of path p: measured (and recorded on the tape by a forward that has
+// one), or read back from the tape by the adjoint.
+template <bool ADJ>
+FLG_D Track track(const DiffDev& D, uint32_t key, uint32_t p, uint32_t w, V3 o, V3 d, float tmax,
+                  uint64_t& mask) {
+  if (ADJ && D.tape) {
+    return Track{D.tape[(int64_t)w * D.npaths + p], (int)((mask >> w) & 1u)};
+  }
+  const Track r = track_measure(D, key, o, d, tmax);
+  if (!ADJ && D.tape) {
+    D.tape[(int64_t)w * D.npaths + p] = r.zeros >= 2 ? 0.0f : r.nonzero;
+    if (r.zeros == 1) mask |= (uint64_t)1 << w;
+  }
+  return r;
 }
 
 // Light choice and sample (UniformSampleOneLight / EstimateDirect,
@@ -156,6 +207,8 @@ struct DiffPath {
   V3 o, d;
   float beta, acc, delta, L_total;
   int bounces;
+  uint32_t path;
+  uint64_t mask;  // tape bits of the path's walks with exactly one zero factor
 };
 
 // Returns true when the path is complete (or, in the adjoint, has no weight).
@@ -166,11 +219,14 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
   const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
   const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
   s.acc = 0.0f;
+  s.path = p;
+  s.mask = 0;
   if (ADJ) {
     const float* g = dl_drgb + 3 * (int64_t)pix;
     s.delta = (g[0] + g[1] + g[2]) / (float)spp;
     if (s.delta == 0.0f) return true;
     s.L_total = lpath[p];
+    if (D.tape) s.mask = ((uint64_t)D.mask_hi[p] << 32) | D.mask_lo[p];
   }
   s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
   s.ps.dim = 0;
@@ -184,20 +240,11 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
     s.acc = Le;
     return true;
   }
-  float T;
-  if (ADJ) {
-    const uint32_t dim0 = s.ps.dim;
-    int nz = 0;
-    T = ratio_track<1>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, 0.0f, 0.0f, grad);
-    if (nz < 2 && s.delta * Le != 0.0f) {  // otherwise every deposit is 0
-      s.ps.dim = dim0;
-      (void)ratio_track<2>(D, tab, hp, s.ps, o, d, __builtin_inff(), &nz, T, s.delta * Le, grad);
-    }
-    if (nz > 0) T = 0.0f;
-  } else {
-    T = ratio_track<0>(D, tab, hp, s.ps, o, d, __builtin_inff(), nullptr, 0.0f, 0.0f, grad);
-  }
-  s.acc += T * Le;
+  const uint32_t key = walk_key(p, 0);
+  const Track tr = track<ADJ>(D, key, p, 0, o, d, __builtin_inff(), s.mask);
+  if (ADJ && track_live(tr) && s.delta * Le != 0.0f)
+    track_deposit(D, key, o, d, __builtin_inff(), tr, s.delta * Le, grad);
+  s.acc += track_value(tr) * Le;
   s.o = o + d * t_min;
   s.d = d;
   s.beta = 1.0f;
@@ -221,7 +268,7 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
     const int cap = walk_cap(D.mu, t, tb1);
     for (int step = 0; step < cap; ++step) {
       const float u = s.ps.next(tab, hp);
-      if (u == 0.0f) break;  // as in ratio_track
+      if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
       t += -flg_logf(1.0f - u) / D.mu;
       if (t >= tb1) break;
       const V3 p = s.o + s.d * t;
@@ -257,20 +304,12 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
     const V3 diff = p_light - pos;
     const float nrm = sqrtf(sqnorm(diff));
     const V3 dd = diff / nrm;
-    float Tsh;
-    if (ADJ) {
-      const uint32_t dim0 = s.ps.dim;
-      int nz = 0;
-      Tsh = ratio_track<1>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, 0.0f, 0.0f, grad);
-      if (nz < 2 && s.delta * s.beta * K != 0.0f) {
-        s.ps.dim = dim0;
-        (void)ratio_track<2>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, &nz, Tsh, s.delta * s.beta * K, grad);
-      }
-      if (nz > 0) Tsh = 0.0f;
-    } else {
-      Tsh = ratio_track<0>(D, tab, hp, s.ps, pos, dd, nrm - SHADOW_EPS, nullptr, 0.0f, 0.0f, grad);
-    }
-    s.acc += s.beta * K * Tsh;
+    const uint32_t w = (uint32_t)s.bounces;
+    const uint32_t key = walk_key(s.path, w);
+    const Track tr = track<ADJ>(D, key, s.path, w, pos, dd, nrm - SHADOW_EPS, s.mask);
+    if (ADJ && track_live(tr) && s.delta * s.beta * K != 0.0f)
+      track_deposit(D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
+    s.acc += s.beta * K * track_value(tr);
   }
   if (s.bounces >= max_depth || s.beta == 0.0f) return true;
   s.o = pos;
@@ -281,6 +320,15 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
   }
   ++s.bounces;
   return false;
+}
+
+// Forward: the path's radiance and its tape bits.
+FLG_D void diff_finish(const DiffDev& D, uint32_t p, const DiffPath& s, float* lpath) {
+  lpath[p] = s.acc;
+  if (D.tape) {
+    D.mask_lo[p] = (uint32_t)s.mask;
+    D.mask_hi[p] = (uint32_t)(s.mask >> 32);
+  }
 }
 
 template <bool ADJ>
@@ -306,7 +354,7 @@ __global__ __launch_bounds__(256) void k_diff(DiffDev D, LightsDev L, HaltonTabl
     DiffPath s;
     bool done = diff_begin<ADJ>(D, L, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, grad, s);
     while (!done) done = diff_step<ADJ>(D, L, tab, hp, max_depth, grad, &ga, s);
-    if (!ADJ) lpath[p] = s.acc;
+    if (!ADJ) diff_finish(D, (uint32_t)p, s, lpath);
   }
   reduce_galpha<ADJ>(ga, galpha);
 }
@@ -329,7 +377,7 @@ __global__ __launch_bounds__(256) void k_diff_regen(DiffDev D, LightsDev L, Halt
       },
       [&](DiffPath& s) { return diff_step<ADJ>(D, L, tab, hp, max_depth, grad, &ga, s); },
       [&](uint32_t p, const DiffPath& s) {
-        if (!ADJ) lpath[p] = s.acc;
+        if (!ADJ) diff_finish(D, p, s, lpath);
       });
   reduce_galpha<ADJ>(ga, galpha);
 }
@@ -370,9 +418,22 @@ __global__ __launch_bounds__(256) void k_mean(const float* lpath, int64_t npix, 
   rgb[3 * pix + 2] = c;
 }
 
+// Walk tape: one float per walk (camera walk and one shadow walk per bounce)
+// per path, walk-major, then the low and high words of each path's mask.
+constexpr int kMaxTapeDepth = 63;
+
+int64_t tape_bytes(const flg_render_cfg* cfg) {
+  if (!cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 || cfg->max_depth < 1 ||
+      cfg->max_depth > kMaxTapeDepth)
+    return -1;
+  const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+  if (npaths >= ((int64_t)1 << 31)) return -1;
+  return npaths * 4 * ((int64_t)cfg->max_depth + 3);
+}
+
 int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg_vol_scene* scene,
-            const flg_render_cfg* cfg, const flg_diff_params* prm, DiffDev* D, LightsDev* L,
-            HaltonTables* tab, HaltonParams* hp, CamDevV* cam) {
+            const flg_render_cfg* cfg, const flg_diff_params* prm, const void* tape, DiffDev* D,
+            LightsDev* L, HaltonTables* tab, HaltonParams* hp, CamDevV* cam) {
   if (!ctx || !vol || !scene || !cfg || !prm || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
       cfg->max_depth < 1)
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: bad arguments");
@@ -397,6 +458,17 @@ int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg
   D->alpha = prm->albedo;
   D->g = prm->g;
   D->mu = prm->majorant;
+  const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+  D->npaths = (uint32_t)npaths;
+  D->tape = nullptr;
+  D->mask_lo = D->mask_hi = nullptr;
+  if (tape) {
+    if (cfg->max_depth > kMaxTapeDepth)
+      return set_error(ctx, FLG_ERR_RANGE, "differentiable render: a walk tape needs max_depth <= 63");
+    D->tape = (float*)tape;
+    D->mask_lo = (uint32_t*)(D->tape + (int64_t)(cfg->max_depth + 1) * npaths);
+    D->mask_hi = D->mask_lo + npaths;
+  }
   *cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
   return FLG_OK;
 }
@@ -409,15 +481,17 @@ using namespace flg;
 
 extern "C" {
 
+int64_t flg_volpath_diff_tape_bytes(const flg_render_cfg* cfg) { return tape_bytes(cfg); }
+
 int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                              const flg_vol_scene* scene, const flg_render_cfg* cfg,
-                             const flg_diff_params* params, float* d_rgb, float* d_lpath) {
+                             const flg_diff_params* params, float* d_rgb, float* d_lpath, void* d_tape) {
   DiffDev D;
   LightsDev L;
   HaltonTables tab;
   HaltonParams hp;
   CamDevV cam;
-  int st = prepare(ctx, vol, proposal, scene, cfg, params, &D, &L, &tab, &hp, &cam);
+  int st = prepare(ctx, vol, proposal, scene, cfg, params, d_tape, &D, &L, &tab, &hp, &cam);
   if (st) return st;
   if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_diff_forward: no output");
   const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
@@ -437,14 +511,14 @@ int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* pr
 
 int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                         const flg_vol_scene* scene, const flg_render_cfg* cfg,
-                        const flg_diff_params* params, const float* d_lpath,
+                        const flg_diff_params* params, const float* d_lpath, const void* d_tape,
                         const float* d_dl_drgb, float* d_grad, float* d_grad_albedo) {
   DiffDev D;
   LightsDev L;
   HaltonTables tab;
   HaltonParams hp;
   CamDevV cam;
-  int st = prepare(ctx, vol, proposal, scene, cfg, params, &D, &L, &tab, &hp, &cam);
+  int st = prepare(ctx, vol, proposal, scene, cfg, params, d_tape, &D, &L, &tab, &hp, &cam);
   if (st) return st;
   if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_adjoint: null buffer");

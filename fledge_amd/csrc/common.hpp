@@ -123,6 +123,26 @@ FLG_HD void flg_sincosf(float xf, float* s, float* c) {
   }
 }
 
+// ------------------------------------------------- counter-based draws ---
+// Integer hash "lowbias32" (C. Wellons, hash-prospector): xorshift-multiply
+// rounds, a bijection on 32 bits.
+FLG_HD uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+// Stream of one transmittance walk of the differentiable estimator: draw n of
+// walk w of path p is hash(key(p, w) + n), so a walk neither consumes nor
+// shifts the path's Halton dimensions and can stop at any step.
+FLG_HD uint32_t walk_key(uint32_t path, uint32_t walk) { return lowbias32(lowbias32(path) + walk); }
+// 1 - u for u uniform on {0, 2^-24, ..., 1 - 2^-24}: an exact float in (0, 1].
+FLG_HD float walk_draw(uint32_t key, uint32_t n) {
+  return (float)((lowbias32(key + n) >> 8) + 1u) * 0x1p-24f;
+}
+
 // ----------------------------------------------------------- host side ----
 struct KernelTimer {
   double ms = 0;

@@ -94,6 +94,9 @@ __global__ void debug_eval_kernel(int kind, const void* in, void* out, int64_t n
   } else if (kind == 3) {
     const uint32_t* p = (const uint32_t*)in;
     ((float*)out)[i] = halton_get1d(tab, p[2 * i], p[2 * i + 1], 128, 243);
+  } else if (kind == 4) {
+    const uint32_t* p = (const uint32_t*)in;
+    ((float*)out)[i] = walk_draw(walk_key(p[3 * i], p[3 * i + 1]), p[3 * i + 2]);
   }
 }
 
@@ -211,7 +214,7 @@ int flg_memcpy_dtoh(flg_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) 
 }
 
 int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n) {
-  if (!ctx || kind < 0 || kind > 3 || n < 0 || (n && (!d_in || !d_out)))
+  if (!ctx || kind < 0 || kind > 4 || n < 0 || (n && (!d_in || !d_out)))
     return set_error(ctx, FLG_ERR_INVALID, "flg_debug_eval: bad arguments");
   if (n == 0) return FLG_OK;
   HaltonTables tab;

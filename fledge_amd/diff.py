@@ -50,23 +50,45 @@ def _cfg(w, h, spp, depth, schedule=0):
     return c
 
 
-def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0):
+def tape_bytes(ctx, w, h, spp, depth):
+    """Bytes of the walk tape of one forward (flg_volpath_diff_tape_bytes)."""
+    n = int(ctx.lib.flg_volpath_diff_tape_bytes(ctypes.byref(_cfg(w, h, spp, depth))))
+    if n < 0:
+        raise ValueError("no walk tape for this configuration (max_depth must be <= 63)")
+    return n
+
+
+def new_tape(ctx, w, h, spp, depth):
+    """A device buffer for the walk tape of one forward."""
+    return torch.empty(tape_bytes(ctx, w, h, spp, depth), dtype=torch.uint8,
+                       device=torch.device("cuda", ctx.device))
+
+
+def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0,
+            tape=None):
+    """Image and per-path radiance; with `tape` (a buffer from new_tape) also
+    the walk tape that the adjoint of this call can read."""
     ctx = dv.ctx
     dev = torch.device("cuda", ctx.device)
     if out is None:
         out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
     if lpath is None:
         lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
+    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
+        raise ValueError("walk tape too small")
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_diff_forward(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                                ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                                ctypes.byref(_params(s, albedo, dv.g, majorant)),
-                                               _ptr(out), _ptr(lpath)))
+                                               _ptr(out), _ptr(lpath), _ptr(tape)))
     return out, lpath
 
 
 def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None,
-            schedule=0):
+            schedule=0, tape=None):
+    """Adds dLoss/dDensity and dLoss/dAlbedo into grad and galpha (zeros when
+    not given); lpath and tape (or None) from the forward of the same
+    arguments."""
     ctx = dv.ctx
     dev = torch.device("cuda", ctx.device)
     nz, ny, nx = dv.vol.shape
@@ -74,12 +96,14 @@ def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, gr
         grad = torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev)
     if galpha is None:
         galpha = torch.zeros(1, dtype=torch.float32, device=dev)
+    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
+        raise ValueError("walk tape too small")
     prop = dv.proposal.h if dv.proposal is not None else None
     dl = dl_drgb.contiguous()
     ctx.check(ctx.lib.flg_volpath_adjoint(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                           ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                           ctypes.byref(_params(s, albedo, dv.g, majorant)),
-                                          _ptr(lpath), _ptr(dl), _ptr(grad), _ptr(galpha)))
+                                          _ptr(lpath), _ptr(tape), _ptr(dl), _ptr(grad), _ptr(galpha)))
     return grad, galpha
 
 
@@ -93,8 +117,10 @@ class VolumeRender(torch.autograd.Function):
     def forward(ctx_, density, albedo, dv, scene, w, h, spp, depth, s, majorant):
         dv.set_density(density)
         a = float(albedo.item())
-        img, lpath = forward(dv, scene, w, h, spp, depth, s, a, majorant)
+        tape = new_tape(dv.ctx, w, h, spp, depth) if depth <= 63 else None
+        img, lpath = forward(dv, scene, w, h, spp, depth, s, a, majorant, tape=tape)
         ctx_.save_for_backward(lpath)
+        ctx_.tape = tape
         ctx_.args = (dv, scene, w, h, spp, depth, s, a, majorant)
         return img
 
@@ -103,7 +129,7 @@ class VolumeRender(torch.autograd.Function):
         (lpath,) = ctx_.saved_tensors
         dv, scene, w, h, spp, depth, s, a, majorant = ctx_.args
         grad, galpha = adjoint(dv, scene, w, h, spp, depth, s, a, majorant, lpath,
-                               grad_img.float().contiguous())
+                               grad_img.float().contiguous(), tape=ctx_.tape)
         return grad, galpha.reshape(()), None, None, None, None, None, None, None, None
 
 
@@ -130,29 +156,43 @@ def bench_adjoint(ctx, timed, sync):
     img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     grad = torch.zeros((128, 128, 128), dtype=torch.float32, device=dev)
     galpha = torch.zeros(1, dtype=torch.float32, device=dev)
+    tape = new_tape(ctx, W, H, SPP, DEPTH)
 
-    t_plain = timed(lambda: plain.render(scene, W, H, SPP, DEPTH, out=img_plain), 5, 2, sync)
-    t_fwd = timed(lambda: forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath), 5, 2, sync)
+    # every pass at both schedules (1: one lane per path, 2: persistent lanes
+    # with path regeneration); the ratio compares the fastest of each
+    t_plain = {sc: timed(lambda: plain.render(scene, W, H, SPP, DEPTH, out=img_plain, schedule=sc), 5, 2, sync)
+               for sc in (1, 2)}
+    t_fwd = {sc: timed(lambda: forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath,
+                                       schedule=sc, tape=tape), 5, 2, sync) for sc in (1, 2)}
     dl = 2.0 * (img - target) / img.numel()
 
-    def adj():
+    def adj(sc, tp=tape):
         grad.zero_()
         galpha.zero_()
-        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, dl, grad, galpha)
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, dl, grad, galpha, schedule=sc, tape=tp)
 
-    t_adj = timed(adj, 5, 2, sync)
+    t_adj = {sc: timed(lambda: adj(sc), 5, 2, sync) for sc in (1, 2)}
+    t_adj_notape = {sc: timed(lambda: adj(sc, None), 5, 2, sync) for sc in (1, 2)}
+    s_plain = min(t_plain, key=t_plain.get)
+    s_fwd = min(t_fwd, key=t_fwd.get)
+    s_adj = min(t_adj, key=t_adj.get)
 
     def iteration():
-        forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath)
+        forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath, schedule=s_fwd, tape=tape)
         d = 2.0 * (img - target) / img.numel()
         grad.zero_()
         galpha.zero_()
-        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, d, grad, galpha)
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, d, grad, galpha, schedule=s_adj, tape=tape)
 
     t_iter = timed(iteration, 5, 2, sync)
     return {"adjoint_config": "128^3 grid, 256x256, 16 spp, depth 32",
-            "plain_forward_ms": round(t_plain * 1e3, 3),
-            "diff_forward_ms": round(t_fwd * 1e3, 3),
-            "adjoint_ms": round(t_adj * 1e3, 3),
+            "plain_forward_ms": round(t_plain[s_plain] * 1e3, 3),
+            "plain_forward_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_plain.items()},
+            "diff_forward_ms": round(t_fwd[s_fwd] * 1e3, 3),
+            "diff_forward_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_fwd.items()},
+            "adjoint_ms": round(t_adj[s_adj] * 1e3, 3),
+            "adjoint_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_adj.items()},
+            "adjoint_without_tape_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_adj_notape.items()},
+            "walk_tape_bytes": tape.numel(),
             "fwd_adjoint_ms_per_iter": round(t_iter * 1e3, 3),
-            "fwd_adjoint_vs_plain": round(t_iter / t_plain, 3)}
+            "fwd_adjoint_vs_plain": round(t_iter / t_plain[s_plain], 3)}

@@ -286,12 +286,17 @@ FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scen
 /* Differentiable forward of the same path structure (differential delta
  * tracking on the proposal grid, ratio tracking without roulette on camera
  * and shadow rays, fixed majorant): sigma_t = sigma_t_scale * density,
- * sigma_s = albedo * sigma_t.  Every branch depends only on the Halton
+ * sigma_s = albedo * sigma_t.  Every branch depends only on the random
  * numbers and the proposal grid, so the image is a smooth function of `vol`'s
- * densities.  `proposal` (NULL: `vol` itself) supplies the collision
- * decisions.  d_rgb: width*height RGB (may be NULL); d_lpath: per-path
- * radiance, width*height*spp floats, path = pixel * spp + s (may be NULL;
- * the adjoint needs it). */
+ * densities.  The main path draws the Halton dimensions; each ratio-tracking
+ * walk draws from a counter-based stream of its own (path, walk), so it can
+ * stop once its result is fixed.  `proposal` (NULL: `vol` itself) supplies
+ * the collision decisions.  d_rgb: width*height RGB (may be NULL); d_lpath:
+ * per-path radiance, width*height*spp floats, path = pixel * spp + s (may be
+ * NULL; the adjoint needs it); d_tape: walk tape of
+ * flg_volpath_diff_tape_bytes(cfg) bytes (may be NULL), the measurement of
+ * every ratio-tracking walk, which spares the adjoint one of its two walks
+ * per ratio track (no reference counterpart). */
 typedef struct flg_diff_params {
   float sigma_t_scale; /* sigma_t per unit density */
   float albedo;        /* single-scatter albedo alpha, > 0 */
@@ -299,23 +304,32 @@ typedef struct flg_diff_params {
   float majorant;      /* fixed majorant per unit length, > 0 */
 } flg_diff_params;
 
+/* Bytes of the walk tape of flg_volpath_diff_forward for cfg:
+ * width*height*spp * 4 * (max_depth + 3); -1 if cfg is invalid or
+ * max_depth > 63 (a tape holds at most 64 walks per path). */
+FLG_API int64_t flg_volpath_diff_tape_bytes(const flg_render_cfg* cfg);
+
 FLG_API int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                                      const flg_vol_scene* scene, const flg_render_cfg* cfg,
-                                     const flg_diff_params* params, float* d_rgb, float* d_lpath);
-/* Path-replay adjoint of flg_volpath_diff_forward (same arguments, d_lpath
- * from that call): given dLoss/dImage (RGB, d_dl_drgb) adds dLoss/dDensity
- * into d_grad (nx*ny*nz floats, x fastest) and dLoss/dAlbedo into
- * d_grad_albedo (1 float).  Accumulation uses float atomics, so the
+                                     const flg_diff_params* params, float* d_rgb, float* d_lpath,
+                                     void* d_tape);
+/* Path-replay adjoint of flg_volpath_diff_forward (same arguments; d_lpath,
+ * and d_tape when not NULL, from that call; with d_tape NULL every ratio
+ * track is walked twice): given dLoss/dImage (RGB, d_dl_drgb) adds
+ * dLoss/dDensity into d_grad (nx*ny*nz floats, x fastest) and dLoss/dAlbedo
+ * into d_grad_albedo (1 float).  Accumulation uses float atomics, so the
  * summation order (not the terms) varies between runs. */
 FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                                 const flg_vol_scene* scene, const flg_render_cfg* cfg,
                                 const flg_diff_params* params, const float* d_lpath,
-                                const float* d_dl_drgb, float* d_grad, float* d_grad_albedo);
+                                const void* d_tape, const float* d_dl_drgb, float* d_grad,
+                                float* d_grad_albedo);
 
 /* ------------------------------------------------------------- checks --- */
 /* Device copies of the path's scalar helpers, for bit-exactness tests:
  * kind 0: flg_logf(x); 1: sin; 2: cos; 3: Halton get1D with x = (index, dim)
- * packed as two uint32 in the input. */
+ * packed as two uint32 in the input; 4: the ratio-tracking walks' draw
+ * (1 - u) with x = (path, walk, n) packed as three uint32. */
 FLG_API int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n);
 
 #ifdef __cplusplus

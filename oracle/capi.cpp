@@ -75,6 +75,10 @@ void orc_logf(const float* x, float* y, int64_t n) {
 void orc_sincosf(const float* x, float* s, float* c, int64_t n) {
   for (int64_t i = 0; i < n; ++i) flg_sincosf(x[i], &s[i], &c[i]);
 }
+// draws of the ratio-tracking walks: x = (path, walk, n) triples
+void orc_walk_draw(const uint32_t* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = walk_draw(walk_key(x[3 * i], x[3 * i + 1]), x[3 * i + 2]);
+}
 void orc_pcg32(uint64_t initstate, uint64_t initseq, int64_t n, uint32_t* out) {
   Pcg32 r;
   r.seed(initstate, initseq);

@@ -96,6 +96,25 @@ static inline float flg_logf(float x) {
   return s * (hfsq + R) + dk * ln2_lo - hfsq + f + dk * ln2_hi;
 }
 
+// Counter-based draws of the differentiable estimator's ratio-tracking walks
+// (no reference counterpart).  lowbias32: C. Wellons' integer hash
+// (hash-prospector), x ^= x>>16, x *= 0x7feb352d, x ^= x>>15,
+// x *= 0x846ca68b, x ^= x>>16.  Walk w of path p has key
+// lowbias32(lowbias32(p) + w); its draw n is 1 - u = ((lowbias32(key + n) >> 8)
+// + 1) * 2^-24, an exact float in (0, 1].
+static inline uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+static inline uint32_t walk_key(uint32_t path, uint32_t walk) { return lowbias32(lowbias32(path) + walk); }
+static inline float walk_draw(uint32_t key, uint32_t n) {
+  return (float)((lowbias32(key + n) >> 8) + 1u) * 0x1p-24f;
+}
+
 // flg_sincosf: sine and cosine of a float, FreeBSD msun / musl method:
 // double-precision reduction by pi/2 (two-part constant) and the __sindf /
 // __cosdf minimax kernels on [-pi/4, pi/4], rounded once to float.

@@ -6,8 +6,9 @@
 // src/integrator.cpp:328-405).
 //
 // Estimator ("differential delta tracking", Nimier-David et al. 2022, with
-// path replay, Vicini et al. 2021): the same path structure and Halton
-// dimensions as SVolIntegrator::Li, with
+// path replay, Vicini et al. 2021): the same path structure as
+// SVolIntegrator::Li, the main path drawing the Halton dimensions in its
+// order, with
 //  * a fixed majorant mu (caller-given);
 //  * collision decisions taken on a proposal grid theta_p: real iff
 //    u > 1 - s rho_p / mu (float, as the GPU kernel), weight
@@ -15,7 +16,9 @@
 //    (1 - s rho / mu) / (1 - s rho_p / mu) on a null one, so the estimator is
 //    a smooth function of the evaluated grid theta (no branch depends on it);
 //  * ratio tracking without Russian roulette on the camera and shadow rays,
-//    factor 1 - s rho / mu per tentative collision (no clamp at 0);
+//    factor 1 - s rho / mu per tentative collision (no clamp at 0), each walk
+//    drawing from its own counter-based stream (walk_draw; walk 0 is the
+//    camera ray, walk b the shadow ray of bounce b), walked to its end here;
 //  * the path-level roulette after 10 bounces kept (it depends on u only).
 // Positions, free-flight distances and decisions are computed in float
 // exactly like the GPU kernel; densities of theta, weights, radiance and
@@ -34,7 +37,7 @@ constexpr int kMaxTrackSteps = 4096;
 
 // Steps a walk may take over [t, t_max]: 8 times the expected count
 // (mu * length) plus 64, at most kMaxTrackSteps (the estimator's rule for
-// the minute free flights of the first Halton indices).
+// the minute free flights of the first Halton indices; it bounds every walk).
 int walk_cap(float mu, float t, float t_max) {
   const float e = 8.0f * mu * (t_max - t) + 64.0f;
   return e < (float)kMaxTrackSteps ? (int)e : kMaxTrackSteps;
@@ -79,8 +82,9 @@ Tri lookup2(const DiffScene& sc, V3 pos) {
   return t;
 }
 
-// Ratio tracking without roulette; records the lookups and returns T.
-double ratio_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, float tmax, std::vector<Tri>* rec) {
+// Ratio tracking without roulette on the stream of walk `key`; records the
+// lookups and returns T.
+double ratio_track(const DiffScene& sc, uint32_t key, V3 o, V3 d, float tmax, std::vector<Tri>* rec) {
   float t_min, t_max;
   if (!box_intersect_pbrt(sc.gp.bmin, sc.gp.bmax, o, d, tmax, &t_min, &t_max)) return 1.0;
   float t = smax(t_min, 0.0f);
@@ -89,9 +93,7 @@ double ratio_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, float tmax, std:
   double T = 1.0;
   const int cap = walk_cap(sc.mu_f, t, t_max);
   for (int step = 0; step < cap; ++step) {
-    const float u = hs.get1D();
-    if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
-    t += -flg_logf(1.0f - u) / sc.mu_f;
+    t += -flg_logf(walk_draw(key, (uint32_t)step)) / sc.mu_f;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     Tri q = lookup2(sc, pos);
@@ -185,8 +187,8 @@ NeeSample nee_sample(const DiffScene& sc, Halton& hs, V3 p, V3 wo) {
 // Replays one path.  grad == nullptr: forward only (returns L).  Otherwise
 // accumulates delta * dL/dtheta into grad and delta * dL/dalpha into *galpha,
 // given the path's total L_total.
-double diff_path(const DiffScene& sc, Halton& hs, V3 o, V3 d, double delta, double L_total,
-                 double* grad, double* galpha, uint64_t* lookups) {
+double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, double delta,
+                 double L_total, double* grad, double* galpha, uint64_t* lookups) {
   double L = 0.0;
   const double Le = sc.env.present ? (double)sc.env.radiance : 0.0;
   float t_min, t_max;
@@ -210,7 +212,7 @@ double diff_path(const DiffScene& sc, Halton& hs, V3 o, V3 d, double delta, doub
     for (size_t i = n; i-- > 0;) suf[i] = suf[i + 1] * f[i];
     for (size_t i = 0; i < n; ++i) deposit(r[i], scale * (-sc.s / sc.mu) * pre[i] * suf[i + 1]);
   };
-  double T = ratio_track(sc, hs, o, d, INFINITY, grad ? &rec : nullptr);
+  double T = ratio_track(sc, walk_key(path, 0), o, d, INFINITY, grad ? &rec : nullptr);
   if (lookups) *lookups += rec.size();
   if (grad) deposit_tr(rec, delta * Le);
   L += T * Le;
@@ -247,7 +249,8 @@ double diff_path(const DiffScene& sc, Halton& hs, V3 o, V3 d, double delta, doub
       float nrm = norm(diff);
       V3 dd = diff / nrm;
       rec.clear();
-      double Tsh = ratio_track(sc, hs, pos, dd, nrm - SHADOW_EPS, grad ? &rec : nullptr);
+      double Tsh = ratio_track(sc, walk_key(path, (uint32_t)bounces), pos, dd, nrm - SHADOW_EPS,
+                               grad ? &rec : nullptr);
       if (lookups) *lookups += rec.size();
       if (grad) deposit_tr(rec, delta * beta * ns.K);
       L += beta * ns.K * Tsh;
@@ -332,9 +335,10 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
           cam.generate((float)x + ux, (float)y + uy, W, H, &o, &d);
           int64_t p = pix * spp + s;
           if (adj) {
-            diff_path(sc, hs, o, d, dldpix[pix] / spp, lpath[p], gpart[t].data(), &apart[t], &lk[t]);
+            diff_path(sc, hs, (uint32_t)p, o, d, dldpix[pix] / spp, lpath[p], gpart[t].data(), &apart[t],
+                      &lk[t]);
           } else {
-            double L = diff_path(sc, hs, o, d, 0.0, 0.0, nullptr, nullptr, &lk[t]);
+            double L = diff_path(sc, hs, (uint32_t)p, o, d, 0.0, 0.0, nullptr, nullptr, &lk[t]);
             if (lpath) lpath[p] = L;
             acc += L;
           }

@@ -18,6 +18,7 @@ f32 = ctypes.c_float
 _SIGS = {
     "orc_logf": (None, [P, P, i64]),
     "orc_sincosf": (None, [P, P, P, i64]),
+    "orc_walk_draw": (None, [P, P, i64]),
     "orc_pcg32": (None, [u64, u64, i64, P]),
     "orc_gen_soup": (None, [u64, u64, i32, P, P]),
     "orc_left_shift3": (u32, [u32]),
@@ -211,6 +212,14 @@ def sincosf(x):
     c = np.zeros_like(x)
     lib().orc_sincosf(_p(x), _p(s), _p(c), x.size)
     return s, c
+
+
+def walk_draw(triples):
+    """1 - u of the ratio-tracking walks' stream for (path, walk, n) rows."""
+    x = np.ascontiguousarray(triples, np.uint32).reshape(-1, 3)
+    y = np.zeros(x.shape[0], np.float32)
+    lib().orc_walk_draw(_p(x), _p(y), x.shape[0])
+    return y
 
 
 def halton_get(resx, resy, spp, px, py, sample, n):

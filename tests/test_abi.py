@@ -51,3 +51,22 @@ def test_status_strings_without_gpu():
     assert lib.flg_status_string(0).decode()
     for st in range(1, 6):
         assert lib.flg_status_string(st).decode()
+
+
+def test_diff_tape_bytes_without_gpu():
+    """flg_volpath_diff_tape_bytes: width*height*spp * 4 * (max_depth + 3);
+    -1 beyond 63 bounces or for an invalid configuration."""
+    lib = N.hip()
+
+    def tb(w, h, spp, depth):
+        c = N.RenderCfg()
+        c.width, c.height, c.spp, c.max_depth = w, h, spp, depth
+        return lib.flg_volpath_diff_tape_bytes(ctypes.byref(c))
+
+    assert tb(256, 256, 16, 32) == 256 * 256 * 16 * 4 * 35
+    assert tb(3, 5, 7, 1) == 3 * 5 * 7 * 4 * 4
+    assert tb(8, 8, 1, 63) == 64 * 4 * 66
+    assert tb(8, 8, 1, 64) == -1
+    assert tb(0, 8, 1, 8) == -1 and tb(8, 8, 1, 0) == -1
+    assert tb(1 << 16, 1 << 15, 1, 8) == -1  # 2^31 paths
+    assert lib.flg_volpath_diff_tape_bytes(None) == -1

@@ -70,10 +70,12 @@ def _adjoint_case(ctx, n, W, H, spp, depth, seed):
     grid = _grid(n, seed)
     mu = S * float(grid.max())
     dv, scene, img, lpath = _setup(ctx, grid, W, H, spp, depth, mu)
+    tape = diff.new_tape(ctx, W, H, spp, depth)
+    diff.forward(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lpath=lpath, tape=tape)
     rng = np.random.default_rng(seed)
     dl = rng.normal(size=(H, W, 3)).astype(np.float32)
     grad, galpha = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lpath,
-                                torch.from_numpy(dl).cuda())
+                                torch.from_numpy(dl).cuda(), tape=tape)
     lp64, _ = orc.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0,
                               W, H, spp, depth)
     g64, ga64 = orc.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0,
@@ -194,3 +196,27 @@ def test_diff_schedules_agree(ctx, schedule):
     g, ga = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lp, dl, schedule=schedule)
     rel = float((g - g_ref).norm() / g_ref.norm())
     assert rel < 1e-5 and abs(float(ga) - float(ga_ref)) <= 1e-5 * abs(float(ga_ref))
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_walk_tape_matches_replay(ctx, schedule):
+    """A forward that records the walk tape renders the same image and path
+    values bit for bit, and the adjoint that reads the tape gives the gradient
+    of the one that walks every ratio track twice (up to the float-atomic
+    summation order).  Covers walks with zero factors: theta at the majorant."""
+    W = H = 40
+    spp, depth = 4, 32
+    for grid in (_grid(20, 17), np.full((20, 20, 20), 0.5, np.float32)):
+        mu = S * float(grid.max())
+        dv, scene, img0, lp0 = _setup(ctx, grid, W, H, spp, depth, mu)
+        tape = diff.new_tape(ctx, W, H, spp, depth)
+        assert tape.numel() == W * H * spp * 4 * (depth + 3)
+        img, lp = diff.forward(dv, scene, W, H, spp, depth, S, ALBEDO, mu, schedule=schedule, tape=tape)
+        assert torch.equal(img.view(torch.int32), img0.view(torch.int32))
+        assert torch.equal(lp.view(torch.int32), lp0.view(torch.int32))
+        dl = torch.from_numpy(np.random.default_rng(3).normal(size=(H, W, 3)).astype(np.float32)).cuda()
+        g_ref, ga_ref = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lp0, dl, schedule=schedule)
+        g, ga = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lp, dl, schedule=schedule, tape=tape)
+        assert float(g_ref.abs().max()) > 0
+        rel = float((g - g_ref).norm() / g_ref.norm())
+        assert rel < 1e-5 and abs(float(ga) - float(ga_ref)) <= 1e-5 * abs(float(ga_ref)), rel

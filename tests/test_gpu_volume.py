@@ -62,6 +62,19 @@ def test_halton_bitexact(ctx):
     assert np.array_equal(g.view(np.uint32), exp.view(np.uint32))
 
 
+def test_walk_draw_bitexact(ctx):
+    """The differentiable estimator's ratio-tracking stream (path, walk, n)
+    against the oracle's restatement, over the full counter ranges."""
+    rng = np.random.default_rng(4)
+    n = 300000
+    x = np.stack([rng.integers(0, 2 ** 31, n), rng.integers(0, 64, n), rng.integers(0, 4096, n)],
+                 axis=1).astype(np.uint32)
+    x[:4096] = np.stack([np.zeros(4096), np.zeros(4096), np.arange(4096)], axis=1).astype(np.uint32)
+    x[4096:8192, 0] = 0xFFFFFFFF
+    g = _debug_eval(ctx, 4, x.ravel(), n=n)
+    assert np.array_equal(g.view(np.uint32), orc.walk_draw(x).view(np.uint32))
+
+
 def _scene(ctx, grid, spp, W, H, max_depth, point=True, env=True, sigma=(6.4, 1.6), g=0.3):
     vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], sigma[0], sigma[1], g)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])

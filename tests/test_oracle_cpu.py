@@ -502,3 +502,36 @@ def test_volume_point_light_single_scatter_scale():
                                  CAM0, 16, 16, 4, 8)
     assert a.max() > 0
     assert np.allclose(b, 2 * a, rtol=1e-6, atol=0)
+
+
+def test_walk_stream_uniform_and_independent():
+    """The ratio-tracking walks' counter-based stream: every draw an exact
+    multiple of 2^-24 in (0, 1]; uniform (mean, chi-square over 256 bins),
+    free flights -log(1 - u) with the exponential's mean and variance 1, and no
+    correlation between consecutive draws of a walk, between walks of a path,
+    or between neighbouring paths."""
+    npath, nwalk, nstep = 4096, 4, 64
+    p, w, n = np.meshgrid(np.arange(npath), np.arange(nwalk), np.arange(nstep), indexing="ij")
+    x = np.stack([p.ravel(), w.ravel(), n.ravel()], axis=1).astype(np.uint32)
+    c = orc.walk_draw(x).astype(np.float64)
+    N = c.size
+    assert c.min() > 0.0 and c.max() <= 1.0
+    k = c * 2.0 ** 24
+    assert np.array_equal(k, np.round(k)) and k.min() >= 1 and k.max() <= 2 ** 24
+    u = 1.0 - c
+    assert abs(u.mean() - 0.5) < 5 * math.sqrt(1 / 12 / N)
+    hist = np.bincount(np.minimum((u * 256).astype(int), 255), minlength=256)
+    e = N / 256
+    chi2 = ((hist - e) ** 2 / e).sum()
+    assert chi2 < 255 + 6 * math.sqrt(2 * 255)
+    ff = -np.log(c)
+    assert abs(ff.mean() - 1.0) < 5 / math.sqrt(N) and abs(ff.var() - 1.0) < 0.02
+    cube = u.reshape(npath, nwalk, nstep) - 0.5
+    tol = 5 / math.sqrt(N)
+    for a, b in [(cube[:, :, 1:], cube[:, :, :-1]),      # consecutive steps
+                 (cube[:, 1:, :], cube[:, :-1, :]),      # walks of one path
+                 (cube[1:, :, :], cube[:-1, :, :])]:     # neighbouring paths
+        r = (a * b).mean() / (1 / 12)
+        assert abs(r) < tol, r
+    # distinct (path, walk) streams do not repeat one another
+    assert len(np.unique(c.reshape(npath * nwalk, nstep)[:, :8], axis=0)) == npath * nwalk
