@@ -226,10 +226,13 @@ class Volume:
         density = density.contiguous()
         self.ctx.check(self.ctx.lib.flg_volume_update(self.ctx.h, self.h, _ptr(density)))
 
-    def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0):
+    def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0):
+        """RGB image (height, width, 3).  film=1: the reference's render loop
+        and Mitchell-filtered film instead of the per-pixel sample mean."""
         cfg = N.RenderCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
         cfg.schedule = schedule
+        cfg.film = film
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
         st = (ctypes.c_uint64 * 2)()

@@ -483,7 +483,7 @@ GPUVolIntegrator::GPUVolIntegrator(int max_depth, const GPUVolLights& lights)
 
 void GPUVolIntegrator::render(const GridVolume& volume, const Vector3f& origin, const Vector3f& target,
                               const Vector3f& up, Float fov, int width, int height, int spp,
-                              std::vector<float>& rgb) {
+                              std::vector<float>& rgb, int film) {
   if (width <= 0 || height <= 0 || spp <= 0)
     throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator::render: bad resolution or spp");
   gpu::Context& ctx = volume.context();
@@ -506,6 +506,7 @@ void GPUVolIntegrator::render(const GridVolume& volume, const Vector3f& origin, 
   cfg.height = height;
   cfg.spp = spp;
   cfg.max_depth = m_maxDepth;
+  cfg.film = film;
   const size_t n = size_t(width) * height * 3;
   gpu::DeviceBuffer d(ctx, n * sizeof(float));
   ctx.check(flg_volpath_render(ctx.get(), volume.handle(), &sc, &cfg, static_cast<float*>(d.data()), m_stats),
@@ -520,7 +521,7 @@ void GPUVolIntegrator::render(const Scene& scene) {
   if (!scene.m_film) throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator: scene has no film");
   std::vector<float> rgb;
   render(*vol, scene.m_origin, scene.m_target, scene.m_up, scene.m_FoV, scene.m_resX, scene.m_resY,
-         scene.m_SPP, rgb);
+         scene.m_SPP, rgb, /*film=*/1);
   for (int y = 0; y < scene.m_resY; ++y)
     for (int x = 0; x < scene.m_resX; ++x) {
       const float* c = &rgb[(size_t(y) * scene.m_resX + x) * 3];

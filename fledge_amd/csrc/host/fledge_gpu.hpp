@@ -260,11 +260,16 @@ class GPUVolIntegrator : public Integrator {
   GPUVolIntegrator(int max_depth, const GPUVolLights& lights);
   ~GPUVolIntegrator() override = default;
 
+  // The reference's render loop and film (ParallelIntegrator::render,
+  // integrator.cpp:123-235): centre path, samples 1..SPP-1 splatted through the
+  // film's Mitchell filter, commitSamples; writes the film's colour buffer.
   void render(const Scene& scene) override;
   // Direct form: rgb is height*width*3 floats, row-major from pixel (0, 0).
+  // film 0: per-pixel mean of spp samples on the per-sample Halton schedule;
+  // film 1: the reference's render loop and film, as render(const Scene&).
   void render(const GridVolume& volume, const Vector3f& origin, const Vector3f& target,
               const Vector3f& up, Float fov, int width, int height, int spp,
-              std::vector<float>& rgb);
+              std::vector<float>& rgb, int film = 0);
   uint64_t lastLookups() const { return m_stats[1]; }
 
  private:

@@ -4,7 +4,9 @@
 // HG phase sampling, a constant environment light (light.cpp:93-159) and a
 // point light.  One thread per path sample (pixel, s); the per-pixel value is
 // the in-order float sum over s divided by spp, as in eval_pixel
-// (integrator.cpp:136-159) under the per-sample Halton schedule.
+// (integrator.cpp:136-159) under the per-sample Halton schedule, or the
+// reference's render loop resolved through its film (Film::addSample /
+// commitSamples, film.hpp:143-174).
 #include <algorithm>
 #include <cmath>
 
@@ -148,22 +150,22 @@ struct SvolPath {
   V3 o, d;
   float Lr, beta;
   int bounces;
+  // film mode: the pixel, the film position of the splatted sample, and
+  // whether the path in flight is the pixel-centre path (not splatted)
+  uint32_t pix;
+  float fx, fy;
+  bool centre;
 };
 
-// Starts path p (pixel p / spp, sample p % spp).  Returns true when the path
-// is already complete (its camera ray misses the volume).
+// Starts a path at film position (fx, fy) with the sampler state in s.ps:
+// camera ray, entry into the box, primary transmittance.  Returns true when
+// the path is already complete (its camera ray misses the volume).
 template <bool COUNT>
-FLG_D bool svol_begin(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                      const HaltonParams& hp, const CamDevV& cam, int32_t spp, uint32_t p,
-                      SvolPath& s, uint32_t& nlook) {
-  const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
-  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
-  s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
-  s.ps.dim = 0;
-  const float ux = s.ps.next(tab, hp);
-  const float uy = s.ps.next(tab, hp);
+FLG_D bool svol_start(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, float fx, float fy, SvolPath& s,
+                      uint32_t& nlook) {
   V3 o, d;
-  gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
+  gen_ray_v(cam, fx, fy, &o, &d);
   s.Lr = 0.0f;
   s.beta = 1.0f;
   float t_min, t_max;
@@ -178,6 +180,61 @@ FLG_D bool svol_begin(const VolDev& v, const LightsDev& L, const HaltonTables& t
   s.d = d;
   s.bounces = 1;
   return false;
+}
+
+// Starts path p (pixel p / spp, sample p % spp) of the per-sample schedule:
+// Halton index offset(pixel) + s * stride, dimensions 0 and 1 for the film
+// position, the estimator from dimension 2.
+template <bool COUNT>
+FLG_D bool svol_begin(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, int32_t spp, uint32_t p,
+                      SvolPath& s, uint32_t& nlook) {
+  const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+  s.ps.dim = 0;
+  const float ux = s.ps.next(tab, hp);
+  const float uy = s.ps.next(tab, hp);
+  return svol_start<COUNT>(v, L, tab, hp, cam, (float)x + ux, (float)y + uy, s, nlook);
+}
+
+// The reference's render loop (eval_pixel, integrator.cpp:136-159, with
+// HaltonSampler::setPixel / getPixelSample / reset, sampler.h:164-186).  Per
+// pixel: a path from the pixel centre (x + 0.5, y + 0.5) on Halton index
+// offset + 0 * stride from dimension 0, whose radiance is discarded; sample 1,
+// whose film position takes the next two dimensions of the same index and
+// whose path continues from there; sample s >= 2 on index offset +
+// (s - 1) * stride from dimension 0, which is the per-sample schedule's sample
+// s - 1.  Task t of a pixel (spp - 1 tasks) is sample t + 1; task 0 runs the
+// centre path first.
+template <bool COUNT>
+FLG_D bool film_sample1(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                        const HaltonParams& hp, const CamDevV& cam, SvolPath& s, uint32_t& nlook) {
+  const int x = (int)(s.pix % (uint32_t)cam.w), y = (int)(s.pix / (uint32_t)cam.w);
+  s.centre = false;
+  s.fx = (float)x + s.ps.next(tab, hp);
+  s.fy = (float)y + s.ps.next(tab, hp);
+  return svol_start<COUNT>(v, L, tab, hp, cam, s.fx, s.fy, s, nlook);
+}
+
+template <bool COUNT>
+FLG_D bool film_begin(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, int32_t tasks, uint32_t p,
+                      SvolPath& s, uint32_t& nlook) {
+  const uint32_t pix = p / (uint32_t)tasks, t = p - pix * (uint32_t)tasks;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.pix = pix;
+  s.ps.index = halton_pixel_offset(hp, x, y) + t * hp.stride;
+  s.ps.dim = 0;
+  if (t == 0) {
+    s.centre = true;
+    if (!svol_start<COUNT>(v, L, tab, hp, cam, (float)x + 0.5f, (float)y + 0.5f, s, nlook)) return false;
+    return film_sample1<COUNT>(v, L, tab, hp, cam, s, nlook);
+  }
+  s.centre = false;
+  s.fx = (float)x + s.ps.next(tab, hp);
+  s.fy = (float)y + s.ps.next(tab, hp);
+  return svol_start<COUNT>(v, L, tab, hp, cam, s.fx, s.fy, s, nlook);
 }
 
 // One scattering event of the path: delta tracking to the next collision, HG
@@ -209,6 +266,16 @@ FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& ta
   return false;
 }
 
+// svol_step for film tasks: the end of the centre path starts sample 1.
+template <bool COUNT>
+FLG_D bool film_step(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                     const HaltonParams& hp, const CamDevV& cam, int max_depth, SvolPath& s,
+                     uint32_t& nlook) {
+  if (!svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook)) return false;
+  if (!s.centre) return true;
+  return film_sample1<COUNT>(v, L, tab, hp, cam, s, nlook);
+}
+
 template <bool COUNT>
 FLG_D void count_lookups(uint32_t nlook, unsigned long long* nlookups) {
   if (COUNT) {
@@ -218,22 +285,32 @@ FLG_D void count_lookups(uint32_t nlook, unsigned long long* nlookups) {
   }
 }
 
-// One lane per path sample.
-template <bool COUNT>
+// Per-sample outputs: radiance, and in film mode the film position.
+template <bool FILM>
+FLG_D void svol_finish(uint32_t p, const SvolPath& s, float* lpath, float2* fpos) {
+  lpath[p] = s.Lr;
+  if (FILM) fpos[p] = make_float2(s.fx, s.fy);
+}
+
+// One lane per path sample (FILM: per film task; `tasks` per pixel).
+template <bool COUNT, bool FILM>
 __global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTables tab_g,
-                                              HaltonParams hp, CamDevV cam, int32_t spp,
-                                              int32_t max_depth, float* lpath,
+                                              HaltonParams hp, CamDevV cam, int32_t tasks,
+                                              int32_t max_depth, float* lpath, float2* fpos,
                                               unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  const int64_t npaths = (int64_t)cam.w * cam.h * tasks;
   uint32_t nlook = 0;
   if (p < npaths) {
     SvolPath s;
-    bool done = svol_begin<COUNT>(v, L, tab, hp, cam, spp, (uint32_t)p, s, nlook);
-    while (!done) done = svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook);
-    lpath[p] = s.Lr;
+    bool done = FILM ? film_begin<COUNT>(v, L, tab, hp, cam, tasks, (uint32_t)p, s, nlook)
+                     : svol_begin<COUNT>(v, L, tab, hp, cam, tasks, (uint32_t)p, s, nlook);
+    while (!done)
+      done = FILM ? film_step<COUNT>(v, L, tab, hp, cam, max_depth, s, nlook)
+                  : svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook);
+    svol_finish<FILM>((uint32_t)p, s, lpath, fpos);
   }
   count_lookups<COUNT>(nlook, nlookups);
 }
@@ -242,21 +319,27 @@ __global__ __launch_bounds__(256) void k_svol(VolDev v, LightsDev L, HaltonTable
 // the next unstarted path (wave-aggregated fetch from a global counter), so
 // short and long paths no longer idle each other's wave.  Every path runs
 // the same per-path code as k_svol: results are identical.
-template <bool COUNT>
+template <bool COUNT, bool FILM>
 __global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, HaltonTables tab_g,
-                                                    HaltonParams hp, CamDevV cam, int32_t spp,
-                                                    int32_t max_depth, float* lpath,
+                                                    HaltonParams hp, CamDevV cam, int32_t tasks,
+                                                    int32_t max_depth, float* lpath, float2* fpos,
                                                     unsigned int* next_path,
                                                     unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
-  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
+  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * tasks);
   uint32_t nlook = 0;
   regen_loop<SvolPath>(
       npaths, next_path,
-      [&](uint32_t p, SvolPath& s) { return svol_begin<COUNT>(v, L, tab, hp, cam, spp, p, s, nlook); },
-      [&](SvolPath& s) { return svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook); },
-      [&](uint32_t p, const SvolPath& s) { lpath[p] = s.Lr; });
+      [&](uint32_t p, SvolPath& s) {
+        return FILM ? film_begin<COUNT>(v, L, tab, hp, cam, tasks, p, s, nlook)
+                    : svol_begin<COUNT>(v, L, tab, hp, cam, tasks, p, s, nlook);
+      },
+      [&](SvolPath& s) {
+        return FILM ? film_step<COUNT>(v, L, tab, hp, cam, max_depth, s, nlook)
+                    : svol_step<COUNT>(v, L, tab, hp, max_depth, s, nlook);
+      },
+      [&](uint32_t p, const SvolPath& s) { svol_finish<FILM>(p, s, lpath, fpos); });
   count_lookups<COUNT>(nlook, nlookups);
 }
 
@@ -271,6 +354,57 @@ __global__ __launch_bounds__(256) void k_pixel_mean(const float* lpath, int64_t 
   rgb[3 * pix] = c;
   rgb[3 * pix + 1] = c;
   rgb[3 * pix + 2] = c;
+}
+
+// MitchellFilter::mitchell1D (common/filter.h:44-53) of the film's filter
+// (film.hpp:55: radius 2, B = C = 1 / 3 in integer arithmetic, i.e. 0), for
+// an offset d in pixels: x = |2 (d / 2)|; 0 for x > 1, else
+// (12 x^3 - 18 x^2 + 6) / 6 in the reference's operation order.
+FLG_D float film_mitchell1d(float d) {
+  const float B = 0.0f, C = 0.0f;
+  const float x = fabsf(2.0f * (d * 0.5f));
+  if (x > 1.0f)
+    return ((-B - 6.0f * C) * x * x * x + (6.0f * B + 30.0f * C) * x * x + (-12.0f * B - 48.0f * C) * x +
+            (8.0f * B + 24.0f * C)) *
+           (1.0f / 6.0f);
+  return ((12.0f - 9.0f * B - 6.0f * C) * x * x * x + (-18.0f + 12.0f * B + 6.0f * C) * x * x +
+          (6.0f - 2.0f * B)) *
+         (1.0f / 6.0f);
+}
+
+// Film::addSample + commitSamples (film.hpp:143-174) as a gather: pixel (x, y)
+// sums w * L and w over the samples of the 3x3 pixels around it (a sample of
+// pixel q lies in [q, q + 1)^2, so only those reach |offset| < 1 where the
+// filter is non-zero), in the order (qx, qy, sample); w = filter(sample - 0.5
+// - pixel).  Pixels with sum(w) == 0 stay 0.
+__global__ __launch_bounds__(256) void k_film_resolve(const float* lpath, const float2* fpos, int32_t w,
+                                                      int32_t h, int32_t tasks, float* rgb) {
+  const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= (int64_t)w * h) return;
+  const int x = (int)(pix % w), y = (int)(pix / w);
+  float I = 0.0f, Ls = 0.0f;
+  for (int qx = x - 1; qx <= x + 1; ++qx) {
+    if (qx < 0 || qx >= w) continue;
+    for (int qy = y - 1; qy <= y + 1; ++qy) {
+      if (qy < 0 || qy >= h) continue;
+      const int64_t base = ((int64_t)qy * w + qx) * tasks;
+      for (int t = 0; t < tasks; ++t) {
+        const float2 uv = fpos[base + t];
+        const float wgt = film_mitchell1d((uv.x - 0.5f) - (float)x) * film_mitchell1d((uv.y - 0.5f) - (float)y);
+        I = I + wgt;
+        Ls = Ls + lpath[base + t] * wgt;
+      }
+    }
+  }
+  const float c = I == 0.0f ? 0.0f : Ls / I;
+  rgb[3 * pix] = c;
+  rgb[3 * pix + 1] = c;
+  rgb[3 * pix + 2] = c;
+}
+
+__global__ __launch_bounds__(256) void k_fill(float* p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -387,56 +521,73 @@ int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
   if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: too many paths");
+  if (cfg->film < 0 || cfg->film > 1)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: film must be 0 or 1");
+  const int sched = cfg->schedule ? cfg->schedule : 2;
+  if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
+  const bool film = cfg->film == 1;
   const int64_t npix = (int64_t)cfg->width * cfg->height;
-  const int64_t npaths = npix * cfg->spp;
-  HaltonTables tab;
-  int st = get_halton_tables(ctx, &tab);
-  if (st) return st;
-  LightsDev L;
-  if ((st = make_lights(ctx, vol, scene, &L))) return st;
-  VolDev v = make_voldev(vol);
-  CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
-  char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * 4 + 256, &st);
+  // film mode: spp - 1 splatted samples per pixel (the first task also runs
+  // the pixel-centre path); with spp == 1 nothing reaches the film
+  const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
+  const int64_t npaths = npix * tasks;
+  int st = FLG_OK;
+  // [256 B counters][npaths radiance floats][pad to 8 B][npaths film positions]
+  char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * (film ? 12 : 4) + 264, &st);
   if (!sc) return st;
-  float* lpath = (float*)(sc + 256);
   unsigned long long* nl = (unsigned long long*)sc;
-  const bool count = h_stats != nullptr;
   unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
-  const int sched = cfg->schedule ? cfg->schedule : 2;
-  if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
-  {
+  float* lpath = (float*)(sc + 256);
+  float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
+  const bool count = h_stats != nullptr;
+  k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
+  if (npaths > 0) {
+    HaltonTables tab;
+    if ((st = get_halton_tables(ctx, &tab))) return st;
+    LightsDev L;
+    if ((st = make_lights(ctx, vol, scene, &L))) return st;
+    VolDev v = make_voldev(vol);
+    CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
     ScopedTimer tm(ctx, "volpath_paths");
-    k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
-    if (sched == 1) {
-      if (count)
-        k_svol<true><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
-                                                                   cfg->max_depth, lpath, nl);
-      else
-        k_svol<false><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp,
-                                                                    cfg->max_depth, lpath, nl);
+    const int blocks1 = (int)div_up(npaths, 256);
+    const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+#define FLG_SVOL_LAUNCH(C, F)                                                                              \
+  if (sched == 1)                                                                                          \
+    k_svol<C, F><<<blocks1, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, fpos, \
+                                                   nl);                                                    \
+  else                                                                                                     \
+    k_svol_regen<C, F><<<blocks2, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, \
+                                                         fpos, next_path, nl);
+    if (count && film) {
+      FLG_SVOL_LAUNCH(true, true)
+    } else if (count) {
+      FLG_SVOL_LAUNCH(true, false)
+    } else if (film) {
+      FLG_SVOL_LAUNCH(false, true)
     } else {
-      const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
-      if (count)
-        k_svol_regen<true><<<grid, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp, cfg->max_depth,
-                                                          lpath, next_path, nl);
-      else
-        k_svol_regen<false><<<grid, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, cfg->spp, cfg->max_depth,
-                                                           lpath, next_path, nl);
+      FLG_SVOL_LAUNCH(false, false)
     }
+#undef FLG_SVOL_LAUNCH
   }
   {
     ScopedTimer tm(ctx, "volpath_mean");
-    k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+    if (film && npaths > 0)
+      k_film_resolve<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, fpos, cfg->width, cfg->height, tasks,
+                                                                 d_rgb);
+    else if (film)
+      k_fill<<<div_up(npix * 3, 256), 256, 0, ctx->stream>>>(d_rgb, npix * 3, 0.0f);
+    else
+      k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
   }
   FLG_CHECK_LAUNCH(ctx);
   if (h_stats) {
     unsigned long long c = 0;
     FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
     FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    h_stats[0] = (uint64_t)npaths;
+    h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
     h_stats[1] = c;
   }
   return FLG_OK;

@@ -275,11 +275,25 @@ typedef struct flg_render_cfg {
   int32_t width, height, spp, max_depth;
   int32_t schedule; /* 1: one lane per path sample; 2: persistent lanes that
                        take a new path when theirs ends; 0: 2.  Same results. */
-  int32_t reserved[7];
+  int32_t film;     /* flg_volpath_render only.  0: pixel = mean of spp path
+                       samples, sample s on Halton index offset + s*stride
+                       (eval_pixel's per-sample value).  1: the reference's
+                       render loop and film (ParallelIntegrator::render,
+                       integrator.cpp:136-159; Film::addSample/commitSamples,
+                       film.hpp:143-174): a pixel-centre path that is not
+                       splatted, then samples 1..spp-1 (sample 1 continues
+                       the centre path's Halton index and dimensions, sample
+                       s >= 2 uses index s-1) splatted with the Mitchell
+                       filter as constructed there (radius 2, B = C = 0) and
+                       resolved as sum(w L) / sum(w); race-free and
+                       deterministic.  SVolIntegrator's albedo and normal
+                       buffers are zero, so none is produced. */
+  int32_t reserved[6];
 } flg_render_cfg;
 
 /* Renders width*height RGB float pixels (d_rgb, 3 floats per pixel) with the
- * volume-only integrator.  h_stats (may be NULL): {paths, density lookups}. */
+ * volume-only integrator (SVolIntegrator::Li, integrator.cpp:328-405).
+ * h_stats (may be NULL): {path samples traced, density lookups}. */
 FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
 

@@ -240,6 +240,71 @@ void orc_volpath_render(const float* grid, int32_t nx, int32_t ny, int32_t nz, c
   svol_render(sc, rgb, nthreads, lookups, paths);
 }
 
+// The reference's render loop and film (film.cpp); arguments as
+// orc_volpath_render.  task_L / task_uv (may be NULL): per pixel the spp - 1
+// splatted samples (radiance; film position x, y).
+void orc_volpath_render_film(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
+                             const float* medium3, const float* lights7, const float* cam10, int32_t W,
+                             int32_t H, int32_t spp, int32_t max_depth, float* rgb, float* task_L,
+                             float* task_uv, uint64_t* lookups, uint64_t* paths) {
+  VolScene sc;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
+  sc.point.present = lights7[0] != 0.0f;
+  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
+  sc.point.intensity = lights7[4];
+  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  Camera cam = make_camera(cam10);
+  sc.cam = &cam;
+  sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
+  svol_render_film(sc, rgb, task_L, task_uv, lookups, paths);
+}
+
+// Per-sample radiance and film position of the per-sample schedule (sample s
+// of pixel p on Halton index offset(p) + s * stride, dimensions 0 and 1 for
+// the film position): L[W*H*spp], uv[2*W*H*spp], pixel-major.
+void orc_volpath_samples(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
+                         const float* medium3, const float* lights7, const float* cam10, int32_t W, int32_t H,
+                         int32_t spp, int32_t max_depth, float* L, float* uv) {
+  VolScene sc;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
+  sc.point.present = lights7[0] != 0.0f;
+  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
+  sc.point.intensity = lights7[4];
+  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  Camera cam = make_camera(cam10);
+  sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
+  Halton hs((uint64_t)spp, W, H);
+  uint64_t lk = 0;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x)
+      for (int s = 0; s < spp; ++s) {
+        hs.start(x, y, (uint64_t)s);
+        const float ux = (float)x + hs.get1D();
+        const float uy = (float)y + hs.get1D();
+        V3 o, d;
+        cam.generate(ux, uy, W, H, &o, &d);
+        const size_t k = ((size_t)y * W + x) * (size_t)spp + (size_t)s;
+        L[k] = (float)svol_Li(sc, hs, o, d, &lk);
+        uv[2 * k] = ux;
+        uv[2 * k + 1] = uy;
+      }
+}
+
+float orc_mitchell1d(float d) { return film_mitchell1d(d); }
+
+// HaltonSampler driven as the reference's own sampler test does
+// (tests/sampler_tests.cpp:24-45): setPixel(px, py), then per sample `ndraw`
+// get1D() and reset().  out: spp * ndraw floats.
+void orc_halton_stateful(int32_t spp, int32_t resx, int32_t resy, int32_t px, int32_t py, int32_t ndraw,
+                         float* out) {
+  Halton hs((uint64_t)spp, resx, resy);
+  hs.set_pixel(px, py);
+  for (int s = 0; s < spp; ++s) {
+    for (int i = 0; i < ndraw; ++i) out[(size_t)s * ndraw + i] = hs.get1D();
+    hs.reset();
+  }
+}
+
 // ------------------------------------------------------------ path trace ---
 void orc_diffuse_render(void* p, int32_t compat, const float* cam10, int32_t W, int32_t H,
                         int32_t spp, int32_t max_depth, int64_t p0, int64_t p1, int32_t* nseg,

@@ -47,10 +47,13 @@ struct Halton {  // HaltonSampler sampler.h:143-252
   uint64_t offset = 0;
   uint64_t index = 0;
   uint64_t cnt = 0;
+  uint64_t psi = 0;  // m_pixel_sample_index
   bool overflow = false;
   Halton(uint64_t spp_, int resx, int resy);
   uint64_t offset_for_pixel(int x, int y) const;
   void start(int x, int y, uint64_t sample);  // setPixel + getIndexForSample(sample)
+  void set_pixel(int x, int y);                // setPixel (sampler.h:164-169)
+  bool reset();                                // reset (sampler.h:175-181)
   float sample_dimension(uint64_t idx, uint64_t dim);
   float get1D();
 };
@@ -132,5 +135,9 @@ float vol_tr_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax);
 float vol_sample_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success,
                      V3* pos);
 void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups, uint64_t* paths);
+// the reference's render loop and film (film.cpp)
+float film_mitchell1d(float d);
+void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups,
+                      uint64_t* paths);
 
 }  // namespace orc

@@ -109,6 +109,19 @@ void Halton::start(int x, int y, uint64_t sample) {
   cnt = 0;
 }
 
+// setPixel: dimension 0 of the pixel's sample 0
+void Halton::set_pixel(int x, int y) {
+  start(x, y, 0);
+  psi = 0;
+}
+
+// reset: dimension 0 of the next sample of the pixel
+bool Halton::reset() {
+  cnt = 0;
+  index = offset + (psi + 1) * stride;
+  return ++psi < spp;
+}
+
 // sampleDimension sampler.h:241-251
 float Halton::sample_dimension(uint64_t idx, uint64_t dim) {
   const auto& pr = prime_list();

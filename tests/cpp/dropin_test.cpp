@@ -303,9 +303,10 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
   sc.height = H;
   sc.spp = spp;
   sc.max_depth = depth;
+  // render(const Scene&) is the reference's render loop and film
   std::vector<float> rgb(size_t(W) * H * 3);
   uint64_t lk = 0, paths = 0;
-  orc::svol_render(sc, rgb.data(), 8, &lk, &paths);
+  orc::svol_render_film(sc, rgb.data(), nullptr, nullptr, &lk, &paths);
   double worst = 0;
   for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) {

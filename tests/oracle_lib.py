@@ -40,6 +40,10 @@ _SIGS = {
                                   ctypes.c_int, P, P]),
     "orc_diffuse_render": (None, [P, i32, P, i32, i32, i32, i32, i64, i64, P, P, P, ctypes.c_int, P]),
     "orc_offset_ray_origin": (None, [P, P, P, P]),
+    "orc_volpath_render_film": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P]),
+    "orc_volpath_samples": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P]),
+    "orc_mitchell1d": (f32, [f32]),
+    "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
 }
 
 _lib = None
@@ -153,6 +157,43 @@ def volpath_render(grid, bounds, medium, lights, cam10, W, H, spp, max_depth, nt
                              _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
                              nthreads or NTHREADS, _p(lk), _p(pa))
     return rgb, int(lk[0]), int(pa[0])
+
+
+def volpath_render_film(grid, bounds, medium, lights, cam10, W, H, spp, max_depth):
+    """The reference's render loop and film: (rgb (H, W, 3), per-task radiance
+    (H*W, spp-1), per-task film position (H*W, spp-1, 2), lookups)."""
+    grid = np.ascontiguousarray(grid, np.float32)
+    nz, ny, nx = grid.shape
+    rgb = np.zeros((H, W, 3), np.float32)
+    T = max(spp - 1, 0)
+    tl = np.zeros((H * W, max(T, 1)), np.float32)
+    tu = np.zeros((H * W, max(T, 1), 2), np.float32)
+    lk = np.zeros(1, np.uint64)
+    pa = np.zeros(1, np.uint64)
+    lib().orc_volpath_render_film(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                                  _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb), _p(tl),
+                                  _p(tu), _p(lk), _p(pa))
+    return rgb, tl[:, :T], tu[:, :T], int(lk[0])
+
+
+def volpath_samples(grid, bounds, medium, lights, cam10, W, H, spp, max_depth):
+    """Per-sample radiance (H*W, spp) and film position (H*W, spp, 2) of the
+    per-sample schedule."""
+    grid = np.ascontiguousarray(grid, np.float32)
+    nz, ny, nx = grid.shape
+    L = np.zeros((H * W, spp), np.float32)
+    uv = np.zeros((H * W, spp, 2), np.float32)
+    lib().orc_volpath_samples(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                              _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                              _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(L), _p(uv))
+    return L, uv
+
+
+def halton_stateful(spp, resx, resy, px, py, ndraw):
+    out = np.zeros((spp, ndraw), np.float32)
+    lib().orc_halton_stateful(spp, resx, resy, px, py, ndraw, _p(out))
+    return out
 
 
 def diff_render(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, bounds=(0, 0, 0, 1, 1, 1),

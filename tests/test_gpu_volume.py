@@ -129,3 +129,40 @@ def test_volpath_schedules_identical(ctx, schedule):
                                     CAM0, 96, 80, 8, 32)
     assert st["lookups"] == lk
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_volpath_film_matches_oracle(ctx, schedule):
+    """film = 1: the reference's render loop (pixel-centre path, samples
+    1..spp-1 with the render loop's Halton schedule) resolved through the
+    Mitchell film, against the oracle's single-threaded restatement: equal up
+    to the film's summation order (the GPU gathers in (qx, qy, sample) order;
+    the oracle splats in the reference's tile order), same lookup count.  The
+    image spans several 16 x 16 tiles and an odd number of tasks."""
+    grid = api.gen_fbm_grid(32, seed=10)
+    W, H, spp = 45, 37, 4
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    sc = api.vol_scene(cam)
+    out, st = vol.render(sc, W, H, spp, 32, stats=True, schedule=schedule, film=1)
+    ref, _, _, lk = orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3],
+                                            [1, 2, 2, -2, 50.0, 1, 0.5], CAM0, W, H, spp, 32)
+    got = out.cpu().numpy()
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
+    assert rel.max() <= 1e-5, rel.max()
+    assert st["lookups"] == lk and st["paths"] == W * H * spp
+    other, _ = vol.render(sc, W, H, spp, 32, stats=True, schedule=3 - schedule, film=1)
+    assert torch.equal(out.view(torch.int32), other.view(torch.int32))
+
+
+def test_volpath_film_constant_radiance_and_one_sample(ctx):
+    """An empty medium under the environment light gives the environment
+    radiance in every film pixel; one sample per pixel leaves the film at 0."""
+    grid = np.zeros((8, 8, 8), np.float32)
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    sc = api.vol_scene(cam)
+    out = vol.render(sc, 33, 17, 6, 8, film=1).cpu().numpy()
+    assert np.abs(out - 0.5).max() <= 2e-7
+    one, st = vol.render(sc, 33, 17, 1, 8, film=1, stats=True)
+    assert torch.count_nonzero(one) == 0 and st["paths"] == 0

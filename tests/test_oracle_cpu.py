@@ -535,3 +535,101 @@ def test_walk_stream_uniform_and_independent():
         assert abs(r) < tol, r
     # distinct (path, walk) streams do not repeat one another
     assert len(np.unique(c.reshape(npath * nwalk, nstep)[:, :8], axis=0)) == npath * nwalk
+
+
+# ------------------------------------------------------------- film (f2) --
+CAMF = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+LIGHTSF = [1.0, 2.0, 2.0, -2.0, 50.0, 1.0, 0.5]
+
+
+def test_halton_stateful_matches_reference_sampler_test():
+    """The reference's own sampler test (sampler_tests.cpp:24-45) driven
+    through setPixel / get1D / reset: its histogram bound holds, and reset()
+    lands on the next sample's index at dimension 0 (the per-sample sequence)."""
+    out = orc.halton_stateful(128, 5, 5, 1, 1, 128)
+    hist = np.bincount(np.minimum((out.ravel() * 10).astype(int), 9), minlength=10) / out.size
+    assert np.abs(hist - 0.1).max() < 1e-2
+    ref = np.zeros(128 * 128, np.float32)
+    L.orc_halton_sequence(5, 5, 128, 1, 1, 128, ref.ctypes.data)
+    assert np.array_equal(out.ravel().view(np.uint32), ref.view(np.uint32))
+
+
+def test_film_filter_is_mitchell_with_b_c_zero():
+    """The film's filter (radius 2, B = C = 1/3 in integer arithmetic = 0):
+    Mitchell-Netravali's cubic at B = C = 0, (12|x|^3 - 18|x|^2 + 6) / 6 on
+    |x| <= 1 and the outer piece, which vanishes at B = C = 0; so the weights
+    of the pixels around a sample form a partition of unity in each axis."""
+    def mn(x, B, C):
+        x = abs(x)
+        if x > 1:
+            return ((-B - 6 * C) * x ** 3 + (6 * B + 30 * C) * x ** 2 + (-12 * B - 48 * C) * x + (8 * B + 24 * C)) / 6
+        return ((12 - 9 * B - 6 * C) * x ** 3 + (-18 + 12 * B + 6 * C) * x ** 2 + (6 - 2 * B)) / 6
+
+    xs = np.linspace(-2.5, 2.5, 2001, dtype=np.float32)
+    got = np.array([L.orc_mitchell1d(float(x)) for x in xs], np.float64)
+    exp = np.array([mn(float(x), 0.0, 0.0) for x in xs])
+    assert np.abs(got - exp).max() < 1e-6
+    assert L.orc_mitchell1d(0.0) == 1.0 and L.orc_mitchell1d(1.0) == 0.0 and L.orc_mitchell1d(-1.5) == 0.0
+    assert abs(L.orc_mitchell1d(0.5) - 0.5) < 1e-7
+    # partition of unity over the integer lattice
+    for f in np.linspace(0.0, 1.0, 101, dtype=np.float32):
+        s = sum(L.orc_mitchell1d(float(f) - k) for k in (-1, 0, 1, 2))
+        assert abs(s - 1.0) < 1e-6
+    # with the intended B = C = 1/3 the weights would differ (the quirk matters)
+    assert abs(mn(0.5, 1 / 3, 1 / 3) - 0.5) > 1e-2
+
+
+def _film_case(grid, W, H, spp, depth=16):
+    return orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTSF, CAMF, W, H, spp, depth)
+
+
+def test_film_schedule_maps_to_per_sample_schedule():
+    """Render-loop schedule (Q10): sample s >= 2 of a pixel is the per-sample
+    schedule's sample s - 1 (same Halton index, dimensions from 0), bit for
+    bit; sample 1 shares index 0 with the pixel-centre path and continues its
+    dimensions, so it is not sample 0 of the per-sample schedule."""
+    grid = (api.gen_fbm_grid(16, seed=3)).astype(np.float32)
+    W, H, spp = 18, 11, 5
+    rgb, tl, tu, lk = _film_case(grid, W, H, spp)
+    cl, cu = orc.volpath_samples(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTSF, CAMF, W, H, spp, 16)
+    assert np.array_equal(tl[:, 1:].view(np.uint32), cl[:, 1:spp - 1].view(np.uint32))
+    assert np.array_equal(tu[:, 1:].view(np.uint32), cu[:, 1:spp - 1].view(np.uint32))
+    assert not np.array_equal(tu[:, 0], cu[:, 0])
+    # film positions lie in their pixel
+    px = np.arange(W * H) % W
+    py = np.arange(W * H) // W
+    assert np.all((tu[..., 0] >= px[:, None]) & (tu[..., 0] < px[:, None] + 1))
+    assert np.all((tu[..., 1] >= py[:, None]) & (tu[..., 1] < py[:, None] + 1))
+
+
+def test_film_resolve_matches_float64_splat():
+    """The film equals sum(w L) / sum(w) over every sample within the filter's
+    support, recomputed in float64 from the per-sample values with the closed
+    form 2|d|^3 - 3|d|^2 + 1, d = sample - 0.5 - pixel (bounds, offset and
+    normalisation of addSample / commitSamples)."""
+    grid = (api.gen_fbm_grid(16, seed=4)).astype(np.float32)
+    W, H, spp = 21, 19, 4
+    rgb, tl, tu, lk = _film_case(grid, W, H, spp)
+    f = lambda d: np.where(np.abs(d) < 1, 2 * np.abs(d) ** 3 - 3 * d * d + 1, 0.0)
+    uv = tu.reshape(-1, 2).astype(np.float64)
+    Ls = tl.reshape(-1).astype(np.float64)
+    out = np.zeros((H, W))
+    for y in range(H):
+        for x in range(W):
+            w = f(uv[:, 0] - 0.5 - x) * f(uv[:, 1] - 0.5 - y)
+            out[y, x] = (w * Ls).sum() / w.sum() if w.sum() > 0 else 0.0
+    got = rgb[:, :, 0].astype(np.float64)
+    assert np.allclose(got, out, rtol=2e-6, atol=1e-7)
+    assert np.array_equal(rgb[:, :, 0], rgb[:, :, 1]) and np.array_equal(rgb[:, :, 0], rgb[:, :, 2])
+
+
+def test_film_constant_radiance_and_single_sample():
+    """An empty medium under the environment light: every path returns the
+    environment radiance, so every film pixel is that radiance; with one sample
+    per pixel nothing is splatted and the film stays 0."""
+    grid = np.zeros((8, 8, 8), np.float32)
+    rgb, tl, tu, lk = _film_case(grid, 16, 12, 6)
+    assert np.all(tl == np.float32(0.5)) and lk == 0
+    assert np.abs(rgb - 0.5).max() <= 2e-7
+    rgb1, tl1, _, _ = _film_case(grid, 16, 12, 1)
+    assert tl1.size == 0 and np.all(rgb1 == 0)
