@@ -124,6 +124,7 @@ _SIGS = {
     "flg_volpath_adjoint": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
                                     c_vp, c_vp, c_vp, c_vp]),
     "flg_debug_eval": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i64]),
+    "flg_film_save_exr": (c_int, [ctypes.c_char_p, c_i32, c_i32, c_vp]),
 }
 
 _SCENE_SIGS = {

@@ -515,6 +515,17 @@ void GPUVolIntegrator::render(const GridVolume& volume, const Vector3f& origin, 
   d.download(rgb.data(), n * sizeof(float));
 }
 
+#ifndef FLEDGE_HAS_REFERENCE_HEADERS
+bool Film::saveBuffer(const std::string& name, EFilmBufferType) const {
+  const size_t dot = name.rfind('.');
+  if (dot == std::string::npos || name.substr(dot) != ".exr") return false;
+  std::vector<float> rgb(size_t(m_resX) * m_resY * 3);
+  for (size_t i = 0; i < m_color.size(); ++i)
+    for (int c = 0; c < 3; ++c) rgb[3 * i + c] = m_color[i][c];
+  return flg_film_save_exr(name.c_str(), m_resX, m_resY, rgb.data()) == FLG_OK;
+}
+#endif
+
 void GPUVolIntegrator::render(const Scene& scene) {
   const GridVolume* vol = dynamic_cast<const GridVolume*>(scene.m_volume);
   if (!vol) throw gpu::Error(FLG_ERR_INVALID, "GPUVolIntegrator: scene.m_volume is not a GridVolume");

@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <limits>
 #include <memory_resource>
+#include <string>
 #include <vector>
 
 namespace fledge {
@@ -135,6 +136,9 @@ struct Film {
   Film(int resX, int resY) : m_resX(resX), m_resY(resY), m_color(size_t(resX) * resY) {}
   int getBufferIdx(int x, int y) const { return y * m_resX + x; }
   Vector3f& getBuffer(int x, int y, EFilmBufferType) { return m_color[getBufferIdx(x, y)]; }
+  // Film::saveBuffer (film.cpp:13-49) for ".exr" names, through
+  // flg_film_save_exr; other formats need OpenImageIO and return false.
+  bool saveBuffer(const std::string& name, EFilmBufferType) const;
   int m_resX, m_resY;
   std::vector<Vector3f> m_color;
 };

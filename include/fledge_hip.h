@@ -339,6 +339,15 @@ FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume*
                                 const void* d_tape, const float* d_dl_drgb, float* d_grad,
                                 float* d_grad_albedo);
 
+/* --------------------------------------------------------------- film --- */
+/* Film::saveBuffer for a ".exr" name (src/film.cpp:13-49): writes h_rgb
+ * (host memory, height rows of width RGB floats, row 0 first) as an OpenEXR
+ * scanline image of FLOAT channels B, G, R, uncompressed, with the last row of
+ * h_rgb as the file's first scanline, as the reference's negative-stride
+ * write.  Returns FLG_ERR_INVALID if the file cannot be written.  Needs no
+ * context or GPU. */
+FLG_API int flg_film_save_exr(const char* path, int32_t width, int32_t height, const float* h_rgb);
+
 /* ------------------------------------------------------------- checks --- */
 /* Device copies of the path's scalar helpers, for bit-exactness tests:
  * kind 0: flg_logf(x); 1: sin; 2: cos; 3: Halton get1D with x = (index, dim)
