@@ -93,10 +93,10 @@ FLG_HD bool pluecker_uvw(V3 v0, V3 v1, V3 v2, V3 D) {
   return smin(U, smin(V, W)) >= -eps || smax(U, smax(V, W)) <= eps;
 }
 
-// Full test: returns true and t / Ng on a hit (t >= 0, den != 0).
-FLG_HD bool pluecker(V3 a, V3 b, V3 c, V3 O, V3 D, bool compat, float* thit, V3* ng) {
+// The part of the test after the edge test: returns true and t / Ng on a hit
+// (t >= 0, den != 0).
+FLG_HD bool pluecker_t(V3 a, V3 b, V3 c, V3 O, V3 D, bool compat, float* thit, V3* ng) {
   const V3 v0 = a - O, v1 = b - O, v2 = c - O;
-  if (!pluecker_uvw(v0, v1, v2, D)) return false;
   const V3 e0 = v2 - v0, e1 = v0 - v1, e2 = v1 - v2;
   const V3 Ng = stable_normalize(stable_triangle_normal(e0, e1, e2), compat);
   const float dd = dot(Ng, D);
@@ -111,30 +111,40 @@ FLG_HD bool pluecker(V3 a, V3 b, V3 c, V3 O, V3 D, bool compat, float* thit, V3*
   return true;
 }
 
+// Full test (PlueckerTriangleIntersect): the edge test, then t and Ng.
+FLG_HD bool pluecker(V3 a, V3 b, V3 c, V3 O, V3 D, bool compat, float* thit, V3* ng) {
+  if (!pluecker_uvw(a - O, b - O, c - O, D)) return false;
+  return pluecker_t(a, b, c, O, D, compat, thit, ng);
+}
+
 // BoundIntersect (intersector.hpp:103-120) with a precomputed 1/d (the
-// reference computes the same IEEE quotient per call).
+// reference computes the same IEEE quotient per call).  The reference returns
+// as soon as t0 > t1 after an axis; t0 only grows and t1 only shrinks from
+// one axis to the next (each update keeps the old value unless the new one
+// compares past it, NaN included), so one test after the three axes gives the
+// same answer and the same t0, without a branch per axis.  t0 and t1 are never
+// NaN, so `tn > t0 ? tn : t0` is fmaxf(t0, tn) (a NaN operand yields the
+// other) up to the sign of a zero result, which no comparison sees; the swap
+// keeps the reference's comparison, under which a NaN bound is not swapped.
 FLG_HD bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, V3 o, V3 inv,
                  float* tnear) {
   float t0 = 0.0f, t1 = 3.402823466e+38f;
   float tn = (lox - o.x) * inv.x, tf = (hix - o.x) * inv.x;
   if (tn > tf) { float t = tn; tn = tf; tf = t; }
-  t0 = tn > t0 ? tn : t0;
-  t1 = tf < t1 ? tf : t1;
-  if (t0 > t1) return false;
+  t0 = fmaxf(t0, tn);
+  t1 = fminf(t1, tf);
   tn = (loy - o.y) * inv.y;
   tf = (hiy - o.y) * inv.y;
   if (tn > tf) { float t = tn; tn = tf; tf = t; }
-  t0 = tn > t0 ? tn : t0;
-  t1 = tf < t1 ? tf : t1;
-  if (t0 > t1) return false;
+  t0 = fmaxf(t0, tn);
+  t1 = fminf(t1, tf);
   tn = (loz - o.z) * inv.z;
   tf = (hiz - o.z) * inv.z;
   if (tn > tf) { float t = tn; tn = tf; tf = t; }
-  t0 = tn > t0 ? tn : t0;
-  t1 = tf < t1 ? tf : t1;
-  if (t0 > t1) return false;
+  t0 = fmaxf(t0, tn);
+  t1 = fminf(t1, tf);
   *tnear = t0;
-  return true;
+  return !(t0 > t1);
 }
 
 }  // namespace flg

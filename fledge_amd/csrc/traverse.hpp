@@ -28,19 +28,36 @@ struct Counters {
   uint32_t nodes, tris;
 };
 
-template <bool COMPAT, bool COUNT>
-FLG_HD void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, int32_t& hit,
-                    Counters& c) {
+// A leaf test in two parts.  leaf_edges is the edge test (it does not depend
+// on the closest hit so far); leaf_finish computes t for a triangle that
+// passed it and applies the closest-hit rule (t <= best replaces).  Running
+// both parts back to back is the reference's test; a node's two leaves can
+// run their edge tests first and finish in order, which gives the same result.
+template <bool COUNT>
+FLG_HD bool leaf_edges(const TraceGeom& g, uint32_t p, V3 o, V3 d, Counters& c) {
   const BTri t = g.tris[p];
   if (COUNT) ++c.tris;
+  return pluecker_uvw(v3(t.v0.x, t.v0.y, t.v0.z) - o, v3(t.v1.x, t.v1.y, t.v1.z) - o,
+                      v3(t.v2.x, t.v2.y, t.v2.z) - o, d);
+}
+
+template <bool COMPAT>
+FLG_HD void leaf_finish(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, int32_t& hit) {
+  const BTri t = g.tris[p];
   float th;
   V3 ng;
-  if (!pluecker(v3(t.v0.x, t.v0.y, t.v0.z), v3(t.v1.x, t.v1.y, t.v1.z), v3(t.v2.x, t.v2.y, t.v2.z),
-                o, d, COMPAT, &th, &ng))
+  if (!pluecker_t(v3(t.v0.x, t.v0.y, t.v0.z), v3(t.v1.x, t.v1.y, t.v1.z), v3(t.v2.x, t.v2.y, t.v2.z), o,
+                  d, COMPAT, &th, &ng))
     return;
   if (th > best) return;
   best = th;
   hit = (int32_t)p;  // sorted position; the caller maps it to the original id
+}
+
+template <bool COMPAT, bool COUNT>
+FLG_HD void test_tri(const TraceGeom& g, uint32_t p, V3 o, V3 d, float& best, int32_t& hit,
+                    Counters& c) {
+  if (leaf_edges<COUNT>(g, p, o, d, c)) leaf_finish<COMPAT>(g, p, o, d, best, hit);
 }
 
 // Traversal stacks of (node, tnear) entries.  The depth of the tree is below
@@ -158,8 +175,14 @@ FLG_HD bool trace_visit(const TraceGeom& g, TraceRay& r, Stack& st, Counters& c)
   const V3 o = r.o, d = r.d, inv = r.inv;
   const BNode nd = g.nodes[r.node];
   const int32_t refL = nd.d.x, refR = nd.d.y;
-  if (refL < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refL & 0x7fffffffu, o, d, r.best, r.hit, c);
-  if (refR < 0) test_tri<COMPAT, COUNT>(g, (uint32_t)refR & 0x7fffffffu, o, d, r.best, r.hit, c);
+  // leaf children, left then right: both edge tests, then t for the
+  // candidates in that order (a wave runs the costly part once unless some
+  // lane has two candidates)
+  const uint32_t pL = (uint32_t)refL & 0x7fffffffu, pR = (uint32_t)refR & 0x7fffffffu;
+  const bool candL = refL < 0 && leaf_edges<COUNT>(g, pL, o, d, c);
+  const bool candR = refR < 0 && leaf_edges<COUNT>(g, pR, o, d, c);
+  if (candL || candR) leaf_finish<COMPAT>(g, candL ? pL : pR, o, d, r.best, r.hit);
+  if (candL && candR) leaf_finish<COMPAT>(g, pR, o, d, r.best, r.hit);
   float tL = 0.0f, tR = 0.0f;
   const bool goL = refL >= 0 && slab(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, &tL) &&
                    !(r.best < tL);
