@@ -167,7 +167,11 @@ def main():
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-                "kernel": "k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)",
+                "kernel": ("k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)"
+                           if not tuning or tuning.get("ray_fetch", 1) != 2 else
+                           "k_pt_bounce (first bounce) then k_pt_trace + k_pt_shade per bounce "
+                           "(closest-hit traversal with per-wave ray refill, then the diffuse bounce); "
+                           "a launch = one bounce"),
                 "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                 "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
                 "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,

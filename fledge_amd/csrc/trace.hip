@@ -192,6 +192,7 @@ struct QRay {  // 32 bytes: origin + path id, direction + Halton index
 struct PtState {
   QRay* q[2];
   uint32_t* count;  // [max_depth + 2] queue sizes
+  uint2* hits;      // split bounces: closest hit (sorted position, t bits) per queued ray
   int32_t* nseg;
   int32_t* seg_id;
   float* seg_t;
@@ -217,6 +218,64 @@ struct Frame {
   FLG_D V3 to_world(V3 p) const { return (t * p.x + b * p.y) + n * p.z; }
 };
 
+// After the closest hit (p, t) of segment `bounce` of a path: the segment
+// record, and on a hit the diffuse sampling in the shading frame
+// (PathIntegrator::Li with a diffuse material, integrator.cpp:252-326) that
+// makes the next segment.  Returns true when nr holds that segment.
+template <bool COMPAT>
+FLG_D bool pt_finish(const TraceGeom& g, const PtState& s, const HaltonTables& tab, const HaltonParams& hp,
+                     int bounce, int max_depth, V3 o, V3 d, uint32_t path, uint32_t hidx, int32_t p,
+                     float t, QRay& nr) {
+  BTri tri;
+  if (p >= 0) tri = g.tris[p];
+  if (s.seg_id) {
+    int64_t k = (int64_t)path * s.stride_seg + bounce;
+    s.seg_id[k] = p >= 0 ? (int32_t)f2u(tri.v0.w) : -1;
+    s.seg_t[k] = t;
+  }
+  bool spawn = false;
+  if (p >= 0 && bounce < max_depth) {
+    const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
+             v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
+    const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
+    const V3 hp3 = o + d * t;
+    Frame fr(ng);
+    PathSampler ps;
+    ps.index = hidx;
+    ps.dim = 2 + 2 * (uint32_t)bounce;
+    float u0 = ps.next(tab, hp);
+    float u1 = ps.next(tab, hp);
+    V3 wo_l = fr.to_local(-d);
+    V3 wi_l = cosine_hemisphere(u0, u1);
+    float pdf = wo_l.z * wi_l.z > 0 ? fabsf(wi_l.z) * INV_PI : 0.0f;
+    V3 wi = fr.to_world(wi_l);
+    if (pdf != 0.0f) {
+      V3 no = offset_ray_origin(hp3, ng, wi);
+      V3 nd = normalize(wi);
+      nr.o = make_float4(no.x, no.y, no.z, u2f(path));
+      nr.d = make_float4(nd.x, nd.y, nd.z, u2f(hidx));
+      spawn = true;
+    }
+  }
+  if (!spawn && s.nseg) s.nseg[path] = bounce + 1;
+  return spawn;
+}
+
+// Wave-aggregated append of the lanes' new segments to the next queue.
+FLG_D void pt_append(const PtState& s, int bounce, bool spawn, const QRay& nr) {
+  const uint64_t m = __ballot(spawn);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    uint32_t wbase = 0;
+    if (lane == 0) wbase = atomicAdd(&s.count[bounce + 1], (uint32_t)__popcll(m));
+    wbase = __shfl(wbase, 0, 64);
+    if (spawn) {
+      const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      s.q[(bounce + 1) & 1][wbase + (uint32_t)__popcll(m & lt)] = nr;
+    }
+  }
+}
+
 // One bounce of every live path: (bounce 0) camera ray, closest hit, segment
 // record, diffuse sampling in the shading frame and the spawn of the next
 // segment into the other queue (wave-aggregated append).  Persistent grid:
@@ -230,7 +289,6 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
   const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
   if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
   const QRay* qin = s.q[bounce & 1];
-  QRay* qout = s.q[(bounce + 1) & 1];
   Counters c{0, 0};
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + threadIdx.x;
@@ -259,51 +317,9 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
       }
       float t = __builtin_inff();
       const int32_t p = trace_lane<COMPAT, COUNT, BS, RING>(g, o, d, t, c, ring);
-      BTri tri;
-      if (p >= 0) tri = g.tris[p];
-      if (s.seg_id) {
-        int64_t k = (int64_t)path * s.stride_seg + bounce;
-        s.seg_id[k] = p >= 0 ? (int32_t)f2u(tri.v0.w) : -1;
-        s.seg_t[k] = t;
-      }
-      bool end = true;
-      if (p >= 0 && bounce < max_depth) {
-        const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
-                 v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
-        const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
-        const V3 hp3 = o + d * t;
-        Frame fr(ng);
-        PathSampler ps;
-        ps.index = hidx;
-        ps.dim = 2 + 2 * (uint32_t)bounce;
-        float u0 = ps.next(tab, hp);
-        float u1 = ps.next(tab, hp);
-        V3 wo_l = fr.to_local(-d);
-        V3 wi_l = cosine_hemisphere(u0, u1);
-        float pdf = wo_l.z * wi_l.z > 0 ? fabsf(wi_l.z) * INV_PI : 0.0f;
-        V3 wi = fr.to_world(wi_l);
-        if (pdf != 0.0f) {
-          V3 no = offset_ray_origin(hp3, ng, wi);
-          V3 nd = normalize(wi);
-          nr.o = make_float4(no.x, no.y, no.z, u2f(path));
-          nr.d = make_float4(nd.x, nd.y, nd.z, u2f(hidx));
-          spawn = true;
-          end = false;
-        }
-      }
-      if (end && s.nseg) s.nseg[path] = bounce + 1;
+      spawn = pt_finish<COMPAT>(g, s, tab, hp, bounce, max_depth, o, d, path, hidx, p, t, nr);
     }
-    uint64_t m = __ballot(spawn);
-    if (m) {
-      const int lane = threadIdx.x & 63;
-      uint32_t wbase = 0;
-      if (lane == 0) wbase = atomicAdd(&s.count[bounce + 1], (uint32_t)__popcll(m));
-      wbase = __shfl(wbase, 0, 64);
-      if (spawn) {
-        uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        qout[wbase + (uint32_t)__popcll(m & lt)] = nr;
-      }
-    }
+    pt_append(s, bounce, spawn, nr);
   }
   if (COUNT) {
     unsigned long long cn = c.nodes, ct = c.tris;
@@ -318,124 +334,110 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
   }
 }
 
-// The same bounce with dynamic ray fetch (persistent threads, "while-while"):
-// bounce rays are incoherent and their traversal lengths heavy-tailed, so a
-// wave that takes one ray per lane idles most lanes behind its longest ray.
-// Here a lane whose traversal has ended shades its ray, appends the
-// continuation and takes the next ray from the queue (one atomic per wave for
-// all idle lanes); traversals advance kVisitsPerRound nodes between fetches.
-// Every ray runs exactly the traversal and shading of k_pt_bounce.
-constexpr int kVisitsPerRound = 8;
+// Bounces after the first, split in two launches (ray_fetch 2).  The
+// traversal kernel gives each wave a contiguous slice of the queue (the queue
+// length is read on the device); a lane whose ray has ended takes the next
+// entry of its wave's slice whenever enough of the wave's lanes are idle, so
+// a wave no longer idles behind its longest traversal, with no atomic per
+// fetch.  It only writes each ray's closest hit.  The shading kernel then
+// records the segments and spawns the next ones, one lane per ray, with the
+// code of the fused kernel: results are the same bits.
+constexpr int kRefillIdle = 16;   // refill once this many lanes of a wave are idle
+constexpr uint32_t kMinSlice = 64;  // rays per wave at least: short queues keep full waves
 
-template <bool COMPAT, bool FIRST, int BS, int RING>
-__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_dyn(TraceGeom g, PtState s, CamDev cam,
-                                                            HaltonTables tab, HaltonParams hp,
-                                                            int32_t spp, int bounce, int max_depth,
-                                                            uint32_t* fetch) {
+template <bool COMPAT, int BS, int RING>
+__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtState s, int bounce) {
   __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
-  const uint32_t n = FIRST ? (uint32_t)((int64_t)cam.w * cam.h * spp) : s.count[bounce];
-  if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) s.count[0] = n;
+  const uint32_t n = s.count[bounce];
   const QRay* qin = s.q[bounce & 1];
-  QRay* qout = s.q[(bounce + 1) & 1];
   const int lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (BS / 64);
+  const uint32_t wave = blockIdx.x * (BS / 64) + threadIdx.x / 64;
+  uint32_t per = (uint32_t)(((uint64_t)n + nwaves - 1) / nwaves);
+  per = per < kMinSlice ? kMinSlice : per;
+  const uint64_t lo = (uint64_t)wave * per;
+  const uint32_t end = (uint32_t)(lo + per < n ? lo + per : n);
+  uint32_t next = (uint32_t)(lo < n ? lo : n);  // the same in every lane of the wave
   using Stack = typename std::conditional<(RING > 0), RingStack<(RING > 0 ? RING : 1)>, PrivStack>::type;
   uint2 priv[kStack];
   Stack st = make_stack<Stack>(ring + threadIdx.x, BS, priv);
   Counters c{0, 0};
   TraceRay r;
-  uint32_t path = 0, hidx = 0;
-  bool have = false, active = false, exhausted = false;
+  uint32_t cur = 0;
+  bool have = false, active = false;
+  // every wave leaves once its slice is taken and all its lanes are idle; each
+  // traversal ends (a node is visited at most once)
   while (true) {
-    const uint64_t want = __ballot(!have && !exhausted);
-    if (want) {
-      const int leader = __builtin_ctzll(want);
-      uint32_t base = 0;
-      if (lane == leader) base = atomicAdd(fetch, (uint32_t)__popcll(want));
-      base = __shfl(base, leader, 64);
-      if (!have && !exhausted) {
+    const uint64_t idle = __ballot(!have);
+    const uint64_t live = ~idle & __ballot(true);
+    if (next < end && idle != 0 && (live == 0 || __popcll(idle) >= kRefillIdle)) {
+      if (!have) {
         const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const uint32_t i = base + (uint32_t)__popcll(want & below);
-        if (i >= n) {
-          exhausted = true;
-        } else {
-          V3 o, d;
-          if (FIRST) {
-            path = i;
-            const uint32_t pix = i / (uint32_t)spp, smp = i - pix * (uint32_t)spp;
-            const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
-            PathSampler ps;
-            ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
-            ps.dim = 0;
-            const float ux = ps.next(tab, hp);
-            const float uy = ps.next(tab, hp);
-            gen_ray(cam, (float)x + ux, (float)y + uy, &o, &d);
-            hidx = ps.index;
-          } else {
-            const QRay q = qin[i];
-            o = v3(q.o.x, q.o.y, q.o.z);
-            d = v3(q.d.x, q.d.y, q.d.z);
-            path = f2u(q.o.w);
-            hidx = f2u(q.d.w);
-          }
+        const uint32_t i = next + (uint32_t)__popcll(idle & below);
+        if (i < end) {
+          const QRay q = qin[i];
           st.reset();
+          cur = i;
           have = true;
-          active = trace_begin<COMPAT, false>(g, o, d, __builtin_inff(), r, c);
+          active = trace_begin<COMPAT, false>(g, v3(q.o.x, q.o.y, q.o.z), v3(q.d.x, q.d.y, q.d.z),
+                                              __builtin_inff(), r, c);
         }
       }
+      next += (uint32_t)__popcll(idle);
+    } else if (live == 0) {
+      break;
     }
-    if (__ballot(have) == 0 && __ballot(!exhausted) == 0) break;
-    for (int k = 0; k < kVisitsPerRound && active; ++k) active = trace_visit<COMPAT, false>(g, r, st, c);
+    if (have) {
+      if (active) active = trace_visit<COMPAT, false>(g, r, st, c);
+      if (!active) {
+        s.hits[cur] = make_uint2((uint32_t)r.hit, f2u(r.best));
+        have = false;
+      }
+    }
+  }
+}
+
+// Shading of the split bounces.  The new segments of a block's 1024 lanes are
+// appended with one atomic per block (per-wave counts summed in LDS), since
+// with nothing to overlap them one atomic per wave on the one queue counter
+// would serialise.
+constexpr int kShadeBlock = 1024;
+
+template <bool COMPAT>
+__global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s, HaltonTables tab,
+                                                          HaltonParams hp, int bounce, int max_depth) {
+  __shared__ uint32_t wave_n[kShadeBlock / 64];
+  __shared__ uint32_t block_base;
+  const uint32_t n = s.count[bounce];
+  const QRay* qin = s.q[bounce & 1];
+  QRay* qout = s.q[(bounce + 1) & 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
+  for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
+    const uint32_t i = base + threadIdx.x;
     bool spawn = false;
     QRay nr;
-    if (have && !active) {
-      have = false;
-      const int32_t p = r.hit;
-      const float t = r.best;
-      const V3 o = r.o, d = r.d;
-      BTri tri;
-      if (p >= 0) tri = g.tris[p];
-      if (s.seg_id) {
-        const int64_t k = (int64_t)path * s.stride_seg + bounce;
-        s.seg_id[k] = p >= 0 ? (int32_t)f2u(tri.v0.w) : -1;
-        s.seg_t[k] = t;
-      }
-      bool end = true;
-      if (p >= 0 && bounce < max_depth) {
-        const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
-                 v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
-        const V3 ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
-        const V3 hp3 = o + d * t;
-        Frame fr(ng);
-        PathSampler ps;
-        ps.index = hidx;
-        ps.dim = 2 + 2 * (uint32_t)bounce;
-        const float u0 = ps.next(tab, hp);
-        const float u1 = ps.next(tab, hp);
-        const V3 wo_l = fr.to_local(-d);
-        const V3 wi_l = cosine_hemisphere(u0, u1);
-        const float pdf = wo_l.z * wi_l.z > 0 ? fabsf(wi_l.z) * INV_PI : 0.0f;
-        const V3 wi = fr.to_world(wi_l);
-        if (pdf != 0.0f) {
-          const V3 no = offset_ray_origin(hp3, ng, wi);
-          const V3 ndir = normalize(wi);
-          nr.o = make_float4(no.x, no.y, no.z, u2f(path));
-          nr.d = make_float4(ndir.x, ndir.y, ndir.z, u2f(hidx));
-          spawn = true;
-          end = false;
-        }
-      }
-      if (end && s.nseg) s.nseg[path] = bounce + 1;
+    if (i < n) {
+      const QRay q = qin[i];
+      const uint2 h = s.hits[i];
+      spawn = pt_finish<COMPAT>(g, s, tab, hp, bounce, max_depth, v3(q.o.x, q.o.y, q.o.z),
+                                v3(q.d.x, q.d.y, q.d.z), f2u(q.o.w), f2u(q.d.w), (int32_t)h.x, u2f(h.y), nr);
     }
     const uint64_t m = __ballot(spawn);
-    if (m) {
-      uint32_t wbase = 0;
-      if (lane == 0) wbase = atomicAdd(&s.count[bounce + 1], (uint32_t)__popcll(m));
-      wbase = __shfl(wbase, 0, 64);
-      if (spawn) {
-        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        qout[wbase + (uint32_t)__popcll(m & lt)] = nr;
-      }
+    if (lane == 0) wave_n[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+      for (int k = 0; k < kShadeBlock / 64; ++k) tot += wave_n[k];
+      block_base = tot ? atomicAdd(&s.count[bounce + 1], tot) : 0u;
     }
+    __syncthreads();
+    if (spawn) {
+      uint32_t off = block_base;
+      for (int k = 0; k < w; ++k) off += wave_n[k];
+      const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      qout[off + (uint32_t)__popcll(m & lt)] = nr;
+    }
+    __syncthreads();  // wave_n and block_base are rewritten by the next trip
   }
 }
 
@@ -489,31 +491,29 @@ void launch_bounce_bs(int grid, int block, hipStream_t st, const TraceGeom& g, c
   }
 }
 
-template <bool COMPAT, bool FIRST, int RING>
-void launch_dyn_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s,
-                   const CamDev& cam, const HaltonTables& tab, const HaltonParams& hp, int spp, int b,
-                   int D, uint32_t* fetch) {
+template <bool COMPAT, int RING>
+void launch_split_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s, int b) {
   switch (block) {
-    case 64: k_pt_dyn<COMPAT, FIRST, 64, RING><<<grid, 64, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
-    case 128: k_pt_dyn<COMPAT, FIRST, 128, RING><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
-    case 512: k_pt_dyn<COMPAT, FIRST, 512, RING><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
-    default: k_pt_dyn<COMPAT, FIRST, 256, RING><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, fetch); break;
+    case 64: k_pt_trace<COMPAT, 64, RING><<<grid, 64, 0, st>>>(g, s, b); break;
+    case 128: k_pt_trace<COMPAT, 128, RING><<<grid, 128, 0, st>>>(g, s, b); break;
+    case 512: k_pt_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, s, b); break;
+    default: k_pt_trace<COMPAT, 256, RING><<<grid, 256, 0, st>>>(g, s, b); break;
   }
 }
 
 template <bool COMPAT>
 void launch_bounce(bool first, bool count, bool dyn, int grid, int block, int stack_mode, hipStream_t st,
                    const TraceGeom& g, const PtState& s, const CamDev& cam, const HaltonTables& tab,
-                   const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits,
-                   uint32_t* fetch) {
-  if (dyn && !count) {
-    if (ring_of(stack_mode)) {
-      if (first) launch_dyn_bs<COMPAT, true, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
-      else launch_dyn_bs<COMPAT, false, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
-    } else {
-      if (first) launch_dyn_bs<COMPAT, true, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
-      else launch_dyn_bs<COMPAT, false, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, fetch);
-    }
+                   const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
+  // the first bounce's camera rays are coherent: a wave's lanes stay in step
+  // in the fused kernel, which refilling them one by one would undo
+  if (dyn && !count && !first) {
+    if (ring_of(stack_mode))
+      launch_split_bs<COMPAT, kRing>(grid, block, st, g, s, b);
+    else
+      launch_split_bs<COMPAT, 0>(grid, block, st, g, s, b);
+    const int sgrid = grid * block / kShadeBlock > 0 ? grid * block / kShadeBlock : 1;
+    k_pt_shade<COMPAT><<<sgrid, kShadeBlock, 0, st>>>(g, s, tab, hp, b, D);
     return;
   }
   if (count) {  // instrumentation: one shape
@@ -659,6 +659,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   PtState s;
   s.q[0] = (QRay*)sc;
   s.q[1] = (QRay*)(sc + qbytes);
+  s.hits = (uint2*)(sc + 2 * qbytes);
   s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
   unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
   const bool count_visits = cfg->count_visits != 0;
@@ -670,8 +671,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
   const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
   const int pgrid = ctx->num_cus * bpc;
   // the whole frame: clear the queue counts, then one fused launch per bounce
-  // s.count[0 .. D+1]: queue sizes; s.count[128 + b]: fetch counter of bounce b
-  uint32_t* fetch = s.count + 128;
+  // s.count[0 .. D+1]: queue sizes
   const bool dyn = fmode == 2;
   auto enqueue = [&](hipStream_t stream, bool timed) {
     k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
@@ -680,10 +680,10 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
       ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
       if (compat)
         launch_bounce<true>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
-                            b, D, visits, fetch + b);
+                            b, D, visits);
       else
         launch_bounce<false>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
-                             b, D, visits, fetch + b);
+                             b, D, visits);
     }
   };
   if (cfg->use_graph) {

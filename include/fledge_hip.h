@@ -213,10 +213,12 @@ typedef struct flg_pt_cfg {
   int32_t blocks_per_cu; /* persistent grid of each bounce = CUs x this (1..64);
                             0: tuned value, else 8 */
   int32_t stack_mode;    /* as flg_trace_cfg.stack_mode */
-  int32_t ray_fetch;     /* 1: each lane traces one queue entry per loop trip;
-                            2: lanes whose ray has ended take the next entry
-                            while the others traverse (persistent threads);
-                            0: tuned value, else 1.  Same results. */
+  int32_t ray_fetch;     /* 1: one fused launch per bounce, each lane traces
+                            one queue entry per loop trip; 2: bounces after
+                            the first in two launches, a traversal whose lanes
+                            take the next entry whenever enough of a wave's
+                            lanes are idle (persistent threads), then the
+                            shading; 0: tuned value, else 1.  Same results. */
   int32_t reserved[2];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
