@@ -242,6 +242,21 @@ class Volume:
             return out, {"paths": int(st[0]), "lookups": int(st[1])}
         return out
 
+    def render_mesh(self, bvh, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0):
+        """VolPathIntegrator: `bvh`'s closed mesh as the interface around this
+        volume's medium (placed on the mesh's bounding box); RGB image."""
+        cfg = N.RenderCfg()
+        cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
+        cfg.schedule = schedule
+        if out is None:
+            out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
+        st = (ctypes.c_uint64 * 2)()
+        self.ctx.check(self.ctx.lib.flg_volpath_mesh_render(self.ctx.h, bvh.h, self.h, ctypes.byref(scene),
+                                                            ctypes.byref(cfg), _ptr(out), st if stats else None))
+        if stats:
+            return out, {"paths": int(st[0]), "lookups": int(st[1])}
+        return out
+
     def close(self):
         if self.h:
             self.ctx.lib.flg_volume_destroy(self.h)

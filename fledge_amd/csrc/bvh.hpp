@@ -117,6 +117,21 @@ FLG_HD bool pluecker(V3 a, V3 b, V3 c, V3 O, V3 D, bool compat, float* thit, V3*
   return pluecker_t(a, b, c, O, D, compat, thit, ng);
 }
 
+// OffsetRayOrigin (src/common/ray.h:64-93)
+FLG_HD V3 offset_ray_origin(V3 p, V3 n, V3 dir) {
+  const float origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f, int_scale = 256.0f;
+  V3 offset = n;
+  if (dot(n, dir) <= 0) offset = -n;
+  int oi0 = (int)(offset.x * int_scale), oi1 = (int)(offset.y * int_scale),
+      oi2 = (int)(offset.z * int_scale);
+  float px = u2f((uint32_t)((int32_t)f2u(p.x) + ((p.x < 0) ? -oi0 : oi0)));
+  float py = u2f((uint32_t)((int32_t)f2u(p.y) + ((p.y < 0) ? -oi1 : oi1)));
+  float pz = u2f((uint32_t)((int32_t)f2u(p.z) + ((p.z < 0) ? -oi2 : oi2)));
+  return v3(fabsf(p.x) < origin ? p.x + float_scale * offset.x : px,
+            fabsf(p.y) < origin ? p.y + float_scale * offset.y : py,
+            fabsf(p.z) < origin ? p.z + float_scale * offset.z : pz);
+}
+
 // BoundIntersect (intersector.hpp:103-120) with a precomputed 1/d (the
 // reference computes the same IEEE quotient per call).  The reference returns
 // as soon as t0 > t1 after an axis; t0 only grows and t1 only shrinks from

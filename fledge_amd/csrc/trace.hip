@@ -147,21 +147,6 @@ __global__ __launch_bounds__(256) void k_camera(CamDev cam, HaltonTables tab, Ha
 }
 
 // ------------------------------------------------------------ path tracer --
-// OffsetRayOrigin (src/common/ray.h:64-93)
-FLG_D V3 offset_ray_origin(V3 p, V3 n, V3 dir) {
-  const float origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f, int_scale = 256.0f;
-  V3 offset = n;
-  if (dot(n, dir) <= 0) offset = -n;
-  int oi0 = (int)(offset.x * int_scale), oi1 = (int)(offset.y * int_scale),
-      oi2 = (int)(offset.z * int_scale);
-  float px = u2f((uint32_t)((int32_t)f2u(p.x) + ((p.x < 0) ? -oi0 : oi0)));
-  float py = u2f((uint32_t)((int32_t)f2u(p.y) + ((p.y < 0) ? -oi1 : oi1)));
-  float pz = u2f((uint32_t)((int32_t)f2u(p.z) + ((p.z < 0) ? -oi2 : oi2)));
-  return v3(fabsf(p.x) < origin ? p.x + float_scale * offset.x : px,
-            fabsf(p.y) < origin ? p.y + float_scale * offset.y : py,
-            fabsf(p.z) < origin ? p.z + float_scale * offset.z : pz);
-}
-
 // ConcentricSampleDisk / CosineSampleHemisphere (math_utils.h:69-88)
 FLG_D V3 cosine_hemisphere(float u0, float u1) {
   float ux = (float)((double)u0 * 2.0 - 1.0), uy = (float)((double)u1 * 2.0 - 1.0);

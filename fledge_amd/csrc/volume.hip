@@ -55,6 +55,23 @@ void dist_init(const float* f, int n, float* cdf, float* integral) {
     for (int i = 1; i < n + 1; ++i) cdf[i] /= *integral;
 }
 
+// The scene's bounding sphere as InfiniteAreaLight::preprocess uses it
+// (AABB::boundSphere, aabb.h:67-70): centre, and radius * 1000 capped at
+// FLT_MAX / 2.
+void env_sphere(const float* bounds, float* center, float* radius) {
+  V3 lo = v3(bounds[0], bounds[1], bounds[2]);
+  V3 hi = v3(bounds[3], bounds[4], bounds[5]);
+  V3 c = (lo + hi) / 2.0f;
+  V3 dd = hi - c;
+  float r = sqrtf(sqnorm(dd));
+  r *= 1000.0f;
+  r = smin(3.402823466e+38f / 2, r);
+  center[0] = c.x;
+  center[1] = c.y;
+  center[2] = c.z;
+  *radius = r;
+}
+
 // InfiniteAreaLight(texture) with a ConstTexture (light.cpp:93-108) and
 // preprocess (light.cpp:110-117) over the scene bound (= the volume box).
 int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
@@ -84,60 +101,8 @@ int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
   FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   vol->env_radiance = radiance;
   vol->env_marg_int = marg_int;
-  // AABB::boundSphere (aabb.h:67-70), radius * 1000 capped at FLT_MAX / 2
-  V3 lo = v3(vol->bounds[0], vol->bounds[1], vol->bounds[2]);
-  V3 hi = v3(vol->bounds[3], vol->bounds[4], vol->bounds[5]);
-  V3 c = (lo + hi) / 2.0f;
-  V3 dd = hi - c;
-  float r = sqrtf(sqnorm(dd));
-  r *= 1000.0f;
-  r = smin(3.402823466e+38f / 2, r);
-  vol->env_center[0] = c.x;
-  vol->env_center[1] = c.y;
-  vol->env_center[2] = c.z;
-  vol->env_radius = r;
+  env_sphere(vol->bounds, vol->env_center, &vol->env_radius);
   return FLG_OK;
-}
-
-// UniformSampleOneLight + EstimateDirect for a volume vertex
-// (integrator.cpp:69-120); lights ordered [point, environment], p = 1/n.
-template <bool COUNT>
-FLG_D float sample_one_light(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                             const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo,
-                             uint32_t& nlook) {
-  HaltonDraw rng{ps, tab, hp};
-  const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
-  int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
-  if (light_num > n_lights - 1) light_num = n_lights - 1;
-  const float light_pdf = (float)(1.0 / (double)n_lights);
-  const bool use_point = L.has_point && light_num == 0;
-  float u0 = ps.next(tab, hp);
-  float u1 = ps.next(tab, hp);
-  V3 wi, p_light;
-  float pdf, Li;
-  if (use_point) {
-    V3 diff = L.point_pos - p;
-    float d2 = sqnorm(diff);
-    wi = normalize(diff);
-    Li = L.point_intensity / d2;
-    pdf = 1.0f;
-    p_light = L.point_pos;
-  } else {
-    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, &p_light);
-  }
-  float out = 0.0f;
-  if (pdf == 0.0f || Li == 0.0f) return out / light_pdf;
-  float f = hg_p(wi, wo, v.g);
-  if (f != 0.0f) {
-    V3 diff = p_light - p;
-    float nrm = sqrtf(sqnorm(diff));
-    V3 d = diff / nrm;
-    float tr = vol_tr<COUNT>(v, rng, p, d, nrm - SHADOW_EPS, nlook);
-    Li *= tr;
-    out += f * Li / pdf;
-  }
-  return out / light_pdf;
 }
 
 // SVolIntegrator::Li (integrator.cpp:328-405), m_rrThreshold = 0.01, split
@@ -253,7 +218,10 @@ FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& ta
   const float u = s.ps.next(tab, hp);
   const float vv = s.ps.next(tab, hp);
   const V3 wi = hg_sample(wo, u, vv, v.g);
-  s.Lr += s.beta * sample_one_light<COUNT>(v, L, tab, hp, s.ps, pos, wo, nlook);
+  s.Lr += s.beta * sample_one_light(L, v.g, tab, hp, s.ps, pos, wo, [&](V3 p, V3 d, float tmax) {
+            HaltonDraw rng{s.ps, tab, hp};
+            return vol_tr<COUNT>(v, rng, p, d, tmax, nlook);
+          });
   if (s.bounces >= max_depth || s.beta * s.beta + s.beta * s.beta + s.beta * s.beta == 0.0f) return true;
   s.o = pos;
   s.d = normalize(wi);
@@ -418,7 +386,8 @@ VolDev make_voldev(const flg_volume* vol) {
   return voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, vol->bounds, vol->medium, vol->max_density);
 }
 
-int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L) {
+int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L,
+                const float* scene_bounds) {
   *L = LightsDev{};
   L->has_point = sc->has_point != 0;
   L->point_pos = v3(sc->point_pos[0], sc->point_pos[1], sc->point_pos[2]);
@@ -437,6 +406,11 @@ int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDe
     L->env.radiance = sc->env_radiance;
     L->env.center = v3(vol->env_center[0], vol->env_center[1], vol->env_center[2]);
     L->env.radius = vol->env_radius;
+    if (scene_bounds) {
+      float c[3];
+      env_sphere(scene_bounds, c, &L->env.radius);
+      L->env.center = v3(c[0], c[1], c[2]);
+    }
   }
   return FLG_OK;
 }

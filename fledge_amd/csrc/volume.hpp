@@ -353,6 +353,48 @@ struct LightsDev {
   float point_intensity;
 };
 
+// UniformSampleOneLight + EstimateDirect for a volume vertex
+// (integrator.cpp:69-120): lights in the order [point, environment], one
+// picked with p = 1/n, u_light drawn before sampleLi, the HG phase as f.  The
+// shadow transmittance is shadow(p, d, tmax): the medium's ratio tracking
+// (SVolIntegrator) or VolEstimateTr through interfaces (VolPathIntegrator).
+template <class Shadow>
+FLG_D float sample_one_light(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
+                             PathSampler& ps, V3 p, V3 wo, Shadow shadow) {
+  const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
+  if (n_lights == 0) return 0.0f;
+  int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
+  if (light_num > n_lights - 1) light_num = n_lights - 1;
+  const float light_pdf = (float)(1.0 / (double)n_lights);
+  const bool use_point = L.has_point && light_num == 0;
+  float u0 = ps.next(tab, hp);
+  float u1 = ps.next(tab, hp);
+  V3 wi, p_light;
+  float pdf, Li;
+  if (use_point) {
+    V3 diff = L.point_pos - p;
+    float d2 = sqnorm(diff);
+    wi = normalize(diff);
+    Li = L.point_intensity / d2;
+    pdf = 1.0f;
+    p_light = L.point_pos;
+  } else {
+    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, &p_light);
+  }
+  float out = 0.0f;
+  if (pdf == 0.0f || Li == 0.0f) return out / light_pdf;
+  float f = hg_p(wi, wo, g);
+  if (f != 0.0f) {
+    V3 diff = p_light - p;
+    float nrm = sqrtf(sqnorm(diff));
+    V3 d = diff / nrm;
+    float tr = shadow(p, d, nrm - SHADOW_EPS);
+    Li *= tr;
+    out += f * Li / pdf;
+  }
+  return out / light_pdf;
+}
+
 // Camera (src/common/camera.h:11-31): ctor on the host in float, per-ray
 // offsets through the double literal as in generateRay.
 struct CamDevV {
@@ -386,6 +428,9 @@ inline CamDevV make_cam_v(const flg_camera* c, int w, int h) {
 }
 
 VolDev make_voldev(const flg_volume* vol);
-int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L);
+// scene_bounds (may be NULL: the volume's box): the scene bound that places the
+// environment light's sample points (InfiniteAreaLight::preprocess).
+int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L,
+                const float* scene_bounds = nullptr);
 
 }  // namespace flg

@@ -299,6 +299,21 @@ typedef struct flg_render_cfg {
 FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
 
+/* Surfaces and media together: VolPathIntegrator::Li (integrator.cpp:407-510)
+ * for the closed mesh `bvh`, whose triangles are index-matched interfaces
+ * (no material), with `vol`'s medium inside it, filling the mesh's bounding
+ * box as ShapePrimitive::setBound places it (primitive.cpp:17-24); point and
+ * environment lights from `scene`.  An interface is crossed in the ray's
+ * direction (SInteraction::SpawnRay, interaction.cpp:16-26) without counting
+ * a bounce, where the reference dereferences the missing material
+ * (integrator.cpp:433,487,501); shadow rays cross interfaces as
+ * VolEstimateTr does (integrator.cpp:40-64).  Per-pixel mean of the path
+ * samples (cfg->film must be 0); h_stats (may be NULL): {path samples,
+ * density lookups}. */
+FLG_API int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
+                                    const flg_vol_scene* scene, const flg_render_cfg* cfg, float* d_rgb,
+                                    uint64_t* h_stats);
+
 /* Differentiable forward of the same path structure (differential delta
  * tracking on the proposal grid, ratio tracking without roulette on camera
  * and shadow rays, fixed majorant): sigma_t = sigma_t_scale * density,

@@ -305,6 +305,26 @@ void orc_halton_stateful(int32_t spp, int32_t resx, int32_t resy, int32_t px, in
   }
 }
 
+// VolPathIntegrator over the closed interface mesh of handle p with the
+// density grid inside, on the mesh's bounding box; medium, lights and camera
+// as orc_volpath_render.
+void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t nx, int32_t ny, int32_t nz,
+                             const float* medium3, const float* lights7, const float* cam10, int32_t W, int32_t H,
+                             int32_t spp, int32_t max_depth, float* rgb, int nthreads, uint64_t* lookups) {
+  const Lbvh& b = ((Handle*)p)->b;
+  const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
+  VolScene sc;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
+  sc.point.present = lights7[0] != 0.0f;
+  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
+  sc.point.intensity = lights7[4];
+  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  Camera cam = make_camera(cam10);
+  sc.cam = &cam;
+  sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
+  volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
+}
+
 // ------------------------------------------------------------ path trace ---
 void orc_diffuse_render(void* p, int32_t compat, const float* cam10, int32_t W, int32_t H,
                         int32_t spp, int32_t max_depth, int64_t p0, int64_t p1, int32_t* nseg,

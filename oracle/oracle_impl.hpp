@@ -135,6 +135,14 @@ float vol_tr_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax);
 float vol_sample_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success,
                      V3* pos);
 void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups, uint64_t* paths);
+// the trackers with the lookup count
+float vol_tr_lk(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, uint64_t* lookups);
+float vol_sample_lk(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success, V3* pos,
+                    uint64_t* lookups);
+V3 offset_ray_origin(V3 p, V3 n, V3 dir);  // OffsetRayOrigin (pathtrace.cpp)
+// VolPathIntegrator over a closed interface mesh (volpath_mesh.cpp)
+void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, int nthreads,
+                         uint64_t* lookups);
 // the reference's render loop and film (film.cpp)
 float film_mitchell1d(float d);
 void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups,

@@ -163,6 +163,15 @@ static float vol_sample(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tm
   return 1.0f;
 }
 
+float vol_tr_lk(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, uint64_t* lookups) {
+  return vol_tr(sc, hs, o, d, ray_tmax, lookups);
+}
+
+float vol_sample_lk(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax, bool* success, V3* pos,
+                    uint64_t* lookups) {
+  return vol_sample(sc, hs, o, d, ray_tmax, success, pos, lookups);
+}
+
 float vol_tr_pub(const VolScene& sc, Halton& hs, V3 o, V3 d, float ray_tmax) {
   uint64_t lk = 0;
   return vol_tr(sc, hs, o, d, ray_tmax, &lk);

@@ -43,6 +43,8 @@ _SIGS = {
     "orc_volpath_render_film": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P]),
     "orc_volpath_samples": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P]),
     "orc_mitchell1d": (f32, [f32]),
+    "orc_volpath_mesh_render": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P,
+                                       ctypes.c_int, P]),
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
 }
 
@@ -119,6 +121,19 @@ class Lbvh:
         lib().orc_diffuse_render(self.h, 1 if compat else 0, _p(cam), W, H, spp, max_depth, p0, p1,
                                  _p(nseg), _p(ids), _p(ts), nthreads or NTHREADS, _p(rays))
         return nseg, ids, ts, int(rays[0])
+
+    def volpath_mesh(self, grid, medium, lights, cam10, W, H, spp, max_depth, compat=False, nthreads=None):
+        """VolPathIntegrator with this mesh as the interface around the grid's
+        medium (on the mesh's bounding box): (rgb (H, W, 3), lookups)."""
+        grid = np.ascontiguousarray(grid, np.float32)
+        nz, ny, nx = grid.shape
+        rgb = np.zeros((H, W, 3), np.float32)
+        lk = np.zeros(1, np.uint64)
+        lib().orc_volpath_mesh_render(self.h, 1 if compat else 0, _p(grid), nx, ny, nz,
+                                      _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                                      _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
+                                      nthreads or NTHREADS, _p(lk))
+        return rgb, int(lk[0])
 
     def __del__(self):
         try:
