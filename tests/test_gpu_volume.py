@@ -86,7 +86,7 @@ def _scene(ctx, grid, spp, W, H, max_depth, point=True, env=True, sigma=(6.4, 1.
     return out.cpu().numpy(), ref, st, lk
 
 
-@pytest.mark.parametrize("point,env", [(True, True), (False, True), (True, False)])
+@pytest.mark.parametrize("point,env", [(True, True), (False, True), (True, False), (False, False)])
 def test_volpath_parity_small(ctx, point, env):
     grid = api.gen_fbm_grid(32, seed=3)
     print("volpath small point=%s env=%s" % (point, env), flush=True)
