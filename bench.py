@@ -241,7 +241,9 @@ def main():
         bricks = np.random.default_rng(4).random((64, 64, 64)) < 0.15
         gc *= np.repeat(np.repeat(np.repeat(bricks, 8, 0), 8, 1), 8, 2)
         bc = api.LBVH.build(ctx, torch.from_numpy(vc).to(dev), torch.from_numpy(ic).to(dev))
-        volc = api.Volume(ctx, torch.from_numpy(gc).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+        gct = torch.from_numpy(gc).to(dev)
+        volc = api.Volume(ctx, gct, [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+        gmax = float(gc.max())
         del gc, vc, ic
         scc = api.vol_scene(cam2)
         outc = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
@@ -253,6 +255,29 @@ def main():
         secondary["combined_config"] = ("VolPathIntegrator, 8,388,608-triangle interface mesh around a 512^3 grid "
                                         "(8^3 bricks, 15% active), 1920x1080, 64 spp, depth 16")
         volc.close()
+        # the same scene through the differentiable estimator (sigma_t 8 per
+        # unit density, albedo 0.8: the medium above), forward with the walk
+        # tape, then the adjoint of an L2-like loss with respect to the
+        # active bricks' voxels
+        from fledge_amd import diff
+        dvc = diff.DiffVolume(ctx, gct, g=0.3)
+        layout = diff.BrickLayout(ctx, gct, 8)
+        del gct
+        tape = diff.new_tape(ctx, W, H, 64, 16)
+        lpc = torch.empty(W * H * 64, dtype=torch.float32, device=dev)
+        dlc = torch.full((H, W, 3), 1.0 / (W * H * 3), dtype=torch.float32, device=dev)
+        gb = layout.zeros()
+        gac = torch.zeros(1, dtype=torch.float32, device=dev)
+        mu_c = 8.0 * gmax
+        tf = timed(lambda: diff.mesh_forward(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, out=outc, lpath=lpc,
+                                             tape=tape), 1, 1, sync)
+        ta = timed(lambda: diff.mesh_adjoint(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, lpc, dlc, grad=gb,
+                                             galpha=gac, tape=tape, bricks=layout), 1, 0, sync)
+        secondary["combined_diff_forward_ms"] = round(tf * 1e3, 2)
+        secondary["combined_adjoint_ms"] = round(ta * 1e3, 2)
+        secondary["combined_active_bricks"] = layout.n_active
+        secondary["combined_fwd_adj_over_forward"] = round((tf + ta) / tc, 3)
+        del tape, lpc, gb
         bc.close()
 
     # ------------------------------------------------------ CPU baseline

@@ -88,6 +88,10 @@ class DiffParams(ctypes.Structure):
     _fields_ = [("sigma_t_scale", c_f), ("albedo", c_f), ("g", c_f), ("majorant", c_f)]
 
 
+class BrickLayout(ctypes.Structure):
+    _fields_ = [("brick", c_i32), ("n_active", c_i32), ("d_slot", c_vp)]
+
+
 _SIGS = {
     "flg_ctx_create": (c_int, [c_int, c_vp, P(c_vp)]),
     "flg_ctx_destroy": (c_int, [c_vp]),
@@ -124,6 +128,11 @@ _SIGS = {
                                          c_vp, c_vp, c_vp]),
     "flg_volpath_adjoint": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
                                     c_vp, c_vp, c_vp, c_vp]),
+    "flg_bricks_build": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i32)]),
+    "flg_volpath_mesh_diff_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams),
+                                              c_vp, c_vp, c_vp]),
+    "flg_volpath_mesh_adjoint": (c_int, [c_vp, c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
+                                         c_vp, c_vp, P(BrickLayout), c_vp, c_vp]),
     "flg_debug_eval": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i64]),
     "flg_film_save_exr": (c_int, [ctypes.c_char_p, c_i32, c_i32, c_vp]),
 }

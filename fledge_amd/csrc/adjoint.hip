@@ -28,174 +28,25 @@
 #include <algorithm>
 #include <cmath>
 
-#include "volume.hpp"
+#include "diff.hpp"
 
 namespace flg {
 
 namespace {
 
-constexpr int kMaxSteps = 4096;
-
-// Steps a walk of this estimator may take over [t, t_max]: 8 times the
-// expected count (mu * length) plus 64, at most kMaxSteps.  A walk of ordinary
-// draws never comes near it; on the main path it trims the delta-tracking
-// walks of the first Halton indices, whose high dimensions are tiny
-// (index / prime) and give minute free flights, and it bounds every loop.
-FLG_D int walk_cap(float mu, float t, float t_max) {
-  const float e = 8.0f * mu * (t_max - t) + 64.0f;
-  return e < (float)kMaxSteps ? (int)e : kMaxSteps;
-}
-
-struct DiffDev {
-  VolDev v;   // evaluated grid theta (densities rho)
-  VolDev vp;  // proposal grid theta_p (decisions)
-  bool same;  // theta_p is theta
-  float s, alpha, g, mu;
-  // walk tape (NULL: none).  Walk w of path p: tape[w * npaths + p] = product
-  // of its non-zero factors (0 once two factors are 0); bit w of
-  // mask_hi[p]:mask_lo[p] set iff exactly one factor is 0.
-  float* tape;
-  uint32_t* mask_lo;
-  uint32_t* mask_hi;
-  uint32_t npaths;
-};
-
-struct Look2 {
-  Tri8 t;     // lookup on theta (weights, base, rho)
-  float rho_p;
-};
-
-FLG_D Look2 look2(const DiffDev& D, V3 p) {
-  Look2 r;
-  r.t = lookup(D.v, p);
-  r.rho_p = D.same ? r.t.rho : lookup(D.vp, p).rho;
-  return r;
-}
-
-// gradient scatter: g * trilinear weight into the caller's grid (x fastest)
-FLG_D void deposit(const DiffDev& D, float* grad, const Tri8& t, float g) {
-  if (t.base < 0 || g == 0.0f) return;
-  const int px = D.v.px, py = D.v.ny + 2;
-  const int b = t.base;
-  const int i = b % px - 1, j = (b / px) % py - 1, k = b / D.v.pxy - 1;
-  const float wx[2] = {1.0f - t.fx, t.fx}, wy[2] = {1.0f - t.fy, t.fy}, wz[2] = {1.0f - t.fz, t.fz};
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-    const int ii = i + dx, jj = j + dy, kk = k + dz;
-    if (ii < 0 || jj < 0 || kk < 0 || ii >= D.v.nx || jj >= D.v.ny || kk >= D.v.nz) continue;
-    const float w = wx[dx] * wy[dy] * wz[dz];
-    unsafeAtomicAdd(grad + ((int64_t)kk * D.v.ny + jj) * D.v.nx + ii, g * w);
-  }
-}
-
-// A ratio-tracking product: nonzero = product of the non-zero factors,
-// zeros = number of factors equal to 0.
-struct Track {
-  float nonzero;
-  int zeros;
-};
-
-// First walk of a ratio track without roulette (the camera and shadow rays):
-// factor 1 - s rho / mu per tentative collision.  Stops once the product and
-// every derivative are fixed: two zero factors (every derivative is 0), or a
-// product of the non-zero factors that reached 0.
-FLG_D Track track_measure(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
+// The ratio track of walk `key` along (o, d) up to tmax: measured, and its
+// derivative deposited.
+FLG_D Track box_track_measure(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
   Track r{1.0f, 0};
-  float t_min, t_max;
-  if (!box_pbrt(D.v.bmin, D.v.bmax, o, d, tmax, &t_min, &t_max)) return r;
-  float t = smax(t_min, 0.0f);
-  t_max = smin(t_max, tmax);
-  const int cap = walk_cap(D.mu, t, t_max);
-  for (int step = 0; step < cap; ++step) {
-    t += -flg_logf(walk_draw(key, (uint32_t)step)) / D.mu;
-    if (t >= t_max) break;
-    const float f = 1.0f - D.s * lookup(D.v, o + d * t).rho / D.mu;
-    if (f == 0.0f) {
-      if (++r.zeros >= 2) break;
-    } else {
-      r.nonzero *= f;
-      if (r.nonzero == 0.0f) break;
-    }
-  }
+  uint32_t n = 0;
+  seg_measure(D, key, n, o, d, tmax, r);
   return r;
 }
 
-// The product's value.
-FLG_D float track_value(const Track& r) { return r.zeros > 0 ? 0.0f : r.nonzero; }
-
-// Whether a measured track has a non-zero derivative.
-FLG_D bool track_live(const Track& r) { return r.zeros < 2 && r.nonzero != 0.0f; }
-
-// Second walk (same draws): deposits scale * d(product)/d(rho_i) for every
-// lookup i, d/d(rho_i) = k prod_{j != i} f_j with k = -s / mu.
-FLG_D void track_deposit(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax, const Track& r,
-                         float scale, float* grad) {
-  float t_min, t_max;
-  if (!box_pbrt(D.v.bmin, D.v.bmax, o, d, tmax, &t_min, &t_max)) return;
-  float t = smax(t_min, 0.0f);
-  t_max = smin(t_max, tmax);
-  const float k = -D.s / D.mu;
-  const int cap = walk_cap(D.mu, t, t_max);
-  for (int step = 0; step < cap; ++step) {
-    t += -flg_logf(walk_draw(key, (uint32_t)step)) / D.mu;
-    if (t >= t_max) break;
-    const Tri8 q = lookup(D.v, o + d * t);
-    const float f = 1.0f - D.s * q.rho / D.mu;
-    if (r.zeros == 0) {
-      deposit(D, grad, q, scale * k * (r.nonzero / f));
-    } else if (f == 0.0f) {  // the one zero factor: the others' product
-      deposit(D, grad, q, scale * k * r.nonzero);
-      break;
-    }
-  }
-}
-
-// Walk w of path p: measured (and recorded on the tape by a forward that has
-// one), or read back from the tape by the adjoint.
-template <bool ADJ>
-FLG_D Track track(const DiffDev& D, uint32_t key, uint32_t p, uint32_t w, V3 o, V3 d, float tmax,
-                  uint64_t& mask) {
-  if (ADJ && D.tape) {
-    return Track{D.tape[(int64_t)w * D.npaths + p], (int)((mask >> w) & 1u)};
-  }
-  const Track r = track_measure(D, key, o, d, tmax);
-  if (!ADJ && D.tape) {
-    D.tape[(int64_t)w * D.npaths + p] = r.zeros >= 2 ? 0.0f : r.nonzero;
-    if (r.zeros == 1) mask |= (uint64_t)1 << w;
-  }
-  return r;
-}
-
-// Light choice and sample (UniformSampleOneLight / EstimateDirect,
-// integrator.cpp:69-120) without the shadow transmittance: returns
-// K = f Li / pdf / light_pdf, 0 if no contribution.
-FLG_D float nee_sample(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
-                       const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo, V3* p_light) {
-  const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
-  int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
-  if (light_num > n_lights - 1) light_num = n_lights - 1;
-  const float light_pdf = (float)(1.0 / (double)n_lights);
-  const bool use_point = L.has_point && light_num == 0;
-  float u0 = ps.next(tab, hp);
-  float u1 = ps.next(tab, hp);
-  V3 wi;
-  float pdf, Li;
-  if (use_point) {
-    V3 diff = L.point_pos - p;
-    float d2 = sqnorm(diff);
-    wi = normalize(diff);
-    Li = L.point_intensity / d2;
-    pdf = 1.0f;
-    *p_light = L.point_pos;
-  } else {
-    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, p_light);
-  }
-  if (pdf == 0.0f || Li == 0.0f) return 0.0f;
-  float f = hg_p(wi, wo, D.g);
-  if (f == 0.0f) return 0.0f;
-  return f * Li / pdf / light_pdf;
+FLG_D void box_track_deposit(const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax, const Track& r, float scale,
+                             float* grad) {
+  uint32_t n = 0;
+  seg_deposit(D, key, n, o, d, tmax, r, scale, grad);
 }
 
 // One path, as a start (camera ray, box entry, primary transmittance) and one
@@ -221,6 +72,7 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
   s.acc = 0.0f;
   s.path = p;
   s.mask = 0;
+  s.delta = s.L_total = 0.0f;
   if (ADJ) {
     const float* g = dl_drgb + 3 * (int64_t)pix;
     s.delta = (g[0] + g[1] + g[2]) / (float)spp;
@@ -241,9 +93,8 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
     return true;
   }
   const uint32_t key = walk_key(p, 0);
-  const Track tr = track<ADJ>(D, key, p, 0, o, d, __builtin_inff(), s.mask);
-  if (ADJ && track_live(tr) && s.delta * Le != 0.0f)
-    track_deposit(D, key, o, d, __builtin_inff(), tr, s.delta * Le, grad);
+  const Track tr = track<ADJ>(D, p, 0, s.mask, [&] { return box_track_measure(D, key, o, d, kInf); });
+  if (ADJ && track_live(tr) && s.delta * Le != 0.0f) box_track_deposit(D, key, o, d, kInf, tr, s.delta * Le, grad);
   s.acc += track_value(tr) * Le;
   s.o = o + d * t_min;
   s.d = d;
@@ -257,42 +108,9 @@ template <bool ADJ>
 FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
                      const HaltonParams& hp, int max_depth, float* grad, float* galpha, DiffPath& s) {
   const float rr = 0.01f;
-  const float k = -D.s / D.mu;
-  // differential delta tracking
-  float tb0, tb1;
-  bool hit = false;
-  V3 pos = s.o;
-  if (box_pbrt(D.v.bmin, D.v.bmax, s.o, s.d, __builtin_inff(), &tb0, &tb1) && D.mu > 0.0f) {
-    float t = smax(tb0, 0.0f);
-    tb1 = smin(tb1, __builtin_inff());
-    const int cap = walk_cap(D.mu, t, tb1);
-    for (int step = 0; step < cap; ++step) {
-      const float u = s.ps.next(tab, hp);
-      if (u == 0.0f) break;  // a zero draw never advances t (Halton index 0): the walk ends
-      t += -flg_logf(1.0f - u) / D.mu;
-      if (t >= tb1) break;
-      const V3 p = s.o + s.d * t;
-      const Look2 q = look2(D, p);
-      const float p_n = 1.0f - D.s * q.rho_p / D.mu;
-      const bool real = s.ps.next(tab, hp) > p_n;
-      if (real) {
-        if (ADJ) {
-          const float S = s.L_total - s.acc;
-          if (q.t.rho != 0.0f) deposit(D, grad, q.t, s.delta * S / q.t.rho);
-          *galpha += s.delta * S / D.alpha;
-        }
-        s.beta *= D.alpha * (q.t.rho / q.rho_p);
-        pos = p;
-        hit = true;
-        break;
-      }
-      const float fnull = 1.0f - D.s * q.t.rho / D.mu;
-      if (ADJ && fnull != 0.0f) deposit(D, grad, q.t, s.delta * (s.L_total - s.acc) * k / fnull);
-      // a null collision has probability p_n > 0 except for u == 0
-      // exactly; that measure-zero case keeps weight 1
-      if (p_n != 0.0f) s.beta *= fnull / p_n;
-    }
-  }
+  V3 pos;
+  const bool hit =
+      diff_delta_track<ADJ>(D, tab, hp, s.ps, s.o, s.d, kInf, s.beta, s.acc, s.delta, s.L_total, grad, galpha, &pos);
   if (!hit) return true;
   const V3 wo = -s.d;
   const float u = s.ps.next(tab, hp);
@@ -306,9 +124,10 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
     const V3 dd = diff / nrm;
     const uint32_t w = (uint32_t)s.bounces;
     const uint32_t key = walk_key(s.path, w);
-    const Track tr = track<ADJ>(D, key, s.path, w, pos, dd, nrm - SHADOW_EPS, s.mask);
+    const Track tr =
+        track<ADJ>(D, s.path, w, s.mask, [&] { return box_track_measure(D, key, pos, dd, nrm - SHADOW_EPS); });
     if (ADJ && track_live(tr) && s.delta * s.beta * K != 0.0f)
-      track_deposit(D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
+      box_track_deposit(D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
     s.acc += s.beta * K * track_value(tr);
   }
   if (s.bounces >= max_depth || s.beta == 0.0f) return true;
@@ -328,14 +147,6 @@ FLG_D void diff_finish(const DiffDev& D, uint32_t p, const DiffPath& s, float* l
   if (D.tape) {
     D.mask_lo[p] = (uint32_t)s.mask;
     D.mask_hi[p] = (uint32_t)(s.mask >> 32);
-  }
-}
-
-template <bool ADJ>
-FLG_D void reduce_galpha(float ga, float* galpha) {
-  if (ADJ) {
-    for (int off = 32; off >= 1; off >>= 1) ga += __shfl_xor(ga, off, 64);
-    if ((threadIdx.x & 63) == 0 && ga != 0.0f) unsafeAtomicAdd(galpha, ga);
   }
 }
 
@@ -418,10 +229,6 @@ __global__ __launch_bounds__(256) void k_mean(const float* lpath, int64_t npix, 
   rgb[3 * pix + 2] = c;
 }
 
-// Walk tape: one float per walk (camera walk and one shadow walk per bounce)
-// per path, walk-major, then the low and high words of each path's mask.
-constexpr int kMaxTapeDepth = 63;
-
 int64_t tape_bytes(const flg_render_cfg* cfg) {
   if (!cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 || cfg->max_depth < 1 ||
       cfg->max_depth > kMaxTapeDepth)
@@ -431,14 +238,62 @@ int64_t tape_bytes(const flg_render_cfg* cfg) {
   return npaths * 4 * ((int64_t)cfg->max_depth + 3);
 }
 
-int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg_vol_scene* scene,
-            const flg_render_cfg* cfg, const flg_diff_params* prm, const void* tape, DiffDev* D,
-            LightsDev* L, HaltonTables* tab, HaltonParams* hp, CamDevV* cam) {
+// Active bricks: one block per brick, active iff one of its voxels is
+// non-zero (flag written into the slot array).
+__global__ __launch_bounds__(256) void k_brick_flags(const float* rho, int32_t nx, int32_t ny, int32_t nz,
+                                                     int32_t b, int32_t nbx, int32_t nby, int32_t* slot) {
+  const int64_t br = blockIdx.x;
+  const int bx = (int)(br % nbx), by = (int)((br / nbx) % nby), bz = (int)(br / ((int64_t)nbx * nby));
+  const int nvox = b * b * b;
+  int any = 0;
+  for (int v = threadIdx.x; v < nvox; v += blockDim.x) {
+    const int ix = bx * b + v % b, iy = by * b + (v / b) % b, iz = bz * b + v / (b * b);
+    if (ix < nx && iy < ny && iz < nz && rho[((int64_t)iz * ny + iy) * nx + ix] != 0.0f) any = 1;
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) slot[br] = any;
+}
+
+// Flags to slots in brick order (-1 inactive), in one 1024-lane block; the
+// number of active bricks into *count.
+__global__ __launch_bounds__(1024) void k_brick_slots(int32_t* slot, int64_t nb, int32_t* count) {
+  __shared__ int32_t wsum[16];
+  __shared__ int32_t base;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int64_t c = 0; c < nb; c += 1024) {
+    const int64_t i = c + threadIdx.x;
+    const int f = i < nb ? slot[i] : 0;
+    const uint64_t m = __ballot(f != 0);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wid] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    if (i < nb) slot[i] = f ? off + rank : -1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < 16; ++w) tot += wsum[w];
+      base += tot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = base;
+}
+
+}  // namespace
+
+int diff_prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg_vol_scene* scene,
+                 const flg_render_cfg* cfg, const flg_diff_params* prm, const void* tape, const float* bounds,
+                 DiffDev* D, LightsDev* L, HaltonTables* tab, HaltonParams* hp, CamDevV* cam) {
   if (!ctx || !vol || !scene || !cfg || !prm || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
       cfg->max_depth < 1)
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: bad arguments");
   if (cfg->schedule < 0 || cfg->schedule > 2)
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: schedule must be 0, 1 or 2");
+  if (cfg->film != 0) return set_error(ctx, FLG_ERR_INVALID, "differentiable render: film must be 0");
   if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
     return set_error(ctx, FLG_ERR_RANGE, "differentiable render: too many paths");
   if (proposal && (proposal->nx != vol->nx || proposal->ny != vol->ny || proposal->nz != vol->nz))
@@ -450,9 +305,12 @@ int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg
     return set_error(ctx, FLG_ERR_RANGE, "differentiable render: spp * Halton stride must stay below 2^24");
   int st = get_halton_tables(ctx, tab);
   if (st) return st;
-  if ((st = make_lights(ctx, vol, scene, L))) return st;
-  D->v = make_voldev(vol);
-  D->vp = proposal ? make_voldev(proposal) : D->v;
+  if ((st = make_lights(ctx, vol, scene, L, bounds))) return st;
+  const float* b = bounds ? bounds : vol->bounds;
+  D->v = voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, b, vol->medium, vol->max_density);
+  D->vp = proposal ? voldev_of(proposal->grid, proposal->nx, proposal->ny, proposal->nz, b, proposal->medium,
+                               proposal->max_density)
+                   : D->v;
   D->same = proposal == nullptr || proposal == vol;
   D->s = prm->sigma_t_scale;
   D->alpha = prm->albedo;
@@ -462,6 +320,8 @@ int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg
   D->npaths = (uint32_t)npaths;
   D->tape = nullptr;
   D->mask_lo = D->mask_hi = nullptr;
+  D->brick_slot = nullptr;
+  D->brick_shift = D->nbx = D->nby = 0;
   if (tape) {
     if (cfg->max_depth > kMaxTapeDepth)
       return set_error(ctx, FLG_ERR_RANGE, "differentiable render: a walk tape needs max_depth <= 63");
@@ -473,7 +333,19 @@ int prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, const flg
   return FLG_OK;
 }
 
-}  // namespace
+
+int diff_set_bricks(flg_ctx* ctx, const flg_brick_layout* bl, DiffDev* D) {
+  if (!bl) return FLG_OK;
+  int sh = 0;
+  while (sh <= 5 && (1 << sh) != bl->brick) ++sh;
+  if (sh > 5 || !bl->d_slot || bl->n_active < 0)
+    return set_error(ctx, FLG_ERR_INVALID, "gradient bricks: brick must be 1, 2, 4, 8, 16 or 32 voxels, with a slot map");
+  D->brick_slot = bl->d_slot;
+  D->brick_shift = sh;
+  D->nbx = (D->v.nx + bl->brick - 1) >> sh;
+  D->nby = (D->v.ny + bl->brick - 1) >> sh;
+  return FLG_OK;
+}
 
 }  // namespace flg
 
@@ -483,6 +355,26 @@ extern "C" {
 
 int64_t flg_volpath_diff_tape_bytes(const flg_render_cfg* cfg) { return tape_bytes(cfg); }
 
+int flg_bricks_build(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz, int32_t brick,
+                     int32_t* d_slot, int32_t* h_n_active) {
+  if (!ctx || !d_density || !d_slot || !h_n_active || nx <= 0 || ny <= 0 || nz <= 0)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: bad arguments");
+  if (brick != 1 && brick != 2 && brick != 4 && brick != 8 && brick != 16 && brick != 32)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: brick must be 1, 2, 4, 8, 16 or 32");
+  const int32_t nbx = (nx + brick - 1) / brick, nby = (ny + brick - 1) / brick, nbz = (nz + brick - 1) / brick;
+  const int64_t nb = (int64_t)nbx * nby * nbz;
+  if (nb >= ((int64_t)1 << 31)) return set_error(ctx, FLG_ERR_RANGE, "flg_bricks_build: too many bricks");
+  int st = FLG_OK;
+  int32_t* count = (int32_t*)scratch(ctx, "brick_count", 256, &st);
+  if (!count) return st;
+  k_brick_flags<<<(unsigned)nb, 256, 0, ctx->stream>>>(d_density, nx, ny, nz, brick, nbx, nby, d_slot);
+  k_brick_slots<<<1, 1024, 0, ctx->stream>>>(d_slot, nb, count);
+  FLG_CHECK_LAUNCH(ctx);
+  FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_n_active, count, 4, hipMemcpyDeviceToHost, ctx->stream));
+  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FLG_OK;
+}
+
 int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                              const flg_vol_scene* scene, const flg_render_cfg* cfg,
                              const flg_diff_params* params, float* d_rgb, float* d_lpath, void* d_tape) {
@@ -491,7 +383,7 @@ int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* pr
   HaltonTables tab;
   HaltonParams hp;
   CamDevV cam;
-  int st = prepare(ctx, vol, proposal, scene, cfg, params, d_tape, &D, &L, &tab, &hp, &cam);
+  int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
   if (st) return st;
   if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_diff_forward: no output");
   const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
@@ -518,7 +410,7 @@ int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposa
   HaltonTables tab;
   HaltonParams hp;
   CamDevV cam;
-  int st = prepare(ctx, vol, proposal, scene, cfg, params, d_tape, &D, &L, &tab, &hp, &cam);
+  int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
   if (st) return st;
   if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_adjoint: null buffer");

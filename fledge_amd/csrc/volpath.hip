@@ -19,8 +19,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include "diff.hpp"
 #include "traverse.hpp"
-#include "volume.hpp"
 
 namespace flg {
 
@@ -201,6 +201,223 @@ __global__ __launch_bounds__(256) void k_vp_mean(const float* lpath, int64_t npi
   rgb[3 * pix + 2] = c;
 }
 
+// ---------------------------------------------------------------------------
+// The differentiable estimator (diff.hpp) over the same scene: VolPath's path
+// structure and draw order (light sample, then phase sample; no roulette),
+// differential delta tracking inside the medium up to the next interface,
+// and each shadow ray's ratio track through the interfaces as VolEstimateTr
+// walks it, on one walk stream whose draws continue from segment to segment.
+
+// The ratio track of walk `key` from a medium vertex along (o, d) up to tmax
+// (product of its factors; nothing after the last interface, as
+// VolEstimateTr).
+template <bool COMPAT>
+FLG_D Track mesh_track_measure(const TraceGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
+  Track r{1.0f, 0};
+  uint32_t n = 0;
+  bool inside = true;
+  float t_max = tmax;
+  for (int k = 0; k < kMaxCrossings; ++k) {
+    float t;
+    V3 ng;
+    if (mesh_hit<COMPAT>(g, o, d, t_max, &t, &ng) < 0) return r;
+    if (inside && !seg_measure(D, key, n, o, d, t, r)) return r;
+    t_max -= t;
+    if (t_max < 0.0f) return r;
+    const V3 ph = o + d * t;
+    o = offset_ray_origin(ph, ng, d);
+    d = normalize(d);
+    inside = dot(d, ng) < 0.0f;
+  }
+  return r;
+}
+
+// Second walk of the same track: its derivative, times scale, deposited.
+template <bool COMPAT>
+FLG_D void mesh_track_deposit(const TraceGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax,
+                              const Track& r, float scale, float* grad) {
+  uint32_t n = 0;
+  bool inside = true;
+  float t_max = tmax;
+  for (int k = 0; k < kMaxCrossings; ++k) {
+    float t;
+    V3 ng;
+    if (mesh_hit<COMPAT>(g, o, d, t_max, &t, &ng) < 0) return;
+    if (inside && !seg_deposit(D, key, n, o, d, t, r, scale, grad)) return;
+    t_max -= t;
+    if (t_max < 0.0f) return;
+    const V3 ph = o + d * t;
+    o = offset_ray_origin(ph, ng, d);
+    d = normalize(d);
+    inside = dot(d, ng) < 0.0f;
+  }
+}
+
+struct DiffVPath {
+  PathSampler ps;
+  V3 o, d;
+  float beta, acc, delta, L_total;
+  int bounces, crossings;
+  bool inside;
+  uint32_t path;
+  uint64_t mask;  // tape bits of the path's walks with exactly one zero factor
+};
+
+// Camera ray of path p; the adjoint (ADJ) also takes the path's radiance and
+// adjoint weight.  Returns true when the path has nothing to do.
+template <bool ADJ>
+FLG_D bool vpd_begin(const DiffDev& D, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
+                     int32_t spp, uint32_t p, const float* lpath, const float* dl_drgb, DiffVPath& s) {
+  const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.acc = 0.0f;
+  s.path = p;
+  s.mask = 0;
+  s.delta = s.L_total = 0.0f;
+  if (ADJ) {
+    const float* gp = dl_drgb + 3 * (int64_t)pix;
+    s.delta = (gp[0] + gp[1] + gp[2]) / (float)spp;
+    if (s.delta == 0.0f) return true;
+    s.L_total = lpath[p];
+    if (D.tape) s.mask = ((uint64_t)D.mask_hi[p] << 32) | D.mask_lo[p];
+  }
+  s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
+  s.ps.dim = 0;
+  const float ux = s.ps.next(tab, hp);
+  const float uy = s.ps.next(tab, hp);
+  gen_ray_v(cam, (float)x + ux, (float)y + uy, &s.o, &s.d);
+  s.beta = 1.0f;
+  s.bounces = 0;
+  s.crossings = 0;
+  s.inside = false;
+  return false;
+}
+
+// One trip of VolPath's loop; returns true when the path is complete.
+template <bool ADJ, bool COMPAT>
+FLG_D bool vpd_step(const TraceGeom& g, const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
+                    const HaltonParams& hp, int max_depth, float* grad, float* galpha, DiffVPath& s) {
+  float t;
+  V3 ng;
+  const bool hit = mesh_hit<COMPAT>(g, s.o, s.d, kInf, &t, &ng) >= 0;
+  if (s.bounces == 0 && !hit && L.env.present) s.acc += s.beta * L.env.radiance;
+  if (!hit || s.bounces >= max_depth) return true;
+  V3 pos;
+  if (s.inside && diff_delta_track<ADJ>(D, tab, hp, s.ps, s.o, s.d, t, s.beta, s.acc, s.delta, s.L_total, grad,
+                                        galpha, &pos)) {
+    const V3 wo = -s.d;
+    V3 p_light;
+    const float K = nee_sample(D, L, tab, hp, s.ps, pos, wo, &p_light);
+    if (K != 0.0f) {
+      const V3 diff = p_light - pos;
+      const float nrm = sqrtf(sqnorm(diff));
+      const V3 dd = diff / nrm;
+      const uint32_t w = (uint32_t)s.bounces + 1;
+      const uint32_t key = walk_key(s.path, w);
+      const Track tr = track<ADJ>(D, s.path, w, s.mask, [&] {
+        return mesh_track_measure<COMPAT>(g, D, key, pos, dd, nrm - SHADOW_EPS);
+      });
+      if (ADJ && track_live(tr) && s.delta * s.beta * K != 0.0f)
+        mesh_track_deposit<COMPAT>(g, D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
+      s.acc += s.beta * K * track_value(tr);
+    }
+    const float u = s.ps.next(tab, hp);
+    const float vv = s.ps.next(tab, hp);
+    const V3 wi = hg_sample(wo, u, vv, D.g);
+    if (s.beta == 0.0f) return true;
+    s.o = pos;  // VInteraction::SpawnRay: no normal, no offset
+    s.d = normalize(wi);
+    ++s.bounces;
+    return false;
+  }
+  // an interface: cross it
+  if (++s.crossings > kMaxCrossings) return true;
+  const V3 ph = s.o + s.d * t;
+  s.o = offset_ray_origin(ph, ng, s.d);
+  s.d = normalize(s.d);
+  s.inside = dot(s.d, ng) < 0.0f;
+  return false;
+}
+
+// Forward: the path's radiance and its tape bits.
+FLG_D void vpd_finish(const DiffDev& D, uint32_t p, const DiffVPath& s, float* lpath) {
+  lpath[p] = s.acc;
+  if (D.tape) {
+    D.mask_lo[p] = (uint32_t)s.mask;
+    D.mask_hi[p] = (uint32_t)(s.mask >> 32);
+  }
+}
+
+// One lane per path sample.
+template <bool ADJ, bool COMPAT>
+__global__ __launch_bounds__(256) void k_vpdiff(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
+                                                HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
+                                                float* lpath, const float* dl_drgb, float* grad, float* galpha) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  float ga = 0.0f;
+  if (p < npaths) {
+    DiffVPath s;
+    bool done = vpd_begin<ADJ>(D, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, s);
+    while (!done) done = vpd_step<ADJ, COMPAT>(g, D, L, tab, hp, max_depth, grad, &ga, s);
+    if (!ADJ) vpd_finish(D, (uint32_t)p, s, lpath);
+  }
+  reduce_galpha<ADJ>(ga, galpha);
+}
+
+// Persistent lanes with path regeneration (regen_loop); same per-path code.
+template <bool ADJ, bool COMPAT>
+__global__ __launch_bounds__(256) void k_vpdiff_regen(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
+                                                      HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
+                                                      float* lpath, const float* dl_drgb, float* grad,
+                                                      float* galpha, unsigned int* next_path) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
+  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
+  float ga = 0.0f;
+  regen_loop<DiffVPath>(
+      npaths, next_path,
+      [&](uint32_t p, DiffVPath& s) { return vpd_begin<ADJ>(D, tab, hp, cam, spp, p, lpath, dl_drgb, s); },
+      [&](DiffVPath& s) { return vpd_step<ADJ, COMPAT>(g, D, L, tab, hp, max_depth, grad, &ga, s); },
+      [&](uint32_t p, const DiffVPath& s) {
+        if (!ADJ) vpd_finish(D, p, s, lpath);
+      });
+  reduce_galpha<ADJ>(ga, galpha);
+}
+
+template <bool ADJ, bool COMPAT>
+int launch_vpdiff(flg_ctx* ctx, const flg_render_cfg* cfg, const TraceGeom& g, const DiffDev& D,
+                  const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
+                  float* lpath, const float* dl, float* grad, float* galpha) {
+  const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+  const int sched = cfg->schedule ? cfg->schedule : 2;
+  if (sched == 1) {
+    k_vpdiff<ADJ, COMPAT><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp,
+                                                                        cfg->max_depth, lpath, dl, grad, galpha);
+    return FLG_OK;
+  }
+  int st = FLG_OK;
+  unsigned long long* counter =
+      (unsigned long long*)scratch(ctx, ADJ ? "vpdiff_counter_adj" : "vpdiff_counter_fwd", 256, &st);
+  if (!counter) return st;
+  k_vp_zero<<<1, 64, 0, ctx->stream>>>(counter);
+  const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+  k_vpdiff_regen<ADJ, COMPAT><<<grid, 256, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp, cfg->max_depth,
+                                                             lpath, dl, grad, galpha, (unsigned int*)counter);
+  return FLG_OK;
+}
+
+// The mesh's checks shared by the entry points over an interface mesh.
+int check_mesh(flg_ctx* ctx, const flg_bvh* bvh, const char* who) {
+  if (!bvh || bvh->n <= 0) return set_error(ctx, FLG_ERR_INVALID, who);
+  const float* mb = bvh->bound;
+  if (!(mb[3] > mb[0] && mb[4] > mb[1] && mb[5] > mb[2]))
+    return set_error(ctx, FLG_ERR_INVALID, "interface mesh: the mesh's bounding box is flat");
+  return FLG_OK;
+}
+
 template <bool COMPAT, bool COUNT>
 void launch_volpath(int sched, int blocks1, int blocks2, hipStream_t st, const TraceGeom& g, const VolDev& v,
                     const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
@@ -230,15 +447,13 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
   const int sched = cfg->schedule ? cfg->schedule : 2;
   if (sched < 1 || sched > 2)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: schedule must be 0, 1 or 2");
-  if (bvh->n <= 0) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: empty mesh");
+  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_render: empty mesh");
+  if (st) return st;
   const float* mb = bvh->bound;
-  if (!(mb[3] > mb[0] && mb[4] > mb[1] && mb[5] > mb[2]))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: the mesh's bounding box is flat");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay below 2^24");
   const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
-  int st = FLG_OK;
   char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)npaths * 4 + 256, &st);
   if (!sc) return st;
   unsigned long long* nl = (unsigned long long*)sc;
@@ -281,6 +496,67 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     h_stats[0] = (uint64_t)npaths;
     h_stats[1] = c;
   }
+  return FLG_OK;
+}
+
+int flg_volpath_mesh_diff_forward(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, const flg_volume* proposal,
+                                  const flg_vol_scene* scene, const flg_render_cfg* cfg,
+                                  const flg_diff_params* params, float* d_rgb, float* d_lpath, void* d_tape) {
+  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_diff_forward: no mesh");
+  if (st) return st;
+  DiffDev D;
+  LightsDev L;
+  HaltonTables tab;
+  HaltonParams hp;
+  CamDevV cam;
+  if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
+    return st;
+  if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_diff_forward: no output");
+  const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
+  float* lpath = d_lpath;
+  if (!lpath) {
+    lpath = (float*)scratch(ctx, "vpdiff_lpath", (size_t)npaths * 4, &st);
+    if (!lpath) return st;
+  }
+  const TraceGeom g = geom_of(bvh);
+  {
+    ScopedTimer tm(ctx, "vpdiff_forward");
+    st = (bvh->flags & FLG_NORMAL_COMPAT)
+             ? launch_vpdiff<false, true>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr)
+             : launch_vpdiff<false, false>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr);
+    if (st) return st;
+    if (d_rgb) k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  return FLG_OK;
+}
+
+int flg_volpath_mesh_adjoint(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, const flg_volume* proposal,
+                             const flg_vol_scene* scene, const flg_render_cfg* cfg, const flg_diff_params* params,
+                             const float* d_lpath, const void* d_tape, const float* d_dl_drgb,
+                             const flg_brick_layout* bricks, float* d_grad, float* d_grad_albedo) {
+  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_adjoint: no mesh");
+  if (st) return st;
+  DiffDev D;
+  LightsDev L;
+  HaltonTables tab;
+  HaltonParams hp;
+  CamDevV cam;
+  if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
+    return st;
+  if ((st = diff_set_bricks(ctx, bricks, &D))) return st;
+  if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_adjoint: null buffer");
+  const TraceGeom g = geom_of(bvh);
+  {
+    ScopedTimer tm(ctx, "vpdiff_adjoint");
+    float* lp = (float*)d_lpath;
+    st = (bvh->flags & FLG_NORMAL_COMPAT)
+             ? launch_vpdiff<true, true>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo)
+             : launch_vpdiff<true, false>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo);
+    if (st) return st;
+  }
+  FLG_CHECK_LAUNCH(ctx);
   return FLG_OK;
 }
 

@@ -107,6 +107,90 @@ def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, gr
     return grad, galpha
 
 
+class BrickLayout:
+    """Active bricks of a dense grid (flg_bricks_build): a brick of `brick`
+    voxels per edge is active iff one of its voxels is non-zero.  A gradient
+    in this layout is (n_active, brick, brick, brick), brick after brick in
+    slot order."""
+
+    def __init__(self, ctx, density, brick=8):
+        nz, ny, nx = density.shape
+        self.ctx = ctx
+        self.brick = brick
+        self.shape = (nz, ny, nx)
+        self.nb = tuple((n + brick - 1) // brick for n in (nz, ny, nx))
+        self.slot = torch.empty(self.nb, dtype=torch.int32, device=torch.device("cuda", ctx.device))
+        n = ctypes.c_int32(0)
+        ctx.check(ctx.lib.flg_bricks_build(ctx.h, _ptr(density.contiguous()), nx, ny, nz, brick, _ptr(self.slot),
+                                           ctypes.byref(n)))
+        self.n_active = int(n.value)
+        self.c = N.BrickLayout()
+        self.c.brick = brick
+        self.c.n_active = self.n_active
+        self.c.d_slot = self.slot.data_ptr()
+
+    def zeros(self):
+        """A gradient buffer in this layout."""
+        b = self.brick
+        return torch.zeros((self.n_active, b, b, b), dtype=torch.float32, device=self.slot.device)
+
+    def to_dense(self, bricks):
+        """Dense (nz, ny, nx) copy of a gradient in this layout (0 on inactive
+        bricks)."""
+        b = self.brick
+        nbz, nby, nbx = self.nb
+        full = torch.zeros((nbz, nby, nbx, b, b, b), dtype=bricks.dtype, device=bricks.device)
+        act = self.slot >= 0
+        full[act] = bricks[self.slot[act].long()]
+        dense = full.permute(0, 3, 1, 4, 2, 5).reshape(nbz * b, nby * b, nbx * b)
+        nz, ny, nx = self.shape
+        return dense[:nz, :ny, :nx].contiguous()
+
+
+def mesh_forward(dv, bvh, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0,
+                 tape=None):
+    """The differentiable estimator over `bvh`'s closed mesh as the interface
+    around the medium (flg_volpath_mesh_diff_forward): image and per-path
+    radiance; with `tape` also the walk tape."""
+    ctx = dv.ctx
+    dev = torch.device("cuda", ctx.device)
+    if out is None:
+        out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
+    if lpath is None:
+        lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
+    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
+        raise ValueError("walk tape too small")
+    prop = dv.proposal.h if dv.proposal is not None else None
+    ctx.check(ctx.lib.flg_volpath_mesh_diff_forward(ctx.h, bvh.h, dv.vol.h, prop, ctypes.byref(scene),
+                                                    ctypes.byref(_cfg(w, h, spp, depth, schedule)),
+                                                    ctypes.byref(_params(s, albedo, dv.g, majorant)),
+                                                    _ptr(out), _ptr(lpath), _ptr(tape)))
+    return out, lpath
+
+
+def mesh_adjoint(dv, bvh, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None,
+                 schedule=0, tape=None, bricks=None):
+    """Adjoint of mesh_forward (flg_volpath_mesh_adjoint): adds dLoss/dDensity
+    into grad, dense or, with `bricks` (a BrickLayout), over its active
+    bricks, and dLoss/dAlbedo into galpha."""
+    ctx = dv.ctx
+    dev = torch.device("cuda", ctx.device)
+    if grad is None:
+        grad = bricks.zeros() if bricks is not None else torch.zeros(dv.vol.shape, dtype=torch.float32, device=dev)
+    if galpha is None:
+        galpha = torch.zeros(1, dtype=torch.float32, device=dev)
+    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
+        raise ValueError("walk tape too small")
+    prop = dv.proposal.h if dv.proposal is not None else None
+    dl = dl_drgb.contiguous()
+    ctx.check(ctx.lib.flg_volpath_mesh_adjoint(ctx.h, bvh.h, dv.vol.h, prop, ctypes.byref(scene),
+                                               ctypes.byref(_cfg(w, h, spp, depth, schedule)),
+                                               ctypes.byref(_params(s, albedo, dv.g, majorant)), _ptr(lpath),
+                                               _ptr(tape), _ptr(dl), ctypes.byref(bricks.c) if bricks else None,
+                                               _ptr(grad), _ptr(galpha)))
+    return grad, galpha
+
+
 class VolumeRender(torch.autograd.Function):
     """image = VolumeRender.apply(density, albedo, dv, scene, w, h, spp, depth, s, majorant)
 

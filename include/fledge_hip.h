@@ -356,6 +356,48 @@ FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume*
                                 const void* d_tape, const float* d_dl_drgb, float* d_grad,
                                 float* d_grad_albedo);
 
+/* Gradient over the active bricks of a grid (no reference counterpart; a
+ * sparse-occupancy grid whose empty bricks are not parameters).  The grid's
+ * bricks are cubes of `brick` voxels per edge, ceil(n / brick) per axis,
+ * x fastest; d_slot[brick] is the brick's index among the active bricks, -1
+ * for an inactive one.  A gradient in this layout is n_active * brick^3
+ * floats: active brick after active brick in slot order, each x fastest.
+ * Gradient terms that fall on an inactive brick are dropped. */
+typedef struct flg_brick_layout {
+  int32_t brick;          /* 1, 2, 4, 8, 16 or 32 */
+  int32_t n_active;
+  const int32_t* d_slot;  /* device, one per brick */
+} flg_brick_layout;
+
+/* Active bricks of a dense device grid (nx*ny*nz floats, x fastest): a brick
+ * is active iff one of its voxels is non-zero; slots in brick order.  Writes
+ * d_slot (ceil(nx/b)*ceil(ny/b)*ceil(nz/b) int32, device) and the number of
+ * active bricks (host). */
+FLG_API int flg_bricks_build(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz,
+                             int32_t brick, int32_t* d_slot, int32_t* h_n_active);
+
+/* The differentiable estimator of flg_volpath_diff_forward over the scene of
+ * flg_volpath_mesh_render: the mesh in `bvh` is the index-matched interface
+ * around the medium, which fills the mesh's bounding box
+ * (VolPathIntegrator::Li, src/integrator.cpp:407-510, with the reference's
+ * draw order: light sample before the phase sample, no roulette).  Each
+ * shadow ray's ratio track runs through the interfaces (VolEstimateTr,
+ * src/integrator.cpp:40-64) on one walk stream; walk b is the shadow ray of
+ * bounce b (b >= 1).  The tape has flg_volpath_diff_tape_bytes(cfg) bytes. */
+FLG_API int flg_volpath_mesh_diff_forward(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
+                                          const flg_volume* proposal, const flg_vol_scene* scene,
+                                          const flg_render_cfg* cfg, const flg_diff_params* params,
+                                          float* d_rgb, float* d_lpath, void* d_tape);
+/* Path-replay adjoint of flg_volpath_mesh_diff_forward (same arguments; its
+ * d_lpath, and d_tape when not NULL).  d_grad is dense (nx*ny*nz floats)
+ * when bricks is NULL, else in the layout of *bricks.  Adds into d_grad and
+ * d_grad_albedo with float atomics. */
+FLG_API int flg_volpath_mesh_adjoint(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
+                                     const flg_volume* proposal, const flg_vol_scene* scene,
+                                     const flg_render_cfg* cfg, const flg_diff_params* params,
+                                     const float* d_lpath, const void* d_tape, const float* d_dl_drgb,
+                                     const flg_brick_layout* bricks, float* d_grad, float* d_grad_albedo);
+
 /* --------------------------------------------------------------- film --- */
 /* Film::saveBuffer for a ".exr" name (src/film.cpp:13-49): writes h_rgb
  * (host memory, height rows of width RGB floats, row 0 first) as an OpenEXR

@@ -325,6 +325,16 @@ void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t
   volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
 }
 
+// Differentiable estimator over this mesh as the interface (arguments as
+// orc_diff_render, the medium on the mesh's bounding box).
+void orc_diff_mesh_render(void* p, int32_t compat, const double* theta, const float* theta_p, int32_t nx,
+                          int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                          int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
+                          const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups) {
+  diff_mesh_render(((Handle*)p)->b, compat != 0, theta, theta_p, nx, ny, nz, params4, lights7, cam10, W, H, spp,
+                   max_depth, lpath, img, dldpix, grad, galpha, nthreads, lookups);
+}
+
 // ------------------------------------------------------------ path trace ---
 void orc_diffuse_render(void* p, int32_t compat, const float* cam10, int32_t W, int32_t H,
                         int32_t spp, int32_t max_depth, int64_t p0, int64_t p1, int32_t* nseg,

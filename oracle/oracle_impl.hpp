@@ -143,6 +143,13 @@ V3 offset_ray_origin(V3 p, V3 n, V3 dir);  // OffsetRayOrigin (pathtrace.cpp)
 // VolPathIntegrator over a closed interface mesh (volpath_mesh.cpp)
 void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, int nthreads,
                          uint64_t* lookups);
+
+// The differentiable estimator over the interface mesh b (volume_diff.cpp):
+// forward (lpath, img) or, with dldpix, the adjoint (grad, galpha).
+void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const float* theta_p, int32_t nx,
+                      int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                      int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
+                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups);
 // the reference's render loop and film (film.cpp)
 float film_mitchell1d(float d);
 void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups,

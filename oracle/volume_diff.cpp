@@ -23,6 +23,15 @@
 // Positions, free-flight distances and decisions are computed in float
 // exactly like the GPU kernel; densities of theta, weights, radiance and
 // gradients are computed in double.
+//
+// The same estimator over an interface mesh (diff_path_mesh) follows
+// VolPathIntegrator::Li (src/integrator.cpp:407-510) as the oracle's
+// volpath_mesh.cpp restates it: closest hit on the mesh, the environment at
+// bounce 0 on escape, differential delta tracking inside the medium up to the
+// hit, the light sample before the phase sample, no roulette; each shadow
+// ray's ratio track runs through the interfaces (VolEstimateTr,
+// integrator.cpp:40-64) on walk stream b (the shadow ray of bounce b >= 1),
+// its draws continuing from one inside segment to the next.
 #include <atomic>
 #include <cmath>
 #include <thread>
@@ -84,7 +93,11 @@ Tri lookup2(const DiffScene& sc, V3 pos) {
 
 // Ratio tracking without roulette on the stream of walk `key`; records the
 // lookups and returns T.
-double ratio_track(const DiffScene& sc, uint32_t key, V3 o, V3 d, float tmax, std::vector<Tri>* rec) {
+// *n: the stream's next draw (advanced; 0 when not given).
+double ratio_track(const DiffScene& sc, uint32_t key, V3 o, V3 d, float tmax, std::vector<Tri>* rec,
+                   uint32_t* n = nullptr) {
+  uint32_t n0 = 0;
+  uint32_t& draw = n ? *n : n0;
   float t_min, t_max;
   if (!box_intersect_pbrt(sc.gp.bmin, sc.gp.bmax, o, d, tmax, &t_min, &t_max)) return 1.0;
   float t = smax(t_min, 0.0f);
@@ -93,7 +106,7 @@ double ratio_track(const DiffScene& sc, uint32_t key, V3 o, V3 d, float tmax, st
   double T = 1.0;
   const int cap = walk_cap(sc.mu_f, t, t_max);
   for (int step = 0; step < cap; ++step) {
-    t += -flg_logf(walk_draw(key, (uint32_t)step)) / sc.mu_f;
+    t += -flg_logf(walk_draw(key, draw++)) / sc.mu_f;
     if (t >= t_max) break;
     V3 pos = o + d * t;
     Tri q = lookup2(sc, pos);
@@ -110,11 +123,11 @@ struct Event {  // main-path tentative collision
 
 // Differential delta tracking along (o, d); records events; returns true on a
 // real collision (position in *pos).
-bool delta_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, V3* pos, std::vector<Event>* ev) {
+bool delta_track(const DiffScene& sc, Halton& hs, V3 o, V3 d, float tmax, V3* pos, std::vector<Event>* ev) {
   float t_min, t_max;
-  if (!box_intersect_pbrt(sc.gp.bmin, sc.gp.bmax, o, d, INFINITY, &t_min, &t_max)) return false;
+  if (!box_intersect_pbrt(sc.gp.bmin, sc.gp.bmax, o, d, tmax, &t_min, &t_max)) return false;
   float t = smax(t_min, 0.0f);
-  t_max = smin(t_max, INFINITY);
+  t_max = smin(t_max, tmax);
   if (!(sc.mu_f > 0.0f)) return false;
   const int cap = walk_cap(sc.mu_f, t, t_max);
   for (int step = 0; step < cap; ++step) {
@@ -184,6 +197,47 @@ NeeSample nee_sample(const DiffScene& sc, Halton& hs, V3 p, V3 wo) {
   return r;
 }
 
+// grad += g * trilinear weights of lookup q (dense grid, x fastest)
+void deposit(const DiffScene& sc, double* grad, const Tri& q, double g) {
+  if (!grad || g == 0.0) return;
+  for (int c = 0; c < 8; ++c) {
+    int ii = q.i + (c & 1), jj = q.j + ((c >> 1) & 1), kk = q.k + (c >> 2);
+    if (ii < 0 || jj < 0 || kk < 0 || ii >= sc.gp.nx || jj >= sc.gp.ny || kk >= sc.gp.nz) continue;
+    grad[((int64_t)kk * sc.gp.ny + jj) * sc.gp.nx + ii] += g * q.w[c];
+  }
+}
+
+// scale * d(prod f)/d rho_i for every recorded ratio-tracking lookup
+void deposit_tr(const DiffScene& sc, double* grad, const std::vector<Tri>& r, double scale) {
+  if (!grad) return;
+  size_t n = r.size();
+  std::vector<double> f(n), pre(n + 1, 1.0), suf(n + 1, 1.0);
+  for (size_t i = 0; i < n; ++i) f[i] = 1.0 - sc.s * r[i].rho / sc.mu;
+  for (size_t i = 0; i < n; ++i) pre[i + 1] = pre[i] * f[i];
+  for (size_t i = n; i-- > 0;) suf[i] = suf[i + 1] * f[i];
+  for (size_t i = 0; i < n; ++i) deposit(sc, grad, r[i], scale * (-sc.s / sc.mu) * pre[i] * suf[i + 1]);
+}
+
+// The weights of a delta-tracking walk's events into beta; with grad, each
+// event's derivative weighted by the radiance gathered after it.
+void apply_events(const DiffScene& sc, const std::vector<Event>& ev, double L, double L_total, double delta,
+                  double& beta, double* grad, double* galpha) {
+  for (const Event& e : ev) {
+    if (e.real) {
+      if (grad) {
+        double S = L_total - L;
+        if (e.q.rho != 0.0) deposit(sc, grad, e.q, delta * S / e.q.rho);
+        *galpha += delta * S / sc.alpha;
+      }
+      beta *= w_real(sc, e.q);
+    } else {
+      double fnull = 1.0 - sc.s * e.q.rho / sc.mu;
+      if (grad && fnull != 0.0) deposit(sc, grad, e.q, delta * (L_total - L) * (-sc.s / sc.mu) / fnull);
+      beta *= w_null(sc, e.q);
+    }
+  }
+}
+
 // Replays one path.  grad == nullptr: forward only (returns L).  Otherwise
 // accumulates delta * dL/dtheta into grad and delta * dL/dalpha into *galpha,
 // given the path's total L_total.
@@ -195,26 +249,9 @@ double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, dou
   if (!box_intersect(sc.gp.bmin, sc.gp.bmax, o, d, &t_min, &t_max)) return Le;
   std::vector<Tri> rec;
   std::vector<Event> ev;
-  auto deposit = [&](const Tri& q, double g) {
-    if (!grad || g == 0.0) return;
-    for (int c = 0; c < 8; ++c) {
-      int ii = q.i + (c & 1), jj = q.j + ((c >> 1) & 1), kk = q.k + (c >> 2);
-      if (ii < 0 || jj < 0 || kk < 0 || ii >= sc.gp.nx || jj >= sc.gp.ny || kk >= sc.gp.nz) continue;
-      grad[((int64_t)kk * sc.gp.ny + jj) * sc.gp.nx + ii] += g * q.w[c];
-    }
-  };
-  // d(prod f)/d rho_i * scale for every recorded ratio-tracking lookup
-  auto deposit_tr = [&](const std::vector<Tri>& r, double scale) {
-    size_t n = r.size();
-    std::vector<double> f(n), pre(n + 1, 1.0), suf(n + 1, 1.0);
-    for (size_t i = 0; i < n; ++i) f[i] = 1.0 - sc.s * r[i].rho / sc.mu;
-    for (size_t i = 0; i < n; ++i) pre[i + 1] = pre[i] * f[i];
-    for (size_t i = n; i-- > 0;) suf[i] = suf[i + 1] * f[i];
-    for (size_t i = 0; i < n; ++i) deposit(r[i], scale * (-sc.s / sc.mu) * pre[i] * suf[i + 1]);
-  };
   double T = ratio_track(sc, walk_key(path, 0), o, d, INFINITY, grad ? &rec : nullptr);
   if (lookups) *lookups += rec.size();
-  if (grad) deposit_tr(rec, delta * Le);
+  deposit_tr(sc, grad, rec, delta * Le);
   L += T * Le;
   o = o + d * t_min;
   double beta = 1.0;
@@ -222,22 +259,9 @@ double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, dou
   for (int bounces = 1;; ++bounces) {
     ev.clear();
     V3 pos;
-    bool hit = delta_track(sc, hs, o, d, &pos, &ev);
+    bool hit = delta_track(sc, hs, o, d, INFINITY, &pos, &ev);
     if (lookups) *lookups += ev.size();
-    for (const Event& e : ev) {
-      if (e.real) {
-        if (grad) {
-          double S = L_total - L;
-          if (e.q.rho != 0.0) deposit(e.q, delta * S / e.q.rho);
-          *galpha += delta * S / sc.alpha;
-        }
-        beta *= w_real(sc, e.q);
-      } else {
-        double fnull = 1.0 - sc.s * e.q.rho / sc.mu;
-        if (grad && fnull != 0.0) deposit(e.q, delta * (L_total - L) * (-sc.s / sc.mu) / fnull);
-        beta *= w_null(sc, e.q);
-      }
-    }
+    apply_events(sc, ev, L, L_total, delta, beta, grad, galpha);
     if (!hit) break;
     V3 wo = -d;
     float u = hs.get1D();
@@ -252,7 +276,7 @@ double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, dou
       double Tsh = ratio_track(sc, walk_key(path, (uint32_t)bounces), pos, dd, nrm - SHADOW_EPS,
                                grad ? &rec : nullptr);
       if (lookups) *lookups += rec.size();
-      if (grad) deposit_tr(rec, delta * beta * ns.K);
+      deposit_tr(sc, grad, rec, delta * beta * ns.K);
       L += beta * ns.K * Tsh;
     }
     if (bounces >= sc.max_depth || beta == 0.0) break;
@@ -266,27 +290,91 @@ double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, dou
   return L;
 }
 
-}  // namespace
+constexpr int kMaxCrossings = 256;
 
-}  // namespace orc
+// Ratio track of walk `key` from a medium vertex through the interfaces
+// (VolEstimateTr): the product over the inside segments up to the last
+// interface before tmax; lookups recorded.
+double mesh_ratio_track(const Lbvh& b, bool compat, const DiffScene& sc, uint32_t key, V3 o, V3 d, float tmax,
+                        std::vector<Tri>* rec) {
+  double T = 1.0;
+  uint32_t n = 0;
+  bool inside = true;
+  float t_max = tmax;
+  for (int k = 0; k < kMaxCrossings; ++k) {
+    float t = 0;
+    V3 ng{};
+    if (lbvh_trace(b, o, d, t_max, compat, &t, &ng, nullptr) < 0) return T;
+    if (inside) T *= ratio_track(sc, key, o, d, t, rec, &n);
+    t_max -= t;
+    if (t_max < 0) return T;
+    const V3 ph = o + d * t;
+    o = offset_ray_origin(ph, ng, d);
+    d = normalize(d);
+    inside = dot(d, ng) < 0;
+  }
+  return T;
+}
 
-using namespace orc;
+// One path of the estimator over the interface mesh (VolPathIntegrator's
+// loop); grad as diff_path.
+double diff_path_mesh(const Lbvh& b, bool compat, const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d,
+                      double delta, double L_total, double* grad, double* galpha, uint64_t* lookups) {
+  double L = 0.0, beta = 1.0;
+  const double Le = sc.env.present ? (double)sc.env.radiance : 0.0;
+  bool inside = false;  // the camera is outside the mesh
+  int bounces = 0, crossings = 0;
+  std::vector<Tri> rec;
+  std::vector<Event> ev;
+  while (true) {
+    float t = 0;
+    V3 ng{};
+    const bool hit = lbvh_trace(b, o, d, INFINITY, compat, &t, &ng, nullptr) >= 0;
+    if (bounces == 0 && !hit && sc.env.present) L += beta * Le;
+    if (!hit || bounces >= sc.max_depth) break;
+    bool scattered = false;
+    V3 pos{};
+    if (inside) {
+      ev.clear();
+      scattered = delta_track(sc, hs, o, d, t, &pos, &ev);
+      if (lookups) *lookups += ev.size();
+      apply_events(sc, ev, L, L_total, delta, beta, grad, galpha);
+    }
+    if (scattered) {
+      const V3 wo = -d;
+      NeeSample ns = nee_sample(sc, hs, pos, wo);
+      if (ns.valid) {
+        V3 diff = ns.p_light - pos;
+        float nrm = norm(diff);
+        V3 dd = diff / nrm;
+        rec.clear();
+        double Tsh = mesh_ratio_track(b, compat, sc, walk_key(path, (uint32_t)bounces + 1), pos, dd,
+                                      nrm - SHADOW_EPS, grad ? &rec : nullptr);
+        if (lookups) *lookups += rec.size();
+        deposit_tr(sc, grad, rec, delta * beta * ns.K);
+        L += beta * ns.K * Tsh;
+      }
+      const float u = hs.get1D();
+      const float v = hs.get1D();
+      const V3 wi = hg_sample(wo, u, v, sc.g_f);
+      if (beta == 0.0) break;
+      o = pos;  // VInteraction::SpawnRay: no normal, no offset
+      d = normalize(wi);
+      ++bounces;
+      continue;
+    }
+    if (++crossings > kMaxCrossings) break;
+    const V3 ph = o + d * t;
+    o = offset_ray_origin(ph, ng, d);
+    d = normalize(d);
+    inside = dot(d, ng) < 0;
+  }
+  return L;
+}
 
-extern "C" {
-
-// theta: evaluated grid (double, nx*ny*nz, x fastest); theta_p: proposal
-// grid (float).  params: s, alpha, g, mu.  lights as orc_volpath_render.
-// Outputs: lpath[W*H*spp] (double, may be NULL), img[W*H] (double mean over
-// samples, may be NULL).  With dldpix != NULL the adjoint runs instead:
-// grad (nx*ny*nz doubles, overwritten) and galpha receive
-// sum_pixels dldpix[pixel] * dI[pixel]/dtheta (dI = mean over samples),
-// using lpath (from a previous forward call) as the paths' totals.
-void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int32_t ny, int32_t nz,
-                     const float* bounds6, const float* params4, const float* lights7,
-                     const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
-                     double* lpath, double* img, const double* dldpix, double* grad,
-                     double* galpha, int nthreads, uint64_t* lookups) {
-  DiffScene sc;
+void setup_scene(DiffScene& sc, Camera& cam, const double* theta, const float* theta_p, int32_t nx, int32_t ny,
+                 int32_t nz, const float* bounds6, const float* params4, const float* lights7, int32_t W, int32_t H,
+                 int32_t spp, int32_t max_depth) {
   sc.gp.nx = nx; sc.gp.ny = ny; sc.gp.nz = nz;
   sc.gp.data = theta_p;
   sc.gp.bmin = v3(bounds6[0], bounds6[1], bounds6[2]);
@@ -305,12 +393,18 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
   sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
   sc.point.intensity = lights7[4];
   if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.gp.bmin, sc.gp.bmax);
-  Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
-             v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
   sc.cam = &cam;
   sc.W = W; sc.H = H; sc.spp = spp; sc.max_depth = max_depth;
+}
+
+// Every path of the image on the per-sample schedule, over nthreads threads:
+// the forward (lpath, img) or, with dldpix, the adjoint (grad, galpha).
+template <class PathFn>
+void run_paths(const DiffScene& sc, PathFn path_fn, double* lpath, double* img, const double* dldpix,
+               double* grad, double* galpha, int nthreads, uint64_t* lookups) {
+  const int W = sc.W, H = sc.H, spp = sc.spp;
   const bool adj = dldpix != nullptr;
-  const int64_t nvox = (int64_t)nx * ny * nz;
+  const int64_t nvox = (int64_t)sc.gp.nx * sc.gp.ny * sc.gp.nz;
   if (nthreads < 1) nthreads = 1;
   std::vector<std::vector<double>> gpart(adj ? nthreads : 0);
   std::vector<double> apart(nthreads, 0.0);
@@ -332,13 +426,12 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
           float ux = hs.get1D();
           float uy = hs.get1D();
           V3 o, d;
-          cam.generate((float)x + ux, (float)y + uy, W, H, &o, &d);
+          sc.cam->generate((float)x + ux, (float)y + uy, W, H, &o, &d);
           int64_t p = pix * spp + s;
           if (adj) {
-            diff_path(sc, hs, (uint32_t)p, o, d, dldpix[pix] / spp, lpath[p], gpart[t].data(), &apart[t],
-                      &lk[t]);
+            path_fn(hs, (uint32_t)p, o, d, dldpix[pix] / spp, lpath[p], gpart[t].data(), &apart[t], &lk[t]);
           } else {
-            double L = diff_path(sc, hs, (uint32_t)p, o, d, 0.0, 0.0, nullptr, nullptr, &lk[t]);
+            double L = path_fn(hs, (uint32_t)p, o, d, 0.0, 0.0, nullptr, nullptr, &lk[t]);
             if (lpath) lpath[p] = L;
             acc += L;
           }
@@ -362,6 +455,56 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
     for (auto v : lk) tot += v;
     *lookups = tot;
   }
+}
+
+}  // namespace
+
+// The estimator over the interface mesh b (the medium on b's bounding box).
+void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const float* theta_p, int32_t nx,
+                      int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                      int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
+                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups) {
+  const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
+  DiffScene sc;
+  Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
+             v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  run_paths(
+      sc,
+      [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {
+        return diff_path_mesh(b, compat, sc, hs, p, o, d, delta, Lt, g, ga, lk);
+      },
+      lpath, img, dldpix, grad, galpha, nthreads, lookups);
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// theta: evaluated grid (double, nx*ny*nz, x fastest); theta_p: proposal
+// grid (float).  params: s, alpha, g, mu.  lights as orc_volpath_render.
+// Outputs: lpath[W*H*spp] (double, may be NULL), img[W*H] (double mean over
+// samples, may be NULL).  With dldpix != NULL the adjoint runs instead:
+// grad (nx*ny*nz doubles, overwritten) and galpha receive
+// sum_pixels dldpix[pixel] * dI[pixel]/dtheta (dI = mean over samples),
+// using lpath (from a previous forward call) as the paths' totals.
+void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int32_t ny, int32_t nz,
+                     const float* bounds6, const float* params4, const float* lights7,
+                     const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
+                     double* lpath, double* img, const double* dldpix, double* grad,
+                     double* galpha, int nthreads, uint64_t* lookups) {
+  DiffScene sc;
+  Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
+             v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  run_paths(
+      sc,
+      [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {
+        return diff_path(sc, hs, p, o, d, delta, Lt, g, ga, lk);
+      },
+      lpath, img, dldpix, grad, galpha, nthreads, lookups);
 }
 
 }  // extern "C"

@@ -46,6 +46,8 @@ _SIGS = {
     "orc_volpath_mesh_render": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P,
                                        ctypes.c_int, P]),
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
+    "orc_diff_mesh_render": (None, [P, i32, P, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
+                                    ctypes.c_int, P]),
 }
 
 _lib = None
@@ -134,6 +136,40 @@ class Lbvh:
                                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
                                       nthreads or NTHREADS, _p(lk))
         return rgb, int(lk[0])
+
+    def diff_render(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, compat=False,
+                    nthreads=None):
+        """FP64 differentiable forward over this mesh as the interface around
+        the medium (on the mesh's bounding box): (per-path radiance, per-pixel
+        mean, lookups)."""
+        theta = np.ascontiguousarray(theta, np.float64)
+        theta_p = np.ascontiguousarray(theta_p, np.float32)
+        nz, ny, nx = theta_p.shape
+        lp = np.zeros(W * H * spp, np.float64)
+        img = np.zeros(W * H, np.float64)
+        lk = np.zeros(1, np.uint64)
+        lib().orc_diff_mesh_render(self.h, 1 if compat else 0, _p(theta), _p(theta_p), nx, ny, nz,
+                                   _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                   _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), _p(img),
+                                   None, None, None, nthreads or NTHREADS, _p(lk))
+        return lp, img.reshape(H, W), int(lk[0])
+
+    def diff_grad(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, lpath, dldpix,
+                  compat=False, nthreads=None):
+        """FP64 analytic gradient of sum(dldpix * image) over this mesh: (dense
+        gradient (nz, ny, nx), d/dalbedo)."""
+        theta = np.ascontiguousarray(theta, np.float64)
+        theta_p = np.ascontiguousarray(theta_p, np.float32)
+        nz, ny, nx = theta_p.shape
+        grad = np.zeros(nx * ny * nz, np.float64)
+        ga = np.zeros(1, np.float64)
+        dl = np.ascontiguousarray(dldpix, np.float64).ravel()
+        lp = np.ascontiguousarray(lpath, np.float64)
+        lib().orc_diff_mesh_render(self.h, 1 if compat else 0, _p(theta), _p(theta_p), nx, ny, nz,
+                                   _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                   _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), None,
+                                   _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None)
+        return grad.reshape(nz, ny, nx), float(ga[0])
 
     def __del__(self):
         try:

@@ -1,0 +1,172 @@
+"""The differentiable estimator over a closed interface mesh (the combined
+scene's forward and adjoint) on the GPU against the FP64 twin of the same
+estimator: forward per path, gradient dense and over the active bricks of a
+sparse-occupancy grid, the walk tape, both schedules."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as orc
+from fledge_amd import _native as N
+from fledge_amd import api, diff
+
+pytestmark = pytest.mark.gpu
+
+CAM0 = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+LIGHTS = [1.0, 2.0, 2.0, -2.0, 50.0, 1.0, 0.5]
+S, ALBEDO, G = 8.0, 0.8, 0.3
+
+
+def _sphere(level=4):
+    v, ind = api.gen_octasphere(level, 0.05, seed=7, base_freq=2.0)
+    return (v * 0.5 + 0.5).astype(np.float32), ind
+
+
+def _grid(n, seed):
+    return (0.1 + 0.8 * api.gen_fbm_grid(n, seed=seed)).astype(np.float32)
+
+
+def _sparse_grid(n, brick, p, seed):
+    """fBm in the bricks that a coin of probability p keeps, 0 elsewhere."""
+    g = _grid(n, seed)
+    nb = n // brick
+    keep = np.random.default_rng(seed).random((nb, nb, nb)) < p
+    mask = np.repeat(np.repeat(np.repeat(keep, brick, 0), brick, 1), brick, 2)
+    return (g * mask).astype(np.float32), keep
+
+
+def _setup(ctx, v, ind, grid):
+    bvh = api.LBVH.build(ctx, torch.from_numpy(v).cuda(), torch.from_numpy(ind).cuda())
+    dv = diff.DiffVolume(ctx, torch.from_numpy(grid).cuda(), g=G)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    return bvh, dv, api.vol_scene(cam)
+
+
+def test_mesh_diff_forward_matches_fp64_twin(ctx):
+    v, ind = _sphere(4)
+    grid = _grid(20, 5)
+    W, H, spp, D = 40, 32, 4, 16
+    mu = S * float(grid.max())
+    print("mesh diff forward %dx%d spp %d" % (W, H, spp), flush=True)
+    bvh, dv, scene = _setup(ctx, v, ind, grid)
+    img, lpath = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu)
+    lp64, img64, _ = orc.Lbvh(v, ind).diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS,
+                                                  CAM0, W, H, spp, D)
+    got = img.cpu().numpy()[:, :, 0]
+    rel = np.abs(got - img64) / np.maximum(np.abs(img64), 1e-6)
+    assert rel.max() < 1e-4
+    # per path, relative to the path's value or, for paths far darker than
+    # average, to the mean path (float and double products round apart there)
+    lp = lpath.cpu().numpy().astype(np.float64)
+    scale = max(float(np.abs(lp64).mean()), 1e-6)
+    rel_p = np.abs(lp - lp64) / np.maximum(np.abs(lp64), scale)
+    assert rel_p.max() < 1e-4, rel_p.max()
+
+
+def _adjoint(ctx, v, ind, grid, W, H, spp, D, seed, bricks_of=None, schedule=0):
+    mu = S * float(grid.max())
+    bvh, dv, scene = _setup(ctx, v, ind, grid)
+    tape = diff.new_tape(ctx, W, H, spp, D)
+    img, lpath = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, tape=tape, schedule=schedule)
+    dl = np.random.default_rng(seed).normal(size=(H, W, 3)).astype(np.float32)
+    dlt = torch.from_numpy(dl).cuda()
+    grad, ga = diff.mesh_adjoint(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, lpath, dlt, tape=tape,
+                                 schedule=schedule)
+    out = {"bvh": bvh, "dv": dv, "scene": scene, "mu": mu, "tape": tape, "lpath": lpath, "dl": dl, "dlt": dlt,
+           "grad": grad.cpu().numpy().astype(np.float64), "galpha": float(ga.item())}
+    if bricks_of is not None:
+        layout = diff.BrickLayout(ctx, torch.from_numpy(grid).cuda(), bricks_of)
+        gb, gab = diff.mesh_adjoint(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, lpath, dlt, tape=tape,
+                                    bricks=layout, schedule=schedule)
+        out.update(layout=layout, gbricks=gb, galpha_b=float(gab.item()))
+    return out
+
+
+def test_mesh_adjoint_matches_fp64_gradient(ctx):
+    v, ind = _sphere(4)
+    grid = _grid(20, 7)
+    W, H, spp, D = 40, 32, 4, 16
+    print("mesh adjoint dense %dx%d spp %d" % (W, H, spp), flush=True)
+    r = _adjoint(ctx, v, ind, grid, W, H, spp, D, 7)
+    mu = r["mu"]
+    m = orc.Lbvh(v, ind)
+    lp64, _, _ = m.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D)
+    g64, ga64 = m.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D, lp64,
+                            r["dl"].astype(np.float64).sum(axis=2))
+    g = r["grad"]
+    assert np.abs(g64).max() > 0
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 1e-3
+    big = np.abs(g64) > 1e-2 * np.abs(g64).max()
+    assert (np.abs(g - g64)[big] / np.abs(g64)[big]).max() < 1e-3
+    assert abs(r["galpha"] - ga64) / abs(ga64) < 1e-3
+
+
+def test_mesh_adjoint_over_active_bricks(ctx):
+    """A 32^3 grid of 8^3 bricks, each kept with probability 0.4: the slot
+    map against numpy, and the gradient over the active bricks equal to the
+    dense gradient on those bricks (float atomics: summation order only) and
+    to the FP64 gradient there."""
+    v, ind = _sphere(4)
+    grid, keep = _sparse_grid(32, 8, 0.4, 3)
+    W, H, spp, D = 40, 32, 4, 16
+    print("mesh adjoint bricks %dx%d spp %d" % (W, H, spp), flush=True)
+    r = _adjoint(ctx, v, ind, grid, W, H, spp, D, 9, bricks_of=8)
+    layout = r["layout"]
+    slots = layout.slot.cpu().numpy()
+    want = np.full(keep.shape, -1, np.int32)
+    want[keep] = np.arange(int(keep.sum()), dtype=np.int32)  # brick order, x fastest
+    assert layout.n_active == int(keep.sum()) and np.array_equal(slots, want)
+    gb = r["gbricks"]
+    assert tuple(gb.shape) == (layout.n_active, 8, 8, 8)
+    dense_b = layout.to_dense(gb).cpu().numpy().astype(np.float64)
+    mask = np.repeat(np.repeat(np.repeat(keep, 8, 0), 8, 1), 8, 2)
+    g = r["grad"]
+    scale = np.abs(g).max()
+    assert scale > 0
+    assert np.abs(dense_b[~mask]).max() == 0
+    assert np.abs(dense_b[mask] - g[mask]).max() <= 1e-5 * scale
+    assert abs(r["galpha_b"] - r["galpha"]) <= 1e-5 * abs(r["galpha"])
+    mu = r["mu"]
+    m = orc.Lbvh(v, ind)
+    lp64, _, _ = m.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D)
+    g64, _ = m.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D, lp64,
+                         r["dl"].astype(np.float64).sum(axis=2))
+    gm, g64m = dense_b[mask], g64[mask]
+    assert np.linalg.norm(gm - g64m) / np.linalg.norm(g64m) < 1e-3
+
+
+def test_mesh_diff_schedules_and_tape(ctx):
+    """Both schedules give the same forward bits; the adjoint that reads the
+    walk tape and the one that walks every track twice agree (summation
+    order only)."""
+    v, ind = _sphere(4)
+    grid = _grid(20, 11)
+    W, H, spp, D = 40, 32, 4, 16
+    r = _adjoint(ctx, v, ind, grid, W, H, spp, D, 11, schedule=1)
+    bvh, dv, scene, mu = r["bvh"], r["dv"], r["scene"], r["mu"]
+    _, lp2 = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, schedule=2)
+    assert torch.equal(r["lpath"].view(torch.int32), lp2.view(torch.int32))
+    g_nt, ga_nt = diff.mesh_adjoint(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, r["lpath"], r["dlt"])
+    g = r["grad"]
+    assert np.abs(g_nt.cpu().numpy() - g).max() <= 1e-5 * np.abs(g).max()
+    assert abs(float(ga_nt.item()) - r["galpha"]) <= 1e-5 * abs(r["galpha"])
+
+
+def test_mesh_diff_rejects_bad_bricks(ctx):
+    v, ind = _sphere(3)
+    grid = _grid(16, 2)
+    bvh, dv, scene = _setup(ctx, v, ind, grid)
+    W, H, spp, D = 8, 8, 1, 4
+    mu = S * float(grid.max())
+    _, lpath = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu)
+    bad = N.BrickLayout()
+    bad.brick, bad.n_active, bad.d_slot = 3, 0, None
+    g = torch.zeros(16, dtype=torch.float32, device="cuda")
+    ga = torch.zeros(1, dtype=torch.float32, device="cuda")
+    dl = torch.ones((H, W, 3), dtype=torch.float32, device="cuda")
+    import ctypes
+    st = ctx.lib.flg_volpath_mesh_adjoint(ctx.h, bvh.h, dv.vol.h, None, ctypes.byref(scene),
+                                          ctypes.byref(diff._cfg(W, H, spp, D)),
+                                          ctypes.byref(diff._params(S, ALBEDO, G, mu)), lpath.data_ptr(), None,
+                                          dl.data_ptr(), ctypes.byref(bad), g.data_ptr(), ga.data_ptr())
+    assert st != 0
