@@ -209,7 +209,34 @@ def main():
         secondary["batched_build_256_meshes_ms"] = round(tb * 1e3, 3)
         secondary["batched_build_total_tris"] = int(sizes.sum())
         secondary["batched_build_mtris_per_s"] = round(int(sizes.sum()) / tb / 1e6, 2)
-        del meshes
+        # the scene accel over the same 256 meshes, each moved to a cell of a
+        # 16 x 16 grid (pitch 1.25), against one LBVH over all their
+        # triangles: 1024^2 camera rays from above
+        placed, voff, allv, alli = [], 0, [], []
+        for k, (vv, ii) in enumerate(meshes):
+            off = torch.tensor([1.25 * (k % 16), 1.25 * (k // 16), 0.0], dtype=torch.float32, device=dev)
+            pv = (vv.view(-1, 3) + off).reshape(-1)
+            placed.append((pv, ii))
+            allv.append(pv)
+            alli.append(ii + voff)
+            voff += pv.numel() // 3
+        blas = api.LBVH.build_batched(ctx, placed)
+        sacc = api.SceneAccel(ctx, blas)
+        cam_s = N.Camera.make((10.0, 10.0, 30.0), (10.0, 10.0, 0.0), (0.0, 1.0, 0.0), 40.0)
+        rays_s = api.camera_rays(ctx, cam_s, 1024, 1024)
+        hits_s = torch.empty((1024 * 1024, 4), dtype=torch.int32, device=dev)
+        ts_ = timed(lambda: sacc.trace(rays_s, hits_s), 10, 2, sync)
+        secondary["scene_256_meshes_closest_hit_mrays_per_s"] = round(1024 * 1024 / ts_ / 1e6, 2)
+        secondary["scene_256_meshes_hit_fraction"] = round(float((hits_s[:, 0] >= 0).float().mean()), 4)
+        merged = api.LBVH.build(ctx, torch.cat(allv), torch.cat(alli))
+        hits_m = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
+        tm_ = timed(lambda: merged.trace(rays_s, hits_m), 10, 2, sync)
+        secondary["merged_single_lbvh_closest_hit_mrays_per_s"] = round(1024 * 1024 / tm_ / 1e6, 2)
+        sacc.close()
+        merged.close()
+        for h in blas:
+            h.close()
+        del placed, allv, alli, meshes
         # volumetric path tracing: 256^3 fBm grid, 512x512, 64 spp, depth 32
         g1 = api.gen_fbm_grid(256, seed=1)
         vol = api.Volume(ctx, torch.from_numpy(g1).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)

@@ -88,6 +88,11 @@ class DiffParams(ctypes.Structure):
     _fields_ = [("sigma_t_scale", c_f), ("albedo", c_f), ("g", c_f), ("majorant", c_f)]
 
 
+class PlyMesh(ctypes.Structure):
+    _fields_ = [("nverts", c_i32), ("ntris", c_i32), ("verts", P(c_f)), ("ind", P(c_i32)), ("normals", P(c_f)),
+                ("uv", P(c_f))]
+
+
 class BrickLayout(ctypes.Structure):
     _fields_ = [("brick", c_i32), ("n_active", c_i32), ("d_slot", c_vp)]
 
@@ -128,6 +133,12 @@ _SIGS = {
                                          c_vp, c_vp, c_vp]),
     "flg_volpath_adjoint": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams), c_vp,
                                     c_vp, c_vp, c_vp, c_vp]),
+    "flg_ply_load": (c_int, [ctypes.c_char_p, P(PlyMesh)]),
+    "flg_ply_free": (None, [P(PlyMesh)]),
+    "flg_scene_accel_build": (c_int, [c_vp, P(c_vp), c_i32, P(c_vp)]),
+    "flg_scene_accel_destroy": (c_int, [c_vp]),
+    "flg_scene_accel_rank": (c_int, [c_vp, P(c_i32)]),
+    "flg_scene_trace_closest": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64]),
     "flg_bricks_build": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, P(c_i32)]),
     "flg_volpath_mesh_diff_forward": (c_int, [c_vp, c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams),
                                               c_vp, c_vp, c_vp]),

@@ -8,6 +8,7 @@ Volume         flg_volume_create / flg_volpath_render / differentiable pair
 Scene helpers  numpy generators from libfledge_scene.so
 """
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -188,6 +189,62 @@ class LBVH:
     def close(self):
         if self.h:
             self.ctx.lib.flg_lbvh_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_ply(path):
+    """MakeTriangleMesh through flg_ply_load: (verts (n, 3) float32, ind
+    (m, 3) int32, normals (n, 3) or None, uv (n, 2) or None)."""
+    lib = N.hip()
+    m = N.PlyMesh()
+    N.check(lib.flg_ply_load(os.fsencode(path), ctypes.byref(m)))
+    try:
+        def arr(ptr, n, k, dt):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(n * k,)).astype(dt, copy=True).reshape(n, k)
+        return (arr(m.verts, m.nverts, 3, np.float32), arr(m.ind, m.ntris, 3, np.int32),
+                arr(m.normals, m.nverts, 3, np.float32), arr(m.uv, m.nverts, 2, np.float32))
+    finally:
+        lib.flg_ply_free(ctypes.byref(m))
+
+
+class SceneAccel:
+    """The scene's accel over several meshes (flg_scene_accel_build): closest
+    hit over the meshes as fledge's Scene::intersect gives it.  Keeps the
+    meshes alive."""
+
+    def __init__(self, ctx, meshes):
+        self.ctx = ctx
+        self.meshes = list(meshes)
+        hs = (ctypes.c_void_p * len(self.meshes))(*[m.h.value if isinstance(m.h, ctypes.c_void_p) else m.h
+                                                     for m in self.meshes])
+        self.h = ctypes.c_void_p()
+        ctx.check(ctx.lib.flg_scene_accel_build(ctx.h, hs, len(self.meshes), ctypes.byref(self.h)))
+
+    def rank(self):
+        r = (ctypes.c_int32 * len(self.meshes))()
+        self.ctx.check(self.ctx.lib.flg_scene_accel_rank(self.h, r))
+        return list(r)
+
+    def trace(self, rays, hits=None):
+        """rays: (n, 8) float32 CUDA tensor; hits: (n, 4) int32 (mesh, id,
+        t bits, 0)."""
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        self.ctx.check(self.ctx.lib.flg_scene_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n))
+        return hits
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.flg_scene_accel_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

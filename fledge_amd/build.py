@@ -19,7 +19,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ctx.hip", "halton.hip", "lbvh.hip", "trace.hip", "volume.hip", "adjoint.hip", "volpath.hip", "film_io.hip"]
+HIP_SOURCES = ["ctx.hip", "halton.hip", "lbvh.hip", "trace.hip", "volume.hip", "adjoint.hip", "volpath.hip", "film_io.hip",
+               "scene_accel.hip", "ply_io.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -244,6 +244,12 @@ LBVHAccel::LBVHAccel(const experimental::InternalTriangleMesh& mesh, gpu::Contex
   build(v.data(), mesh.nVert, reinterpret_cast<const int32_t*>(mesh.ind), mesh.nInd / 3, flags);
 }
 
+LBVHAccel::LBVHAccel(const TriangleMesh& mesh, gpu::Context& ctx, uint32_t flags) : m_ctx(&ctx) {
+  if (mesh.nInd % 3) throw gpu::Error(FLG_ERR_INVALID, "LBVHAccel: nInd is not a multiple of 3");
+  std::vector<float> v = flatten(mesh.p, size_t(mesh.nVert));
+  build(v.data(), mesh.nVert, reinterpret_cast<const int32_t*>(mesh.ind), mesh.nInd / 3, flags);
+}
+
 void LBVHAccel::build(const float* verts, int32_t nverts, const int32_t* ind, int32_t ntris,
                       uint32_t flags) {
   m_bvh = build_bvh(*m_ctx, verts, nverts, ind, ntris, flags);
@@ -538,6 +544,124 @@ void GPUVolIntegrator::render(const Scene& scene) {
       const float* c = &rgb[(size_t(y) * scene.m_resX + x) * 3];
       scene.m_film->getBuffer(x, y, EFilmBufferType::EColor) = Vector3f(c[0], c[1], c[2]);
     }
+}
+
+// ------------------------------------------------------------------ PLY ---
+std::unique_ptr<PlyMesh> MakeTriangleMesh(const std::string& path) {
+  flg_ply_mesh m;
+  const int st = flg_ply_load(path.c_str(), &m);
+  if (st != FLG_OK) throw gpu::Error(st, "MakeTriangleMesh: cannot read " + path);
+  auto out = std::make_unique<PlyMesh>();
+  out->p.resize((size_t)m.nverts);
+  for (int32_t i = 0; i < m.nverts; ++i) out->p[i] = Vector3f(m.verts[3 * i], m.verts[3 * i + 1], m.verts[3 * i + 2]);
+  if (m.normals) {
+    out->n.resize((size_t)m.nverts);
+    for (int32_t i = 0; i < m.nverts; ++i)
+      out->n[i] = Vector3f(m.normals[3 * i], m.normals[3 * i + 1], m.normals[3 * i + 2]);
+  }
+  if (m.uv) {
+    out->uv.resize((size_t)m.nverts);
+    for (int32_t i = 0; i < m.nverts; ++i) out->uv[i] = Vector2f{{m.uv[2 * i], m.uv[2 * i + 1]}};
+  }
+  out->ind.assign(m.ind, m.ind + (size_t)m.ntris * 3);
+  flg_ply_free(&m);
+  out->mesh.nVert = (int)out->p.size();
+  out->mesh.nInd = (int)out->ind.size();
+  out->mesh.p = out->p.data();
+  out->mesh.n = out->n.empty() ? nullptr : out->n.data();
+  out->mesh.uv = out->uv.empty() ? nullptr : out->uv.data();
+  out->mesh.ind = out->ind.data();
+  return out;
+}
+
+// ---------------------------------------------------------- SceneAccel ---
+namespace {
+
+// AABB::intersect_pbrt (src/common/aabb.h:38-59)
+bool intersect_pbrt(const AABB& b, const Ray& ray, Float* t_min) {
+  Float t0 = 0, t1 = ray.m_tMax;
+  for (int i = 0; i < 3; ++i) {
+    const Float inv = 1 / ray.m_d[i];
+    Float tn = (b.m_min[i] - ray.m_o[i]) * inv, tf = (b.m_max[i] - ray.m_o[i]) * inv;
+    if (tn > tf) std::swap(tn, tf);
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (t0 > t1) return false;
+  }
+  *t_min = t0;
+  return true;
+}
+
+}  // namespace
+
+SceneAccel::SceneAccel(const std::vector<const LBVHAccel*>& meshes, gpu::Context& ctx)
+    : m_ctx(&ctx), m_meshes(meshes) {
+  if (meshes.empty()) throw gpu::Error(FLG_ERR_INVALID, "SceneAccel: no meshes");
+  std::vector<flg_bvh*> h(meshes.size());
+  for (size_t i = 0; i < meshes.size(); ++i) h[i] = meshes[i]->handle();
+  m_ctx->check(flg_scene_accel_build(m_ctx->get(), h.data(), (int32_t)h.size(), &m_accel), "flg_scene_accel_build");
+  std::vector<int32_t> rank(meshes.size());
+  m_ctx->check(flg_scene_accel_rank(m_accel, rank.data()), "flg_scene_accel_rank");
+  m_by_rank.resize(meshes.size());
+  for (size_t i = 0; i < meshes.size(); ++i) m_by_rank[rank[i]] = (int32_t)i;
+  // the root box: the default box at the origin merged with every bound
+  Vector3f lo(0.0f), hi(0.0f);
+  for (const LBVHAccel* m : meshes) {
+    const AABB b = m->getBound();
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b.m_min[k]);
+      hi[k] = std::max(hi[k], b.m_max[k]);
+    }
+  }
+  m_box = AABB(lo, hi);
+}
+
+SceneAccel::~SceneAccel() {
+  if (m_accel) flg_scene_accel_destroy(m_accel);
+}
+
+bool SceneAccel::intersect(const Ray& ray, SInteraction& isect) const {
+  return intersect(ray, isect, nullptr, nullptr);
+}
+
+bool SceneAccel::intersect(const Ray& ray, SInteraction& isect, int32_t* mesh, int32_t* tri_id) const {
+  if (mesh) *mesh = -1;
+  if (tri_id) *tri_id = -1;
+  Float t_min;
+  if (!intersect_pbrt(m_box, ray, &t_min) || ray.m_tMax < t_min) return false;
+  const Float o_tmax = ray.m_tMax;
+  Float best = o_tmax;
+  SInteraction t_isect;
+  for (int32_t m : m_by_rank) {
+    int32_t id = -1;
+    if (m_meshes[m]->intersect(ray, t_isect, &id) && ray.m_tMax < best) {
+      isect = t_isect;
+      best = ray.m_tMax;
+      if (mesh) *mesh = m;
+      if (tri_id) *tri_id = id;
+    }
+  }
+  return best != o_tmax;
+}
+
+AABB SceneAccel::getBound() const { return m_box; }
+
+void SceneAccel::intersectBatch(const flg_ray* d_rays, flg_scene_hit* d_hits, int64_t n) const {
+  m_ctx->check(flg_scene_trace_closest(m_ctx->get(), m_accel, d_rays, d_hits, n), "flg_scene_trace_closest");
+}
+
+std::vector<flg_scene_hit> SceneAccel::intersectBatch(const std::vector<Ray>& rays) const {
+  std::vector<flg_ray> r(rays.size());
+  for (size_t i = 0; i < rays.size(); ++i) r[i] = to_flg_ray(rays[i].m_o, rays[i].m_d, rays[i].m_tMax);
+  std::vector<flg_scene_hit> h(rays.size());
+  if (rays.empty()) return h;
+  gpu::DeviceBuffer dr(*m_ctx, r.data(), r.size() * sizeof(flg_ray));
+  gpu::DeviceBuffer dh(*m_ctx, h.size() * sizeof(flg_scene_hit));
+  intersectBatch(static_cast<const flg_ray*>(dr.data()), static_cast<flg_scene_hit*>(dh.data()),
+                 (int64_t)rays.size());
+  m_ctx->check(flg_memcpy_dtoh(m_ctx->get(), h.data(), dh.data(), h.size() * sizeof(flg_scene_hit)),
+               "flg_memcpy_dtoh");
+  return h;
 }
 
 }  // namespace fledge

@@ -148,6 +148,9 @@ class LBVHAccel : public Accel {
             gpu::Context& ctx = gpu::Context::global(), uint32_t flags = FLG_NORMAL_FIXED);
   LBVHAccel(const experimental::InternalTriangleMesh& mesh,
             gpu::Context& ctx = gpu::Context::global(), uint32_t flags = FLG_NORMAL_FIXED);
+  // A mesh primitive's accel (MeshPrimitive ctor, src/primitive.cpp:54-68).
+  LBVHAccel(const TriangleMesh& mesh, gpu::Context& ctx = gpu::Context::global(),
+            uint32_t flags = FLG_NORMAL_FIXED);
   ~LBVHAccel() override;
 
   bool intersect(const Ray& ray, SInteraction& isect) const override;
@@ -169,6 +172,46 @@ class LBVHAccel : public Accel {
   flg_bvh* m_bvh = nullptr;
   gpu::HostMirror m_mirror;
   AABB m_bound;
+};
+
+// MakeTriangleMesh (src/plymesh.cpp:19-86) through flg_ply_load.  The
+// reference allocates the arrays from its Resource; here the PlyMesh owns
+// them and `mesh` points into it.  Throws gpu::Error.
+struct PlyMesh {
+  TriangleMesh mesh{0, 0, nullptr, nullptr, nullptr, nullptr};
+  std::vector<Vector3f> p, n;
+  std::vector<Vector2f> uv;
+  std::vector<int> ind;
+};
+std::unique_ptr<PlyMesh> MakeTriangleMesh(const std::string& path);
+
+// Scene::init's accel over the scene's meshes (src/scene.cpp:46: NaiveBVHAccel
+// over its MeshPrimitives, src/accel.cpp:35-133), each mesh's accel an
+// LBVHAccel.  intersect() is the single-ray contract on the host mirrors:
+// the closest of the meshes' hits, ties between meshes to the one that
+// NaiveBVHAccel asks first.  intersectBatch() traces on the GPU
+// (flg_scene_trace_closest), same answer.  The meshes must outlive it.
+class SceneAccel : public Accel {
+ public:
+  explicit SceneAccel(const std::vector<const LBVHAccel*>& meshes,
+                      gpu::Context& ctx = gpu::Context::global());
+  ~SceneAccel() override;
+
+  bool intersect(const Ray& ray, SInteraction& isect) const override;
+  // Same as intersect(); also reports the mesh (scene index) and the original
+  // triangle index within it.
+  bool intersect(const Ray& ray, SInteraction& isect, int32_t* mesh, int32_t* tri_id) const;
+  AABB getBound() const override;
+
+  void intersectBatch(const flg_ray* d_rays, flg_scene_hit* d_hits, int64_t n) const;
+  std::vector<flg_scene_hit> intersectBatch(const std::vector<Ray>& rays) const;
+
+ private:
+  gpu::Context* m_ctx;
+  std::vector<const LBVHAccel*> m_meshes;
+  std::vector<int32_t> m_by_rank;  // scene indices in the order NaiveBVHAccel asks them
+  flg_scene_accel* m_accel = nullptr;
+  AABB m_box;                      // NaiveBVHAccel's root box (holds the origin)
 };
 
 namespace experimental {

@@ -7,6 +7,7 @@
 //   Sampler                    src/common/sampler.h:17-36
 //   Integrator                 src/integrator.hpp:10-14
 //   InternalTriangleMesh, BVHBuilderBase  src/experimental/base_bvh.hpp:26-125
+//   TriangleMesh               src/shape.hpp:53-62
 // Inside the fledge tree these come from fledge's own headers: define
 // FLEDGE_HAS_REFERENCE_HEADERS and include them before fledge_gpu.hpp.  This
 // file lets the drop-ins and their tests build where fledge's dependencies
@@ -88,6 +89,13 @@ class VInteraction : public Interaction {
     m_wo = wo;
   }
   Float m_g{0};
+};
+
+struct TriangleMesh {
+  int nInd, nVert;
+  int* ind;
+  Vector3f *p, *n;
+  Vector2f* uv;
 };
 
 class Primitive {

@@ -186,6 +186,53 @@ FLG_API int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d
 FLG_API int flg_trace_count(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, int64_t n,
                             uint64_t* d_counts);
 
+/* ---------------------------------------------------------------- PLY --- */
+/* MakeTriangleMesh (src/plymesh.cpp:19-86): a PLY file's "vertex" element
+ * (x, y, z; nx, ny, nz and a texture-coordinate pair when present) and its
+ * "face" element's "vertex_indices", three per face.  Formats ascii,
+ * binary_little_endian and binary_big_endian 1.0, any scalar type.  Returns
+ * FLG_ERR_INVALID for a missing or malformed file, a face that is not a
+ * triangle or an index outside the vertex range; FLG_ERR_RANGE beyond 2^31
+ * vertices or 2^30 faces.  Host memory, released by flg_ply_free; needs no
+ * context or GPU. */
+typedef struct flg_ply_mesh {
+  int32_t nverts, ntris;
+  float* verts;   /* nverts * 3 */
+  int32_t* ind;   /* ntris * 3 */
+  float* normals; /* nverts * 3, or NULL */
+  float* uv;      /* nverts * 2, or NULL */
+} flg_ply_mesh;
+FLG_API int flg_ply_load(const char* path, flg_ply_mesh* out);
+FLG_API void flg_ply_free(flg_ply_mesh* mesh);
+
+/* ------------------------------------------------------- scene accel --- */
+/* The scene's accel over its meshes: Scene::init's NaiveBVHAccel over the
+ * scene's MeshPrimitives (src/scene.cpp:46, src/accel.cpp:35-133), each
+ * mesh's own accel (src/primitive.cpp:54-90) being its LBVH.  Built on the
+ * host from the meshes' BVHs (flg_lbvh_build or flg_lbvh_build_batched, same
+ * context, same FLG_NORMAL_COMPAT), which must outlive it.  Mesh m is
+ * h_meshes[m], the scene's order. */
+typedef struct flg_scene_accel flg_scene_accel;
+FLG_API int flg_scene_accel_build(flg_ctx* ctx, flg_bvh* const* h_meshes, int32_t n, flg_scene_accel** out);
+FLG_API int flg_scene_accel_destroy(flg_scene_accel* s);
+/* h_rank[m]: mesh m's position in the order in which NaiveBVHAccel asks its
+ * leaves (the scene's order up to 64 meshes; beyond, its median splits'
+ * order, by std::sort as the reference's). */
+FLG_API int flg_scene_accel_rank(const flg_scene_accel* s, int32_t* h_rank);
+
+typedef struct flg_scene_hit {
+  int32_t mesh; /* scene index of the mesh hit, -1 on miss */
+  int32_t id;   /* original triangle index within that mesh, -1 on miss */
+  float t;      /* hit distance, tmax on miss */
+  int32_t pad;
+} flg_scene_hit; /* 16 bytes */
+
+/* Closest hit of n rays over the scene (Scene::intersect, src/scene.cpp:57):
+ * the closest over the meshes of each mesh's flg_trace_closest hit; between
+ * meshes at the same distance, the one of lowest rank. */
+FLG_API int flg_scene_trace_closest(flg_ctx* ctx, const flg_scene_accel* s, const flg_ray* d_rays,
+                                    flg_scene_hit* d_hits, int64_t n);
+
 typedef struct flg_camera {
   float pos[3], look_at[3], up[3];
   float fov_deg;

@@ -325,6 +325,14 @@ void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t
   volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
 }
 
+// Scene::intersect over the meshes of handles[0..n) in scene order.
+void orc_scene_trace(void* const* handles, int32_t n, int32_t compat, const float* rays, int64_t nrays,
+                     int32_t* mesh, int32_t* ids, float* ts, int nthreads) {
+  std::vector<const Lbvh*> meshes(n);
+  for (int32_t i = 0; i < n; ++i) meshes[i] = &((Handle*)handles[i])->b;
+  scene_trace(meshes, compat != 0, rays, nrays, mesh, ids, ts, nthreads);
+}
+
 // Differentiable estimator over this mesh as the interface (arguments as
 // orc_diff_render, the medium on the mesh's bounding box).
 void orc_diff_mesh_render(void* p, int32_t compat, const double* theta, const float* theta_p, int32_t nx,

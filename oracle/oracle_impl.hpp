@@ -144,6 +144,11 @@ V3 offset_ray_origin(V3 p, V3 n, V3 dir);  // OffsetRayOrigin (pathtrace.cpp)
 void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, int nthreads,
                          uint64_t* lookups);
 
+// Closest hit over a scene of meshes (scene.cpp): NaiveBVHAccel over the
+// meshes in scene order.  rays: n x 8 floats (o, tmax, d, pad).
+void scene_trace(const std::vector<const Lbvh*>& meshes, bool compat, const float* rays, int64_t n, int32_t* mesh,
+                 int32_t* ids, float* ts, int nthreads);
+
 // The differentiable estimator over the interface mesh b (volume_diff.cpp):
 // forward (lpath, img) or, with dldpix, the adjoint (grad, galpha).
 void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const float* theta_p, int32_t nx,

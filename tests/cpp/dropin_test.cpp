@@ -8,9 +8,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "fledge_gpu.hpp"
 #include "fledge_scene.h"
@@ -261,6 +264,136 @@ void test_lbvh_accel(fledge::gpu::Context& ctx) {
   std::printf("PASS lbvh_accel (%d hits of %zu rays)\n", nhit, rays.size());
 }
 
+// ------------------------------------------------------------------ PLY
+// An octahedral sphere written as a PLY file (ascii or binary little-endian)
+// with an extra vertex property and a comment, for MakeTriangleMesh.
+struct PlyCase {
+  std::vector<float> v;
+  std::vector<int32_t> ind;
+};
+
+PlyCase sphere_case(int level, float dx) {
+  PlyCase c;
+  int64_t nv = 0;
+  const int64_t nt = flg_scene_octasphere_counts(level, &nv);
+  c.v.resize(size_t(nv) * 3);
+  c.ind.resize(size_t(nt) * 3);
+  flg_scene_octasphere(level, 0.05f, 7, 2.0f, c.v.data(), c.ind.data());
+  for (size_t i = 0; i < c.v.size(); i += 3) c.v[i] += dx;
+  return c;
+}
+
+std::string write_ply(const std::string& name, bool binary, const PlyCase& c) {
+  const std::string path = "/tmp/fledge_dropin_" + std::to_string((long)getpid()) + "_" + name + ".ply";
+  FILE* f = std::fopen(path.c_str(), "wb");
+  const size_t nv = c.v.size() / 3, nt = c.ind.size() / 3;
+  std::fprintf(f, "ply\nformat %s 1.0\ncomment written by dropin_test\n", binary ? "binary_little_endian" : "ascii");
+  std::fprintf(f, "element vertex %zu\nproperty float x\nproperty float y\nproperty float z\n", nv);
+  std::fprintf(f, "property uchar flags\nelement face %zu\nproperty list uchar int vertex_indices\nend_header\n", nt);
+  for (size_t i = 0; i < nv; ++i) {
+    if (binary) {
+      std::fwrite(&c.v[3 * i], 4, 3, f);
+      const unsigned char fl = (unsigned char)(i & 0xff);
+      std::fwrite(&fl, 1, 1, f);
+    } else {
+      std::fprintf(f, "%.9g %.9g %.9g %d\n", c.v[3 * i], c.v[3 * i + 1], c.v[3 * i + 2], (int)(i & 0xff));
+    }
+  }
+  for (size_t t = 0; t < nt; ++t) {
+    if (binary) {
+      const unsigned char three = 3;
+      std::fwrite(&three, 1, 1, f);
+      std::fwrite(&c.ind[3 * t], 4, 3, f);
+    } else {
+      std::fprintf(f, "3 %d %d %d\n", c.ind[3 * t], c.ind[3 * t + 1], c.ind[3 * t + 2]);
+    }
+  }
+  std::fclose(f);
+  return path;
+}
+
+void test_ply_host() {
+  const PlyCase c = sphere_case(3, 0.0f);
+  for (bool binary : {false, true}) {
+    const std::string path = write_ply(binary ? "bin" : "ascii", binary, c);
+    auto m = fledge::MakeTriangleMesh(path);
+    std::remove(path.c_str());
+    CHECK(m->mesh.nVert == (int)(c.v.size() / 3) && m->mesh.nInd == (int)c.ind.size(), "ply counts");
+    CHECK(m->mesh.n == nullptr && m->mesh.uv == nullptr, "ply: no normals or texture coordinates");
+    for (int i = 0; i < m->mesh.nVert; ++i)
+      for (int k = 0; k < 3; ++k)
+        CHECK(bits(m->mesh.p[i][k]) == bits(c.v[3 * i + k]), "ply vertex %d", i);
+    for (int i = 0; i < m->mesh.nInd; ++i) CHECK(m->mesh.ind[i] == c.ind[i], "ply index %d", i);
+  }
+  bool threw = false;
+  try {
+    (void)fledge::MakeTriangleMesh("/tmp/fledge_dropin_no_such_file.ply");
+  } catch (const fledge::gpu::Error& e) {
+    threw = e.status() == FLG_ERR_INVALID;
+  }
+  CHECK(threw, "a missing file must throw");
+  std::printf("PASS ply_host (ascii and binary, %zu vertices)\n", c.v.size() / 3);
+}
+
+// Scene::intersect through SceneAccel over three mesh primitives read from
+// PLY files (the third repeats the first: they tie wherever it is hit), host
+// mirror and GPU batch against the oracle's NaiveBVHAccel restatement.
+void test_scene_accel(fledge::gpu::Context& ctx) {
+  const PlyCase a = sphere_case(4, 0.0f), b = sphere_case(3, 0.7f);
+  std::vector<std::unique_ptr<fledge::PlyMesh>> plys;
+  for (const PlyCase* c : {&a, &b, &a}) {
+    const std::string path = write_ply("scene" + std::to_string(plys.size()), true, *c);
+    plys.push_back(fledge::MakeTriangleMesh(path));
+    std::remove(path.c_str());
+  }
+  std::vector<std::unique_ptr<fledge::LBVHAccel>> accels;
+  std::vector<const fledge::LBVHAccel*> ptrs;
+  for (auto& m : plys) {
+    accels.push_back(std::make_unique<fledge::LBVHAccel>(m->mesh, ctx));
+    ptrs.push_back(accels.back().get());
+  }
+  fledge::SceneAccel scene(ptrs, ctx);
+  std::vector<orc::Lbvh> refs(3);
+  std::vector<const orc::Lbvh*> rp;
+  for (int i = 0; i < 3; ++i) {
+    const PlyCase& c = i == 1 ? b : a;
+    orc::lbvh_build(refs[i], c.v.data(), c.ind.data(), (int)c.ind.size() / 3, (int)c.v.size() / 3);
+    rp.push_back(&refs[i]);
+  }
+  orc::Camera cam(orc::v3(0.3f, 0.2f, -4.0f), orc::v3(0.3f, 0.0f, 0.0f), orc::v3(0, 1, 0), 40.0f);
+  const int W = 96, H = 96;
+  std::vector<fledge::Ray> rays;
+  std::vector<float> rr;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      orc::V3 o, d;
+      cam.generate(x + 0.5f, y + 0.5f, W, H, &o, &d);
+      rays.emplace_back(Vector3f(o.x, o.y, o.z), Vector3f(d.x, d.y, d.z));
+      rr.insert(rr.end(), {o.x, o.y, o.z, INFINITY, d.x, d.y, d.z, 0.0f});
+    }
+  std::vector<int32_t> mref(rays.size()), iref(rays.size());
+  std::vector<float> tref(rays.size());
+  orc::scene_trace(rp, false, rr.data(), (int64_t)rays.size(), mref.data(), iref.data(), tref.data(), 4);
+  const std::vector<flg_scene_hit> hits = scene.intersectBatch(rays);
+  int nhit = 0, n_second = 0;
+  for (size_t i = 0; i < rays.size(); ++i) {
+    fledge::Ray r = rays[i];
+    fledge::SInteraction isect;
+    int32_t mesh = -2, id = -2;
+    const bool hit = scene.intersect(r, isect, &mesh, &id);
+    CHECK(mesh == mref[i] && id == iref[i] && hits[i].mesh == mref[i] && hits[i].id == iref[i],
+          "ray %zu: mesh/id %d/%d, gpu %d/%d, oracle %d/%d", i, mesh, id, hits[i].mesh, hits[i].id, mref[i], iref[i]);
+    CHECK(hit == (mref[i] >= 0), "ray %zu: hit flag", i);
+    if (!hit) continue;
+    ++nhit;
+    n_second += mesh == 1;
+    CHECK(bits(r.m_tMax) == bits(tref[i]) && bits(hits[i].t) == bits(tref[i]), "ray %zu: t", i);
+    CHECK(mesh != 2, "ray %zu: the repeated mesh must lose every tie", i);
+  }
+  CHECK(nhit > 1000 && n_second > 100, "too few hits (%d, %d on the second mesh)", nhit, n_second);
+  std::printf("PASS scene_accel (%d hits of %zu rays)\n", nhit, rays.size());
+}
+
 void test_vol_integrator(fledge::gpu::Context& ctx) {
   const int n = 32, W = 40, H = 32, spp = 4, depth = 16;
   std::vector<float> grid(size_t(n) * n * n);
@@ -360,6 +493,7 @@ int main(int argc, char** argv) {
   const bool cpu_only = argc > 1 && std::string(argv[1]) == "--cpu";
   test_grid_volume_host();
   test_host_errors();
+  if (!g_failures) test_ply_host();
   if (!cpu_only && g_failures == 0) {
     try {
       fledge::gpu::Context ctx(0, nullptr);
@@ -368,6 +502,7 @@ int main(int argc, char** argv) {
       if (!g_failures) test_memory_resources(ctx);
       if (!g_failures) test_lbvh_builder(ctx);
       if (!g_failures) test_lbvh_accel(ctx);
+      if (!g_failures) test_scene_accel(ctx);
       if (!g_failures) test_vol_integrator(ctx);
       ctx.synchronize();
     } catch (const std::exception& e) {

@@ -46,6 +46,7 @@ _SIGS = {
     "orc_volpath_mesh_render": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P,
                                        ctypes.c_int, P]),
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
+    "orc_scene_trace": (None, [P, i32, i32, P, i64, P, P, P, ctypes.c_int]),
     "orc_diff_mesh_render": (None, [P, i32, P, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
                                     ctypes.c_int, P]),
 }
@@ -176,6 +177,19 @@ class Lbvh:
             lib().orc_lbvh_free(self.h)
         except Exception:
             pass
+
+
+def scene_trace(meshes, rays, compat=False):
+    """Scene::intersect over the oracle LBVHs `meshes` (scene order): (mesh,
+    id, t) per ray."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    hs = (ctypes.c_void_p * len(meshes))(*[m.h for m in meshes])
+    mesh = np.zeros(n, np.int32)
+    ids = np.zeros(n, np.int32)
+    ts = np.zeros(n, np.float32)
+    lib().orc_scene_trace(hs, len(meshes), 1 if compat else 0, _p(rays), n, _p(mesh), _p(ids), _p(ts), NTHREADS)
+    return mesh, ids, ts
 
 
 def brute_trace(verts, ind, rays, compat=False):

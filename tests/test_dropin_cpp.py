@@ -25,7 +25,8 @@ def _run(args, timeout):
 
 def test_dropins_host_side():
     """GridVolume::tr/sample (host mirror of the device trackers) bit-equal to
-    the oracle's OpenVDBVolume::tr/sample; error paths.  No GPU call."""
+    the oracle's OpenVDBVolume::tr/sample; error paths; MakeTriangleMesh on
+    ascii and binary PLY files.  No GPU call."""
     _run(["--cpu"], 120)
 
 
@@ -49,13 +50,15 @@ def test_host_library_exports_dropins():
                 "fledge::GridVolume::tr", "fledge::GridVolume::sample",
                 "fledge::GPUVolIntegrator::render", "fledge::gpu_memory_resource::do_allocate",
                 "fledge::managed_memory_resource::do_allocate",
-                "fledge::async_memory_resource::do_allocate", "fledge::GlobalManagedMemoryResource"):
+                "fledge::async_memory_resource::do_allocate", "fledge::GlobalManagedMemoryResource",
+                "fledge::SceneAccel::intersect", "fledge::MakeTriangleMesh"):
         assert sym in out, sym
 
 
 @pytest.mark.gpu
 def test_dropins_on_gpu():
     """bvh_test-style: LBVHBuilder host mirror, GPU batch and oracle agree bit
-    for bit on 20000 rays; LBVHAccel's intersect contract; GPUVolIntegrator's
-    film against the oracle; memory resources."""
+    for bit on 20000 rays; LBVHAccel's intersect contract; SceneAccel over PLY
+    meshes (host mirror and GPU) against the oracle's Scene::intersect;
+    GPUVolIntegrator's film against the oracle; memory resources."""
     _run([], 300)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
+from scene_cases import reference_rank, scene_meshes, scene_rays
 from fledge_amd import api
 
 L = orc.lib()
@@ -760,3 +761,40 @@ def test_diff_mesh_gradient_central_differences():
     a0, h = float(np.float32(A)), 2.0 ** -14
     fa = (F(theta, a0 + h) - F(theta, a0 - h)) / (2 * h)
     assert abs(ga - fa) <= 1e-6 * abs(fa) + 1e-9
+
+
+@pytest.mark.parametrize("n,dup,extra", [(40, [(5, 2), (17, 2), (30, 11)], []),
+                                         (150, [], [(7, 3), (90, 40), (140, 41)])])
+def test_scene_trace_is_lexicographic_min_over_meshes(n, dup, extra):
+    """Scene::intersect over NaiveBVHAccel (its tree, box tests and strict
+    replacement in each leaf) equals, per ray, the closest of the meshes' own
+    hits and, between meshes at the same distance, the one first in the
+    reference's leaf order: scene order up to 64 meshes (duplicated meshes
+    tie everywhere), the median splits' order beyond."""
+    meshes = scene_meshes(n, 3, dup, extra)
+    lb = [orc.Lbvh(v, i) for v, i in meshes]
+    rays = np.concatenate([scene_rays(3000, 4), scene_rays(1000, 5, tmax=6.0)])
+    mesh, ids, ts = orc.scene_trace(lb, rays)
+    rank = reference_rank(meshes)
+    per = [b.trace(rays) for b in lb]
+    best_t = np.full(len(rays), np.inf, np.float32)
+    best_m = np.full(len(rays), -1)
+    for m, (pid, pt) in enumerate(per):
+        hit = pid >= 0
+        better = hit & ((pt < best_t) | ((pt == best_t) & (best_m >= 0) & (rank[m] < rank[np.maximum(best_m, 0)])))
+        better |= hit & (best_m < 0)
+        best_t = np.where(better, pt, best_t)
+        best_m = np.where(better, m, best_m)
+    assert np.array_equal(mesh, best_m)
+    hit = best_m >= 0
+    want_id = np.array([per[m][0][i] if m >= 0 else -1 for i, m in enumerate(best_m)])
+    assert np.array_equal(ids, want_id)
+    assert np.array_equal(ts[hit].view(np.uint32), best_t[hit].view(np.uint32))
+    assert np.array_equal(ts[~hit], rays[~hit, 3])
+    assert hit.mean() > 0.3
+    for j, k in dup:  # the duplicates tie and the lower scene index wins
+        assert not np.any(mesh == j)
+    for j, k in extra:  # ties on the shared surface go to the lower rank
+        first, second = (j, k) if rank[j] < rank[k] else (k, j)
+        both = (per[j][0] >= 0) & (per[k][0] >= 0) & (per[j][1] == per[k][1])
+        assert both.sum() > 0 and not np.any(mesh[both] == second)
