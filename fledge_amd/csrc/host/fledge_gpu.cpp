@@ -664,4 +664,78 @@ std::vector<flg_scene_hit> SceneAccel::intersectBatch(const std::vector<Ray>& ra
   return h;
 }
 
+// -------------------------------------------------------------- Render ----
+HIPRender::HIPRender(Scene* scene, const GPUVolLights& lights) : m_scene(scene), m_lights(lights) {}
+
+void HIPRender::init() {
+  m_init = false;
+  m_mode = Mode::kNone;
+  if (!m_scene || !m_scene->m_film) throw gpu::Error(FLG_ERR_INVALID, "HIPRender: no scene or no film");
+  if (m_scene->m_integrator_type != EIntegratorType::EVolPathIntegrator)
+    throw gpu::Error(FLG_ERR_INVALID, "HIPRender: only the volume integrators are available");
+  m_volume = dynamic_cast<const GridVolume*>(m_scene->m_volume);
+  if (!m_volume) throw gpu::Error(FLG_ERR_INVALID, "HIPRender: scene.m_volume is not a GridVolume");
+  if (m_scene->m_primitives.empty()) {
+    m_mode = Mode::kVolume;
+  } else if (m_scene->m_primitives.size() == 1 &&
+             (m_mesh = dynamic_cast<const LBVHAccel*>(m_scene->m_primitives[0])) != nullptr) {
+    m_mode = Mode::kMeshMedium;
+  } else {
+    throw gpu::Error(FLG_ERR_INVALID, "HIPRender: primitives other than one interface mesh");
+  }
+  m_init = true;
+}
+
+bool HIPRender::render() {
+  if (!m_init) return false;
+  Scene& sc = *m_scene;
+  if (m_mode == Mode::kVolume) {
+    GPUVolIntegrator integ(sc.m_maxDepth, m_lights);
+    integ.render(sc);
+    m_lookups = integ.lastLookups();
+    return true;
+  }
+  gpu::Context& ctx = m_mesh->context();
+  flg_vol_scene vs;
+  std::memset(&vs, 0, sizeof(vs));
+  for (int k = 0; k < 3; ++k) {
+    vs.camera.pos[k] = sc.m_origin[k];
+    vs.camera.look_at[k] = sc.m_target[k];
+    vs.camera.up[k] = sc.m_up[k];
+    vs.point_pos[k] = m_lights.point_pos[k];
+  }
+  vs.camera.fov_deg = sc.m_FoV;
+  vs.has_point = m_lights.has_point ? 1 : 0;
+  vs.point_intensity = m_lights.point_intensity;
+  vs.has_env = m_lights.has_env ? 1 : 0;
+  vs.env_radiance = m_lights.env_radiance;
+  flg_render_cfg cfg;
+  std::memset(&cfg, 0, sizeof(cfg));
+  cfg.width = sc.m_resX;
+  cfg.height = sc.m_resY;
+  cfg.spp = sc.m_SPP;
+  cfg.max_depth = sc.m_maxDepth;
+  const size_t n = size_t(sc.m_resX) * sc.m_resY * 3;
+  gpu::DeviceBuffer d(ctx, n * sizeof(float));
+  uint64_t stats[2] = {0, 0};
+  ctx.check(flg_volpath_mesh_render(ctx.get(), m_mesh->handle(), m_volume->handle(), &vs, &cfg,
+                                    static_cast<float*>(d.data()), stats),
+            "flg_volpath_mesh_render");
+  m_lookups = stats[1];
+  std::vector<float> rgb(n);
+  d.download(rgb.data(), n * sizeof(float));
+  for (int y = 0; y < sc.m_resY; ++y)
+    for (int x = 0; x < sc.m_resX; ++x) {
+      const float* c = &rgb[(size_t(y) * sc.m_resX + x) * 3];
+      sc.m_film->getBuffer(x, y, EFilmBufferType::EColor) = Vector3f(c[0], c[1], c[2]);
+    }
+  return true;
+}
+
+bool HIPRender::saveImage(const std::string& name, bool denoise) {
+  if (!m_init || denoise) return false;
+  return m_scene->m_film->saveBuffer(name, EFilmBufferType::EColor);
+}
+
 }  // namespace fledge
+

@@ -321,4 +321,44 @@ class GPUVolIntegrator : public Integrator {
   uint64_t m_stats[2] = {0, 0};
 };
 
+// -------------------------------------------------------------- Render ----
+// The backend bit of this renderer; EBackendType names the reference's own.
+constexpr EBackendType kHIPBackend = static_cast<EBackendType>(1 << 3);
+
+// A GPU RenderBase (src/render.hpp:17-26) in the place of the OptiX stub
+// (src/optix/optix_render.cpp:25-66, chosen in src/main/fledge.cpp:26-30):
+// init() reads the scene and picks what renders it, render() renders into
+// scene.m_film's colour buffer, saveImage() writes it (".exr").
+//   * a volume-integrator scene with no primitives and a GridVolume: the
+//     volume-only integrator with the reference's render loop and film
+//     (GPUVolIntegrator);
+//   * a volume-integrator scene whose one primitive is an LBVHAccel (a
+//     closed interface mesh) and whose m_volume is a GridVolume (the medium
+//     inside the mesh, on its bounding box): VolPathIntegrator over the mesh
+//     (flg_volpath_mesh_render), the per-pixel mean of the samples.
+// Any other scene makes init() throw gpu::Error (FLG_ERR_INVALID).
+class HIPRender : public RenderBase {
+ public:
+  HIPRender(Scene* scene, const GPUVolLights& lights);
+  void init() override;
+  bool preProcess() override { return m_init; }
+  // denoise is not available (the reference's OIDN pass): false
+  bool saveImage(const std::string& name, bool denoise = false) override;
+  bool render() override;
+  EBackendType getBackends() override { return kHIPBackend; }
+
+  uint64_t lastLookups() const { return m_lookups; }
+
+ private:
+  enum class Mode { kNone, kVolume, kMeshMedium };
+  Scene* m_scene;
+  GPUVolLights m_lights;
+  Mode m_mode = Mode::kNone;
+  bool m_init = false;
+  const LBVHAccel* m_mesh = nullptr;
+  const GridVolume* m_volume = nullptr;
+  uint64_t m_lookups = 0;
+};
+
 }  // namespace fledge
+

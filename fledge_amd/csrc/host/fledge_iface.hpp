@@ -8,6 +8,8 @@
 //   Integrator                 src/integrator.hpp:10-14
 //   InternalTriangleMesh, BVHBuilderBase  src/experimental/base_bvh.hpp:26-125
 //   TriangleMesh               src/shape.hpp:53-62
+//   RenderBase                 src/render.hpp:17-26
+//   EIntegratorType, EBackendType  src/fledge.h:87-96
 // Inside the fledge tree these come from fledge's own headers: define
 // FLEDGE_HAS_REFERENCE_HEADERS and include them before fledge_gpu.hpp.  This
 // file lets the drop-ins and their tests build where fledge's dependencies
@@ -151,15 +153,39 @@ struct Film {
   std::vector<Vector3f> m_color;
 };
 
+enum class EIntegratorType {
+  EPathIntegrator = 0,
+  EVolPathIntegrator,
+};
+
+enum class EBackendType {
+  ECPUBackend = (1),
+  EOptiXBackend = (1 << 1),
+  EOptiXWaveFrontBackend = (1 << 2),
+};
+
 // The scene-description fields of Scene (src/scene.hpp:85-101) that the
-// volume integrator reads.
+// integrators and the render read.
 class Scene {
  public:
   int m_resX{0}, m_resY{0}, m_SPP{0}, m_maxDepth{0};
   Float m_FoV{0};
   Vector3f m_up{0}, m_origin{0}, m_target{0};
+  EIntegratorType m_integrator_type{EIntegratorType::EPathIntegrator};
+  std::vector<Primitive*> m_primitives{};
   Volume* m_volume{nullptr};
   Film* m_film{nullptr};
+};
+
+class RenderBase {
+ public:
+  RenderBase() = default;
+  virtual ~RenderBase() = default;
+  virtual void init() = 0;
+  virtual bool preProcess() = 0;
+  virtual bool saveImage(const std::string& name, bool denoise = false) = 0;
+  virtual bool render() = 0;
+  virtual EBackendType getBackends() = 0;
 };
 
 class Integrator {

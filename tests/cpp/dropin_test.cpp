@@ -462,6 +462,110 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
   std::printf("PASS vol_integrator (film vs oracle worst rel %.2g)\n", worst);
 }
 
+// HIPRender in the OptiX stub's place: a volume-only scene (the volume
+// integrator's film) and a closed interface mesh holding the medium
+// (VolPathIntegrator against the oracle, bit for bit), saveImage, and the
+// scenes it refuses.
+void test_hip_render(fledge::gpu::Context& ctx) {
+  const int n = 24, W = 40, H = 32, spp = 4, depth = 16;
+  std::vector<float> grid(size_t(n) * n * n);
+  flg_scene_fbm_grid(n, n, n, 5, 4.0f, 5, 0.0f, 1.0f, grid.data());
+  fledge::GridVolume vol(grid.data(), n, n, n, fledge::AABB(Vector3f(0.0f), Vector3f(1.0f)), Vector3f(6.4f),
+                         Vector3f(1.6f), 0.3f, &ctx);
+  fledge::GPUVolLights lights;
+  lights.has_point = true;
+  lights.point_pos = Vector3f(2.0f, 2.0f, -2.0f);
+  lights.point_intensity = 50.0f;
+  lights.has_env = true;
+  lights.env_radiance = 0.5f;
+  fledge::Film film(W, H);
+  fledge::Scene scene;
+  scene.m_resX = W;
+  scene.m_resY = H;
+  scene.m_SPP = spp;
+  scene.m_maxDepth = depth;
+  scene.m_FoV = 45.0f;
+  scene.m_origin = Vector3f(0.5f, 0.5f, -1.5f);
+  scene.m_target = Vector3f(0.5f, 0.5f, 0.0f);
+  scene.m_up = Vector3f(0.0f, 1.0f, 0.0f);
+  scene.m_integrator_type = fledge::EIntegratorType::EVolPathIntegrator;
+  scene.m_volume = &vol;
+  scene.m_film = &film;
+  {  // volume only: the volume integrator's render loop and film
+    fledge::HIPRender r(&scene, lights);
+    r.init();
+    CHECK(r.getBackends() == fledge::kHIPBackend && r.preProcess() && r.render(), "volume-only render");
+    fledge::GPUVolIntegrator integ(depth, lights);
+    fledge::Film ref(W, H);
+    fledge::Scene s2 = scene;
+    s2.m_film = &ref;
+    integ.render(s2);
+    for (int i = 0; i < W * H; ++i)
+      CHECK(bits(film.m_color[i][0]) == bits(ref.m_color[i][0]), "volume-only pixel %d", i);
+    CHECK(r.lastLookups() == integ.lastLookups(), "lookup counts");
+  }
+  // a closed interface mesh around the medium
+  int64_t nv = 0;
+  const int64_t nt = flg_scene_octasphere_counts(4, &nv);
+  std::vector<float> v(size_t(nv) * 3);
+  std::vector<int32_t> ind(size_t(nt) * 3);
+  flg_scene_octasphere(4, 0.05f, 7, 2.0f, v.data(), ind.data());
+  for (float& x : v) x = x * 0.5f + 0.5f;
+  std::vector<Vector3f> p((size_t)nv);
+  for (int64_t i = 0; i < nv; ++i) p[i] = Vector3f(v[3 * i], v[3 * i + 1], v[3 * i + 2]);
+  fledge::LBVHAccel mesh(p, std::vector<int>(ind.begin(), ind.end()), ctx);
+  scene.m_primitives = {&mesh};
+  fledge::HIPRender r(&scene, lights);
+  r.init();
+  CHECK(r.render(), "mesh + medium render");
+  orc::Lbvh b;
+  orc::lbvh_build(b, v.data(), ind.data(), (int)nt, (int)nv);
+  const float mb[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
+  const float medium[3] = {6.4f, 1.6f, 0.3f};
+  orc::VolScene sc;
+  orc::init_vol_grid(sc, grid.data(), n, n, n, mb, medium);
+  sc.point.present = true;
+  sc.point.pos = orc::v3(2, 2, -2);
+  sc.point.intensity = 50.0f;
+  sc.env.build(0.5f, sc.grid.bmin, sc.grid.bmax);
+  orc::Camera cam(orc::v3(0.5f, 0.5f, -1.5f), orc::v3(0.5f, 0.5f, 0.0f), orc::v3(0, 1, 0), 45.0f);
+  sc.cam = &cam;
+  sc.width = W;
+  sc.height = H;
+  sc.spp = spp;
+  sc.max_depth = depth;
+  std::vector<float> rgb(size_t(W) * H * 3);
+  uint64_t lk = 0;
+  orc::volpath_mesh_render(b, false, sc, rgb.data(), 4, &lk);
+  for (int i = 0; i < W * H; ++i)
+    CHECK(bits(film.m_color[i][0]) == bits(rgb[3 * i]), "mesh + medium pixel %d: %.9g vs oracle %.9g", i,
+          film.m_color[i][0], rgb[3 * i]);
+  CHECK(r.lastLookups() == lk, "mesh + medium lookup counts");
+  const std::string out = "/tmp/fledge_dropin_" + std::to_string((long)getpid()) + "_render.exr";
+  CHECK(r.saveImage(out) && !r.saveImage(out, /*denoise=*/true), "saveImage");
+  FILE* f = std::fopen(out.c_str(), "rb");
+  unsigned char magic[4] = {0, 0, 0, 0};
+  CHECK(f && std::fread(magic, 1, 4, f) == 4, "the EXR file");
+  std::fclose(f);
+  std::remove(out.c_str());
+  CHECK(magic[0] == 0x76 && magic[1] == 0x2f && magic[2] == 0x31 && magic[3] == 0x01, "EXR magic");
+  // refused scenes
+  int refused = 0;
+  for (int k = 0; k < 2; ++k) {
+    fledge::Scene bad = scene;
+    if (k == 0) bad.m_integrator_type = fledge::EIntegratorType::EPathIntegrator;
+    if (k == 1) bad.m_primitives = {&mesh, &mesh};
+    try {
+      fledge::HIPRender rb(&bad, lights);
+      rb.init();
+    } catch (const fledge::gpu::Error& e) {
+      refused += e.status() == FLG_ERR_INVALID;
+    }
+  }
+  CHECK(refused == 2, "HIPRender must refuse the path integrator and several primitives");
+  std::printf("PASS hip_render (volume-only and mesh + medium, %llu lookups)\n", (unsigned long long)lk);
+}
+
 void test_memory_resources(fledge::gpu::Context& ctx) {
   fledge::gpu_memory_resource gmr;
   void* p = gmr.allocate(1 << 20, 256);
@@ -504,6 +608,7 @@ int main(int argc, char** argv) {
       if (!g_failures) test_lbvh_accel(ctx);
       if (!g_failures) test_scene_accel(ctx);
       if (!g_failures) test_vol_integrator(ctx);
+      if (!g_failures) test_hip_render(ctx);
       ctx.synchronize();
     } catch (const std::exception& e) {
       std::printf("FAIL exception: %s\n", e.what());

@@ -51,7 +51,7 @@ def test_host_library_exports_dropins():
                 "fledge::GPUVolIntegrator::render", "fledge::gpu_memory_resource::do_allocate",
                 "fledge::managed_memory_resource::do_allocate",
                 "fledge::async_memory_resource::do_allocate", "fledge::GlobalManagedMemoryResource",
-                "fledge::SceneAccel::intersect", "fledge::MakeTriangleMesh"):
+                "fledge::SceneAccel::intersect", "fledge::MakeTriangleMesh", "fledge::HIPRender::render"):
         assert sym in out, sym
 
 
@@ -60,5 +60,7 @@ def test_dropins_on_gpu():
     """bvh_test-style: LBVHBuilder host mirror, GPU batch and oracle agree bit
     for bit on 20000 rays; LBVHAccel's intersect contract; SceneAccel over PLY
     meshes (host mirror and GPU) against the oracle's Scene::intersect;
-    GPUVolIntegrator's film against the oracle; memory resources."""
+    GPUVolIntegrator's film against the oracle; HIPRender (the RenderBase in
+    the OptiX stub's place) on a volume-only and a mesh + medium scene;
+    memory resources."""
     _run([], 300)
