@@ -39,25 +39,36 @@ void release_halton_tables(flg_ctx* ctx);
 HaltonParams make_halton_params(int resx, int resy);
 constexpr uint32_t kMaxHaltonIndex = 1u << 24;
 
+// Every quantity of the digit loop is an integer that a float holds exactly
+// (a < 2^24, its quotients and digits) or that a double holds exactly (the
+// reversed digits, below base * 2^24 < 2^53), so the loop runs on the float
+// and double units: the quotient from the float reciprocal, off by at most
+// one (|a * inv_base - a / base| <= 2 / base), corrected with the exact
+// remainder a - q * base (one fma, exact because the result is an integer
+// below 2^24), and the reversed digits accumulated with an exact fma in
+// double.  The result is the reference's: (float) of the same integer,
+// times the same float product chain of 1 / base.
 FLG_D float radical_inverse(int32_t base, float inv_base, uint32_t a) {
-  uint64_t reversed = 0;
+  const float b = (float)base;
+  const double bd = (double)base;
+  float af = (float)a;
+  double reversed = 0.0;
   float inv_base_n = 1.0f;
-  while (a) {
-    uint32_t q = (uint32_t)((float)a * inv_base);
-    int32_t r = (int32_t)(a - q * (uint32_t)base);
-    if (r < 0) {
-      q -= 1;
-      r += base;
-    } else if (r >= base) {
-      q += 1;
-      r -= base;
+  while (af != 0.0f) {
+    float q = floorf(af * inv_base);
+    float r = __builtin_fmaf(-q, b, af);
+    if (r < 0.0f) {
+      q -= 1.0f;
+      r += b;
+    } else if (r >= b) {
+      q += 1.0f;
+      r -= b;
     }
-    reversed = reversed * (uint64_t)base + (uint64_t)r;
+    reversed = __builtin_fma(reversed, bd, (double)r);
     inv_base_n *= inv_base;
-    a = q;
+    af = q;
   }
-  float rf = (reversed >> 32) ? (float)reversed : (float)(uint32_t)reversed;
-  return rf * inv_base_n;
+  return (float)reversed * inv_base_n;
 }
 
 // HaltonSampler::get1D (sampler.h:171-174) for global index `index` and
