@@ -37,20 +37,6 @@ FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride, uint2* priv
   return RingStack<1>(ring, stride, priv);
 }
 
-// Traversal with the stack policy of RING: 0 keeps the whole stack in private
-// memory, K > 0 keeps the K most recent entries of each lane in LDS.
-template <bool COMPAT, bool COUNT, int BS, int RING>
-FLG_D int32_t trace_lane(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, uint2* ring) {
-  if constexpr (RING > 0) {
-    uint2 spill[kStack - RING];
-    RingStack<RING> st(ring + threadIdx.x, BS, spill);
-    return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
-  } else {
-    (void)ring;
-    return trace_one<COMPAT, COUNT>(g, o, d, best, c);
-  }
-}
-
 // Traversal is latency-bound: the kernels ask for 8 waves per SIMD (at most 64
 // VGPRs); the few values that no longer fit spill to scratch, which measured
 // faster than the 4-5 waves the unconstrained allocation allows.

@@ -226,4 +226,18 @@ FLG_HD int32_t trace_one(const TraceGeom& g, V3 o, V3 d, float& best, Counters& 
   return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
 }
 
+// Traversal with the stack policy of RING: 0 keeps the whole stack in private
+// memory, K > 0 keeps the K most recent entries of each lane in LDS.
+template <bool COMPAT, bool COUNT, int BS, int RING>
+FLG_D int32_t trace_lane(const TraceGeom& g, V3 o, V3 d, float& best, Counters& c, uint2* ring) {
+  if constexpr (RING > 0) {
+    uint2 spill[kStack - RING];
+    RingStack<RING> st(ring + threadIdx.x, BS, spill);
+    return trace_one_s<COMPAT, COUNT>(g, o, d, best, c, st);
+  } else {
+    (void)ring;
+    return trace_one<COMPAT, COUNT>(g, o, d, best, c);
+  }
+}
+
 }  // namespace flg

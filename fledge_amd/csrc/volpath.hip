@@ -31,16 +31,31 @@ namespace {
 // re-hitting its own surface.
 constexpr int kMaxCrossings = 256;
 
+// Every kernel over the mesh runs 256-lane blocks and keeps the 8 most recent
+// traversal stack entries of each lane in LDS (the ring stack of the path
+// tracer's traversal), the older ones in private memory.
+constexpr int kMeshBlock = 256;
+constexpr int kMeshRing = 8;
+// The kernels ask for 6 waves per SIMD: their traversals are latency-bound,
+// and the register cap (the few values beyond it spill to scratch) measured
+// faster than the 3-4 waves of the unconstrained allocation.
+constexpr int kMeshWaves = 6;
+
+struct MeshGeom {
+  TraceGeom g;
+  uint2* ring;  // the block's LDS ring, kMeshBlock * kMeshRing entries
+};
+
 // Closest hit on the mesh within tmax: sorted position (-1 on a miss), t, and
 // the geometric normal of the hit as the Pluecker test computes it.
 template <bool COMPAT>
-FLG_D int32_t mesh_hit(const TraceGeom& g, V3 o, V3 d, float tmax, float* t, V3* ng) {
+FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng) {
   float best = tmax;
   Counters c{0, 0};
-  const int32_t p = trace_one<COMPAT, false>(g, o, d, best, c);
+  const int32_t p = trace_lane<COMPAT, false, kMeshBlock, kMeshRing>(m.g, o, d, best, c, m.ring);
   *t = best;
   if (p >= 0) {
-    const BTri tri = g.tris[p];
+    const BTri tri = m.g.tris[p];
     const V3 v0 = v3(tri.v0.x, tri.v0.y, tri.v0.z) - o, v1 = v3(tri.v1.x, tri.v1.y, tri.v1.z) - o,
              v2 = v3(tri.v2.x, tri.v2.y, tri.v2.z) - o;
     *ng = stable_normalize(stable_triangle_normal(v2 - v0, v0 - v1, v1 - v2), COMPAT);
@@ -51,7 +66,7 @@ FLG_D int32_t mesh_hit(const TraceGeom& g, V3 o, V3 d, float tmax, float* t, V3*
 // VolEstimateTr (integrator.cpp:40-64) along (o, d) up to tmax, starting
 // inside the medium or not.
 template <bool COMPAT, bool COUNT>
-FLG_D float shadow_tr(const TraceGeom& g, const VolDev& v, HaltonDraw& rng, V3 o, V3 d, float tmax,
+FLG_D float shadow_tr(const MeshGeom& g, const VolDev& v, HaltonDraw& rng, V3 o, V3 d, float tmax,
                       bool inside, uint32_t& nlook) {
   float tr = 1.0f;
   float t_max = tmax;
@@ -98,7 +113,7 @@ FLG_D void vp_begin(const HaltonTables& tab, const HaltonParams& hp, const CamDe
 
 // One trip of the reference's loop; returns true when the path is complete.
 template <bool COMPAT, bool COUNT>
-FLG_D bool vp_step(const TraceGeom& g, const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const HaltonTables& tab,
                    const HaltonParams& hp, int max_depth, VPath& s, uint32_t& nlook) {
   float t;
   V3 ng;
@@ -146,10 +161,12 @@ FLG_D void vp_count(uint32_t nlook, unsigned long long* nlookups) {
 
 // One lane per path sample.
 template <bool COMPAT, bool COUNT>
-__global__ __launch_bounds__(256) void k_volpath(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
                                                  HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
                                                  float* lpath, unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
+  __shared__ uint2 ring[kMeshBlock * kMeshRing];
+  const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t npaths = (int64_t)cam.w * cam.h * spp;
@@ -157,7 +174,7 @@ __global__ __launch_bounds__(256) void k_volpath(TraceGeom g, VolDev v, LightsDe
   if (p < npaths) {
     VPath s;
     vp_begin(tab, hp, cam, spp, (uint32_t)p, s);
-    while (!vp_step<COMPAT, COUNT>(g, v, L, tab, hp, max_depth, s, nlook)) {
+    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook)) {
     }
     lpath[p] = s.L;
   }
@@ -166,11 +183,13 @@ __global__ __launch_bounds__(256) void k_volpath(TraceGeom g, VolDev v, LightsDe
 
 // Persistent lanes with path regeneration (regen_loop); same per-path code.
 template <bool COMPAT, bool COUNT>
-__global__ __launch_bounds__(256) void k_volpath_regen(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
                                                        HaltonParams hp, CamDevV cam, int32_t spp,
                                                        int32_t max_depth, float* lpath, unsigned int* next_path,
                                                        unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
+  __shared__ uint2 ring[kMeshBlock * kMeshRing];
+  const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
   uint32_t nlook = 0;
@@ -180,7 +199,7 @@ __global__ __launch_bounds__(256) void k_volpath_regen(TraceGeom g, VolDev v, Li
         vp_begin(tab, hp, cam, spp, p, s);
         return false;
       },
-      [&](VPath& s) { return vp_step<COMPAT, COUNT>(g, v, L, tab, hp, max_depth, s, nlook); },
+      [&](VPath& s) { return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook); },
       [&](uint32_t p, const VPath& s) { lpath[p] = s.L; });
   vp_count<COUNT>(nlook, nlookups);
 }
@@ -212,7 +231,7 @@ __global__ __launch_bounds__(256) void k_vp_mean(const float* lpath, int64_t npi
 // (product of its factors; nothing after the last interface, as
 // VolEstimateTr).
 template <bool COMPAT>
-FLG_D Track mesh_track_measure(const TraceGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
+FLG_D Track mesh_track_measure(const MeshGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax) {
   Track r{1.0f, 0};
   uint32_t n = 0;
   bool inside = true;
@@ -234,7 +253,7 @@ FLG_D Track mesh_track_measure(const TraceGeom& g, const DiffDev& D, uint32_t ke
 
 // Second walk of the same track: its derivative, times scale, deposited.
 template <bool COMPAT>
-FLG_D void mesh_track_deposit(const TraceGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax,
+FLG_D void mesh_track_deposit(const MeshGeom& g, const DiffDev& D, uint32_t key, V3 o, V3 d, float tmax,
                               const Track& r, float scale, float* grad) {
   uint32_t n = 0;
   bool inside = true;
@@ -295,7 +314,7 @@ FLG_D bool vpd_begin(const DiffDev& D, const HaltonTables& tab, const HaltonPara
 
 // One trip of VolPath's loop; returns true when the path is complete.
 template <bool ADJ, bool COMPAT>
-FLG_D bool vpd_step(const TraceGeom& g, const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
+FLG_D bool vpd_step(const MeshGeom& g, const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
                     const HaltonParams& hp, int max_depth, float* grad, float* galpha, DiffVPath& s) {
   float t;
   V3 ng;
@@ -350,10 +369,12 @@ FLG_D void vpd_finish(const DiffDev& D, uint32_t p, const DiffVPath& s, float* l
 
 // One lane per path sample.
 template <bool ADJ, bool COMPAT>
-__global__ __launch_bounds__(256) void k_vpdiff(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
                                                 HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
                                                 float* lpath, const float* dl_drgb, float* grad, float* galpha) {
   __shared__ int2 lds_primes[kLocalPrimes];
+  __shared__ uint2 ring[kMeshBlock * kMeshRing];
+  const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t npaths = (int64_t)cam.w * cam.h * spp;
@@ -361,7 +382,7 @@ __global__ __launch_bounds__(256) void k_vpdiff(TraceGeom g, DiffDev D, LightsDe
   if (p < npaths) {
     DiffVPath s;
     bool done = vpd_begin<ADJ>(D, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, s);
-    while (!done) done = vpd_step<ADJ, COMPAT>(g, D, L, tab, hp, max_depth, grad, &ga, s);
+    while (!done) done = vpd_step<ADJ, COMPAT>(mg, D, L, tab, hp, max_depth, grad, &ga, s);
     if (!ADJ) vpd_finish(D, (uint32_t)p, s, lpath);
   }
   reduce_galpha<ADJ>(ga, galpha);
@@ -369,18 +390,20 @@ __global__ __launch_bounds__(256) void k_vpdiff(TraceGeom g, DiffDev D, LightsDe
 
 // Persistent lanes with path regeneration (regen_loop); same per-path code.
 template <bool ADJ, bool COMPAT>
-__global__ __launch_bounds__(256) void k_vpdiff_regen(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff_regen(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
                                                       HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
                                                       float* lpath, const float* dl_drgb, float* grad,
                                                       float* galpha, unsigned int* next_path) {
   __shared__ int2 lds_primes[kLocalPrimes];
+  __shared__ uint2 ring[kMeshBlock * kMeshRing];
+  const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
   float ga = 0.0f;
   regen_loop<DiffVPath>(
       npaths, next_path,
       [&](uint32_t p, DiffVPath& s) { return vpd_begin<ADJ>(D, tab, hp, cam, spp, p, lpath, dl_drgb, s); },
-      [&](DiffVPath& s) { return vpd_step<ADJ, COMPAT>(g, D, L, tab, hp, max_depth, grad, &ga, s); },
+      [&](DiffVPath& s) { return vpd_step<ADJ, COMPAT>(mg, D, L, tab, hp, max_depth, grad, &ga, s); },
       [&](uint32_t p, const DiffVPath& s) {
         if (!ADJ) vpd_finish(D, p, s, lpath);
       });
@@ -394,7 +417,7 @@ int launch_vpdiff(flg_ctx* ctx, const flg_render_cfg* cfg, const TraceGeom& g, c
   const int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
   const int sched = cfg->schedule ? cfg->schedule : 2;
   if (sched == 1) {
-    k_vpdiff<ADJ, COMPAT><<<div_up(npaths, 256), 256, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp,
+    k_vpdiff<ADJ, COMPAT><<<div_up(npaths, kMeshBlock), kMeshBlock, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp,
                                                                         cfg->max_depth, lpath, dl, grad, galpha);
     return FLG_OK;
   }
@@ -403,8 +426,8 @@ int launch_vpdiff(flg_ctx* ctx, const flg_render_cfg* cfg, const TraceGeom& g, c
       (unsigned long long*)scratch(ctx, ADJ ? "vpdiff_counter_adj" : "vpdiff_counter_fwd", 256, &st);
   if (!counter) return st;
   k_vp_zero<<<1, 64, 0, ctx->stream>>>(counter);
-  const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
-  k_vpdiff_regen<ADJ, COMPAT><<<grid, 256, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp, cfg->max_depth,
+  const int grid = (int)std::min<int64_t>(div_up(npaths, kMeshBlock), (int64_t)ctx->num_cus * 8);
+  k_vpdiff_regen<ADJ, COMPAT><<<grid, kMeshBlock, 0, ctx->stream>>>(g, D, L, tab, hp, cam, cfg->spp, cfg->max_depth,
                                                              lpath, dl, grad, galpha, (unsigned int*)counter);
   return FLG_OK;
 }
@@ -423,9 +446,9 @@ void launch_volpath(int sched, int blocks1, int blocks2, hipStream_t st, const T
                     const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
                     int32_t spp, int32_t D, float* lpath, unsigned int* next_path, unsigned long long* nl) {
   if (sched == 1)
-    k_volpath<COMPAT, COUNT><<<blocks1, 256, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, nl);
+    k_volpath<COMPAT, COUNT><<<blocks1, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, nl);
   else
-    k_volpath_regen<COMPAT, COUNT><<<blocks2, 256, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, next_path, nl);
+    k_volpath_regen<COMPAT, COUNT><<<blocks2, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, next_path, nl);
 }
 
 }  // namespace
@@ -469,8 +492,8 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
   const TraceGeom g = geom_of(bvh);
   const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
   const bool count = h_stats != nullptr;
-  const int blocks1 = (int)div_up(npaths, 256);
-  const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+  const int blocks1 = (int)div_up(npaths, kMeshBlock);
+  const int blocks2 = (int)std::min<int64_t>(div_up(npaths, kMeshBlock), (int64_t)ctx->num_cus * 8);
   {
     ScopedTimer tm(ctx, "volpath_mesh_paths");
     k_vp_zero<<<1, 64, 0, ctx->stream>>>(nl);
