@@ -61,6 +61,12 @@ namespace flg {
 namespace {
 
 constexpr int kTopStack = 64;  // > depth of a median-split tree over < 2^31 meshes
+// Launch shape of the trace: 256-lane blocks, the 8 most recent entries of
+// each lane's mesh traversal stack in LDS (the path tracer's ring stack), 8
+// waves per SIMD.
+constexpr int kSceneBlock = 256;
+constexpr int kSceneRing = 8;
+constexpr int kSceneWaves = 8;
 
 struct Box {
   float lo[3], hi[3];
@@ -139,8 +145,9 @@ struct SceneDev {
 };
 
 template <bool COMPAT>
-__global__ __launch_bounds__(256) void k_scene_trace(SceneDev S, const flg_ray* rays, flg_scene_hit* hits,
-                                                     int64_t n) {
+__global__ __launch_bounds__(kSceneBlock, kSceneWaves) void k_scene_trace(SceneDev S, const flg_ray* rays,
+                                                                          flg_scene_hit* hits, int64_t n) {
+  __shared__ uint2 ring[kSceneBlock * kSceneRing];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4* r = reinterpret_cast<const float4*>(rays + i);
@@ -168,7 +175,7 @@ __global__ __launch_bounds__(256) void k_scene_trace(SceneDev S, const flg_ray* 
         for (int k = 0; k < 6; ++k) g.root[k] = M.root[k];
         float tb = best;
         Counters c{0, 0};
-        const int32_t p = trace_one<COMPAT, false>(g, o, d, tb, c);
+        const int32_t p = trace_lane<COMPAT, false, kSceneBlock, kSceneRing>(g, o, d, tb, c, ring);
         if (p >= 0 && (tb < best || M.rank < best_rank)) {
           best = tb;
           best_rank = M.rank;
@@ -303,9 +310,9 @@ int flg_scene_trace_closest(flg_ctx* ctx, const flg_scene_accel* s, const flg_ra
   for (int k = 0; k < 6; ++k) S.ref_box[k] = s->ref_box[k];
   ScopedTimer tm(ctx, "scene_trace");
   if (s->compat)
-    k_scene_trace<true><<<div_up(n, 256), 256, 0, ctx->stream>>>(S, d_rays, d_hits, n);
+    k_scene_trace<true><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
   else
-    k_scene_trace<false><<<div_up(n, 256), 256, 0, ctx->stream>>>(S, d_rays, d_hits, n);
+    k_scene_trace<false><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
   FLG_CHECK_LAUNCH(ctx);
   return FLG_OK;
 }
