@@ -44,3 +44,29 @@ def test_scene_accel_rejects_mixed_normal_policies(ctx):
                        torch.from_numpy(meshes[1][1].ravel()).cuda(), compat=True)
     with pytest.raises(Exception):
         api.SceneAccel(ctx, [a, b])
+
+
+def test_scene_trace_edge_cases(ctx):
+    """Single-triangle meshes (including one on the plane of another), rays
+    with tmax 0 and rays along the triangles' plane, and an empty batch."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    meshes = [(tri + np.float32(k) * np.array([0.3, 0.2, 0.0], np.float32), np.arange(3, dtype=np.int32))
+              for k in range(5)]
+    meshes.append((tri.copy(), np.arange(3, dtype=np.int32)))  # the first mesh again: ties with it
+    rng = np.random.default_rng(8)
+    n = 4000
+    o = np.concatenate([rng.uniform(-0.5, 2.0, (n, 2)), np.full((n, 1), -1.0)], axis=1).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = np.inf
+    rays[:, 4:7] = np.array([0, 0, 1], np.float32)
+    rays[: n // 8, 3] = 0.0                              # tmax 0: nothing is closer
+    rays[n // 8: n // 4, 2] = 0.0                        # in the plane of the triangles
+    rays[n // 8: n // 4, 4:7] = np.array([1, 0, 0], np.float32)
+    sa, hits, mesh, ids, ts = _case(ctx, meshes, rays)
+    assert np.array_equal(hits[:, 0], mesh) and np.array_equal(hits[:, 1], ids)
+    assert np.array_equal(hits[:, 2], ts.view(np.int32))
+    assert np.all(mesh[: n // 8] == -1)
+    assert not np.any(mesh == 5)                         # the repeated mesh loses every tie
+    empty = sa.trace(torch.zeros((0, 8), dtype=torch.float32, device="cuda"))
+    assert empty.shape == (0, 4)

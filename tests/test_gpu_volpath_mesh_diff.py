@@ -170,3 +170,33 @@ def test_mesh_diff_rejects_bad_bricks(ctx):
                                           ctypes.byref(diff._params(S, ALBEDO, G, mu)), lpath.data_ptr(), None,
                                           dl.data_ptr(), ctypes.byref(bad), g.data_ptr(), ga.data_ptr())
     assert st != 0
+
+
+def test_mesh_adjoint_ragged_bricks_and_depth_one(ctx):
+    """A 30 x 20 x 17 grid (no dimension a multiple of the 8-voxel brick, so
+    the last bricks are partial) and max_depth 1: the brick gradient equals
+    the dense one on the active bricks, and both match the FP64 twin."""
+    v, ind = _sphere(3)
+    rng = np.random.default_rng(5)
+    grid = (0.1 + 0.8 * rng.random((17, 20, 30))).astype(np.float32)
+    keep = rng.random((3, 3, 4)) < 0.5
+    mask = np.repeat(np.repeat(np.repeat(keep, 8, 0), 8, 1), 8, 2)[:17, :20, :30]
+    grid = (grid * mask).astype(np.float32)
+    W, H, spp, D = 24, 20, 2, 1
+    print("mesh adjoint ragged bricks", flush=True)
+    r = _adjoint(ctx, v, ind, grid, W, H, spp, D, 13, bricks_of=8)
+    layout = r["layout"]
+    assert layout.n_active == int(keep.sum())
+    dense_b = layout.to_dense(r["gbricks"]).cpu().numpy().astype(np.float64)
+    g = r["grad"]
+    assert np.abs(g).max() > 0
+    assert np.abs(dense_b[~mask]).max() == 0
+    assert np.abs(dense_b[mask] - g[mask]).max() <= 1e-5 * np.abs(g).max()
+    mu = r["mu"]
+    m = orc.Lbvh(v, ind)
+    lp64, _, _ = m.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D)
+    lp = r["lpath"].cpu().numpy().astype(np.float64)
+    assert np.abs(lp - lp64).max() <= 1e-4 * max(float(np.abs(lp64).max()), 1e-6)
+    g64, _ = m.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D, lp64,
+                         r["dl"].astype(np.float64).sum(axis=2))
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) < 1e-3

@@ -137,3 +137,14 @@ def test_ply_errors(tmp_path):
     bad_magic = str(tmp_path / "magic.ply")
     open(bad_magic, "wb").write(b"plx\nformat ascii 1.0\nend_header\n")
     _expect_error(bad_magic)
+
+
+def test_ply_vertices_without_faces(tmp_path):
+    """A face element of count 0 is a mesh of no triangles; no face element
+    at all is an error (MakeTriangleMesh fails without faces)."""
+    v, _ = _sphere()
+    p = str(tmp_path / "nofaces.ply")
+    elems = _mesh_elems(v, np.zeros((0, 3), np.int32))
+    _write(p, "binary_little_endian", elems)
+    gv, gi, _, _ = api.load_ply(p)
+    assert gv.shape == v.shape and (gi is None or gi.size == 0)
