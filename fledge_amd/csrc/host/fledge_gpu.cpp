@@ -7,7 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstring>
+#include <fstream>
+#include <iterator>
 #include <new>
 
 #include "../traverse.hpp"
@@ -662,6 +665,285 @@ std::vector<flg_scene_hit> SceneAccel::intersectBatch(const std::vector<Ray>& ra
   m_ctx->check(flg_memcpy_dtoh(m_ctx->get(), h.data(), dh.data(), h.size() * sizeof(flg_scene_hit)),
                "flg_memcpy_dtoh");
   return h;
+}
+
+// ----------------------------------------------------------- scene file ---
+namespace {
+
+// An XML element: name, attributes in order, child elements (text is not
+// kept: the scene format carries its values in attributes).
+struct XNode {
+  std::string name;
+  std::vector<std::pair<std::string, std::string>> attrs;
+  std::vector<XNode> kids;
+  const std::string* attr(const char* k) const {
+    for (const auto& a : attrs)
+      if (a.first == k) return &a.second;
+    return nullptr;
+  }
+  const XNode* child(const char* k) const {
+    for (const auto& c : kids)
+      if (c.name == k) return &c;
+    return nullptr;
+  }
+};
+
+[[noreturn]] void scene_error(const std::string& what) { throw gpu::Error(FLG_ERR_INVALID, "scene file: " + what); }
+
+// Elements, attributes, comments, processing instructions, a DOCTYPE line
+// and the five predefined entities.
+class XReader {
+ public:
+  explicit XReader(const std::string& text) : s_(text) {}
+  XNode document() {
+    skip_misc();
+    XNode root = element();
+    skip_misc();
+    if (i_ != s_.size()) scene_error("content after the root element");
+    return root;
+  }
+
+ private:
+  const std::string& s_;
+  size_t i_ = 0;
+  bool starts(const char* p) const { return s_.compare(i_, std::strlen(p), p) == 0; }
+  void ws() {
+    while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
+  }
+  void skip_to(const char* end) {
+    const size_t e = s_.find(end, i_);
+    if (e == std::string::npos) scene_error("unterminated markup");
+    i_ = e + std::strlen(end);
+  }
+  void skip_misc() {  // whitespace, comments, <?...?>, <!DOCTYPE ...>
+    while (true) {
+      ws();
+      if (starts("<!--")) skip_to("-->");
+      else if (starts("<?")) skip_to("?>");
+      else if (starts("<!")) skip_to(">");
+      else return;
+    }
+  }
+  std::string name() {
+    const size_t b = i_;
+    while (i_ < s_.size() && (std::isalnum((unsigned char)s_[i_]) || s_[i_] == '_' || s_[i_] == '-' ||
+                              s_[i_] == '.' || s_[i_] == ':'))
+      ++i_;
+    if (i_ == b) scene_error("a name was expected");
+    return s_.substr(b, i_ - b);
+  }
+  std::string unescape(const std::string& v) {
+    std::string o;
+    for (size_t k = 0; k < v.size(); ++k) {
+      if (v[k] != '&') {
+        o += v[k];
+        continue;
+      }
+      const size_t e = v.find(';', k);
+      if (e == std::string::npos) scene_error("bad entity");
+      const std::string ent = v.substr(k + 1, e - k - 1);
+      if (ent == "lt") o += '<';
+      else if (ent == "gt") o += '>';
+      else if (ent == "amp") o += '&';
+      else if (ent == "quot") o += '"';
+      else if (ent == "apos") o += '\'';
+      else scene_error("unknown entity &" + ent + ";");
+      k = e;
+    }
+    return o;
+  }
+  XNode element() {
+    if (!starts("<")) scene_error("an element was expected");
+    ++i_;
+    XNode n;
+    n.name = name();
+    while (true) {
+      ws();
+      if (starts("/>")) {
+        i_ += 2;
+        return n;
+      }
+      if (starts(">")) {
+        ++i_;
+        break;
+      }
+      std::string k = name();
+      ws();
+      if (!starts("=")) scene_error("'=' expected after attribute " + k);
+      ++i_;
+      ws();
+      if (i_ >= s_.size() || (s_[i_] != '"' && s_[i_] != '\'')) scene_error("a quoted value was expected");
+      const char q = s_[i_++];
+      const size_t e = s_.find(q, i_);
+      if (e == std::string::npos) scene_error("unterminated attribute value");
+      n.attrs.emplace_back(k, unescape(s_.substr(i_, e - i_)));
+      i_ = e + 1;
+    }
+    while (true) {  // content: child elements; text and comments are skipped
+      const size_t lt = s_.find('<', i_);
+      if (lt == std::string::npos) scene_error("unterminated element " + n.name);
+      i_ = lt;
+      if (starts("</")) {
+        i_ += 2;
+        if (name() != n.name) scene_error("mismatched end tag of " + n.name);
+        ws();
+        if (!starts(">")) scene_error("'>' expected");
+        ++i_;
+        return n;
+      }
+      if (starts("<!--")) skip_to("-->");
+      else if (starts("<?")) skip_to("?>");
+      else n.kids.push_back(element());
+    }
+  }
+};
+
+const std::string& need_attr(const XNode& n, const char* k) {
+  const std::string* v = n.attr(k);
+  if (!v) scene_error("<" + n.name + "> has no " + k);
+  return *v;
+}
+
+int to_int(const std::string& v) {
+  size_t used = 0;
+  int x = 0;
+  try {
+    x = std::stoi(v, &used);
+  } catch (const std::exception&) {
+    scene_error("not an integer: " + v);
+  }
+  if (used != v.size()) scene_error("not an integer: " + v);
+  return x;
+}
+
+Float to_float(const std::string& v) {
+  size_t used = 0;
+  Float x = 0;
+  try {
+    x = std::stof(v, &used);
+  } catch (const std::exception&) {
+    scene_error("not a number: " + v);
+  }
+  if (used != v.size()) scene_error("not a number: " + v);
+  return x;
+}
+
+// parseVector3f (scene.cpp:70-80): three comma-separated numbers
+Vector3f to_vec3(const std::string& v) {
+  std::vector<std::string> tok;
+  size_t b = 0;
+  while (true) {
+    const size_t e = v.find(',', b);
+    tok.push_back(v.substr(b, e == std::string::npos ? std::string::npos : e - b));
+    if (e == std::string::npos) break;
+    b = e + 1;
+  }
+  if (tok.size() != 3) scene_error("a vector of three components was expected: " + v);
+  Vector3f r;
+  for (int k = 0; k < 3; ++k) {
+    size_t used = 0;
+    try {
+      r[k] = std::stof(tok[k], &used);
+    } catch (const std::exception&) {
+      scene_error("not a number: " + tok[k]);
+    }
+  }
+  return r;
+}
+
+std::string resolve(const std::string& base, const std::string& p) {
+  if (!p.empty() && p[0] == '/') return p;
+  if (base.empty()) return p;
+  return base + "/" + p;
+}
+
+}  // namespace
+
+SceneFileDesc ParseSceneXML(const std::string& filename, Scene& scene) {
+  std::ifstream f(filename, std::ios::binary);
+  if (!f) scene_error("cannot read " + filename);
+  const std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  const XNode root = XReader(text).document();
+  if (root.name != "scene") scene_error("the root element is not <scene>");
+  (void)need_attr(root, "version");
+  SceneFileDesc out;
+  const size_t slash = filename.rfind('/');
+  out.base_dir = slash == std::string::npos ? std::string(".") : filename.substr(0, slash);
+  // setIntegrator (scene.cpp:83-112)
+  const XNode* integ = root.child("integrator");
+  if (!integ) scene_error("no <integrator>");
+  const std::string& itype = need_attr(*integ, "type");
+  if (itype == "path") scene.m_integrator_type = EIntegratorType::EPathIntegrator;
+  else if (itype == "volpath") scene.m_integrator_type = EIntegratorType::EVolPathIntegrator;
+  else scene_error("integrator type " + itype);
+  for (const XNode& c : integ->kids)
+    if (c.name == "integer" && need_attr(c, "name") == "maxDepth") scene.m_maxDepth = to_int(need_attr(c, "value"));
+  // setSensor (scene.cpp:114-163)
+  const XNode* sensor = root.child("sensor");
+  if (!sensor) scene_error("no <sensor>");
+  if (need_attr(*sensor, "type") != "perspective") scene_error("sensor type " + need_attr(*sensor, "type"));
+  for (const XNode& c : sensor->kids)
+    if (c.name == "float" && need_attr(c, "name") == "fov") scene.m_FoV = to_float(need_attr(c, "value"));
+  const XNode* tr = sensor->child("transform");
+  const XNode* look = tr ? tr->child("lookat") : nullptr;
+  if (!look) scene_error("no <transform><lookat>");
+  scene.m_up = to_vec3(need_attr(*look, "up"));
+  scene.m_origin = to_vec3(need_attr(*look, "origin"));
+  scene.m_target = to_vec3(need_attr(*look, "target"));
+  const XNode* sampler = sensor->child("sampler");
+  const XNode* count = sampler ? sampler->child("integer") : nullptr;
+  if (!count) scene_error("no <sampler><integer>");
+  scene.m_SPP = to_int(need_attr(*count, "value"));
+  const XNode* film = sensor->child("film");
+  if (!film) scene_error("no <film>");
+  for (const XNode& c : film->kids) {
+    if (c.name != "integer") continue;
+    const std::string& nm = need_attr(c, "name");
+    if (nm == "height") scene.m_resY = to_int(need_attr(c, "value"));
+    if (nm == "width") scene.m_resX = to_int(need_attr(c, "value"));
+  }
+  // shapes and emitters in file order (scene.cpp:165-303)
+  for (const XNode& c : root.kids) {
+    if (c.name == "shape") {
+      SceneShapeDesc sh;
+      sh.type = need_attr(c, "type");
+      const XNode* bsdf = c.child("bsdf");
+      if (!bsdf) scene_error("a shape without <bsdf>");
+      sh.bsdf = need_attr(*bsdf, "type");
+      if (sh.bsdf == "dielectric") {
+        for (const XNode& b : bsdf->kids) {
+          if (b.name != "float") continue;
+          const std::string& nm = need_attr(b, "name");
+          if (nm == "intIOR") sh.int_ior = to_float(need_attr(b, "value"));
+          if (nm == "extIOR") sh.ext_ior = to_float(need_attr(b, "value"));
+        }
+      } else if (sh.bsdf != "diffuse" && sh.bsdf != "roughconductor") {
+        scene_error("bsdf type " + sh.bsdf);
+      }
+      if (sh.type == "ply") {
+        for (const XNode& s2 : c.kids)
+          if (s2.name == "string" && need_attr(s2, "name") == "filename") {
+            SceneShapeDesc one = sh;
+            one.filename = resolve(out.base_dir, need_attr(s2, "value"));
+            out.shapes.push_back(one);
+          }
+      } else if (sh.type == "sphere") {
+        const XNode* pt = c.child("point");
+        const XNode* rad = c.child("float");
+        if (!pt || !rad) scene_error("a sphere without <point> or <float>");
+        sh.center = Vector3f(to_float(need_attr(*pt, "x")), to_float(need_attr(*pt, "y")),
+                             to_float(need_attr(*pt, "z")));
+        sh.radius = to_float(need_attr(*rad, "value"));
+        out.shapes.push_back(sh);
+      }
+    } else if (c.name == "emitter") {
+      if (need_attr(c, "type") != "envmap") scene_error("emitter type " + need_attr(c, "type"));
+      for (const XNode& s2 : c.kids)
+        if (s2.name == "string" && need_attr(s2, "name") == "filename")
+          out.envmaps.push_back(resolve(out.base_dir, need_attr(s2, "value")));
+    }
+  }
+  return out;
 }
 
 // -------------------------------------------------------------- Render ----

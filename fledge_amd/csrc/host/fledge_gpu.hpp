@@ -321,6 +321,32 @@ class GPUVolIntegrator : public Integrator {
   uint64_t m_stats[2] = {0, 0};
 };
 
+// ----------------------------------------------------------- scene file ---
+// The entries of a scene file that Scene::parseXML reads (src/scene.cpp:
+// 165-326; Mitsuba 0.6 XML through boost's property tree in the reference,
+// a reader of that XML subset here).  File names are resolved against the
+// scene file's directory, as Scene::getPath does (absolute names are kept).
+struct SceneShapeDesc {
+  std::string type;      // "ply" or "sphere"
+  std::string filename;  // ply: the resolved file name
+  std::string bsdf;      // "dielectric", "diffuse" or "roughconductor"
+  Float int_ior = 1.0f, ext_ior = 1.0f;  // dielectric
+  Vector3f center{0.0f};                 // sphere
+  Float radius = 0;
+};
+struct SceneFileDesc {
+  std::string base_dir;
+  std::vector<SceneShapeDesc> shapes;  // in file order (the reference's m_primitives order)
+  std::vector<std::string> envmaps;    // resolved file names of the envmap emitters
+};
+// Fills scene's resolution, spp, max depth, fov, camera and integrator type
+// and returns the shapes and emitters.  Throws gpu::Error (FLG_ERR_INVALID)
+// where the reference stops: an unreadable or malformed file, a missing
+// version, integrator or sensor, an integrator other than path / volpath, a
+// sensor that is not perspective, a bsdf other than the three above, an
+// emitter that is not an envmap, a lookat vector without three components.
+SceneFileDesc ParseSceneXML(const std::string& filename, Scene& scene);
+
 // -------------------------------------------------------------- Render ----
 // The backend bit of this renderer; EBackendType names the reference's own.
 constexpr EBackendType kHIPBackend = static_cast<EBackendType>(1 << 3);

@@ -7,12 +7,14 @@
 // Prints one line per check and exits non-zero on the first failure.
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
 #include <unistd.h>
 
 #include "fledge_gpu.hpp"
@@ -335,6 +337,95 @@ void test_ply_host() {
   std::printf("PASS ply_host (ascii and binary, %zu vertices)\n", c.v.size() / 3);
 }
 
+// Scene::parseXML's subset through ParseSceneXML on a file written here in
+// the reference's scene format, then its PLY shapes through MakeTriangleMesh;
+// and the files the reference refuses.
+void test_scene_file_host() {
+  const std::string dir = "/tmp/fledge_dropin_" + std::to_string((long)getpid()) + "_scene";
+  ::mkdir(dir.c_str(), 0755);
+  const PlyCase c = sphere_case(3, 0.0f);
+  const std::string ply = write_ply("scene_mesh", false, c);
+  (void)std::rename(ply.c_str(), (dir + "/mesh.ply").c_str());
+  const char* text = R"(<?xml version='1.0' encoding='utf-8'?>
+<!-- a scene in the reference's format -->
+<scene version="0.6.0">
+  <integrator type="volpath">
+    <integer name="maxDepth" value="12" />
+  </integrator>
+  <sensor type="perspective">
+    <float name="fov" value="35.5" />
+    <string name="fovAxis" value="y" />
+    <transform name="toWorld">
+      <lookat up="0, 1, 0" origin="1.5, 2, -7.25" target="0,0.5,0" />
+    </transform>
+    <sampler type="independent"><integer name="sampleCount" value="24" /></sampler>
+    <film type="hdrfilm">
+      <integer name="width" value="320" />
+      <integer name="height" value="200" />
+      <rfilter type="box"><float name="size" value="0.5" /></rfilter>
+    </film>
+  </sensor>
+  <medium id="m" type="homogeneous"><spectrum name="sigmaS" value="3.0" /></medium>
+  <shape type="ply">
+    <string name="filename" value="mesh.ply" />
+    <bsdf type="roughconductor"><float name="alpha" value="0.1" /></bsdf>
+    <ref name="interior" id="m" />
+  </shape>
+  <shape type="sphere">
+    <point x="0.5" y="-1" z="2" />
+    <float name="radius" value="0.75" />
+    <bsdf type="dielectric"><float name="intIOR" value="1.5" /><float name="extIOR" value="1.25" /></bsdf>
+  </shape>
+  <emitter type="envmap"><string name="filename" value="/abs/sky.exr" /></emitter>
+</scene>
+)";
+  const std::string xml = dir + "/scene.xml";
+  {
+    FILE* f = std::fopen(xml.c_str(), "wb");
+    std::fputs(text, f);
+    std::fclose(f);
+  }
+  fledge::Scene sc;
+  const fledge::SceneFileDesc d = fledge::ParseSceneXML(xml, sc);
+  CHECK(sc.m_integrator_type == fledge::EIntegratorType::EVolPathIntegrator && sc.m_maxDepth == 12, "integrator");
+  CHECK(sc.m_resX == 320 && sc.m_resY == 200 && sc.m_SPP == 24 && sc.m_FoV == 35.5f, "sensor");
+  CHECK(sc.m_origin[0] == 1.5f && sc.m_origin[2] == -7.25f && sc.m_target[1] == 0.5f && sc.m_up[1] == 1.0f,
+        "lookat");
+  CHECK(d.shapes.size() == 2 && d.envmaps.size() == 1 && d.envmaps[0] == "/abs/sky.exr", "entries");
+  CHECK(d.shapes[0].type == "ply" && d.shapes[0].filename == dir + "/mesh.ply" &&
+            d.shapes[0].bsdf == "roughconductor",
+        "ply shape");
+  CHECK(d.shapes[1].type == "sphere" && d.shapes[1].radius == 0.75f && d.shapes[1].center[1] == -1.0f &&
+            d.shapes[1].int_ior == 1.5f && d.shapes[1].ext_ior == 1.25f,
+        "sphere shape");
+  auto m = fledge::MakeTriangleMesh(d.shapes[0].filename);
+  CHECK(m->mesh.nInd == (int)c.ind.size(), "the scene's ply mesh");
+  // what the reference refuses
+  const char* bad[] = {"<scene><integrator type=\"path\"/></scene>",
+                       "<scene version=\"0.6.0\"><integrator type=\"bdpt\"/></scene>",
+                       "<scene version=\"0.6.0\"><integrator type=\"path\"/><sensor type=\"orthographic\"/></scene>",
+                       "<scene version=\"0.6.0\"><integrator type=\"path\"/>",
+                       "<scene version=\"0.6.0\"><integrator type=\"path\"/><sensor type=\"perspective\">"
+                       "<transform><lookat up=\"0,1\" origin=\"0,0,0\" target=\"0,0,1\"/></transform></sensor></scene>"};
+  int refused = 0;
+  for (const char* b : bad) {
+    FILE* f = std::fopen(xml.c_str(), "wb");
+    std::fputs(b, f);
+    std::fclose(f);
+    try {
+      fledge::Scene s2;
+      (void)fledge::ParseSceneXML(xml, s2);
+    } catch (const fledge::gpu::Error& e) {
+      refused += e.status() == FLG_ERR_INVALID;
+    }
+  }
+  std::remove(xml.c_str());
+  std::remove((dir + "/mesh.ply").c_str());
+  ::rmdir(dir.c_str());
+  CHECK(refused == 5, "the malformed scene files must be refused (%d of 5)", refused);
+  std::printf("PASS scene_file_host (2 shapes, 1 emitter, 5 refused files)\n");
+}
+
 // Scene::intersect through SceneAccel over three mesh primitives read from
 // PLY files (the third repeats the first: they tie wherever it is hit), host
 // mirror and GPU batch against the oracle's NaiveBVHAccel restatement.
@@ -598,6 +689,7 @@ int main(int argc, char** argv) {
   test_grid_volume_host();
   test_host_errors();
   if (!g_failures) test_ply_host();
+  if (!g_failures) test_scene_file_host();
   if (!cpu_only && g_failures == 0) {
     try {
       fledge::gpu::Context ctx(0, nullptr);

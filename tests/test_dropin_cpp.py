@@ -26,7 +26,8 @@ def _run(args, timeout):
 def test_dropins_host_side():
     """GridVolume::tr/sample (host mirror of the device trackers) bit-equal to
     the oracle's OpenVDBVolume::tr/sample; error paths; MakeTriangleMesh on
-    ascii and binary PLY files.  No GPU call."""
+    ascii and binary PLY files; ParseSceneXML on a scene file in the
+    reference's format and on the files it refuses.  No GPU call."""
     _run(["--cpu"], 120)
 
 
@@ -51,7 +52,8 @@ def test_host_library_exports_dropins():
                 "fledge::GPUVolIntegrator::render", "fledge::gpu_memory_resource::do_allocate",
                 "fledge::managed_memory_resource::do_allocate",
                 "fledge::async_memory_resource::do_allocate", "fledge::GlobalManagedMemoryResource",
-                "fledge::SceneAccel::intersect", "fledge::MakeTriangleMesh", "fledge::HIPRender::render"):
+                "fledge::SceneAccel::intersect", "fledge::MakeTriangleMesh", "fledge::HIPRender::render",
+                "fledge::ParseSceneXML"):
         assert sym in out, sym
 
 
