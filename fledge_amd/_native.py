@@ -94,7 +94,8 @@ class PlyMesh(ctypes.Structure):
 
 
 class BrickLayout(ctypes.Structure):
-    _fields_ = [("brick", c_i32), ("n_active", c_i32), ("d_slot", c_vp)]
+    _fields_ = [("brick", c_i32), ("n_active", c_i32), ("nx", c_i32), ("ny", c_i32), ("nz", c_i32),
+                ("d_slot", c_vp)]
 
 
 _SIGS = {

@@ -340,6 +340,8 @@ int diff_set_bricks(flg_ctx* ctx, const flg_brick_layout* bl, DiffDev* D) {
   while (sh <= 5 && (1 << sh) != bl->brick) ++sh;
   if (sh > 5 || !bl->d_slot || bl->n_active < 0)
     return set_error(ctx, FLG_ERR_INVALID, "gradient bricks: brick must be 1, 2, 4, 8, 16 or 32 voxels, with a slot map");
+  if (bl->nx != D->v.nx || bl->ny != D->v.ny || bl->nz != D->v.nz)
+    return set_error(ctx, FLG_ERR_INVALID, "gradient bricks: the layout was built for a grid of other dimensions");
   D->brick_slot = bl->d_slot;
   D->brick_shift = sh;
   D->nbx = (D->v.nx + bl->brick - 1) >> sh;

@@ -127,6 +127,7 @@ class BrickLayout:
         self.c = N.BrickLayout()
         self.c.brick = brick
         self.c.n_active = self.n_active
+        self.c.nx, self.c.ny, self.c.nz = nx, ny, nz
         self.c.d_slot = self.slot.data_ptr()
 
     def zeros(self):

@@ -3,7 +3,10 @@
  * radix-tree LBVH build, closest-hit traversal with the watertight Pluecker
  * test, primary-ray generation with the Halton sampler, the volume-only path
  * integrator (delta + ratio tracking, HG phase, next-event estimation) and
- * its path-replay adjoint.
+ * its path-replay adjoint; around them the volumetric path integrator over an
+ * interface mesh (forward and adjoint over active bricks), the reference's
+ * render loop and film, EXR output, PLY reading and the accel over a scene's
+ * meshes.
  *
  * Conventions
  *  - Every function returns an int status (FLG_OK == 0); the message of the
@@ -38,6 +41,12 @@
  *    (src/integrator.cpp:123-235, 328-405).
  *  - flg_pathtrace_diffuse: PathIntegrator::Li's segment loop
  *    (src/integrator.cpp:252-326) over a diffuse mesh.
+ *  - flg_volpath_mesh_render: VolPathIntegrator::Li (src/integrator.cpp:
+ *    407-510) with VolEstimateTr (:40-64) over one interface mesh.
+ *  - flg_film_save_exr: Film::saveBuffer (src/film.cpp:13-49).
+ *  - flg_ply_load: MakeTriangleMesh (src/plymesh.cpp:19-86).
+ *  - flg_scene_accel_build / flg_scene_trace_closest: Scene::init's accel and
+ *    Scene::intersect (src/scene.cpp:46-59) over the scene's MeshPrimitives.
  */
 #ifndef FLEDGE_HIP_H_
 #define FLEDGE_HIP_H_
@@ -413,7 +422,8 @@ FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume*
 typedef struct flg_brick_layout {
   int32_t brick;          /* 1, 2, 4, 8, 16 or 32 */
   int32_t n_active;
-  const int32_t* d_slot;  /* device, one per brick */
+  int32_t nx, ny, nz;     /* the grid the layout was built for (checked) */
+  const int32_t* d_slot;  /* device, one per brick, as flg_bricks_build writes it */
 } flg_brick_layout;
 
 /* Active bricks of a dense device grid (nx*ny*nz floats, x fastest): a brick

@@ -159,8 +159,13 @@ def test_mesh_diff_rejects_bad_bricks(ctx):
     W, H, spp, D = 8, 8, 1, 4
     mu = S * float(grid.max())
     _, lpath = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu)
+    layout = diff.BrickLayout(ctx, torch.from_numpy(grid).cuda(), 8)
+    wrong_dims = N.BrickLayout()
+    wrong_dims.brick, wrong_dims.n_active, wrong_dims.d_slot = 8, layout.n_active, layout.slot.data_ptr()
+    wrong_dims.nx, wrong_dims.ny, wrong_dims.nz = 32, 16, 16
     bad = N.BrickLayout()
     bad.brick, bad.n_active, bad.d_slot = 3, 0, None
+    bad.nx, bad.ny, bad.nz = 16, 16, 16
     g = torch.zeros(16, dtype=torch.float32, device="cuda")
     ga = torch.zeros(1, dtype=torch.float32, device="cuda")
     dl = torch.ones((H, W, 3), dtype=torch.float32, device="cuda")
@@ -169,6 +174,11 @@ def test_mesh_diff_rejects_bad_bricks(ctx):
                                           ctypes.byref(diff._cfg(W, H, spp, D)),
                                           ctypes.byref(diff._params(S, ALBEDO, G, mu)), lpath.data_ptr(), None,
                                           dl.data_ptr(), ctypes.byref(bad), g.data_ptr(), ga.data_ptr())
+    assert st != 0
+    st = ctx.lib.flg_volpath_mesh_adjoint(ctx.h, bvh.h, dv.vol.h, None, ctypes.byref(scene),
+                                          ctypes.byref(diff._cfg(W, H, spp, D)),
+                                          ctypes.byref(diff._params(S, ALBEDO, G, mu)), lpath.data_ptr(), None,
+                                          dl.data_ptr(), ctypes.byref(wrong_dims), g.data_ptr(), ga.data_ptr())
     assert st != 0
 
 
