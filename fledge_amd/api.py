@@ -299,12 +299,14 @@ class Volume:
             return out, {"paths": int(st[0]), "lookups": int(st[1])}
         return out
 
-    def render_mesh(self, bvh, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0):
+    def render_mesh(self, bvh, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0):
         """VolPathIntegrator: `bvh`'s closed mesh as the interface around this
-        volume's medium (placed on the mesh's bounding box); RGB image."""
+        volume's medium (placed on the mesh's bounding box); RGB image.
+        film=1: the reference's render loop and Mitchell-filtered film."""
         cfg = N.RenderCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
         cfg.schedule = schedule
+        cfg.film = film
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
         st = (ctypes.c_uint64 * 2)()

@@ -997,6 +997,7 @@ bool HIPRender::render() {
   cfg.height = sc.m_resY;
   cfg.spp = sc.m_SPP;
   cfg.max_depth = sc.m_maxDepth;
+  cfg.film = 1;  // the reference's render loop and film, as CPURender's integrators use
   const size_t n = size_t(sc.m_resX) * sc.m_resY * 3;
   gpu::DeviceBuffer d(ctx, n * sizeof(float));
   uint64_t stats[2] = {0, 0};

@@ -361,7 +361,7 @@ constexpr EBackendType kHIPBackend = static_cast<EBackendType>(1 << 3);
 //   * a volume-integrator scene whose one primitive is an LBVHAccel (a
 //     closed interface mesh) and whose m_volume is a GridVolume (the medium
 //     inside the mesh, on its bounding box): VolPathIntegrator over the mesh
-//     (flg_volpath_mesh_render), the per-pixel mean of the samples.
+//     (flg_volpath_mesh_render) with the reference's render loop and film.
 // Any other scene makes init() throw gpu::Error (FLG_ERR_INVALID).
 class HIPRender : public RenderBase {
  public:

@@ -91,10 +91,25 @@ struct VPath {
   float L, beta;
   int bounces, crossings;
   bool inside;
+  // film mode: the pixel, the film position of the splatted sample, and
+  // whether the path in flight is the pixel-centre path (not splatted)
+  uint32_t pix;
+  float fx, fy;
+  bool centre;
 };
 
+// The camera ray at film position (fx, fy); the camera is outside the mesh.
+FLG_D void vp_start(const CamDevV& cam, float fx, float fy, VPath& s) {
+  gen_ray_v(cam, fx, fy, &s.o, &s.d);
+  s.L = 0.0f;
+  s.beta = 1.0f;
+  s.bounces = 0;
+  s.crossings = 0;
+  s.inside = false;
+}
+
 // Camera ray of path p (pixel p / spp, sample p % spp) on the per-sample
-// Halton schedule; the camera is outside the mesh.
+// Halton schedule.
 FLG_D void vp_begin(const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam, int32_t spp,
                     uint32_t p, VPath& s) {
   const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
@@ -103,12 +118,41 @@ FLG_D void vp_begin(const HaltonTables& tab, const HaltonParams& hp, const CamDe
   s.ps.dim = 0;
   const float ux = s.ps.next(tab, hp);
   const float uy = s.ps.next(tab, hp);
-  gen_ray_v(cam, (float)x + ux, (float)y + uy, &s.o, &s.d);
-  s.L = 0.0f;
-  s.beta = 1.0f;
-  s.bounces = 0;
-  s.crossings = 0;
-  s.inside = false;
+  vp_start(cam, (float)x + ux, (float)y + uy, s);
+}
+
+// Film task p of the reference's render loop (tasks = spp - 1 per pixel, the
+// schedule of the volume-only integrator's film mode): task 0 runs the
+// pixel-centre path, then sample 1 on the same Halton index from the
+// dimension the centre path reached; task t >= 1 is sample t + 1 on index
+// offset + t * stride from dimension 0.
+FLG_D void vp_film_begin(const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam, int32_t tasks,
+                         uint32_t p, VPath& s) {
+  const uint32_t pix = p / (uint32_t)tasks, t = p - pix * (uint32_t)tasks;
+  const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+  s.pix = pix;
+  s.ps.index = halton_pixel_offset(hp, x, y) + t * hp.stride;
+  s.ps.dim = 0;
+  s.centre = t == 0;
+  if (s.centre) {
+    vp_start(cam, (float)x + 0.5f, (float)y + 0.5f, s);
+    return;
+  }
+  s.fx = (float)x + s.ps.next(tab, hp);
+  s.fy = (float)y + s.ps.next(tab, hp);
+  vp_start(cam, s.fx, s.fy, s);
+}
+
+// A film task's path has completed: the centre path goes on to sample 1
+// (false: a path is running again); any other path completes the task.
+FLG_D bool vp_film_next(const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam, VPath& s) {
+  if (!s.centre) return true;
+  const int x = (int)(s.pix % (uint32_t)cam.w), y = (int)(s.pix / (uint32_t)cam.w);
+  s.centre = false;
+  s.fx = (float)x + s.ps.next(tab, hp);
+  s.fy = (float)y + s.ps.next(tab, hp);
+  vp_start(cam, s.fx, s.fy, s);
+  return false;
 }
 
 // One trip of the reference's loop; returns true when the path is complete.
@@ -159,48 +203,70 @@ FLG_D void vp_count(uint32_t nlook, unsigned long long* nlookups) {
   }
 }
 
-// One lane per path sample.
-template <bool COMPAT, bool COUNT>
-__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
-                                                 HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
-                                                 float* lpath, unsigned long long* nlookups) {
+// Task p: path sample p (pixel p / tasks, sample p % tasks), or with FILM
+// film task p of the render loop.
+template <bool FILM>
+FLG_D void vp_task_begin(const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam, int32_t tasks,
+                         uint32_t p, VPath& s) {
+  if (FILM) vp_film_begin(tab, hp, cam, tasks, p, s);
+  else vp_begin(tab, hp, cam, tasks, p, s);
+}
+
+template <bool FILM>
+FLG_D void vp_task_finish(uint32_t p, const VPath& s, float* lpath, float2* fpos) {
+  lpath[p] = s.L;
+  if (FILM) fpos[p] = make_float2(s.fx, s.fy);
+}
+
+// One lane per task.
+template <bool COMPAT, bool COUNT, bool FILM>
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g, VolDev v, LightsDev L,
+                                                                    HaltonTables tab_g, HaltonParams hp,
+                                                                    CamDevV cam, int32_t tasks, int32_t max_depth,
+                                                                    float* lpath, float2* fpos,
+                                                                    unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
   __shared__ uint2 ring[kMeshBlock * kMeshRing];
   const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t npaths = (int64_t)cam.w * cam.h * spp;
+  const int64_t ntasks = (int64_t)cam.w * cam.h * tasks;
   uint32_t nlook = 0;
-  if (p < npaths) {
+  if (p < ntasks) {
     VPath s;
-    vp_begin(tab, hp, cam, spp, (uint32_t)p, s);
-    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook)) {
+    vp_task_begin<FILM>(tab, hp, cam, tasks, (uint32_t)p, s);
+    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook) || (FILM && !vp_film_next(tab, hp, cam, s))) {
     }
-    lpath[p] = s.L;
+    vp_task_finish<FILM>((uint32_t)p, s, lpath, fpos);
   }
   vp_count<COUNT>(nlook, nlookups);
 }
 
-// Persistent lanes with path regeneration (regen_loop); same per-path code.
-template <bool COMPAT, bool COUNT>
-__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceGeom g, VolDev v, LightsDev L, HaltonTables tab_g,
-                                                       HaltonParams hp, CamDevV cam, int32_t spp,
-                                                       int32_t max_depth, float* lpath, unsigned int* next_path,
-                                                       unsigned long long* nlookups) {
+// Persistent lanes with path regeneration (regen_loop); same per-task code.
+template <bool COMPAT, bool COUNT, bool FILM>
+__global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceGeom g, VolDev v, LightsDev L,
+                                                                          HaltonTables tab_g, HaltonParams hp,
+                                                                          CamDevV cam, int32_t tasks,
+                                                                          int32_t max_depth, float* lpath,
+                                                                          float2* fpos, unsigned int* next_path,
+                                                                          unsigned long long* nlookups) {
   __shared__ int2 lds_primes[kLocalPrimes];
   __shared__ uint2 ring[kMeshBlock * kMeshRing];
   const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
-  const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
+  const uint32_t ntasks = (uint32_t)((int64_t)cam.w * cam.h * tasks);
   uint32_t nlook = 0;
   regen_loop<VPath>(
-      npaths, next_path,
+      ntasks, next_path,
       [&](uint32_t p, VPath& s) {
-        vp_begin(tab, hp, cam, spp, p, s);
+        vp_task_begin<FILM>(tab, hp, cam, tasks, p, s);
         return false;
       },
-      [&](VPath& s) { return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook); },
-      [&](uint32_t p, const VPath& s) { lpath[p] = s.L; });
+      [&](VPath& s) {
+        return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook) &&
+               (!FILM || vp_film_next(tab, hp, cam, s));
+      },
+      [&](uint32_t p, const VPath& s) { vp_task_finish<FILM>(p, s, lpath, fpos); });
   vp_count<COUNT>(nlook, nlookups);
 }
 
@@ -441,14 +507,29 @@ int check_mesh(flg_ctx* ctx, const flg_bvh* bvh, const char* who) {
   return FLG_OK;
 }
 
-template <bool COMPAT, bool COUNT>
+template <bool COMPAT, bool COUNT, bool FILM>
 void launch_volpath(int sched, int blocks1, int blocks2, hipStream_t st, const TraceGeom& g, const VolDev& v,
                     const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
-                    int32_t spp, int32_t D, float* lpath, unsigned int* next_path, unsigned long long* nl) {
+                    int32_t tasks, int32_t D, float* lpath, float2* fpos, unsigned int* next_path,
+                    unsigned long long* nl) {
   if (sched == 1)
-    k_volpath<COMPAT, COUNT><<<blocks1, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, nl);
+    k_volpath<COMPAT, COUNT, FILM><<<blocks1, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, tasks, D, lpath, fpos, nl);
   else
-    k_volpath_regen<COMPAT, COUNT><<<blocks2, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, spp, D, lpath, next_path, nl);
+    k_volpath_regen<COMPAT, COUNT, FILM><<<blocks2, kMeshBlock, 0, st>>>(g, v, L, tab, hp, cam, tasks, D, lpath, fpos,
+                                                                        next_path, nl);
+}
+
+template <bool COMPAT, bool COUNT>
+void launch_volpath_film(bool film, int sched, int blocks1, int blocks2, hipStream_t st, const TraceGeom& g,
+                         const VolDev& v, const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp,
+                         const CamDevV& cam, int32_t tasks, int32_t D, float* lpath, float2* fpos,
+                         unsigned int* next_path, unsigned long long* nl) {
+  if (film)
+    launch_volpath<COMPAT, COUNT, true>(sched, blocks1, blocks2, st, g, v, L, tab, hp, cam, tasks, D, lpath, fpos,
+                                        next_path, nl);
+  else
+    launch_volpath<COMPAT, COUNT, false>(sched, blocks1, blocks2, st, g, v, L, tab, hp, cam, tasks, D, lpath, fpos,
+                                         next_path, nl);
 }
 
 }  // namespace
@@ -464,7 +545,8 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
   if (!ctx || !bvh || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
       cfg->max_depth < 1)
     return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: bad arguments");
-  if (cfg->film != 0) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: film must be 0");
+  if (cfg->film < 0 || cfg->film > 1)
+    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: film must be 0 or 1");
   if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: too many paths");
   const int sched = cfg->schedule ? cfg->schedule : 2;
@@ -476,12 +558,19 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay below 2^24");
-  const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
-  char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)npaths * 4 + 256, &st);
+  const bool film = cfg->film == 1;
+  const int64_t npix = (int64_t)cfg->width * cfg->height;
+  // film mode: spp - 1 splatted samples per pixel (the first task also runs
+  // the pixel-centre path); with spp == 1 nothing reaches the film
+  const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
+  const int64_t ntasks = npix * tasks;
+  // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
+  char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
   if (!sc) return st;
   unsigned long long* nl = (unsigned long long*)sc;
   unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
   float* lpath = (float*)(sc + 256);
+  float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 4 + ((ntasks & 1) ? 4 : 0)) : nullptr;
   HaltonTables tab;
   if ((st = get_halton_tables(ctx, &tab))) return st;
   LightsDev L;
@@ -492,31 +581,34 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
   const TraceGeom g = geom_of(bvh);
   const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
   const bool count = h_stats != nullptr;
-  const int blocks1 = (int)div_up(npaths, kMeshBlock);
-  const int blocks2 = (int)std::min<int64_t>(div_up(npaths, kMeshBlock), (int64_t)ctx->num_cus * 8);
-  {
+  const int blocks1 = (int)div_up(ntasks, kMeshBlock);
+  const int blocks2 = (int)std::min<int64_t>(div_up(ntasks, kMeshBlock), (int64_t)ctx->num_cus * 8);
+  k_vp_zero<<<1, 64, 0, ctx->stream>>>(nl);
+  if (ntasks > 0) {
     ScopedTimer tm(ctx, "volpath_mesh_paths");
-    k_vp_zero<<<1, 64, 0, ctx->stream>>>(nl);
     if (compat && count)
-      launch_volpath<true, true>(sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, cfg->spp,
-                                 cfg->max_depth, lpath, next_path, nl);
+      launch_volpath_film<true, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                      cfg->max_depth, lpath, fpos, next_path, nl);
     else if (compat)
-      launch_volpath<true, false>(sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, cfg->spp,
-                                  cfg->max_depth, lpath, next_path, nl);
+      launch_volpath_film<true, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                       cfg->max_depth, lpath, fpos, next_path, nl);
     else if (count)
-      launch_volpath<false, true>(sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, cfg->spp,
-                                  cfg->max_depth, lpath, next_path, nl);
+      launch_volpath_film<false, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                       cfg->max_depth, lpath, fpos, next_path, nl);
     else
-      launch_volpath<false, false>(sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, cfg->spp,
-                                   cfg->max_depth, lpath, next_path, nl);
+      launch_volpath_film<false, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                        cfg->max_depth, lpath, fpos, next_path, nl);
   }
-  k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+  if (film)
+    film_resolve(ctx->stream, ntasks > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
+  else
+    k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
   FLG_CHECK_LAUNCH(ctx);
   if (h_stats) {
     unsigned long long c = 0;
     FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
     FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    h_stats[0] = (uint64_t)npaths;
+    h_stats[0] = (uint64_t)(ntasks > 0 ? npix * cfg->spp : 0);
     h_stats[1] = c;
   }
   return FLG_OK;

@@ -382,6 +382,15 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 }  // namespace
 
+void film_resolve(hipStream_t st, const float* lpath, const float2* fpos, int32_t w, int32_t h, int32_t tasks,
+                  float* rgb) {
+  const int64_t npix = (int64_t)w * h;
+  if (lpath && tasks > 0)
+    k_film_resolve<<<div_up(npix, 256), 256, 0, st>>>(lpath, fpos, w, h, tasks, rgb);
+  else
+    k_fill<<<div_up(npix * 3, 256), 256, 0, st>>>(rgb, npix * 3, 0.0f);
+}
+
 VolDev make_voldev(const flg_volume* vol) {
   return voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, vol->bounds, vol->medium, vol->max_density);
 }
@@ -548,11 +557,8 @@ int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene
   }
   {
     ScopedTimer tm(ctx, "volpath_mean");
-    if (film && npaths > 0)
-      k_film_resolve<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, fpos, cfg->width, cfg->height, tasks,
-                                                                 d_rgb);
-    else if (film)
-      k_fill<<<div_up(npix * 3, 256), 256, 0, ctx->stream>>>(d_rgb, npix * 3, 0.0f);
+    if (film)
+      film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
     else
       k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
   }

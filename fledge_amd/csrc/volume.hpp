@@ -428,6 +428,11 @@ inline CamDevV make_cam_v(const flg_camera* c, int w, int h) {
 }
 
 VolDev make_voldev(const flg_volume* vol);
+// The film of the reference's render loop from the film tasks' radiance and
+// film positions (tasks per pixel, pixel-major), into rgb (w*h*3); lpath NULL
+// or tasks 0: nothing reached the film, rgb is 0.
+void film_resolve(hipStream_t st, const float* lpath, const float2* fpos, int32_t w, int32_t h, int32_t tasks,
+                  float* rgb);
 // scene_bounds (may be NULL: the volume's box): the scene bound that places the
 // environment light's sample points (InfiniteAreaLight::preprocess).
 int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L,

@@ -363,9 +363,10 @@ FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scen
  * direction (SInteraction::SpawnRay, interaction.cpp:16-26) without counting
  * a bounce, where the reference dereferences the missing material
  * (integrator.cpp:433,487,501); shadow rays cross interfaces as
- * VolEstimateTr does (integrator.cpp:40-64).  Per-pixel mean of the path
- * samples (cfg->film must be 0); h_stats (may be NULL): {path samples,
- * density lookups}. */
+ * VolEstimateTr does (integrator.cpp:40-64).  cfg->film 0: per-pixel mean
+ * of the path samples on the per-sample Halton schedule; 1: the reference's
+ * render loop and film, as flg_volpath_render's film mode.  h_stats (may be
+ * NULL): {path samples, density lookups}. */
 FLG_API int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
                                     const flg_vol_scene* scene, const flg_render_cfg* cfg, float* d_rgb,
                                     uint64_t* h_stats);

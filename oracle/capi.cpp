@@ -308,9 +308,11 @@ void orc_halton_stateful(int32_t spp, int32_t resx, int32_t resy, int32_t px, in
 // VolPathIntegrator over the closed interface mesh of handle p with the
 // density grid inside, on the mesh's bounding box; medium, lights and camera
 // as orc_volpath_render.
+// film = 1: the reference's render loop and film (single-threaded)
 void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t nx, int32_t ny, int32_t nz,
                              const float* medium3, const float* lights7, const float* cam10, int32_t W, int32_t H,
-                             int32_t spp, int32_t max_depth, float* rgb, int nthreads, uint64_t* lookups) {
+                             int32_t spp, int32_t max_depth, float* rgb, int nthreads, uint64_t* lookups,
+                             int32_t film) {
   const Lbvh& b = ((Handle*)p)->b;
   const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
   VolScene sc;
@@ -322,7 +324,10 @@ void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
-  volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
+  if (film)
+    volpath_mesh_render_film(b, compat != 0, sc, rgb, lookups);
+  else
+    volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
 }
 
 // Scene::intersect over the meshes of handles[0..n) in scene order.

@@ -1,5 +1,5 @@
 // TEST INFRASTRUCTURE — CPU restatement of the reference's render loop and
-// film for the volume-only integrator: ParallelIntegrator::render and
+// film, for the volume-only integrator and VolPathIntegrator: ParallelIntegrator::render and
 // eval_pixel (src/integrator.cpp:123-235), HaltonSampler::setPixel /
 // getPixelSample / reset (src/common/sampler.h:164-186), Film::addSample /
 // commitSamples (src/film.hpp:143-174) and the film's filter, MitchellFilter
@@ -12,6 +12,7 @@
 // reproduced.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "oracle_impl.hpp"
@@ -64,10 +65,11 @@ struct FilmAcc {
 
 }  // namespace
 
-// rgb: W*H*3.  task_L / task_uv (may be NULL): per pixel (x fastest) the
-// spp - 1 splatted samples in order, radiance and film position.
-void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups_out,
-                      uint64_t* paths_out) {
+// The render loop over an integrator's Li(sampler, o, d, lookups).  rgb:
+// W*H*3.  task_L / task_uv (may be NULL): per pixel (x fastest) the spp - 1
+// splatted samples in order, radiance and film position.
+void film_render(const VolScene& sc, const std::function<double(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
+                 float* task_L, float* task_uv, uint64_t* lookups_out, uint64_t* paths_out) {
   const int W = sc.width, H = sc.height, spp = sc.spp;
   const int T = spp - 1;
   FilmAcc film(W, H);
@@ -84,12 +86,12 @@ void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task
           hs.set_pixel(x, y);
           V3 o, d;
           sc.cam->generate((float)x + 0.5f, (float)y + 0.5f, W, H, &o, &d);
-          (void)svol_Li(sc, hs, o, d, &lk);  // the centre sample: not splatted
+          (void)Li(hs, o, d, &lk);  // the centre sample: not splatted
           for (int s = 1; s < spp; ++s) {
             const float ux = (float)x + hs.get1D();
             const float uy = (float)y + hs.get1D();
             sc.cam->generate(ux, uy, W, H, &o, &d);
-            const float L = (float)svol_Li(sc, hs, o, d, &lk);
+            const float L = (float)Li(hs, o, d, &lk);
             film.add(ux, uy, L);
             const size_t k = ((size_t)y * W + x) * (size_t)T + (size_t)(s - 1);
             if (task_L) task_L[k] = L;
@@ -104,6 +106,13 @@ void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task
   film.commit(rgb);
   if (lookups_out) *lookups_out = lk;
   if (paths_out) *paths_out = (uint64_t)W * H * (uint64_t)spp;
+}
+
+void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups_out,
+                      uint64_t* paths_out) {
+  film_render(
+      sc, [&](Halton& hs, V3 o, V3 d, uint64_t* lk) { return svol_Li(sc, hs, o, d, lk); }, rgb, task_L, task_uv,
+      lookups_out, paths_out);
 }
 
 }  // namespace orc

@@ -1,5 +1,6 @@
 // TEST INFRASTRUCTURE — internal declarations of the CPU oracle (see oracle.hpp).
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "oracle.hpp"
@@ -143,6 +144,9 @@ V3 offset_ray_origin(V3 p, V3 n, V3 dir);  // OffsetRayOrigin (pathtrace.cpp)
 // VolPathIntegrator over a closed interface mesh (volpath_mesh.cpp)
 void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, int nthreads,
                          uint64_t* lookups);
+// The same integrator through the reference's render loop and film
+// (film.cpp), single-threaded.
+void volpath_mesh_render_film(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, uint64_t* lookups);
 
 // Closest hit over a scene of meshes (scene.cpp): NaiveBVHAccel over the
 // meshes in scene order.  rays: n x 8 floats (o, tmax, d, pad).
@@ -157,6 +161,8 @@ void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const flo
                       const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups);
 // the reference's render loop and film (film.cpp)
 float film_mitchell1d(float d);
+void film_render(const VolScene& sc, const std::function<double(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
+                 float* task_L, float* task_uv, uint64_t* lookups_out, uint64_t* paths_out);
 void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups,
                       uint64_t* paths);
 

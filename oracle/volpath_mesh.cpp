@@ -161,4 +161,10 @@ void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* 
   }
 }
 
+void volpath_mesh_render_film(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, uint64_t* lookups) {
+  film_render(
+      sc, [&](Halton& hs, V3 o, V3 d, uint64_t* lk) { return (double)volpath_Li(b, compat, sc, hs, o, d, lk); }, rgb,
+      nullptr, nullptr, lookups, nullptr);
+}
+
 }  // namespace orc

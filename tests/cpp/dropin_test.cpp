@@ -555,8 +555,8 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
 
 // HIPRender in the OptiX stub's place: a volume-only scene (the volume
 // integrator's film) and a closed interface mesh holding the medium
-// (VolPathIntegrator against the oracle, bit for bit), saveImage, and the
-// scenes it refuses.
+// (VolPathIntegrator through the render loop and film, against the oracle's),
+// saveImage, and the scenes it refuses.
 void test_hip_render(fledge::gpu::Context& ctx) {
   const int n = 24, W = 40, H = 32, spp = 4, depth = 16;
   std::vector<float> grid(size_t(n) * n * n);
@@ -627,10 +627,12 @@ void test_hip_render(fledge::gpu::Context& ctx) {
   sc.max_depth = depth;
   std::vector<float> rgb(size_t(W) * H * 3);
   uint64_t lk = 0;
-  orc::volpath_mesh_render(b, false, sc, rgb.data(), 4, &lk);
+  orc::volpath_mesh_render_film(b, false, sc, rgb.data(), &lk);
+  double worst = 0;  // the film sums in another order than the oracle's splats
   for (int i = 0; i < W * H; ++i)
-    CHECK(bits(film.m_color[i][0]) == bits(rgb[3 * i]), "mesh + medium pixel %d: %.9g vs oracle %.9g", i,
-          film.m_color[i][0], rgb[3 * i]);
+    worst = std::max(worst, std::fabs(double(film.m_color[i][0]) - rgb[3 * i]) /
+                                std::max(std::fabs(double(rgb[3 * i])), 1e-6));
+  CHECK(worst <= 1e-5, "mesh + medium film vs oracle: worst relative error %.3g", worst);
   CHECK(r.lastLookups() == lk, "mesh + medium lookup counts");
   const std::string out = "/tmp/fledge_dropin_" + std::to_string((long)getpid()) + "_render.exr";
   CHECK(r.saveImage(out) && !r.saveImage(out, /*denoise=*/true), "saveImage");

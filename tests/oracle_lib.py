@@ -44,7 +44,7 @@ _SIGS = {
     "orc_volpath_samples": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P]),
     "orc_mitchell1d": (f32, [f32]),
     "orc_volpath_mesh_render": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P,
-                                       ctypes.c_int, P]),
+                                       ctypes.c_int, P, i32]),
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
     "orc_scene_trace": (None, [P, i32, i32, P, i64, P, P, P, ctypes.c_int]),
     "orc_diff_mesh_render": (None, [P, i32, P, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
@@ -125,9 +125,11 @@ class Lbvh:
                                  _p(nseg), _p(ids), _p(ts), nthreads or NTHREADS, _p(rays))
         return nseg, ids, ts, int(rays[0])
 
-    def volpath_mesh(self, grid, medium, lights, cam10, W, H, spp, max_depth, compat=False, nthreads=None):
+    def volpath_mesh(self, grid, medium, lights, cam10, W, H, spp, max_depth, compat=False, nthreads=None,
+                     film=0):
         """VolPathIntegrator with this mesh as the interface around the grid's
-        medium (on the mesh's bounding box): (rgb (H, W, 3), lookups)."""
+        medium (on the mesh's bounding box): (rgb (H, W, 3), lookups).  film=1:
+        through the reference's render loop and film."""
         grid = np.ascontiguousarray(grid, np.float32)
         nz, ny, nx = grid.shape
         rgb = np.zeros((H, W, 3), np.float32)
@@ -135,7 +137,7 @@ class Lbvh:
         lib().orc_volpath_mesh_render(self.h, 1 if compat else 0, _p(grid), nx, ny, nz,
                                       _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
                                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
-                                      nthreads or NTHREADS, _p(lk))
+                                      nthreads or NTHREADS, _p(lk), film)
         return rgb, int(lk[0])
 
     def diff_render(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, compat=False,

@@ -62,3 +62,34 @@ def test_volpath_cube_interface_matches_svol_in_expectation(ctx):
     a = float(vol.render_mesh(bvh, sc, 64, 64, 64, 64).mean())
     b = float(vol.render(sc, 64, 64, 64, 64).mean())
     assert abs(a - b) / b < 0.01, (a, b)
+
+
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_volpath_mesh_film_matches_oracle(ctx, schedule):
+    """film = 1 over the interface mesh: the reference's render loop (the
+    pixel-centre path, samples 1..spp-1 on the loop's Halton schedule)
+    resolved through the Mitchell film, against the oracle's single-threaded
+    loop: equal up to the film's summation order, same lookup count, and the
+    same bits from both schedules.  The image spans several 16 x 16 tiles."""
+    v, ind = _sphere(4)
+    grid = api.gen_fbm_grid(24, seed=6)
+    W, H, spp, D = 37, 29, 4, 16
+    bvh, vol, sc = _scene(ctx, v, ind, grid)
+    print("volpath mesh film schedule %d" % schedule, flush=True)
+    out, st = vol.render_mesh(bvh, sc, W, H, spp, D, stats=True, schedule=schedule, film=1)
+    ref, lk = orc.Lbvh(v, ind).volpath_mesh(grid, MED, LIGHTS, CAM0, W, H, spp, D, film=1)
+    got = out.cpu().numpy()
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
+    assert rel.max() <= 1e-5, rel.max()
+    assert st["lookups"] == lk and st["paths"] == W * H * spp
+    other, _ = vol.render_mesh(bvh, sc, W, H, spp, D, stats=True, schedule=3 - schedule, film=1)
+    assert torch.equal(out.view(torch.int32), other.view(torch.int32))
+
+
+def test_volpath_mesh_film_one_sample_is_empty(ctx):
+    """One sample per pixel: the render loop's only sample is the unsplatted
+    pixel centre, so the film stays 0."""
+    v, ind = _sphere(3)
+    bvh, vol, sc = _scene(ctx, v, ind, api.gen_fbm_grid(16, seed=2))
+    out, st = vol.render_mesh(bvh, sc, 20, 12, 1, 8, stats=True, film=1)
+    assert torch.count_nonzero(out) == 0 and st["paths"] == 0
