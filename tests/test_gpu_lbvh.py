@@ -271,3 +271,31 @@ def test_autotuner_choices_give_identical_results(ctx):
         for sm in (1, 2):
             gb = api.hits_as_numpy(g.trace(rays, block_size=bs, stack_mode=sm))
             assert np.array_equal(gb[0], oi) and np.array_equal(gb[1].view(np.uint32), ot.view(np.uint32))
+
+
+def test_pathtrace_full_headline_frame(ctx):
+    """The benchmark's frame at full size (the 2,097,152-triangle displaced
+    octahedral sphere, amplitude 0.1, 1920 x 1080, 16 spp, depth 8, the
+    library's tuned launch shape): the build's topology and bounds against
+    the oracle's, and every segment of all 33,177,600 paths (triangle id and
+    t, bit for bit) against the oracle's traversal of the same frame."""
+    v, i = api.gen_octasphere(9, 0.1, seed=7, base_freq=2.0)
+    print("full frame: building", flush=True)
+    g, o = _build_both(ctx, v, i)
+    _compare_export(g.export(), o.export())
+    cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
+    cam = N.Camera.make(cam10[0:3], cam10[3:6], cam10[6:9], cam10[9])
+    W, H, spp, D = 1920, 1080, 16, 8
+    npaths = W * H * spp
+    r = g.pathtrace(cam, W, H, spp, D, record=True)
+    gn = r["nseg"].cpu().numpy()
+    gi = r["seg_id"].cpu().numpy()
+    gt = r["seg_t"].cpu().numpy()
+    del r
+    print("full frame: oracle", flush=True)
+    nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, npaths)
+    assert nrays == int(gn.sum()) == 45139179
+    assert np.array_equal(gn, nseg)
+    mask = np.arange(D + 1)[None, :] < nseg[:, None]
+    assert np.array_equal(gi[mask], ids[mask])
+    assert np.array_equal(gt[mask].view(np.uint32), ts[mask].view(np.uint32))
