@@ -166,3 +166,15 @@ def test_volpath_film_constant_radiance_and_one_sample(ctx):
     assert np.abs(out - 0.5).max() <= 2e-7
     one, st = vol.render(sc, 33, 17, 1, 8, film=1, stats=True)
     assert torch.count_nonzero(one) == 0 and st["paths"] == 0
+
+
+def test_volpath_full_benchmark_frame(ctx):
+    """The benchmark's volume frame at full size (the 256^3 fBm grid,
+    sigma_s 6.4, sigma_a 1.6, g 0.3, both lights, 512 x 512, 64 spp, depth
+    32: 16,777,216 path samples): every pixel bit-equal to the oracle, and the
+    same number of density lookups."""
+    grid = api.gen_fbm_grid(256, seed=1)
+    print("full volume frame", flush=True)
+    out, ref, st, lk = _scene(ctx, grid, 64, 512, 512, 32)
+    assert st["lookups"] == lk and st["paths"] == 512 * 512 * 64
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
