@@ -299,3 +299,36 @@ def test_pathtrace_full_headline_frame(ctx):
     mask = np.arange(D + 1)[None, :] < nseg[:, None]
     assert np.array_equal(gi[mask], ids[mask])
     assert np.array_equal(gt[mask].view(np.uint32), ts[mask].view(np.uint32))
+
+
+def test_batched_build_full_size(ctx):
+    """The benchmark's batch at full size: 256 meshes of log-uniform size
+    between 1K and 256K triangles (11.5 M in all) built in one batch.  Every
+    mesh's build equals its single build (topology, bounds), and eight of
+    them, the largest among them, equal the oracle's build; rays through two
+    of them hit the same triangles as the oracle's traversal."""
+    sizes = api.gen_batch_sizes(256)
+    host, meshes = [], []
+    for k, n in enumerate(sizes):
+        v, i = api.gen_soup(int(n), initstate=42, initseq=1000 + k)
+        host.append((v, i))
+        meshes.append((_dev(v), _dev(i)))
+    print("batched full size: %d triangles" % int(sizes.sum()), flush=True)
+    batch = api.LBVH.build_batched(ctx, meshes)
+    for k, (v, i) in enumerate(meshes):
+        single = api.LBVH.build(ctx, v, i)
+        a, b = batch[k].export(), single.export()
+        for key in a:
+            assert np.array_equal(np.asarray(a[key]).view(np.uint8), np.asarray(b[key]).view(np.uint8)), (k, key)
+        single.close()
+    rng = np.random.default_rng(2)
+    pick = list(rng.choice(256, size=7, replace=False)) + [int(np.argmax(sizes))]
+    rays = orc.camera_rays(CAM0, 128, 128)
+    for n, k in enumerate(pick):
+        v, i = host[k]
+        o = orc.Lbvh(v, i)
+        _compare_export(batch[k].export(), o.export())
+        if n < 2:
+            gid, gt = api.hits_as_numpy(batch[k].trace(_dev(rays)))
+            oid, ot = o.trace(rays)
+            assert np.array_equal(gid, oid) and np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
