@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_diff_regen(DiffDev D, LightsDev L, Halt
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * spp);
   float ga = 0.0f;
-  regen_loop<DiffPath>(
+  regen_loop<DiffPath, kVolChunk>(
       npaths, next_path,
       [&](uint32_t p, DiffPath& s) {
         return diff_begin<ADJ>(D, L, tab, hp, cam, spp, p, lpath, dl_drgb, grad, s);

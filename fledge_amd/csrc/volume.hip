@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_svol_regen(VolDev v, LightsDev L, Halto
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t npaths = (uint32_t)((int64_t)cam.w * cam.h * tasks);
   uint32_t nlook = 0;
-  regen_loop<SvolPath>(
+  regen_loop<SvolPath, kVolChunk>(
       npaths, next_path,
       [&](uint32_t p, SvolPath& s) {
         return FILM ? film_begin<COUNT>(v, L, tab, hp, cam, tasks, p, s, nlook)

