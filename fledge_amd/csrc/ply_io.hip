@@ -16,7 +16,9 @@
 // destination type as C does (float from double rounds to nearest).  Indices
 // outside [0, vertex count) are an error (the reference asserts them in debug
 // builds, src/shape.cpp:111-113).  Host code only.
+#include <algorithm>
 #include <cerrno>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -78,7 +80,8 @@ struct Value {
   float as_float() const { return is_float ? (float)f : (float)i; }
   bool as_int(int64_t* out) const {
     if (is_float) {
-      if (!(f == (double)(int64_t)f)) return false;
+      // within int64's range (NaN fails) before the cast, then integral
+      if (!(f >= -9.0e18 && f <= 9.0e18) || f != (double)(int64_t)f) return false;
       *out = (int64_t)f;
       return true;
     }
@@ -121,7 +124,10 @@ struct Reader {
       } else {
         v.i = std::strtoll(tok.c_str(), &e, 10);
       }
-      if (!e || *e != '\0' || errno == ERANGE) ok = false;
+      // ERANGE on a float is underflow (a subnormal or zero result, which
+      // the text legitimately names) unless the value overflowed to inf
+      const bool range = errno == ERANGE && (!v.is_float || std::isinf(v.f));
+      if (!e || *e != '\0' || range) ok = false;
       return v;
     }
     const int n = type_size(t);
@@ -243,6 +249,18 @@ int load(const char* path, flg_ply_mesh* out) {
     if (fe->props[k].list && fe->props[k].name == "vertex_indices") ifl = (int)k;
   if (ifl < 0) return FLG_ERR_INVALID;
   if (ve->count >= ((int64_t)1 << 31) || fe->count >= ((int64_t)1 << 30)) return FLG_ERR_RANGE;
+  // A row takes at least one byte per value in ascii (a token) and the
+  // values' sizes in binary; counts the body cannot hold are an error before
+  // any buffer is sized from them.
+  auto min_row = [&](const Element& e) {
+    int64_t b = 0;
+    for (const Prop& pr : e.props)
+      b += fmt == kAscii ? 1 : type_size(pr.list ? pr.count_type : pr.type);
+    return b;
+  };
+  const int64_t body = (int64_t)(data.size() - std::min(pos, data.size()));
+  for (const Element* e : {ve, fe})
+    if (e->count * min_row(*e) > body) return FLG_ERR_INVALID;
   std::vector<float> verts((size_t)ve->count * 3), nrm(normals ? (size_t)ve->count * 3 : 0),
       uv(iu >= 0 ? (size_t)ve->count * 2 : 0);
   std::vector<int32_t> ind((size_t)fe->count * 3);

@@ -148,3 +148,30 @@ def test_ply_vertices_without_faces(tmp_path):
     _write(p, "binary_little_endian", elems)
     gv, gi, _, _ = api.load_ply(p)
     assert gv.shape == v.shape and (gi is None or gi.size == 0)
+
+
+def test_ply_hostile_counts_and_values(tmp_path):
+    """Counts the body cannot hold fail before buffers are sized from them; an
+    index given as a float outside int64 (or NaN) is an error, not undefined
+    behaviour; a subnormal ascii coordinate is a value, not a range error."""
+    huge = str(tmp_path / "huge.ply")
+    open(huge, "wb").write(b"ply\nformat binary_little_endian 1.0\nelement vertex 2000000000\n"
+                           b"property float x\nproperty float y\nproperty float z\n"
+                           b"element face 1\nproperty list uchar int vertex_indices\nend_header\n" + bytes(64))
+    _expect_error(huge)
+    for bad in ("1e300", "nan", "-1e19"):
+        p = str(tmp_path / "fidx.ply")
+        open(p, "w").write("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                           "property float z\nelement face 1\nproperty list uchar double vertex_indices\n"
+                           "end_header\n0 0 0\n1 0 0\n0 1 0\n3 0 1 %s\n" % bad)
+        _expect_error(p)
+    sub = str(tmp_path / "subnormal.ply")
+    open(sub, "w").write("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                         "property float z\nelement face 1\nproperty list uchar int vertex_indices\n"
+                         "end_header\n1e-40 0 0\n1 0 0\n0 1 0\n3 0 1 2\n")
+    gv, gi, _, _ = api.load_ply(sub)
+    assert gv[0, 0] == np.float32(1e-40) and gv[0, 0] != 0
+    assert gi.reshape(-1).tolist() == [0, 1, 2]
+    over = str(tmp_path / "overflow.ply")
+    open(over, "w").write(open(sub).read().replace("1e-40", "1e39"))
+    _expect_error(over)
