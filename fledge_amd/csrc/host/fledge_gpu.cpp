@@ -697,7 +697,7 @@ class XReader {
   explicit XReader(const std::string& text) : s_(text) {}
   XNode document() {
     skip_misc();
-    XNode root = element();
+    XNode root = element(0);
     skip_misc();
     if (i_ != s_.size()) scene_error("content after the root element");
     return root;
@@ -752,7 +752,11 @@ class XReader {
     }
     return o;
   }
-  XNode element() {
+  // Scene files nest a few levels (scene, shape, transform, lookat); the
+  // bound keeps a hostile file from exhausting the stack.
+  static constexpr int kMaxNesting = 256;
+  XNode element(int depth) {
+    if (depth >= kMaxNesting) scene_error("elements nested too deeply");
     if (!starts("<")) scene_error("an element was expected");
     ++i_;
     XNode n;
@@ -793,7 +797,7 @@ class XReader {
       }
       if (starts("<!--")) skip_to("-->");
       else if (starts("<?")) skip_to("?>");
-      else n.kids.push_back(element());
+      else n.kids.push_back(element(depth + 1));
     }
   }
 };

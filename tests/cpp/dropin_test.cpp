@@ -407,10 +407,14 @@ void test_scene_file_host() {
                        "<scene version=\"0.6.0\"><integrator type=\"path\"/>",
                        "<scene version=\"0.6.0\"><integrator type=\"path\"/><sensor type=\"perspective\">"
                        "<transform><lookat up=\"0,1\" origin=\"0,0,0\" target=\"0,0,1\"/></transform></sensor></scene>"};
+  std::string deep = "<scene version=\"0.6.0\">";  // nesting deep enough to exhaust a recursive parser's stack
+  for (int k = 0; k < 200000; ++k) deep += "<a>";
+  std::vector<std::string> files(std::begin(bad), std::end(bad));
+  files.push_back(deep);
   int refused = 0;
-  for (const char* b : bad) {
+  for (const std::string& b : files) {
     FILE* f = std::fopen(xml.c_str(), "wb");
-    std::fputs(b, f);
+    std::fputs(b.c_str(), f);
     std::fclose(f);
     try {
       fledge::Scene s2;
@@ -422,8 +426,8 @@ void test_scene_file_host() {
   std::remove(xml.c_str());
   std::remove((dir + "/mesh.ply").c_str());
   ::rmdir(dir.c_str());
-  CHECK(refused == 5, "the malformed scene files must be refused (%d of 5)", refused);
-  std::printf("PASS scene_file_host (2 shapes, 1 emitter, 5 refused files)\n");
+  CHECK(refused == 6, "the malformed scene files must be refused (%d of 6)", refused);
+  std::printf("PASS scene_file_host (2 shapes, 1 emitter, 6 refused files)\n");
 }
 
 // Scene::intersect through SceneAccel over three mesh primitives read from
