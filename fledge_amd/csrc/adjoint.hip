@@ -359,71 +359,77 @@ int64_t flg_volpath_diff_tape_bytes(const flg_render_cfg* cfg) { return tape_byt
 
 int flg_bricks_build(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz, int32_t brick,
                      int32_t* d_slot, int32_t* h_n_active) {
-  if (!ctx || !d_density || !d_slot || !h_n_active || nx <= 0 || ny <= 0 || nz <= 0)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: bad arguments");
-  if (brick != 1 && brick != 2 && brick != 4 && brick != 8 && brick != 16 && brick != 32)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: brick must be 1, 2, 4, 8, 16 or 32");
-  const int32_t nbx = (nx + brick - 1) / brick, nby = (ny + brick - 1) / brick, nbz = (nz + brick - 1) / brick;
-  const int64_t nb = (int64_t)nbx * nby * nbz;
-  if (nb >= ((int64_t)1 << 31)) return set_error(ctx, FLG_ERR_RANGE, "flg_bricks_build: too many bricks");
-  int st = FLG_OK;
-  int32_t* count = (int32_t*)scratch(ctx, "brick_count", 256, &st);
-  if (!count) return st;
-  k_brick_flags<<<(unsigned)nb, 256, 0, ctx->stream>>>(d_density, nx, ny, nz, brick, nbx, nby, d_slot);
-  k_brick_slots<<<1, 1024, 0, ctx->stream>>>(d_slot, nb, count);
-  FLG_CHECK_LAUNCH(ctx);
-  FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_n_active, count, 4, hipMemcpyDeviceToHost, ctx->stream));
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return FLG_OK;
+  return host_guard(ctx, "flg_bricks_build", [&]() -> int {
+    if (!ctx || !d_density || !d_slot || !h_n_active || nx <= 0 || ny <= 0 || nz <= 0)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: bad arguments");
+    if (brick != 1 && brick != 2 && brick != 4 && brick != 8 && brick != 16 && brick != 32)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_bricks_build: brick must be 1, 2, 4, 8, 16 or 32");
+    const int32_t nbx = (nx + brick - 1) / brick, nby = (ny + brick - 1) / brick, nbz = (nz + brick - 1) / brick;
+    const int64_t nb = (int64_t)nbx * nby * nbz;
+    if (nb >= ((int64_t)1 << 31)) return set_error(ctx, FLG_ERR_RANGE, "flg_bricks_build: too many bricks");
+    int st = FLG_OK;
+    int32_t* count = (int32_t*)scratch(ctx, "brick_count", 256, &st);
+    if (!count) return st;
+    k_brick_flags<<<(unsigned)nb, 256, 0, ctx->stream>>>(d_density, nx, ny, nz, brick, nbx, nby, d_slot);
+    k_brick_slots<<<1, 1024, 0, ctx->stream>>>(d_slot, nb, count);
+    FLG_CHECK_LAUNCH(ctx);
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_n_active, count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FLG_OK;
+  });
 }
 
 int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                              const flg_vol_scene* scene, const flg_render_cfg* cfg,
                              const flg_diff_params* params, float* d_rgb, float* d_lpath, void* d_tape) {
-  DiffDev D;
-  LightsDev L;
-  HaltonTables tab;
-  HaltonParams hp;
-  CamDevV cam;
-  int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
-  if (st) return st;
-  if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_diff_forward: no output");
-  const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
-  float* lpath = d_lpath;
-  if (!lpath) {
-    lpath = (float*)scratch(ctx, "diff_lpath", (size_t)npaths * 4, &st);
-    if (!lpath) return st;
-  }
-  {
-    ScopedTimer tm(ctx, "diff_forward");
-    if ((st = launch_diff<false>(ctx, cfg, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr))) return st;
-    if (d_rgb) k_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_volpath_diff_forward", [&]() -> int {
+    DiffDev D;
+    LightsDev L;
+    HaltonTables tab;
+    HaltonParams hp;
+    CamDevV cam;
+    int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
+    if (st) return st;
+    if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_diff_forward: no output");
+    const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
+    float* lpath = d_lpath;
+    if (!lpath) {
+      lpath = (float*)scratch(ctx, "diff_lpath", (size_t)npaths * 4, &st);
+      if (!lpath) return st;
+    }
+    {
+      ScopedTimer tm(ctx, "diff_forward");
+      if ((st = launch_diff<false>(ctx, cfg, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr))) return st;
+      if (d_rgb) k_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                         const flg_vol_scene* scene, const flg_render_cfg* cfg,
                         const flg_diff_params* params, const float* d_lpath, const void* d_tape,
                         const float* d_dl_drgb, float* d_grad, float* d_grad_albedo) {
-  DiffDev D;
-  LightsDev L;
-  HaltonTables tab;
-  HaltonParams hp;
-  CamDevV cam;
-  int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
-  if (st) return st;
-  if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_adjoint: null buffer");
-  {
-    ScopedTimer tm(ctx, "diff_adjoint");
-    if ((st = launch_diff<true>(ctx, cfg, D, L, tab, hp, cam, (float*)d_lpath, d_dl_drgb, d_grad,
-                                d_grad_albedo)))
-      return st;
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_volpath_adjoint", [&]() -> int {
+    DiffDev D;
+    LightsDev L;
+    HaltonTables tab;
+    HaltonParams hp;
+    CamDevV cam;
+    int st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, nullptr, &D, &L, &tab, &hp, &cam);
+    if (st) return st;
+    if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_adjoint: null buffer");
+    {
+      ScopedTimer tm(ctx, "diff_adjoint");
+      if ((st = launch_diff<true>(ctx, cfg, D, L, tab, hp, cam, (float*)d_lpath, d_dl_drgb, d_grad,
+                                  d_grad_albedo)))
+        return st;
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -177,6 +178,28 @@ struct flg_ctx {
 namespace flg {
 
 int set_error(flg_ctx* ctx, int status, const std::string& msg);
+
+// Runs an entry point's body so that no C++ exception crosses the C ABI: a
+// failed host allocation (std::bad_alloc) becomes FLG_ERR_OOM and any other
+// exception FLG_ERR_INVALID, with the entry point's name in the context's
+// last error when there is a context.
+template <class F>
+int host_guard(flg_ctx* ctx, const char* what, F&& body) noexcept {
+  int status = FLG_ERR_INVALID;
+  const char* why = "unexpected host error";
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    status = FLG_ERR_OOM;
+    why = "host allocation failed";
+  } catch (...) {
+  }
+  try {
+    return set_error(ctx, status, std::string(what) + ": " + why);
+  } catch (...) {  // no memory left for the message either
+    return status;
+  }
+}
 void resolve_timers(flg_ctx* ctx);
 int check_hip(flg_ctx* ctx, hipError_t e, const char* what);
 void* scratch(flg_ctx* ctx, const char* key, size_t bytes, int* status);

@@ -11,7 +11,13 @@
 namespace flg {
 
 int set_error(flg_ctx* ctx, int status, const std::string& msg) {
-  if (ctx) ctx->last_error = msg;
+  if (ctx) {
+    try {
+      ctx->last_error = msg;
+    } catch (...) {  // the status still reports the error
+      ctx->last_error.clear();
+    }
+  }
   return status;
 }
 
@@ -119,40 +125,46 @@ const char* flg_status_string(int status) {
 }
 
 int flg_ctx_create(int device, void* stream, flg_ctx** out) {
-  if (!out) return FLG_ERR_INVALID;
-  *out = nullptr;
-  int n = 0;
-  hipError_t e = hipGetDeviceCount(&n);
-  if (e != hipSuccess || n == 0) return FLG_ERR_NO_DEVICE;
-  if (device < 0 || device >= n) return FLG_ERR_INVALID;
-  if (hipSetDevice(device) != hipSuccess) return FLG_ERR_NO_DEVICE;
-  flg_ctx* ctx = new flg_ctx;
-  ctx->device = device;
-  ctx->stream = (hipStream_t)stream;  // NULL: the device's default stream
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
-  *out = ctx;
-  return FLG_OK;
+  return host_guard(nullptr, "flg_ctx_create", [&]() -> int {
+    if (!out) return FLG_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return FLG_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return FLG_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return FLG_ERR_NO_DEVICE;
+    flg_ctx* ctx = new flg_ctx;
+    ctx->device = device;
+    ctx->stream = (hipStream_t)stream;  // NULL: the device's default stream
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return FLG_OK;
+  });
 }
 
 int flg_ctx_destroy(flg_ctx* ctx) {
-  if (!ctx) return FLG_ERR_INVALID;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  resolve_timers(ctx);
-  for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
-  for (auto& kv : ctx->scratch)
-    if (kv.second.first) (void)hipFree(kv.second.first);
-  release_halton_tables(ctx);
-  delete ctx;
-  return FLG_OK;
+  return host_guard(ctx, "flg_ctx_destroy", [&]() -> int {
+    if (!ctx) return FLG_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    resolve_timers(ctx);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    for (auto& kv : ctx->scratch)
+      if (kv.second.first) (void)hipFree(kv.second.first);
+    release_halton_tables(ctx);
+    delete ctx;
+    return FLG_OK;
+  });
 }
 
 int flg_ctx_synchronize(flg_ctx* ctx) {
-  if (!ctx) return FLG_ERR_INVALID;
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return FLG_OK;
+  return host_guard(ctx, "flg_ctx_synchronize", [&]() -> int {
+    if (!ctx) return FLG_ERR_INVALID;
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FLG_OK;
+  });
 }
 
 void* flg_ctx_stream(flg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
@@ -160,69 +172,85 @@ void* flg_ctx_stream(flg_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; 
 const char* flg_last_error(flg_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
 int flg_ctx_set_profiling(flg_ctx* ctx, int enable) {
-  if (!ctx) return FLG_ERR_INVALID;
-  ctx->profiling = enable != 0;
-  return FLG_OK;
+  return host_guard(ctx, "flg_ctx_set_profiling", [&]() -> int {
+    if (!ctx) return FLG_ERR_INVALID;
+    ctx->profiling = enable != 0;
+    return FLG_OK;
+  });
 }
 
 int flg_ctx_kernel_time(flg_ctx* ctx, const char* name, double* ms, int64_t* launches) {
-  if (!ctx || !name) return FLG_ERR_INVALID;
-  resolve_timers(ctx);
-  auto it = ctx->timers.find(name);
-  if (it == ctx->timers.end()) return set_error(ctx, FLG_ERR_INVALID, std::string("no timer ") + name);
-  if (ms) *ms = it->second.ms;
-  if (launches) *launches = it->second.launches;
-  return FLG_OK;
+  return host_guard(ctx, "flg_ctx_kernel_time", [&]() -> int {
+    if (!ctx || !name) return FLG_ERR_INVALID;
+    resolve_timers(ctx);
+    auto it = ctx->timers.find(name);
+    if (it == ctx->timers.end()) return set_error(ctx, FLG_ERR_INVALID, std::string("no timer ") + name);
+    if (ms) *ms = it->second.ms;
+    if (launches) *launches = it->second.launches;
+    return FLG_OK;
+  });
 }
 
 int flg_ctx_reset_profiling(flg_ctx* ctx) {
-  if (!ctx) return FLG_ERR_INVALID;
-  resolve_timers(ctx);
-  ctx->timers.clear();
-  return FLG_OK;
+  return host_guard(ctx, "flg_ctx_reset_profiling", [&]() -> int {
+    if (!ctx) return FLG_ERR_INVALID;
+    resolve_timers(ctx);
+    ctx->timers.clear();
+    return FLG_OK;
+  });
 }
 
 int flg_malloc(flg_ctx* ctx, size_t bytes, void** d_out) {
-  if (!ctx || !d_out) return FLG_ERR_INVALID;
-  *d_out = nullptr;
-  FLG_CHECK_HIP(ctx, hipMalloc(d_out, bytes ? bytes : 1));
-  return FLG_OK;
+  return host_guard(ctx, "flg_malloc", [&]() -> int {
+    if (!ctx || !d_out) return FLG_ERR_INVALID;
+    *d_out = nullptr;
+    FLG_CHECK_HIP(ctx, hipMalloc(d_out, bytes ? bytes : 1));
+    return FLG_OK;
+  });
 }
 
 int flg_free(flg_ctx* ctx, void* d_ptr) {
-  if (!ctx) return FLG_ERR_INVALID;
-  if (!d_ptr) return FLG_OK;
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  FLG_CHECK_HIP(ctx, hipFree(d_ptr));
-  return FLG_OK;
+  return host_guard(ctx, "flg_free", [&]() -> int {
+    if (!ctx) return FLG_ERR_INVALID;
+    if (!d_ptr) return FLG_OK;
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FLG_CHECK_HIP(ctx, hipFree(d_ptr));
+    return FLG_OK;
+  });
 }
 
 int flg_memcpy_htod(flg_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
-  if (!ctx || (!d_dst && bytes) || (!h_src && bytes)) return FLG_ERR_INVALID;
-  if (!bytes) return FLG_OK;
-  FLG_CHECK_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return FLG_OK;
+  return host_guard(ctx, "flg_memcpy_htod", [&]() -> int {
+    if (!ctx || (!d_dst && bytes) || (!h_src && bytes)) return FLG_ERR_INVALID;
+    if (!bytes) return FLG_OK;
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FLG_OK;
+  });
 }
 
 int flg_memcpy_dtoh(flg_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
-  if (!ctx || (!d_src && bytes) || (!h_dst && bytes)) return FLG_ERR_INVALID;
-  if (!bytes) return FLG_OK;
-  FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return FLG_OK;
+  return host_guard(ctx, "flg_memcpy_dtoh", [&]() -> int {
+    if (!ctx || (!d_src && bytes) || (!h_dst && bytes)) return FLG_ERR_INVALID;
+    if (!bytes) return FLG_OK;
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FLG_OK;
+  });
 }
 
 int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n) {
-  if (!ctx || kind < 0 || kind > 4 || n < 0 || (n && (!d_in || !d_out)))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_debug_eval: bad arguments");
-  if (n == 0) return FLG_OK;
-  HaltonTables tab;
-  int st = get_halton_tables(ctx, &tab);
-  if (st) return st;
-  debug_eval_kernel<<<div_up(n, 256), 256, 0, ctx->stream>>>(kind, d_in, d_out, n, tab);
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_debug_eval", [&]() -> int {
+    if (!ctx || kind < 0 || kind > 4 || n < 0 || (n && (!d_in || !d_out)))
+      return set_error(ctx, FLG_ERR_INVALID, "flg_debug_eval: bad arguments");
+    if (n == 0) return FLG_OK;
+    HaltonTables tab;
+    int st = get_halton_tables(ctx, &tab);
+    if (st) return st;
+    debug_eval_kernel<<<div_up(n, 256), 256, 0, ctx->stream>>>(kind, d_in, d_out, n, tab);
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

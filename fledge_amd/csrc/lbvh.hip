@@ -694,101 +694,113 @@ extern "C" {
 
 int flg_lbvh_build(flg_ctx* ctx, const float* d_verts, const int32_t* d_ind, int32_t ntris,
                    int32_t nverts, uint32_t flags, flg_bvh** out) {
-  if (!out) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_build: null out");
-  *out = nullptr;
-  flg_mesh_desc m{d_verts, d_ind, ntris, nverts};
-  return build_segments(ctx, &m, 1, flags, out);
+  return host_guard(ctx, "flg_lbvh_build", [&]() -> int {
+    if (!out) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_build: null out");
+    *out = nullptr;
+    flg_mesh_desc m{d_verts, d_ind, ntris, nverts};
+    return build_segments(ctx, &m, 1, flags, out);
+  });
 }
 
 int flg_lbvh_build_batched(flg_ctx* ctx, const flg_mesh_desc* h_meshes, int32_t n, uint32_t flags,
                            flg_bvh** out) {
-  if (!out || n <= 0) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_build_batched: bad arguments");
-  for (int i = 0; i < n; ++i) out[i] = nullptr;
-  return build_segments(ctx, h_meshes, n, flags, out);
+  return host_guard(ctx, "flg_lbvh_build_batched", [&]() -> int {
+    if (!out || n <= 0) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_build_batched: bad arguments");
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    return build_segments(ctx, h_meshes, n, flags, out);
+  });
 }
 
 int flg_lbvh_destroy(flg_bvh* bvh) {
-  if (!bvh) return FLG_ERR_INVALID;
-  SharedBlock* sb = bvh->block;
-  if (sb && --sb->refs == 0) {
-    (void)hipSetDevice(bvh->ctx->device);
-    (void)hipStreamSynchronize(bvh->ctx->stream);
-    (void)hipFree(sb->ptr);
-    delete sb;
-  }
-  delete bvh;
-  return FLG_OK;
+  return host_guard(nullptr, "flg_lbvh_destroy", [&]() -> int {
+    if (!bvh) return FLG_ERR_INVALID;
+    SharedBlock* sb = bvh->block;
+    if (sb && --sb->refs == 0) {
+      (void)hipSetDevice(bvh->ctx->device);
+      (void)hipStreamSynchronize(bvh->ctx->stream);
+      (void)hipFree(sb->ptr);
+      delete sb;
+    }
+    delete bvh;
+    return FLG_OK;
+  });
 }
 
 int flg_lbvh_get_info(flg_bvh* bvh, flg_lbvh_info* out) {
-  if (!bvh || !out) return FLG_ERR_INVALID;
-  out->ntris = bvh->n;
-  for (int k = 0; k < 6; ++k) out->bound[k] = bvh->bound[k];
-  out->device_bytes = bvh->bytes;
-  return FLG_OK;
+  return host_guard(nullptr, "flg_lbvh_get_info", [&]() -> int {
+    if (!bvh || !out) return FLG_ERR_INVALID;
+    out->ntris = bvh->n;
+    for (int k = 0; k < 6; ++k) out->bound[k] = bvh->bound[k];
+    out->device_bytes = bvh->bytes;
+    return FLG_OK;
+  });
 }
 
 int flg_lbvh_export(flg_ctx* ctx, flg_bvh* bvh, const flg_lbvh_view* v) {
-  if (!ctx || !bvh || !v) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_export: bad arguments");
-  const size_t n = (size_t)bvh->n, ni = n > 0 ? n - 1 : 0;
-  auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
-    if (!dst || !bytes) return FLG_OK;
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    return FLG_OK;
-  };
-  int st;
-  if ((st = cp(v->h_morton, bvh->morton, n * 4))) return st;
-  if ((st = cp(v->h_morton_sorted, bvh->morton_sorted, n * 4))) return st;
-  if ((st = cp(v->h_order, bvh->order, n * 4))) return st;
-  if ((st = cp(v->h_split, bvh->split, ni * 4))) return st;
-  if ((st = cp(v->h_range_lo, bvh->rlo, ni * 4))) return st;
-  if ((st = cp(v->h_range_hi, bvh->rhi, ni * 4))) return st;
-  if ((st = cp(v->h_parent, bvh->parent, ni * 4))) return st;
-  if ((st = cp(v->h_leaf_parent, bvh->leaf_parent, n * 4))) return st;
-  std::vector<uint8_t> lf(ni);
-  std::vector<BNode> nodes(ni);
-  if (ni) {
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(lf.data(), bvh->leafflags, ni, hipMemcpyDeviceToHost, ctx->stream));
-    if (v->h_node_bound)
-      FLG_CHECK_HIP(ctx, hipMemcpyAsync(nodes.data(), bvh->nodes, ni * sizeof(BNode),
-                                        hipMemcpyDeviceToHost, ctx->stream));
-  }
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (size_t i = 0; i < ni; ++i) {
-    if (v->h_left_leaf) v->h_left_leaf[i] = lf[i] & 1;
-    if (v->h_right_leaf) v->h_right_leaf[i] = (lf[i] >> 1) & 1;
-    if (v->h_node_bound) {
-      const float* f = reinterpret_cast<const float*>(&nodes[i]);
-      // node bound = union of its two child boxes (merge order of the refit)
-      float* q = v->h_node_bound + 6 * i;
-      for (int k = 0; k < 3; ++k) {
-        float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
-        lo = smin(smin(lo, f[6 + k]), f[k]);
-        hi = smax(smax(hi, f[9 + k]), f[3 + k]);
-        q[k] = lo;
-        q[3 + k] = hi;
+  return host_guard(ctx, "flg_lbvh_export", [&]() -> int {
+    if (!ctx || !bvh || !v) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_export: bad arguments");
+    const size_t n = (size_t)bvh->n, ni = n > 0 ? n - 1 : 0;
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+      if (!dst || !bytes) return FLG_OK;
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+      return FLG_OK;
+    };
+    int st;
+    if ((st = cp(v->h_morton, bvh->morton, n * 4))) return st;
+    if ((st = cp(v->h_morton_sorted, bvh->morton_sorted, n * 4))) return st;
+    if ((st = cp(v->h_order, bvh->order, n * 4))) return st;
+    if ((st = cp(v->h_split, bvh->split, ni * 4))) return st;
+    if ((st = cp(v->h_range_lo, bvh->rlo, ni * 4))) return st;
+    if ((st = cp(v->h_range_hi, bvh->rhi, ni * 4))) return st;
+    if ((st = cp(v->h_parent, bvh->parent, ni * 4))) return st;
+    if ((st = cp(v->h_leaf_parent, bvh->leaf_parent, n * 4))) return st;
+    std::vector<uint8_t> lf(ni);
+    std::vector<BNode> nodes(ni);
+    if (ni) {
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(lf.data(), bvh->leafflags, ni, hipMemcpyDeviceToHost, ctx->stream));
+      if (v->h_node_bound)
+        FLG_CHECK_HIP(ctx, hipMemcpyAsync(nodes.data(), bvh->nodes, ni * sizeof(BNode),
+                                          hipMemcpyDeviceToHost, ctx->stream));
+    }
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < ni; ++i) {
+      if (v->h_left_leaf) v->h_left_leaf[i] = lf[i] & 1;
+      if (v->h_right_leaf) v->h_right_leaf[i] = (lf[i] >> 1) & 1;
+      if (v->h_node_bound) {
+        const float* f = reinterpret_cast<const float*>(&nodes[i]);
+        // node bound = union of its two child boxes (merge order of the refit)
+        float* q = v->h_node_bound + 6 * i;
+        for (int k = 0; k < 3; ++k) {
+          float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+          lo = smin(smin(lo, f[6 + k]), f[k]);
+          hi = smax(smax(hi, f[9 + k]), f[3 + k]);
+          q[k] = lo;
+          q[3 + k] = hi;
+        }
       }
     }
-  }
-  return FLG_OK;
+    return FLG_OK;
+  });
 }
 
 int flg_lbvh_download(flg_ctx* ctx, const flg_bvh* bvh, void* h_nodes, size_t nodes_bytes,
                       void* h_tris, size_t tris_bytes) {
-  if (!ctx || !bvh) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_download: bad arguments");
-  const size_t n = (size_t)bvh->n, ni = n > 0 ? n - 1 : 0;
-  if ((h_nodes && nodes_bytes != ni * sizeof(BNode)) || (h_tris && tris_bytes != n * sizeof(BTri)))
-    return set_error(ctx, FLG_ERR_INVALID,
-                     "flg_lbvh_download: nodes need " + std::to_string(ni * sizeof(BNode)) +
-                         " bytes and tris " + std::to_string(n * sizeof(BTri)) + " bytes");
-  if (h_nodes && ni)
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_nodes, bvh->nodes, nodes_bytes, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-  if (h_tris && n)
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_tris, bvh->tris, tris_bytes, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-  FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return FLG_OK;
+  return host_guard(ctx, "flg_lbvh_download", [&]() -> int {
+    if (!ctx || !bvh) return set_error(ctx, FLG_ERR_INVALID, "flg_lbvh_download: bad arguments");
+    const size_t n = (size_t)bvh->n, ni = n > 0 ? n - 1 : 0;
+    if ((h_nodes && nodes_bytes != ni * sizeof(BNode)) || (h_tris && tris_bytes != n * sizeof(BTri)))
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "flg_lbvh_download: nodes need " + std::to_string(ni * sizeof(BNode)) +
+                           " bytes and tris " + std::to_string(n * sizeof(BTri)) + " bytes");
+    if (h_nodes && ni)
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_nodes, bvh->nodes, nodes_bytes, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    if (h_tris && n)
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(h_tris, bvh->tris, tris_bytes, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

@@ -336,14 +336,16 @@ using namespace flg;
 extern "C" {
 
 int flg_ply_load(const char* path, flg_ply_mesh* out) {
-  if (!path || !out) return FLG_ERR_INVALID;
-  *out = flg_ply_mesh{0, 0, nullptr, nullptr, nullptr, nullptr};
-  try {
-    return load(path, out);
-  } catch (...) {  // allocation failures of the parser's own buffers
-    flg_ply_free(out);
-    return FLG_ERR_OOM;
-  }
+  return host_guard(nullptr, "flg_ply_load", [&]() -> int {
+    if (!path || !out) return FLG_ERR_INVALID;
+    *out = flg_ply_mesh{0, 0, nullptr, nullptr, nullptr, nullptr};
+    try {
+      return load(path, out);
+    } catch (...) {  // allocation failures of the parser's own buffers
+      flg_ply_free(out);
+      return FLG_ERR_OOM;
+    }
+  });
 }
 
 void flg_ply_free(flg_ply_mesh* m) {

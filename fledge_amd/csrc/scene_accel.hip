@@ -216,105 +216,113 @@ using namespace flg;
 extern "C" {
 
 int flg_scene_accel_build(flg_ctx* ctx, flg_bvh* const* h_meshes, int32_t n, flg_scene_accel** out) {
-  if (!ctx || !h_meshes || n <= 0 || !out) return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: bad arguments");
-  *out = nullptr;
-  std::vector<Box> boxes(n);
-  const bool compat = (h_meshes[0] && (h_meshes[0]->flags & FLG_NORMAL_COMPAT));
-  for (int32_t m = 0; m < n; ++m) {
-    const flg_bvh* b = h_meshes[m];
-    if (!b || b->n <= 0 || b->ctx != ctx)
-      return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: a mesh is missing, empty or on another context");
-    if (((b->flags & FLG_NORMAL_COMPAT) != 0) != compat)
-      return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: meshes differ in FLG_NORMAL_COMPAT");
-    for (int k = 0; k < 3; ++k) {
-      boxes[m].lo[k] = b->bound[k];
-      boxes[m].hi[k] = b->bound[3 + k];
+  return host_guard(ctx, "flg_scene_accel_build", [&]() -> int {
+    if (!ctx || !h_meshes || n <= 0 || !out) return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: bad arguments");
+    *out = nullptr;
+    std::vector<Box> boxes(n);
+    const bool compat = (h_meshes[0] && (h_meshes[0]->flags & FLG_NORMAL_COMPAT));
+    for (int32_t m = 0; m < n; ++m) {
+      const flg_bvh* b = h_meshes[m];
+      if (!b || b->n <= 0 || b->ctx != ctx)
+        return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: a mesh is missing, empty or on another context");
+      if (((b->flags & FLG_NORMAL_COMPAT) != 0) != compat)
+        return set_error(ctx, FLG_ERR_INVALID, "flg_scene_accel_build: meshes differ in FLG_NORMAL_COMPAT");
+      for (int k = 0; k < 3; ++k) {
+        boxes[m].lo[k] = b->bound[k];
+        boxes[m].hi[k] = b->bound[3 + k];
+      }
     }
-  }
-  flg_scene_accel* s = new flg_scene_accel;
-  s->ctx = ctx;
-  s->n = n;
-  s->compat = compat;
-  // the reference's leaf order and root box (merged into the default box at
-  // the origin, accel.cpp:41-42 with aabb.h's AABB())
-  std::vector<int32_t> idx(n), order;
-  std::iota(idx.begin(), idx.end(), 0);
-  order.reserve(n);
-  reference_order(idx, 0, (size_t)n, 0, boxes, order);
-  s->rank.assign(n, 0);
-  for (int32_t k = 0; k < n; ++k) s->rank[order[k]] = k;
-  Box rb{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-  for (int32_t m = 0; m < n; ++m) rb = box_union(rb, boxes[m]);
-  for (int k = 0; k < 3; ++k) {
-    s->ref_box[k] = rb.lo[k];
-    s->ref_box[3 + k] = rb.hi[k];
-  }
-  std::vector<BNode> nodes;
-  nodes.reserve(n > 1 ? n - 1 : 0);
-  std::iota(idx.begin(), idx.end(), 0);
-  Box all;
-  s->root_ref = build_top(idx, 0, (size_t)n, boxes, nodes, &all);
-  std::vector<MeshRef> refs(n);
-  for (int32_t m = 0; m < n; ++m) {
-    const flg_bvh* b = h_meshes[m];
-    refs[m].nodes = b->nodes;
-    refs[m].tris = b->tris;
-    refs[m].n = b->n;
-    refs[m].rank = s->rank[m];
-    for (int k = 0; k < 6; ++k) refs[m].root[k] = b->bound[k];
-    refs[m].pad[0] = refs[m].pad[1] = 0;
-  }
-  (void)hipSetDevice(ctx->device);
-  if (hipMalloc((void**)&s->meshes, sizeof(MeshRef) * (size_t)n) != hipSuccess ||
-      (!nodes.empty() && hipMalloc((void**)&s->nodes, sizeof(BNode) * nodes.size()) != hipSuccess)) {
-    flg_scene_accel_destroy(s);
-    return set_error(ctx, FLG_ERR_OOM, "flg_scene_accel_build: device allocation failed");
-  }
-  if (hipMemcpyAsync(s->meshes, refs.data(), sizeof(MeshRef) * (size_t)n, hipMemcpyHostToDevice, ctx->stream) !=
-          hipSuccess ||
-      (!nodes.empty() && hipMemcpyAsync(s->nodes, nodes.data(), sizeof(BNode) * nodes.size(),
-                                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess) {
-    flg_scene_accel_destroy(s);
-    return set_error(ctx, FLG_ERR_HIP, "flg_scene_accel_build: upload failed");
-  }
-  *out = s;
-  return FLG_OK;
+    flg_scene_accel* s = new flg_scene_accel;
+    s->ctx = ctx;
+    s->n = n;
+    s->compat = compat;
+    // the reference's leaf order and root box (merged into the default box at
+    // the origin, accel.cpp:41-42 with aabb.h's AABB())
+    std::vector<int32_t> idx(n), order;
+    std::iota(idx.begin(), idx.end(), 0);
+    order.reserve(n);
+    reference_order(idx, 0, (size_t)n, 0, boxes, order);
+    s->rank.assign(n, 0);
+    for (int32_t k = 0; k < n; ++k) s->rank[order[k]] = k;
+    Box rb{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    for (int32_t m = 0; m < n; ++m) rb = box_union(rb, boxes[m]);
+    for (int k = 0; k < 3; ++k) {
+      s->ref_box[k] = rb.lo[k];
+      s->ref_box[3 + k] = rb.hi[k];
+    }
+    std::vector<BNode> nodes;
+    nodes.reserve(n > 1 ? n - 1 : 0);
+    std::iota(idx.begin(), idx.end(), 0);
+    Box all;
+    s->root_ref = build_top(idx, 0, (size_t)n, boxes, nodes, &all);
+    std::vector<MeshRef> refs(n);
+    for (int32_t m = 0; m < n; ++m) {
+      const flg_bvh* b = h_meshes[m];
+      refs[m].nodes = b->nodes;
+      refs[m].tris = b->tris;
+      refs[m].n = b->n;
+      refs[m].rank = s->rank[m];
+      for (int k = 0; k < 6; ++k) refs[m].root[k] = b->bound[k];
+      refs[m].pad[0] = refs[m].pad[1] = 0;
+    }
+    (void)hipSetDevice(ctx->device);
+    if (hipMalloc((void**)&s->meshes, sizeof(MeshRef) * (size_t)n) != hipSuccess ||
+        (!nodes.empty() && hipMalloc((void**)&s->nodes, sizeof(BNode) * nodes.size()) != hipSuccess)) {
+      flg_scene_accel_destroy(s);
+      return set_error(ctx, FLG_ERR_OOM, "flg_scene_accel_build: device allocation failed");
+    }
+    if (hipMemcpyAsync(s->meshes, refs.data(), sizeof(MeshRef) * (size_t)n, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        (!nodes.empty() && hipMemcpyAsync(s->nodes, nodes.data(), sizeof(BNode) * nodes.size(),
+                                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      flg_scene_accel_destroy(s);
+      return set_error(ctx, FLG_ERR_HIP, "flg_scene_accel_build: upload failed");
+    }
+    *out = s;
+    return FLG_OK;
+  });
 }
 
 int flg_scene_accel_destroy(flg_scene_accel* s) {
-  if (!s) return FLG_ERR_INVALID;
-  (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->meshes) (void)hipFree(s->meshes);
-  if (s->nodes) (void)hipFree(s->nodes);
-  delete s;
-  return FLG_OK;
+  return host_guard(nullptr, "flg_scene_accel_destroy", [&]() -> int {
+    if (!s) return FLG_ERR_INVALID;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->meshes) (void)hipFree(s->meshes);
+    if (s->nodes) (void)hipFree(s->nodes);
+    delete s;
+    return FLG_OK;
+  });
 }
 
 int flg_scene_accel_rank(const flg_scene_accel* s, int32_t* h_rank) {
-  if (!s || !h_rank) return FLG_ERR_INVALID;
-  std::copy(s->rank.begin(), s->rank.end(), h_rank);
-  return FLG_OK;
+  return host_guard(nullptr, "flg_scene_accel_rank", [&]() -> int {
+    if (!s || !h_rank) return FLG_ERR_INVALID;
+    std::copy(s->rank.begin(), s->rank.end(), h_rank);
+    return FLG_OK;
+  });
 }
 
 int flg_scene_trace_closest(flg_ctx* ctx, const flg_scene_accel* s, const flg_ray* d_rays, flg_scene_hit* d_hits,
                             int64_t n) {
-  if (!ctx || !s || s->ctx != ctx || (n > 0 && (!d_rays || !d_hits)) || n < 0)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_scene_trace_closest: bad arguments");
-  if (n == 0) return FLG_OK;
-  SceneDev S;
-  S.meshes = s->meshes;
-  S.nodes = s->nodes;
-  S.root_ref = s->root_ref;
-  for (int k = 0; k < 6; ++k) S.ref_box[k] = s->ref_box[k];
-  ScopedTimer tm(ctx, "scene_trace");
-  if (s->compat)
-    k_scene_trace<true><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
-  else
-    k_scene_trace<false><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_scene_trace_closest", [&]() -> int {
+    if (!ctx || !s || s->ctx != ctx || (n > 0 && (!d_rays || !d_hits)) || n < 0)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_scene_trace_closest: bad arguments");
+    if (n == 0) return FLG_OK;
+    SceneDev S;
+    S.meshes = s->meshes;
+    S.nodes = s->nodes;
+    S.root_ref = s->root_ref;
+    for (int k = 0; k < 6; ++k) S.ref_box[k] = s->ref_box[k];
+    ScopedTimer tm(ctx, "scene_trace");
+    if (s->compat)
+      k_scene_trace<true><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
+    else
+      k_scene_trace<false><<<div_up(n, kSceneBlock), kSceneBlock, 0, ctx->stream>>>(S, d_rays, d_hits, n);
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

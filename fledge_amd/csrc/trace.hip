@@ -539,269 +539,281 @@ extern "C" {
 
 int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, flg_hit* d_hits,
                       int64_t n, const flg_trace_cfg* cfg) {
-  if (!ctx || !bvh || n < 0 || (n > 0 && (!d_rays || !d_hits)))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: bad arguments");
-  int block = cfg && cfg->block_size ? cfg->block_size : ctx->tuned_trace_block;
-  if (block == 0) block = 256;
-  int smode = cfg && cfg->stack_mode ? cfg->stack_mode : ctx->tuned_trace_stack;
-  if (!valid_block(block) || smode < 0 || smode > 2)
-    return set_error(ctx, FLG_ERR_INVALID,
-                     "flg_trace_closest: block_size must be 64, 128, 256 or 512, stack_mode 0..2");
-  if (n == 0) return FLG_OK;
-  if (div_up(n, 64) > (int64_t)INT32_MAX) return set_error(ctx, FLG_ERR_RANGE, "flg_trace_closest: too many rays");
-  TraceGeom g = geom_of(bvh);
-  {
-    ScopedTimer tm(ctx, "trace_closest");
-    if (bvh->flags & FLG_NORMAL_COMPAT)
-      launch_trace<true>(block, smode, ctx->stream, g, d_rays, d_hits, n);
-    else
-      launch_trace<false>(block, smode, ctx->stream, g, d_rays, d_hits, n);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_trace_closest", [&]() -> int {
+    if (!ctx || !bvh || n < 0 || (n > 0 && (!d_rays || !d_hits)))
+      return set_error(ctx, FLG_ERR_INVALID, "flg_trace_closest: bad arguments");
+    int block = cfg && cfg->block_size ? cfg->block_size : ctx->tuned_trace_block;
+    if (block == 0) block = 256;
+    int smode = cfg && cfg->stack_mode ? cfg->stack_mode : ctx->tuned_trace_stack;
+    if (!valid_block(block) || smode < 0 || smode > 2)
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "flg_trace_closest: block_size must be 64, 128, 256 or 512, stack_mode 0..2");
+    if (n == 0) return FLG_OK;
+    if (div_up(n, 64) > (int64_t)INT32_MAX) return set_error(ctx, FLG_ERR_RANGE, "flg_trace_closest: too many rays");
+    TraceGeom g = geom_of(bvh);
+    {
+      ScopedTimer tm(ctx, "trace_closest");
+      if (bvh->flags & FLG_NORMAL_COMPAT)
+        launch_trace<true>(block, smode, ctx->stream, g, d_rays, d_hits, n);
+      else
+        launch_trace<false>(block, smode, ctx->stream, g, d_rays, d_hits, n);
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 int flg_trace_count(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, int64_t n,
                     uint64_t* d_counts) {
-  if (!ctx || !bvh || n < 0 || !d_counts || (n > 0 && !d_rays))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_trace_count: bad arguments");
-  TraceGeom g = geom_of(bvh);
-  k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)d_counts, 4);
-  if (n == 0) return FLG_OK;
-  int grid = div_up(n, 256);
-  if (bvh->flags & FLG_NORMAL_COMPAT)
-    k_trace_count<true><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
-  else
-    k_trace_count<false><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_trace_count", [&]() -> int {
+    if (!ctx || !bvh || n < 0 || !d_counts || (n > 0 && !d_rays))
+      return set_error(ctx, FLG_ERR_INVALID, "flg_trace_count: bad arguments");
+    TraceGeom g = geom_of(bvh);
+    k_pt_zero<<<1, 256, 0, ctx->stream>>>((uint32_t*)d_counts, 4);
+    if (n == 0) return FLG_OK;
+    int grid = div_up(n, 256);
+    if (bvh->flags & FLG_NORMAL_COMPAT)
+      k_trace_count<true><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
+    else
+      k_trace_count<false><<<grid, 256, 0, ctx->stream>>>(g, d_rays, n, (unsigned long long*)d_counts);
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 int flg_camera_rays(flg_ctx* ctx, const flg_camera* cam, int32_t width, int32_t height, int32_t spp,
                     int32_t sample, flg_ray* d_rays) {
-  if (!ctx || !cam || width <= 0 || height <= 0 || spp <= 0 || sample < 0 || sample >= spp || !d_rays)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_camera_rays: bad arguments");
-  HaltonParams hp = make_halton_params(width, height);
-  if ((uint64_t)hp.stride * (uint64_t)spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, "flg_camera_rays: spp * Halton stride must stay below 2^24");
-  HaltonTables tab;
-  int st = get_halton_tables(ctx, &tab);
-  if (st) return st;
-  CamDev cd = make_cam(cam, width, height);
-  int64_t npix = (int64_t)width * height;
-  {
-    ScopedTimer tm(ctx, "camera_rays");
-    k_camera<<<div_up(npix, 256), 256, 0, ctx->stream>>>(cd, tab, hp, (uint32_t)sample, d_rays);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+  return host_guard(ctx, "flg_camera_rays", [&]() -> int {
+    if (!ctx || !cam || width <= 0 || height <= 0 || spp <= 0 || sample < 0 || sample >= spp || !d_rays)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_camera_rays: bad arguments");
+    HaltonParams hp = make_halton_params(width, height);
+    if ((uint64_t)hp.stride * (uint64_t)spp > kMaxHaltonIndex)
+      return set_error(ctx, FLG_ERR_RANGE, "flg_camera_rays: spp * Halton stride must stay below 2^24");
+    HaltonTables tab;
+    int st = get_halton_tables(ctx, &tab);
+    if (st) return st;
+    CamDev cd = make_cam(cam, width, height);
+    int64_t npix = (int64_t)width * height;
+    {
+      ScopedTimer tm(ctx, "camera_rays");
+      k_camera<<<div_up(npix, 256), 256, 0, ctx->stream>>>(cd, tab, hp, (uint32_t)sample, d_rays);
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                           const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
                           float* d_seg_t, uint64_t* h_stats) {
-  if (!ctx || !bvh || !cam || !cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
-      cfg->max_depth < 0 || cfg->max_depth > 63 || ((d_seg_id == nullptr) != (d_seg_t == nullptr)))
-    return set_error(ctx, FLG_ERR_INVALID, "flg_pathtrace_diffuse: bad arguments");
-  int tb = cfg->trace_block ? cfg->trace_block : ctx->tuned_pt_block;
-  if (tb == 0) tb = 256;
-  int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
-  if (bpc == 0) bpc = 8;
-  const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
-  int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
-  if (fmode == 0) fmode = 1;
-  if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 2)
-    return set_error(ctx, FLG_ERR_INVALID,
-                     "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
-                     "stack_mode and ray_fetch 0..2");
-  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
-  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
-  int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
-  if (npaths >= (int64_t)1 << 31) return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: too many paths");
-  HaltonTables tab;
-  int st = get_halton_tables(ctx, &tab);
-  if (st) return st;
-  CamDev cd = make_cam(cam, cfg->width, cfg->height);
-  const int D = cfg->max_depth;
-  size_t qbytes = (size_t)npaths * sizeof(QRay);
-  char* sc = (char*)scratch(ctx, "pt_queues", 2 * qbytes + (size_t)npaths * 8 + 4096, &st);
-  if (!sc) return st;
-  PtState s;
-  s.q[0] = (QRay*)sc;
-  s.q[1] = (QRay*)(sc + qbytes);
-  s.hits = (uint2*)(sc + 2 * qbytes);
-  s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
-  unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
-  const bool count_visits = cfg->count_visits != 0;
-  s.nseg = d_nseg;
-  s.seg_id = d_seg_id;
-  s.seg_t = d_seg_t;
-  s.stride_seg = D + 1;
-  TraceGeom g = geom_of(bvh);
-  const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
-  const int pgrid = ctx->num_cus * bpc;
-  // the whole frame: clear the queue counts, then one fused launch per bounce
-  // s.count[0 .. D+1]: queue sizes
-  const bool dyn = fmode == 2;
-  auto enqueue = [&](hipStream_t stream, bool timed) {
-    k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
-    if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
-    for (int b = 0; b <= D; ++b) {
-      ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
-      if (compat)
-        launch_bounce<true>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
-                            b, D, visits);
-      else
-        launch_bounce<false>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
-                             b, D, visits);
-    }
-  };
-  if (cfg->use_graph) {
-    // every kernel argument of the frame is in the key: a cached graph is
-    // replayed only for an identical launch sequence
-    std::string key;
-    auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
-    put(&g, sizeof g);
-    put(&s, sizeof s);
-    put(&cd, sizeof cd);
-    put(&tab, sizeof tab);
-    put(&hp, sizeof hp);
-    const int32_t misc[8] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0};
-    put(misc, sizeof misc);
-    put(&visits, sizeof visits);
-    hipGraphExec_t exec = nullptr;
-    for (auto& kv : ctx->graphs)
-      if (kv.first == key) exec = kv.second;
-    if (!exec) {
-      // captured on a private stream (the default stream cannot be captured)
-      // and replayed on the context's stream
-      if (!ctx->capture_stream)
-        FLG_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-      FLG_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-      enqueue(ctx->capture_stream, false);
-      hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(ctx->capture_stream, &graph);
-      if (e != hipSuccess) return check_hip(ctx, e, "hipStreamEndCapture (path tracer)");
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) return check_hip(ctx, e, "hipGraphInstantiate (path tracer)");
-      if (ctx->graphs.size() >= 8) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipGraphExecDestroy(ctx->graphs.front().second);
-        ctx->graphs.erase(ctx->graphs.begin());
+  return host_guard(ctx, "flg_pathtrace_diffuse", [&]() -> int {
+    if (!ctx || !bvh || !cam || !cfg || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
+        cfg->max_depth < 0 || cfg->max_depth > 63 || ((d_seg_id == nullptr) != (d_seg_t == nullptr)))
+      return set_error(ctx, FLG_ERR_INVALID, "flg_pathtrace_diffuse: bad arguments");
+    int tb = cfg->trace_block ? cfg->trace_block : ctx->tuned_pt_block;
+    if (tb == 0) tb = 256;
+    int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
+    if (bpc == 0) bpc = 8;
+    const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
+    int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
+    if (fmode == 0) fmode = 1;
+    if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 2)
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
+                       "stack_mode and ray_fetch 0..2");
+    HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+    if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+      return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
+    int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
+    if (npaths >= (int64_t)1 << 31) return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: too many paths");
+    HaltonTables tab;
+    int st = get_halton_tables(ctx, &tab);
+    if (st) return st;
+    CamDev cd = make_cam(cam, cfg->width, cfg->height);
+    const int D = cfg->max_depth;
+    size_t qbytes = (size_t)npaths * sizeof(QRay);
+    char* sc = (char*)scratch(ctx, "pt_queues", 2 * qbytes + (size_t)npaths * 8 + 4096, &st);
+    if (!sc) return st;
+    PtState s;
+    s.q[0] = (QRay*)sc;
+    s.q[1] = (QRay*)(sc + qbytes);
+    s.hits = (uint2*)(sc + 2 * qbytes);
+    s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
+    unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
+    const bool count_visits = cfg->count_visits != 0;
+    s.nseg = d_nseg;
+    s.seg_id = d_seg_id;
+    s.seg_t = d_seg_t;
+    s.stride_seg = D + 1;
+    TraceGeom g = geom_of(bvh);
+    const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
+    const int pgrid = ctx->num_cus * bpc;
+    // the whole frame: clear the queue counts, then one fused launch per bounce
+    // s.count[0 .. D+1]: queue sizes
+    const bool dyn = fmode == 2;
+    auto enqueue = [&](hipStream_t stream, bool timed) {
+      k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
+      if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
+      for (int b = 0; b <= D; ++b) {
+        ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
+        if (compat)
+          launch_bounce<true>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+                              b, D, visits);
+        else
+          launch_bounce<false>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+                               b, D, visits);
       }
-      ctx->graphs.emplace_back(std::move(key), exec);
+    };
+    if (cfg->use_graph) {
+      // every kernel argument of the frame is in the key: a cached graph is
+      // replayed only for an identical launch sequence
+      std::string key;
+      auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+      put(&g, sizeof g);
+      put(&s, sizeof s);
+      put(&cd, sizeof cd);
+      put(&tab, sizeof tab);
+      put(&hp, sizeof hp);
+      const int32_t misc[8] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0};
+      put(misc, sizeof misc);
+      put(&visits, sizeof visits);
+      hipGraphExec_t exec = nullptr;
+      for (auto& kv : ctx->graphs)
+        if (kv.first == key) exec = kv.second;
+      if (!exec) {
+        // captured on a private stream (the default stream cannot be captured)
+        // and replayed on the context's stream
+        if (!ctx->capture_stream)
+          FLG_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        FLG_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+        enqueue(ctx->capture_stream, false);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->capture_stream, &graph);
+        if (e != hipSuccess) return check_hip(ctx, e, "hipStreamEndCapture (path tracer)");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return check_hip(ctx, e, "hipGraphInstantiate (path tracer)");
+        if (ctx->graphs.size() >= 8) {
+          (void)hipStreamSynchronize(ctx->stream);
+          (void)hipGraphExecDestroy(ctx->graphs.front().second);
+          ctx->graphs.erase(ctx->graphs.begin());
+        }
+        ctx->graphs.emplace_back(std::move(key), exec);
+      }
+      ScopedTimer tm(ctx, "pathtrace_total");
+      FLG_CHECK_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+    } else {
+      ScopedTimer tm(ctx, "pathtrace_total");
+      enqueue(ctx->stream, true);
     }
-    ScopedTimer tm(ctx, "pathtrace_total");
-    FLG_CHECK_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
-  } else {
-    ScopedTimer tm(ctx, "pathtrace_total");
-    enqueue(ctx->stream, true);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  if (h_stats) {
-    std::vector<uint32_t> cnt(D + 2);
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(cnt.data(), s.count, (D + 2) * 4, hipMemcpyDeviceToHost, ctx->stream));
-    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    uint64_t tot = 0;
-    for (int b = 0; b <= D; ++b) tot += cnt[b];
-    h_stats[0] = tot;
-    h_stats[1] = (uint64_t)npaths;
-    if (count_visits) {
-      unsigned long long v[2] = {0, 0};
-      FLG_CHECK_HIP(ctx, hipMemcpyAsync(v, visits, 16, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_LAUNCH(ctx);
+    if (h_stats) {
+      std::vector<uint32_t> cnt(D + 2);
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(cnt.data(), s.count, (D + 2) * 4, hipMemcpyDeviceToHost, ctx->stream));
       FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      h_stats[2] = v[0];
-      h_stats[3] = v[1];
+      uint64_t tot = 0;
+      for (int b = 0; b <= D; ++b) tot += cnt[b];
+      h_stats[0] = tot;
+      h_stats[1] = (uint64_t)npaths;
+      if (count_visits) {
+        unsigned long long v[2] = {0, 0};
+        FLG_CHECK_HIP(ctx, hipMemcpyAsync(v, visits, 16, hipMemcpyDeviceToHost, ctx->stream));
+        FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        h_stats[2] = v[0];
+        h_stats[3] = v[1];
+      }
     }
-  }
-  return FLG_OK;
+    return FLG_OK;
+  });
 }
 
 int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, flg_hit* d_hits,
                        int64_t n, flg_tuning* out) {
-  if (!ctx || !bvh || n <= 0 || !d_rays || !d_hits || !out)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_trace: bad arguments");
-  const int blocks[4] = {64, 128, 256, 512};
-  int best_b = 256, best_s = 1, ncand = 0;
-  float best = 1e30f, def = 0.0f;
-  for (int sm = 1; sm <= 2; ++sm)
-    for (int b : blocks) {
-      flg_trace_cfg c{};
-      c.block_size = b;
-      c.stack_mode = sm;
-      float ms = 0.0f;
-      int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
-      if (st) return st;
-      ++ncand;
-      if (b == 256 && sm == 1) def = ms;
-      if (ms < best) {
-        best = ms;
-        best_b = b;
-        best_s = sm;
+  return host_guard(ctx, "flg_autotune_trace", [&]() -> int {
+    if (!ctx || !bvh || n <= 0 || !d_rays || !d_hits || !out)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_trace: bad arguments");
+    const int blocks[4] = {64, 128, 256, 512};
+    int best_b = 256, best_s = 1, ncand = 0;
+    float best = 1e30f, def = 0.0f;
+    for (int sm = 1; sm <= 2; ++sm)
+      for (int b : blocks) {
+        flg_trace_cfg c{};
+        c.block_size = b;
+        c.stack_mode = sm;
+        float ms = 0.0f;
+        int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
+        if (st) return st;
+        ++ncand;
+        if (b == 256 && sm == 1) def = ms;
+        if (ms < best) {
+          best = ms;
+          best_b = b;
+          best_s = sm;
+        }
       }
-    }
-  ctx->tuned_trace_block = best_b;
-  ctx->tuned_trace_stack = best_s;
-  *out = flg_tuning{};
-  out->block_size = best_b;
-  out->stack_mode = best_s;
-  out->ms = best;
-  out->ms_default = def;
-  out->candidates = ncand;
-  return FLG_OK;
+    ctx->tuned_trace_block = best_b;
+    ctx->tuned_trace_stack = best_s;
+    *out = flg_tuning{};
+    out->block_size = best_b;
+    out->stack_mode = best_s;
+    out->ms = best;
+    out->ms_default = def;
+    out->candidates = ncand;
+    return FLG_OK;
+  });
 }
 
 int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                            const flg_pt_cfg* cfg, flg_tuning* out) {
-  if (!ctx || !bvh || !cam || !cfg || !out)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_pathtrace: bad arguments");
-  // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
-  const int blocks[4] = {64, 128, 256, 512};
-  const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
-  int best_b = 256, best_bpc = 8, best_s = 1, best_f = 1, n = 0;
-  float best = 1e30f, def = 0.0f;
-  for (int fm = 1; fm <= 2; ++fm)
-  for (int sm = 1; sm <= 2; ++sm)
-    for (int b : blocks)
-      for (int t : threads_per_cu) {
-        flg_pt_cfg c = *cfg;
-        c.trace_block = b;
-        c.blocks_per_cu = t / b;
-        c.stack_mode = sm;
-        c.ray_fetch = fm;
-        c.count_visits = 0;
-        c.use_graph = 0;
-        if (c.blocks_per_cu > 64) continue;
-        float ms = 0.0f;
-        int st = time_call(
-            ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); },
-            2, &ms);
-        if (st) return st;
-        ++n;
-        if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 1) def = ms;
-        if (ms < best) {
-          best = ms;
-          best_b = b;
-          best_bpc = c.blocks_per_cu;
-          best_s = sm;
-          best_f = fm;
+  return host_guard(ctx, "flg_autotune_pathtrace", [&]() -> int {
+    if (!ctx || !bvh || !cam || !cfg || !out)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_pathtrace: bad arguments");
+    // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
+    const int blocks[4] = {64, 128, 256, 512};
+    const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
+    int best_b = 256, best_bpc = 8, best_s = 1, best_f = 1, n = 0;
+    float best = 1e30f, def = 0.0f;
+    for (int fm = 1; fm <= 2; ++fm)
+    for (int sm = 1; sm <= 2; ++sm)
+      for (int b : blocks)
+        for (int t : threads_per_cu) {
+          flg_pt_cfg c = *cfg;
+          c.trace_block = b;
+          c.blocks_per_cu = t / b;
+          c.stack_mode = sm;
+          c.ray_fetch = fm;
+          c.count_visits = 0;
+          c.use_graph = 0;
+          if (c.blocks_per_cu > 64) continue;
+          float ms = 0.0f;
+          int st = time_call(
+              ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); },
+              2, &ms);
+          if (st) return st;
+          ++n;
+          if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 1) def = ms;
+          if (ms < best) {
+            best = ms;
+            best_b = b;
+            best_bpc = c.blocks_per_cu;
+            best_s = sm;
+            best_f = fm;
+          }
         }
-      }
-  ctx->tuned_pt_block = best_b;
-  ctx->tuned_pt_bpc = best_bpc;
-  ctx->tuned_pt_stack = best_s;
-  ctx->tuned_pt_fetch = best_f;
-  *out = flg_tuning{};
-  out->block_size = best_b;
-  out->blocks_per_cu = best_bpc;
-  out->stack_mode = best_s;
-  out->ray_fetch = best_f;
-  out->ms = best;
-  out->ms_default = def;
-  out->candidates = n;
-  return FLG_OK;
+    ctx->tuned_pt_block = best_b;
+    ctx->tuned_pt_bpc = best_bpc;
+    ctx->tuned_pt_stack = best_s;
+    ctx->tuned_pt_fetch = best_f;
+    *out = flg_tuning{};
+    out->block_size = best_b;
+    out->blocks_per_cu = best_bpc;
+    out->stack_mode = best_s;
+    out->ray_fetch = best_f;
+    out->ms = best;
+    out->ms_default = def;
+    out->candidates = n;
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

@@ -542,137 +542,143 @@ extern "C" {
 
 int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, const flg_vol_scene* scene,
                             const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats) {
-  if (!ctx || !bvh || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
-      cfg->max_depth < 1)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: bad arguments");
-  if (cfg->film < 0 || cfg->film > 1)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: film must be 0 or 1");
-  if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
-    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: too many paths");
-  const int sched = cfg->schedule ? cfg->schedule : 2;
-  if (sched < 1 || sched > 2)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: schedule must be 0, 1 or 2");
-  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_render: empty mesh");
-  if (st) return st;
-  const float* mb = bvh->bound;
-  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
-  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay below 2^24");
-  const bool film = cfg->film == 1;
-  const int64_t npix = (int64_t)cfg->width * cfg->height;
-  // film mode: spp - 1 splatted samples per pixel (the first task also runs
-  // the pixel-centre path); with spp == 1 nothing reaches the film
-  const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
-  const int64_t ntasks = npix * tasks;
-  // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
-  char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
-  if (!sc) return st;
-  unsigned long long* nl = (unsigned long long*)sc;
-  unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
-  float* lpath = (float*)(sc + 256);
-  float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 4 + ((ntasks & 1) ? 4 : 0)) : nullptr;
-  HaltonTables tab;
-  if ((st = get_halton_tables(ctx, &tab))) return st;
-  LightsDev L;
-  if ((st = make_lights(ctx, vol, scene, &L, mb))) return st;
-  // the medium on the mesh's bounding box (setBound)
-  const VolDev v = voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, mb, vol->medium, vol->max_density);
-  const CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
-  const TraceGeom g = geom_of(bvh);
-  const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
-  const bool count = h_stats != nullptr;
-  const int blocks1 = (int)div_up(ntasks, kMeshBlock);
-  const int blocks2 = (int)std::min<int64_t>(div_up(ntasks, kMeshBlock), (int64_t)ctx->num_cus * 8);
-  k_vp_zero<<<1, 64, 0, ctx->stream>>>(nl);
-  if (ntasks > 0) {
-    ScopedTimer tm(ctx, "volpath_mesh_paths");
-    if (compat && count)
-      launch_volpath_film<true, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
-                                      cfg->max_depth, lpath, fpos, next_path, nl);
-    else if (compat)
-      launch_volpath_film<true, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
-                                       cfg->max_depth, lpath, fpos, next_path, nl);
-    else if (count)
-      launch_volpath_film<false, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
-                                       cfg->max_depth, lpath, fpos, next_path, nl);
-    else
-      launch_volpath_film<false, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+  return host_guard(ctx, "flg_volpath_mesh_render", [&]() -> int {
+    if (!ctx || !bvh || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
+        cfg->max_depth < 1)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: bad arguments");
+    if (cfg->film < 0 || cfg->film > 1)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: film must be 0 or 1");
+    if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: too many paths");
+    const int sched = cfg->schedule ? cfg->schedule : 2;
+    if (sched < 1 || sched > 2)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_render: schedule must be 0, 1 or 2");
+    int st = check_mesh(ctx, bvh, "flg_volpath_mesh_render: empty mesh");
+    if (st) return st;
+    const float* mb = bvh->bound;
+    HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+    if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay below 2^24");
+    const bool film = cfg->film == 1;
+    const int64_t npix = (int64_t)cfg->width * cfg->height;
+    // film mode: spp - 1 splatted samples per pixel (the first task also runs
+    // the pixel-centre path); with spp == 1 nothing reaches the film
+    const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
+    const int64_t ntasks = npix * tasks;
+    // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
+    char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
+    if (!sc) return st;
+    unsigned long long* nl = (unsigned long long*)sc;
+    unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
+    float* lpath = (float*)(sc + 256);
+    float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 4 + ((ntasks & 1) ? 4 : 0)) : nullptr;
+    HaltonTables tab;
+    if ((st = get_halton_tables(ctx, &tab))) return st;
+    LightsDev L;
+    if ((st = make_lights(ctx, vol, scene, &L, mb))) return st;
+    // the medium on the mesh's bounding box (setBound)
+    const VolDev v = voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, mb, vol->medium, vol->max_density);
+    const CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
+    const TraceGeom g = geom_of(bvh);
+    const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
+    const bool count = h_stats != nullptr;
+    const int blocks1 = (int)div_up(ntasks, kMeshBlock);
+    const int blocks2 = (int)std::min<int64_t>(div_up(ntasks, kMeshBlock), (int64_t)ctx->num_cus * 8);
+    k_vp_zero<<<1, 64, 0, ctx->stream>>>(nl);
+    if (ntasks > 0) {
+      ScopedTimer tm(ctx, "volpath_mesh_paths");
+      if (compat && count)
+        launch_volpath_film<true, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
                                         cfg->max_depth, lpath, fpos, next_path, nl);
-  }
-  if (film)
-    film_resolve(ctx->stream, ntasks > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
-  else
-    k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
-  FLG_CHECK_LAUNCH(ctx);
-  if (h_stats) {
-    unsigned long long c = 0;
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
-    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    h_stats[0] = (uint64_t)(ntasks > 0 ? npix * cfg->spp : 0);
-    h_stats[1] = c;
-  }
-  return FLG_OK;
+      else if (compat)
+        launch_volpath_film<true, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                         cfg->max_depth, lpath, fpos, next_path, nl);
+      else if (count)
+        launch_volpath_film<false, true>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                         cfg->max_depth, lpath, fpos, next_path, nl);
+      else
+        launch_volpath_film<false, false>(film, sched, blocks1, blocks2, ctx->stream, g, v, L, tab, hp, cam, tasks,
+                                          cfg->max_depth, lpath, fpos, next_path, nl);
+    }
+    if (film)
+      film_resolve(ctx->stream, ntasks > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
+    else
+      k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+    FLG_CHECK_LAUNCH(ctx);
+    if (h_stats) {
+      unsigned long long c = 0;
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+      FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      h_stats[0] = (uint64_t)(ntasks > 0 ? npix * cfg->spp : 0);
+      h_stats[1] = c;
+    }
+    return FLG_OK;
+  });
 }
 
 int flg_volpath_mesh_diff_forward(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, const flg_volume* proposal,
                                   const flg_vol_scene* scene, const flg_render_cfg* cfg,
                                   const flg_diff_params* params, float* d_rgb, float* d_lpath, void* d_tape) {
-  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_diff_forward: no mesh");
-  if (st) return st;
-  DiffDev D;
-  LightsDev L;
-  HaltonTables tab;
-  HaltonParams hp;
-  CamDevV cam;
-  if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
-    return st;
-  if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_diff_forward: no output");
-  const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
-  float* lpath = d_lpath;
-  if (!lpath) {
-    lpath = (float*)scratch(ctx, "vpdiff_lpath", (size_t)npaths * 4, &st);
-    if (!lpath) return st;
-  }
-  const TraceGeom g = geom_of(bvh);
-  {
-    ScopedTimer tm(ctx, "vpdiff_forward");
-    st = (bvh->flags & FLG_NORMAL_COMPAT)
-             ? launch_vpdiff<false, true>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr)
-             : launch_vpdiff<false, false>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr);
+  return host_guard(ctx, "flg_volpath_mesh_diff_forward", [&]() -> int {
+    int st = check_mesh(ctx, bvh, "flg_volpath_mesh_diff_forward: no mesh");
     if (st) return st;
-    if (d_rgb) k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+    DiffDev D;
+    LightsDev L;
+    HaltonTables tab;
+    HaltonParams hp;
+    CamDevV cam;
+    if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
+      return st;
+    if (!d_rgb && !d_lpath) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_diff_forward: no output");
+    const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
+    float* lpath = d_lpath;
+    if (!lpath) {
+      lpath = (float*)scratch(ctx, "vpdiff_lpath", (size_t)npaths * 4, &st);
+      if (!lpath) return st;
+    }
+    const TraceGeom g = geom_of(bvh);
+    {
+      ScopedTimer tm(ctx, "vpdiff_forward");
+      st = (bvh->flags & FLG_NORMAL_COMPAT)
+               ? launch_vpdiff<false, true>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr)
+               : launch_vpdiff<false, false>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr);
+      if (st) return st;
+      if (d_rgb) k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 int flg_volpath_mesh_adjoint(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, const flg_volume* proposal,
                              const flg_vol_scene* scene, const flg_render_cfg* cfg, const flg_diff_params* params,
                              const float* d_lpath, const void* d_tape, const float* d_dl_drgb,
                              const flg_brick_layout* bricks, float* d_grad, float* d_grad_albedo) {
-  int st = check_mesh(ctx, bvh, "flg_volpath_mesh_adjoint: no mesh");
-  if (st) return st;
-  DiffDev D;
-  LightsDev L;
-  HaltonTables tab;
-  HaltonParams hp;
-  CamDevV cam;
-  if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
-    return st;
-  if ((st = diff_set_bricks(ctx, bricks, &D))) return st;
-  if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_adjoint: null buffer");
-  const TraceGeom g = geom_of(bvh);
-  {
-    ScopedTimer tm(ctx, "vpdiff_adjoint");
-    float* lp = (float*)d_lpath;
-    st = (bvh->flags & FLG_NORMAL_COMPAT)
-             ? launch_vpdiff<true, true>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo)
-             : launch_vpdiff<true, false>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo);
+  return host_guard(ctx, "flg_volpath_mesh_adjoint", [&]() -> int {
+    int st = check_mesh(ctx, bvh, "flg_volpath_mesh_adjoint: no mesh");
     if (st) return st;
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  return FLG_OK;
+    DiffDev D;
+    LightsDev L;
+    HaltonTables tab;
+    HaltonParams hp;
+    CamDevV cam;
+    if ((st = diff_prepare(ctx, vol, proposal, scene, cfg, params, d_tape, bvh->bound, &D, &L, &tab, &hp, &cam)))
+      return st;
+    if ((st = diff_set_bricks(ctx, bricks, &D))) return st;
+    if (!d_lpath || !d_dl_drgb || !d_grad || !d_grad_albedo)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_mesh_adjoint: null buffer");
+    const TraceGeom g = geom_of(bvh);
+    {
+      ScopedTimer tm(ctx, "vpdiff_adjoint");
+      float* lp = (float*)d_lpath;
+      st = (bvh->flags & FLG_NORMAL_COMPAT)
+               ? launch_vpdiff<true, true>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo)
+               : launch_vpdiff<true, false>(ctx, cfg, g, D, L, tab, hp, cam, lp, d_dl_drgb, d_grad, d_grad_albedo);
+      if (st) return st;
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"

@@ -449,128 +449,136 @@ extern "C" {
 
 int flg_volume_create(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz,
                       const float* h_bounds, const flg_medium* medium, flg_volume** out) {
-  if (!ctx || !d_density || !h_bounds || !medium || !out || nx <= 0 || ny <= 0 || nz <= 0)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: bad arguments");
-  *out = nullptr;
-  if ((int64_t)(nx + 2) * (ny + 2) * (nz + 2) >= ((int64_t)1 << 31))
-    return set_error(ctx, FLG_ERR_RANGE, "flg_volume_create: grid too large");
-  for (int k = 0; k < 3; ++k)
-    if (!(h_bounds[3 + k] > h_bounds[k])) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: empty bounds");
-  FLG_CHECK_HIP(ctx, hipSetDevice(ctx->device));
-  flg_volume* vol = new flg_volume;
-  vol->ctx = ctx;
-  vol->nx = nx;
-  vol->ny = ny;
-  vol->nz = nz;
-  for (int k = 0; k < 6; ++k) vol->bounds[k] = h_bounds[k];
-  vol->medium = *medium;
-  const int64_t tot = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
-  hipError_t e = hipMalloc(&vol->grid, tot * 4);
-  if (e != hipSuccess) {
-    delete vol;
-    return check_hip(ctx, e, "hipMalloc(volume grid)");
-  }
-  *out = vol;
-  int st = flg_volume_update(ctx, vol, d_density);
-  if (st) {
-    flg_volume_destroy(vol);
+  return host_guard(ctx, "flg_volume_create", [&]() -> int {
+    if (!ctx || !d_density || !h_bounds || !medium || !out || nx <= 0 || ny <= 0 || nz <= 0)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: bad arguments");
     *out = nullptr;
-  }
-  return st;
+    if ((int64_t)(nx + 2) * (ny + 2) * (nz + 2) >= ((int64_t)1 << 31))
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volume_create: grid too large");
+    for (int k = 0; k < 3; ++k)
+      if (!(h_bounds[3 + k] > h_bounds[k])) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_create: empty bounds");
+    FLG_CHECK_HIP(ctx, hipSetDevice(ctx->device));
+    flg_volume* vol = new flg_volume;
+    vol->ctx = ctx;
+    vol->nx = nx;
+    vol->ny = ny;
+    vol->nz = nz;
+    for (int k = 0; k < 6; ++k) vol->bounds[k] = h_bounds[k];
+    vol->medium = *medium;
+    const int64_t tot = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+    hipError_t e = hipMalloc(&vol->grid, tot * 4);
+    if (e != hipSuccess) {
+      delete vol;
+      return check_hip(ctx, e, "hipMalloc(volume grid)");
+    }
+    *out = vol;
+    int st = flg_volume_update(ctx, vol, d_density);
+    if (st) {
+      flg_volume_destroy(vol);
+      *out = nullptr;
+    }
+    return st;
+  });
 }
 
 int flg_volume_update(flg_ctx* ctx, flg_volume* vol, const float* d_density) {
-  if (!ctx || !vol || !d_density) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_update: bad arguments");
-  const int64_t tot = (int64_t)(vol->nx + 2) * (vol->ny + 2) * (vol->nz + 2);
-  k_pad<<<div_up(tot, 256), 256, 0, ctx->stream>>>(d_density, vol->grid, vol->nx, vol->ny, vol->nz);
-  FLG_CHECK_LAUNCH(ctx);
-  return refresh_max(ctx, vol, d_density);
+  return host_guard(ctx, "flg_volume_update", [&]() -> int {
+    if (!ctx || !vol || !d_density) return set_error(ctx, FLG_ERR_INVALID, "flg_volume_update: bad arguments");
+    const int64_t tot = (int64_t)(vol->nx + 2) * (vol->ny + 2) * (vol->nz + 2);
+    k_pad<<<div_up(tot, 256), 256, 0, ctx->stream>>>(d_density, vol->grid, vol->nx, vol->ny, vol->nz);
+    FLG_CHECK_LAUNCH(ctx);
+    return refresh_max(ctx, vol, d_density);
+  });
 }
 
 int flg_volume_destroy(flg_volume* vol) {
-  if (!vol) return FLG_ERR_INVALID;
-  (void)hipSetDevice(vol->ctx->device);
-  (void)hipStreamSynchronize(vol->ctx->stream);
-  (void)hipFree(vol->grid);
-  if (vol->env) (void)hipFree(vol->env);
-  delete vol;
-  return FLG_OK;
+  return host_guard(nullptr, "flg_volume_destroy", [&]() -> int {
+    if (!vol) return FLG_ERR_INVALID;
+    (void)hipSetDevice(vol->ctx->device);
+    (void)hipStreamSynchronize(vol->ctx->stream);
+    (void)hipFree(vol->grid);
+    if (vol->env) (void)hipFree(vol->env);
+    delete vol;
+    return FLG_OK;
+  });
 }
 
 int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                        const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats) {
-  if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 ||
-      cfg->spp <= 0 || cfg->max_depth < 1)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
-  if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
-    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: too many paths");
-  if (cfg->film < 0 || cfg->film > 1)
-    return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: film must be 0 or 1");
-  const int sched = cfg->schedule ? cfg->schedule : 2;
-  if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
-  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
-  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
-  const bool film = cfg->film == 1;
-  const int64_t npix = (int64_t)cfg->width * cfg->height;
-  // film mode: spp - 1 splatted samples per pixel (the first task also runs
-  // the pixel-centre path); with spp == 1 nothing reaches the film
-  const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
-  const int64_t npaths = npix * tasks;
-  int st = FLG_OK;
-  // [256 B counters][npaths radiance floats][pad to 8 B][npaths film positions]
-  char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * (film ? 12 : 4) + 264, &st);
-  if (!sc) return st;
-  unsigned long long* nl = (unsigned long long*)sc;
-  unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
-  float* lpath = (float*)(sc + 256);
-  float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
-  const bool count = h_stats != nullptr;
-  k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
-  if (npaths > 0) {
-    HaltonTables tab;
-    if ((st = get_halton_tables(ctx, &tab))) return st;
-    LightsDev L;
-    if ((st = make_lights(ctx, vol, scene, &L))) return st;
-    VolDev v = make_voldev(vol);
-    CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
-    ScopedTimer tm(ctx, "volpath_paths");
-    const int blocks1 = (int)div_up(npaths, 256);
-    const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
-#define FLG_SVOL_LAUNCH(C, F)                                                                              \
-  if (sched == 1)                                                                                          \
-    k_svol<C, F><<<blocks1, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, fpos, \
-                                                   nl);                                                    \
-  else                                                                                                     \
-    k_svol_regen<C, F><<<blocks2, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, \
-                                                         fpos, next_path, nl);
-    if (count && film) {
-      FLG_SVOL_LAUNCH(true, true)
-    } else if (count) {
-      FLG_SVOL_LAUNCH(true, false)
-    } else if (film) {
-      FLG_SVOL_LAUNCH(false, true)
-    } else {
-      FLG_SVOL_LAUNCH(false, false)
+  return host_guard(ctx, "flg_volpath_render", [&]() -> int {
+    if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 ||
+        cfg->spp <= 0 || cfg->max_depth < 1)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
+    if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: too many paths");
+    if (cfg->film < 0 || cfg->film > 1)
+      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: film must be 0 or 1");
+    const int sched = cfg->schedule ? cfg->schedule : 2;
+    if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
+    HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+    if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
+    const bool film = cfg->film == 1;
+    const int64_t npix = (int64_t)cfg->width * cfg->height;
+    // film mode: spp - 1 splatted samples per pixel (the first task also runs
+    // the pixel-centre path); with spp == 1 nothing reaches the film
+    const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
+    const int64_t npaths = npix * tasks;
+    int st = FLG_OK;
+    // [256 B counters][npaths radiance floats][pad to 8 B][npaths film positions]
+    char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * (film ? 12 : 4) + 264, &st);
+    if (!sc) return st;
+    unsigned long long* nl = (unsigned long long*)sc;
+    unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
+    float* lpath = (float*)(sc + 256);
+    float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
+    const bool count = h_stats != nullptr;
+    k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
+    if (npaths > 0) {
+      HaltonTables tab;
+      if ((st = get_halton_tables(ctx, &tab))) return st;
+      LightsDev L;
+      if ((st = make_lights(ctx, vol, scene, &L))) return st;
+      VolDev v = make_voldev(vol);
+      CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
+      ScopedTimer tm(ctx, "volpath_paths");
+      const int blocks1 = (int)div_up(npaths, 256);
+      const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+  #define FLG_SVOL_LAUNCH(C, F)                                                                              \
+    if (sched == 1)                                                                                          \
+      k_svol<C, F><<<blocks1, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, fpos, \
+                                                     nl);                                                    \
+    else                                                                                                     \
+      k_svol_regen<C, F><<<blocks2, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, \
+                                                           fpos, next_path, nl);
+      if (count && film) {
+        FLG_SVOL_LAUNCH(true, true)
+      } else if (count) {
+        FLG_SVOL_LAUNCH(true, false)
+      } else if (film) {
+        FLG_SVOL_LAUNCH(false, true)
+      } else {
+        FLG_SVOL_LAUNCH(false, false)
+      }
+  #undef FLG_SVOL_LAUNCH
     }
-#undef FLG_SVOL_LAUNCH
-  }
-  {
-    ScopedTimer tm(ctx, "volpath_mean");
-    if (film)
-      film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
-    else
-      k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
-  }
-  FLG_CHECK_LAUNCH(ctx);
-  if (h_stats) {
-    unsigned long long c = 0;
-    FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
-    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
-    h_stats[1] = c;
-  }
-  return FLG_OK;
+    {
+      ScopedTimer tm(ctx, "volpath_mean");
+      if (film)
+        film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
+      else
+        k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+    }
+    FLG_CHECK_LAUNCH(ctx);
+    if (h_stats) {
+      unsigned long long c = 0;
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+      FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
+      h_stats[1] = c;
+    }
+    return FLG_OK;
+  });
 }
 
 }  // extern "C"
