@@ -117,16 +117,20 @@ def test_volume_white_furnace(ctx):
 
 
 @pytest.mark.parametrize("schedule", [1, 2])
-def test_volpath_schedules_identical(ctx, schedule):
+@pytest.mark.parametrize("W,H,spp", [(96, 80, 8), (37, 23, 3), (211, 197, 5)])
+def test_volpath_schedules_identical(ctx, schedule, W, H, spp):
     """One lane per path and persistent lanes with path regeneration give the
-    same image and lookup count, bit for bit, and both match the oracle."""
+    same image and lookup count, bit for bit, and both match the oracle.  The
+    regenerating lanes claim paths 128 at a time per wave: 61,440 paths fill
+    whole chunks, 2,553 leave most waves without one, and 207,835 send waves
+    back for more with a partial last chunk."""
     grid = api.gen_fbm_grid(32, seed=8)
     vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
     sc = api.vol_scene(cam)
-    out, st = vol.render(sc, 96, 80, 8, 32, stats=True, schedule=schedule)
+    out, st = vol.render(sc, W, H, spp, 32, stats=True, schedule=schedule)
     ref, lk, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], [1, 2, 2, -2, 50.0, 1, 0.5],
-                                    CAM0, 96, 80, 8, 32)
+                                    CAM0, W, H, spp, 32)
     assert st["lookups"] == lk
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
