@@ -242,7 +242,8 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g,
   vp_count<COUNT>(nlook, nlookups);
 }
 
-// Persistent lanes with path regeneration (regen_loop); same per-task code.
+// Persistent lanes with path regeneration (regen_loop), paths started in
+// tile_order; same per-task code.
 template <bool COMPAT, bool COUNT, bool FILM>
 __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceGeom g, VolDev v, LightsDev L,
                                                                           HaltonTables tab_g, HaltonParams hp,
@@ -258,15 +259,17 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceG
   uint32_t nlook = 0;
   regen_loop<VPath>(
       ntasks, next_path,
-      [&](uint32_t p, VPath& s) {
-        vp_task_begin<FILM>(tab, hp, cam, tasks, p, s);
+      [&](uint32_t q, VPath& s) {
+        vp_task_begin<FILM>(tab, hp, cam, tasks, tile_order(q, tasks, cam.w, cam.h), s);
         return false;
       },
       [&](VPath& s) {
         return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook) &&
                (!FILM || vp_film_next(tab, hp, cam, s));
       },
-      [&](uint32_t p, const VPath& s) { vp_task_finish<FILM>(p, s, lpath, fpos); });
+      [&](uint32_t q, const VPath& s) {
+        vp_task_finish<FILM>(tile_order(q, tasks, cam.w, cam.h), s, lpath, fpos);
+      });
   vp_count<COUNT>(nlook, nlookups);
 }
 
@@ -454,7 +457,8 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff(TraceGeom g, 
   reduce_galpha<ADJ>(ga, galpha);
 }
 
-// Persistent lanes with path regeneration (regen_loop); same per-path code.
+// Persistent lanes with path regeneration (regen_loop), paths started in
+// tile_order; same per-path code.
 template <bool ADJ, bool COMPAT>
 __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff_regen(TraceGeom g, DiffDev D, LightsDev L, HaltonTables tab_g,
                                                       HaltonParams hp, CamDevV cam, int32_t spp, int32_t max_depth,
@@ -468,10 +472,12 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff_regen(TraceGe
   float ga = 0.0f;
   regen_loop<DiffVPath>(
       npaths, next_path,
-      [&](uint32_t p, DiffVPath& s) { return vpd_begin<ADJ>(D, tab, hp, cam, spp, p, lpath, dl_drgb, s); },
+      [&](uint32_t q, DiffVPath& s) {
+        return vpd_begin<ADJ>(D, tab, hp, cam, spp, tile_order(q, spp, cam.w, cam.h), lpath, dl_drgb, s);
+      },
       [&](DiffVPath& s) { return vpd_step<ADJ, COMPAT>(mg, D, L, tab, hp, max_depth, grad, &ga, s); },
-      [&](uint32_t p, const DiffVPath& s) {
-        if (!ADJ) vpd_finish(D, p, s, lpath);
+      [&](uint32_t q, const DiffVPath& s) {
+        if (!ADJ) vpd_finish(D, tile_order(q, spp, cam.w, cam.h), s, lpath);
       });
   reduce_galpha<ADJ>(ga, galpha);
 }
