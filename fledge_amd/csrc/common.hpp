@@ -60,6 +60,18 @@ constexpr float SHADOW_EPS = 1e-4f;
 constexpr float ONE_MINUS_EPSILON = 0.99999994f;  // src/common/sampler.h:15
 
 // ------------------------------------------------------ transcendentals ---
+// The polynomial coefficients and evaluation order of flg_logf, flg_sindf and
+// flg_cosdf below come from fdlibm's e_logf.c and FreeBSD msun's k_sinf.c and
+// k_cosf.c (float conversion by Ian Lance Taylor, Cygnus Support; kernels
+// optimised by Bruce D. Evans), distributed under this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
 // Natural log of a float: fdlibm e_logf.c method (reduce to m in
 // [sqrt(2)/2, sqrt(2)), s = f / (2 + f), degree-4 minimax polynomial in s^2),
 // plain IEEE float operations in a fixed order; < 1 ulp.

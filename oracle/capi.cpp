@@ -13,6 +13,8 @@ void diffuse_render(const Lbvh& b, bool compat, const Camera& cam, int W, int H,
                     int max_depth, int64_t p0, int64_t p1, int32_t* nseg, int32_t* ids, float* ts,
                     int nthreads, uint64_t* rays_out);
 V3 offset_ray_origin(V3 p, V3 n, V3 dir);
+void diffuse_sample(V3 ng, V3 wo, float u0, float u1, V3* wi, float* pdf);
+void shading_frame(V3 n, V3* t, V3* b, V3* nn);
 }  // namespace orc
 
 using namespace orc;
@@ -361,6 +363,37 @@ void orc_diffuse_render(void* p, int32_t compat, const float* cam10, int32_t W, 
 void orc_offset_ray_origin(const float* p, const float* n, const float* d, float* out) {
   V3 r = offset_ray_origin(v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]), v3(d[0], d[1], d[2]));
   out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+// Camera::generateRay (camera.h:21-31) at film position (x, y) of a W x H
+// image: origin and unit direction.
+void orc_camera_generate(const float* cam10, float x, float y, int32_t W, int32_t H, float* o, float* d) {
+  Camera cam = make_camera(cam10);
+  V3 ro, rd;
+  cam.generate(x, y, W, H, &ro, &rd);
+  o[0] = ro.x; o[1] = ro.y; o[2] = ro.z;
+  d[0] = rd.x; d[1] = rd.y; d[2] = rd.z;
+}
+
+// The diffuse bounce of the path tracer for n samples: normal ng[3n], outgoing
+// wo[3n], draws u[2n]; writes wi[3n] and pdf[n].
+void orc_diffuse_sample(const float* ng, const float* wo, const float* u, int64_t n, float* wi, float* pdf) {
+  for (int64_t i = 0; i < n; ++i) {
+    V3 w;
+    diffuse_sample(v3(ng[3 * i], ng[3 * i + 1], ng[3 * i + 2]), v3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]),
+                   u[2 * i], u[2 * i + 1], &w, &pdf[i]);
+    wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+  }
+}
+
+// CoordinateTransition's axes for n normals: tangent, binormal, normal (9 floats each).
+void orc_shading_frame(const float* n, int64_t count, float* out9) {
+  for (int64_t i = 0; i < count; ++i) {
+    V3 t, b, nn;
+    shading_frame(v3(n[3 * i], n[3 * i + 1], n[3 * i + 2]), &t, &b, &nn);
+    float* q = out9 + 9 * i;
+    q[0] = t.x; q[1] = t.y; q[2] = t.z; q[3] = b.x; q[4] = b.y; q[5] = b.z; q[6] = nn.x; q[7] = nn.y; q[8] = nn.z;
+  }
 }
 
 }  // extern "C"

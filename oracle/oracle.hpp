@@ -63,6 +63,18 @@ static inline V3 vmax(V3 a, V3 b) { return {smax(a.x, b.x), smax(a.y, b.y), smax
 static inline float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 // ------------------------------------------------------ transcendentals ---
+// The polynomial coefficients and evaluation order of flg_logf, flg_sindf and
+// flg_cosdf below come from fdlibm's e_logf.c and FreeBSD msun's k_sinf.c and
+// k_cosf.c (float conversion by Ian Lance Taylor, Cygnus Support; kernels
+// optimised by Bruce D. Evans), distributed under this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunPro, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
+//
 // flg_logf: natural log, fdlibm e_logf.c method (reduce to m in
 // [sqrt(2)/2, sqrt(2)), s = f/(2+f), degree-4 minimax in s^2), evaluated with
 // plain IEEE float operations in a fixed order.  < 1 ulp from the true value.

@@ -73,6 +73,26 @@ static V3 cosine_hemisphere(float u0, float u1) {
   return v3(dx, dy, z);
 }
 
+// The diffuse bounce at a hit with geometric normal ng seen along -wo: the
+// shading frame of ng, a cosine-weighted local direction from (u0, u1), the
+// pdf (DiffuseMaterial::pdf_impl builtin_materials.cpp:38-41: zero unless wo
+// and wi lie on the same side of ng) and wi back in world space.
+void diffuse_sample(V3 ng, V3 wo, float u0, float u1, V3* wi, float* pdf) {
+  Frame fr(ng);
+  V3 wo_l = fr.to_local(wo);
+  V3 wi_l = cosine_hemisphere(u0, u1);
+  *pdf = wo_l.z * wi_l.z > 0 ? std::fabs(wi_l.z) * INV_PI : 0.0f;
+  *wi = fr.to_world(wi_l);
+}
+
+// The shading frame's axes (tangent, binormal, normal) for a normal.
+void shading_frame(V3 n, V3* t, V3* b, V3* nn) {
+  Frame fr(n);
+  *t = fr.t;
+  *b = fr.b;
+  *nn = fr.n;
+}
+
 // One path: returns the number of segments; writes (id, t) per segment.
 int diffuse_path(const Lbvh& b, bool compat, Halton& hs, V3 o, V3 d, int max_depth, int32_t* ids,
                  float* ts, TraceCounters* ctr) {
@@ -86,14 +106,11 @@ int diffuse_path(const Lbvh& b, bool compat, Halton& hs, V3 o, V3 d, int max_dep
     ++nseg;
     if (id < 0 || bounces >= max_depth) break;
     V3 p = o + d * t;
-    Frame fr(ng);
-    V3 wo = -d;
     float u0 = hs.get1D();
     float u1 = hs.get1D();
-    V3 wo_l = fr.to_local(wo);
-    V3 wi_l = cosine_hemisphere(u0, u1);
-    float pdf = wo_l.z * wi_l.z > 0 ? std::fabs(wi_l.z) * INV_PI : 0.0f;
-    V3 wi = fr.to_world(wi_l);
+    V3 wi;
+    float pdf;
+    diffuse_sample(ng, -d, u0, u1, &wi, &pdf);
     if (pdf == 0.0f) break;
     o = offset_ray_origin(p, ng, wi);
     d = normalize(wi);

@@ -40,6 +40,9 @@ _SIGS = {
                                   ctypes.c_int, P, P]),
     "orc_diffuse_render": (None, [P, i32, P, i32, i32, i32, i32, i64, i64, P, P, P, ctypes.c_int, P]),
     "orc_offset_ray_origin": (None, [P, P, P, P]),
+    "orc_camera_generate": (None, [P, f32, f32, i32, i32, P, P]),
+    "orc_diffuse_sample": (None, [P, P, P, i64, P, P]),
+    "orc_shading_frame": (None, [P, i64, P]),
     "orc_volpath_render_film": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P]),
     "orc_volpath_samples": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P]),
     "orc_mitchell1d": (f32, [f32]),

@@ -1,0 +1,318 @@
+"""Pins of the oracle's camera, diffuse bounce and point-light next-event
+estimation against what geometry and integration fix independently of the
+oracle's own code: hand-computed camera rays (src/common/camera.h:21-31), the
+cosine-weighted hemisphere in the shading frame (math_utils.h:69-88,
+material_base.hpp:14-41, builtin_materials.cpp:38-50) by its distribution and
+by chord lengths inside a sphere, and single scattering of a point light in a
+homogeneous box by quadrature (integrator.cpp:69-120, 328-405), with the
+light pick checked by additivity over the lights.  CPU only."""
+import math
+
+import numpy as np
+
+import oracle_lib as orc
+from fledge_amd import api
+
+
+def _gen(cam10, x, y, W, H):
+    c = np.asarray(cam10, np.float32)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    orc.lib().orc_camera_generate(c.ctypes.data, x, y, W, H, o.ctypes.data, d.ctypes.data)
+    return o.astype(np.float64), d.astype(np.float64)
+
+
+def _unit(*v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+# ---------------------------------------------------------------- camera ---
+def test_camera_axes_kat():
+    """Camera at the origin looking +z with up +y and a 90 degree field of view
+    (tan 45 = 1): right = up x forward = +x and the image's up axis =
+    forward x right = +y, so film corner (0, 0) of a 2 x 2 image looks along
+    (-1, -1, 1), corner (2, 0) along (1, -1, 1), and the centre along +z."""
+    cam = [0, 0, 0, 0, 0, 1, 0, 1, 0, 90.0]
+    cases = {(0.0, 0.0): _unit(-1, -1, 1), (2.0, 2.0): _unit(1, 1, 1), (2.0, 0.0): _unit(1, -1, 1),
+             (0.0, 2.0): _unit(-1, 1, 1), (1.0, 1.0): _unit(0, 0, 1)}
+    for (x, y), want in cases.items():
+        o, d = _gen(cam, x, y, 2, 2)
+        assert np.array_equal(o, [0, 0, 0])
+        assert np.abs(d - want).max() < 2e-7, ((x, y), d, want)
+    o, d = _gen(cam, 1.0, 1.0, 2, 2)
+    assert np.array_equal(d, [0.0, 0.0, 1.0])  # exactly forward
+
+
+def test_camera_aspect_scales_the_horizontal_axis_only():
+    """A 4 x 2 image (aspect 2): the horizontal offset is tan(fov/2) * aspect,
+    the vertical one tan(fov/2)."""
+    cam = [0, 0, 0, 0, 0, 1, 0, 1, 0, 90.0]
+    for (x, y), want in {(0.0, 1.0): _unit(-2, 0, 1), (4.0, 1.0): _unit(2, 0, 1),
+                         (2.0, 0.0): _unit(0, -1, 1), (1.0, 2.0): _unit(-1, 1, 1)}.items():
+        _, d = _gen(cam, x, y, 4, 2)
+        assert np.abs(d - want).max() < 2e-7, ((x, y), d, want)
+
+
+def test_camera_rotated_and_general_fov():
+    """Looking along -x from (1, 2, 3) with up +y: right = up x forward = +z,
+    the image's up axis = +y.  Then a 60 degree camera at an arbitrary film
+    point against the pinhole model computed in float64."""
+    cam = [1, 2, 3, 0, 2, 3, 0, 1, 0, 90.0]
+    o, d = _gen(cam, 0.0, 0.0, 2, 2)
+    assert np.array_equal(o, [1, 2, 3])
+    assert np.abs(d - _unit(-1, -1, -1)).max() < 2e-7
+    o, d = _gen(cam, 2.0, 1.0, 2, 2)
+    assert np.abs(d - _unit(-1, 0, 1)).max() < 2e-7
+    r = math.tan(math.radians(30.0))
+    cam = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 60.0]
+    for x, y, W, H in [(1.25, 6.5, 8, 8), (13.75, 2.0, 16, 4), (0.3, 0.9, 3, 7)]:
+        xo = r * (W / H) * (2 * x / W - 1)
+        yo = r * (2 * y / H - 1)
+        want = _unit(xo, yo, 1.0)
+        _, d = _gen(cam, x, y, W, H)
+        assert np.abs(d - want).max() < 6e-7, ((x, y, W, H), d, want)
+
+
+def test_camera_rays_are_the_jittered_pinhole_rays():
+    """orc_camera_rays = the pinhole ray at (pixel + Halton dims 0, 1)."""
+    cam = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+    W, H = 16, 8
+    rays = orc.camera_rays(cam, W, H, 4, 3)
+    for pix in (0, 5, 77, W * H - 1):
+        x, y = pix % W, pix // W
+        u = orc.halton_get(W, H, 4, x, y, 3, 2)
+        _, d = _gen(cam, np.float32(x + u[0]), np.float32(y + u[1]), W, H)
+        assert np.array_equal(rays[pix, 4:7], d.astype(np.float32))
+
+
+# ------------------------------------------------------- diffuse bounce ---
+def _diffuse(ng, wo, u):
+    ng = np.ascontiguousarray(ng, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    n = u.shape[0]
+    wi = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    orc.lib().orc_diffuse_sample(ng.ctypes.data, wo.ctypes.data, u.ctypes.data, n, wi.ctypes.data,
+                                 pdf.ctypes.data)
+    return wi.astype(np.float64), pdf.astype(np.float64)
+
+
+def _frames(n):
+    n = np.ascontiguousarray(n, np.float32).reshape(-1, 3)
+    out = np.zeros((n.shape[0], 9), np.float32)
+    orc.lib().orc_shading_frame(n.ctypes.data, n.shape[0], out.ctypes.data)
+    return out.reshape(-1, 3, 3).astype(np.float64)
+
+
+def test_shading_frame_orthonormal_right_handed():
+    """CoordinateTransition: (tangent, binormal, normal) orthonormal,
+    tangent x binormal = normal, the normal the input normalised; and the
+    hand values for +z (t = -x, b = -y) and +x (t = -z, b = +y)."""
+    rng = np.random.default_rng(3)
+    n = rng.normal(size=(2000, 3))
+    n[:10] = [[0, 0, 1], [1, 0, 0], [0, 1, 0], [0, 0, -1], [-1, 0, 0], [0, -1, 0],
+              [1, 1, 0], [0, 1, 1], [1, 0, 1], [3, -4, 0.5]]
+    f = _frames(n)
+    t, b, nn = f[:, 0], f[:, 1], f[:, 2]
+    for a in (t, b, nn):
+        assert np.abs(np.linalg.norm(a, axis=1) - 1).max() < 1e-6
+    assert np.abs((t * b).sum(1)).max() < 1e-6
+    assert np.abs((t * nn).sum(1)).max() < 1e-6
+    assert np.abs((b * nn).sum(1)).max() < 1e-6
+    assert np.abs(np.cross(t, b) - nn).max() < 2e-6
+    nu = n / np.linalg.norm(n, axis=1, keepdims=True)
+    assert np.abs(nn - nu).max() < 1e-6
+    assert np.array_equal(f[0], [[-1, 0, 0], [0, -1, 0], [0, 0, 1]])
+    assert np.array_equal(f[1], [[0, 0, -1], [0, 1, 0], [1, 0, 0]])
+
+
+def test_diffuse_bounce_kats():
+    """(0.5, 0.5) is the disk centre: wi = the normal.  (1, 0.5) is the disk
+    point (1, 0), the tangent; (0.5, 1) the point (0, 1), the binormal."""
+    ng = np.array([[0, 0, 1]] * 3, np.float32)
+    wo = np.array([[0, 0, 1]] * 3, np.float32)
+    u = np.array([[0.5, 0.5], [1.0, 0.5], [0.5, 1.0]], np.float32)
+    wi, pdf = _diffuse(ng, wo, u)
+    assert np.array_equal(wi[0], [0, 0, 1]) and pdf[0] == np.float32(1 / math.pi)
+    assert np.abs(wi[1] - [-1, 0, 0]).max() < 1e-7 and pdf[1] == 0  # grazing: cos 0
+    assert np.abs(wi[2] - [0, -1, 0]).max() < 1e-7
+
+
+def _chi2_uniform(x, bins):
+    h, _ = np.histogram(x, bins=bins, range=(0.0, 1.0))
+    e = len(x) / bins
+    return float(((h - e) ** 2 / e).sum())
+
+
+def test_diffuse_bounce_is_cosine_weighted_about_the_normal():
+    """For several normals and outgoing directions on the normal's side: wi is
+    a unit vector in the normal's hemisphere whose cos(theta) has density
+    2 cos(theta) (cos^2 uniform, mean 2/3) and whose azimuth about the normal,
+    measured in a basis built here, is uniform; pdf = cos(theta) / pi.  An
+    outgoing direction on the far side gives pdf 0."""
+    rng = np.random.default_rng(11)
+    m = 100_000
+    normals = [(0, 0, 1), (1, 0, 0), (0, -1, 0), (0.3, -0.8, 0.52), (-2.0, 0.5, -1.0)]
+    for nrm in normals:
+        n = _unit(*nrm)
+        u = rng.random((m, 2), dtype=np.float32)
+        wo = _unit(*(n + 0.4 * rng.normal(size=3)))
+        if wo @ n <= 0.05:
+            wo = n
+        wi, pdf = _diffuse(np.tile(np.float32(nrm), (m, 1)), np.tile(wo, (m, 1)), u)
+        assert np.abs(np.linalg.norm(wi, axis=1) - 1).max() < 2e-6
+        c = wi @ n
+        assert c.min() > -1e-6
+        assert abs(c.mean() - 2 / 3) < 5 * math.sqrt(1 / 18 / m)
+        assert _chi2_uniform(np.clip(c * c, 0, 1 - 1e-12), 32) < 70  # chi2(31): p ~ 1e-4
+        a = np.array([1.0, 0, 0]) if abs(n[0]) < 0.9 else np.array([0, 1.0, 0])
+        e1 = np.cross(n, a)
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(n, e1)
+        phi = (np.arctan2(wi @ e2, wi @ e1) / (2 * math.pi)) % 1.0
+        assert _chi2_uniform(phi, 32) < 70
+        ok = pdf > 0
+        assert ok.mean() > 0.999
+        assert np.abs(pdf[ok] - c[ok] / math.pi).max() < 2e-6
+        _, pdf_back = _diffuse(np.tile(np.float32(nrm), (1000, 1)), np.tile(-wo, (1000, 1)), u[:1000])
+        assert not pdf_back.any()
+
+
+def test_diffuse_paths_inside_a_sphere_have_cosine_chords():
+    """End to end through the oracle's path tracer: inside a unit sphere wound
+    inward, a ray leaving the surface at angle theta to the inward normal
+    travels the chord 2 cos(theta) before its next hit.  So every segment after
+    the first has t / 2 distributed with density 2x (t^2 / 4 uniform, mean 2/3),
+    and no path leaves the sphere before its depth limit."""
+    v, ind = api.gen_octasphere(6, 0.0)
+    tri = ind.reshape(-1, 3)[:, [0, 2, 1]].copy().ravel()  # inward winding
+    b = orc.Lbvh(v, tri)
+    W = H = 64
+    spp, depth = 4, 4
+    cam = [0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 60.0]
+    nseg, ids, ts, _ = b.diffuse_render(cam, W, H, spp, depth, 0, W * H * spp)
+    assert (nseg == depth + 1).all()
+    assert (ids >= 0).all()
+    assert np.abs(ts[:, 0] - 1).max() < 2e-3  # from the centre: the radius
+    x = ts[:, 1:].ravel().astype(np.float64) / 2
+    assert x.max() < 1.0 + 1e-3
+    n = x.size
+    assert abs(x.mean() - 2 / 3) < 5 * math.sqrt(1 / 18 / n) + 2e-4
+    assert _chi2_uniform(np.clip(x * x, 0, 1 - 1e-12), 32) < 70
+
+
+# ------------------------------------------------- point-light NEE, pick ---
+CAM0 = [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0]
+LIGHT_P = np.array([2.0, 2.0, -2.0])
+LIGHT_I = 50.0
+
+
+def _ramp(u, n):
+    """Trilinear density of an n^3 grid of ones along one axis of the unit
+    box: index u n - 0.5, zero outside, so it rises from 1/2 at a face to 1
+    half a voxel inside."""
+    f = np.minimum(np.minimum(u * n + 0.5, n - u * n + 0.5), 1.0)
+    return np.where((u >= 0) & (u <= 1), f, 0.0)
+
+
+def _rho(p, n):
+    return _ramp(p[..., 0], n) * _ramp(p[..., 1], n) * _ramp(p[..., 2], n)
+
+
+def _box_span(o, d):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t0, t1 = (0.0 - o) / d, (1.0 - o) / d
+    tn = np.maximum(np.nanmax(np.minimum(t0, t1), axis=-1), 0.0)
+    tf = np.nanmin(np.maximum(t0, t1), axis=-1)
+    return tn, np.maximum(tf, tn)
+
+
+def _single_scatter(rays, n, sigma_s, sigma_t, g, M=96, M2=32):
+    """E[L] of one scattering event lit by the point light, per camera ray:
+    int T(0,t) sigma_s rho(x) p_HG(theta) I / |x - l|^2 T(x, l) dt, midpoint
+    quadrature in float64.  theta is the scattering angle between the light's
+    propagation direction (l to x) and the direction back to the camera; the
+    Henyey-Greenstein density is the textbook (1 - g^2) / (4 pi (1 + g^2 -
+    2 g cos theta)^1.5)."""
+    o, d = rays[:, :3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+    tn, tf = _box_span(o, d)
+    h = (tf - tn) / M
+    t = tn[:, None] + h[:, None] * (np.arange(M) + 0.5)[None, :]
+    x = o[:, None, :] + d[:, None, :] * t[:, :, None]  # [R, M, 3]
+    rho = _rho(x, n)
+    tau = sigma_t * h[:, None] * (np.cumsum(rho, axis=1) - 0.5 * rho)
+    to_l = LIGHT_P[None, None, :] - x
+    dist = np.linalg.norm(to_l, axis=-1)
+    a = to_l / dist[..., None]
+    _, sf = _box_span(x.reshape(-1, 3), a.reshape(-1, 3))
+    sf = sf.reshape(dist.shape)
+    h2 = sf / M2
+    s = h2[..., None] * (np.arange(M2) + 0.5)
+    y = x[:, :, None, :] + a[:, :, None, :] * s[..., None]
+    tau_l = sigma_t * h2 * _rho(y, n).sum(-1)
+    cos_t = (-a * -d[:, None, :]).sum(-1)
+    p = (1 - g * g) / (4 * math.pi * (1 + g * g - 2 * g * cos_t) ** 1.5)
+    f = np.exp(-tau) * sigma_s * rho * p * LIGHT_I / dist ** 2 * np.exp(-tau_l)
+    return (f * h[:, None]).sum(1)
+
+
+def test_constant_grid_density_model():
+    """The separable ramp used by the quadrature equals the trilinear lookup of
+    a grid of ones (zero outside the grid) computed corner by corner."""
+    from test_oracle_cpu import _trilinear
+    n = 8
+    p = np.random.default_rng(2).random((4000, 3)) * 1.2 - 0.1
+    grid = np.ones((n, n, n))
+    inside = ((p >= 0) & (p <= 1)).all(1)
+    assert np.abs(_rho(p[inside], n) - _trilinear(grid, p[inside])).max() < 1e-12
+
+
+def _per_sample(lights, W, H, spp, medium, n):
+    grid = np.ones((n, n, n), np.float32)
+    L, _ = orc.volpath_samples(grid, [0, 0, 0, 1, 1, 1], medium, lights, CAM0, W, H, spp, 1)
+    return L.astype(np.float64)
+
+
+def test_point_light_single_scatter_matches_quadrature():
+    """Depth 1, environment off, a homogeneous box (sigma_s 1.5, sigma_a 0.5,
+    g 0.3): the mean radiance over the image equals the single-scattering
+    integral of the point light over the same camera rays.  A dropped albedo
+    (25%), 1/d in place of 1/d^2, or a flipped phase angle (the light is
+    behind the camera's side: g > 0 then favours it 5-fold) all fail."""
+    W, H, spp, n = 4, 4, 1024, 8
+    medium = [1.5, 0.5, 0.3]
+    L = _per_sample([1, 2, 2, -2, LIGHT_I, 0, 0], W, H, spp, medium, n)
+    want = np.zeros(W * H)
+    for s in range(spp):
+        want += _single_scatter(orc.camera_rays(CAM0, W, H, spp, s), n, 1.5, 2.0, 0.3)
+    want /= spp
+    got = L.mean()
+    sd = L.std() / math.sqrt(L.size)
+    assert got > 0.02
+    assert abs(got - want.mean()) < 4 * sd + 1e-3 * want.mean(), (got, want.mean(), sd)
+    # per pixel, looser (fewer samples each)
+    pix = L.mean(axis=1)
+    psd = L.std(axis=1) / math.sqrt(spp)
+    assert (np.abs(pix - want) < 5 * psd + 1e-3).all()
+
+
+def test_light_pick_is_additive_over_lights():
+    """One light chosen uniformly and divided by its probability 1/2: in
+    expectation the image lit by both lights equals the image lit by the point
+    light alone plus the image lit by the environment alone (the environment's
+    unscattered term counted once).  A missing 1/pdf halves the scattered
+    part and fails."""
+    W, H, spp, n = 4, 4, 2048, 8
+    medium = [1.5, 0.5, 0.3]
+    both = _per_sample([1, 2, 2, -2, LIGHT_I, 1, 0.5], W, H, spp, medium, n)
+    pt = _per_sample([1, 2, 2, -2, LIGHT_I, 0, 0], W, H, spp, medium, n)
+    env = _per_sample([0, 0, 0, 0, 0, 1, 0.5], W, H, spp, medium, n)
+    # the environment's unscattered term (bounce 0) is the part no pick touches
+    env0 = _per_sample([0, 0, 0, 0, 0, 1, 0.5], W, H, spp, [0.0, 2.0, 0.3], n)
+    sd = math.sqrt(both.var() / both.size + pt.var() / pt.size + env.var() / env.size)
+    lhs, rhs = both.mean(), pt.mean() + env.mean()
+    assert abs(lhs - rhs) < 4 * sd, (lhs, rhs, sd)
+    scattered = pt.mean() + env.mean() - env0.mean()
+    assert scattered > 12 * sd  # a halved scattered part would be far outside
