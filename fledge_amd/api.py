@@ -20,6 +20,47 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def check_buffer(t, name, dtypes, device, numel=None, min_numel=None, ndim=None, shape=None):
+    """Raises ValueError unless `t` is a contiguous tensor of one of `dtypes`
+    on CUDA device `device` (None: checked later by check_device) holding exactly `numel` (or at least `min_numel`)
+    elements, with `ndim` dimensions or the given `shape` (None: any extent): the kernels behind
+    the C ABI index these buffers by the sizes passed beside them."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s: expected a torch tensor, got %s" % (name, type(t).__name__))
+    if isinstance(dtypes, torch.dtype):
+        dtypes = (dtypes,)
+    if t.dtype not in dtypes:
+        raise ValueError("%s: dtype %s, expected %s" % (name, t.dtype, " or ".join(str(d) for d in dtypes)))
+    if not t.is_contiguous():
+        raise ValueError("%s: must be contiguous (got strides %s for shape %s)"
+                         % (name, tuple(t.stride()), tuple(t.shape)))
+    if ndim is not None and t.dim() != ndim:
+        raise ValueError("%s: %d dimensions, expected %d" % (name, t.dim(), ndim))
+    if shape is not None and (t.dim() != len(shape) or
+                              any(w is not None and w != g for g, w in zip(t.shape, shape))):
+        raise ValueError("%s: shape %s, expected %s" % (name, tuple(t.shape),
+                                                        tuple("n" if w is None else w for w in shape)))
+    if numel is not None and t.numel() != numel:
+        raise ValueError("%s: %d elements, expected %d" % (name, t.numel(), numel))
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError("%s: %d elements, at least %d needed" % (name, t.numel(), min_numel))
+    if device is not None:
+        check_device(device, **{name: t})
+    return t
+
+
+def check_device(device, **bufs):
+    """Raises ValueError unless every tensor given (None skipped) lives on
+    CUDA device `device`.  Wrappers call it after the shape checks of all
+    their buffers."""
+    for name, t in bufs.items():
+        if t is not None and (t.device.type != "cuda" or (t.device.index or 0) != device):
+            raise ValueError("%s: on %s, expected cuda:%d" % (name, t.device, device))
+
+
+_HIT_TYPES = (torch.int32, torch.float32)
+
+
 class Context:
     def __init__(self, device=0, stream=None):
         self.lib = N.hip()
@@ -71,24 +112,55 @@ class LBVH:
         self.ntris = ntris
         self._keep = keep
 
+    @staticmethod
+    def _check_mesh(ctx, verts, ind, k=""):
+        check_buffer(verts, "verts" + k, torch.float32, None)
+        check_buffer(ind, "ind" + k, torch.int32, None)
+        if verts.numel() % 3 or ind.numel() % 3:
+            raise ValueError("verts%s / ind%s: sizes must be multiples of 3 (got %d, %d)"
+                             % (k, k, verts.numel(), ind.numel()))
+        if ind.numel() and verts.numel() == 0:
+            raise ValueError("ind%s: indices into an empty vertex array" % k)
+
+    @staticmethod
+    def _check_indices(meshes):
+        """Every index in [0, nverts): one reduction per mesh, then one sync."""
+        bad = []
+        for k, (v, ind) in enumerate(meshes):
+            if ind.numel():
+                lo, hi = torch.aminmax(ind)
+                bad.append((k, lo, hi, v.numel() // 3))
+        for k, lo, hi, nv in bad:
+            if int(lo) < 0 or int(hi) >= nv:
+                raise ValueError("mesh %d: vertex indices in [%d, %d] outside [0, %d)" % (k, int(lo), int(hi), nv))
+
     @classmethod
-    def build(cls, ctx, verts, ind, compat=False):
-        verts = verts.contiguous()
-        ind = ind.contiguous()
-        assert verts.dtype == torch.float32 and ind.dtype == torch.int32 and verts.is_cuda
+    def build(cls, ctx, verts, ind, compat=False, check_indices=True):
+        """check_indices: reject indices outside the vertex array first (one
+        device reduction and a synchronisation)."""
+        cls._check_mesh(ctx, verts, ind)
+        check_device(ctx.device, verts=verts, ind=ind)
+        if check_indices:
+            cls._check_indices([(verts, ind)])
         h = ctypes.c_void_p()
         ctx.check(ctx.lib.flg_lbvh_build(ctx.h, _ptr(verts), _ptr(ind), ind.numel() // 3,
                                          verts.numel() // 3, 1 if compat else 0, ctypes.byref(h)))
         return cls(ctx, h, ind.numel() // 3, keep=(verts, ind))
 
     @classmethod
-    def build_batched(cls, ctx, meshes, compat=False):
+    def build_batched(cls, ctx, meshes, compat=False, check_indices=True):
         n = len(meshes)
+        if n == 0:
+            raise ValueError("build_batched: no meshes")
+        for i, (v, ind) in enumerate(meshes):
+            cls._check_mesh(ctx, v, ind, "[%d]" % i)
+        for i, (v, ind) in enumerate(meshes):
+            check_device(ctx.device, **{"verts[%d]" % i: v, "ind[%d]" % i: ind})
+        if check_indices:
+            cls._check_indices(meshes)
         descs = (N.MeshDesc * n)()
         keep = []
         for i, (v, ind) in enumerate(meshes):
-            v = v.contiguous()
-            ind = ind.contiguous()
             keep += [v, ind]
             descs[i].d_verts = v.data_ptr()
             descs[i].d_ind = ind.data_ptr()
@@ -122,9 +194,15 @@ class LBVH:
         return arrs
 
     def trace(self, rays, hits=None, block_size=0, stack_mode=0):
+        """rays: (n, 8) float32 (o, tmax, d, pad); hits: (n, 2) int32 or
+        float32 (id, t bits)."""
+        dev = self.ctx.device
+        check_buffer(rays, "rays", torch.float32, None, shape=(None, 8))
         n = rays.shape[0]
         if hits is None:
             hits = torch.empty((n, 2), dtype=torch.int32, device=rays.device)
+        check_buffer(hits, "hits", _HIT_TYPES, None, min_numel=2 * n)
+        check_device(dev, rays=rays, hits=hits)
         cfg = N.TraceCfg()
         cfg.block_size = block_size
         cfg.stack_mode = stack_mode
@@ -133,6 +211,8 @@ class LBVH:
         return hits
 
     def count(self, rays):
+        check_buffer(rays, "rays", torch.float32, None, shape=(None, 8))
+        check_device(self.ctx.device, rays=rays)
         c = torch.zeros(2, dtype=torch.int64, device=rays.device)
         self.ctx.check(self.ctx.lib.flg_trace_count(self.ctx.h, self.h, _ptr(rays), rays.shape[0], _ptr(c)))
         return c
@@ -156,6 +236,14 @@ class LBVH:
         nseg = seg_id = seg_t = None
         if out is not None:
             nseg, seg_id, seg_t = out
+            S = max_depth + 1
+            if nseg is not None:
+                check_buffer(nseg, "nseg", torch.int32, None, min_numel=npaths)
+            if seg_id is not None:
+                check_buffer(seg_id, "seg_id", torch.int32, None, min_numel=npaths * S)
+            if seg_t is not None:
+                check_buffer(seg_t, "seg_t", torch.float32, None, min_numel=npaths * S)
+            check_device(self.ctx.device, nseg=nseg, seg_id=seg_id, seg_t=seg_t)
         elif record:
             nseg = torch.zeros(npaths, dtype=torch.int32, device=dev)
             seg_id = torch.full((npaths, max_depth + 1), -2, dtype=torch.int32, device=dev)
@@ -180,6 +268,9 @@ class LBVH:
                 "ms_default": t.ms_default, "candidates": t.candidates}
 
     def autotune_trace(self, rays, hits):
+        check_buffer(rays, "rays", torch.float32, None, shape=(None, 8))
+        check_buffer(hits, "hits", _HIT_TYPES, None, min_numel=2 * rays.shape[0])
+        check_device(self.ctx.device, rays=rays, hits=hits)
         t = N.Tuning()
         self.ctx.check(self.ctx.lib.flg_autotune_trace(self.ctx.h, self.h, _ptr(rays), _ptr(hits),
                                                        rays.shape[0], ctypes.byref(t)))
@@ -236,9 +327,13 @@ class SceneAccel:
     def trace(self, rays, hits=None):
         """rays: (n, 8) float32 CUDA tensor; hits: (n, 4) int32 (mesh, id,
         t bits, 0)."""
+        dev = self.ctx.device
+        check_buffer(rays, "rays", torch.float32, None, shape=(None, 8))
         n = rays.shape[0]
         if hits is None:
             hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        check_buffer(hits, "hits", _HIT_TYPES, None, min_numel=4 * n)
+        check_device(dev, rays=rays, hits=hits)
         self.ctx.check(self.ctx.lib.flg_scene_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n))
         return hits
 
@@ -257,6 +352,7 @@ class SceneAccel:
 def camera_rays(ctx, cam, width, height, spp=1, sample=0, out=None):
     if out is None:
         out = torch.empty((width * height, 8), dtype=torch.float32, device=torch.device("cuda", ctx.device))
+    check_buffer(out, "out", torch.float32, ctx.device, min_numel=width * height * 8)
     ctx.check(ctx.lib.flg_camera_rays(ctx.h, ctypes.byref(cam), width, height, spp, sample, _ptr(out)))
     return out
 
@@ -269,8 +365,7 @@ def hits_as_numpy(hits):
 class Volume:
     def __init__(self, ctx, density, bounds, sigma_s, sigma_a, g):
         self.ctx = ctx
-        density = density.contiguous()
-        assert density.dtype == torch.float32 and density.dim() == 3 and density.is_cuda
+        check_buffer(density, "density", torch.float32, ctx.device, ndim=3)
         nz, ny, nx = density.shape
         self.shape = (nz, ny, nx)
         med = N.Medium(sigma_s, sigma_a, g)
@@ -280,7 +375,7 @@ class Volume:
                                             ctypes.byref(self.h)))
 
     def update(self, density):
-        density = density.contiguous()
+        check_buffer(density, "density", torch.float32, self.ctx.device, shape=self.shape)
         self.ctx.check(self.ctx.lib.flg_volume_update(self.ctx.h, self.h, _ptr(density)))
 
     def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0):
@@ -292,6 +387,7 @@ class Volume:
         cfg.film = film
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
+        check_buffer(out, "out", torch.float32, self.ctx.device, min_numel=height * width * 3)
         st = (ctypes.c_uint64 * 2)()
         self.ctx.check(self.ctx.lib.flg_volpath_render(self.ctx.h, self.h, ctypes.byref(scene),
                                                        ctypes.byref(cfg), _ptr(out), st if stats else None))
@@ -309,6 +405,7 @@ class Volume:
         cfg.film = film
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
+        check_buffer(out, "out", torch.float32, self.ctx.device, min_numel=height * width * 3)
         st = (ctypes.c_uint64 * 2)()
         self.ctx.check(self.ctx.lib.flg_volpath_mesh_render(self.ctx.h, bvh.h, self.h, ctypes.byref(scene),
                                                             ctypes.byref(cfg), _ptr(out), st if stats else None))

@@ -87,6 +87,8 @@ ScopedTimer::~ScopedTimer() {
 }
 
 // ------------------------------------------------------------ debug eval ---
+// Test hook (flg_debug_eval): left out of builds with FLG_NO_TEST_HOOKS.
+#ifndef FLG_NO_TEST_HOOKS
 __global__ void debug_eval_kernel(int kind, const void* in, void* out, int64_t n,
                                   HaltonTables tab) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -105,6 +107,7 @@ __global__ void debug_eval_kernel(int kind, const void* in, void* out, int64_t n
     ((float*)out)[i] = walk_draw(walk_key(p[3 * i], p[3 * i + 1]), p[3 * i + 2]);
   }
 }
+#endif
 
 }  // namespace flg
 
@@ -239,6 +242,7 @@ int flg_memcpy_dtoh(flg_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) 
   });
 }
 
+#ifndef FLG_NO_TEST_HOOKS
 int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n) {
   return host_guard(ctx, "flg_debug_eval", [&]() -> int {
     if (!ctx || kind < 0 || kind > 4 || n < 0 || (n && (!d_in || !d_out)))
@@ -252,5 +256,7 @@ int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, in
     return FLG_OK;
   });
 }
+
+#endif
 
 }  // extern "C"

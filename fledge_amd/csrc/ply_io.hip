@@ -279,8 +279,14 @@ int load(const char* path, flg_ply_mesh* out) {
         int64_t cnt = 0;
         if (!R.read(pr.count_type).as_int(&cnt) || cnt < 0) return FLG_ERR_INVALID;
         if (is_f && (int)k == ifl && cnt != 3) return FLG_ERR_INVALID;
+        // a list item takes at least one byte in ascii and its type's size
+        // in binary: a count the rest of the body cannot hold is an error
+        // before the items are read
+        const int64_t item = fmt == kAscii ? 1 : type_size(pr.type);
+        if (cnt > (int64_t)(R.end - R.p) / item) return FLG_ERR_INVALID;
         for (int64_t c = 0; c < cnt; ++c) {
           const Value v = R.read(pr.type);
+          if (!R.ok) return FLG_ERR_INVALID;
           if (is_f && (int)k == ifl) {
             int64_t idx = 0;
             if (!v.as_int(&idx) || idx < 0 || idx >= ve->count) return FLG_ERR_INVALID;

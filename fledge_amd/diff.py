@@ -10,7 +10,25 @@ import ctypes
 import torch
 
 from . import _native as N
-from .api import Volume, _ptr
+from .api import Volume, _ptr, check_buffer, check_device
+
+
+def _check_forward(ctx, w, h, spp, depth, out, lpath, tape):
+    check_buffer(out, "out", torch.float32, None, min_numel=w * h * 3)
+    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp)
+    if tape is not None:
+        check_buffer(tape, "tape", torch.uint8, None, min_numel=tape_bytes(ctx, w, h, spp, depth))
+    check_device(ctx.device, out=out, lpath=lpath, tape=tape)
+
+
+def _check_adjoint(ctx, w, h, spp, depth, lpath, dl, grad, grad_numel, galpha, tape):
+    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp)
+    check_buffer(dl, "dl_drgb", torch.float32, None, numel=w * h * 3)
+    check_buffer(grad, "grad", torch.float32, None, numel=grad_numel)
+    check_buffer(galpha, "galpha", torch.float32, None, min_numel=1)
+    if tape is not None:
+        check_buffer(tape, "tape", torch.uint8, None, min_numel=tape_bytes(ctx, w, h, spp, depth))
+    check_device(ctx.device, lpath=lpath, dl_drgb=dl, grad=grad, galpha=galpha, tape=tape)
 
 
 class DiffVolume:
@@ -74,8 +92,7 @@ def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=No
         out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
     if lpath is None:
         lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
-    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
-        raise ValueError("walk tape too small")
+    _check_forward(ctx, w, h, spp, depth, out, lpath, tape)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_diff_forward(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                                ctypes.byref(_cfg(w, h, spp, depth, schedule)),
@@ -96,10 +113,9 @@ def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, gr
         grad = torch.zeros((nz, ny, nx), dtype=torch.float32, device=dev)
     if galpha is None:
         galpha = torch.zeros(1, dtype=torch.float32, device=dev)
-    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
-        raise ValueError("walk tape too small")
-    prop = dv.proposal.h if dv.proposal is not None else None
     dl = dl_drgb.contiguous()
+    _check_adjoint(ctx, w, h, spp, depth, lpath, dl, grad, nz * ny * nx, galpha, tape)
+    prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_adjoint(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                           ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                           ctypes.byref(_params(s, albedo, dv.g, majorant)),
@@ -114,6 +130,7 @@ class BrickLayout:
     slot order."""
 
     def __init__(self, ctx, density, brick=8):
+        check_buffer(density, "density", torch.float32, ctx.device, ndim=3)
         nz, ny, nx = density.shape
         self.ctx = ctx
         self.brick = brick
@@ -121,7 +138,7 @@ class BrickLayout:
         self.nb = tuple((n + brick - 1) // brick for n in (nz, ny, nx))
         self.slot = torch.empty(self.nb, dtype=torch.int32, device=torch.device("cuda", ctx.device))
         n = ctypes.c_int32(0)
-        ctx.check(ctx.lib.flg_bricks_build(ctx.h, _ptr(density.contiguous()), nx, ny, nz, brick, _ptr(self.slot),
+        ctx.check(ctx.lib.flg_bricks_build(ctx.h, _ptr(density), nx, ny, nz, brick, _ptr(self.slot),
                                            ctypes.byref(n)))
         self.n_active = int(n.value)
         self.c = N.BrickLayout()
@@ -159,8 +176,7 @@ def mesh_forward(dv, bvh, scene, w, h, spp, depth, s, albedo, majorant, out=None
         out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
     if lpath is None:
         lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
-    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
-        raise ValueError("walk tape too small")
+    _check_forward(ctx, w, h, spp, depth, out, lpath, tape)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_mesh_diff_forward(ctx.h, bvh.h, dv.vol.h, prop, ctypes.byref(scene),
                                                     ctypes.byref(_cfg(w, h, spp, depth, schedule)),
@@ -180,10 +196,11 @@ def mesh_adjoint(dv, bvh, scene, w, h, spp, depth, s, albedo, majorant, lpath, d
         grad = bricks.zeros() if bricks is not None else torch.zeros(dv.vol.shape, dtype=torch.float32, device=dev)
     if galpha is None:
         galpha = torch.zeros(1, dtype=torch.float32, device=dev)
-    if tape is not None and tape.numel() < tape_bytes(ctx, w, h, spp, depth):
-        raise ValueError("walk tape too small")
-    prop = dv.proposal.h if dv.proposal is not None else None
     dl = dl_drgb.contiguous()
+    nz, ny, nx = dv.vol.shape
+    gnum = bricks.n_active * bricks.brick ** 3 if bricks is not None else nz * ny * nx
+    _check_adjoint(ctx, w, h, spp, depth, lpath, dl, grad, gnum, galpha, tape)
+    prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_mesh_adjoint(ctx.h, bvh.h, dv.vol.h, prop, ctypes.byref(scene),
                                                ctypes.byref(_cfg(w, h, spp, depth, schedule)),
                                                ctypes.byref(_params(s, albedo, dv.g, majorant)), _ptr(lpath),

@@ -466,11 +466,16 @@ FLG_API int flg_volpath_mesh_adjoint(flg_ctx* ctx, const flg_bvh* bvh, flg_volum
 FLG_API int flg_film_save_exr(const char* path, int32_t width, int32_t height, const float* h_rgb);
 
 /* ------------------------------------------------------------- checks --- */
-/* Device copies of the path's scalar helpers, for bit-exactness tests:
+/* TEST HOOK, not part of the rendering interface and with no counterpart in
+ * the reference: evaluates the device copies of the path's scalar helpers so
+ * that tests can compare them bit for bit with the CPU restatement.
  * kind 0: flg_logf(x); 1: sin; 2: cos; 3: Halton get1D with x = (index, dim)
  * packed as two uint32 in the input; 4: the ratio-tracking walks' draw
- * (1 - u) with x = (path, walk, n) packed as three uint32. */
+ * (1 - u) with x = (path, walk, n) packed as three uint32.  A build with
+ * -DFLG_NO_TEST_HOOKS leaves it out of the library. */
+#ifndef FLG_NO_TEST_HOOKS
 FLG_API int flg_debug_eval(flg_ctx* ctx, int32_t kind, const void* d_in, void* d_out, int64_t n);
+#endif
 
 #ifdef __cplusplus
 }

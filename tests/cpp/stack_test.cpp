@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../fledge_amd/csrc/traverse.hpp"
+#include "../../fledge_amd/csrc/volume.hpp"
 #include "fledge_scene.h"
 #include "oracle_impl.hpp"
 
@@ -121,7 +122,57 @@ void run_case(const char* name, const std::vector<float>& verts, const std::vect
 
 }  // namespace
 
+// tile_order (the mesh kernels' path start order) is a bijection on
+// [0, w h tasks): exhaustively on small ragged shapes; at widths of 2^27 and
+// above (h = 1: the identity; h = 8: the tile layout checked by hand).
+void check_tile_order() {
+  const int shapes[][3] = {{1, 1, 1}, {17, 33, 3}, {16, 16, 2}, {40, 7, 5}, {3, 100, 1}, {1920, 24, 2}};
+  for (const auto& s : shapes) {
+    const uint32_t n = (uint32_t)s[0] * s[1] * s[2];
+    std::vector<uint8_t> seen(n, 0);
+    bool ok = true;
+    for (uint32_t q = 0; q < n; ++q) {
+      const uint32_t p = flg::tile_order(q, s[2], s[0], s[1]);
+      if (p >= n || seen[p]) ok = false;
+      else seen[p] = 1;
+    }
+    if (!ok) {
+      std::printf("FAIL tile_order not a bijection at %d x %d x %d\n", s[0], s[1], s[2]);
+      ++failures;
+    }
+  }
+  std::mt19937 gen(5);
+  for (int32_t w : {(1 << 28) + 3, (1 << 30) + 1, 2147483647}) {
+    std::uniform_int_distribution<uint32_t> U(0, (uint32_t)w - 1);
+    for (int k = 0; k < 100000; ++k) {
+      const uint32_t q = U(gen);
+      if (flg::tile_order(q, 1, w, 1) != q) {
+        std::printf("FAIL tile_order at width %d: %u -> %u\n", w, q, flg::tile_order(q, 1, w, 1));
+        ++failures;
+        break;
+      }
+    }
+  }
+  {  // w = 2^27 + 5, h = 8, 1 task: one band of 8 rows; tile c holds pixels
+     // (16 c + i % 16, i / 16) for i < 128, the last tile 5 wide
+    const int32_t w = (1 << 27) + 5, h = 8;
+    const uint32_t last = (uint32_t)w / 16;  // the narrow tile
+    const uint32_t cases[][3] = {{0, 0, 0}, {17, 1, 1}, {127, 15, 7}, {128 * 5 + 33, 16 * 5 + 1, 2},
+                                 {128 * last + 7, 16 * last + 2, 1}, {128 * last + 39, 16 * last + 4, 7}};
+    for (const auto& c : cases) {
+      const uint32_t want = c[2] * (uint32_t)w + c[1];
+      if (flg::tile_order(c[0], 1, w, h) != want) {
+        std::printf("FAIL tile_order(%u) at %d x %d: %u, expected %u\n", c[0], w, h,
+                    flg::tile_order(c[0], 1, w, h), want);
+        ++failures;
+      }
+    }
+  }
+  std::printf("tile_order: checked\n");
+}
+
 int main() {
+  check_tile_order();
   {  // bvh_test: MakeMeshedSphere(200, 200, 5), rays from inside
     const int nt = 200, np = 200;
     const int32_t ntris = flg_scene_uv_sphere(nt, np, 5.0f, nullptr, nullptr);

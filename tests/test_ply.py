@@ -175,3 +175,19 @@ def test_ply_hostile_counts_and_values(tmp_path):
     over = str(tmp_path / "overflow.ply")
     open(over, "w").write(open(sub).read().replace("1e-40", "1e39"))
     _expect_error(over)
+    # a list of some other element whose count (near 2^32) the body cannot
+    # hold: refused at the count, before any item is read
+    import time
+    for fmt, cnt in (("binary_little_endian", struct.pack("<I", 4000000000)), ("ascii", b"4000000000")):
+        lst = str(tmp_path / "biglist.ply")
+        hdr = ("ply\nformat %s 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+               "element face 1\nproperty list uchar int vertex_indices\nelement extra 1\n"
+               "property list uint int values\nend_header\n" % fmt).encode()
+        if fmt == "ascii":
+            body = b"0 0 0\n1 0 0\n0 1 0\n3 0 1 2\n" + cnt + b" 1 2 3\n"
+        else:
+            body = struct.pack("<9f", 0, 0, 0, 1, 0, 0, 0, 1, 0) + struct.pack("<B3i", 3, 0, 1, 2) + cnt + bytes(12)
+        open(lst, "wb").write(hdr + body)
+        t0 = time.perf_counter()
+        _expect_error(lst)
+        assert time.perf_counter() - t0 < 1.0
