@@ -6,11 +6,20 @@ on the 2,097,152-triangle fBm-displaced octahedral sphere (amplitude 0.1),
 bounces) is one closest-hit query of the radix-tree LBVH.  One step = one full
 frame; the BVH build and scene generation are outside the timed region.
 
-Also reported (`secondary`): LBVH build time of that mesh, primary-ray
-closest-hit Mrays/s (65,536-triangle soup, 1024^2 primary rays), the batched
-build of 256 meshes, volumetric path samples/s (256^3 fBm grid, 512^2, 64 spp,
-depth 32) and the differentiable forward + adjoint cost against the plain
-forward (128^3 grid, 256^2, 16 spp).
+The metric's other legs are reported under `legs`, each with its own value,
+roofline (algorithmic bytes over the kernel's HIP-event time, against the
+8 TB/s HBM peak) and CPU baseline (the CPU restatement of the reference's
+algorithm timed on this host's cores over a bounded sample):
+  vol_path       volumetric path samples/s (256^3 fBm grid, 512^2, 64 spp, depth 32)
+  fwd_adjoint    differentiable forward + path-replay adjoint, ms per iteration
+                 and its ratio to the plain forward (128^3 grid, 256^2, 16 spp),
+                 at the uniform start state and at a non-uniform fBm state
+  lbvh_build     LBVH build of the 2M-triangle mesh and of the 65,536-triangle soup
+  batched_build  256 meshes (log-uniform 1K..256K triangles) in one build
+  primary_rays   closest hit of 1024^2 jittered primary rays on the 65,536-triangle soup
+  combined       the 8,388,608-triangle interface mesh around a 512^3 brick grid
+                 (1920x1080, 64 spp, depth 16): forward, differentiable forward
+                 and adjoint over the active bricks
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--fast]
 """
@@ -24,6 +33,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+CPU_SHARE = 16  # host cores of one GPU's share on the measurement machine
+BUILD_BYTES_PER_TRI = 248  # 36 verts + 8 key/index + 4 passes x 16 + 16 gathers + 64 node + 60 refit
 
 
 def log(*a):
@@ -45,21 +56,49 @@ def timed(fn, steps, warmup, sync):
     return (time.perf_counter() - t0) / steps
 
 
+def roofline(achieved_gbs, kernel, bytes_per_unit, unit_name, **extra):
+    r = {"bound": "hbm", "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+         "algorithmic_bytes_per_" + unit_name: bytes_per_unit}
+    r.update(extra)
+    return r
+
+
+def kernel_ms(ctx, names, calls):
+    """Mean HIP-event time per call of the library timers `names` (summed)."""
+    tot = 0.0
+    for n in names:
+        ms, _ = ctx.kernel_time(n)
+        tot += ms
+    return tot / max(calls, 1)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, CPU_SHARE))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--fast", action="store_true", help="headline only (no secondary workloads)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--fast", action="store_true", help="headline only (no other legs)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU work of the headline's baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--graph", action="store_true", help="replay each frame from a captured hipGraph")
     ap.add_argument("--shape", default="",
-                    help="fixed launch shape block,blocks_per_cu,stack_mode,ray_fetch for every frame "
-                         "(skips the autotuner; profiling runs use the shape it picked)")
+                    help="fixed launch shape block,blocks_per_cu,stack_mode,ray_fetch[,ring_depth] for every "
+                         "frame (skips the autotuner; profiling runs use the shape it picked)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one calibration copy + one headline frame, no JSON (for counter runs)")
+    ap.add_argument("--only", default="",
+                    help="comma-separated legs to run besides the headline (vol_path, fwd_adjoint, "
+                         "lbvh_build, batched_build, primary_rays, combined); default all")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the hot path has no multi-GPU exchange to time "
@@ -76,6 +115,10 @@ def main():
     dev = torch.device("cuda", 0)
     ctx = api.Context(0)
     sync = torch.cuda.synchronize
+    want = set(x for x in args.only.split(",") if x) or {"vol_path", "fwd_adjoint", "lbvh_build",
+                                                           "batched_build", "primary_rays", "combined"}
+    if args.fast:
+        want = set()
 
     # ------------------------------------------------------ headline workload
     W, H, SPP, DEPTH = 1920, 1080, 16, 8
@@ -84,20 +127,19 @@ def main():
     ntris = ind.size // 3
     dv, di = torch.from_numpy(v).to(dev), torch.from_numpy(ind).to(dev)
     sync()
-    t0 = time.perf_counter()
     bvh = api.LBVH.build(ctx, dv, di)
-    sync()
-    build_ms_first = (time.perf_counter() - t0) * 1e3
-    build_ms = timed(lambda: api.LBVH.build(ctx, dv, di).close(), 3, 1, sync) * 1e3
     cam2 = cam_of(N, (0.0, 0.0, -3.0), (0.0, 0.0, 0.0))
 
     shape = {}
     if args.shape:
-        b, bpc, sm, rf = (int(x) for x in args.shape.split(","))
-        shape = {"trace_block": b, "blocks_per_cu": bpc, "stack_mode": sm, "ray_fetch": rf}
+        vals = [int(x) for x in args.shape.split(",")]
+        b, bpc, sm, rf = vals[:4]
+        rd = vals[4] if len(vals) > 4 else 0
+        shape = {"trace_block": b, "blocks_per_cu": bpc, "stack_mode": sm, "ray_fetch": rf, "ring_depth": rd}
     info = bvh.pathtrace(cam2, W, H, SPP, DEPTH, **shape)  # also warms the queues
     segments = info["segments"]
-    log("frame: %d paths, %d ray segments" % (info["paths"], segments))
+    paths = info["paths"]
+    log("frame: %d paths, %d ray segments" % (paths, segments))
 
     if args.profile_pass:
         # for hardware-counter runs: a plain device copy of known size (the
@@ -117,7 +159,8 @@ def main():
     tuning = None
     if shape:
         tuning = {"block_size": shape["trace_block"], "blocks_per_cu": shape["blocks_per_cu"],
-                  "stack_mode": shape["stack_mode"], "ray_fetch": shape["ray_fetch"], "fixed": True}
+                  "stack_mode": shape["stack_mode"], "ray_fetch": shape["ray_fetch"],
+                  "ring_depth": shape["ring_depth"], "fixed": True}
     elif not args.no_tune:
         tuning = bvh.autotune(cam2, W, H, SPP, DEPTH)
         log("autotune: %s" % tuning)
@@ -129,6 +172,7 @@ def main():
         if shape:
             cfg.trace_block, cfg.blocks_per_cu = shape["trace_block"], shape["blocks_per_cu"]
             cfg.stack_mode, cfg.ray_fetch = shape["stack_mode"], shape["ray_fetch"]
+            cfg.ring_depth = shape["ring_depth"]
         ctx.check(ctx.lib.flg_pathtrace_diffuse(ctx.h, bvh.h, cam2, cfg, None, None, None, None))
 
     # timed region: K frames, HIP-event kernel timers recorded live
@@ -142,16 +186,15 @@ def main():
         frame()
     sync()
     dt = (time.perf_counter() - t0) / args.steps
+    if args.graph:  # a replayed graph has no per-launch timers: timed eager frames after it
+        ctx.reset_profiling()
+        for _ in range(args.steps):
+            frame(use_graph=False)
+        sync()
     trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
     total_ms, _ = ctx.kernel_time("pathtrace_total")
-    if args.graph:  # a replayed graph has no per-launch timers: one timed eager frame
-        ctx.reset_profiling()
-        frame(use_graph=False)
-        sync()
-        trace_ms, trace_launches = ctx.kernel_time("pathtrace_bounce")
-        total_ms, _ = ctx.kernel_time("pathtrace_total")
-        trace_ms, trace_launches = trace_ms * args.steps, trace_launches * args.steps
-        total_ms *= args.steps
+    first_ms, _ = ctx.kernel_time("pathtrace_first_bounce")
+    later_ms, _ = ctx.kernel_time("pathtrace_later_bounces")
     ctx.profiling(False)
     value = segments / dt / 1e6
 
@@ -165,152 +208,74 @@ def main():
     avg_launch_ms = trace_ms / trace_launches
     bytes_per_launch = bytes_per_ray * segments / launches_per_frame
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-                "kernel": ("k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)"
-                           if not tuning or tuning.get("ray_fetch", 1) != 2 else
-                           "k_pt_bounce (first bounce) then k_pt_trace + k_pt_shade per bounce "
-                           "(closest-hit traversal with per-wave ray refill, then the diffuse bounce); "
-                           "a launch = one bounce"),
-                "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
-                "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
-                "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,
-                "trace_kernel_share": round(trace_ms / max(total_ms, 1e-9), 4),
-                "launch_shape": tuning, "graph": bool(args.graph)}
+    traffic = measured_traffic()
+    head_roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                 "kernel": {1: "k_pt_bounce (closest-hit traversal + diffuse bounce, one launch per bounce)",
+                            2: "k_pt_bounce (first bounce) then k_pt_trace + k_pt_shade per bounce "
+                               "(closest-hit traversal with per-wave ray refill, then the diffuse bounce); "
+                               "a launch = one bounce",
+                            3: "k_pt_bounce (first bounce) then k_pt_paths (every later bounce in one launch: "
+                               "each lane follows its path to the end, refilling from the queue); a launch = "
+                               "one of these two"}[(tuning or {}).get("ray_fetch", 1) or 1],
+                 "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
+                 "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
+                 "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,
+                 "trace_kernel_share": round(trace_ms / max(total_ms, 1e-9), 4),
+                 "launch_shape": tuning, "graph": bool(args.graph)}
+    if traffic:
+        # the nodes and triangles stay in the 256 MB MALL and the L2s: HBM
+        # itself moves far fewer bytes than the traversal reads, and the
+        # kernels are limited by VALU issue and lane divergence, not bandwidth
+        meas = traffic / (avg_launch_ms * 1e-3) / 1e9
+        head_roof.update({"hbm_measured_GBps": round(meas, 2), "hbm_measured_frac": round(meas / HBM_PEAK_GBS, 5),
+                          "limiter": "VALU issue and lane divergence (per-lane traversal); the algorithmic "
+                                     "bytes are served mostly from L2 and MALL"})
+    incoherent = segments - paths  # every segment after a path's camera ray
+    secondary = {
+        "first_bounce_ms": round(first_ms / args.steps, 4),
+        "incoherent_bounces_ms": round(later_ms / args.steps, 4),
+        "incoherent_segments": incoherent,
+        "incoherent_grays_per_s": round(incoherent / (later_ms / args.steps * 1e-3) / 1e9, 4),
+        "camera_segments": paths,
+        "camera_grays_per_s": round(paths / (first_ms / args.steps * 1e-3) / 1e9, 4),
+    }
+    legs = {}
+    cpu_jobs = []  # CPU baselines run after all GPU work
 
-    secondary = {"lbvh_build_2M_ms": round(build_ms, 3), "lbvh_build_2M_first_ms": round(build_ms_first, 3),
-                 "lbvh_build_2M_mtris_per_s": round(ntris / build_ms / 1e3, 2)}
+    # ------------------------------------------------------------ LBVH build
+    log("headline: %.1f Mrays/s" % value)
+    if "lbvh_build" in want:
+        log("leg: lbvh_build")
+        legs["lbvh_build"] = bench_build(ctx, api, torch, dev, sync, v, ind, dv, di, cpu_jobs)
+    if "batched_build" in want:
+        log("leg: batched_build")
+        legs["batched_build"] = bench_batched(ctx, api, torch, dev, sync, cpu_jobs)
+    if "primary_rays" in want:
+        log("leg: primary_rays")
+        legs["primary_rays"] = bench_primary(ctx, N, api, torch, dev, sync, cpu_jobs, secondary)
+    if "vol_path" in want:
+        log("leg: vol_path")
+        legs["vol_path"] = bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs)
+    if "fwd_adjoint" in want:
+        log("leg: fwd_adjoint")
+        legs["fwd_adjoint"] = bench_fwd_adjoint(ctx, sync, cpu_jobs)
+    if "combined" in want:
+        log("leg: combined")
+        legs["combined"] = bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H)
 
-    # -------------------------------------------------- secondary workloads
-    if not args.fast:
-        # 65,536-triangle soup, 1024^2 jittered primary rays
-        v0, i0 = api.gen_soup(65536)
-        b0 = api.LBVH.build(ctx, torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev))
-        cam0 = cam_of(N, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
-        rays0 = api.camera_rays(ctx, cam0, 1024, 1024)
-        hits0 = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
-        t = timed(lambda: b0.trace(rays0, hits0), 20, 5, sync)
-        secondary["soup65k_primary_closest_hit_mrays_per_s"] = round(1024 * 1024 / t / 1e6, 2)
-        secondary["soup65k_build_ms"] = round(timed(
-            lambda: api.LBVH.build(ctx, torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev)).close(),
-            5, 2, sync) * 1e3, 3)
-        # batched build: 256 meshes, log-uniform 1K..256K triangles, one batch
-        sizes = api.gen_batch_sizes(256)
-        meshes = []
-        for k, n in enumerate(sizes):
-            vv, ii = api.gen_soup(int(n), initstate=42, initseq=1000 + k)
-            meshes.append((torch.from_numpy(vv).to(dev), torch.from_numpy(ii).to(dev)))
-        sync()
-
-        def batch():
-            for h in api.LBVH.build_batched(ctx, meshes):
-                h.close()
-        tb = timed(batch, 3, 1, sync)
-        secondary["batched_build_256_meshes_ms"] = round(tb * 1e3, 3)
-        secondary["batched_build_total_tris"] = int(sizes.sum())
-        secondary["batched_build_mtris_per_s"] = round(int(sizes.sum()) / tb / 1e6, 2)
-        # the scene accel over the same 256 meshes, each moved to a cell of a
-        # 16 x 16 grid (pitch 1.25), against one LBVH over all their
-        # triangles: 1024^2 camera rays from above
-        placed, voff, allv, alli = [], 0, [], []
-        for k, (vv, ii) in enumerate(meshes):
-            off = torch.tensor([1.25 * (k % 16), 1.25 * (k // 16), 0.0], dtype=torch.float32, device=dev)
-            pv = (vv.view(-1, 3) + off).reshape(-1)
-            placed.append((pv, ii))
-            allv.append(pv)
-            alli.append(ii + voff)
-            voff += pv.numel() // 3
-        blas = api.LBVH.build_batched(ctx, placed)
-        sacc = api.SceneAccel(ctx, blas)
-        cam_s = N.Camera.make((10.0, 10.0, 30.0), (10.0, 10.0, 0.0), (0.0, 1.0, 0.0), 40.0)
-        rays_s = api.camera_rays(ctx, cam_s, 1024, 1024)
-        hits_s = torch.empty((1024 * 1024, 4), dtype=torch.int32, device=dev)
-        ts_ = timed(lambda: sacc.trace(rays_s, hits_s), 10, 2, sync)
-        secondary["scene_256_meshes_closest_hit_mrays_per_s"] = round(1024 * 1024 / ts_ / 1e6, 2)
-        secondary["scene_256_meshes_hit_fraction"] = round(float((hits_s[:, 0] >= 0).float().mean()), 4)
-        merged = api.LBVH.build(ctx, torch.cat(allv), torch.cat(alli))
-        hits_m = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
-        tm_ = timed(lambda: merged.trace(rays_s, hits_m), 10, 2, sync)
-        secondary["merged_single_lbvh_closest_hit_mrays_per_s"] = round(1024 * 1024 / tm_ / 1e6, 2)
-        sacc.close()
-        merged.close()
-        for h in blas:
-            h.close()
-        del placed, allv, alli, meshes
-        # volumetric path tracing: 256^3 fBm grid, 512x512, 64 spp, depth 32
-        g1 = api.gen_fbm_grid(256, seed=1)
-        vol = api.Volume(ctx, torch.from_numpy(g1).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
-        sc1 = api.vol_scene(cam0)
-        out1 = torch.empty((512, 512, 3), dtype=torch.float32, device=dev)
-        _, st1 = vol.render(sc1, 512, 512, 64, 32, out=out1, stats=True)
-        t1 = timed(lambda: vol.render(sc1, 512, 512, 64, 32, out=out1), 3, 1, sync)
-        paths1 = 512 * 512 * 64
-        t1b = timed(lambda: vol.render(sc1, 512, 512, 64, 32, out=out1, schedule=1), 3, 1, sync)
-        secondary["vol_ms_per_frame_lane_per_path"] = round(t1b * 1e3, 3)
-        secondary["vol_path_samples_per_s"] = round(paths1 / t1, 1)
-        secondary["vol_ms_per_frame"] = round(t1 * 1e3, 3)
-        secondary["vol_lookups_per_path"] = round(st1["lookups"] / paths1, 3)
-        lb = 32 * st1["lookups"] + 12 * 512 * 512
-        secondary["vol_algorithmic_GBps"] = round(lb / t1 / 1e9, 2)
-        if hasattr(ctx.lib, "flg_volpath_adjoint"):
-            try:
-                from fledge_amd import diff
-                secondary.update(diff.bench_adjoint(ctx, timed, sync))
-            except Exception as e:  # reported, not hidden
-                secondary["adjoint_error"] = repr(e)
-
-        # surfaces + media (VolPathIntegrator): the 8,388,608-triangle displaced
-        # octahedral sphere (amplitude 0.05) as the interface around a 512^3
-        # grid whose 8^3 bricks are active with probability 0.15 and hold fBm,
-        # 1920x1080, 64 spp, depth 16, one frame
-        vc, ic = api.gen_octasphere(10, 0.05, seed=7, base_freq=2.0)
-        gc = api.gen_fbm_grid(512, seed=3)
-        bricks = np.random.default_rng(4).random((64, 64, 64)) < 0.15
-        gc *= np.repeat(np.repeat(np.repeat(bricks, 8, 0), 8, 1), 8, 2)
-        bc = api.LBVH.build(ctx, torch.from_numpy(vc).to(dev), torch.from_numpy(ic).to(dev))
-        gct = torch.from_numpy(gc).to(dev)
-        volc = api.Volume(ctx, gct, [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
-        gmax = float(gc.max())
-        del gc, vc, ic
-        scc = api.vol_scene(cam2)
-        outc = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
-        _, stc = volc.render_mesh(bc, scc, W, H, 64, 16, out=outc, stats=True)
-        tc = timed(lambda: volc.render_mesh(bc, scc, W, H, 64, 16, out=outc), 1, 0, sync)
-        secondary["combined_8M_512cube_ms_per_frame"] = round(tc * 1e3, 2)
-        secondary["combined_path_samples_per_s"] = round(W * H * 64 / tc, 1)
-        secondary["combined_lookups_per_path"] = round(stc["lookups"] / (W * H * 64), 3)
-        secondary["combined_config"] = ("VolPathIntegrator, 8,388,608-triangle interface mesh around a 512^3 grid "
-                                        "(8^3 bricks, 15% active), 1920x1080, 64 spp, depth 16")
-        volc.close()
-        # the same scene through the differentiable estimator (sigma_t 8 per
-        # unit density, albedo 0.8: the medium above), forward with the walk
-        # tape, then the adjoint of an L2-like loss with respect to the
-        # active bricks' voxels
-        from fledge_amd import diff
-        dvc = diff.DiffVolume(ctx, gct, g=0.3)
-        layout = diff.BrickLayout(ctx, gct, 8)
-        del gct
-        tape = diff.new_tape(ctx, W, H, 64, 16)
-        lpc = torch.empty(W * H * 64, dtype=torch.float32, device=dev)
-        dlc = torch.full((H, W, 3), 1.0 / (W * H * 3), dtype=torch.float32, device=dev)
-        gb = layout.zeros()
-        gac = torch.zeros(1, dtype=torch.float32, device=dev)
-        mu_c = 8.0 * gmax
-        tf = timed(lambda: diff.mesh_forward(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, out=outc, lpath=lpc,
-                                             tape=tape), 1, 1, sync)
-        ta = timed(lambda: diff.mesh_adjoint(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, lpc, dlc, grad=gb,
-                                             galpha=gac, tape=tape, bricks=layout), 1, 0, sync)
-        secondary["combined_diff_forward_ms"] = round(tf * 1e3, 2)
-        secondary["combined_adjoint_ms"] = round(ta * 1e3, 2)
-        secondary["combined_active_bricks"] = layout.n_active
-        secondary["combined_fwd_adj_over_forward"] = round((tf + ta) / tc, 3)
-        del tape, lpc, gb
-        bc.close()
-
-    # ------------------------------------------------------ CPU baseline
+    # ------------------------------------------------------ CPU baselines
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(v, ind, W, H, SPP, DEPTH, args.cpu_seconds)
+        cores = host_cores()
+        cpu = cpu_baseline(v, ind, W, H, SPP, DEPTH, args.cpu_seconds, cores)
+        if cores > 1:
+            c1 = cpu_baseline(v, ind, W, H, SPP, DEPTH, args.cpu_seconds / 2, cores - 1)
+            cpu["value_nproc_minus_1"] = c1["value"]
+            cpu["cores_nproc_minus_1"] = cores - 1
+        for job in cpu_jobs:
+            log("cpu baseline: %s" % job.__qualname__.split(".")[0])
+            job(cores)
 
     out = {
         "metric": "Mrays/s closest-hit (2M tris); vol path samples/s; fwd+adjoint ms/iter vs plain",
@@ -327,15 +292,309 @@ def main():
         "data": "synthetic",
         "config": {"workload": "incoherent diffuse closest-hit on a 2,097,152-triangle fBm-displaced "
                                "octahedral sphere (amplitude 0.1), 1920x1080, 16 spp, depth 8",
-                   "triangles": ntris, "rays_per_step": segments, "paths_per_step": W * H * SPP,
+                   "triangles": ntris, "rays_per_step": segments, "paths_per_step": paths,
                    "max_depth": DEPTH},
-        "roofline": roofline,
+        "roofline": head_roof,
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "legs": legs,
     }
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------- legs ---
+def bench_build(ctx, api, torch, dev, sync, v, ind, dv, di, cpu_jobs):
+    """LBVH build of the 2M-triangle octasphere and the 65,536-triangle soup:
+    wall time of the build call (output pooled per context after the first
+    build) and the build kernels' HIP-event time."""
+    names = ["lbvh_bound", "lbvh_morton", "lbvh_sort", "lbvh_emit", "lbvh_refit", "lbvh_soup"]
+    res = {}
+    v0, i0 = api.gen_soup(65536)
+    dv0, di0 = torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev)
+    for tag, (vv, ii, n) in {"octasphere_2M": (dv, di, ind.size // 3),
+                             "soup_65536": (dv0, di0, i0.size // 3)}.items():
+        def one():
+            api.LBVH.build(ctx, vv, ii, check_indices=False).close()
+        one()
+        wall = timed(one, 10, 2, sync)
+        ctx.profiling(True)
+        ctx.reset_profiling()
+        for _ in range(5):
+            one()
+        sync()
+        kms = kernel_ms(ctx, names, 5)
+        parts = {n: round(ctx.kernel_time(n)[0] / 5, 4) for n in names}
+        ctx.profiling(False)
+        res[tag] = {"ms": round(wall * 1e3, 4), "kernel_ms": round(kms, 4), "mtris_per_s": round(n / wall / 1e3, 2),
+                    "stages_ms": parts, "triangles": n,
+                    "roofline": roofline(BUILD_BYTES_PER_TRI * n / (kms * 1e-3) / 1e9,
+                                         "the build's kernels (bound, Morton, sort, emit, refit + soup)",
+                                         BUILD_BYTES_PER_TRI, "triangle")}
+    out = {"metric": "LBVH build ms (2,097,152-triangle displaced octasphere)", "value": res["octasphere_2M"]["ms"],
+           "unit": "ms", "higher_is_better": False, "roofline": res["octasphere_2M"]["roofline"],
+           "builds": res, "cpu_baseline": None}
+
+    def cpu_job(cores):
+        out["cpu_baseline"] = cpu_build_baseline(v, ind)
+    cpu_jobs.append(cpu_job)
+    return out
+
+
+def bench_batched(ctx, api, torch, dev, sync, cpu_jobs):
+    sizes = api.gen_batch_sizes(256)
+    meshes = []
+    for k, n in enumerate(sizes):
+        vv, ii = api.gen_soup(int(n), initstate=42, initseq=1000 + k)
+        meshes.append((torch.from_numpy(vv).to(dev), torch.from_numpy(ii).to(dev)))
+    sync()
+
+    def batch():
+        for h in api.LBVH.build_batched(ctx, meshes, check_indices=False):
+            h.close()
+    batch()
+    tb = timed(batch, 5, 1, sync)
+    names = ["lbvh_bound", "lbvh_morton", "lbvh_sort", "lbvh_emit", "lbvh_refit", "lbvh_soup"]
+    ctx.profiling(True)
+    ctx.reset_profiling()
+    for _ in range(3):
+        batch()
+    sync()
+    kms = kernel_ms(ctx, names, 3)
+    parts = {n: round(ctx.kernel_time(n)[0] / 3, 4) for n in names}
+    ctx.profiling(False)
+    tot = int(sizes.sum())
+    out = {"metric": "batched LBVH build of 256 meshes (log-uniform 1K..256K triangles) in one call, ms",
+           "value": round(tb * 1e3, 3), "unit": "ms", "higher_is_better": False, "triangles": tot,
+           "mtris_per_s": round(tot / tb / 1e6, 2), "kernel_ms": round(kms, 4), "stages_ms": parts,
+           "roofline": roofline(BUILD_BYTES_PER_TRI * tot / (kms * 1e-3) / 1e9, "the batch's build kernels",
+                                BUILD_BYTES_PER_TRI, "triangle"),
+           "cpu_baseline": None}
+    host = [(m[0].cpu().numpy(), m[1].cpu().numpy()) for m in meshes[:32]]
+
+    def cpu_job(cores):
+        import oracle_lib as orc
+        t0 = time.perf_counter()
+        n = 0
+        for vv, ii in host:
+            orc.Lbvh(vv, ii)
+            n += ii.size // 3
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(dtc * 1e3 * tot / n, 1), "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "the first 32 of the 256 meshes (%d triangles) built one after another "
+                                         "by the oracle's serial build, %.2f s, scaled to the batch's %d "
+                                         "triangles" % (n, dtc, tot)}
+    cpu_jobs.append(cpu_job)
+    # the scene accel over the same 256 meshes, each moved to a cell of a
+    # 16 x 16 grid (pitch 1.25), against one LBVH over all their triangles:
+    # 1024^2 camera rays from above
+    from fledge_amd import _native as N
+    placed, voff, allv, alli = [], 0, [], []
+    for k, (vv, ii) in enumerate(meshes):
+        off = torch.tensor([1.25 * (k % 16), 1.25 * (k // 16), 0.0], dtype=torch.float32, device=dev)
+        pv = (vv.view(-1, 3) + off).reshape(-1)
+        placed.append((pv, ii))
+        allv.append(pv)
+        alli.append(ii + voff)
+        voff += pv.numel() // 3
+    blas = api.LBVH.build_batched(ctx, placed)
+    sacc = api.SceneAccel(ctx, blas)
+    cam_s = N.Camera.make((10.0, 10.0, 30.0), (10.0, 10.0, 0.0), (0.0, 1.0, 0.0), 40.0)
+    rays_s = api.camera_rays(ctx, cam_s, 1024, 1024)
+    hits_s = torch.empty((1024 * 1024, 4), dtype=torch.int32, device=dev)
+    ts_ = timed(lambda: sacc.trace(rays_s, hits_s), 10, 2, sync)
+    out["scene_accel_256_meshes_mrays_per_s"] = round(1024 * 1024 / ts_ / 1e6, 2)
+    out["scene_accel_hit_fraction"] = round(float((hits_s[:, 0] >= 0).float().mean()), 4)
+    merged = api.LBVH.build(ctx, torch.cat(allv), torch.cat(alli))
+    hits_m = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
+    tm_ = timed(lambda: merged.trace(rays_s, hits_m), 10, 2, sync)
+    out["merged_single_lbvh_mrays_per_s"] = round(1024 * 1024 / tm_ / 1e6, 2)
+    sacc.close()
+    merged.close()
+    for h in blas:
+        h.close()
+    return out
+
+
+def bench_primary(ctx, N, api, torch, dev, sync, cpu_jobs, secondary):
+    v0, i0 = api.gen_soup(65536)
+    b0 = api.LBVH.build(ctx, torch.from_numpy(v0).to(dev), torch.from_numpy(i0).to(dev))
+    cam0 = cam_of(N, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
+    rays0 = api.camera_rays(ctx, cam0, 1024, 1024)
+    hits0 = torch.empty((1024 * 1024, 2), dtype=torch.int32, device=dev)
+    t = timed(lambda: b0.trace(rays0, hits0), 20, 5, sync)
+    ctx.profiling(True)
+    ctx.reset_profiling()
+    for _ in range(10):
+        b0.trace(rays0, hits0)
+    sync()
+    kms = kernel_ms(ctx, ["trace_closest"], 10)
+    ctx.profiling(False)
+    c = b0.count(rays0).cpu().numpy()
+    R = 1024 * 1024
+    vn, vt = c[0] / R, c[1] / R
+    bpr = 32 + 8 + 64 * vn + 36 * vt
+    out = {"metric": "closest-hit Mrays/s, 1024^2 jittered primary rays on the 65,536-triangle soup",
+           "value": round(R / t / 1e6, 2), "unit": "Mrays/s", "higher_is_better": True,
+           "kernel_ms": round(kms, 4),
+           "roofline": roofline(bpr * R / (kms * 1e-3) / 1e9, "k_trace", round(bpr, 2), "ray",
+                                node_visits_per_ray=round(vn, 3), tri_tests_per_ray=round(vt, 3)),
+           "cpu_baseline": None}
+    rays_h = rays0.cpu().numpy()
+
+    def cpu_job(cores):
+        import oracle_lib as orc
+        orc.NTHREADS = cores
+        o = orc.Lbvh(v0, i0)
+        sub = rays_h[::8]
+        t0 = time.perf_counter()
+        o.trace(sub)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(len(sub) / dtc / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+                               "kind": "port",
+                               "sample": "every 8th of the 1024^2 rays (%d), the oracle's per-ray DFS, %.2f s"
+                                         % (len(sub), dtc)}
+    cpu_jobs.append(cpu_job)
+    b0.close()
+    return out
+
+
+def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
+    """256^3 fBm grid, 512x512, 64 spp, depth 32: path samples/s; roofline of
+    the path kernel over 32 B per trilinear lookup + 12 B per pixel."""
+    g1 = api.gen_fbm_grid(256, seed=1)
+    vol = api.Volume(ctx, torch.from_numpy(g1).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam0 = cam_of(N, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
+    sc1 = api.vol_scene(cam0)
+    Wv = Hv = 512
+    S1 = 64
+    out1 = torch.empty((Hv, Wv, 3), dtype=torch.float32, device=dev)
+    _, st1 = vol.render(sc1, Wv, Hv, S1, 32, out=out1, stats=True)
+    t1 = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1), 5, 2, sync)
+    t1b = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1, schedule=1), 3, 1, sync)
+    ctx.profiling(True)
+    ctx.reset_profiling()
+    for _ in range(5):
+        vol.render(sc1, Wv, Hv, S1, 32, out=out1)
+    sync()
+    kms = kernel_ms(ctx, ["volpath_paths"], 5)
+    ctx.profiling(False)
+    P = Wv * Hv * S1
+    L = st1["lookups"] / P
+    lb = 32 * st1["lookups"] + 12 * Wv * Hv
+    out = {"metric": "vol path samples/s (256^3 fBm grid, 512x512, 64 spp, depth 32)",
+           "value": round(P / t1, 1), "unit": "path samples/s", "higher_is_better": True,
+           "ms_per_frame": round(t1 * 1e3, 3), "ms_per_frame_lane_per_path": round(t1b * 1e3, 3),
+           "kernel_ms": round(kms, 4), "lookups_per_path": round(L, 4),
+           "roofline": roofline(lb / (kms * 1e-3) / 1e9, "k_svol_regen (persistent lanes with path regeneration)",
+                                round(32 * L + 12 / S1, 3), "path"),
+           "cpu_baseline": None}
+
+    def cpu_job(cores):
+        import oracle_lib as orc
+        import numpy as np
+        lib = orc.lib()
+        rgb = np.zeros((Hv, Wv, 3), np.float32)
+        lk = np.zeros(1, np.uint64)
+        pa = np.zeros(1, np.uint64)
+        cam10 = np.array([0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0], np.float32)
+        stride = 3  # every third pixel, spread over rows and columns (87,382 of 262,144)
+        t0 = time.perf_counter()
+        lib.orc_volpath_render_strided(
+            orc._p(g1), 256, 256, 256, orc._p(np.array([0, 0, 0, 1, 1, 1], np.float32)),
+            orc._p(np.array([6.4, 1.6, 0.3], np.float32)), orc._p(np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)),
+            orc._p(cam10), Wv, Hv, S1, 32, stride, orc._p(rgb), cores, orc._p(lk), orc._p(pa))
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(float(pa[0]) / dtc, 1), "unit": "path samples/s", "cores": cores,
+                               "kind": "port",
+                               "sample": "every %d-th pixel of the frame (%d paths, all 64 spp each), the oracle's "
+                                         "SVolIntegrator restatement, %.2f s" % (stride, int(pa[0]), dtc)}
+    cpu_jobs.append(cpu_job)
+    vol.close()
+    return out
+
+
+def bench_fwd_adjoint(ctx, sync, cpu_jobs):
+    from fledge_amd import diff
+    states = {}
+    for st in ("uniform", "fbm"):
+        states[st] = diff.bench_adjoint(ctx, timed, sync, state=st)
+    out = {"metric": "differentiable forward + adjoint ms/iter vs the plain forward (128^3, 256x256, 16 spp)",
+           "unit": "ms/iter", "higher_is_better": False, "states": {}}
+    for st, r in states.items():
+        P = r["paths"]
+        lf, la = r["lookups_forward"] / P, r["lookups_adjoint"] / P
+        kms = r["kernel_forward_ms"] + r["kernel_adjoint_ms"]
+        bytes_iter = P * (32 * lf + 64 * la)
+        rec = {k: (round(x, 4) if isinstance(x, float) else x) for k, x in r.items()
+               if not isinstance(x, dict)}
+        rec.update({k: {kk: round(vv, 4) for kk, vv in x.items()} for k, x in r.items() if isinstance(x, dict)})
+        rec["lookups_per_path_forward"] = round(lf, 4)
+        rec["lookups_per_path_adjoint"] = round(la, 4)
+        rec["roofline"] = roofline(bytes_iter / (kms * 1e-3) / 1e9,
+                                   "k_diff_regen<false> + k_diff_regen<true> (forward, then the replay)",
+                                   round(32 * lf + 64 * la, 3), "path",
+                                   bytes_per_iteration=int(bytes_iter))
+        out["states"][st] = rec
+    out["value"] = out["states"]["fbm"]["fwd_adjoint_ms_per_iter"]
+    out["ratio_vs_plain_forward"] = out["states"]["fbm"]["fwd_adjoint_vs_plain"]
+    out["ratio_vs_plain_forward_uniform_state"] = out["states"]["uniform"]["fwd_adjoint_vs_plain"]
+    out["roofline"] = out["states"]["fbm"]["roofline"]
+    out["cpu_baseline"] = None
+
+    def cpu_job(cores):
+        out["cpu_baseline"] = cpu_adjoint_baseline(cores)
+    cpu_jobs.append(cpu_job)
+    return out
+
+
+def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H):
+    """VolPathIntegrator: the 8,388,608-triangle displaced octahedral sphere
+    (amplitude 0.05) as the interface around a 512^3 grid whose 8^3 bricks
+    are active with probability 0.15 and hold fBm; 1920x1080, 64 spp, depth
+    16; then the differentiable forward with the walk tape and the adjoint
+    over the active bricks."""
+    from fledge_amd import diff
+    vc, ic = api.gen_octasphere(10, 0.05, seed=7, base_freq=2.0)
+    gc = api.gen_fbm_grid(512, seed=3)
+    bricks = np.random.default_rng(4).random((64, 64, 64)) < 0.15
+    gc *= np.repeat(np.repeat(np.repeat(bricks, 8, 0), 8, 1), 8, 2)
+    bc = api.LBVH.build(ctx, torch.from_numpy(vc).to(dev), torch.from_numpy(ic).to(dev))
+    gct = torch.from_numpy(gc).to(dev)
+    volc = api.Volume(ctx, gct, [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    gmax = float(gc.max())
+    del gc, vc, ic
+    scc = api.vol_scene(cam2)
+    outc = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    _, stc = volc.render_mesh(bc, scc, W, H, 64, 16, out=outc, stats=True)
+    tc = timed(lambda: volc.render_mesh(bc, scc, W, H, 64, 16, out=outc), 1, 0, sync)
+    volc.close()
+    dvc = diff.DiffVolume(ctx, gct, g=0.3)
+    layout = diff.BrickLayout(ctx, gct, 8)
+    del gct
+    tape = diff.new_tape(ctx, W, H, 64, 16)
+    lpc = torch.empty(W * H * 64, dtype=torch.float32, device=dev)
+    dlc = torch.full((H, W, 3), 1.0 / (W * H * 3), dtype=torch.float32, device=dev)
+    gb = layout.zeros()
+    gac = torch.zeros(1, dtype=torch.float32, device=dev)
+    mu_c = 8.0 * gmax
+    tf = timed(lambda: diff.mesh_forward(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, out=outc, lpath=lpc,
+                                         tape=tape), 1, 1, sync)
+    ta = timed(lambda: diff.mesh_adjoint(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, lpc, dlc, grad=gb,
+                                         galpha=gac, tape=tape, bricks=layout), 1, 0, sync)
+    P = W * H * 64
+    out = {"metric": "combined scene: VolPathIntegrator path samples/s (8,388,608-triangle interface mesh around "
+                     "a 512^3 grid of 8^3 bricks, 15% active; 1920x1080, 64 spp, depth 16)",
+           "value": round(P / tc, 1), "unit": "path samples/s", "higher_is_better": True,
+           "ms_per_frame": round(tc * 1e3, 2), "lookups_per_path": round(stc["lookups"] / P, 3),
+           "diff_forward_ms": round(tf * 1e3, 2), "adjoint_ms": round(ta * 1e3, 2),
+           "active_bricks": layout.n_active, "fwd_adjoint_over_forward": round((tf + ta) / tc, 3),
+           "roofline": None, "cpu_baseline": None}
+    del tape, lpc, gb
+    bc.close()
+    return out
+
+
+# ------------------------------------------------------- CPU baselines ---
 def measured_traffic():
     """HBM bytes per k_pt_bounce launch from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE runs of `bench.py --profile-pass`, each scaled
@@ -350,26 +609,20 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(v, ind, W, H, SPP, DEPTH, seconds):
+def cpu_baseline(v, ind, W, H, SPP, DEPTH, seconds, cores):
     """The oracle's LBVH + per-path DFS traversal (the restated fledge CPU path)
     on a bounded sample of the same frame: whole image rows spread over the
-    frame, sized to about `seconds` of work on this host's cores."""
+    frame, sized to about `seconds` of work on `cores` host threads."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
     o = orc.Lbvh(v, ind)
     cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
     row = W * SPP
     # calibrate on one central row
     t0 = time.perf_counter()
-    _, _, _, r = o.diffuse_render(cam10, W, H, SPP, DEPTH, (H // 2) * row, (H // 2 + 1) * row,
-                                  record=False, nthreads=cores)
+    o.diffuse_render(cam10, W, H, SPP, DEPTH, (H // 2) * row, (H // 2 + 1) * row, record=False, nthreads=cores)
     per_row = time.perf_counter() - t0
     nrows = int(max(4, min(H, seconds / max(per_row, 1e-6))))
     rows = np.unique(np.linspace(0, H - 1, nrows).astype(int))
@@ -381,8 +634,61 @@ def cpu_baseline(v, ind, W, H, SPP, DEPTH, seconds):
         rays += r
     dt = time.perf_counter() - t0
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": "%d of %d image rows (evenly spaced), %d paths, %d ray segments, %.1f s"
-                      % (len(rows), H, len(rows) * row, rays, dt)}
+            "sample": "%d of %d image rows (evenly spaced), %d paths, %d ray segments, %.1f s on %d threads "
+                      "(the host share of one GPU is %d cores)"
+                      % (len(rows), H, len(rows) * row, rays, dt, cores, CPU_SHARE)}
+
+
+def cpu_build_baseline(v, ind):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    t0 = time.perf_counter()
+    orc.Lbvh(v, ind)
+    dtc = time.perf_counter() - t0
+    return {"value": round(dtc * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+            "sample": "one build of the whole 2,097,152-triangle mesh by the oracle's serial restatement "
+                      "(stable sort, Karras emission, bottom-up refit), %.2f s" % dtc}
+
+
+def cpu_adjoint_baseline(cores):
+    """The FP64 CPU twin of the differentiable estimator (forward, then the
+    analytic adjoint) over the whole 256x256 x 16 spp configuration at the
+    non-uniform state (a few seconds of CPU work)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    from fledge_amd import api
+    P = orc._p
+    L = orc.lib()
+    theta = api.gen_fbm_grid(128, seed=3)
+    th64 = theta.astype(np.float64)
+    W = H = 256
+    SPP, DEPTH = 16, 32
+    mu = 8.0 * float(theta.max())
+    params = np.array([8.0, 0.8, 0.3, mu], np.float32)
+    lights = np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)
+    cam10 = np.array([0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0], np.float32)
+    bounds = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    stride = 1
+    lp = np.zeros(W * H * SPP, np.float64)
+    img = np.zeros(W * H, np.float64)
+    dl = np.full(W * H, 1.0 / (W * H), np.float64)
+    grad = np.zeros(128 ** 3, np.float64)
+    ga = np.zeros(1, np.float64)
+    lk = np.zeros(1, np.uint64)
+    t0 = time.perf_counter()
+    L.orc_diff_render_strided(P(th64), P(theta), 128, 128, 128, P(bounds), P(params), P(lights), P(cam10), W, H,
+                              SPP, DEPTH, stride, P(lp), P(img), None, None, None, cores, P(lk))
+    t1 = time.perf_counter()
+    L.orc_diff_render_strided(P(th64), P(theta), 128, 128, 128, P(bounds), P(params), P(lights), P(cam10), W, H,
+                              SPP, DEPTH, stride, P(lp), None, P(dl), P(grad), P(ga), cores, None)
+    t2 = time.perf_counter()
+    npx = (W * H + stride - 1) // stride
+    scale = W * H / npx
+    return {"value": round((t2 - t0) * scale * 1e3, 1), "unit": "ms/iter", "cores": cores, "kind": "port",
+            "forward_ms": round((t1 - t0) * scale * 1e3, 1), "adjoint_ms": round((t2 - t1) * scale * 1e3, 1),
+            "sample": "%d of %d pixels (16 spp each) of the fBm state through the FP64 CPU twin "
+                      "(forward %.2f s, adjoint %.2f s)" % (npx, W * H, t1 - t0, t2 - t1)}
 
 
 if __name__ == "__main__":

@@ -19,6 +19,11 @@ c_size = ctypes.c_size_t
 P = ctypes.POINTER
 
 FLG_OK = 0
+FLG_ERR_INVALID = 1
+FLG_ERR_HIP = 2
+FLG_ERR_NO_DEVICE = 3
+FLG_ERR_OOM = 4
+FLG_ERR_RANGE = 5
 FLG_NORMAL_FIXED = 0
 FLG_NORMAL_COMPAT = 1
 
@@ -38,7 +43,7 @@ class LbvhView(ctypes.Structure):
 
 
 class TraceCfg(ctypes.Structure):
-    _fields_ = [("block_size", c_i32), ("stack_mode", c_i32), ("reserved", c_i32 * 6)]
+    _fields_ = [("block_size", c_i32), ("stack_mode", c_i32), ("ring_depth", c_i32), ("reserved", c_i32 * 5)]
 
 
 class Camera(ctypes.Structure):
@@ -61,13 +66,13 @@ class PtCfg(ctypes.Structure):
     _fields_ = [("width", c_i32), ("height", c_i32), ("spp", c_i32), ("max_depth", c_i32),
                 ("trace_block", c_i32), ("use_graph", c_i32), ("count_visits", c_i32),
                 ("blocks_per_cu", c_i32), ("stack_mode", c_i32), ("ray_fetch", c_i32),
-                ("reserved", c_i32 * 2)]
+                ("ring_depth", c_i32), ("reserved", c_i32 * 1)]
 
 
 class Tuning(ctypes.Structure):
     _fields_ = [("block_size", c_i32), ("blocks_per_cu", c_i32), ("ms", c_f), ("ms_default", c_f),
                 ("candidates", c_i32), ("stack_mode", c_i32), ("ray_fetch", c_i32),
-                ("reserved", c_i32 * 1)]
+                ("ring_depth", c_i32)]
 
 
 class Medium(ctypes.Structure):
@@ -85,7 +90,7 @@ class RenderCfg(ctypes.Structure):
 
 
 class DiffParams(ctypes.Structure):
-    _fields_ = [("sigma_t_scale", c_f), ("albedo", c_f), ("g", c_f), ("majorant", c_f)]
+    _fields_ = [("sigma_t_scale", c_f), ("albedo", c_f), ("g", c_f), ("majorant", c_f), ("d_lookups", c_vp)]
 
 
 class PlyMesh(ctypes.Structure):

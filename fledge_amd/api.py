@@ -193,7 +193,7 @@ class LBVH:
         arrs["bound"] = np.array(self.info().bound, np.float32)
         return arrs
 
-    def trace(self, rays, hits=None, block_size=0, stack_mode=0):
+    def trace(self, rays, hits=None, block_size=0, stack_mode=0, ring_depth=0):
         """rays: (n, 8) float32 (o, tmax, d, pad); hits: (n, 2) int32 or
         float32 (id, t bits)."""
         dev = self.ctx.device
@@ -206,6 +206,7 @@ class LBVH:
         cfg = N.TraceCfg()
         cfg.block_size = block_size
         cfg.stack_mode = stack_mode
+        cfg.ring_depth = ring_depth
         self.ctx.check(self.ctx.lib.flg_trace_closest(self.ctx.h, self.h, _ptr(rays), _ptr(hits), n,
                                                       ctypes.byref(cfg)))
         return hits
@@ -219,7 +220,7 @@ class LBVH:
 
     def pathtrace(self, cam, width, height, spp, max_depth, record=False, trace_block=0,
                   count_visits=False, use_graph=False, blocks_per_cu=0, stack_mode=0, ray_fetch=0,
-                  out=None):
+                  out=None, ring_depth=0):
         """One frame of the diffuse path tracer.  record: also return per-path
         segment counts, ids and t (into `out` = (nseg, seg_id, seg_t) when
         given, so that a replayed graph writes the same buffers)."""
@@ -229,6 +230,7 @@ class LBVH:
         cfg.blocks_per_cu = blocks_per_cu
         cfg.stack_mode = stack_mode
         cfg.ray_fetch = ray_fetch
+        cfg.ring_depth = ring_depth
         cfg.count_visits = 1 if count_visits else 0
         cfg.use_graph = 1 if use_graph else 0
         npaths = width * height * spp
@@ -264,7 +266,7 @@ class LBVH:
         self.ctx.check(self.ctx.lib.flg_autotune_pathtrace(self.ctx.h, self.h, ctypes.byref(cam),
                                                            ctypes.byref(cfg), ctypes.byref(t)))
         return {"block_size": t.block_size, "blocks_per_cu": t.blocks_per_cu,
-                "stack_mode": t.stack_mode, "ray_fetch": t.ray_fetch, "ms": t.ms,
+                "stack_mode": t.stack_mode, "ring_depth": t.ring_depth, "ray_fetch": t.ray_fetch, "ms": t.ms,
                 "ms_default": t.ms_default, "candidates": t.candidates}
 
     def autotune_trace(self, rays, hits):
@@ -274,8 +276,8 @@ class LBVH:
         t = N.Tuning()
         self.ctx.check(self.ctx.lib.flg_autotune_trace(self.ctx.h, self.h, _ptr(rays), _ptr(hits),
                                                        rays.shape[0], ctypes.byref(t)))
-        return {"block_size": t.block_size, "stack_mode": t.stack_mode, "ms": t.ms,
-                "ms_default": t.ms_default}
+        return {"block_size": t.block_size, "stack_mode": t.stack_mode, "ring_depth": t.ring_depth, "ms": t.ms,
+                "ms_default": t.ms_default, "candidates": t.candidates}
 
     def close(self):
         if self.h:

@@ -322,6 +322,7 @@ int diff_prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, cons
   D->mask_lo = D->mask_hi = nullptr;
   D->brick_slot = nullptr;
   D->brick_shift = D->nbx = D->nby = 0;
+  D->nlook = prm->d_lookups;
   if (tape) {
     if (cfg->max_depth > kMaxTapeDepth)
       return set_error(ctx, FLG_ERR_RANGE, "differentiable render: a walk tape needs max_depth <= 63");

@@ -31,8 +31,14 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 
 struct SharedBlock {
   void* ptr = nullptr;
+  size_t bytes = 0;
   int refs = 0;
 };
+
+// The context's pool of LBVH output blocks (lbvh.hip): a destroyed BVH's
+// block is kept for a later build; released with the context.
+void bvh_pool_give(flg_ctx* ctx, void* p, size_t bytes);
+void bvh_pool_release(flg_ctx* ctx);
 
 }  // namespace flg
 

@@ -180,11 +180,13 @@ struct flg_ctx {
   std::map<std::string, std::pair<void*, size_t>> scratch;
   int num_cus = 256;
   // launch shapes chosen by the autotuner (0: not tuned)
-  int tuned_trace_block = 0, tuned_trace_stack = 0, tuned_pt_block = 0, tuned_pt_bpc = 0,
-      tuned_pt_stack = 0, tuned_pt_fetch = 0;
+  int tuned_trace_block = 0, tuned_trace_stack = 0, tuned_trace_ring = 0, tuned_pt_block = 0, tuned_pt_bpc = 0,
+      tuned_pt_stack = 0, tuned_pt_ring = 0, tuned_pt_fetch = 0;
   // captured bounce loops, keyed by the bytes of their launch arguments
   hipStream_t capture_stream = nullptr;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  // output blocks of destroyed BVHs, kept for later builds (lbvh.hip)
+  std::vector<std::pair<void*, size_t>> bvh_pool;
 };
 
 namespace flg {

@@ -5,7 +5,7 @@
 // status code plus flg_last_error().
 #include <cstdio>
 
-#include "common.hpp"
+#include "bvh.hpp"
 #include "halton.hpp"
 
 namespace flg {
@@ -156,6 +156,7 @@ int flg_ctx_destroy(flg_ctx* ctx) {
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     for (auto& kv : ctx->scratch)
       if (kv.second.first) (void)hipFree(kv.second.first);
+    bvh_pool_release(ctx);
     release_halton_tables(ctx);
     delete ctx;
     return FLG_OK;

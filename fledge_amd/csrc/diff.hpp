@@ -46,7 +46,13 @@ struct DiffDev {
   // after brick in slot order, each x fastest
   const int32_t* brick_slot;
   int brick_shift, nbx, nby;
+  // NULL, or a counter of the grid lookups (fetches of 8 voxels) made
+  unsigned long long* nlook;
 };
+
+FLG_D void count_lookups(const DiffDev& D, unsigned n) {
+  if (D.nlook) atomicAdd(D.nlook, (unsigned long long)n);
+}
 
 struct Look2 {
   Tri8 t;     // lookup on theta (weights, base, rho)
@@ -57,6 +63,7 @@ FLG_D Look2 look2(const DiffDev& D, V3 p) {
   Look2 r;
   r.t = lookup(D.v, p);
   r.rho_p = D.same ? r.t.rho : lookup(D.vp, p).rho;
+  count_lookups(D, D.same ? 1u : 2u);
   return r;
 }
 
@@ -113,6 +120,7 @@ FLG_D bool seg_measure(const DiffDev& D, uint32_t key, uint32_t& n, V3 o, V3 d, 
     t += -flg_logf(walk_draw(key, n++)) / D.mu;
     if (t >= t_max) break;
     const float f = 1.0f - D.s * lookup(D.v, o + d * t).rho / D.mu;
+    count_lookups(D, 1u);
     if (f == 0.0f) {
       if (++r.zeros >= 2) return false;
     } else {
@@ -139,6 +147,7 @@ FLG_D bool seg_deposit(const DiffDev& D, uint32_t key, uint32_t& n, V3 o, V3 d, 
     t += -flg_logf(walk_draw(key, n++)) / D.mu;
     if (t >= t_max) break;
     const Tri8 q = lookup(D.v, o + d * t);
+    count_lookups(D, 1u);
     const float f = 1.0f - D.s * q.rho / D.mu;
     if (r.zeros == 0) {
       deposit(D, grad, q, scale * k * (r.nonzero / f));

@@ -22,19 +22,15 @@ namespace flg {
 
 namespace {
 
-template <class Stack>
-FLG_D Stack make_stack(uint2* ring, int stride, uint2* priv);
-template <>
-FLG_D PrivStack make_stack<PrivStack>(uint2*, int, uint2* priv) {
-  return PrivStack(priv);
-}
-template <>
-FLG_D RingStack<8> make_stack<RingStack<8>>(uint2* ring, int stride, uint2* priv) {
-  return RingStack<8>(ring, stride, priv);
-}
-template <>
-FLG_D RingStack<1> make_stack<RingStack<1>>(uint2* ring, int stride, uint2* priv) {
-  return RingStack<1>(ring, stride, priv);
+// The traversal stack of a lane: RING > 0 keeps its RING most recent entries
+// in LDS (ring slot k of the lane at ring[k * stride]) and spills older ones
+// to priv; RING == 0 keeps all of them in priv.
+template <int RING>
+using StackOf = typename std::conditional<(RING > 0), RingStack<(RING > 0 ? RING : 1)>, PrivStack>::type;
+template <int RING>
+FLG_D StackOf<RING> make_stack(uint2* ring, int stride, uint2* priv) {
+  if constexpr (RING > 0) return RingStack<RING>(ring, stride, priv);
+  else return PrivStack(priv);
 }
 
 // Traversal is latency-bound: the kernels ask for 8 waves per SIMD (at most 64
@@ -329,9 +325,8 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtSta
   const uint64_t lo = (uint64_t)wave * per;
   const uint32_t end = (uint32_t)(lo + per < n ? lo + per : n);
   uint32_t next = (uint32_t)(lo < n ? lo : n);  // the same in every lane of the wave
-  using Stack = typename std::conditional<(RING > 0), RingStack<(RING > 0 ? RING : 1)>, PrivStack>::type;
   uint2 priv[kStack];
-  Stack st = make_stack<Stack>(ring + threadIdx.x, BS, priv);
+  auto st = make_stack<RING>(ring + threadIdx.x, BS, priv);
   Counters c{0, 0};
   TraceRay r;
   uint32_t cur = 0;
@@ -412,14 +407,116 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
   }
 }
 
+// Bounces after the first in one persistent launch (ray_fetch 3).  Each wave
+// owns a contiguous slice of the first bounce's queue; a lane takes a queued
+// ray and follows its path to the end: when its traversal ends it records the
+// segment and samples the diffuse bounce (pt_finish, the code of the other
+// schedules, so the same bits) and traces the next segment itself, with no
+// queue or launch in between.  Lanes whose traversals have ended wait until
+// kFinishBatch of the wave's lanes (or all its traversing ones) are done, so
+// the costly shading runs for many lanes at once; idle lanes refill from the
+// slice as in k_pt_trace.  The segments traced after the queue's are counted
+// into count[bounce + 1] for the frame's statistics.
+constexpr int kFinishBatch = 16;
+
+template <bool COMPAT, int BS, int RING>
+__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_paths(TraceGeom g, PtState s, HaltonTables tab,
+                                                 HaltonParams hp, int bounce0, int max_depth) {
+  __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
+  const uint32_t n = s.count[bounce0];
+  const QRay* qin = s.q[bounce0 & 1];
+  const int lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * (BS / 64);
+  const uint32_t wave = blockIdx.x * (BS / 64) + threadIdx.x / 64;
+  uint32_t per = (uint32_t)(((uint64_t)n + nwaves - 1) / nwaves);
+  per = per < kMinSlice ? kMinSlice : per;
+  const uint64_t lo = (uint64_t)wave * per;
+  const uint32_t end = (uint32_t)(lo + per < n ? lo + per : n);
+  uint32_t next = (uint32_t)(lo < n ? lo : n);  // the same in every lane of the wave
+  uint2 priv[kStack];
+  auto st = make_stack<RING>(ring + threadIdx.x, BS, priv);
+  Counters c{0, 0};
+  TraceRay r;
+  V3 o = v3(0.0f, 0.0f, 0.0f), d = v3(0.0f, 0.0f, 1.0f);
+  uint32_t path = 0, hidx = 0, spawned = 0;
+  int b = bounce0;
+  bool have = false, active = false;
+  // a wave leaves once its slice is taken and all its lanes are idle; every
+  // trip traverses, shades or refills some lane, each traversal ends (a node
+  // is visited at most once) and each path ends by max_depth
+  while (true) {
+    const uint64_t busy = __ballot(have && active);
+    const uint64_t done = __ballot(have && !active);
+    if (done != 0 && (busy == 0 || __popcll(done) >= kFinishBatch)) {
+      if (have && !active) {
+        QRay nr;
+        if (pt_finish<COMPAT>(g, s, tab, hp, b, max_depth, o, d, path, hidx, r.hit, r.best, nr)) {
+          o = v3(nr.o.x, nr.o.y, nr.o.z);
+          d = v3(nr.d.x, nr.d.y, nr.d.z);
+          ++b;
+          ++spawned;
+          st.reset();
+          active = trace_begin<COMPAT, false>(g, o, d, __builtin_inff(), r, c);
+        } else {
+          have = false;
+        }
+      }
+      continue;
+    }
+    const uint64_t idle = __ballot(!have);
+    if (next < end && idle != 0 && (busy == 0 || __popcll(idle) >= kRefillIdle)) {
+      if (!have) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t i = next + (uint32_t)__popcll(idle & below);
+        if (i < end) {
+          const QRay q = qin[i];
+          o = v3(q.o.x, q.o.y, q.o.z);
+          d = v3(q.d.x, q.d.y, q.d.z);
+          path = f2u(q.o.w);
+          hidx = f2u(q.d.w);
+          b = bounce0;
+          st.reset();
+          have = true;
+          active = trace_begin<COMPAT, false>(g, o, d, __builtin_inff(), r, c);
+        }
+      }
+      next += (uint32_t)__popcll(idle);
+      continue;
+    }
+    if (busy == 0) break;  // nothing traversing, nothing to shade, nothing left to take
+    if (have && active) active = trace_visit<COMPAT, false>(g, r, st, c);
+  }
+  // segments after the queue's own, for the frame's count
+  uint32_t tot = spawned;
+  for (int k = 32; k >= 1; k >>= 1) tot += __shfl_xor(tot, k, 64);
+  if (lane == 0 && tot) atomicAdd(&s.count[bounce0 + 1], tot);
+}
+
 // Launch shapes: threads per block in {64, 128, 256, 512}; the counting
 // variants (instrumentation only) exist at 256.
 bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 
-// Stack policy of a launch: stack_mode 2 keeps the 8 most recent entries of
-// each lane in LDS (RingStack), anything else the whole stack in scratch.
-constexpr int kRing = 8;
-int ring_of(int mode) { return mode == 2 ? kRing : 0; }
+// Stack policy of a launch: stack_mode 2 keeps the `ring` (4, 8, 16 or 32)
+// most recent entries of each lane in LDS (RingStack), anything else the
+// whole stack in scratch.  A block's ring takes block * ring * 8 bytes of
+// LDS, at most 64 KiB: a ring of 32 needs blocks of at most 256 threads.
+constexpr int kRingDefault = 8;
+int ring_of(int mode, int ring) { return mode == 2 ? (ring ? ring : kRingDefault) : 0; }
+bool valid_ring(int block, int ring) {
+  return ring == 0 || ((ring == 4 || ring == 8 || ring == 16 || ring == 32) && block * ring * 8 <= 65536);
+}
+
+// Calls f with the ring depth as a compile-time constant (0: private stack).
+template <class F>
+void with_ring(int ring, F&& f) {
+  switch (ring) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    default: f(std::integral_constant<int, 0>{}); break;
+  }
+}
 
 template <bool COMPAT, int RING>
 void launch_trace_r(int block, hipStream_t st, const TraceGeom& g, const flg_ray* rays, flg_hit* hits,
@@ -428,18 +525,17 @@ void launch_trace_r(int block, hipStream_t st, const TraceGeom& g, const flg_ray
   switch (block) {
     case 64: k_trace<COMPAT, 64, RING><<<grid, 64, 0, st>>>(g, rays, hits, n); break;
     case 128: k_trace<COMPAT, 128, RING><<<grid, 128, 0, st>>>(g, rays, hits, n); break;
-    case 512: k_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, rays, hits, n); break;
+    case 512:
+      if constexpr (RING <= 16) k_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, rays, hits, n);
+      break;
     default: k_trace<COMPAT, 256, RING><<<grid, 256, 0, st>>>(g, rays, hits, n); break;
   }
 }
 
 template <bool COMPAT>
-void launch_trace(int block, int stack_mode, hipStream_t st, const TraceGeom& g, const flg_ray* rays,
+void launch_trace(int block, int ring, hipStream_t st, const TraceGeom& g, const flg_ray* rays,
                   flg_hit* hits, int64_t n) {
-  if (ring_of(stack_mode))
-    launch_trace_r<COMPAT, kRing>(block, st, g, rays, hits, n);
-  else
-    launch_trace_r<COMPAT, 0>(block, st, g, rays, hits, n);
+  with_ring(ring, [&](auto R) { launch_trace_r<COMPAT, decltype(R)::value>(block, st, g, rays, hits, n); });
 }
 
 template <bool COMPAT, bool FIRST, int RING>
@@ -454,7 +550,8 @@ void launch_bounce_bs(int grid, int block, hipStream_t st, const TraceGeom& g, c
       k_pt_bounce<COMPAT, FIRST, false, 128, RING><<<grid, 128, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
     case 512:
-      k_pt_bounce<COMPAT, FIRST, false, 512, RING><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
+      if constexpr (RING <= 16)
+        k_pt_bounce<COMPAT, FIRST, false, 512, RING><<<grid, 512, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
       break;
     default:
       k_pt_bounce<COMPAT, FIRST, false, 256, RING><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
@@ -467,22 +564,41 @@ void launch_split_bs(int grid, int block, hipStream_t st, const TraceGeom& g, co
   switch (block) {
     case 64: k_pt_trace<COMPAT, 64, RING><<<grid, 64, 0, st>>>(g, s, b); break;
     case 128: k_pt_trace<COMPAT, 128, RING><<<grid, 128, 0, st>>>(g, s, b); break;
-    case 512: k_pt_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, s, b); break;
+    case 512:
+      if constexpr (RING <= 16) k_pt_trace<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, s, b);
+      break;
     default: k_pt_trace<COMPAT, 256, RING><<<grid, 256, 0, st>>>(g, s, b); break;
   }
 }
 
+template <bool COMPAT, int RING>
+void launch_paths_bs(int grid, int block, hipStream_t st, const TraceGeom& g, const PtState& s,
+                     const HaltonTables& tab, const HaltonParams& hp, int b, int D) {
+  switch (block) {
+    case 64: k_pt_paths<COMPAT, 64, RING><<<grid, 64, 0, st>>>(g, s, tab, hp, b, D); break;
+    case 128: k_pt_paths<COMPAT, 128, RING><<<grid, 128, 0, st>>>(g, s, tab, hp, b, D); break;
+    case 512:
+      if constexpr (RING <= 16) k_pt_paths<COMPAT, 512, RING><<<grid, 512, 0, st>>>(g, s, tab, hp, b, D);
+      break;
+    default: k_pt_paths<COMPAT, 256, RING><<<grid, 256, 0, st>>>(g, s, tab, hp, b, D); break;
+  }
+}
+
+// One bounce of the frame (fetch 1: the fused kernel; fetch 2: traversal
+// with refill, then shading) or, fetch 3 at bounce 1, every remaining bounce
+// in one launch.
 template <bool COMPAT>
-void launch_bounce(bool first, bool count, bool dyn, int grid, int block, int stack_mode, hipStream_t st,
+void launch_bounce(bool first, bool count, int fetch, int grid, int block, int ring, hipStream_t st,
                    const TraceGeom& g, const PtState& s, const CamDev& cam, const HaltonTables& tab,
                    const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
   // the first bounce's camera rays are coherent: a wave's lanes stay in step
   // in the fused kernel, which refilling them one by one would undo
-  if (dyn && !count && !first) {
-    if (ring_of(stack_mode))
-      launch_split_bs<COMPAT, kRing>(grid, block, st, g, s, b);
-    else
-      launch_split_bs<COMPAT, 0>(grid, block, st, g, s, b);
+  if (fetch == 3 && !count && !first) {
+    with_ring(ring, [&](auto R) { launch_paths_bs<COMPAT, decltype(R)::value>(grid, block, st, g, s, tab, hp, b, D); });
+    return;
+  }
+  if (fetch == 2 && !count && !first) {
+    with_ring(ring, [&](auto R) { launch_split_bs<COMPAT, decltype(R)::value>(grid, block, st, g, s, b); });
     const int sgrid = grid * block / kShadeBlock > 0 ? grid * block / kShadeBlock : 1;
     k_pt_shade<COMPAT><<<sgrid, kShadeBlock, 0, st>>>(g, s, tab, hp, b, D);
     return;
@@ -492,17 +608,14 @@ void launch_bounce(bool first, bool count, bool dyn, int grid, int block, int st
       k_pt_bounce<COMPAT, true, true, 256, 0><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
     else
       k_pt_bounce<COMPAT, false, true, 256, 0><<<grid, 256, 0, st>>>(g, s, cam, tab, hp, spp, b, D, visits);
-  } else if (ring_of(stack_mode)) {
-    if (first)
-      launch_bounce_bs<COMPAT, true, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
-    else
-      launch_bounce_bs<COMPAT, false, kRing>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
-  } else {
-    if (first)
-      launch_bounce_bs<COMPAT, true, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
-    else
-      launch_bounce_bs<COMPAT, false, 0>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+    return;
   }
+  with_ring(ring, [&](auto R) {
+    if (first)
+      launch_bounce_bs<COMPAT, true, decltype(R)::value>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+    else
+      launch_bounce_bs<COMPAT, false, decltype(R)::value>(grid, block, st, g, s, cam, tab, hp, spp, b, D, visits);
+  });
 }
 
 // Milliseconds of one call of fn on the context stream: one warm-up, then the
@@ -545,18 +658,24 @@ int flg_trace_closest(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, f
     int block = cfg && cfg->block_size ? cfg->block_size : ctx->tuned_trace_block;
     if (block == 0) block = 256;
     int smode = cfg && cfg->stack_mode ? cfg->stack_mode : ctx->tuned_trace_stack;
+    const int rdepth = cfg && cfg->stack_mode ? cfg->ring_depth : ctx->tuned_trace_ring;
     if (!valid_block(block) || smode < 0 || smode > 2)
       return set_error(ctx, FLG_ERR_INVALID,
                        "flg_trace_closest: block_size must be 64, 128, 256 or 512, stack_mode 0..2");
+    const int ring = ring_of(smode, rdepth);
+    if (!valid_ring(block, ring))
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "flg_trace_closest: ring_depth must be 0, 4, 8, 16 or 32, with block * ring_depth * 8 "
+                       "<= 65536 bytes of LDS");
     if (n == 0) return FLG_OK;
     if (div_up(n, 64) > (int64_t)INT32_MAX) return set_error(ctx, FLG_ERR_RANGE, "flg_trace_closest: too many rays");
     TraceGeom g = geom_of(bvh);
     {
       ScopedTimer tm(ctx, "trace_closest");
       if (bvh->flags & FLG_NORMAL_COMPAT)
-        launch_trace<true>(block, smode, ctx->stream, g, d_rays, d_hits, n);
+        launch_trace<true>(block, ring, ctx->stream, g, d_rays, d_hits, n);
       else
-        launch_trace<false>(block, smode, ctx->stream, g, d_rays, d_hits, n);
+        launch_trace<false>(block, ring, ctx->stream, g, d_rays, d_hits, n);
     }
     FLG_CHECK_LAUNCH(ctx);
     return FLG_OK;
@@ -615,12 +734,18 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     int bpc = cfg->blocks_per_cu ? cfg->blocks_per_cu : ctx->tuned_pt_bpc;
     if (bpc == 0) bpc = 8;
     const int smode = cfg->stack_mode ? cfg->stack_mode : ctx->tuned_pt_stack;
+    const int rdepth = cfg->stack_mode ? cfg->ring_depth : ctx->tuned_pt_ring;
     int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
     if (fmode == 0) fmode = 1;
-    if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 2)
+    if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 3)
       return set_error(ctx, FLG_ERR_INVALID,
                        "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
-                       "stack_mode and ray_fetch 0..2");
+                       "stack_mode 0..2 and ray_fetch 0..3");
+    const int ring = ring_of(smode, rdepth);
+    if (!valid_ring(tb, ring))
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "flg_pathtrace_diffuse: ring_depth must be 0, 4, 8, 16 or 32, with trace_block * "
+                       "ring_depth * 8 <= 65536 bytes of LDS");
     HaltonParams hp = make_halton_params(cfg->width, cfg->height);
     if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
       return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
@@ -650,18 +775,20 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     const int pgrid = ctx->num_cus * bpc;
     // the whole frame: clear the queue counts, then one fused launch per bounce
     // s.count[0 .. D+1]: queue sizes
-    const bool dyn = fmode == 2;
     auto enqueue = [&](hipStream_t stream, bool timed) {
       k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
       if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
       for (int b = 0; b <= D; ++b) {
         ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");
+        // the camera-ray bounce apart from the incoherent diffuse bounces
+        ScopedTimer tk(timed ? ctx : nullptr, b == 0 ? "pathtrace_first_bounce" : "pathtrace_later_bounces");
         if (compat)
-          launch_bounce<true>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+          launch_bounce<true>(b == 0, count_visits, fmode, pgrid, tb, ring, stream, g, s, cd, tab, hp, cfg->spp,
                               b, D, visits);
         else
-          launch_bounce<false>(b == 0, count_visits, dyn, pgrid, tb, smode, stream, g, s, cd, tab, hp, cfg->spp,
+          launch_bounce<false>(b == 0, count_visits, fmode, pgrid, tb, ring, stream, g, s, cd, tab, hp, cfg->spp,
                                b, D, visits);
+        if (fmode == 3 && !count_visits && b >= 1) break;  // that launch ran every remaining bounce
       }
     };
     if (cfg->use_graph) {
@@ -674,7 +801,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
       put(&cd, sizeof cd);
       put(&tab, sizeof tab);
       put(&hp, sizeof hp);
-      const int32_t misc[8] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0};
+      const int32_t misc[9] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0, ring};
       put(misc, sizeof misc);
       put(&visits, sizeof visits);
       hipGraphExec_t exec = nullptr;
@@ -733,29 +860,35 @@ int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays, 
     if (!ctx || !bvh || n <= 0 || !d_rays || !d_hits || !out)
       return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_trace: bad arguments");
     const int blocks[4] = {64, 128, 256, 512};
-    int best_b = 256, best_s = 1, ncand = 0;
+    const int rings[5] = {0, 4, 8, 16, 32};  // 0: private stack
+    int best_b = 256, best_s = 1, best_r = 0, ncand = 0;
     float best = 1e30f, def = 0.0f;
-    for (int sm = 1; sm <= 2; ++sm)
+    for (int rg : rings)
       for (int b : blocks) {
+        if (!valid_ring(b, rg)) continue;
         flg_trace_cfg c{};
         c.block_size = b;
-        c.stack_mode = sm;
+        c.stack_mode = rg ? 2 : 1;
+        c.ring_depth = rg;
         float ms = 0.0f;
         int st = time_call(ctx, [&] { return flg_trace_closest(ctx, bvh, d_rays, d_hits, n, &c); }, 3, &ms);
         if (st) return st;
         ++ncand;
-        if (b == 256 && sm == 1) def = ms;
+        if (b == 256 && rg == 0) def = ms;
         if (ms < best) {
           best = ms;
           best_b = b;
-          best_s = sm;
+          best_s = c.stack_mode;
+          best_r = rg;
         }
       }
     ctx->tuned_trace_block = best_b;
     ctx->tuned_trace_stack = best_s;
+    ctx->tuned_trace_ring = best_r;
     *out = flg_tuning{};
     out->block_size = best_b;
     out->stack_mode = best_s;
+    out->ring_depth = best_r;
     out->ms = best;
     out->ms_default = def;
     out->candidates = ncand;
@@ -768,47 +901,72 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
   return host_guard(ctx, "flg_autotune_pathtrace", [&]() -> int {
     if (!ctx || !bvh || !cam || !cfg || !out)
       return set_error(ctx, FLG_ERR_INVALID, "flg_autotune_pathtrace: bad arguments");
-    // resident threads per CU: 1024 .. 8192, each split into blocks of 64 .. 512
+    // two stages: ray fetch x stack (private, or an LDS ring of 4..32
+    // entries) at two launch shapes, then every shape for the best of those:
+    // resident threads per CU 1024 .. 8192, in blocks of 64 .. 512
+    struct Cand {
+      int b, bpc, sm, rg, fm;
+    };
+    int n = 0;
+    float best = 1e30f, def = 0.0f;
+    Cand bc{256, 8, 1, 0, 1};
+    auto run = [&](const Cand& k, float* ms) -> int {
+      flg_pt_cfg c = *cfg;
+      c.trace_block = k.b;
+      c.blocks_per_cu = k.bpc;
+      c.stack_mode = k.sm;
+      c.ring_depth = k.rg;
+      c.ray_fetch = k.fm;
+      c.count_visits = 0;
+      c.use_graph = 0;
+      return time_call(
+          ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); }, 2, ms);
+    };
+    auto consider = [&](const Cand& k) -> int {
+      if (!valid_ring(k.b, k.rg) || k.bpc < 1 || k.bpc > 64) return FLG_OK;
+      float ms = 0.0f;
+      int st = run(k, &ms);
+      if (st) return st;
+      ++n;
+      if (k.b == 256 && k.bpc == 8 && k.sm == 1 && k.fm == 1) def = ms;
+      if (ms < best) {
+        best = ms;
+        bc = k;
+      }
+      return FLG_OK;
+    };
+    const int rings[5] = {0, 4, 8, 16, 32};
+    const int shapes[2][2] = {{256, 8}, {128, 64}};
+    for (int fm = 1; fm <= 3; ++fm)
+      for (int rg : rings)
+        for (const auto& sh : shapes) {
+          int st = consider(Cand{sh[0], sh[1], rg ? 2 : 1, rg, fm});
+          if (st) return st;
+        }
+    const Cand stage1 = bc;
     const int blocks[4] = {64, 128, 256, 512};
     const int threads_per_cu[4] = {1024, 2048, 4096, 8192};
-    int best_b = 256, best_bpc = 8, best_s = 1, best_f = 1, n = 0;
-    float best = 1e30f, def = 0.0f;
-    for (int fm = 1; fm <= 2; ++fm)
-    for (int sm = 1; sm <= 2; ++sm)
-      for (int b : blocks)
-        for (int t : threads_per_cu) {
-          flg_pt_cfg c = *cfg;
-          c.trace_block = b;
-          c.blocks_per_cu = t / b;
-          c.stack_mode = sm;
-          c.ray_fetch = fm;
-          c.count_visits = 0;
-          c.use_graph = 0;
-          if (c.blocks_per_cu > 64) continue;
-          float ms = 0.0f;
-          int st = time_call(
-              ctx, [&] { return flg_pathtrace_diffuse(ctx, bvh, cam, &c, nullptr, nullptr, nullptr, nullptr); },
-              2, &ms);
-          if (st) return st;
-          ++n;
-          if (b == 256 && c.blocks_per_cu == 8 && sm == 1 && fm == 1) def = ms;
-          if (ms < best) {
-            best = ms;
-            best_b = b;
-            best_bpc = c.blocks_per_cu;
-            best_s = sm;
-            best_f = fm;
-          }
-        }
-    ctx->tuned_pt_block = best_b;
-    ctx->tuned_pt_bpc = best_bpc;
-    ctx->tuned_pt_stack = best_s;
-    ctx->tuned_pt_fetch = best_f;
+    for (int b : blocks)
+      for (int t : threads_per_cu) {
+        if ((b == 256 && t / b == 8) || (b == 128 && t / b == 64)) continue;  // timed in stage one
+        int st = consider(Cand{b, t / b, stage1.sm, stage1.rg, stage1.fm});
+        if (st) return st;
+      }
+    if (def == 0.0f) {  // the untuned shape, for reference
+      int st = run(Cand{256, 8, 1, 0, 1}, &def);
+      if (st) return st;
+    }
+    ctx->tuned_pt_block = bc.b;
+    ctx->tuned_pt_bpc = bc.bpc;
+    ctx->tuned_pt_stack = bc.sm;
+    ctx->tuned_pt_ring = bc.rg;
+    ctx->tuned_pt_fetch = bc.fm;
     *out = flg_tuning{};
-    out->block_size = best_b;
-    out->blocks_per_cu = best_bpc;
-    out->stack_mode = best_s;
-    out->ray_fetch = best_f;
+    out->block_size = bc.b;
+    out->blocks_per_cu = bc.bpc;
+    out->stack_mode = bc.sm;
+    out->ring_depth = bc.rg;
+    out->ray_fetch = bc.fm;
     out->ms = best;
     out->ms_default = def;
     out->candidates = n;

@@ -52,13 +52,19 @@ class DiffVolume:
             self.proposal = Volume(self.ctx, density.detach(), self.bounds, 1.0, 0.0, self.g)
 
 
-def _params(s, albedo, g, majorant):
+def _params(s, albedo, g, majorant, lookups=None):
     p = N.DiffParams()
     p.sigma_t_scale = float(s)
     p.albedo = float(albedo)
     p.g = float(g)
     p.majorant = float(majorant)
+    p.d_lookups = lookups.data_ptr() if lookups is not None else None
     return p
+
+
+def _check_counter(ctx, lookups):
+    if lookups is not None:
+        check_buffer(lookups, "lookups", torch.int64, ctx.device, min_numel=1)
 
 
 def _cfg(w, h, spp, depth, schedule=0):
@@ -83,9 +89,10 @@ def new_tape(ctx, w, h, spp, depth):
 
 
 def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0,
-            tape=None):
+            tape=None, lookups=None):
     """Image and per-path radiance; with `tape` (a buffer from new_tape) also
-    the walk tape that the adjoint of this call can read."""
+    the walk tape that the adjoint of this call can read; with `lookups` (an
+    int64 CUDA tensor) the call adds its grid lookups to lookups[0]."""
     ctx = dv.ctx
     dev = torch.device("cuda", ctx.device)
     if out is None:
@@ -93,16 +100,17 @@ def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=No
     if lpath is None:
         lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
     _check_forward(ctx, w, h, spp, depth, out, lpath, tape)
+    _check_counter(ctx, lookups)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_diff_forward(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                                ctypes.byref(_cfg(w, h, spp, depth, schedule)),
-                                               ctypes.byref(_params(s, albedo, dv.g, majorant)),
+                                               ctypes.byref(_params(s, albedo, dv.g, majorant, lookups)),
                                                _ptr(out), _ptr(lpath), _ptr(tape)))
     return out, lpath
 
 
 def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, grad=None, galpha=None,
-            schedule=0, tape=None):
+            schedule=0, tape=None, lookups=None):
     """Adds dLoss/dDensity and dLoss/dAlbedo into grad and galpha (zeros when
     not given); lpath and tape (or None) from the forward of the same
     arguments."""
@@ -115,10 +123,11 @@ def adjoint(dv, scene, w, h, spp, depth, s, albedo, majorant, lpath, dl_drgb, gr
         galpha = torch.zeros(1, dtype=torch.float32, device=dev)
     dl = dl_drgb.contiguous()
     _check_adjoint(ctx, w, h, spp, depth, lpath, dl, grad, nz * ny * nx, galpha, tape)
+    _check_counter(ctx, lookups)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_adjoint(ctx.h, dv.vol.h, prop, ctypes.byref(scene),
                                           ctypes.byref(_cfg(w, h, spp, depth, schedule)),
-                                          ctypes.byref(_params(s, albedo, dv.g, majorant)),
+                                          ctypes.byref(_params(s, albedo, dv.g, majorant, lookups)),
                                           _ptr(lpath), _ptr(tape), _ptr(dl), _ptr(grad), _ptr(galpha)))
     return grad, galpha
 
@@ -235,10 +244,17 @@ class VolumeRender(torch.autograd.Function):
         return grad, galpha.reshape(()), None, None, None, None, None, None, None, None
 
 
-def bench_adjoint(ctx, timed, sync):
-    """128^3 grid initialised to 0.5, 256x256, 16 spp, L2 loss against
-    a target rendered from a different fBm grid; point light and environment
-    as in the 256^3 volume render, albedo 0.8, g 0.3, sigma_t = 8 density."""
+def bench_adjoint(ctx, timed, sync, state="uniform"):
+    """128^3 grid, 256x256, 16 spp, depth 32, L2 loss against a target
+    rendered from an fBm grid (seed 2); point light and environment as in the
+    256^3 volume render, albedo 0.8, g 0.3, sigma_t = 8 density.  The
+    optimised grid theta is `state`: "uniform" (0.5 everywhere, the start of
+    the optimisation: every ratio-tracking factor 1 - s theta / mu is 0 and
+    walks end at once) or "fbm" (a different fBm grid, seed 3: a non-uniform
+    state with walks of ordinary length).  Returns timings of the plain
+    forward, the differentiable forward and the adjoint at both schedules,
+    one fwd+adjoint iteration, the library's kernel times and the grid
+    lookups per path of each pass."""
     from . import api
 
     dev = torch.device("cuda", ctx.device)
@@ -249,9 +265,12 @@ def bench_adjoint(ctx, timed, sync):
     target_grid = torch.from_numpy(api.gen_fbm_grid(128, seed=2)).to(dev)
     tv = DiffVolume(ctx, target_grid)
     target, _ = forward(tv, scene, W, H, SPP, DEPTH, S, 0.8, S * float(target_grid.max()))
-    theta = torch.full((128, 128, 128), 0.5, dtype=torch.float32, device=dev)
+    if state == "uniform":
+        theta = torch.full((128, 128, 128), 0.5, dtype=torch.float32, device=dev)
+    else:
+        theta = torch.from_numpy(api.gen_fbm_grid(128, seed=3)).to(dev)
     dv = DiffVolume(ctx, theta)
-    mu = S * 0.5
+    mu = S * float(theta.max())
     plain = Volume(ctx, theta, (0, 0, 0, 1, 1, 1), 0.8 * S, 0.2 * S, 0.3)
     img_plain = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     lpath = torch.empty(W * H * SPP, dtype=torch.float32, device=dev)
@@ -268,10 +287,11 @@ def bench_adjoint(ctx, timed, sync):
                                        schedule=sc, tape=tape), 5, 2, sync) for sc in (1, 2)}
     dl = 2.0 * (img - target) / img.numel()
 
-    def adj(sc, tp=tape):
+    def adj(sc, tp=tape, lookups=None):
         grad.zero_()
         galpha.zero_()
-        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, dl, grad, galpha, schedule=sc, tape=tp)
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, dl, grad, galpha, schedule=sc, tape=tp,
+                lookups=lookups)
 
     t_adj = {sc: timed(lambda: adj(sc), 5, 2, sync) for sc in (1, 2)}
     t_adj_notape = {sc: timed(lambda: adj(sc, None), 5, 2, sync) for sc in (1, 2)}
@@ -287,14 +307,33 @@ def bench_adjoint(ctx, timed, sync):
         adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, d, grad, galpha, schedule=s_adj, tape=tape)
 
     t_iter = timed(iteration, 5, 2, sync)
-    return {"adjoint_config": "128^3 grid, 256x256, 16 spp, depth 32",
-            "plain_forward_ms": round(t_plain[s_plain] * 1e3, 3),
-            "plain_forward_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_plain.items()},
-            "diff_forward_ms": round(t_fwd[s_fwd] * 1e3, 3),
-            "diff_forward_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_fwd.items()},
-            "adjoint_ms": round(t_adj[s_adj] * 1e3, 3),
-            "adjoint_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_adj.items()},
-            "adjoint_without_tape_ms_by_schedule": {str(k): round(v * 1e3, 3) for k, v in t_adj_notape.items()},
+    # kernel times of one iteration (HIP events on the library's stream)
+    ctx.profiling(True)
+    ctx.reset_profiling()
+    for _ in range(3):
+        iteration()
+    sync()
+    k_fwd, n_fwd = ctx.kernel_time("diff_forward")
+    k_adj, n_adj = ctx.kernel_time("diff_adjoint")
+    ctx.profiling(False)
+    # grid lookups per path of each pass (a separate counting pass)
+    cnt = torch.zeros(2, dtype=torch.int64, device=dev)
+    forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath, schedule=s_fwd, tape=tape,
+            lookups=cnt[0:1])
+    adj(s_adj, tape, cnt[1:2])
+    sync()
+    P = W * H * SPP
+    return {"state": state, "config": "128^3 grid, 256x256, 16 spp, depth 32",
+            "plain_forward_ms": t_plain[s_plain] * 1e3,
+            "plain_forward_ms_by_schedule": {str(k): v * 1e3 for k, v in t_plain.items()},
+            "diff_forward_ms": t_fwd[s_fwd] * 1e3,
+            "diff_forward_ms_by_schedule": {str(k): v * 1e3 for k, v in t_fwd.items()},
+            "adjoint_ms": t_adj[s_adj] * 1e3,
+            "adjoint_ms_by_schedule": {str(k): v * 1e3 for k, v in t_adj.items()},
+            "adjoint_without_tape_ms_by_schedule": {str(k): v * 1e3 for k, v in t_adj_notape.items()},
             "walk_tape_bytes": tape.numel(),
-            "fwd_adjoint_ms_per_iter": round(t_iter * 1e3, 3),
-            "fwd_adjoint_vs_plain": round(t_iter / t_plain[s_plain], 3)}
+            "fwd_adjoint_ms_per_iter": t_iter * 1e3,
+            "fwd_adjoint_vs_plain": t_iter / t_plain[s_plain],
+            "kernel_forward_ms": k_fwd / max(n_fwd, 1), "kernel_adjoint_ms": k_adj / max(n_adj, 1),
+            "paths": P, "lookups_forward": int(cnt[0]), "lookups_adjoint": int(cnt[1]),
+            "majorant": mu}

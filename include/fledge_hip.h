@@ -180,9 +180,12 @@ typedef struct flg_hit {
 typedef struct flg_trace_cfg {
   int32_t block_size; /* 64, 128, 256 or 512 threads; 0: the context's tuned
                          value (flg_autotune_trace), else 256 */
-  int32_t stack_mode; /* traversal stack: 1 private memory, 2 the 8 most recent
-                         entries per ray in LDS; 0: tuned value, else 1 */
-  int32_t reserved[6];
+  int32_t stack_mode; /* traversal stack: 1 private memory, 2 the ring_depth
+                         most recent entries per ray in LDS; 0: tuned value
+                         (with its ring depth), else 1 */
+  int32_t ring_depth; /* stack_mode 2: 4, 8, 16 or 32 entries in LDS (0: 8);
+                         block_size * ring_depth * 8 must not exceed 65536 */
+  int32_t reserved[5];
 } flg_trace_cfg;
 
 /* Closest hit of n rays; the visit order and tie rule are those of
@@ -274,8 +277,13 @@ typedef struct flg_pt_cfg {
                             the first in two launches, a traversal whose lanes
                             take the next entry whenever enough of a wave's
                             lanes are idle (persistent threads), then the
-                            shading; 0: tuned value, else 1.  Same results. */
-  int32_t reserved[2];
+                            shading; 3: every bounce after the first in one
+                            launch, each lane following its path to the end
+                            and taking a new one when enough of its wave's
+                            lanes are idle; 0: tuned value, else 1.  Same
+                            results. */
+  int32_t ring_depth;    /* as flg_trace_cfg.ring_depth */
+  int32_t reserved[1];
 } flg_pt_cfg;
 FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* cam,
                                   const flg_pt_cfg* cfg, int32_t* d_nseg, int32_t* d_seg_id,
@@ -286,10 +294,10 @@ FLG_API int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_ca
  * the fastest in the context, where calls whose block_size / trace_block /
  * blocks_per_cu are 0 pick it up.  The tuned calls run the real work: their
  * outputs are written as by a normal call.  Tuned: threads per block, resident
- * blocks per CU and ray fetch (path tracer) and where the traversal stack
- * lives.  Stack
- * depth (64, the bound on the tree depth) and node width (binary, which the
- * reference's visiting order needs) are fixed. */
+ * blocks per CU and ray fetch (path tracer), where the traversal stack lives
+ * and how many of its entries stay in LDS (4, 8, 16 or 32).  The stack's
+ * total depth (64, the bound on the tree depth) and node width (binary, which
+ * the reference's visiting order needs) are fixed. */
 typedef struct flg_tuning {
   int32_t block_size;    /* chosen threads per block */
   int32_t blocks_per_cu; /* chosen persistent blocks per CU (path tracer; 0 for trace) */
@@ -297,8 +305,8 @@ typedef struct flg_tuning {
   float ms_default;      /* one call with the untuned shape (256 threads, 8 blocks per CU) */
   int32_t candidates;    /* shapes timed */
   int32_t stack_mode;    /* chosen traversal stack (1 private, 2 LDS ring) */
-  int32_t ray_fetch;     /* chosen ray fetch of the path tracer (1 or 2; 0 for trace) */
-  int32_t reserved[1];
+  int32_t ray_fetch;     /* chosen ray fetch of the path tracer (1..3; 0 for trace) */
+  int32_t ring_depth;    /* chosen LDS ring depth (stack_mode 2), else 0 */
 } flg_tuning;
 FLG_API int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
                                flg_hit* d_hits, int64_t n, flg_tuning* out);
@@ -355,6 +363,7 @@ typedef struct flg_render_cfg {
 FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
 
+
 /* Surfaces and media together: VolPathIntegrator::Li (integrator.cpp:407-510)
  * for the closed mesh `bvh`, whose triangles are index-matched interfaces
  * (no material), with `vol`'s medium inside it, filling the mesh's bounding
@@ -390,6 +399,9 @@ typedef struct flg_diff_params {
   float albedo;        /* single-scatter albedo alpha, > 0 */
   float g;             /* Henyey-Greenstein asymmetry */
   float majorant;      /* fixed majorant per unit length, > 0 */
+  unsigned long long* d_lookups; /* NULL, or a device counter to which the call
+                                    adds its density-grid lookups (a
+                                    measurement pass; slower) */
 } flg_diff_params;
 
 /* Bytes of the walk tape of flg_volpath_diff_forward for cfg:

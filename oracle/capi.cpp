@@ -242,6 +242,25 @@ void orc_volpath_render(const float* grid, int32_t nx, int32_t ny, int32_t nz, c
   svol_render(sc, rgb, nthreads, lookups, paths);
 }
 
+// orc_volpath_render over the pixels 0, stride, 2 stride, ... only (the CPU
+// baseline's bounded sample); the other pixels of rgb are left untouched.
+void orc_volpath_render_strided(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
+                                const float* medium3, const float* lights7, const float* cam10, int32_t W,
+                                int32_t H, int32_t spp, int32_t max_depth, int64_t stride, float* rgb,
+                                int nthreads, uint64_t* lookups, uint64_t* paths) {
+  VolScene sc;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
+  sc.point.present = lights7[0] != 0.0f;
+  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
+  sc.point.intensity = lights7[4];
+  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  Camera cam = make_camera(cam10);
+  sc.cam = &cam;
+  sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
+  sc.pix_stride = stride < 1 ? 1 : stride;
+  svol_render(sc, rgb, nthreads, lookups, paths);
+}
+
 // The reference's render loop and film (film.cpp); arguments as
 // orc_volpath_render.  task_L / task_uv (may be NULL): per pixel the spp - 1
 // splatted samples (radiance; film position x, y).

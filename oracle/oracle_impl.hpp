@@ -119,6 +119,9 @@ struct VolScene {
   PointLight point;
   Camera* cam = nullptr;
   int width = 0, height = 0, spp = 1, max_depth = 1;
+  // pixels rendered: 0, pix_stride, 2 pix_stride, ... (a bounded, evenly
+  // spread sample of the image for timing; 1 = every pixel)
+  int64_t pix_stride = 1;
 };
 
 bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, float* t_max);

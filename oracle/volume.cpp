@@ -402,7 +402,7 @@ void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups
     th.emplace_back([&, t] {
       Halton hs((uint64_t)sc.spp, W, H);
       while (true) {
-        int64_t pix = next.fetch_add(1);
+        int64_t pix = next.fetch_add(1) * sc.pix_stride;
         if (pix >= npix) break;
         int x = (int)(pix % W), y = (int)(pix / W);
         float color = 0.0f;
@@ -423,7 +423,7 @@ void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups
   uint64_t tot = 0;
   for (auto v : lk) tot += v;
   if (lookups_out) *lookups_out = tot;
-  if (paths_out) *paths_out = (uint64_t)npix * (uint64_t)sc.spp;
+  if (paths_out) *paths_out = (uint64_t)((npix + sc.pix_stride - 1) / sc.pix_stride) * (uint64_t)sc.spp;
 }
 
 }  // namespace orc

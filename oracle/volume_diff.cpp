@@ -61,6 +61,7 @@ struct DiffScene {
   PointLight point;
   Camera* cam;
   int W, H, spp, max_depth;
+  int64_t pix_stride = 1;  // pixels 0, stride, 2 stride, ... (1: all)
 };
 
 struct Tri {  // lookup record: base voxel, weights, densities
@@ -417,7 +418,7 @@ void run_paths(const DiffScene& sc, PathFn path_fn, double* lpath, double* img, 
       if (adj) gpart[t].assign(nvox, 0.0);
       Halton hs((uint64_t)spp, W, H);
       while (true) {
-        int64_t pix = next.fetch_add(1);
+        int64_t pix = next.fetch_add(1) * sc.pix_stride;
         if (pix >= npix) break;
         int x = (int)(pix % W), y = (int)(pix / W);
         double acc = 0;
@@ -499,6 +500,27 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
   Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
              v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
   setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  run_paths(
+      sc,
+      [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {
+        return diff_path(sc, hs, p, o, d, delta, Lt, g, ga, lk);
+      },
+      lpath, img, dldpix, grad, galpha, nthreads, lookups);
+}
+
+// orc_diff_render over the pixels 0, stride, 2 stride, ... only (the CPU
+// baseline's bounded sample): lpath and img entries of other pixels are left
+// untouched; the adjoint's gradient covers the sampled pixels.
+void orc_diff_render_strided(const double* theta, const float* theta_p, int32_t nx, int32_t ny, int32_t nz,
+                             const float* bounds6, const float* params4, const float* lights7,
+                             const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
+                             int64_t stride, double* lpath, double* img, const double* dldpix, double* grad,
+                             double* galpha, int nthreads, uint64_t* lookups) {
+  DiffScene sc;
+  Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
+             v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  sc.pix_stride = stride < 1 ? 1 : stride;
   run_paths(
       sc,
       [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {

@@ -41,6 +41,10 @@ _SIGS = {
     "orc_diffuse_render": (None, [P, i32, P, i32, i32, i32, i32, i64, i64, P, P, P, ctypes.c_int, P]),
     "orc_offset_ray_origin": (None, [P, P, P, P]),
     "orc_camera_generate": (None, [P, f32, f32, i32, i32, P, P]),
+    "orc_volpath_render_strided": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, i64, P,
+                                          ctypes.c_int, P, P]),
+    "orc_diff_render_strided": (None, [P, P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, i64, P, P, P, P, P,
+                                       ctypes.c_int, P]),
     "orc_diffuse_sample": (None, [P, P, P, i64, P, P]),
     "orc_shading_frame": (None, [P, i64, P]),
     "orc_volpath_render_film": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P]),
@@ -337,3 +341,36 @@ def halton_get(resx, resy, spp, px, py, sample, n):
     out = np.zeros(n, np.float32)
     lib().orc_halton_get(resx, resy, spp, px, py, sample, n, _p(out))
     return out
+
+
+def diff_render_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, stride,
+                        bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
+    """diff_render over the pixels 0, stride, 2 stride, ... only: (per-path
+    radiance, per-pixel mean); entries of other pixels stay 0."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    theta_p = np.ascontiguousarray(theta_p, np.float32)
+    nz, ny, nx = theta_p.shape
+    lp = np.zeros(W * H * spp, np.float64)
+    img = np.zeros(W * H, np.float64)
+    lib().orc_diff_render_strided(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                                  _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, stride, _p(lp),
+                                  _p(img), None, None, None, nthreads or NTHREADS, None)
+    return lp, img.reshape(H, W)
+
+
+def diff_grad_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, stride, lpath, dldpix,
+                      bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
+    """diff_grad over the pixels 0, stride, 2 stride, ... only."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    theta_p = np.ascontiguousarray(theta_p, np.float32)
+    nz, ny, nx = theta_p.shape
+    grad = np.zeros(nx * ny * nz, np.float64)
+    ga = np.zeros(1, np.float64)
+    dl = np.ascontiguousarray(dldpix, np.float64).ravel()
+    lp = np.ascontiguousarray(lpath, np.float64)
+    lib().orc_diff_render_strided(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
+                                  _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, stride, _p(lp), None,
+                                  _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None)
+    return grad.reshape(nz, ny, nx), float(ga[0])

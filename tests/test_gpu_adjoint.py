@@ -124,14 +124,21 @@ def test_adjoint_central_finite_differences(ctx):
     assert abs(ga - fa) <= 1e-3 * abs(fa) + 1e-6
 
 
-def test_adjoint_grid128_full_size(ctx):
-    """Full size: 128^3 grid at 0.5, 256x256, 16 spp; L2 loss against a
-    target from a different fBm grid.  GPU gradient vs the FP64 gradient."""
+@pytest.mark.parametrize("state", ["uniform", "fbm"])
+def test_adjoint_grid128_full_size(ctx, state):
+    """Full size: 128^3 grid, 256x256, 16 spp; L2 loss against a target from
+    an fBm grid (seed 2).  The optimised grid is at the start state (0.5
+    everywhere: every ratio-tracking factor is 0, walks end at once) or at a
+    non-uniform state (another fBm grid, seed 3: walks of ordinary length).
+    GPU gradient vs the FP64 gradient."""
     W = H = 256
     spp, depth = 16, 32
-    theta = np.full((128, 128, 128), 0.5, np.float32)
+    if state == "uniform":
+        theta = np.full((128, 128, 128), 0.5, np.float32)
+    else:
+        theta = api.gen_fbm_grid(128, seed=3)
     target_grid = api.gen_fbm_grid(128, seed=2)
-    mu = S * 0.5
+    mu = S * float(theta.max())
     tv = diff.DiffVolume(ctx, torch.from_numpy(target_grid).cuda(), g=G)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
     scene = api.vol_scene(cam)

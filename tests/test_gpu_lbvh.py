@@ -186,17 +186,20 @@ def _same_frame(a, b):
     assert torch.equal(a["seg_t"][mask].view(torch.int32), b["seg_t"][mask].view(torch.int32))
 
 
-@pytest.mark.parametrize("block,bpc,smode,fetch", [(64, 16, 1, 1), (128, 4, 2, 1), (512, 2, 1, 2),
-                                                    (256, 32, 2, 2), (512, 16, 2, 1), (64, 64, 2, 2),
-                                                    (256, 8, 1, 1), (128, 16, 1, 2)])
-def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode, fetch):
-    """Every launch shape, stack policy and ray fetch the autotuner may pick
-    gives the same frame (segment ids and t bit for bit), equal to the oracle."""
+@pytest.mark.parametrize("block,bpc,smode,ring,fetch", [
+    (64, 16, 1, 0, 1), (128, 4, 2, 8, 1), (512, 2, 1, 0, 2), (256, 32, 2, 8, 2), (512, 16, 2, 16, 1),
+    (64, 64, 2, 32, 2), (256, 8, 1, 0, 1), (128, 16, 1, 0, 2), (256, 8, 2, 4, 3), (128, 64, 2, 8, 3),
+    (512, 4, 1, 0, 3), (64, 32, 2, 32, 3), (256, 16, 2, 32, 3), (512, 8, 2, 16, 3), (128, 8, 2, 4, 2),
+    (256, 4, 2, 16, 2), (64, 8, 2, 4, 1), (256, 2, 2, 32, 1)])
+def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode, ring, fetch):
+    """Every launch shape, stack policy (private, or an LDS ring of 4 to 32
+    entries) and ray fetch the autotuner may pick gives the same frame
+    (segment ids and t bit for bit), equal to the oracle."""
     g, o, cam10, cam = _octa_case(ctx)
     W, H, spp, D = 96, 64, 2, 6
     ref = g.pathtrace(cam, W, H, spp, D, record=True, ray_fetch=1)
     got = g.pathtrace(cam, W, H, spp, D, record=True, trace_block=block, blocks_per_cu=bpc,
-                      stack_mode=smode, ray_fetch=fetch)
+                      stack_mode=smode, ray_fetch=fetch, ring_depth=ring)
     _same_frame(ref, got)
     nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, W * H * spp)
     assert got["segments"] == nrays
@@ -206,14 +209,15 @@ def test_pathtrace_launch_shapes_identical(ctx, block, bpc, smode, fetch):
     assert np.array_equal(got["seg_t"].cpu().numpy()[mask].view(np.uint32), ts[mask].view(np.uint32))
 
 
-@pytest.mark.parametrize("block", [64, 256, 512])
-def test_trace_lds_stack_matches_oracle(ctx, block):
-    """Closest hit with the LDS ring stack on the 65,536-triangle soup and on random
-    rays inside a closed mesh: ids and t bit-equal to the oracle."""
+@pytest.mark.parametrize("block,ring", [(64, 8), (256, 8), (512, 8), (128, 4), (512, 16), (256, 32), (64, 32)])
+def test_trace_lds_stack_matches_oracle(ctx, block, ring):
+    """Closest hit with the LDS ring stack (4 to 32 entries in LDS) on the
+    65,536-triangle soup and on random rays inside a closed mesh: ids and t
+    bit-equal to the oracle."""
     v, i = api.gen_soup(65536)
     g, o = _build_both(ctx, v, i)
     rays = orc.camera_rays(CAM0, 512, 512)
-    gid, gt = api.hits_as_numpy(g.trace(_dev(rays), block_size=block, stack_mode=2))
+    gid, gt = api.hits_as_numpy(g.trace(_dev(rays), block_size=block, stack_mode=2, ring_depth=ring))
     oid, ot = o.trace(rays)
     assert np.array_equal(gid, oid) and np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
     vs, ins = api.gen_octasphere(7, 0.1)
@@ -224,7 +228,7 @@ def test_trace_lds_stack_matches_oracle(ctx, block):
     d = rng.normal(size=(100000, 3))
     r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
     r[:, 3] = np.inf
-    gid, gt = api.hits_as_numpy(gs.trace(_dev(r), block_size=block, stack_mode=2))
+    gid, gt = api.hits_as_numpy(gs.trace(_dev(r), block_size=block, stack_mode=2, ring_depth=ring))
     oid, ot = os_.trace(r)
     assert (oid >= 0).all()
     assert np.array_equal(gid, oid) and np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
@@ -257,13 +261,15 @@ def test_autotuner_choices_give_identical_results(ctx):
     ref = g.pathtrace(cam, W, H, spp, D, record=True)
     t = g.autotune(cam, W, H, spp, D)
     assert t["block_size"] in (64, 128, 256, 512) and 1 <= t["blocks_per_cu"] <= 64
-    assert t["candidates"] >= 48 and t["stack_mode"] in (1, 2) and t["ray_fetch"] in (1, 2)
+    assert t["candidates"] >= 40 and t["stack_mode"] in (1, 2) and t["ray_fetch"] in (1, 2, 3)
+    assert t["ring_depth"] in ((0,) if t["stack_mode"] == 1 else (4, 8, 16, 32))
     assert t["ms"] <= t["ms_default"] * 1.0001
     _same_frame(ref, g.pathtrace(cam, W, H, spp, D, record=True))  # uses the tuned shape
     rays = api.camera_rays(ctx, cam, 128, 128)
     hits = torch.empty((128 * 128, 2), dtype=torch.int32, device="cuda")
     tt = g.autotune_trace(rays, hits)
-    assert tt["block_size"] in (64, 128, 256, 512)
+    assert tt["block_size"] in (64, 128, 256, 512) and tt["candidates"] >= 18
+    assert tt["ring_depth"] in ((0,) if tt["stack_mode"] == 1 else (4, 8, 16, 32))
     gh = api.hits_as_numpy(g.trace(rays))
     oi, ot = o.trace(rays.cpu().numpy())
     assert np.array_equal(gh[0], oi) and np.array_equal(gh[1].view(np.uint32), ot.view(np.uint32))
