@@ -261,7 +261,7 @@ def test_autotuner_choices_give_identical_results(ctx):
     ref = g.pathtrace(cam, W, H, spp, D, record=True)
     t = g.autotune(cam, W, H, spp, D)
     assert t["block_size"] in (64, 128, 256, 512) and 1 <= t["blocks_per_cu"] <= 64
-    assert t["candidates"] >= 40 and t["stack_mode"] in (1, 2) and t["ray_fetch"] in (1, 2, 3)
+    assert t["candidates"] >= 30 and t["stack_mode"] in (1, 2) and t["ray_fetch"] in (1, 2, 3)
     assert t["ring_depth"] in ((0,) if t["stack_mode"] == 1 else (4, 8, 16, 32))
     assert t["ms"] <= t["ms_default"] * 1.0001
     _same_frame(ref, g.pathtrace(cam, W, H, spp, D, record=True))  # uses the tuned shape
