@@ -133,6 +133,7 @@ _SIGS = {
     "flg_volume_destroy": (c_int, [c_vp]),
     "flg_volume_update": (c_int, [c_vp, c_vp, c_vp]),
     "flg_volpath_render": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp, P(c_u64)]),
+    "flg_volpath_render_noalloc": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp]),
     "flg_volpath_mesh_render": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp, P(c_u64)]),
     "flg_volpath_diff_tape_bytes": (c_i64, [P(RenderCfg)]),
     "flg_volpath_diff_forward": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), P(DiffParams),

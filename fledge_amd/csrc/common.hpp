@@ -185,6 +185,9 @@ struct flg_ctx {
   // captured bounce loops, keyed by the bytes of their launch arguments
   hipStream_t capture_stream = nullptr;
   std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+  // this device's address of the library's fixed scratch array (the scratch
+  // of flg_volpath_render_noalloc), set by flg_ctx_create
+  void* noalloc_scratch = nullptr;
   // output blocks of destroyed BVHs, kept for later builds (lbvh.hip)
   std::vector<std::pair<void*, size_t>> bvh_pool;
 };
@@ -217,6 +220,14 @@ int host_guard(flg_ctx* ctx, const char* what, F&& body) noexcept {
 void resolve_timers(flg_ctx* ctx);
 int check_hip(flg_ctx* ctx, hipError_t e, const char* what);
 void* scratch(flg_ctx* ctx, const char* key, size_t bytes, int* status);
+
+// The fixed per-GPU scratch of flg_volpath_render_noalloc: 64 MiB of path
+// radiance (a 512 x 512 frame at 64 spp) plus a page of counters.
+constexpr size_t kNoAllocScratchBytes = ((size_t)64 << 20) + 4096;
+constexpr int kMaxNoAllocDevices = 64;
+// The device address of that array on `device` (which must be current),
+// looked up once per device; FLG_ERR_HIP if the lookup fails.
+int noalloc_scratch_address(int device, void** out);
 
 // Times a kernel launch with HIP events on the context stream when
 // profiling is enabled.

@@ -141,6 +141,18 @@ int flg_ctx_create(int device, void* stream, flg_ctx** out) {
     ctx->stream = (hipStream_t)stream;  // NULL: the device's default stream
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    // set-up that later calls must not repeat: the Halton tables, and this
+    // device's address of the fixed scratch array (no allocation later in
+    // flg_volpath_render_noalloc)
+    HaltonTables tab;
+    int st = get_halton_tables(ctx, &tab);
+    if (st == FLG_OK) st = noalloc_scratch_address(device, &ctx->noalloc_scratch);
+    if (st != FLG_OK) {
+      (void)hipStreamSynchronize(ctx->stream);
+      release_halton_tables(ctx);
+      delete ctx;
+      return st;
+    }
     *out = ctx;
     return FLG_OK;
   });

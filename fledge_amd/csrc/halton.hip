@@ -77,6 +77,16 @@ int get_halton_tables(flg_ctx* ctx, HaltonTables* out) {
   return FLG_OK;
 }
 
+bool halton_tables_ready(flg_ctx* ctx, HaltonTables* out) {
+  auto it = stores().find(ctx);
+  if (it == stores().end() || !it->second.prime) return false;
+  out->prime = it->second.prime;
+  out->inv = it->second.inv;
+  out->nprimes = it->second.n;
+  out->overflow = it->second.overflow;
+  return true;
+}
+
 void release_halton_tables(flg_ctx* ctx) {
   auto it = stores().find(ctx);
   if (it == stores().end()) return;

@@ -35,6 +35,9 @@ struct HaltonParams {
 };
 
 int get_halton_tables(flg_ctx* ctx, HaltonTables* out);
+// The context's tables if they exist (built by flg_ctx_create), without
+// building or allocating anything; false otherwise.
+bool halton_tables_ready(flg_ctx* ctx, HaltonTables* out);
 void release_halton_tables(flg_ctx* ctx);
 HaltonParams make_halton_params(int resx, int resy);
 constexpr uint32_t kMaxHaltonIndex = 1u << 24;

@@ -9,6 +9,7 @@
 // commitSamples, film.hpp:143-174).
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "volume.hpp"
 
@@ -396,13 +397,18 @@ VolDev make_voldev(const flg_volume* vol) {
 }
 
 int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L,
-                const float* scene_bounds) {
+                const float* scene_bounds, bool may_build) {
   *L = LightsDev{};
   L->has_point = sc->has_point != 0;
   L->point_pos = v3(sc->point_pos[0], sc->point_pos[1], sc->point_pos[2]);
   L->point_intensity = sc->point_intensity;
   L->env.present = sc->has_env != 0;
   if (L->env.present) {
+    if (!may_build && (!vol->env || vol->env_radiance != sc->env_radiance))
+      return set_error(ctx, FLG_ERR_INVALID,
+                       "the volume's environment-light table is not built for radiance " +
+                           std::to_string(sc->env_radiance) +
+                           ": render the scene once with flg_volpath_render first");
     int st = build_env(ctx, vol, sc->env_radiance);
     if (st) return st;
     const float* e = vol->env;
@@ -438,6 +444,138 @@ int refresh_max(flg_ctx* ctx, flg_volume* vol, const float* d_density) {
   float m = -3.402823466e+38f;
   for (float x : h) m = smax(m, x);
   vol->max_density = m;
+  return FLG_OK;
+}
+
+// ------------------------------------------------------ frame render ---
+// The per-frame work of flg_volpath_render and flg_volpath_render_noalloc.
+// Scratch: [256 B counters][npaths radiance floats][pad to 8 B][npaths film
+// positions], from the context's grow-only pool, or (noalloc) from the
+// library's fixed per-GPU array, in which case nothing is allocated: the
+// Halton tables come from flg_ctx_create and the environment table must
+// already hold the scene's radiance.
+static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene, const flg_render_cfg* cfg,
+                          float* d_rgb, uint64_t* h_stats, bool noalloc) {
+  const char* what = noalloc ? "flg_volpath_render_noalloc" : "flg_volpath_render";
+  if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 || cfg->spp <= 0 ||
+      cfg->max_depth < 1)
+    return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": bad arguments");
+  if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
+    return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": too many paths");
+  if (cfg->film < 0 || cfg->film > 1) return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": film must be 0 or 1");
+  const int sched = cfg->schedule ? cfg->schedule : 2;
+  if (sched < 1 || sched > 2)
+    return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": schedule must be 0, 1 or 2");
+  HaltonParams hp = make_halton_params(cfg->width, cfg->height);
+  if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
+    return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": spp * Halton stride must stay below 2^24");
+  const bool film = cfg->film == 1;
+  const int64_t npix = (int64_t)cfg->width * cfg->height;
+  // film mode: spp - 1 splatted samples per pixel (the first task also runs
+  // the pixel-centre path); with spp == 1 nothing reaches the film
+  const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
+  const int64_t npaths = npix * tasks;
+  const size_t need = (size_t)npaths * (film ? 12 : 4) + 264;
+  int st = FLG_OK;
+  char* sc = nullptr;
+  if (noalloc) {
+    if (need > kNoAllocScratchBytes)
+      return set_error(ctx, FLG_ERR_RANGE,
+                       std::string(what) + ": the frame needs " + std::to_string(need) +
+                           " bytes of scratch, more than the fixed " + std::to_string(kNoAllocScratchBytes) +
+                           " (at most " + std::to_string((kNoAllocScratchBytes - 264) / 4) +
+                           " paths, or a third of that with film = 1)");
+    sc = (char*)ctx->noalloc_scratch;
+    if (!sc) return set_error(ctx, FLG_ERR_HIP, std::string(what) + ": no device address for the fixed scratch");
+  } else {
+    sc = (char*)scratch(ctx, "vol_paths", need, &st);
+    if (!sc) return st;
+  }
+  unsigned long long* nl = (unsigned long long*)sc;
+  unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
+  float* lpath = (float*)(sc + 256);
+  float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
+  const bool count = h_stats != nullptr;
+  HaltonTables tab;
+  LightsDev L;
+  if (npaths > 0) {
+    if (noalloc) {
+      if (!halton_tables_ready(ctx, &tab))
+        return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": the context has no Halton tables");
+    } else if ((st = get_halton_tables(ctx, &tab))) {
+      return st;
+    }
+    if ((st = make_lights(ctx, vol, scene, &L, nullptr, !noalloc))) return st;
+  }
+  k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
+  if (npaths > 0) {
+    VolDev v = make_voldev(vol);
+    CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
+    ScopedTimer tm(ctx, "volpath_paths");
+    const int blocks1 = (int)div_up(npaths, 256);
+    const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+#define FLG_SVOL_LAUNCH(C, F)                                                                              \
+  if (sched == 1)                                                                                          \
+    k_svol<C, F><<<blocks1, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, fpos, \
+                                                   nl);                                                    \
+  else                                                                                                     \
+    k_svol_regen<C, F><<<blocks2, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, \
+                                                         fpos, next_path, nl);
+    if (count && film) {
+      FLG_SVOL_LAUNCH(true, true)
+    } else if (count) {
+      FLG_SVOL_LAUNCH(true, false)
+    } else if (film) {
+      FLG_SVOL_LAUNCH(false, true)
+    } else {
+      FLG_SVOL_LAUNCH(false, false)
+    }
+#undef FLG_SVOL_LAUNCH
+  }
+  {
+    ScopedTimer tm(ctx, "volpath_mean");
+    if (film)
+      film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
+    else
+      k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+  }
+  FLG_CHECK_LAUNCH(ctx);
+  if (h_stats) {
+    unsigned long long c = 0;
+    FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+    FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
+    h_stats[1] = c;
+  }
+  return FLG_OK;
+}
+
+}  // namespace flg
+
+// ------------------------------------------------ fixed no-alloc scratch ---
+// The scratch of flg_volpath_render_noalloc: a file-scope device array of
+// kNoAllocScratchBytes, one copy on every GPU the library runs on, held until
+// the process ends.  In host code the array's name is not a device address;
+// its address on each device comes from hipGetSymbolAddress with that device
+// current (flg_ctx_create calls noalloc_scratch_address once per device) and
+// only that address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[flg::kNoAllocScratchBytes];
+
+namespace flg {
+
+int noalloc_scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static void* addr[kMaxNoAllocDevices] = {};
+  *out = nullptr;
+  if (device < 0 || device >= kMaxNoAllocDevices) return FLG_ERR_RANGE;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess) return FLG_ERR_HIP;
+    addr[device] = p;
+  }
+  *out = addr[device];
   return FLG_OK;
 }
 
@@ -505,80 +643,14 @@ int flg_volume_destroy(flg_volume* vol) {
 
 int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                        const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats) {
-  return host_guard(ctx, "flg_volpath_render", [&]() -> int {
-    if (!ctx || !vol || !scene || !cfg || !d_rgb || cfg->width <= 0 || cfg->height <= 0 ||
-        cfg->spp <= 0 || cfg->max_depth < 1)
-      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: bad arguments");
-    if ((int64_t)cfg->width * cfg->height * cfg->spp >= ((int64_t)1 << 31))
-      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: too many paths");
-    if (cfg->film < 0 || cfg->film > 1)
-      return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: film must be 0 or 1");
-    const int sched = cfg->schedule ? cfg->schedule : 2;
-    if (sched < 1 || sched > 2) return set_error(ctx, FLG_ERR_INVALID, "flg_volpath_render: schedule must be 0, 1 or 2");
-    HaltonParams hp = make_halton_params(cfg->width, cfg->height);
-    if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_render: spp * Halton stride must stay below 2^24");
-    const bool film = cfg->film == 1;
-    const int64_t npix = (int64_t)cfg->width * cfg->height;
-    // film mode: spp - 1 splatted samples per pixel (the first task also runs
-    // the pixel-centre path); with spp == 1 nothing reaches the film
-    const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
-    const int64_t npaths = npix * tasks;
-    int st = FLG_OK;
-    // [256 B counters][npaths radiance floats][pad to 8 B][npaths film positions]
-    char* sc = (char*)scratch(ctx, "vol_paths", (size_t)npaths * (film ? 12 : 4) + 264, &st);
-    if (!sc) return st;
-    unsigned long long* nl = (unsigned long long*)sc;
-    unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
-    float* lpath = (float*)(sc + 256);
-    float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
-    const bool count = h_stats != nullptr;
-    k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
-    if (npaths > 0) {
-      HaltonTables tab;
-      if ((st = get_halton_tables(ctx, &tab))) return st;
-      LightsDev L;
-      if ((st = make_lights(ctx, vol, scene, &L))) return st;
-      VolDev v = make_voldev(vol);
-      CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
-      ScopedTimer tm(ctx, "volpath_paths");
-      const int blocks1 = (int)div_up(npaths, 256);
-      const int blocks2 = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
-  #define FLG_SVOL_LAUNCH(C, F)                                                                              \
-    if (sched == 1)                                                                                          \
-      k_svol<C, F><<<blocks1, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, fpos, \
-                                                     nl);                                                    \
-    else                                                                                                     \
-      k_svol_regen<C, F><<<blocks2, 256, 0, ctx->stream>>>(v, L, tab, hp, cam, tasks, cfg->max_depth, lpath, \
-                                                           fpos, next_path, nl);
-      if (count && film) {
-        FLG_SVOL_LAUNCH(true, true)
-      } else if (count) {
-        FLG_SVOL_LAUNCH(true, false)
-      } else if (film) {
-        FLG_SVOL_LAUNCH(false, true)
-      } else {
-        FLG_SVOL_LAUNCH(false, false)
-      }
-  #undef FLG_SVOL_LAUNCH
-    }
-    {
-      ScopedTimer tm(ctx, "volpath_mean");
-      if (film)
-        film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
-      else
-        k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
-    }
-    FLG_CHECK_LAUNCH(ctx);
-    if (h_stats) {
-      unsigned long long c = 0;
-      FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
-      FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
-      h_stats[1] = c;
-    }
-    return FLG_OK;
-  });
+  return host_guard(ctx, "flg_volpath_render",
+                    [&]() -> int { return volpath_render(ctx, vol, scene, cfg, d_rgb, h_stats, false); });
+}
+
+int flg_volpath_render_noalloc(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                               const flg_render_cfg* cfg, float* d_rgb) {
+  return host_guard(ctx, "flg_volpath_render_noalloc",
+                    [&]() -> int { return volpath_render(ctx, vol, scene, cfg, d_rgb, nullptr, true); });
 }
 
 }  // extern "C"

@@ -492,7 +492,10 @@ void film_resolve(hipStream_t st, const float* lpath, const float2* fpos, int32_
                   float* rgb);
 // scene_bounds (may be NULL: the volume's box): the scene bound that places the
 // environment light's sample points (InfiniteAreaLight::preprocess).
+// may_build false: an environment table not yet built for the scene's
+// radiance is an error (FLG_ERR_INVALID) instead of being built (which
+// allocates and uploads).
 int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDev* L,
-                const float* scene_bounds = nullptr);
+                const float* scene_bounds = nullptr, bool may_build = true);
 
 }  // namespace flg

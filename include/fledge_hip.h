@@ -363,6 +363,21 @@ typedef struct flg_render_cfg {
 FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                const flg_render_cfg* cfg, float* d_rgb, uint64_t* h_stats);
 
+/* flg_volpath_render without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as flg_volpath_render (no statistics).  Its scratch (path counters
+ * and per-path radiance) is a fixed array of 64 MiB + 4 KiB per GPU, held for
+ * the life of the process: frames of more than 16,778,174 paths (w*h*spp;
+ * film = 1: more than 5,592,724 tasks, w*h*(spp-1)) are refused with
+ * FLG_ERR_RANGE.
+ * The context's Halton tables come from flg_ctx_create; a scene with an
+ * environment light needs the volume's environment table built for the
+ * scene's radiance by an earlier flg_volpath_render of that volume
+ * (FLG_ERR_INVALID otherwise).  Every call on one GPU shares the array, so
+ * calls that overlap on one GPU race: serialise them (one stream). */
+FLG_API int flg_volpath_render_noalloc(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
+                                       const flg_render_cfg* cfg, float* d_rgb);
 
 /* Surfaces and media together: VolPathIntegrator::Li (integrator.cpp:407-510)
  * for the closed mesh `bvh`, whose triangles are index-matched interfaces
