@@ -217,7 +217,10 @@ def main():
                                "a launch = one bounce",
                             3: "k_pt_bounce (first bounce) then k_pt_paths (every later bounce in one launch: "
                                "each lane follows its path to the end, refilling from the queue); a launch = "
-                               "one of these two"}[(tuning or {}).get("ray_fetch", 1) or 1],
+                               "one of these two",
+                            4: "k_pt_bounce (first bounce), k_pt_trace + k_pt_shade (second), then k_pt_paths "
+                               "(every later bounce in one launch, each lane following its path to the end); a "
+                               "launch = one of these three"}[(tuning or {}).get("ray_fetch", 1) or 1],
                  "algorithmic_bytes_per_ray": round(bytes_per_ray, 2),
                  "node_visits_per_ray": round(vn, 3), "tri_tests_per_ray": round(vt, 3),
                  "avg_launch_ms": round(avg_launch_ms, 4), "launches_per_frame": launches_per_frame,

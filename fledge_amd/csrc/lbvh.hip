@@ -431,8 +431,6 @@ struct BuildOut {
   float* pre;          // per global leaf, where wanted: union of its chunk's leaves up to it (6 floats)
   float* suf;          // per global leaf, where wanted: union from it to its chunk's end
   float* ctot;         // per chunk: union of its leaves
-  uint2* toplist;      // crossing nodes: (global node, segment)
-  uint32_t* topcount;
 };
 
 FLG_D bool crosses(int a, int b) { return a / kChunk != b / kChunk; }
@@ -516,18 +514,6 @@ __global__ __launch_bounds__(kBlock) void k_emit(const SegDesc* segs, const int3
         o.need_suf[tb + gamma + 1] = 1;
         o.need_pre[tb + hi] = 1;
       }
-    }
-  }
-  // the crossing nodes, for k_refit_top (one atomic per wave)
-  const uint64_t m = __ballot(top);
-  if (m) {
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(o.topcount, (uint32_t)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (top) {
-      const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-      o.toplist[base + (uint32_t)__popcll(m & below)] = make_uint2((uint32_t)(s.node_base + i), (uint32_t)sid);
     }
   }
 }
@@ -704,29 +690,31 @@ __global__ __launch_bounds__(kChunkThreads) void k_refit_chunks(const SegDesc* s
 
 // Crossing nodes: each child range that crosses a chunk boundary too gets its
 // box from the suffix union at its start, the unions of the whole chunks in
-// between and the prefix union at its end.  One wave per node.
-__global__ __launch_bounds__(kBlock) void k_refit_top(const SegDesc* segs, BuildOut o) {
+// between and the prefix union at its end.  A wave reads the crossing flags of
+// 64 consecutive nodes and serves its crossing nodes one after another.
+__global__ __launch_bounds__(kBlock) void k_refit_top(const SegDesc* segs, int nseg, int64_t nnodes, BuildOut o) {
   const int lane = threadIdx.x & 63;
-  const uint32_t nt = *o.topcount;
-  const uint32_t nwaves = gridDim.x * (kBlock / 64);
-  for (uint32_t k = blockIdx.x * (kBlock / 64) + threadIdx.x / 64; k < nt; k += nwaves) {
-    const uint2 e = o.toplist[k];
-    const int x = (int)e.x;
-    const SegDesc s = segs[e.y];
-    const int lo = o.rlo[x], hi = o.rhi[x], gamma = o.split[x];
-    const uint8_t lf = o.leafflags[x];
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t w0 = ((int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64) * 64; w0 < nnodes; w0 += nwaves * 64) {
+    const int64_t mine = w0 + lane;
+    for (uint64_t m = __ballot(mine < nnodes && (o.ninfo[mine] & kCross)); m; m &= m - 1) {
+      const int x = (int)(w0 + __ffsll((unsigned long long)m) - 1);
+      const SegDesc s = segs[seg_of<&SegDesc::node_base>(segs, nseg, x)];
+      const int lo = o.rlo[x], hi = o.rhi[x], gamma = o.split[x];
+      const uint8_t lf = o.leafflags[x];
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int a = side ? gamma + 1 : lo, z = side ? hi : gamma;
-      if ((lf >> side) & 1 || !crosses(a, z)) continue;  // done by k_refit_chunks
-      Box bx = empty_box();
-      for (int c = a / kChunk + 1 + lane; c < z / kChunk; c += 64)
-        bx = join(bx, get_box(o.ctot + 6 * (int64_t)(s.chunk_base + c)));
-      for (int off = 32; off >= 1; off >>= 1) bx = join(bx, shfl_box(bx, lane ^ off));
-      if (lane == 0) {
-        bx = join(bx, get_box(o.suf + 6 * (int64_t)(s.tri_base + a)));
-        bx = join(bx, get_box(o.pre + 6 * (int64_t)(s.tri_base + z)));
-        store_slot(o.nodes + x, side, bx.lo, bx.hi);
+      for (int side = 0; side < 2; ++side) {
+        const int a = side ? gamma + 1 : lo, z = side ? hi : gamma;
+        if ((lf >> side) & 1 || !crosses(a, z)) continue;  // done by k_refit_chunks
+        Box bx = empty_box();
+        for (int c = a / kChunk + 1 + lane; c < z / kChunk; c += 64)
+          bx = join(bx, get_box(o.ctot + 6 * (int64_t)(s.chunk_base + c)));
+        for (int off = 32; off >= 1; off >>= 1) bx = join(bx, shfl_box(bx, lane ^ off));
+        if (lane == 0) {
+          bx = join(bx, get_box(o.suf + 6 * (int64_t)(s.tri_base + a)));
+          bx = join(bx, get_box(o.pre + 6 * (int64_t)(s.tri_base + z)));
+          store_slot(o.nodes + x, side, bx.lo, bx.hi);
+        }
       }
     }
   }
@@ -850,12 +838,11 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
   const size_t so_k0 = stake(N * 4), so_v0 = stake(N * 4), so_k1 = stake(N * 4), so_v1 = stake(N * 4);
   const size_t so_ninfo = stake(NI * 4), so_lslot = stake(N * 4);
   const size_t so_pre = stake(N * 24), so_suf = stake(N * 24), so_ctot = stake(chunk_total * 24);
-  const size_t so_toplist = stake(NI * 8);
   const size_t so_blkseg = stake(blk_total * 4), so_chunkseg = stake(chunk_total * 4);
   const size_t hist_words = (size_t)nseg * kPasses * kDigits;
   const size_t status_words = (size_t)kPasses * tile_total * kDigits;
   const size_t flag_words = (2 * N + 3) / 4;  // two byte arrays
-  const size_t zero_words = hist_words + status_words + kPasses + 2 + flag_words;
+  const size_t zero_words = hist_words + status_words + kPasses + 1 + flag_words;
   const size_t so_zero = stake(zero_words * 4);
   char* sc = (char*)scratch(ctx, "lbvh_build", s_off ? s_off : 256, &st);
   if (!sc) return fail(st);
@@ -874,8 +861,7 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
   uint32_t* status = hist + hist_words;
   uint32_t* counters = status + status_words;
   uint32_t* err = counters + kPasses;
-  uint32_t* topcount = err + 1;
-  uint8_t* want = (uint8_t*)(topcount + 1);
+  uint8_t* want = (uint8_t*)(err + 1);
 
   BuildOut o;
   o.nodes = (BNode*)(base + o_nodes);
@@ -896,8 +882,6 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
   o.pre = (float*)(sc + so_pre);
   o.suf = (float*)(sc + so_suf);
   o.ctot = (float*)(sc + so_ctot);
-  o.toplist = (uint2*)(sc + so_toplist);
-  o.topcount = topcount;
   float* gb = (float*)(base + o_gb);
 
   const int fill_grid = (int)std::max<int64_t>(1, std::min<int64_t>(div_up((int64_t)zero_words, 256),
@@ -936,9 +920,9 @@ static int build_segments(flg_ctx* ctx, const flg_mesh_desc* meshes, int nseg, u
       ScopedTimer tm(ctx, "lbvh_refit");
       k_refit_chunks<<<(int)chunk_total, kChunkThreads, 0, ctx->stream>>>(d_segs, chunk_seg, o);
       if (NI > 0) {
-        const int tgrid = (int)std::max<int64_t>(1, std::min<int64_t>(div_up((int64_t)NI / 64 + 1, kBlock / 64),
+        const int tgrid = (int)std::max<int64_t>(1, std::min<int64_t>(div_up((int64_t)NI, kBlock),
                                                                        (int64_t)ctx->num_cus * 8));
-        k_refit_top<<<tgrid, kBlock, 0, ctx->stream>>>(d_segs, o);
+        k_refit_top<<<tgrid, kBlock, 0, ctx->stream>>>(d_segs, nseg, (int64_t)NI, o);
       }
     }
     hipError_t e = hipGetLastError();

@@ -407,8 +407,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
   }
 }
 
-// Bounces after the first in one persistent launch (ray_fetch 3).  Each wave
-// owns a contiguous slice of the first bounce's queue; a lane takes a queued
+// Bounces from bounce0 on in one persistent launch (ray_fetch 3: from the
+// second segment of every path; ray_fetch 4: from the third, after the
+// second's traversal and shading kernels).  Each wave owns a contiguous slice
+// of bounce0's queue; a lane takes a queued
 // ray and follows its path to the end: when its traversal ends it records the
 // segment and samples the diffuse bounce (pt_finish, the code of the other
 // schedules, so the same bits) and traces the next segment itself, with no
@@ -418,9 +420,12 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
 // slice as in k_pt_trace.  The segments traced after the queue's are counted
 // into count[bounce + 1] for the frame's statistics.
 constexpr int kFinishBatch = 16;
+// The kernel holds a path's whole state besides the traversal: it asks for 4
+// waves per SIMD (128 VGPRs), which keeps the shading out of scratch.
+constexpr int kPathWaves = 4;
 
 template <bool COMPAT, int BS, int RING>
-__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_paths(TraceGeom g, PtState s, HaltonTables tab,
+__global__ __launch_bounds__(BS, kPathWaves) void k_pt_paths(TraceGeom g, PtState s, HaltonTables tab,
                                                  HaltonParams hp, int bounce0, int max_depth) {
   __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
   const uint32_t n = s.count[bounce0];
@@ -585,19 +590,19 @@ void launch_paths_bs(int grid, int block, hipStream_t st, const TraceGeom& g, co
 }
 
 // One bounce of the frame (fetch 1: the fused kernel; fetch 2: traversal
-// with refill, then shading) or, fetch 3 at bounce 1, every remaining bounce
-// in one launch.
+// with refill, then shading) or every remaining bounce in one launch (fetch 3
+// at bounce 1, fetch 4 at bounce 2).
 template <bool COMPAT>
 void launch_bounce(bool first, bool count, int fetch, int grid, int block, int ring, hipStream_t st,
                    const TraceGeom& g, const PtState& s, const CamDev& cam, const HaltonTables& tab,
                    const HaltonParams& hp, int spp, int b, int D, unsigned long long* visits) {
   // the first bounce's camera rays are coherent: a wave's lanes stay in step
   // in the fused kernel, which refilling them one by one would undo
-  if (fetch == 3 && !count && !first) {
+  if (((fetch == 3 && b >= 1) || (fetch == 4 && b >= 2)) && !count && !first) {
     with_ring(ring, [&](auto R) { launch_paths_bs<COMPAT, decltype(R)::value>(grid, block, st, g, s, tab, hp, b, D); });
     return;
   }
-  if (fetch == 2 && !count && !first) {
+  if ((fetch == 2 || fetch == 4) && !count && !first) {
     with_ring(ring, [&](auto R) { launch_split_bs<COMPAT, decltype(R)::value>(grid, block, st, g, s, b); });
     const int sgrid = grid * block / kShadeBlock > 0 ? grid * block / kShadeBlock : 1;
     k_pt_shade<COMPAT><<<sgrid, kShadeBlock, 0, st>>>(g, s, tab, hp, b, D);
@@ -737,10 +742,10 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     const int rdepth = cfg->stack_mode ? cfg->ring_depth : ctx->tuned_pt_ring;
     int fmode = cfg->ray_fetch ? cfg->ray_fetch : ctx->tuned_pt_fetch;
     if (fmode == 0) fmode = 1;
-    if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 3)
+    if (!valid_block(tb) || bpc < 1 || bpc > 64 || smode < 0 || smode > 2 || fmode < 1 || fmode > 4)
       return set_error(ctx, FLG_ERR_INVALID,
                        "flg_pathtrace_diffuse: trace_block must be 64..512 (power of two), blocks_per_cu 1..64, "
-                       "stack_mode 0..2 and ray_fetch 0..3");
+                       "stack_mode 0..2 and ray_fetch 0..4");
     const int ring = ring_of(smode, rdepth);
     if (!valid_ring(tb, ring))
       return set_error(ctx, FLG_ERR_INVALID,
@@ -788,7 +793,8 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
         else
           launch_bounce<false>(b == 0, count_visits, fmode, pgrid, tb, ring, stream, g, s, cd, tab, hp, cfg->spp,
                                b, D, visits);
-        if (fmode == 3 && !count_visits && b >= 1) break;  // that launch ran every remaining bounce
+        // that launch ran every remaining bounce
+        if (!count_visits && ((fmode == 3 && b >= 1) || (fmode == 4 && b >= 2))) break;
       }
     };
     if (cfg->use_graph) {
@@ -937,7 +943,7 @@ int flg_autotune_pathtrace(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* c
     };
     const int rings[5] = {0, 4, 8, 16, 32};
     const int shapes[2][2] = {{256, 8}, {128, 64}};
-    for (int fm = 1; fm <= 3; ++fm)
+    for (int fm = 1; fm <= 4; ++fm)
       for (int rg : rings)
         for (const auto& sh : shapes) {
           int st = consider(Cand{sh[0], sh[1], rg ? 2 : 1, rg, fm});

@@ -280,7 +280,8 @@ typedef struct flg_pt_cfg {
                             shading; 3: every bounce after the first in one
                             launch, each lane following its path to the end
                             and taking a new one when enough of its wave's
-                            lanes are idle; 0: tuned value, else 1.  Same
+                            lanes are idle; 4: the second segment as 2, every
+                            later one as 3; 0: tuned value, else 1.  Same
                             results. */
   int32_t ring_depth;    /* as flg_trace_cfg.ring_depth */
   int32_t reserved[1];
@@ -305,7 +306,7 @@ typedef struct flg_tuning {
   float ms_default;      /* one call with the untuned shape (256 threads, 8 blocks per CU) */
   int32_t candidates;    /* shapes timed */
   int32_t stack_mode;    /* chosen traversal stack (1 private, 2 LDS ring) */
-  int32_t ray_fetch;     /* chosen ray fetch of the path tracer (1..3; 0 for trace) */
+  int32_t ray_fetch;     /* chosen ray fetch of the path tracer (1..4; 0 for trace) */
   int32_t ring_depth;    /* chosen LDS ring depth (stack_mode 2), else 0 */
 } flg_tuning;
 FLG_API int flg_autotune_trace(flg_ctx* ctx, const flg_bvh* bvh, const flg_ray* d_rays,
