@@ -96,6 +96,9 @@ def main():
                          "frame (skips the autotuner; profiling runs use the shape it picked)")
     ap.add_argument("--profile-pass", action="store_true",
                     help="one calibration copy + one headline frame, no JSON (for counter runs)")
+    ap.add_argument("--profile-leg", default="", choices=["", "vol_path", "fwd_adjoint", "combined", "build"],
+                    help="one calibration copy + one pass of that leg's workload (no headline, no JSON; for "
+                         "counter runs)")
     ap.add_argument("--only", default="",
                     help="comma-separated legs to run besides the headline (vol_path, fwd_adjoint, "
                          "lbvh_build, batched_build, primary_rays, combined); default all")
@@ -115,6 +118,9 @@ def main():
     dev = torch.device("cuda", 0)
     ctx = api.Context(0)
     sync = torch.cuda.synchronize
+    if args.profile_leg:
+        profile_leg(args.profile_leg, ctx, N, api, torch, dev, sync)
+        return
     want = set(x for x in args.only.split(",") if x) or {"vol_path", "fwd_adjoint", "lbvh_build",
                                                            "batched_build", "primary_rays", "combined"}
     if args.fast:
@@ -144,12 +150,7 @@ def main():
     if args.profile_pass:
         # for hardware-counter runs: a plain device copy of known size (the
         # calibration of FETCH_SIZE / WRITE_SIZE), then one headline frame
-        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)  # 1 GiB
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        sync()
-        log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 4, dst.numel() * 4))
-        del src, dst
+        calibration_copy(torch, dev, sync)
         bvh.pathtrace(cam2, W, H, SPP, DEPTH, **shape)
         sync()
         log("profile pass done: one frame of %d ray segments" % segments)
@@ -265,7 +266,7 @@ def main():
         legs["fwd_adjoint"] = bench_fwd_adjoint(ctx, sync, cpu_jobs)
     if "combined" in want:
         log("leg: combined")
-        legs["combined"] = bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H)
+        legs["combined"] = bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, cpu_jobs=cpu_jobs)
 
     # ------------------------------------------------------ CPU baselines
     cpu = None
@@ -305,6 +306,54 @@ def main():
     print(json.dumps(out))
 
 
+def calibration_copy(torch, dev, sync):
+    """A plain device copy of 1 GiB: what FETCH_SIZE and WRITE_SIZE read for
+    it calibrates their counts of a kernel's HBM bytes."""
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    sync()
+    log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 4, dst.numel() * 4))
+
+
+def profile_leg(name, ctx, N, api, torch, dev, sync):
+    """One pass of a leg's workload after the calibration copy, for counter
+    runs: vol_path = one 256^3 frame (k_svol_regen); fwd_adjoint = one plain
+    forward plus one differentiable iteration at the fBm state (k_svol_regen,
+    k_diff_regen<false>, k_diff_regen<true>); combined = one frame of the
+    combined scene, its differentiable forward and its adjoint (k_volpath_regen,
+    k_vpdiff_regen); build = six builds of the 2M-triangle octasphere, then
+    three batched builds of the 256 meshes."""
+    calibration_copy(torch, dev, sync)
+    if name == "vol_path":
+        w = vol_workload(N, api, torch, dev, ctx)
+        w["vol"].render(w["scene"], w["W"], w["H"], w["spp"], 32, out=w["out"])
+        sync()
+        w["vol"].close()
+    elif name == "fwd_adjoint":
+        from fledge_amd import diff
+        diff.bench_adjoint(ctx, None, sync, state="fbm", profile=True)
+    elif name == "combined":
+        bench_combined(ctx, N, api, torch, __import__("numpy"), dev, sync, None, 1920, 1080, profile=True)
+    elif name == "build":
+        v, ind = api.gen_octasphere(9, 0.1, seed=7, base_freq=2.0)
+        dv, di = torch.from_numpy(v).to(dev), torch.from_numpy(ind).to(dev)
+        for _ in range(6):
+            api.LBVH.build(ctx, dv, di, check_indices=False).close()
+        sync()
+        log("six builds of the 2,097,152-triangle octasphere done")
+        meshes = []
+        for k, n in enumerate(api.gen_batch_sizes(256)):
+            vv, ii = api.gen_soup(int(n), initstate=42, initseq=1000 + k)
+            meshes.append((torch.from_numpy(vv).to(dev), torch.from_numpy(ii).to(dev)))
+        for _ in range(3):
+            for h in api.LBVH.build_batched(ctx, meshes, check_indices=False):
+                h.close()
+        sync()
+        log("three batched builds of the 256 meshes done")
+    log("profile pass done: %s" % name)
+
+
 # ---------------------------------------------------------------- legs ---
 def bench_build(ctx, api, torch, dev, sync, v, ind, dv, di, cpu_jobs):
     """LBVH build of the 2M-triangle octasphere and the 65,536-triangle soup:
@@ -328,7 +377,7 @@ def bench_build(ctx, api, torch, dev, sync, v, ind, dv, di, cpu_jobs):
         kms = kernel_ms(ctx, names, 5)
         parts = {n: round(ctx.kernel_time(n)[0] / 5, 4) for n in names}
         ctx.profiling(False)
-        res[tag] = {"ms": round(wall * 1e3, 4), "kernel_ms": round(kms, 4), "mtris_per_s": round(n / wall / 1e3, 2),
+        res[tag] = {"ms": round(wall * 1e3, 4), "kernel_ms": round(kms, 4), "mtris_per_s": round(n / wall / 1e6, 2),
                     "stages_ms": parts, "triangles": n,
                     "roofline": roofline(BUILD_BYTES_PER_TRI * n / (kms * 1e-3) / 1e9,
                                          "the build's kernels (bound, Morton, sort, emit, refit + soup)",
@@ -461,16 +510,21 @@ def bench_primary(ctx, N, api, torch, dev, sync, cpu_jobs, secondary):
     return out
 
 
-def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
-    """256^3 fBm grid, 512x512, 64 spp, depth 32: path samples/s; roofline of
-    the path kernel over 32 B per trilinear lookup + 12 B per pixel."""
+def vol_workload(N, api, torch, dev, ctx):
+    """The 256^3 fBm volume (seed 1, sigma_s 6.4, sigma_a 1.6, g 0.3), its
+    camera and lights, 512x512 at 64 spp."""
     g1 = api.gen_fbm_grid(256, seed=1)
     vol = api.Volume(ctx, torch.from_numpy(g1).to(dev), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
     cam0 = cam_of(N, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
-    sc1 = api.vol_scene(cam0)
-    Wv = Hv = 512
-    S1 = 64
-    out1 = torch.empty((Hv, Wv, 3), dtype=torch.float32, device=dev)
+    out = torch.empty((512, 512, 3), dtype=torch.float32, device=dev)
+    return {"grid": g1, "vol": vol, "scene": api.vol_scene(cam0), "W": 512, "H": 512, "spp": 64, "out": out}
+
+
+def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
+    """256^3 fBm grid, 512x512, 64 spp, depth 32: path samples/s; roofline of
+    the path kernel over 32 B per trilinear lookup + 12 B per pixel."""
+    w = vol_workload(N, api, torch, dev, ctx)
+    g1, vol, sc1, Wv, Hv, S1, out1 = w["grid"], w["vol"], w["scene"], w["W"], w["H"], w["spp"], w["out"]
     _, st1 = vol.render(sc1, Wv, Hv, S1, 32, out=out1, stats=True)
     t1 = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1), 5, 2, sync)
     t1b = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1, schedule=1), 3, 1, sync)
@@ -550,50 +604,110 @@ def bench_fwd_adjoint(ctx, sync, cpu_jobs):
     return out
 
 
-def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H):
+def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, profile=False, cpu_jobs=None):
     """VolPathIntegrator: the 8,388,608-triangle displaced octahedral sphere
-    (amplitude 0.05) as the interface around a 512^3 grid whose 8^3 bricks
-    are active with probability 0.15 and hold fBm; 1920x1080, 64 spp, depth
-    16; then the differentiable forward with the walk tape and the adjoint
-    over the active bricks."""
+    (amplitude 0.05) as the interface around a 512^3 sparse-occupancy grid
+    whose 8^3 bricks are active with probability 0.15 and hold fBm, stored as
+    a brick pool (flg_volume_create_sparse); 1920x1080, 64 spp, depth 16;
+    then the differentiable forward with the walk tape and the adjoint over
+    the active bricks.  The same frame on the dense copy of the grid is timed
+    beside it.  With `profile`: one pass of each sparse kernel, no timing."""
     from fledge_amd import diff
+    if cam2 is None:
+        cam2 = cam_of(N, (0.0, 0.0, -3.0), (0.0, 0.0, 0.0))
+    SPPc, Dc = 64, 16
     vc, ic = api.gen_octasphere(10, 0.05, seed=7, base_freq=2.0)
     gc = api.gen_fbm_grid(512, seed=3)
     bricks = np.random.default_rng(4).random((64, 64, 64)) < 0.15
     gc *= np.repeat(np.repeat(np.repeat(bricks, 8, 0), 8, 1), 8, 2)
     bc = api.LBVH.build(ctx, torch.from_numpy(vc).to(dev), torch.from_numpy(ic).to(dev))
     gct = torch.from_numpy(gc).to(dev)
-    volc = api.Volume(ctx, gct, [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    origins, values = api.leaves_of(gct)
     gmax = float(gc.max())
-    del gc, vc, ic
+    host_grid = gc if (cpu_jobs is not None and not profile) else None
+    host_mesh = (vc, ic) if host_grid is not None else None
+    del gc
+    volc = api.Volume.sparse(ctx, origins, values, (512, 512, 512), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
     scc = api.vol_scene(cam2)
     outc = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
-    _, stc = volc.render_mesh(bc, scc, W, H, 64, 16, out=outc, stats=True)
-    tc = timed(lambda: volc.render_mesh(bc, scc, W, H, 64, 16, out=outc), 1, 0, sync)
-    volc.close()
-    dvc = diff.DiffVolume(ctx, gct, g=0.3)
-    layout = diff.BrickLayout(ctx, gct, 8)
-    del gct
-    tape = diff.new_tape(ctx, W, H, 64, 16)
-    lpc = torch.empty(W * H * 64, dtype=torch.float32, device=dev)
+    _, stc = volc.render_mesh(bc, scc, W, H, SPPc, Dc, out=outc, stats=True)
+    dvc = diff.DiffVolume.sparse(ctx, origins, values, (512, 512, 512), g=0.3)
+    layout = diff.BrickLayout.of_volume(dvc.vol)
+    tape = diff.new_tape(ctx, W, H, SPPc, Dc)
+    lpc = torch.empty(W * H * SPPc, dtype=torch.float32, device=dev)
     dlc = torch.full((H, W, 3), 1.0 / (W * H * 3), dtype=torch.float32, device=dev)
     gb = layout.zeros()
     gac = torch.zeros(1, dtype=torch.float32, device=dev)
     mu_c = 8.0 * gmax
-    tf = timed(lambda: diff.mesh_forward(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, out=outc, lpath=lpc,
+    if profile:
+        volc.render_mesh(bc, scc, W, H, SPPc, Dc, out=outc)
+        diff.mesh_forward(dvc, bc, scc, W, H, SPPc, Dc, 8.0, 0.8, mu_c, out=outc, lpath=lpc, tape=tape)
+        diff.mesh_adjoint(dvc, bc, scc, W, H, SPPc, Dc, 8.0, 0.8, mu_c, lpc, dlc, grad=gb, galpha=gac, tape=tape,
+                          bricks=layout)
+        sync()
+        bc.close()
+        return None
+    tc = timed(lambda: volc.render_mesh(bc, scc, W, H, SPPc, Dc, out=outc), 2, 1, sync)
+    ctx.profiling(True)
+    ctx.reset_profiling()
+    volc.render_mesh(bc, scc, W, H, SPPc, Dc, out=outc)
+    sync()
+    kms = kernel_ms(ctx, ["volpath_mesh_paths"], 1)
+    ctx.profiling(False)
+    tf = timed(lambda: diff.mesh_forward(dvc, bc, scc, W, H, SPPc, Dc, 8.0, 0.8, mu_c, out=outc, lpath=lpc,
                                          tape=tape), 1, 1, sync)
-    ta = timed(lambda: diff.mesh_adjoint(dvc, bc, scc, W, H, 64, 16, 8.0, 0.8, mu_c, lpc, dlc, grad=gb,
+    ta = timed(lambda: diff.mesh_adjoint(dvc, bc, scc, W, H, SPPc, Dc, 8.0, 0.8, mu_c, lpc, dlc, grad=gb,
                                          galpha=gac, tape=tape, bricks=layout), 1, 0, sync)
-    P = W * H * 64
-    out = {"metric": "combined scene: VolPathIntegrator path samples/s (8,388,608-triangle interface mesh around "
-                     "a 512^3 grid of 8^3 bricks, 15% active; 1920x1080, 64 spp, depth 16)",
-           "value": round(P / tc, 1), "unit": "path samples/s", "higher_is_better": True,
-           "ms_per_frame": round(tc * 1e3, 2), "lookups_per_path": round(stc["lookups"] / P, 3),
-           "diff_forward_ms": round(tf * 1e3, 2), "adjoint_ms": round(ta * 1e3, 2),
-           "active_bricks": layout.n_active, "fwd_adjoint_over_forward": round((tf + ta) / tc, 3),
-           "roofline": None, "cpu_baseline": None}
     del tape, lpc, gb
+    # the same frame on the dense grid (512 MiB plus its border)
+    vold = api.Volume(ctx, gct, [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    outd = torch.empty_like(outc)
+    td = timed(lambda: vold.render_mesh(bc, scc, W, H, SPPc, Dc, out=outd), 1, 1, sync)
+    volc.render_mesh(bc, scc, W, H, SPPc, Dc, out=outc)  # (outc last held the differentiable forward)
+    same = bool(torch.equal(outc.view(torch.int32), outd.view(torch.int32)))
+    vold.close()
+    del gct
+    P = W * H * SPPc
+    rays = stc["rays"]
+    look = stc["lookups"]
+    # density bytes: 32 B per trilinear lookup + 4 B slot-map entry, 12 B per
+    # pixel written (the mesh traversal's node and triangle reads are not in it)
+    lb = 36 * look + 12 * W * H
+    pool_bytes = origins.shape[0] * 4000 + 65 * 65 * 65 * 32
+    out = {"metric": "combined scene: VolPathIntegrator path samples/s (8,388,608-triangle interface mesh around "
+                     "a 512^3 sparse grid of 8^3 bricks, 15% active, in a brick pool; 1920x1080, 64 spp, depth 16)",
+           "value": round(P / tc, 1), "unit": "path samples/s", "higher_is_better": True,
+           "ms_per_frame": round(tc * 1e3, 2), "kernel_ms": round(kms, 3),
+           "mrays_per_s": round(rays / tc / 1e6, 1), "rays_per_frame": rays,
+           "lookups_per_path": round(look / P, 3),
+           "dense_grid_ms_per_frame": round(td * 1e3, 2), "dense_and_sparse_images_bit_equal": same,
+           "active_bricks": int(origins.shape[0]), "grid_bytes_sparse": pool_bytes,
+           "grid_bytes_dense": 514 ** 3 * 4,
+           "diff_forward_ms": round(tf * 1e3, 2), "adjoint_ms": round(ta * 1e3, 2),
+           "fwd_adjoint_over_forward": round((tf + ta) / tc, 3),
+           "roofline": roofline(lb / (kms * 1e-3) / 1e9,
+                                "k_volpath_regen (persistent lanes over the interface mesh and the brick pool); "
+                                "density bytes only",
+                                round(36 * look / P + 12 / SPPc, 3), "path"),
+           "cpu_baseline": None}
     bc.close()
+    if host_grid is not None:
+        def cpu_job(cores):
+            import oracle_lib as orc
+            vv, ii = host_mesh
+            m = orc.Lbvh(vv, ii)
+            stride = 19  # 109,137 of 2,073,600 pixels, all 64 samples each
+            cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
+            t0 = time.perf_counter()
+            _, _, pa = m.volpath_mesh_strided(host_grid, [6.4, 1.6, 0.3], [1, 2, 2, -2, 50.0, 1, 0.5], cam10, W, H,
+                                              SPPc, Dc, stride, nthreads=cores)
+            dtc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"value": round(pa / dtc, 1), "unit": "path samples/s", "cores": cores,
+                                   "kind": "port",
+                                   "sample": "every %d-th pixel of the frame (%d paths, all 64 spp each) on the "
+                                             "dense copy of the grid, the oracle's VolPathIntegrator restatement "
+                                             "over its own LBVH, %.2f s" % (stride, pa, dtc)}
+        cpu_jobs.append(cpu_job)
     return out
 
 

@@ -132,6 +132,9 @@ _SIGS = {
     "flg_volume_create": (c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, P(c_f), P(Medium), P(c_vp)]),
     "flg_volume_destroy": (c_int, [c_vp]),
     "flg_volume_update": (c_int, [c_vp, c_vp, c_vp]),
+    "flg_volume_create_sparse": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, P(c_f), P(Medium), P(c_vp)]),
+    "flg_volume_update_sparse": (c_int, [c_vp, c_vp, c_vp]),
+    "flg_volume_brick_layout": (c_int, [c_vp, P(BrickLayout)]),
     "flg_volpath_render": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp, P(c_u64)]),
     "flg_volpath_render_noalloc": (c_int, [c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp]),
     "flg_volpath_mesh_render": (c_int, [c_vp, c_vp, c_vp, P(VolScene), P(RenderCfg), c_vp, P(c_u64)]),

@@ -370,15 +370,46 @@ class Volume:
         check_buffer(density, "density", torch.float32, ctx.device, ndim=3)
         nz, ny, nx = density.shape
         self.shape = (nz, ny, nx)
+        self.n_leaves = None  # dense
         med = N.Medium(sigma_s, sigma_a, g)
         b = (ctypes.c_float * 6)(*[float(x) for x in bounds])
         self.h = ctypes.c_void_p()
         ctx.check(ctx.lib.flg_volume_create(ctx.h, _ptr(density), nx, ny, nz, b, ctypes.byref(med),
                                             ctypes.byref(self.h)))
 
+    @classmethod
+    def sparse(cls, ctx, origins, values, shape, bounds, sigma_s, sigma_a, g):
+        """Sparse-occupancy grid of `shape` (nz, ny, nx): the 8^3 bricks with
+        voxel origins `origins` ((n, 3) int32, x y z, multiples of 8) and
+        values `values` ((n, 8, 8, 8) float32, z y x); every other voxel is 0
+        (flg_volume_create_sparse).  `leaves_of` makes both from a dense
+        grid."""
+        nz, ny, nx = (int(x) for x in shape)
+        check_buffer(origins, "origins", torch.int32, ctx.device, shape=(None, 3))
+        n = origins.shape[0]
+        check_buffer(values, "values", torch.float32, ctx.device, shape=(n, 8, 8, 8))
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.shape = (nz, ny, nx)
+        self.n_leaves = n
+        self.origins = origins
+        med = N.Medium(sigma_s, sigma_a, g)
+        b = (ctypes.c_float * 6)(*[float(x) for x in bounds])
+        self.h = ctypes.c_void_p()
+        ctx.check(ctx.lib.flg_volume_create_sparse(ctx.h, n, _ptr(origins) if n else None,
+                                                   _ptr(values) if n else None, nx, ny, nz, b,
+                                                   ctypes.byref(med), ctypes.byref(self.h)))
+        return self
+
     def update(self, density):
         check_buffer(density, "density", torch.float32, self.ctx.device, shape=self.shape)
         self.ctx.check(self.ctx.lib.flg_volume_update(self.ctx.h, self.h, _ptr(density)))
+
+    def update_sparse(self, values):
+        """New brick values of a sparse volume (same bricks and order)."""
+        check_buffer(values, "values", torch.float32, self.ctx.device, shape=(self.n_leaves, 8, 8, 8))
+        self.ctx.check(self.ctx.lib.flg_volume_update_sparse(self.ctx.h, self.h,
+                                                             _ptr(values) if self.n_leaves else None))
 
     def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0):
         """RGB image (height, width, 3).  film=1: the reference's render loop
@@ -408,11 +439,11 @@ class Volume:
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
         check_buffer(out, "out", torch.float32, self.ctx.device, min_numel=height * width * 3)
-        st = (ctypes.c_uint64 * 2)()
+        st = (ctypes.c_uint64 * 3)()
         self.ctx.check(self.ctx.lib.flg_volpath_mesh_render(self.ctx.h, bvh.h, self.h, ctypes.byref(scene),
                                                             ctypes.byref(cfg), _ptr(out), st if stats else None))
         if stats:
-            return out, {"paths": int(st[0]), "lookups": int(st[1])}
+            return out, {"paths": int(st[0]), "lookups": int(st[1]), "rays": int(st[2])}
         return out
 
     def close(self):
@@ -425,6 +456,26 @@ class Volume:
             self.close()
         except Exception:
             pass
+
+
+def leaves_of(density):
+    """The non-zero 8^3 bricks of a dense (nz, ny, nx) float32 grid (any
+    device): origins ((n, 3) int32, voxel x y z of each brick's first voxel)
+    and values ((n, 8, 8, 8) float32, z y x), bricks in x-fastest brick order.
+    A brick is kept iff one of its voxels has bits other than +0, so the
+    sparse volume of the result equals the dense one bit for bit."""
+    nz, ny, nx = density.shape
+    bz, by, bx = (-(-n // 8) for n in (nz, ny, nx))
+    full = torch.zeros((bz * 8, by * 8, bx * 8), dtype=torch.float32, device=density.device)
+    full[:nz, :ny, :nx] = density
+    bricks = full.reshape(bz, 8, by, 8, bx, 8).permute(0, 2, 4, 1, 3, 5).reshape(-1, 8, 8, 8)
+    keep = (bricks.view(torch.int32) != 0).reshape(bricks.shape[0], -1).any(dim=1)
+    idx = torch.nonzero(keep).flatten()
+    x = idx % bx
+    y = (idx // bx) % by
+    z = idx // (bx * by)
+    origins = (torch.stack([x, y, z], dim=1) * 8).to(torch.int32).contiguous()
+    return origins, bricks[idx].contiguous()
 
 
 def vol_scene(cam, point=(2.0, 2.0, -2.0), intensity=50.0, env=0.5, has_point=True, has_env=True):

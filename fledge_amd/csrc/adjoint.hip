@@ -307,9 +307,9 @@ int diff_prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, cons
   if (st) return st;
   if ((st = make_lights(ctx, vol, scene, L, bounds))) return st;
   const float* b = bounds ? bounds : vol->bounds;
-  D->v = voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, b, vol->medium, vol->max_density);
-  D->vp = proposal ? voldev_of(proposal->grid, proposal->nx, proposal->ny, proposal->nz, b, proposal->medium,
-                               proposal->max_density)
+  D->v = voldev_of(vol->grid, vol->bmap, vol->nx, vol->ny, vol->nz, b, vol->medium, vol->max_density);
+  D->vp = proposal ? voldev_of(proposal->grid, proposal->bmap, proposal->nx, proposal->ny, proposal->nz, b,
+                               proposal->medium, proposal->max_density)
                    : D->v;
   D->same = proposal == nullptr || proposal == vol;
   D->s = prm->sigma_t_scale;

@@ -69,10 +69,8 @@ FLG_D Look2 look2(const DiffDev& D, V3 p) {
 
 // gradient scatter: g * trilinear weight into the caller's gradient
 FLG_D void deposit(const DiffDev& D, float* grad, const Tri8& t, float g) {
-  if (t.base < 0 || g == 0.0f) return;
-  const int px = D.v.px, py = D.v.ny + 2;
-  const int b = t.base;
-  const int i = b % px - 1, j = (b / px) % py - 1, k = b / D.v.pxy - 1;
+  if (!t.inside || g == 0.0f) return;
+  const int i = t.i, j = t.j, k = t.k;
   const float wx[2] = {1.0f - t.fx, t.fx}, wy[2] = {1.0f - t.fy, t.fy}, wz[2] = {1.0f - t.fz, t.fz};
 #pragma unroll
   for (int c = 0; c < 8; ++c) {

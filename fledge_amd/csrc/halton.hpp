@@ -5,9 +5,9 @@
 //
 // Tables (primes of the linear sieve below 262144, sampler.h:39-57, and their
 // float reciprocals 1.0f / base) live in device memory, one copy per context.
-// The digit loop of RadicalInverse_ (sampler.h:59-72) divides by the prime
-// with a float reciprocal estimate and one exact integer correction, valid
-// for indices below 2^24 (checked by the host before launch).
+// The digit loop of RadicalInverse_ (sampler.h:59-72) divides by the prime in
+// double with a refined reciprocal and a small bias, exact for indices below
+// 2^24 (checked by the host before launch).
 #pragma once
 #include "common.hpp"
 
@@ -42,34 +42,29 @@ void release_halton_tables(flg_ctx* ctx);
 HaltonParams make_halton_params(int resx, int resy);
 constexpr uint32_t kMaxHaltonIndex = 1u << 24;
 
-// Every quantity of the digit loop is an integer that a float holds exactly
-// (a < 2^24, its quotients and digits) or that a double holds exactly (the
-// reversed digits, below base * 2^24 < 2^53), so the loop runs on the float
-// and double units: the quotient from the float reciprocal, off by at most
-// one (|a * inv_base - a / base| <= 2 / base), corrected with the exact
-// remainder a - q * base (one fma, exact because the result is an integer
-// below 2^24), and the reversed digits accumulated with an exact fma in
-// double.  The result is the reference's: (float) of the same integer,
-// times the same float product chain of 1 / base.
-FLG_D float radical_inverse(int32_t base, float inv_base, uint32_t a) {
-  const float b = (float)base;
+// The digit loop on the double unit.  r = 1/b (1 + d) comes from the float
+// reciprocal r0 = (1 + e)/b, |e| <= 2^-24, by one Newton step: 1 - b r0 = -e
+// exactly (b r0 has at most 42 significant bits), and r = r0 (1 - e) rounded,
+// so |d| <= 2^-48 + 2^-53.  For a < 2^24 and a prime b < 2^18 the quotient is
+// q = floor(a r + 2^-22): a/b <= 2^23 moves by at most 2^23 |d| < 2^-24.9
+// and the sum's rounding by 2^-29, both below the bias, so an exact multiple
+// stays in (q, q + 1); otherwise a/b - q lies in [1/b, 1 - 1/b] with
+// 1/b >= 2^-18, far from both ends.  The remainder a - q b and the reversed
+// digits (below b a < 2^42) are exact in double fma.  The result is the
+// reference's: (float) of the same integer, times the same float product
+// chain of 1 / base.
+FLG_HD float radical_inverse(int32_t base, float inv_base, uint32_t a) {
   const double bd = (double)base;
-  float af = (float)a;
+  const double r0 = (double)inv_base;
+  const double r = __builtin_fma(r0, __builtin_fma(-bd, r0, 1.0), r0);
+  double ad = (double)a;
   double reversed = 0.0;
   float inv_base_n = 1.0f;
-  while (af != 0.0f) {
-    float q = floorf(af * inv_base);
-    float r = __builtin_fmaf(-q, b, af);
-    if (r < 0.0f) {
-      q -= 1.0f;
-      r += b;
-    } else if (r >= b) {
-      q += 1.0f;
-      r -= b;
-    }
-    reversed = __builtin_fma(reversed, bd, (double)r);
+  while (ad != 0.0) {
+    const double q = floor(__builtin_fma(ad, r, 0x1p-22));
+    reversed = __builtin_fma(reversed, bd, __builtin_fma(-q, bd, ad));
     inv_base_n *= inv_base;
-    af = q;
+    ad = q;
   }
   return (float)reversed * inv_base_n;
 }

@@ -396,6 +396,97 @@ GridVolume::GridVolume(const float* density, int nx, int ny, int nz, const AABB&
   pad(density);
 }
 
+GridVolume::GridVolume(int n_leaves, const int32_t* leaf_origin, const float* leaf_values, int nx, int ny, int nz,
+                       const AABB& bound, const Vector3f& sigma_s, const Vector3f& sigma_a, Float g,
+                       gpu::Context* ctx)
+    : m_ctx(ctx), m_nx(nx), m_ny(ny), m_nz(nz), m_sigma_s(sigma_s), m_sigma_a(sigma_a), m_g(g), m_sparse(true) {
+  using flg::kPoolBrick;
+  if (nx <= 0 || ny <= 0 || nz <= 0 || n_leaves < 0 || (n_leaves > 0 && (!leaf_origin || !leaf_values)))
+    throw gpu::Error(FLG_ERR_INVALID, "GridVolume: bad sparse grid");
+  for (int k = 0; k < 3; ++k)
+    if (!(bound.m_max[k] > bound.m_min[k])) throw gpu::Error(FLG_ERR_INVALID, "GridVolume: empty bounds");
+  const int64_t nbx = (nx + kPoolBrick - 1) / kPoolBrick, nby = (ny + kPoolBrick - 1) / kPoolBrick,
+                nbz = (nz + kPoolBrick - 1) / kPoolBrick;
+  if ((nbx + 1) * (nby + 1) * (nbz + 1) * flg::kPoolClasses >= ((int64_t)1 << 31) ||
+      (int64_t)n_leaves * flg::kPoolFloats >= ((int64_t)1 << 31))
+    throw gpu::Error(FLG_ERR_RANGE, "GridVolume: sparse grid too large");
+  m_aabb = bound;
+  m_origin.assign(leaf_origin, leaf_origin + size_t(n_leaves) * 3);
+  // brick cell -> leaf, as flg_volume_create_sparse checks and builds it
+  m_cell_leaf.assign(size_t(nbx * nby * nbz), -1);
+  uint64_t covered = 0;
+  for (int l = 0; l < n_leaves; ++l) {
+    const int32_t ox = m_origin[3 * l], oy = m_origin[3 * l + 1], oz = m_origin[3 * l + 2];
+    if (ox < 0 || oy < 0 || oz < 0 || ox >= nx || oy >= ny || oz >= nz || ((ox | oy | oz) & (kPoolBrick - 1)))
+      throw gpu::Error(FLG_ERR_INVALID, "GridVolume: a leaf origin is outside the grid or not a multiple of 8");
+    int32_t& c = m_cell_leaf[size_t(((int64_t)(oz >> 3) * nby + (oy >> 3)) * nbx + (ox >> 3))];
+    if (c >= 0) throw gpu::Error(FLG_ERR_INVALID, "GridVolume: two leaves share an origin");
+    c = l;
+    covered += uint64_t(std::min(kPoolBrick, nx - ox)) * std::min(kPoolBrick, ny - oy) * std::min(kPoolBrick, nz - oz);
+  }
+  m_covers_all = covered == uint64_t(nx) * ny * nz;
+  // slot map: per cell (-1 .. nb-1 per axis) and border class, the first
+  // stored brick among the cells the lookup's corners touch
+  const int64_t bcx = nbx + 1, bcy = nby + 1, bcz = nbz + 1;
+  m_bmap.assign(size_t(bcx * bcy * bcz * flg::kPoolClasses), -1);
+  for (int64_t cell = 0; cell < bcx * bcy * bcz; ++cell) {
+    const int cx = int(cell % bcx) - 1, cy = int((cell / bcx) % bcy) - 1, cz = int(cell / (bcx * bcy)) - 1;
+    for (int c = 0; c < flg::kPoolClasses; ++c)
+      for (int d = 0; d < flg::kPoolClasses; ++d) {
+        if (d & ~c) continue;
+        const int x = cx + (d & 1), y = cy + ((d >> 1) & 1), z = cz + ((d >> 2) & 1);
+        if (x < 0 || y < 0 || z < 0 || x >= nbx || y >= nby || z >= nbz) continue;
+        const int32_t l = m_cell_leaf[size_t((z * nby + y) * nbx + x)];
+        if (l >= 0) {
+          m_bmap[size_t(cell * flg::kPoolClasses + c)] = (l << 3) | d;
+          break;
+        }
+      }
+  }
+  fill_pool(leaf_values);
+}
+
+// Pool slots (the leaf and its one-voxel apron) and the maximum density of
+// the grid, from the leaf values.
+void GridVolume::fill_pool(const float* leaf_values) {
+  using flg::kPoolBrick;
+  using flg::kPoolEdge;
+  const int n_leaves = int(m_origin.size() / 3);
+  m_leaf_values.assign(leaf_values, leaf_values + size_t(n_leaves) * 512);
+  const int nbx = (m_nx + kPoolBrick - 1) / kPoolBrick, nby = (m_ny + kPoolBrick - 1) / kPoolBrick;
+  m_pool.assign(size_t(n_leaves > 0 ? n_leaves : 1) * flg::kPoolFloats, 0.0f);
+  float mx = m_covers_all ? -3.402823466e+38f : 0.0f;
+  for (int l = 0; l < n_leaves; ++l) {
+    const int ox = m_origin[3 * l], oy = m_origin[3 * l + 1], oz = m_origin[3 * l + 2];
+    for (int a = 0; a < flg::kPoolFloats; ++a) {
+      const int x = ox - 1 + a % kPoolEdge, y = oy - 1 + (a / kPoolEdge) % kPoolEdge,
+                z = oz - 1 + a / (kPoolEdge * kPoolEdge);
+      float v = 0.0f;
+      if (x >= 0 && y >= 0 && z >= 0 && x < m_nx && y < m_ny && z < m_nz) {
+        const int32_t src = m_cell_leaf[size_t(((int64_t)(z >> 3) * nby + (y >> 3)) * nbx + (x >> 3))];
+        if (src >= 0) v = m_leaf_values[(size_t(src) << 9) + size_t(((z & 7) << 6) | ((y & 7) << 3) | (x & 7))];
+      }
+      m_pool[size_t(l) * flg::kPoolFloats + a] = v;
+    }
+    for (int q = 0; q < 512; ++q) {
+      const int x = ox + (q & 7), y = oy + ((q >> 3) & 7), z = oz + (q >> 6);
+      if (x < m_nx && y < m_ny && z < m_nz) mx = std::max(mx, m_leaf_values[(size_t(l) << 9) + q]);
+    }
+  }
+  m_max_density = mx;
+}
+
+void GridVolume::updateLeaves(const float* leaf_values) {
+  if (!m_sparse) throw gpu::Error(FLG_ERR_INVALID, "GridVolume::updateLeaves: a dense grid takes update()");
+  if (!leaf_values && !m_origin.empty()) throw gpu::Error(FLG_ERR_INVALID, "GridVolume::updateLeaves: no values");
+  fill_pool(leaf_values);
+  if (m_vol && !m_origin.empty()) {
+    m_dev.upload(m_leaf_values.data(), m_leaf_values.size() * 4);
+    context().check(flg_volume_update_sparse(context().get(), m_vol, static_cast<const float*>(m_dev.data())),
+                    "flg_volume_update_sparse");
+  }
+}
+
 gpu::Context& GridVolume::context() const {
   if (!m_ctx) m_ctx = &gpu::Context::global();
   return *m_ctx;
@@ -422,6 +513,7 @@ void GridVolume::pad(const float* density) {
 }
 
 void GridVolume::update(const float* density) {
+  if (m_sparse) throw gpu::Error(FLG_ERR_INVALID, "GridVolume::update: a sparse grid takes updateLeaves()");
   pad(density);
   if (m_vol) {
     m_dev.upload(m_density.data(), m_density.size() * 4);
@@ -442,10 +534,23 @@ void GridVolume::setBound(const AABB& aabb) {
 
 flg_volume* GridVolume::handle() const {
   if (!m_vol) {
-    if (!m_dev.data()) m_dev = gpu::DeviceBuffer(context(), m_density.data(), m_density.size() * 4);
+    if (!m_sparse && !m_dev.data()) m_dev = gpu::DeviceBuffer(context(), m_density.data(), m_density.size() * 4);
     const float b[6] = {m_aabb.m_min[0], m_aabb.m_min[1], m_aabb.m_min[2],
                         m_aabb.m_max[0], m_aabb.m_max[1], m_aabb.m_max[2]};
     flg_medium med{m_sigma_s[0], m_sigma_a[0], m_g};
+    if (m_sparse) {
+      const int n_leaves = int(m_origin.size() / 3);
+      if (n_leaves > 0 && !m_dev_origin.data()) {
+        m_dev_origin = gpu::DeviceBuffer(context(), m_origin.data(), m_origin.size() * 4);
+        m_dev = gpu::DeviceBuffer(context(), m_leaf_values.data(), m_leaf_values.size() * 4);
+      }
+      context().check(flg_volume_create_sparse(context().get(), n_leaves,
+                                               static_cast<const int32_t*>(m_dev_origin.data()),
+                                               static_cast<const float*>(m_dev.data()), m_nx, m_ny, m_nz, b, &med,
+                                               &m_vol),
+                      "flg_volume_create_sparse");
+      return m_vol;
+    }
     context().check(flg_volume_create(context().get(), static_cast<const float*>(m_dev.data()), m_nx, m_ny,
                                    m_nz, b, &med, &m_vol),
                  "flg_volume_create");
@@ -454,16 +559,26 @@ flg_volume* GridVolume::handle() const {
 }
 
 namespace {
-flg::VolDev host_voldev(const std::vector<float>& padded, int nx, int ny, int nz, const AABB& box,
+flg::VolDev host_voldev(const float* data, const int32_t* bmap, int nx, int ny, int nz, const AABB& box,
                         const Vector3f& ss, const Vector3f& sa, Float g, float max_density) {
   const float b[6] = {box.m_min[0], box.m_min[1], box.m_min[2], box.m_max[0], box.m_max[1], box.m_max[2]};
-  return flg::voldev_of(padded.data(), nx, ny, nz, b, flg_medium{ss[0], sa[0], g}, max_density);
+  return flg::voldev_of(data, bmap, nx, ny, nz, b, flg_medium{ss[0], sa[0], g}, max_density);
 }
 }  // namespace
 
+float GridVolume::density(const Vector3f& p) const {
+  const flg::VolDev v = m_sparse ? host_voldev(m_pool.data(), m_bmap.data(), m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density)
+                                 : host_voldev(m_padded.data(), nullptr, m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density);
+  return flg::lookup(v, flg::v3(p[0], p[1], p[2])).rho;
+}
+
 Vector3f GridVolume::tr(const Ray& ray, Sampler& rng) const {
-  const flg::VolDev v = host_voldev(m_padded, m_nx, m_ny, m_nz, m_aabb, m_sigma_s, m_sigma_a, m_g,
-                                    m_max_density);
+  const flg::VolDev v = m_sparse ? host_voldev(m_pool.data(), m_bmap.data(), m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density)
+                                 : host_voldev(m_padded.data(), nullptr, m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density);
   SamplerDraw draw{rng};
   uint32_t nlook = 0;
   const float t = flg::vol_tr<false>(v, draw, flg::v3(ray.m_o[0], ray.m_o[1], ray.m_o[2]),
@@ -472,8 +587,10 @@ Vector3f GridVolume::tr(const Ray& ray, Sampler& rng) const {
 }
 
 Vector3f GridVolume::sample(const Ray& ray, Sampler& rng, VInteraction& vi, bool& success) const {
-  const flg::VolDev v = host_voldev(m_padded, m_nx, m_ny, m_nz, m_aabb, m_sigma_s, m_sigma_a, m_g,
-                                    m_max_density);
+  const flg::VolDev v = m_sparse ? host_voldev(m_pool.data(), m_bmap.data(), m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density)
+                                 : host_voldev(m_padded.data(), nullptr, m_nx, m_ny, m_nz, m_aabb, m_sigma_s,
+                                               m_sigma_a, m_g, m_max_density);
   SamplerDraw draw{rng};
   uint32_t nlook = 0;
   bool ok = false;
@@ -1004,7 +1121,7 @@ bool HIPRender::render() {
   cfg.film = 1;  // the reference's render loop and film, as CPURender's integrators use
   const size_t n = size_t(sc.m_resX) * sc.m_resY * 3;
   gpu::DeviceBuffer d(ctx, n * sizeof(float));
-  uint64_t stats[2] = {0, 0};
+  uint64_t stats[3] = {0, 0, 0};
   ctx.check(flg_volpath_mesh_render(ctx.get(), m_mesh->handle(), m_volume->handle(), &vs, &cfg,
                                     static_cast<float*>(d.data()), stats),
             "flg_volpath_mesh_render");

@@ -247,7 +247,7 @@ class LBVHBuilder : public BVHBuilderBase {
 }  // namespace experimental
 
 // ------------------------------------------------------------ Volume ------
-// Dense density grid (x fastest, nx*ny*nz floats) in a box, with the
+// Dense (x fastest, nx*ny*nz floats) or sparse-occupancy density grid in a box, with the
 // OpenVDBVolume medium model (volume.cpp:33-148): sigma_t = sigma_s + sigma_a
 // (first channel; grey media), majorant sigma_t * max density.  tr() and
 // sample() run the GPU's trackers on a host copy of the grid and make no GPU
@@ -258,14 +258,29 @@ class GridVolume : public Volume {
   GridVolume(const float* density, int nx, int ny, int nz, const AABB& bound,
              const Vector3f& sigma_s, const Vector3f& sigma_a, Float g,
              gpu::Context* ctx = nullptr);
+  // Sparse-occupancy grid: the 8^3 leaf bricks of an OpenVDB tree, which the
+  // reference reads through a grid accessor (volume.cpp:40-59,76-81).
+  // leaf_origin: 3 ints per leaf (voxel x, y, z of its first voxel, multiples
+  // of 8 inside the grid, no two alike); leaf_values: 512 floats per leaf, x
+  // fastest.  Voxels of no leaf are 0.  Host and device keep the brick pool of
+  // flg_volume_create_sparse; results equal the dense grid of the same values.
+  GridVolume(int n_leaves, const int32_t* leaf_origin, const float* leaf_values, int nx, int ny, int nz,
+             const AABB& bound, const Vector3f& sigma_s, const Vector3f& sigma_a, Float g,
+             gpu::Context* ctx = nullptr);
   ~GridVolume() override;
 
   Vector3f tr(const Ray& ray, Sampler& rng) const override;
   Vector3f sample(const Ray& ray, Sampler& rng, VInteraction& vi, bool& success) const override;
   void setBound(const AABB& aabb) override;
 
-  // Replaces the density values (same dimensions).
+  // Replaces the density values (same dimensions; a dense grid).
   void update(const float* density);
+  // Replaces the leaf values (same leaves; a sparse grid).
+  void updateLeaves(const float* leaf_values);
+  bool sparse() const { return m_sparse; }
+  // Trilinear density at world point p (0 outside the grid): the lookup
+  // both trackers make.
+  float density(const Vector3f& p) const;
   flg_volume* handle() const;  // uploads on first use
   float maxDensity() const { return m_max_density; }
   gpu::Context& context() const;
@@ -277,8 +292,18 @@ class GridVolume : public Volume {
   Vector3f m_sigma_s, m_sigma_a;
   Float m_g;
   float m_max_density = 0;
+  void fill_pool(const float* leaf_values);
   std::vector<float> m_padded;  // (nx+2)(ny+2)(nz+2), zero border
   std::vector<float> m_density;
+  // sparse grids: the leaves, the brick pool and its slot map
+  bool m_sparse = false;
+  bool m_covers_all = false;
+  std::vector<int32_t> m_origin;
+  std::vector<float> m_leaf_values;
+  std::vector<int32_t> m_cell_leaf;
+  std::vector<float> m_pool;
+  std::vector<int32_t> m_bmap;
+  mutable gpu::DeviceBuffer m_dev_origin;
   mutable flg_volume* m_vol = nullptr;
   mutable gpu::DeviceBuffer m_dev;
 };

@@ -46,10 +46,16 @@ struct MeshGeom {
   uint2* ring;  // the block's LDS ring, kMeshBlock * kMeshRing entries
 };
 
+// Counts of a measurement pass: density lookups and rays traced on the mesh.
+struct VpCount {
+  uint32_t look = 0, rays = 0;
+};
+
 // Closest hit on the mesh within tmax: sorted position (-1 on a miss), t, and
 // the geometric normal of the hit as the Pluecker test computes it.
-template <bool COMPAT>
-FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng) {
+template <bool COMPAT, bool COUNT>
+FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng, VpCount& cnt) {
+  if (COUNT) ++cnt.rays;
   float best = tmax;
   Counters c{0, 0};
   const int32_t p = trace_lane<COMPAT, false, kMeshBlock, kMeshRing>(m.g, o, d, best, c, m.ring);
@@ -63,18 +69,24 @@ FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* 
   return p;
 }
 
+template <bool COMPAT>
+FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng) {
+  VpCount none;
+  return mesh_hit<COMPAT, false>(m, o, d, tmax, t, ng, none);
+}
+
 // VolEstimateTr (integrator.cpp:40-64) along (o, d) up to tmax, starting
 // inside the medium or not.
 template <bool COMPAT, bool COUNT>
 FLG_D float shadow_tr(const MeshGeom& g, const VolDev& v, HaltonDraw& rng, V3 o, V3 d, float tmax,
-                      bool inside, uint32_t& nlook) {
+                      bool inside, VpCount& cnt) {
   float tr = 1.0f;
   float t_max = tmax;
   for (int k = 0; k < kMaxCrossings; ++k) {
     float t;
     V3 ng;
-    if (mesh_hit<COMPAT>(g, o, d, t_max, &t, &ng) < 0) return tr;
-    if (inside) tr *= vol_tr<COUNT>(v, rng, o, d, t, nlook);
+    if (mesh_hit<COMPAT, COUNT>(g, o, d, t_max, &t, &ng, cnt) < 0) return tr;
+    if (inside) tr *= vol_tr<COUNT>(v, rng, o, d, t, cnt.look);
     t_max -= t;
     if (t_max < 0.0f) return tr;
     const V3 ph = o + d * t;
@@ -158,23 +170,23 @@ FLG_D bool vp_film_next(const HaltonTables& tab, const HaltonParams& hp, const C
 // One trip of the reference's loop; returns true when the path is complete.
 template <bool COMPAT, bool COUNT>
 FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                   const HaltonParams& hp, int max_depth, VPath& s, uint32_t& nlook) {
+                   const HaltonParams& hp, int max_depth, VPath& s, VpCount& cnt) {
   float t;
   V3 ng;
-  const bool hit = mesh_hit<COMPAT>(g, s.o, s.d, kInf, &t, &ng) >= 0;
+  const bool hit = mesh_hit<COMPAT, COUNT>(g, s.o, s.d, kInf, &t, &ng, cnt) >= 0;
   if (s.bounces == 0 && !hit && L.env.present) s.L += s.beta * L.env.radiance;
   if (!hit || s.bounces >= max_depth) return true;
   bool success = false;
   V3 pos;
   if (s.inside) {
     HaltonDraw rng{s.ps, tab, hp};
-    s.beta *= vol_sample<COUNT>(v, rng, s.o, s.d, t, &success, &pos, nlook);
+    s.beta *= vol_sample<COUNT>(v, rng, s.o, s.d, t, &success, &pos, cnt.look);
   }
   if (success) {
     const V3 wo = -s.d;
     s.L += s.beta * sample_one_light(L, v.g, tab, hp, s.ps, pos, wo, [&](V3 p, V3 d, float tmax) {
       HaltonDraw rng{s.ps, tab, hp};
-      return shadow_tr<COMPAT, COUNT>(g, v, rng, p, d, tmax, true, nlook);
+      return shadow_tr<COMPAT, COUNT>(g, v, rng, p, d, tmax, true, cnt);
     });
     const float u = s.ps.next(tab, hp);
     const float vv = s.ps.next(tab, hp);
@@ -194,12 +206,19 @@ FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const
   return false;
 }
 
+// Adds the wave's counts into counts[0] (lookups) and counts[2] (rays).
 template <bool COUNT>
-FLG_D void vp_count(uint32_t nlook, unsigned long long* nlookups) {
+FLG_D void vp_count(const VpCount& cnt, unsigned long long* counts) {
   if (COUNT) {
-    unsigned long long c = nlook;
-    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(nlookups, c);
+    unsigned long long c = cnt.look, r = cnt.rays;
+    for (int o = 32; o >= 1; o >>= 1) {
+      c += __shfl_xor(c, o, 64);
+      r += __shfl_xor(r, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(counts, c);
+      atomicAdd(counts + 2, r);
+    }
   }
 }
 
@@ -231,15 +250,15 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g,
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t ntasks = (int64_t)cam.w * cam.h * tasks;
-  uint32_t nlook = 0;
+  VpCount cnt;
   if (p < ntasks) {
     VPath s;
     vp_task_begin<FILM>(tab, hp, cam, tasks, (uint32_t)p, s);
-    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook) || (FILM && !vp_film_next(tab, hp, cam, s))) {
+    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, cnt) || (FILM && !vp_film_next(tab, hp, cam, s))) {
     }
     vp_task_finish<FILM>((uint32_t)p, s, lpath, fpos);
   }
-  vp_count<COUNT>(nlook, nlookups);
+  vp_count<COUNT>(cnt, nlookups);
 }
 
 // Persistent lanes with path regeneration (regen_loop), paths started in
@@ -256,7 +275,7 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceG
   const MeshGeom mg{g, ring};
   const HaltonTables tab = stage_halton(tab_g, lds_primes);
   const uint32_t ntasks = (uint32_t)((int64_t)cam.w * cam.h * tasks);
-  uint32_t nlook = 0;
+  VpCount cnt;
   regen_loop<VPath>(
       ntasks, next_path,
       [&](uint32_t q, VPath& s) {
@@ -264,17 +283,17 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceG
         return false;
       },
       [&](VPath& s) {
-        return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, nlook) &&
+        return vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, cnt) &&
                (!FILM || vp_film_next(tab, hp, cam, s));
       },
       [&](uint32_t q, const VPath& s) {
         vp_task_finish<FILM>(tile_order(q, tasks, cam.w, cam.h), s, lpath, fpos);
       });
-  vp_count<COUNT>(nlook, nlookups);
+  vp_count<COUNT>(cnt, nlookups);
 }
 
 __global__ void k_vp_zero(unsigned long long* p) {
-  if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+  if (threadIdx.x < 3) p[threadIdx.x] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_vp_mean(const float* lpath, int64_t npix, int32_t spp, float* rgb) {
@@ -574,8 +593,9 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
     char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
     if (!sc) return st;
+    // nl[0]: lookups, nl[1]: the path counter, nl[2]: rays (cleared together)
     unsigned long long* nl = (unsigned long long*)sc;
-    unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
+    unsigned int* next_path = (unsigned int*)(nl + 1);
     float* lpath = (float*)(sc + 256);
     float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 4 + ((ntasks & 1) ? 4 : 0)) : nullptr;
     HaltonTables tab;
@@ -583,7 +603,8 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     LightsDev L;
     if ((st = make_lights(ctx, vol, scene, &L, mb))) return st;
     // the medium on the mesh's bounding box (setBound)
-    const VolDev v = voldev_of(vol->grid, vol->nx, vol->ny, vol->nz, mb, vol->medium, vol->max_density);
+    const VolDev v =
+        voldev_of(vol->grid, vol->bmap, vol->nx, vol->ny, vol->nz, mb, vol->medium, vol->max_density);
     const CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
     const TraceGeom g = geom_of(bvh);
     const bool compat = bvh->flags & FLG_NORMAL_COMPAT;
@@ -612,11 +633,12 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
       k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
     FLG_CHECK_LAUNCH(ctx);
     if (h_stats) {
-      unsigned long long c = 0;
-      FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+      unsigned long long c[3] = {0, 0, 0};
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(c, nl, 24, hipMemcpyDeviceToHost, ctx->stream));
       FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
       h_stats[0] = (uint64_t)(ntasks > 0 ? npix * cfg->spp : 0);
-      h_stats[1] = c;
+      h_stats[1] = c[0];
+      h_stats[2] = c[2];
     }
     return FLG_OK;
   });

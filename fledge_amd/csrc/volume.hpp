@@ -9,8 +9,14 @@ struct flg_volume {
   int32_t nx = 0, ny = 0, nz = 0;
   float bounds[6];
   flg_medium medium;
-  float* grid = nullptr;  // padded
+  float* grid = nullptr;  // dense: padded grid; sparse: the brick pool
   float max_density = 0;
+  // sparse-occupancy volumes (flg_volume_create_sparse), else null / 0
+  int32_t* bmap = nullptr;       // lookup slot map (flg::kPoolClasses entries per cell)
+  int32_t* cell_leaf = nullptr;  // brick cell -> leaf index, -1 for none (x fastest)
+  int32_t* leaf_origin = nullptr;
+  int32_t n_leaves = 0;
+  bool sparse_covers_all = false;  // the leaves cover every voxel of the grid
   // environment-light tables, rebuilt when the radiance changes
   float* env = nullptr;
   float env_radiance = -1.0f;
@@ -21,29 +27,53 @@ struct flg_volume {
 
 namespace flg {
 
-// Dense grid in HBM with a one-voxel zero border: (nx+2)(ny+2)(nz+2) floats,
-// x fastest; voxel (i,j,k) of the caller's grid is at ((k+1)*py + j+1)*px + i+1.
+// Sparse-occupancy grids keep each active 8^3 brick in a pool slot of 10^3
+// floats: the brick and a one-voxel apron on every side holding the
+// neighbouring voxels (0 outside the grid or in bricks that are not stored).
+// A trilinear lookup whose lower corner is voxel (i, j, k) reads voxels
+// {i, i+1} x {j, j+1} x {k, k+1}; every brick that holds one of them holds
+// all eight in its apron, so the lookup reads one slot.  The slot map has, for
+// each brick cell (cells -1 .. nb-1 per axis, so that corner -1 of the zero
+// border has a cell) and each border class c (bit a set when the lower corner
+// is the last voxel of its brick along axis a, so that the eight corners
+// reach into the next brick along a), the entry (slot << 3) | d of a stored
+// brick among the cells the corners touch, d its offset from the cell (a
+// subset of c), or -1 when none is stored: all eight corners are then 0.
+constexpr int kPoolBrick = 8;
+constexpr int kPoolEdge = kPoolBrick + 2;
+constexpr int kPoolFloats = kPoolEdge * kPoolEdge * kPoolEdge;
+constexpr int kPoolClasses = 8;
+
+// Dense grids are in HBM with a one-voxel zero border: (nx+2)(ny+2)(nz+2)
+// floats, x fastest; voxel (i,j,k) of the caller's grid is at
+// ((k+1)*py + j+1)*px + i+1.  Sparse grids: the brick pool and its slot map.
 struct VolDev {
-  const float* grid;
+  const float* grid;    // dense padded grid, or the brick pool when bmap is set
+  const int32_t* bmap;  // null: dense
   int32_t nx, ny, nz;
-  int32_t px, pxy;  // padded row and slice strides
+  int32_t px, pxy;  // padded row and slice strides (dense)
+  int32_t bcx, bcy;  // slot-map cells per row and per layer (sparse)
   V3 bmin, bmax, scale;
   float sigma_s, sigma_a, sigma_t, g;
   float inv_max_density, max_mu;
 };
 
-// Device (or host-mirror) description of a padded grid.  OpenVDBVolume ctor
-// (volume.cpp:33-59): sigma_t = sigma_a + sigma_s, the inverse max density
-// through a double division.
-inline VolDev voldev_of(const float* padded, int32_t nx, int32_t ny, int32_t nz,
+// Device (or host-mirror) description of a grid: `data` is the padded dense
+// grid (bmap null) or the brick pool of the slot map bmap.  OpenVDBVolume
+// ctor (volume.cpp:33-59): sigma_t = sigma_a + sigma_s, the inverse max
+// density through a double division.
+inline VolDev voldev_of(const float* data, const int32_t* bmap, int32_t nx, int32_t ny, int32_t nz,
                         const float* bounds, flg_medium m, float max_density) {
   VolDev v;
-  v.grid = padded;
+  v.grid = data;
+  v.bmap = bmap;
   v.nx = nx;
   v.ny = ny;
   v.nz = nz;
   v.px = nx + 2;
   v.pxy = (nx + 2) * (ny + 2);
+  v.bcx = (nx + kPoolBrick - 1) / kPoolBrick + 1;
+  v.bcy = (ny + kPoolBrick - 1) / kPoolBrick + 1;
   v.bmin = v3(bounds[0], bounds[1], bounds[2]);
   v.bmax = v3(bounds[3], bounds[4], bounds[5]);
   v.scale = v3((float)nx / (v.bmax.x - v.bmin.x), (float)ny / (v.bmax.y - v.bmin.y),
@@ -57,15 +87,36 @@ inline VolDev voldev_of(const float* padded, int32_t nx, int32_t ny, int32_t nz,
   return v;
 }
 
-struct Tri8 {  // one trilinear lookup: padded base index and weights
-  int32_t base;  // -1 when the lookup is outside the grid (density 0)
+struct Tri8 {  // one trilinear lookup: lower corner, weights, value
+  int32_t i, j, k;  // lower corner voxel, each >= -1
+  bool inside;      // false when the lookup is outside the grid (density 0)
   float fx, fy, fz;
   float rho;
 };
 
+// The eight voxels' trilinear blend: rows g0 (y, z), g0 + sy (y + 1), g0 + sz
+// (z + 1), g0 + sz + sy, each two adjacent floats.
+FLG_HD float trilerp(const float* g0, int sy, int sz, float fx, float fy, float fz) {
+  const float* g1 = g0 + sy;
+  const float* g2 = g0 + sz;
+  const float* g3 = g2 + sy;
+  float wx0 = 1.0f - fx, wx1 = fx;
+  float wy0 = 1.0f - fy, wy1 = fy;
+  float wz0 = 1.0f - fz, wz1 = fz;
+  float c00 = g0[0] * wx0 + g0[1] * wx1;
+  float c10 = g1[0] * wx0 + g1[1] * wx1;
+  float c01 = g2[0] * wx0 + g2[1] * wx1;
+  float c11 = g3[0] * wx0 + g3[1] * wx1;
+  float c0 = c00 * wy0 + c10 * wy1;
+  float c1 = c01 * wy0 + c11 * wy1;
+  return c0 * wz0 + c1 * wz1;
+}
+
 // Trilinear lookup: index space g = (p - bmin) * scale - 0.5, zero outside the
 // grid, lerps along x, then y, then z (the library's definition of the dense
 // grid's interpolation; OpenVDB's BoxSampler is not in the reference tree).
+// A sparse grid gives the same bits as the dense grid holding the same
+// values: the same eight voxels, the same operations.
 FLG_HD Tri8 lookup(const VolDev& v, V3 p) {
   float gx = (p.x - v.bmin.x) * v.scale.x - 0.5f;
   float gy = (p.y - v.bmin.y) * v.scale.y - 0.5f;
@@ -78,26 +129,25 @@ FLG_HD Tri8 lookup(const VolDev& v, V3 p) {
   int i = (int)smin(smax(flx, -2.0f), (float)v.nx + 1.0f);
   int j = (int)smin(smax(fly, -2.0f), (float)v.ny + 1.0f);
   int k = (int)smin(smax(flz, -2.0f), (float)v.nz + 1.0f);
-  if (i < -1 || j < -1 || k < -1 || i > v.nx - 1 || j > v.ny - 1 || k > v.nz - 1) {
-    r.base = -1;
-    r.rho = 0.0f;
+  r.i = i;
+  r.j = j;
+  r.k = k;
+  r.rho = 0.0f;
+  r.inside = !(i < -1 || j < -1 || k < -1 || i > v.nx - 1 || j > v.ny - 1 || k > v.nz - 1);
+  if (!r.inside) return r;
+  if (!v.bmap) {  // dense (a uniform branch: one grid per launch)
+    r.rho = trilerp(v.grid + (((k + 1) * (v.ny + 2) + (j + 1)) * v.px + (i + 1)), v.px, v.pxy, r.fx, r.fy, r.fz);
     return r;
   }
-  r.base = ((k + 1) * (v.ny + 2) + (j + 1)) * v.px + (i + 1);
-  const float* g0 = v.grid + r.base;
-  const float* g1 = g0 + v.px;
-  const float* g2 = g0 + v.pxy;
-  const float* g3 = g2 + v.px;
-  float wx0 = 1.0f - r.fx, wx1 = r.fx;
-  float wy0 = 1.0f - r.fy, wy1 = r.fy;
-  float wz0 = 1.0f - r.fz, wz1 = r.fz;
-  float c00 = g0[0] * wx0 + g0[1] * wx1;
-  float c10 = g1[0] * wx0 + g1[1] * wx1;
-  float c01 = g2[0] * wx0 + g2[1] * wx1;
-  float c11 = g3[0] * wx0 + g3[1] * wx1;
-  float c0 = c00 * wy0 + c10 * wy1;
-  float c1 = c01 * wy0 + c11 * wy1;
-  r.rho = c0 * wz0 + c1 * wz1;
+  const int cls = ((i & 7) == 7 ? 1 : 0) | ((j & 7) == 7 ? 2 : 0) | ((k & 7) == 7 ? 4 : 0);
+  const int cell = (((k >> 3) + 1) * v.bcy + ((j >> 3) + 1)) * v.bcx + ((i >> 3) + 1);
+  const int e = v.bmap[cell * kPoolClasses + cls];
+  if (e < 0) return r;  // no stored brick holds a corner: all eight are 0
+  const int lx = (i & 7) + 1 - ((e & 1) << 3);
+  const int ly = (j & 7) + 1 - ((e & 2) << 2);
+  const int lz = (k & 7) + 1 - ((e & 4) << 1);
+  r.rho = trilerp(v.grid + ((e >> 3) * kPoolFloats + (lz * kPoolEdge + ly) * kPoolEdge + lx), kPoolEdge,
+                  kPoolEdge * kPoolEdge, r.fx, r.fy, r.fz);
   return r;
 }
 

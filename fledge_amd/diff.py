@@ -42,8 +42,25 @@ class DiffVolume:
         self.vol = Volume(ctx, density.detach(), bounds, 1.0, 0.0, g)
         self.proposal = None
 
+    @classmethod
+    def sparse(cls, ctx, origins, values, shape, bounds=(0, 0, 0, 1, 1, 1), g=0.3):
+        """The evaluated grid as a sparse-occupancy volume: 8^3 bricks at
+        voxel origins `origins` with values `values` (api.Volume.sparse); a
+        gradient over its bricks uses BrickLayout.of_volume(self.vol)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.bounds = bounds
+        self.g = g
+        self.vol = Volume.sparse(ctx, origins, values.detach(), shape, bounds, 1.0, 0.0, g)
+        self.proposal = None
+        return self
+
     def set_density(self, density):
-        self.vol.update(density.detach())
+        """New densities: a dense grid, or for a sparse volume its brick values."""
+        if getattr(self.vol, "n_leaves", None) is not None:
+            self.vol.update_sparse(density.detach())
+        else:
+            self.vol.update(density.detach())
 
     def set_proposal(self, density):
         if density is None:
@@ -156,6 +173,29 @@ class BrickLayout:
         self.c.nx, self.c.ny, self.c.nz = nx, ny, nz
         self.c.d_slot = self.slot.data_ptr()
 
+    @classmethod
+    def of_volume(cls, vol):
+        """The bricks of a sparse volume (flg_volume_brick_layout): brick 8,
+        slot = the brick's index in the volume's leaf list, so a gradient in
+        this layout is shaped as the volume's brick values."""
+        ctx = vol.ctx
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.brick = 8
+        self.shape = tuple(vol.shape)
+        self.nb = tuple((n + 7) // 8 for n in self.shape)
+        self.c = N.BrickLayout()
+        ctx.check(ctx.lib.flg_volume_brick_layout(vol.h, ctypes.byref(self.c)))
+        self.n_active = int(self.c.n_active)
+        # the same map on the torch side (to_dense), from the leaf origins
+        o = vol.origins.long()
+        self.slot = torch.full(self.nb, -1, dtype=torch.int32, device=torch.device("cuda", ctx.device))
+        if self.n_active:
+            self.slot[o[:, 2] // 8, o[:, 1] // 8, o[:, 0] // 8] = torch.arange(
+                self.n_active, dtype=torch.int32, device=self.slot.device)
+        self.vol = vol  # the C layout points into the volume
+        return self
+
     def zeros(self):
         """A gradient buffer in this layout."""
         b = self.brick
@@ -244,7 +284,7 @@ class VolumeRender(torch.autograd.Function):
         return grad, galpha.reshape(()), None, None, None, None, None, None, None, None
 
 
-def bench_adjoint(ctx, timed, sync, state="uniform"):
+def bench_adjoint(ctx, timed, sync, state="uniform", profile=False):
     """128^3 grid, 256x256, 16 spp, depth 32, L2 loss against a target
     rendered from an fBm grid (seed 2); point light and environment as in the
     256^3 volume render, albedo 0.8, g 0.3, sigma_t = 8 density.  The
@@ -254,7 +294,9 @@ def bench_adjoint(ctx, timed, sync, state="uniform"):
     state with walks of ordinary length).  Returns timings of the plain
     forward, the differentiable forward and the adjoint at both schedules,
     one fwd+adjoint iteration, the library's kernel times and the grid
-    lookups per path of each pass."""
+    lookups per path of each pass.  With `profile`, runs one plain forward and
+    one iteration (forward with the walk tape, then the adjoint) at the
+    regenerating schedule and returns None (for hardware-counter runs)."""
     from . import api
 
     dev = torch.device("cuda", ctx.device)
@@ -278,6 +320,13 @@ def bench_adjoint(ctx, timed, sync, state="uniform"):
     grad = torch.zeros((128, 128, 128), dtype=torch.float32, device=dev)
     galpha = torch.zeros(1, dtype=torch.float32, device=dev)
     tape = new_tape(ctx, W, H, SPP, DEPTH)
+    if profile:
+        plain.render(scene, W, H, SPP, DEPTH, out=img_plain, schedule=2)
+        forward(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, out=img, lpath=lpath, schedule=2, tape=tape)
+        adjoint(dv, scene, W, H, SPP, DEPTH, S, 0.8, mu, lpath, 2.0 * (img - target) / img.numel(), grad, galpha,
+                schedule=2, tape=tape)
+        sync()
+        return None
 
     # every pass at both schedules (1: one lane per path, 2: persistent lanes
     # with path regeneration); the ratio compares the fastest of each

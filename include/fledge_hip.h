@@ -329,6 +329,27 @@ FLG_API int flg_volume_destroy(flg_volume* vol);
 /* Replaces the density values (same dimensions); recomputes the majorant. */
 FLG_API int flg_volume_update(flg_ctx* ctx, flg_volume* vol, const float* d_density);
 
+/* Sparse-occupancy density grid: the leaf level of the OpenVDB tree that the
+ * reference reads through a grid accessor (OpenVDBVolume ctor and
+ * OpenVDBVolume::tr/sample, src/volume.cpp:33-59,75-148): n_leaves active
+ * 8^3 bricks.  d_leaf_origin (device, 3 * n_leaves int32): voxel (x, y, z) of
+ * each brick's first voxel, multiples of 8 inside the grid, no two alike;
+ * d_leaf_values (device, 512 * n_leaves floats): each brick's voxels, x
+ * fastest (voxels past the grid's edge are ignored).  Every voxel in no
+ * brick is 0.  Every render and adjoint entry point takes the result, and
+ * gives the bits of the dense volume holding the same values.  Device memory:
+ * 4000 B per brick (the brick with a one-voxel apron, so that a lookup reads
+ * one brick) plus 32 B per 8^3 cell of the grid (the slot map, one 4-byte
+ * read per lookup); n_leaves * 1000 and the cell count * 8 must stay below
+ * 2^31 (FLG_ERR_RANGE).  FLG_ERR_INVALID for a bad or repeated origin. */
+FLG_API int flg_volume_create_sparse(flg_ctx* ctx, int32_t n_leaves, const int32_t* d_leaf_origin,
+                                     const float* d_leaf_values, int32_t nx, int32_t ny, int32_t nz,
+                                     const float* h_bounds, const flg_medium* medium, flg_volume** out);
+/* New values for a sparse volume's bricks (same bricks, same order);
+ * recomputes the majorant.  flg_volume_update refuses a sparse volume, and
+ * this call a dense one. */
+FLG_API int flg_volume_update_sparse(flg_ctx* ctx, flg_volume* vol, const float* d_leaf_values);
+
 typedef struct flg_vol_scene {
   flg_camera camera;
   int32_t has_point;
@@ -391,7 +412,9 @@ FLG_API int flg_volpath_render_noalloc(flg_ctx* ctx, flg_volume* vol, const flg_
  * VolEstimateTr does (integrator.cpp:40-64).  cfg->film 0: per-pixel mean
  * of the path samples on the per-sample Halton schedule; 1: the reference's
  * render loop and film, as flg_volpath_render's film mode.  h_stats (may be
- * NULL): {path samples, density lookups}. */
+ * NULL): {path samples, density lookups, rays traced on the mesh (camera
+ * rays, continuations after a scattering or an interface, shadow-ray
+ * segments)}, three entries. */
 FLG_API int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
                                     const flg_vol_scene* scene, const flg_render_cfg* cfg, float* d_rgb,
                                     uint64_t* h_stats);
@@ -461,6 +484,12 @@ typedef struct flg_brick_layout {
  * active bricks (host). */
 FLG_API int flg_bricks_build(flg_ctx* ctx, const float* d_density, int32_t nx, int32_t ny, int32_t nz,
                              int32_t brick, int32_t* d_slot, int32_t* h_n_active);
+
+/* The brick layout of a sparse volume's bricks (brick 8, slot = the brick's
+ * index in flg_volume_create_sparse's leaf list), for gradients over exactly
+ * those bricks (shaped as the leaf values).  d_slot belongs to the volume and
+ * stays valid while it lives.  FLG_ERR_INVALID for a dense volume. */
+FLG_API int flg_volume_brick_layout(const flg_volume* vol, flg_brick_layout* out);
 
 /* The differentiable estimator of flg_volpath_diff_forward over the scene of
  * flg_volpath_mesh_render: the mesh in `bvh` is the index-matched interface

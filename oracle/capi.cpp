@@ -351,6 +351,29 @@ void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t
     volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
 }
 
+// orc_volpath_mesh_render (per-sample schedule) on every stride-th pixel
+// only: a bounded sample of a frame for timing; paths_out = path samples run.
+void orc_volpath_mesh_render_strided(void* p, int32_t compat, const float* grid, int32_t nx, int32_t ny,
+                                     int32_t nz, const float* medium3, const float* lights7, const float* cam10,
+                                     int32_t W, int32_t H, int32_t spp, int32_t max_depth, int64_t stride,
+                                     float* rgb, int nthreads, uint64_t* lookups, uint64_t* paths_out) {
+  const Lbvh& b = ((Handle*)p)->b;
+  const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
+  VolScene sc;
+  init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
+  sc.point.present = lights7[0] != 0.0f;
+  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
+  sc.point.intensity = lights7[4];
+  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  Camera cam = make_camera(cam10);
+  sc.cam = &cam;
+  sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
+  sc.pix_stride = stride < 1 ? 1 : stride;
+  volpath_mesh_render(b, compat != 0, sc, rgb, nthreads, lookups);
+  const int64_t npix = (int64_t)W * H;
+  if (paths_out) *paths_out = (uint64_t)((npix + sc.pix_stride - 1) / sc.pix_stride) * (uint64_t)spp;
+}
+
 // Scene::intersect over the meshes of handles[0..n) in scene order.
 void orc_scene_trace(void* const* handles, int32_t n, int32_t compat, const float* rays, int64_t nrays,
                      int32_t* mesh, int32_t* ids, float* ts, int nthreads) {

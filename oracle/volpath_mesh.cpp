@@ -124,7 +124,8 @@ float volpath_Li(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 
 
 }  // namespace
 
-// Per pixel the in-order mean of spp samples on the per-sample schedule.
+// Per pixel the in-order mean of spp samples on the per-sample schedule;
+// pixels 0, pix_stride, 2 pix_stride, ... only (the others keep rgb's values).
 void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, int nthreads,
                          uint64_t* lookups) {
   const int W = sc.width, H = sc.height;
@@ -137,7 +138,7 @@ void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* 
     th.emplace_back([&, t] {
       Halton hs((uint64_t)sc.spp, W, H);
       while (true) {
-        const int64_t pix = next.fetch_add(1);
+        const int64_t pix = next.fetch_add(1) * sc.pix_stride;
         if (pix >= npix) break;
         const int x = (int)(pix % W), y = (int)(pix / W);
         float color = 0.0f;

@@ -101,6 +101,106 @@ void test_grid_volume_host() {
   std::printf("PASS grid_volume_host (%d attenuated, %d collisions)\n", ntr, nsample_hits);
 }
 
+// A sparse GridVolume (8^3 leaves with a brick pool) against the dense one
+// holding the same values: bit-equal densities at random points, at voxel
+// centres and on brick borders (inside, on the zero border, outside), equal
+// tr/sample results and draw counts, equal maxima; grids whose sides are not
+// multiples of 8; no leaves; every leaf; the origins it refuses.
+void test_sparse_volume_host() {
+  const int nx = 37, ny = 29, nz = 21;
+  std::vector<float> grid(size_t(nx) * ny * nz);
+  flg_scene_fbm_grid(nx, ny, nz, 3, 4.0f, 5, 0.0f, 1.0f, grid.data());
+  const int bx = (nx + 7) / 8, by = (ny + 7) / 8, bz = (nz + 7) / 8;
+  std::mt19937 gen(17);
+  std::uniform_real_distribution<float> U(0.0f, 1.0f);
+  for (int mode = 0; mode < 3; ++mode) {  // 0: half the bricks, 1: every brick, 2: none
+    std::vector<float> g(grid);
+    std::vector<int32_t> origin;
+    std::vector<float> values;
+    for (int k = 0; k < bz; ++k)
+      for (int j = 0; j < by; ++j)
+        for (int i = 0; i < bx; ++i) {
+          const bool keep = mode == 1 || (mode == 0 && U(gen) < 0.5f);
+          for (int z = 8 * k; z < std::min(8 * k + 8, nz); ++z)
+            for (int y = 8 * j; y < std::min(8 * j + 8, ny); ++y)
+              for (int x = 8 * i; x < std::min(8 * i + 8, nx); ++x)
+                if (!keep) g[(size_t(z) * ny + y) * nx + x] = 0.0f;
+          if (!keep) continue;
+          origin.insert(origin.end(), {8 * i, 8 * j, 8 * k});
+          for (int q = 0; q < 512; ++q) {
+            const int x = 8 * i + (q & 7), y = 8 * j + ((q >> 3) & 7), z = 8 * k + (q >> 6);
+            // voxels past the grid's edge hold junk: they must be ignored
+            values.push_back(x < nx && y < ny && z < nz ? g[(size_t(z) * ny + y) * nx + x] : 1e30f);
+          }
+        }
+    const fledge::AABB box(Vector3f(-0.5f, 0.0f, 0.25f), Vector3f(1.0f, 1.25f, 1.0f));
+    fledge::GridVolume dense(g.data(), nx, ny, nz, box, Vector3f(6.4f), Vector3f(1.6f), 0.3f);
+    fledge::GridVolume sparse(int(origin.size() / 3), origin.data(), values.data(), nx, ny, nz, box,
+                              Vector3f(6.4f), Vector3f(1.6f), 0.3f);
+    CHECK(bits(dense.maxDensity()) == bits(sparse.maxDensity()), "mode %d: max density %.9g vs %.9g", mode,
+          dense.maxDensity(), sparse.maxDensity());
+    const float lo[3] = {-0.5f, 0.0f, 0.25f}, ext[3] = {1.5f, 1.25f, 0.75f};
+    const int n[3] = {nx, ny, nz};
+    int nonzero = 0;
+    for (int t = 0; t < 200000; ++t) {
+      Vector3f p;
+      for (int a = 0; a < 3; ++a) {
+        float u = U(gen) * 1.2f - 0.1f;
+        if (t % 4 == 1) {  // on a brick border: index-space coordinate 8 m - 0.5 + small
+          const int m = int(U(gen) * (n[a] / 8 + 2)) - 1;
+          u = (8.0f * m + 0.5f * U(gen) - 0.25f + 0.5f) / n[a];
+        } else if (t % 4 == 2) {  // a voxel centre, or halfway between two
+          u = (std::floor(U(gen) * (n[a] + 2)) - 1.0f + 0.5f * (t % 8 == 2 ? 1.0f : 0.0f)) / n[a];
+        }
+        p[a] = lo[a] + u * ext[a];
+      }
+      const float a = dense.density(p), b = sparse.density(p);
+      CHECK(bits(a) == bits(b), "mode %d: density at (%.9g, %.9g, %.9g): %.9g vs %.9g", mode, p[0], p[1], p[2], a,
+            b);
+      nonzero += a != 0.0f;
+    }
+    CHECK(mode == 2 ? nonzero == 0 : nonzero > 20000, "mode %d: %d non-zero lookups", mode, nonzero);
+    int ncoll = 0;
+    for (int i = 0; i < 3000; ++i) {
+      const Vector3f o(U(gen) * 3 - 1.5f, U(gen) * 3 - 1, -1.0f + U(gen));
+      const float tx = U(gen) * 0.6f - 0.2f, ty = 0.5f + U(gen) * 0.5f, tz = 0.6f + U(gen) * 0.2f;
+      const float dx = tx - o[0], dy = ty - o[1], dz = tz - o[2];
+      const float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+      const Vector3f d(dx / len, dy / len, dz / len);
+      const fledge::Ray ray(o, d, (i % 3 == 0) ? 1.5f : INFINITY);
+      HaltonStream a(128, i % 128, (i / 128) % 128, 1 + i / 16384), b = a;
+      CHECK(bits(dense.tr(ray, a)[0]) == bits(sparse.tr(ray, b)[0]) && a.h.cnt == b.h.cnt, "mode %d: tr ray %d",
+            mode, i);
+      fledge::VInteraction va, vb;
+      bool oa = false, ob = false;
+      const Vector3f fa = dense.sample(ray, a, va, oa), fb = sparse.sample(ray, b, vb, ob);
+      CHECK(oa == ob && bits(fa[0]) == bits(fb[0]) && a.h.cnt == b.h.cnt, "mode %d: sample ray %d", mode, i);
+      if (oa) {
+        CHECK(bits(va.m_p[0]) == bits(vb.m_p[0]) && bits(va.m_p[1]) == bits(vb.m_p[1]) &&
+                  bits(va.m_p[2]) == bits(vb.m_p[2]),
+              "mode %d: sample ray %d position", mode, i);
+        ++ncoll;
+      }
+    }
+    CHECK(mode == 2 ? ncoll == 0 : ncoll > 100, "mode %d: %d collisions", mode, ncoll);
+    std::printf("PASS sparse_volume_host mode %d (%zu leaves, %d collisions)\n", mode, origin.size() / 3, ncoll);
+  }
+  // origins it refuses
+  const std::vector<float> v(512 * 2, 1.0f);
+  const int32_t bad[][6] = {{0, 0, 4, 8, 8, 8}, {0, 0, 0, 0, 0, 0}, {0, 0, 0, 40, 0, 0}, {-8, 0, 0, 0, 0, 0}};
+  for (const auto& o : bad) {
+    bool threw = false;
+    try {
+      fledge::GridVolume s(2, o, v.data(), nx, ny, nz, fledge::AABB(Vector3f(0.0f), Vector3f(1.0f)), Vector3f(1.0f),
+                           Vector3f(0.0f), 0.0f);
+    } catch (const fledge::gpu::Error& e) {
+      threw = e.status() == FLG_ERR_INVALID;
+    }
+    CHECK(threw, "leaf origins (%d %d %d), (%d %d %d) must be refused", o[0], o[1], o[2], o[3], o[4], o[5]);
+  }
+  std::printf("PASS sparse_volume_host errors\n");
+}
+
 void test_host_errors() {
   float one = 1.0f;
   bool threw = false;
@@ -693,6 +793,7 @@ void test_memory_resources(fledge::gpu::Context& ctx) {
 int main(int argc, char** argv) {
   const bool cpu_only = argc > 1 && std::string(argv[1]) == "--cpu";
   test_grid_volume_host();
+  if (!g_failures) test_sparse_volume_host();
   test_host_errors();
   if (!g_failures) test_ply_host();
   if (!g_failures) test_scene_file_host();

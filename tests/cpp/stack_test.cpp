@@ -3,6 +3,8 @@
 // private stack and with the LDS ring stack at ring sizes 1, 2 and 8 (host
 // memory standing in for LDS; small rings force the spill path), on a tree
 // packed from the oracle's build, against the oracle's traversal.
+// Also checks the device Halton digit loop (halton.hpp radical_inverse, run
+// on the CPU) against integer division.
 // Built with hipcc as host code; runs without a GPU.
 #include <cstdio>
 #include <cstring>
@@ -171,7 +173,65 @@ void check_tile_order() {
   std::printf("tile_order: checked\n");
 }
 
+// radical_inverse divides in double with a biased refined reciprocal; its
+// result must equal RadicalInverse_'s integer digit loop (sampler.h:59-72)
+// for every prime below 2^18 and every index below 2^24.  Checked: the
+// exact multiples and their neighbours at the top of the range (where the
+// rounding argument is tightest), small powers, random indices for every
+// prime, and every index for bases 2 and 3.
+float radical_inverse_int(uint32_t base, uint32_t a) {
+  const float inv_base = 1.0f / (float)base;
+  uint64_t reversed = 0;
+  float inv_base_n = 1.0f;
+  while (a) {
+    const uint32_t next = a / base;
+    reversed = reversed * base + (a - next * base);
+    inv_base_n *= inv_base;
+    a = next;
+  }
+  return (float)reversed * inv_base_n;
+}
+
+void check_radical_inverse() {
+  std::vector<int32_t> primes;
+  {
+    std::vector<char> comp(1 << 18, 0);
+    for (int i = 2; i < (1 << 18); ++i) {
+      if (comp[i]) continue;
+      primes.push_back(i);
+      for (int64_t j = (int64_t)i * i; j < (1 << 18); j += i) comp[j] = 1;
+    }
+  }
+  const uint32_t top = 1u << 24;
+  std::mt19937 gen(11);
+  std::uniform_int_distribution<uint32_t> U(0, top - 1);
+  int64_t checked = 0;
+  auto one = [&](int32_t p, uint32_t a) {
+    const float got = flg::radical_inverse(p, 1.0f / (float)p, a), want = radical_inverse_int((uint32_t)p, a);
+    ++checked;
+    if (flg::f2u(got) != flg::f2u(want) && failures < 20) {
+      std::printf("FAIL radical_inverse(%d, %u) = %.9g, integer loop %.9g\n", p, a, got, want);
+      ++failures;
+    }
+  };
+  for (int32_t p : primes) {
+    const uint32_t b = (uint32_t)p, kmax = (top - 1) / b;
+    for (uint32_t k : {kmax, kmax - 1, kmax / 2 + 1, 1u, 2u})
+      for (int d = -1; d <= 1; ++d) {
+        const int64_t a = (int64_t)k * b + d;
+        if (a >= 0 && a < top) one(p, (uint32_t)a);
+      }
+    for (uint64_t pw = b; pw < top; pw *= b)
+      for (int d = -1; d <= 1; ++d) one(p, (uint32_t)(pw + d));
+    for (int i = 0; i < 64; ++i) one(p, U(gen));
+  }
+  for (int32_t p : {2, 3})
+    for (uint32_t a = 0; a < top; ++a) one(p, a);
+  std::printf("radical_inverse: %lld cases checked\n", (long long)checked);
+}
+
 int main() {
+  check_radical_inverse();
   check_tile_order();
   {  // bvh_test: MakeMeshedSphere(200, 200, 5), rays from inside
     const int nt = 200, np = 200;

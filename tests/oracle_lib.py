@@ -52,6 +52,8 @@ _SIGS = {
     "orc_mitchell1d": (f32, [f32]),
     "orc_volpath_mesh_render": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P,
                                        ctypes.c_int, P, i32]),
+    "orc_volpath_mesh_render_strided": (None, [P, ctypes.c_int, P, i32, i32, i32, P, P, P, i32, i32, i32, i32,
+                                               i64, P, ctypes.c_int, P, P]),
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
     "orc_scene_trace": (None, [P, i32, i32, P, i64, P, P, P, ctypes.c_int]),
     "orc_diff_mesh_render": (None, [P, i32, P, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
@@ -146,6 +148,20 @@ class Lbvh:
                                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
                                       nthreads or NTHREADS, _p(lk), film)
         return rgb, int(lk[0])
+
+    def volpath_mesh_strided(self, grid, medium, lights, cam10, W, H, spp, max_depth, stride, nthreads=None):
+        """volpath_mesh on every stride-th pixel only: (rgb, lookups, path
+        samples run); the other pixels stay 0."""
+        grid = np.ascontiguousarray(grid, np.float32)
+        nz, ny, nx = grid.shape
+        rgb = np.zeros((H, W, 3), np.float32)
+        lk = np.zeros(1, np.uint64)
+        pa = np.zeros(1, np.uint64)
+        lib().orc_volpath_mesh_render_strided(self.h, 0, _p(grid), nx, ny, nz, _p(np.asarray(medium, np.float32)),
+                                              _p(np.asarray(lights, np.float32)), _p(np.asarray(cam10, np.float32)),
+                                              W, H, spp, max_depth, stride, _p(rgb), nthreads or NTHREADS, _p(lk),
+                                              _p(pa))
+        return rgb, int(lk[0]), int(pa[0])
 
     def diff_render(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, compat=False,
                     nthreads=None):

@@ -798,3 +798,23 @@ def test_scene_trace_is_lexicographic_min_over_meshes(n, dup, extra):
         first, second = (j, k) if rank[j] < rank[k] else (k, j)
         both = (per[j][0] >= 0) & (per[k][0] >= 0) & (per[j][1] == per[k][1])
         assert both.sum() > 0 and not np.any(mesh[both] == second)
+
+
+def test_volpath_mesh_strided_equals_full_on_its_pixels():
+    """The strided oracle render (the combined scene's CPU baseline sample)
+    computes exactly the full render's pixels 0, s, 2s, ... and nothing else."""
+    from fledge_amd import api
+    v, ind = api.gen_octasphere(3, 0.05, seed=7, base_freq=2.0)
+    v = (v * 0.5 + 0.5).astype(np.float32)
+    grid = api.gen_fbm_grid(16, seed=5)
+    m = orc.Lbvh(v, ind)
+    args = (grid, [6.4, 1.6, 0.3], [1, 2, 2, -2, 50.0, 1, 0.5], [0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0],
+            24, 20, 2, 8)
+    full, _ = m.volpath_mesh(*args, nthreads=2)
+    part, _, pa = m.volpath_mesh_strided(*args, 7, nthreads=2)
+    f, p = full.reshape(-1, 3), part.reshape(-1, 3)
+    sel = np.arange(0, 24 * 20, 7)
+    assert pa == sel.size * 2
+    assert np.array_equal(f[sel].view(np.uint32), p[sel].view(np.uint32))
+    rest = np.setdiff1d(np.arange(24 * 20), sel)
+    assert not p[rest].any() and f[sel].any()
