@@ -86,7 +86,7 @@ class VolScene(ctypes.Structure):
 
 class RenderCfg(ctypes.Structure):
     _fields_ = [("width", c_i32), ("height", c_i32), ("spp", c_i32), ("max_depth", c_i32),
-                ("schedule", c_i32), ("film", c_i32), ("reserved", c_i32 * 6)]
+                ("schedule", c_i32), ("film", c_i32), ("use_graph", c_i32), ("reserved", c_i32 * 5)]
 
 
 class DiffParams(ctypes.Structure):

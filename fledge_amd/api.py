@@ -411,13 +411,17 @@ class Volume:
         self.ctx.check(self.ctx.lib.flg_volume_update_sparse(self.ctx.h, self.h,
                                                              _ptr(values) if self.n_leaves else None))
 
-    def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0):
+    def render(self, scene, width, height, spp, max_depth, out=None, stats=False, schedule=0, film=0,
+               use_graph=False):
         """RGB image (height, width, 3).  film=1: the reference's render loop
-        and Mitchell-filtered film instead of the per-pixel sample mean."""
+        and Mitchell-filtered film instead of the per-pixel sample mean.
+        schedule 3 (the queue-based wavefront) with use_graph replays its
+        stage launches from a cached hipGraph."""
         cfg = N.RenderCfg()
         cfg.width, cfg.height, cfg.spp, cfg.max_depth = width, height, spp, max_depth
         cfg.schedule = schedule
         cfg.film = film
+        cfg.use_graph = 1 if use_graph else 0
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
         check_buffer(out, "out", torch.float32, self.ctx.device, min_numel=height * width * 3)

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <new>
 #include <string>
@@ -219,6 +220,12 @@ int host_guard(flg_ctx* ctx, const char* what, F&& body) noexcept {
 }
 void resolve_timers(flg_ctx* ctx);
 int check_hip(flg_ctx* ctx, hipError_t e, const char* what);
+// Runs enqueue's launches on the context's stream through a hipGraph cached
+// in the context under `key` (every kernel argument of the sequence belongs
+// in it): captured once on a private stream (the default stream cannot be
+// captured), replayed afterwards.  At most 8 graphs are kept.
+int graph_launch(flg_ctx* ctx, const std::string& key, const std::function<void(hipStream_t)>& enqueue,
+                 const char* what);
 void* scratch(flg_ctx* ctx, const char* key, size_t bytes, int* status);
 
 // The fixed per-GPU scratch of flg_volpath_render_noalloc: 64 MiB of path

@@ -52,6 +52,33 @@ void* scratch(flg_ctx* ctx, const char* key, size_t bytes, int* status) {
   return p;
 }
 
+int graph_launch(flg_ctx* ctx, const std::string& key, const std::function<void(hipStream_t)>& enqueue,
+                 const char* what) {
+  hipGraphExec_t exec = nullptr;
+  for (auto& kv : ctx->graphs)
+    if (kv.first == key) exec = kv.second;
+  if (!exec) {
+    if (!ctx->capture_stream)
+      FLG_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    FLG_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+    enqueue(ctx->capture_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->capture_stream, &graph);
+    if (e != hipSuccess) return check_hip(ctx, e, (std::string("hipStreamEndCapture (") + what + ")").c_str());
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return check_hip(ctx, e, (std::string("hipGraphInstantiate (") + what + ")").c_str());
+    if (ctx->graphs.size() >= 8) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipGraphExecDestroy(ctx->graphs.front().second);
+      ctx->graphs.erase(ctx->graphs.begin());
+    }
+    ctx->graphs.emplace_back(key, exec);
+  }
+  FLG_CHECK_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  return FLG_OK;
+}
+
 // Events are recorded around the launch without waiting; they are resolved
 // (synchronised and accumulated) when the timers are read, or when too many
 // are pending, so timing does not serialise the stream.

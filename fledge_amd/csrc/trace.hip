@@ -800,7 +800,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     if (cfg->use_graph) {
       // every kernel argument of the frame is in the key: a cached graph is
       // replayed only for an identical launch sequence
-      std::string key;
+      std::string key = "pathtrace:";
       auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
       put(&g, sizeof g);
       put(&s, sizeof s);
@@ -810,31 +810,9 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
       const int32_t misc[9] = {cfg->spp, D, pgrid, tb, smode, fmode, compat ? 1 : 0, count_visits ? 1 : 0, ring};
       put(misc, sizeof misc);
       put(&visits, sizeof visits);
-      hipGraphExec_t exec = nullptr;
-      for (auto& kv : ctx->graphs)
-        if (kv.first == key) exec = kv.second;
-      if (!exec) {
-        // captured on a private stream (the default stream cannot be captured)
-        // and replayed on the context's stream
-        if (!ctx->capture_stream)
-          FLG_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-        FLG_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-        enqueue(ctx->capture_stream, false);
-        hipGraph_t graph = nullptr;
-        hipError_t e = hipStreamEndCapture(ctx->capture_stream, &graph);
-        if (e != hipSuccess) return check_hip(ctx, e, "hipStreamEndCapture (path tracer)");
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return check_hip(ctx, e, "hipGraphInstantiate (path tracer)");
-        if (ctx->graphs.size() >= 8) {
-          (void)hipStreamSynchronize(ctx->stream);
-          (void)hipGraphExecDestroy(ctx->graphs.front().second);
-          ctx->graphs.erase(ctx->graphs.begin());
-        }
-        ctx->graphs.emplace_back(std::move(key), exec);
-      }
       ScopedTimer tm(ctx, "pathtrace_total");
-      FLG_CHECK_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+      int st = graph_launch(ctx, key, [&](hipStream_t cs) { enqueue(cs, false); }, "path tracer");
+      if (st) return st;
     } else {
       ScopedTimer tm(ctx, "pathtrace_total");
       enqueue(ctx->stream, true);

@@ -124,13 +124,12 @@ struct SvolPath {
   bool centre;
 };
 
-// Starts a path at film position (fx, fy) with the sampler state in s.ps:
-// camera ray, entry into the box, primary transmittance.  Returns true when
-// the path is already complete (its camera ray misses the volume).
-template <bool COUNT>
-FLG_D bool svol_start(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                      const HaltonParams& hp, const CamDevV& cam, float fx, float fy, SvolPath& s,
-                      uint32_t& nlook) {
+// The camera ray at film position (fx, fy) and its entry into the box.
+// Returns true when the path is already complete (the ray misses the
+// volume: the environment is its radiance); otherwise s.o is the entry point,
+// s.d the direction, and the primary transmittance walk (from the camera,
+// draws from s.ps) is next.
+FLG_D bool svol_enter(const VolDev& v, const LightsDev& L, const CamDevV& cam, float fx, float fy, SvolPath& s) {
   V3 o, d;
   gen_ray_v(cam, fx, fy, &o, &d);
   s.Lr = 0.0f;
@@ -140,12 +139,27 @@ FLG_D bool svol_start(const VolDev& v, const LightsDev& L, const HaltonTables& t
     if (L.env.present) s.Lr += s.beta * L.env.radiance;
     return true;
   }
-  HaltonDraw rng{s.ps, tab, hp};
-  const float tr = vol_tr<COUNT>(v, rng, o, d, kInf, nlook);
-  if (L.env.present) s.Lr += tr * L.env.radiance;
   s.o = o + d * t_min;
   s.d = d;
   s.bounces = 1;
+  return false;
+}
+
+// The primary transmittance tr's contribution.
+FLG_D void svol_primary(const LightsDev& L, float tr, SvolPath& s) {
+  if (L.env.present) s.Lr += tr * L.env.radiance;
+}
+
+// Starts a path at film position (fx, fy) with the sampler state in s.ps:
+// camera ray, entry into the box, primary transmittance.  Returns true when
+// the path is already complete (its camera ray misses the volume).
+template <bool COUNT>
+FLG_D bool svol_start(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                      const HaltonParams& hp, const CamDevV& cam, float fx, float fy, SvolPath& s,
+                      uint32_t& nlook) {
+  if (svol_enter(v, L, cam, fx, fy, s)) return true;
+  HaltonDraw rng{s.ps, tab, hp};
+  svol_primary(L, vol_tr<COUNT>(v, rng, cam.pos, s.d, kInf, nlook), s);
   return false;
 }
 
@@ -204,29 +218,36 @@ FLG_D bool film_begin(const VolDev& v, const LightsDev& L, const HaltonTables& t
   return svol_start<COUNT>(v, L, tab, hp, cam, s.fx, s.fy, s, nlook);
 }
 
-// One scattering event of the path: delta tracking to the next collision, HG
-// sampling, next-event estimation, termination tests.  Returns true when the
-// path is complete.
+// The first half of a scattering event: delta tracking to the next
+// collision, then at the collision the HG draws and the light sample up to
+// its shadow ray.  Returns false when the path is complete (no collision).
+// s.o, s.d become the collision point and the scattered direction (the
+// termination test that decides whether they are used comes after the light).
 template <bool COUNT>
-FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
-                     const HaltonParams& hp, int max_depth, SvolPath& s, uint32_t& nlook) {
+FLG_D bool svol_collide(const VolDev& v, const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp,
+                        SvolPath& s, LightSample& ls, uint32_t& nlook) {
   HaltonDraw rng{s.ps, tab, hp};
   bool success = false;
   V3 pos;
   const float f = vol_sample<COUNT>(v, rng, s.o, s.d, kInf, &success, &pos, nlook);
-  if (!success) return true;
+  if (!success) return false;
   s.beta = s.beta * f;
   const V3 wo = -s.d;
   const float u = s.ps.next(tab, hp);
   const float vv = s.ps.next(tab, hp);
   const V3 wi = hg_sample(wo, u, vv, v.g);
-  s.Lr += s.beta * sample_one_light(L, v.g, tab, hp, s.ps, pos, wo, [&](V3 p, V3 d, float tmax) {
-            HaltonDraw rng{s.ps, tab, hp};
-            return vol_tr<COUNT>(v, rng, p, d, tmax, nlook);
-          });
-  if (s.bounces >= max_depth || s.beta * s.beta + s.beta * s.beta + s.beta * s.beta == 0.0f) return true;
+  ls = light_pre(L, v.g, tab, hp, s.ps, pos, wo);
   s.o = pos;
   s.d = normalize(wi);
+  return true;
+}
+
+// The second half, given the light estimate ld: radiance, the depth and
+// throughput tests, Russian roulette after bounce 10.  Returns true when the
+// path is complete.
+FLG_D bool svol_continue(const HaltonTables& tab, const HaltonParams& hp, int max_depth, float ld, SvolPath& s) {
+  s.Lr += s.beta * ld;
+  if (s.bounces >= max_depth || s.beta * s.beta + s.beta * s.beta + s.beta * s.beta == 0.0f) return true;
   if (s.bounces > 10) {
     const float rr = 0.01f;
     if (s.ps.next(tab, hp) < rr) return true;
@@ -234,6 +255,27 @@ FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& ta
   }
   ++s.bounces;
   return false;
+}
+
+// One scattering event of the path: delta tracking to the next collision, HG
+// sampling, next-event estimation, termination tests.  Returns true when the
+// path is complete.
+template <bool COUNT>
+FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& tab,
+                     const HaltonParams& hp, int max_depth, SvolPath& s, uint32_t& nlook) {
+  LightSample ls;
+  if (!svol_collide<COUNT>(v, L, tab, hp, s, ls, nlook)) return true;
+  float ld;
+  if (ls.kind == 2) {
+    V3 d;
+    float tmax;
+    light_ray(ls, s.o, &d, &tmax);
+    HaltonDraw rng{s.ps, tab, hp};
+    ld = light_post(ls, vol_tr<COUNT>(v, rng, s.o, d, tmax, nlook));
+  } else {
+    ld = light_unshadowed(ls);
+  }
+  return svol_continue(tab, hp, max_depth, ld, s);
 }
 
 // svol_step for film tasks: the end of the centre path starts sample 1.
@@ -313,6 +355,240 @@ __global__ __launch_bounds__(256, 5) void k_svol_regen(VolDev v, LightsDev L, Ha
       },
       [&](uint32_t p, const SvolPath& s) { svol_finish<FILM>(p, s, lpath, fpos); });
   count_lookups<COUNT>(nlook, nlookups);
+}
+
+// ------------------------------------------------ queue-based wavefront ---
+// Schedule 3: SVolIntegrator::Li as stages over queues of path states (film
+// = 1: the centre path and sample 1 of a task are one entry after the
+// other).  A path's state lives in HBM between stages, in the queue that
+// holds it.  Every stage is one persistent launch over its input queue and
+// appends to the other queue:
+//   begin   every task: Halton index, film position, camera ray, box entry;
+//           a ray that misses completes its path; the others queue for the
+//           primary transmittance walk;
+//   walk    ratio tracking of each queued slot: the primary transmittance
+//           (from the camera) or the shadow ray of its last collision, then
+//           the rest of that step (radiance, depth and throughput tests,
+//           roulette); live paths queue for a collision;
+//   collide delta tracking to the next collision, the HG draws and the light
+//           sample up to its shadow ray; paths that escape complete.
+// A film task whose centre path completes starts sample 1 in the same slot
+// (its primary walk runs at once when the walk stage completes the centre
+// path, as that stage's output is the collision queue).  Each path runs the
+// draws and arithmetic of svol_begin/svol_step, so the bits equal the other
+// schedules'.  The launch sequence is fixed (begin, then walk and collide
+// pairs enough for the deepest path, then a last walk); a stage reads its
+// queue's length from device memory, so the host never waits between stages
+// and the whole sequence can be captured once as a hipGraph.
+enum : uint32_t { kWfCamera = 0, kWfShadow = 1, kWfUnshadowed = 2 };
+
+// A queue of path states, one 80-byte record per entry in five arrays, in
+// queue order: a stage reads its input queue's records at consecutive
+// positions and writes its output records at the positions it claims, so
+// both are coalesced.
+struct WfQueue {
+  float4* od;   // o.xyz, Lr
+  float4* dir;  // d.xyz, beta
+  float4* lw;   // pending light: f, Li, pdf, light_pdf (kWfShadow); estimate in .x (kWfUnshadowed)
+  float4* sh;   // shadow ray: direction, tmax (kWfShadow)
+  uint4* meta;  // Halton index, next dimension, bounces | kind << 16 | centre << 24, task
+};
+
+FLG_D void wf_load(const WfQueue& Q, uint32_t i, SvolPath& s, uint32_t* kind, uint32_t* task) {
+  const float4 a = Q.od[i], b = Q.dir[i];
+  const uint4 m = Q.meta[i];
+  s.o = v3(a.x, a.y, a.z);
+  s.Lr = a.w;
+  s.d = v3(b.x, b.y, b.z);
+  s.beta = b.w;
+  s.ps.index = m.x;
+  s.ps.dim = m.y;
+  s.bounces = (int)(m.z & 0xFFFFu);
+  *kind = (m.z >> 16) & 0xFFu;
+  s.centre = (m.z >> 24) != 0;
+  *task = m.w;
+}
+
+FLG_D void wf_store(const WfQueue& Q, uint32_t i, const SvolPath& s, uint32_t kind, uint32_t task) {
+  Q.od[i] = make_float4(s.o.x, s.o.y, s.o.z, s.Lr);
+  Q.dir[i] = make_float4(s.d.x, s.d.y, s.d.z, s.beta);
+  Q.meta[i] = make_uint4(s.ps.index, s.ps.dim, (uint32_t)s.bounces | (kind << 16) | ((s.centre ? 1u : 0u) << 24),
+                         task);
+}
+
+// The queue position of each lane that pushes (wave-aggregated: one atomic
+// per wave on the queue's length).
+FLG_D uint32_t wf_claim(bool push, uint32_t* n) {
+  const uint64_t m = __ballot(push);
+  if (!m) return 0;
+  const int lane = threadIdx.x & 63;
+  const int leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Task p's path (film: its centre path, or sample 1 after it) is complete:
+// a film task goes on to sample 1 (returns true: its primary walk is next);
+// otherwise the task's outputs are written (false).
+template <bool FILM>
+FLG_D bool wf_complete(const VolDev& v, const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp,
+                       const CamDevV& cam, int32_t tasks, uint32_t p, SvolPath& s, float* lpath, float2* fpos) {
+  while (FILM && s.centre) {
+    const uint32_t pix = p / (uint32_t)tasks;
+    const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+    s.centre = false;
+    s.fx = (float)x + s.ps.next(tab, hp);
+    s.fy = (float)y + s.ps.next(tab, hp);
+    fpos[p] = make_float2(s.fx, s.fy);
+    if (!svol_enter(v, L, cam, s.fx, s.fy, s)) return true;
+  }
+  lpath[p] = s.Lr;
+  return false;
+}
+
+// begin: task p's first path up to its primary walk (svol_begin,
+// film_begin); queued unless it completes at once.
+template <bool FILM>
+__global__ __launch_bounds__(256) void k_wf_begin(VolDev v, LightsDev L, HaltonTables tab_g, HaltonParams hp,
+                                                  CamDevV cam, int32_t tasks, float* lpath, float2* fpos,
+                                                  WfQueue out, uint32_t* nout) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
+  const uint32_t ntasks = (uint32_t)((int64_t)cam.w * cam.h * tasks);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < ntasks; base += gridDim.x * blockDim.x) {
+    const uint32_t p = base + threadIdx.x;
+    bool push = false;
+    SvolPath s;
+    if (p < ntasks) {
+      const uint32_t pix = p / (uint32_t)tasks, t = p - pix * (uint32_t)tasks;
+      const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+      s.ps.index = halton_pixel_offset(hp, x, y) + t * hp.stride;
+      s.ps.dim = 0;
+      s.centre = FILM && t == 0;
+      float fx, fy;
+      if (s.centre) {
+        fx = (float)x + 0.5f;
+        fy = (float)y + 0.5f;
+      } else {
+        fx = (float)x + s.ps.next(tab, hp);
+        fy = (float)y + s.ps.next(tab, hp);
+        if (FILM) fpos[p] = make_float2(fx, fy);
+      }
+      push = !svol_enter(v, L, cam, fx, fy, s) || wf_complete<FILM>(v, L, tab, hp, cam, tasks, p, s, lpath, fpos);
+    }
+    const uint32_t at = wf_claim(push, nout);
+    if (push) wf_store(out, at, s, kWfCamera, p);
+  }
+}
+
+// walk: the pending ratio-tracking walk of each queued path and the rest of
+// its step; paths that go on queue for a collision.
+template <bool COUNT, bool FILM>
+__global__ __launch_bounds__(256) void k_wf_walk(VolDev v, LightsDev L, HaltonTables tab_g, HaltonParams hp,
+                                                 CamDevV cam, int32_t tasks, int32_t max_depth, float* lpath,
+                                                 float2* fpos, WfQueue in, const uint32_t* nin, WfQueue out,
+                                                 uint32_t* nout, unsigned long long* nlookups) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
+  const uint32_t n = *nin;
+  uint32_t nlook = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    bool push = false;
+    uint32_t p = 0;
+    SvolPath s;
+    if (i < n) {
+      uint32_t kind;
+      wf_load(in, i, s, &kind, &p);
+      bool walking = true;
+      while (walking) {
+        walking = false;
+        if (kind == kWfCamera) {
+          HaltonDraw rng{s.ps, tab, hp};
+          svol_primary(L, vol_tr<COUNT>(v, rng, cam.pos, s.d, kInf, nlook), s);
+          push = true;
+        } else {
+          float ld;
+          if (kind == kWfShadow) {
+            const float4 a = in.lw[i], b = in.sh[i];
+            LightSample ls;
+            ls.f = a.x;
+            ls.Li = a.y;
+            ls.pdf = a.z;
+            ls.light_pdf = a.w;
+            HaltonDraw rng{s.ps, tab, hp};
+            ld = light_post(ls, vol_tr<COUNT>(v, rng, s.o, v3(b.x, b.y, b.z), b.w, nlook));
+          } else {
+            ld = in.lw[i].x;
+          }
+          if (!svol_continue(tab, hp, max_depth, ld, s)) {
+            push = true;
+          } else if (wf_complete<FILM>(v, L, tab, hp, cam, tasks, p, s, lpath, fpos)) {
+            kind = kWfCamera;  // sample 1's primary walk, here: this stage feeds the collision queue
+            walking = true;
+          }
+        }
+      }
+    }
+    const uint32_t at = wf_claim(push, nout);
+    if (push) wf_store(out, at, s, kWfCamera, p);
+  }
+  count_lookups<COUNT>(nlook, nlookups);
+}
+
+// collide: delta tracking of each queued path to its next collision and the
+// light sample there; live paths queue for their shadow walk.
+template <bool COUNT, bool FILM>
+__global__ __launch_bounds__(256) void k_wf_collide(VolDev v, LightsDev L, HaltonTables tab_g, HaltonParams hp,
+                                                    CamDevV cam, int32_t tasks, float* lpath, float2* fpos,
+                                                    WfQueue in, const uint32_t* nin, WfQueue out, uint32_t* nout,
+                                                    unsigned long long* nlookups) {
+  __shared__ int2 lds_primes[kLocalPrimes];
+  const HaltonTables tab = stage_halton(tab_g, lds_primes);
+  const uint32_t n = *nin;
+  uint32_t nlook = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    bool push = false;
+    uint32_t p = 0, kind = kWfCamera;
+    SvolPath s;
+    LightSample ls;
+    V3 shd = v3(0.0f, 0.0f, 0.0f);
+    float shmax = 0.0f;
+    if (i < n) {
+      wf_load(in, i, s, &kind, &p);
+      if (svol_collide<COUNT>(v, L, tab, hp, s, ls, nlook)) {
+        push = true;
+        if (ls.kind == 2) {
+          light_ray(ls, s.o, &shd, &shmax);
+          kind = kWfShadow;
+        } else {
+          ls.f = light_unshadowed(ls);
+          kind = kWfUnshadowed;
+        }
+      } else if (wf_complete<FILM>(v, L, tab, hp, cam, tasks, p, s, lpath, fpos)) {
+        push = true;
+        kind = kWfCamera;
+      }
+    }
+    const uint32_t at = wf_claim(push, nout);
+    if (push) {
+      wf_store(out, at, s, kind, p);
+      if (kind == kWfShadow) {
+        out.lw[at] = make_float4(ls.f, ls.Li, ls.pdf, ls.light_pdf);
+        out.sh[at] = make_float4(shd.x, shd.y, shd.z, shmax);
+      } else if (kind == kWfUnshadowed) {
+        out.lw[at].x = ls.f;
+      }
+    }
+  }
+  count_lookups<COUNT>(nlook, nlookups);
+}
+
+__global__ void k_wf_zero(uint32_t* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_pixel_mean(const float* lpath, int64_t npix, int32_t spp,
@@ -591,8 +867,11 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
     return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": too many paths");
   if (cfg->film < 0 || cfg->film > 1) return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": film must be 0 or 1");
   const int sched = cfg->schedule ? cfg->schedule : 2;
-  if (sched < 1 || sched > 2)
-    return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": schedule must be 0, 1 or 2");
+  if (sched < 1 || sched > 3)
+    return set_error(ctx, FLG_ERR_INVALID, std::string(what) + ": schedule must be 0, 1, 2 or 3");
+  if (sched == 3 && noalloc)
+    return set_error(ctx, FLG_ERR_INVALID,
+                     std::string(what) + ": schedule 3 keeps the path state in allocated scratch; use 1 or 2");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
     return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": spp * Halton stride must stay below 2^24");
@@ -635,7 +914,70 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
     if ((st = make_lights(ctx, vol, scene, &L, nullptr, !noalloc))) return st;
   }
   k_zero_u64<<<1, 64, 0, ctx->stream>>>(nl, 2);  // lookup count and path counter
-  if (npaths > 0) {
+  uint32_t* wf_counts = nullptr;
+  int wf_ncounts = 0;
+  if (npaths > 0 && sched == 3) {
+    // [two queues of npaths 80-byte records, five arrays each][stage counts]
+    const int pairs = film ? 2 * cfg->max_depth + 1 : cfg->max_depth + 1;
+    wf_ncounts = 2 * pairs + 2;
+    const size_t rec = (size_t)npaths * 16;
+    char* w = (char*)scratch(ctx, "vol_wavefront", 10 * rec + (size_t)wf_ncounts * 4, &st);
+    if (!w) return st;
+    auto queue = [&](int k) {
+      char* b = w + (size_t)k * 5 * rec;
+      return WfQueue{(float4*)b, (float4*)(b + rec), (float4*)(b + 2 * rec), (float4*)(b + 3 * rec),
+                     (uint4*)(b + 4 * rec)};
+    };
+    const WfQueue q0 = queue(0), q1 = queue(1);
+    wf_counts = (uint32_t*)(w + 10 * rec);
+    const VolDev v = make_voldev(vol);
+    const CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
+    const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
+    const int32_t D = cfg->max_depth;
+    uint32_t* counts = wf_counts;
+    auto enqueue = [&](hipStream_t s) {
+      k_wf_zero<<<1, 256, 0, s>>>(counts, wf_ncounts);
+#define FLG_WF_STAGES(C, F)                                                                                      \
+  k_wf_begin<F><<<grid, 256, 0, s>>>(v, L, tab, hp, cam, tasks, lpath, fpos, q0, counts);                        \
+  for (int k = 0; k <= pairs; ++k) {                                                                             \
+    k_wf_walk<C, F><<<grid, 256, 0, s>>>(v, L, tab, hp, cam, tasks, D, lpath, fpos, q0, counts + 2 * k, q1,      \
+                                         counts + 2 * k + 1, nl);                                                \
+    if (k < pairs)                                                                                               \
+      k_wf_collide<C, F><<<grid, 256, 0, s>>>(v, L, tab, hp, cam, tasks, lpath, fpos, q1, counts + 2 * k + 1,    \
+                                              q0, counts + 2 * k + 2, nl);                                       \
+  }
+      if (count && film) {
+        FLG_WF_STAGES(true, true)
+      } else if (count) {
+        FLG_WF_STAGES(true, false)
+      } else if (film) {
+        FLG_WF_STAGES(false, true)
+      } else {
+        FLG_WF_STAGES(false, false)
+      }
+#undef FLG_WF_STAGES
+    };
+    ScopedTimer tm(ctx, "volpath_paths");
+    if (cfg->use_graph) {
+      std::string key = "volume wavefront:";
+      auto put = [&key](const void* p, size_t n) { key.append(static_cast<const char*>(p), n); };
+      put(&v, sizeof v);
+      put(&L, sizeof L);
+      put(&tab, sizeof tab);
+      put(&hp, sizeof hp);
+      put(&cam, sizeof cam);
+      put(&q0, sizeof q0);
+      put(&q1, sizeof q1);
+      const void* ptrs[4] = {lpath, fpos, counts, nl};
+      put(ptrs, sizeof ptrs);
+      const int32_t misc[6] = {tasks, D, grid, pairs, count ? 1 : 0, film ? 1 : 0};
+      put(misc, sizeof misc);
+      if ((st = graph_launch(ctx, key, enqueue, "volume wavefront"))) return st;
+    } else {
+      enqueue(ctx->stream);
+    }
+  }
+  if (npaths > 0 && sched != 3) {
     VolDev v = make_voldev(vol);
     CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
     ScopedTimer tm(ctx, "volpath_paths");
@@ -669,8 +1011,14 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
   FLG_CHECK_LAUNCH(ctx);
   if (h_stats) {
     unsigned long long c = 0;
+    uint32_t left = 0;  // schedule 3: slots the last walk stage queued (none may be)
     FLG_CHECK_HIP(ctx, hipMemcpyAsync(&c, nl, 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (wf_counts)
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(&left, wf_counts + wf_ncounts - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
     FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (left)
+      return set_error(ctx, FLG_ERR_HIP, std::string(what) + ": " + std::to_string(left) +
+                                             " paths outlived the wavefront's stage sequence");
     h_stats[0] = (uint64_t)(npaths > 0 ? npix * cfg->spp : 0);
     h_stats[1] = c;
   }

@@ -462,44 +462,80 @@ struct LightsDev {
 
 // UniformSampleOneLight + EstimateDirect for a volume vertex
 // (integrator.cpp:69-120): lights in the order [point, environment], one
-// picked with p = 1/n, u_light drawn before sampleLi, the HG phase as f.  The
-// shadow transmittance is shadow(p, d, tmax): the medium's ratio tracking
-// (SVolIntegrator) or VolEstimateTr through interfaces (VolPathIntegrator).
-template <class Shadow>
-FLG_D float sample_one_light(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
-                             PathSampler& ps, V3 p, V3 wo, Shadow shadow) {
+// picked with p = 1/n, u_light drawn before sampleLi, the HG phase as f.
+// light_pre runs it up to the shadow ray: the light
+// choice and its sample.  kind 0: no light (the estimate is 0); 1: no shadow
+// ray (pdf, Li or the phase function is 0; the estimate is 0 / light_pdf);
+// 2: the estimate needs the transmittance along the shadow ray from p
+// towards p_light (light_post).
+struct LightSample {
+  V3 p_light;
+  float f, Li, pdf, light_pdf;
+  int kind;
+};
+
+FLG_D LightSample light_pre(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
+                            PathSampler& ps, V3 p, V3 wo) {
+  LightSample ls;
+  ls.kind = 0;
   const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
+  if (n_lights == 0) return ls;
   int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
   if (light_num > n_lights - 1) light_num = n_lights - 1;
-  const float light_pdf = (float)(1.0 / (double)n_lights);
+  ls.light_pdf = (float)(1.0 / (double)n_lights);
   const bool use_point = L.has_point && light_num == 0;
   float u0 = ps.next(tab, hp);
   float u1 = ps.next(tab, hp);
-  V3 wi, p_light;
-  float pdf, Li;
+  V3 wi;
   if (use_point) {
     V3 diff = L.point_pos - p;
     float d2 = sqnorm(diff);
     wi = normalize(diff);
-    Li = L.point_intensity / d2;
-    pdf = 1.0f;
-    p_light = L.point_pos;
+    ls.Li = L.point_intensity / d2;
+    ls.pdf = 1.0f;
+    ls.p_light = L.point_pos;
   } else {
-    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, &p_light);
+    ls.Li = env_sample_li(L.env, p, u0, u1, &wi, &ls.pdf, &ls.p_light);
   }
+  ls.kind = 1;
+  if (ls.pdf == 0.0f || ls.Li == 0.0f) return ls;
+  ls.f = hg_p(wi, wo, g);
+  if (ls.f != 0.0f) ls.kind = 2;
+  return ls;
+}
+
+// The estimate without a shadow ray (kinds 0 and 1).
+FLG_D float light_unshadowed(const LightSample& ls) { return ls.kind == 0 ? 0.0f : 0.0f / ls.light_pdf; }
+
+// The shadow ray of a kind-2 sample from p: direction and length.
+FLG_D void light_ray(const LightSample& ls, V3 p, V3* d, float* tmax) {
+  V3 diff = ls.p_light - p;
+  float nrm = sqrtf(sqnorm(diff));
+  *d = diff / nrm;
+  *tmax = nrm - SHADOW_EPS;
+}
+
+// The estimate of a kind-2 sample given the shadow ray's transmittance.
+FLG_D float light_post(const LightSample& ls, float tr) {
+  float Li = ls.Li;
+  Li *= tr;
   float out = 0.0f;
-  if (pdf == 0.0f || Li == 0.0f) return out / light_pdf;
-  float f = hg_p(wi, wo, g);
-  if (f != 0.0f) {
-    V3 diff = p_light - p;
-    float nrm = sqrtf(sqnorm(diff));
-    V3 d = diff / nrm;
-    float tr = shadow(p, d, nrm - SHADOW_EPS);
-    Li *= tr;
-    out += f * Li / pdf;
-  }
-  return out / light_pdf;
+  out += ls.f * Li / ls.pdf;
+  return out / ls.light_pdf;
+}
+
+// The whole estimate; `shadow(p, d, tmax)` is the transmittance along the
+// shadow ray: ratio tracking in the medium (SVolIntegrator) or VolEstimateTr
+// through interfaces (VolPathIntegrator).
+template <class Shadow>
+FLG_D float sample_one_light(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
+                             PathSampler& ps, V3 p, V3 wo, Shadow shadow) {
+  const LightSample ls = light_pre(L, g, tab, hp, ps, p, wo);
+  if (ls.kind != 2) return light_unshadowed(ls);
+  V3 d;
+  float tmax;
+  light_ray(ls, p, &d, &tmax);
+  return light_post(ls, shadow(p, d, tmax));
 }
 
 // Camera (src/common/camera.h:11-31): ctor on the host in float, per-ray

@@ -362,7 +362,11 @@ typedef struct flg_vol_scene {
 typedef struct flg_render_cfg {
   int32_t width, height, spp, max_depth;
   int32_t schedule; /* 1: one lane per path sample; 2: persistent lanes that
-                       take a new path when theirs ends; 0: 2.  Same results. */
+                       take a new path when theirs ends; 3 (flg_volpath_render
+                       only): queue-based wavefront, the path state in device
+                       memory between stages (primary walk, then collision
+                       and shadow walk per bounce), about 80 B per path
+                       sample of scratch; 0: 2.  Same results. */
   int32_t film;     /* flg_volpath_render only.  0: pixel = mean of spp path
                        samples, sample s on Halton index offset + s*stride
                        (eval_pixel's per-sample value).  1: the reference's
@@ -376,7 +380,10 @@ typedef struct flg_render_cfg {
                        resolved as sum(w L) / sum(w); race-free and
                        deterministic.  SVolIntegrator's albedo and normal
                        buffers are zero, so none is produced. */
-  int32_t reserved[6];
+  int32_t use_graph; /* schedule 3: the stage launches are replayed from a
+                        hipGraph cached in the context (captured on first
+                        use for the same arguments); ignored otherwise */
+  int32_t reserved[5];
 } flg_render_cfg;
 
 /* Renders width*height RGB float pixels (d_rgb, 3 floats per pixel) with the

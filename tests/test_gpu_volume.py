@@ -116,14 +116,15 @@ def test_volume_white_furnace(ctx):
     assert abs(out.mean() - 0.5) < 0.02
 
 
-@pytest.mark.parametrize("schedule", [1, 2])
+@pytest.mark.parametrize("schedule", [1, 2, 3])
 @pytest.mark.parametrize("W,H,spp", [(96, 80, 8), (37, 23, 3), (211, 197, 5)])
 def test_volpath_schedules_identical(ctx, schedule, W, H, spp):
-    """One lane per path and persistent lanes with path regeneration give the
-    same image and lookup count, bit for bit, and both match the oracle.  The
-    regenerating lanes claim paths 128 at a time per wave: 61,440 paths fill
-    whole chunks, 2,553 leave most waves without one, and 207,835 send waves
-    back for more with a partial last chunk."""
+    """One lane per path, persistent lanes with path regeneration and the
+    queue-based wavefront give the same image and lookup count, bit for bit,
+    and all match the oracle.  The regenerating lanes claim paths 128 at a
+    time per wave: 61,440 paths fill whole chunks, 2,553 leave most waves
+    without one, and 207,835 send waves back for more with a partial last
+    chunk."""
     grid = api.gen_fbm_grid(32, seed=8)
     vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
@@ -135,7 +136,7 @@ def test_volpath_schedules_identical(ctx, schedule, W, H, spp):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("schedule", [1, 2])
+@pytest.mark.parametrize("schedule", [1, 2, 3])
 def test_volpath_film_matches_oracle(ctx, schedule):
     """film = 1: the reference's render loop (pixel-centre path, samples
     1..spp-1 with the render loop's Halton schedule) resolved through the
@@ -155,8 +156,33 @@ def test_volpath_film_matches_oracle(ctx, schedule):
     rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
     assert rel.max() <= 1e-5, rel.max()
     assert st["lookups"] == lk and st["paths"] == W * H * spp
-    other, _ = vol.render(sc, W, H, spp, 32, stats=True, schedule=3 - schedule, film=1)
+    other, _ = vol.render(sc, W, H, spp, 32, stats=True, schedule=1 if schedule != 1 else 2, film=1)
     assert torch.equal(out.view(torch.int32), other.view(torch.int32))
+
+
+@pytest.mark.parametrize("film,depth", [(0, 32), (1, 8), (0, 1)])
+def test_volpath_wavefront_graph_replays(ctx, film, depth):
+    """The wavefront's stage sequence replayed from its hipGraph: every replay,
+    with eager work and synchronisations between them, equals the eager
+    render bit for bit; after the grid changed (a new majorant, so a new
+    graph under a new key) the replay equals the regenerating schedule."""
+    grid = api.gen_fbm_grid(32, seed=12)
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    sc = api.vol_scene(cam)
+    W, H, spp = 53, 41, 5
+    eager = vol.render(sc, W, H, spp, depth, schedule=3, film=film).clone()
+    for k in range(4):
+        g = vol.render(sc, W, H, spp, depth, schedule=3, film=film, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(g.view(torch.int32), eager.view(torch.int32)), k
+        vol.render(sc, 17, 13, 2, 4, schedule=2)  # eager work between the replays
+        torch.cuda.synchronize()
+    vol.update(torch.from_numpy(grid * np.float32(0.5)).cuda())
+    want = vol.render(sc, W, H, spp, depth, schedule=2, film=film)
+    got = vol.render(sc, W, H, spp, depth, schedule=3, film=film, use_graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(got.view(torch.int32), want.view(torch.int32))
 
 
 def test_volpath_film_constant_radiance_and_one_sample(ctx):
@@ -182,3 +208,9 @@ def test_volpath_full_benchmark_frame(ctx):
     out, ref, st, lk = _scene(ctx, grid, 64, 512, 512, 32)
     assert st["lookups"] == lk and st["paths"] == 512 * 512 * 64
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+    # the wavefront schedule at the same full size
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    wf, sw = vol.render(api.vol_scene(cam), 512, 512, 64, 32, stats=True, schedule=3)
+    assert sw["lookups"] == lk
+    assert np.array_equal(wf.cpu().numpy().view(np.uint32), ref.view(np.uint32))
