@@ -528,6 +528,7 @@ def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
     _, st1 = vol.render(sc1, Wv, Hv, S1, 32, out=out1, stats=True)
     t1 = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1), 5, 2, sync)
     t1b = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1, schedule=1), 3, 1, sync)
+    t1c = timed(lambda: vol.render(sc1, Wv, Hv, S1, 32, out=out1, schedule=3, use_graph=True), 3, 1, sync)
     ctx.profiling(True)
     ctx.reset_profiling()
     for _ in range(5):
@@ -541,6 +542,7 @@ def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
     out = {"metric": "vol path samples/s (256^3 fBm grid, 512x512, 64 spp, depth 32)",
            "value": round(P / t1, 1), "unit": "path samples/s", "higher_is_better": True,
            "ms_per_frame": round(t1 * 1e3, 3), "ms_per_frame_lane_per_path": round(t1b * 1e3, 3),
+           "ms_per_frame_wavefront_graph": round(t1c * 1e3, 3),
            "kernel_ms": round(kms, 4), "lookups_per_path": round(L, 4),
            "roofline": roofline(lb / (kms * 1e-3) / 1e9, "k_svol_regen (persistent lanes with path regeneration)",
                                 round(32 * L + 12 / S1, 3), "path"),
@@ -554,7 +556,7 @@ def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
         lk = np.zeros(1, np.uint64)
         pa = np.zeros(1, np.uint64)
         cam10 = np.array([0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0], np.float32)
-        stride = 3  # every third pixel, spread over rows and columns (87,382 of 262,144)
+        stride = 1  # the whole frame
         t0 = time.perf_counter()
         lib.orc_volpath_render_strided(
             orc._p(g1), 256, 256, 256, orc._p(np.array([0, 0, 0, 1, 1, 1], np.float32)),
@@ -563,8 +565,8 @@ def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
         dtc = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(float(pa[0]) / dtc, 1), "unit": "path samples/s", "cores": cores,
                                "kind": "port",
-                               "sample": "every %d-th pixel of the frame (%d paths, all 64 spp each), the oracle's "
-                                         "SVolIntegrator restatement, %.2f s" % (stride, int(pa[0]), dtc)}
+                               "sample": "the whole frame (%d paths: 512 x 512 pixels, 64 spp each), the oracle's "
+                                         "SVolIntegrator restatement, %.2f s" % (int(pa[0]), dtc)}
     cpu_jobs.append(cpu_job)
     vol.close()
     return out
@@ -670,9 +672,11 @@ def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, profile=False,
     P = W * H * SPPc
     rays = stc["rays"]
     look = stc["lookups"]
-    # density bytes: 32 B per trilinear lookup + 4 B slot-map entry, 12 B per
-    # pixel written (the mesh traversal's node and triangle reads are not in it)
-    lb = 36 * look + 12 * W * H
+    # algorithmic bytes: per trilinear lookup 32 B of voxels + the 4-byte
+    # slot-map entry; per ray traced on the mesh 32 B in + 8 B out, 64 B per
+    # node visit and 36 B per triangle test (counted by the same kernel's
+    # counting build); 12 B per pixel written
+    lb = 36 * look + 40 * rays + 64 * stc["node_visits"] + 36 * stc["tri_tests"] + 12 * W * H
     pool_bytes = origins.shape[0] * 4000 + 65 * 65 * 65 * 32
     out = {"metric": "combined scene: VolPathIntegrator path samples/s (8,388,608-triangle interface mesh around "
                      "a 512^3 sparse grid of 8^3 bricks, 15% active, in a brick pool; 1920x1080, 64 spp, depth 16)",
@@ -686,9 +690,10 @@ def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, profile=False,
            "diff_forward_ms": round(tf * 1e3, 2), "adjoint_ms": round(ta * 1e3, 2),
            "fwd_adjoint_over_forward": round((tf + ta) / tc, 3),
            "roofline": roofline(lb / (kms * 1e-3) / 1e9,
-                                "k_volpath_regen (persistent lanes over the interface mesh and the brick pool); "
-                                "density bytes only",
-                                round(36 * look / P + 12 / SPPc, 3), "path"),
+                                "k_volpath_regen (persistent lanes over the interface mesh and the brick pool)",
+                                round(lb / P, 3), "path",
+                                node_visits_per_ray=round(stc["node_visits"] / rays, 3),
+                                tri_tests_per_ray=round(stc["tri_tests"] / rays, 3)),
            "cpu_baseline": None}
     bc.close()
     if host_grid is not None:
@@ -696,7 +701,7 @@ def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, profile=False,
             import oracle_lib as orc
             vv, ii = host_mesh
             m = orc.Lbvh(vv, ii)
-            stride = 19  # 109,137 of 2,073,600 pixels, all 64 samples each
+            stride = 5  # 414,720 of 2,073,600 pixels, all 64 samples each
             cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
             t0 = time.perf_counter()
             _, _, pa = m.volpath_mesh_strided(host_grid, [6.4, 1.6, 0.3], [1, 2, 2, -2, 50.0, 1, 0.5], cam10, W, H,

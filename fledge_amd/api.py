@@ -443,11 +443,12 @@ class Volume:
         if out is None:
             out = torch.empty((height, width, 3), dtype=torch.float32, device=torch.device("cuda", self.ctx.device))
         check_buffer(out, "out", torch.float32, self.ctx.device, min_numel=height * width * 3)
-        st = (ctypes.c_uint64 * 3)()
+        st = (ctypes.c_uint64 * 5)()
         self.ctx.check(self.ctx.lib.flg_volpath_mesh_render(self.ctx.h, bvh.h, self.h, ctypes.byref(scene),
                                                             ctypes.byref(cfg), _ptr(out), st if stats else None))
         if stats:
-            return out, {"paths": int(st[0]), "lookups": int(st[1]), "rays": int(st[2])}
+            return out, {"paths": int(st[0]), "lookups": int(st[1]), "rays": int(st[2]), "node_visits": int(st[3]),
+                         "tri_tests": int(st[4])}
         return out
 
     def close(self):

@@ -1121,7 +1121,7 @@ bool HIPRender::render() {
   cfg.film = 1;  // the reference's render loop and film, as CPURender's integrators use
   const size_t n = size_t(sc.m_resX) * sc.m_resY * 3;
   gpu::DeviceBuffer d(ctx, n * sizeof(float));
-  uint64_t stats[3] = {0, 0, 0};
+  uint64_t stats[5] = {0, 0, 0, 0, 0};
   ctx.check(flg_volpath_mesh_render(ctx.get(), m_mesh->handle(), m_volume->handle(), &vs, &cfg,
                                     static_cast<float*>(d.data()), stats),
             "flg_volpath_mesh_render");

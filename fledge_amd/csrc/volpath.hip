@@ -46,19 +46,24 @@ struct MeshGeom {
   uint2* ring;  // the block's LDS ring, kMeshBlock * kMeshRing entries
 };
 
-// Counts of a measurement pass: density lookups and rays traced on the mesh.
+// Counts of a measurement pass: density lookups, rays traced on the mesh,
+// and their node visits and triangle tests.
 struct VpCount {
-  uint32_t look = 0, rays = 0;
+  uint32_t look = 0, rays = 0, nodes = 0, tris = 0;
 };
 
 // Closest hit on the mesh within tmax: sorted position (-1 on a miss), t, and
 // the geometric normal of the hit as the Pluecker test computes it.
 template <bool COMPAT, bool COUNT>
 FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng, VpCount& cnt) {
-  if (COUNT) ++cnt.rays;
   float best = tmax;
   Counters c{0, 0};
-  const int32_t p = trace_lane<COMPAT, false, kMeshBlock, kMeshRing>(m.g, o, d, best, c, m.ring);
+  const int32_t p = trace_lane<COMPAT, COUNT, kMeshBlock, kMeshRing>(m.g, o, d, best, c, m.ring);
+  if (COUNT) {
+    ++cnt.rays;
+    cnt.nodes += c.nodes;
+    cnt.tris += c.tris;
+  }
   *t = best;
   if (p >= 0) {
     const BTri tri = m.g.tris[p];
@@ -206,18 +211,19 @@ FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const
   return false;
 }
 
-// Adds the wave's counts into counts[0] (lookups) and counts[2] (rays).
+// Adds the wave's counts into counts[0] (lookups), counts[2] (rays),
+// counts[3] (node visits) and counts[4] (triangle tests).
 template <bool COUNT>
 FLG_D void vp_count(const VpCount& cnt, unsigned long long* counts) {
   if (COUNT) {
-    unsigned long long c = cnt.look, r = cnt.rays;
-    for (int o = 32; o >= 1; o >>= 1) {
-      c += __shfl_xor(c, o, 64);
-      r += __shfl_xor(r, o, 64);
-    }
+    unsigned long long c[4] = {cnt.look, cnt.rays, cnt.nodes, cnt.tris};
+    for (int k = 0; k < 4; ++k)
+      for (int o = 32; o >= 1; o >>= 1) c[k] += __shfl_xor(c[k], o, 64);
     if ((threadIdx.x & 63) == 0) {
-      atomicAdd(counts, c);
-      atomicAdd(counts + 2, r);
+      atomicAdd(counts, c[0]);
+      atomicAdd(counts + 2, c[1]);
+      atomicAdd(counts + 3, c[2]);
+      atomicAdd(counts + 4, c[3]);
     }
   }
 }
@@ -293,7 +299,7 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath_regen(TraceG
 }
 
 __global__ void k_vp_zero(unsigned long long* p) {
-  if (threadIdx.x < 3) p[threadIdx.x] = 0ull;
+  if (threadIdx.x < 5) p[threadIdx.x] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_vp_mean(const float* lpath, int64_t npix, int32_t spp, float* rgb) {
@@ -593,7 +599,8 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
     char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
     if (!sc) return st;
-    // nl[0]: lookups, nl[1]: the path counter, nl[2]: rays (cleared together)
+    // nl[0]: lookups, nl[1]: the path counter, nl[2..4]: rays, node visits,
+    // triangle tests (cleared together)
     unsigned long long* nl = (unsigned long long*)sc;
     unsigned int* next_path = (unsigned int*)(nl + 1);
     float* lpath = (float*)(sc + 256);
@@ -633,12 +640,14 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
       k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
     FLG_CHECK_LAUNCH(ctx);
     if (h_stats) {
-      unsigned long long c[3] = {0, 0, 0};
-      FLG_CHECK_HIP(ctx, hipMemcpyAsync(c, nl, 24, hipMemcpyDeviceToHost, ctx->stream));
+      unsigned long long c[5] = {0, 0, 0, 0, 0};
+      FLG_CHECK_HIP(ctx, hipMemcpyAsync(c, nl, 40, hipMemcpyDeviceToHost, ctx->stream));
       FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
       h_stats[0] = (uint64_t)(ntasks > 0 ? npix * cfg->spp : 0);
       h_stats[1] = c[0];
       h_stats[2] = c[2];
+      h_stats[3] = c[3];
+      h_stats[4] = c[4];
     }
     return FLG_OK;
   });

@@ -421,7 +421,8 @@ FLG_API int flg_volpath_render_noalloc(flg_ctx* ctx, flg_volume* vol, const flg_
  * render loop and film, as flg_volpath_render's film mode.  h_stats (may be
  * NULL): {path samples, density lookups, rays traced on the mesh (camera
  * rays, continuations after a scattering or an interface, shadow-ray
- * segments)}, three entries. */
+ * segments), their node visits, their triangle tests}, five entries; a call
+ * with h_stats runs the counting build of the kernels (slower). */
 FLG_API int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol,
                                     const flg_vol_scene* scene, const flg_render_cfg* cfg, float* d_rgb,
                                     uint64_t* h_stats);

@@ -124,6 +124,7 @@ def test_sparse_mesh_render_bit_equal(ctx, schedule):
     b, sb = sparse.render_mesh(bvh, sc, 48, 40, 8, 16, stats=True, schedule=schedule)
     assert np.array_equal(_bits(a), _bits(b))
     assert sa == sb and sa["rays"] >= 48 * 40 * 8
+    assert sa["node_visits"] >= sa["rays"] and sa["tri_tests"] > 0
 
 
 def test_sparse_differentiable_paths(ctx):
