@@ -168,7 +168,8 @@ class LBVH:
             descs[i].nverts = v.numel() // 3
         out = (ctypes.c_void_p * n)()
         ctx.check(ctx.lib.flg_lbvh_build_batched(ctx.h, descs, n, 1 if compat else 0, out))
-        return [cls(ctx, ctypes.c_void_p(out[i]), descs[i].ntris, keep=tuple(keep)) for i in range(n)]
+        keep = tuple(keep)  # one tuple shared by the batch's handles
+        return [cls(ctx, ctypes.c_void_p(out[i]), descs[i].ntris, keep=keep) for i in range(n)]
 
     def info(self):
         inf = N.LbvhInfo()
