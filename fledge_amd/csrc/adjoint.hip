@@ -302,7 +302,7 @@ int diff_prepare(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal, cons
     return set_error(ctx, FLG_ERR_INVALID, "differentiable render: majorant and albedo must be > 0");
   *hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp->stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, "differentiable render: spp * Halton stride must stay below 2^24");
+    return set_error(ctx, FLG_ERR_RANGE, "differentiable render: spp * Halton stride must stay at or below 2^32");
   int st = get_halton_tables(ctx, tab);
   if (st) return st;
   if ((st = make_lights(ctx, vol, scene, L, bounds))) return st;

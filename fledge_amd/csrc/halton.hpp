@@ -6,8 +6,8 @@
 // Tables (primes of the linear sieve below 262144, sampler.h:39-57, and their
 // float reciprocals 1.0f / base) live in device memory, one copy per context.
 // The digit loop of RadicalInverse_ (sampler.h:59-72) divides by the prime in
-// double with a refined reciprocal and a small bias, exact for indices below
-// 2^24 (checked by the host before launch).
+// double with a refined reciprocal and a small bias, exact for every 32-bit
+// index (the host keeps offset + sample * stride below 2^32).
 #pragma once
 #include "common.hpp"
 
@@ -40,28 +40,30 @@ int get_halton_tables(flg_ctx* ctx, HaltonTables* out);
 bool halton_tables_ready(flg_ctx* ctx, HaltonTables* out);
 void release_halton_tables(flg_ctx* ctx);
 HaltonParams make_halton_params(int resx, int resy);
-constexpr uint32_t kMaxHaltonIndex = 1u << 24;
+constexpr uint64_t kMaxHaltonIndex = 1ull << 32;
 
 // The digit loop on the double unit.  r = 1/b (1 + d) comes from the float
-// reciprocal r0 = (1 + e)/b, |e| <= 2^-24, by one Newton step: 1 - b r0 = -e
-// exactly (b r0 has at most 42 significant bits), and r = r0 (1 - e) rounded,
-// so |d| <= 2^-48 + 2^-53.  For a < 2^24 and a prime b < 2^18 the quotient is
-// q = floor(a r + 2^-22): a/b <= 2^23 moves by at most 2^23 |d| < 2^-24.9
-// and the sum's rounding by 2^-29, both below the bias, so an exact multiple
-// stays in (q, q + 1); otherwise a/b - q lies in [1/b, 1 - 1/b] with
-// 1/b >= 2^-18, far from both ends.  The remainder a - q b and the reversed
-// digits (below b a < 2^42) are exact in double fma.  The result is the
-// reference's: (float) of the same integer, times the same float product
-// chain of 1 / base.
+// reciprocal r0 = (1 + e)/b, |e| <= 2^-24, by two Newton steps: 1 - b r0 = -e
+// exactly (b r0 has at most 42 significant bits), r1 = r0 (1 - e) rounded
+// has |d1| <= 2^-48 + 2^-53, and the second step leaves |d| <= 2^-53 plus
+// terms below 2^-95.  For a < 2^32 and a prime b < 2^18 the quotient is
+// q = floor(a r + 2^-19): a/b < 2^31 moves by at most 2^31 |d| <= 2^-22 and
+// the sum's rounding (below 2^31) by 2^-22, together below the bias, so an
+// exact multiple stays in (q, q + 1); otherwise a/b - q lies in
+// [1/b, 1 - 1/b] with 1/b > 2^-18, which the bias and errors cannot cross.
+// The remainder a - q b and the reversed digits (below b a < 2^50) are exact
+// in double fma.  The result is the reference's: (float) of the same
+// integer, times the same float product chain of 1 / base.
 FLG_HD float radical_inverse(int32_t base, float inv_base, uint32_t a) {
   const double bd = (double)base;
   const double r0 = (double)inv_base;
-  const double r = __builtin_fma(r0, __builtin_fma(-bd, r0, 1.0), r0);
+  const double r1 = __builtin_fma(r0, __builtin_fma(-bd, r0, 1.0), r0);
+  const double r = __builtin_fma(r1, __builtin_fma(-bd, r1, 1.0), r1);
   double ad = (double)a;
   double reversed = 0.0;
   float inv_base_n = 1.0f;
   while (ad != 0.0) {
-    const double q = floor(__builtin_fma(ad, r, 0x1p-22));
+    const double q = floor(__builtin_fma(ad, r, 0x1p-19));
     reversed = __builtin_fma(reversed, bd, __builtin_fma(-q, bd, ad));
     inv_base_n *= inv_base;
     ad = q;

@@ -712,7 +712,7 @@ int flg_camera_rays(flg_ctx* ctx, const flg_camera* cam, int32_t width, int32_t 
       return set_error(ctx, FLG_ERR_INVALID, "flg_camera_rays: bad arguments");
     HaltonParams hp = make_halton_params(width, height);
     if ((uint64_t)hp.stride * (uint64_t)spp > kMaxHaltonIndex)
-      return set_error(ctx, FLG_ERR_RANGE, "flg_camera_rays: spp * Halton stride must stay below 2^24");
+      return set_error(ctx, FLG_ERR_RANGE, "flg_camera_rays: spp * Halton stride must stay at or below 2^32");
     HaltonTables tab;
     int st = get_halton_tables(ctx, &tab);
     if (st) return st;
@@ -753,7 +753,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
                        "ring_depth * 8 <= 65536 bytes of LDS");
     HaltonParams hp = make_halton_params(cfg->width, cfg->height);
     if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-      return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay below 2^24");
+      return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: spp * Halton stride must stay at or below 2^32");
     int64_t npaths = (int64_t)cfg->width * cfg->height * cfg->spp;
     if (npaths >= (int64_t)1 << 31) return set_error(ctx, FLG_ERR_RANGE, "flg_pathtrace_diffuse: too many paths");
     HaltonTables tab;

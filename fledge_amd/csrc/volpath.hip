@@ -589,7 +589,7 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     const float* mb = bvh->bound;
     HaltonParams hp = make_halton_params(cfg->width, cfg->height);
     if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay below 2^24");
+      return set_error(ctx, FLG_ERR_RANGE, "flg_volpath_mesh_render: spp * Halton stride must stay at or below 2^32");
     const bool film = cfg->film == 1;
     const int64_t npix = (int64_t)cfg->width * cfg->height;
     // film mode: spp - 1 splatted samples per pixel (the first task also runs

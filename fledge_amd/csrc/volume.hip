@@ -874,7 +874,7 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
                      std::string(what) + ": schedule 3 keeps the path state in allocated scratch; use 1 or 2");
   HaltonParams hp = make_halton_params(cfg->width, cfg->height);
   if ((uint64_t)hp.stride * (uint64_t)cfg->spp > kMaxHaltonIndex)
-    return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": spp * Halton stride must stay below 2^24");
+    return set_error(ctx, FLG_ERR_RANGE, std::string(what) + ": spp * Halton stride must stay at or below 2^32");
   const bool film = cfg->film == 1;
   const int64_t npix = (int64_t)cfg->width * cfg->height;
   // film mode: spp - 1 splatted samples per pixel (the first task also runs

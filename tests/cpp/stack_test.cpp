@@ -175,16 +175,17 @@ void check_tile_order() {
 
 // radical_inverse divides in double with a biased refined reciprocal; its
 // result must equal RadicalInverse_'s integer digit loop (sampler.h:59-72)
-// for every prime below 2^18 and every index below 2^24.  Checked: the
-// exact multiples and their neighbours at the top of the range (where the
-// rounding argument is tightest), small powers, random indices for every
-// prime, and every index for bases 2 and 3.
-float radical_inverse_int(uint32_t base, uint32_t a) {
+// for every prime below 2^18 and every 32-bit index.  Checked: the exact
+// multiples and their neighbours at the top of the range and halfway (where
+// the rounding argument is tightest), prime powers, random indices below
+// 2^32 and below 2^24 for every prime, and every index below 2^24 for bases
+// 2 and 3.
+float radical_inverse_int(uint32_t base, uint64_t a) {
   const float inv_base = 1.0f / (float)base;
   uint64_t reversed = 0;
   float inv_base_n = 1.0f;
   while (a) {
-    const uint32_t next = a / base;
+    const uint64_t next = a / base;
     reversed = reversed * base + (a - next * base);
     inv_base_n *= inv_base;
     a = next;
@@ -202,31 +203,34 @@ void check_radical_inverse() {
       for (int64_t j = (int64_t)i * i; j < (1 << 18); j += i) comp[j] = 1;
     }
   }
-  const uint32_t top = 1u << 24;
-  std::mt19937 gen(11);
-  std::uniform_int_distribution<uint32_t> U(0, top - 1);
+  const uint64_t top = (uint64_t)1 << 32;
+  std::mt19937_64 gen(11);
+  std::uniform_int_distribution<uint64_t> U32(0, top - 1), U24(0, (1u << 24) - 1);
   int64_t checked = 0;
-  auto one = [&](int32_t p, uint32_t a) {
-    const float got = flg::radical_inverse(p, 1.0f / (float)p, a), want = radical_inverse_int((uint32_t)p, a);
+  auto one = [&](int32_t p, uint64_t a) {
+    const float got = flg::radical_inverse(p, 1.0f / (float)p, (uint32_t)a),
+                want = radical_inverse_int((uint32_t)p, a);
     ++checked;
     if (flg::f2u(got) != flg::f2u(want) && failures < 20) {
-      std::printf("FAIL radical_inverse(%d, %u) = %.9g, integer loop %.9g\n", p, a, got, want);
+      std::printf("FAIL radical_inverse(%d, %llu) = %.9g, integer loop %.9g\n", p, (unsigned long long)a, got,
+                  want);
       ++failures;
     }
   };
   for (int32_t p : primes) {
-    const uint32_t b = (uint32_t)p, kmax = (top - 1) / b;
-    for (uint32_t k : {kmax, kmax - 1, kmax / 2 + 1, 1u, 2u})
+    const uint64_t b = (uint64_t)p, kmax = (top - 1) / b;
+    for (uint64_t k : {kmax, kmax - 1, kmax / 2 + 1, (uint64_t)(((1ull << 24) - 1) / b), (uint64_t)1, (uint64_t)2})
       for (int d = -1; d <= 1; ++d) {
-        const int64_t a = (int64_t)k * b + d;
-        if (a >= 0 && a < top) one(p, (uint32_t)a);
+        const int64_t a = (int64_t)(k * b) + d;
+        if (a >= 0 && (uint64_t)a < top) one(p, (uint64_t)a);
       }
     for (uint64_t pw = b; pw < top; pw *= b)
-      for (int d = -1; d <= 1; ++d) one(p, (uint32_t)(pw + d));
-    for (int i = 0; i < 64; ++i) one(p, U(gen));
+      for (int d = -1; d <= 1; ++d) one(p, pw + d);
+    for (int i = 0; i < 32; ++i) one(p, U32(gen));
+    for (int i = 0; i < 32; ++i) one(p, U24(gen));
   }
   for (int32_t p : {2, 3})
-    for (uint32_t a = 0; a < top; ++a) one(p, a);
+    for (uint64_t a = 0; a < (1u << 24); ++a) one(p, a);
   std::printf("radical_inverse: %lld cases checked\n", (long long)checked);
 }
 

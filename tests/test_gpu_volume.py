@@ -45,6 +45,8 @@ def test_halton_bitexact(ctx):
     rng = np.random.default_rng(1)
     n = 200000
     idx = rng.integers(0, 31104 * 64, n).astype(np.uint32)
+    idx[2000:60000] = rng.integers(0, 2 ** 32, 58000, dtype=np.uint64).astype(np.uint32)  # the whole index range
+    idx[60000:60100] = np.uint32(2 ** 32 - 1) - np.arange(100, dtype=np.uint32)
     dim = rng.integers(0, 3000, n).astype(np.uint32)
     dim[:1000] = 0
     dim[1000:2000] = 1
@@ -214,3 +216,29 @@ def test_volpath_full_benchmark_frame(ctx):
     wf, sw = vol.render(api.vol_scene(cam), 512, 512, 64, 32, stats=True, schedule=3)
     assert sw["lookups"] == lk
     assert np.array_equal(wf.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_volpath_halton_indices_beyond_2pow24(ctx):
+    """700 samples per pixel at 128 x 128 (Halton stride 31,104): sample
+    indices reach 21.8 M, past 2^24, where a float quotient would no longer
+    be exact.  Every 37th pixel against the oracle's integer digit loop."""
+    grid = api.gen_fbm_grid(16, seed=4)
+    W = H = 128
+    spp, depth = 700, 8
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    out = vol.render(api.vol_scene(cam), W, H, spp, depth).cpu().numpy().reshape(-1, 3)
+    L = orc.lib()
+    rgb = np.zeros((H, W, 3), np.float32)
+    lk = np.zeros(1, np.uint64)
+    pa = np.zeros(1, np.uint64)
+    stride = 37
+    L.orc_volpath_render_strided(orc._p(grid), 16, 16, 16, orc._p(np.array([0, 0, 0, 1, 1, 1], np.float32)),
+                                 orc._p(np.array([6.4, 1.6, 0.3], np.float32)),
+                                 orc._p(np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)),
+                                 orc._p(np.array(CAM0, np.float32)), W, H, spp, depth, stride, orc._p(rgb), 8,
+                                 orc._p(lk), orc._p(pa))
+    sel = np.arange(0, W * H, stride)
+    assert int(pa[0]) == sel.size * spp
+    ref = rgb.reshape(-1, 3)[sel]
+    assert np.array_equal(out[sel].view(np.uint32), ref.view(np.uint32))
