@@ -560,7 +560,7 @@ def bench_vol(ctx, N, api, torch, dev, sync, cpu_jobs):
         t0 = time.perf_counter()
         lib.orc_volpath_render_strided(
             orc._p(g1), 256, 256, 256, orc._p(np.array([0, 0, 0, 1, 1, 1], np.float32)),
-            orc._p(np.array([6.4, 1.6, 0.3], np.float32)), orc._p(np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)),
+            orc._p(np.array([6.4, 1.6, 0.3], np.float32)), orc._p(orc.lights11([1, 2, 2, -2, 50.0, 1, 0.5])),
             orc._p(cam10), Wv, Hv, S1, 32, stride, orc._p(rgb), cores, orc._p(lk), orc._p(pa))
         dtc = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(float(pa[0]) / dtc, 1), "unit": "path samples/s", "cores": cores,
@@ -636,7 +636,7 @@ def bench_combined(ctx, N, api, torch, np, dev, sync, cam2, W, H, profile=False,
     dvc = diff.DiffVolume.sparse(ctx, origins, values, (512, 512, 512), g=0.3)
     layout = diff.BrickLayout.of_volume(dvc.vol)
     tape = diff.new_tape(ctx, W, H, SPPc, Dc)
-    lpc = torch.empty(W * H * SPPc, dtype=torch.float32, device=dev)
+    lpc = torch.empty((W * H * SPPc, 3), dtype=torch.float32, device=dev)
     dlc = torch.full((H, W, 3), 1.0 / (W * H * 3), dtype=torch.float32, device=dev)
     gb = layout.zeros()
     gac = torch.zeros(1, dtype=torch.float32, device=dev)
@@ -788,7 +788,7 @@ def cpu_adjoint_baseline(cores):
     SPP, DEPTH = 16, 32
     mu = 8.0 * float(theta.max())
     params = np.array([8.0, 0.8, 0.3, mu], np.float32)
-    lights = np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)
+    lights = orc.lights11([1, 2, 2, -2, 50.0, 1, 0.5])  # grey: one channel covers all three
     cam10 = np.array([0.5, 0.5, -1.5, 0.5, 0.5, 0.0, 0.0, 1.0, 0.0, 45.0], np.float32)
     bounds = np.array([0, 0, 0, 1, 1, 1], np.float32)
     stride = 1
@@ -800,10 +800,10 @@ def cpu_adjoint_baseline(cores):
     lk = np.zeros(1, np.uint64)
     t0 = time.perf_counter()
     L.orc_diff_render_strided(P(th64), P(theta), 128, 128, 128, P(bounds), P(params), P(lights), P(cam10), W, H,
-                              SPP, DEPTH, stride, P(lp), P(img), None, None, None, cores, P(lk))
+                              SPP, DEPTH, stride, P(lp), P(img), None, None, None, cores, P(lk), 0)
     t1 = time.perf_counter()
     L.orc_diff_render_strided(P(th64), P(theta), 128, 128, 128, P(bounds), P(params), P(lights), P(cam10), W, H,
-                              SPP, DEPTH, stride, P(lp), None, P(dl), P(grad), P(ga), cores, None)
+                              SPP, DEPTH, stride, P(lp), None, P(dl), P(grad), P(ga), cores, None, 0)
     t2 = time.perf_counter()
     npx = (W * H + stride - 1) // stride
     scale = W * H / npx

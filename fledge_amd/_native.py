@@ -81,7 +81,7 @@ class Medium(ctypes.Structure):
 
 class VolScene(ctypes.Structure):
     _fields_ = [("camera", Camera), ("has_point", c_i32), ("point_pos", c_f * 3),
-                ("point_intensity", c_f), ("has_env", c_i32), ("env_radiance", c_f)]
+                ("point_intensity", c_f * 3), ("has_env", c_i32), ("env_radiance", c_f * 3)]
 
 
 class RenderCfg(ctypes.Structure):

@@ -484,14 +484,28 @@ def leaves_of(density):
     return origins, bricks[idx].contiguous()
 
 
+def _rgb(v, name):
+    """A light's colour: one number (grey) or three (RGB)."""
+    try:
+        c = [float(x) for x in v]
+    except TypeError:
+        c = [float(v)] * 3
+    if len(c) != 3:
+        raise ValueError("%s: one value or three (RGB)" % name)
+    return c
+
+
 def vol_scene(cam, point=(2.0, 2.0, -2.0), intensity=50.0, env=0.5, has_point=True, has_env=True):
+    """Lights of the volume scenes: a point light of radiant intensity
+    `intensity` and a constant environment of radiance `env`, each one value
+    (grey) or three (RGB)."""
     s = N.VolScene()
     s.camera = cam
     s.has_point = 1 if has_point else 0
     s.point_pos[:] = [float(x) for x in point]
-    s.point_intensity = float(intensity)
+    s.point_intensity[:] = _rgb(intensity, "intensity")
     s.has_env = 1 if has_env else 0
-    s.env_radiance = float(env)
+    s.env_radiance[:] = _rgb(env, "env")
     return s
 
 

@@ -51,12 +51,14 @@ FLG_D void box_track_deposit(const DiffDev& D, uint32_t key, V3 o, V3 d, float t
 
 // One path, as a start (camera ray, box entry, primary transmittance) and one
 // step per scattering event, so that persistent lanes can interleave paths.
-// ADJ = false: accumulates the radiance.  ADJ = true: replays with the path's
-// total L_total and adjoint weight delta, depositing gradients.
+// ADJ = false: accumulates the RGB radiance (rgb).  ADJ = true: replays the
+// path in the loss's units: the lights' weights wp and we (light_weights),
+// acc the weighted radiance emitted so far, L_total the path's weighted
+// total, delta = 1 / spp, depositing gradients.
 struct DiffPath {
   PathSampler ps;
-  V3 o, d;
-  float beta, acc, delta, L_total;
+  V3 o, d, rgb;
+  float beta, acc, delta, L_total, wp, we;
   int bounces;
   uint32_t path;
   uint64_t mask;  // tape bits of the path's walks with exactly one zero factor
@@ -70,14 +72,17 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
   const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
   const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
   s.acc = 0.0f;
+  s.rgb = v3(0.0f, 0.0f, 0.0f);
   s.path = p;
   s.mask = 0;
-  s.delta = s.L_total = 0.0f;
+  s.delta = s.L_total = s.wp = s.we = 0.0f;
   if (ADJ) {
     const float* g = dl_drgb + 3 * (int64_t)pix;
-    s.delta = (g[0] + g[1] + g[2]) / (float)spp;
-    if (s.delta == 0.0f) return true;
-    s.L_total = lpath[p];
+    light_weights(L, g, &s.wp, &s.we);
+    if (s.wp == 0.0f && s.we == 0.0f) return true;
+    s.delta = 1.0f / (float)spp;
+    const float* l = lpath + 3 * (int64_t)p;
+    s.L_total = g[0] * l[0] + g[1] * l[1] + g[2] * l[2];
     if (D.tape) s.mask = ((uint64_t)D.mask_hi[p] << 32) | D.mask_lo[p];
   }
   s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
@@ -86,16 +91,18 @@ FLG_D bool diff_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& 
   const float uy = s.ps.next(tab, hp);
   V3 o, d;
   gen_ray_v(cam, (float)x + ux, (float)y + uy, &o, &d);
-  const float Le = L.env.present ? L.env.radiance : 0.0f;
+  const V3 Le = L.env.present ? L.env.radiance : v3(0.0f, 0.0f, 0.0f);
   float t_min, t_max;
   if (!box_enter(D.v.bmin, D.v.bmax, o, d, &t_min, &t_max)) {
-    s.acc = Le;
+    s.rgb = Le;
     return true;
   }
   const uint32_t key = walk_key(p, 0);
   const Track tr = track<ADJ>(D, p, 0, s.mask, [&] { return box_track_measure(D, key, o, d, kInf); });
-  if (ADJ && track_live(tr) && s.delta * Le != 0.0f) box_track_deposit(D, key, o, d, kInf, tr, s.delta * Le, grad);
-  s.acc += track_value(tr) * Le;
+  if (ADJ && track_live(tr) && s.delta * s.we != 0.0f)
+    box_track_deposit(D, key, o, d, kInf, tr, s.delta * s.we, grad);
+  if (ADJ) s.acc += track_value(tr) * s.we;
+  else add_scaled(s.rgb, track_value(tr), Le);
   s.o = o + d * t_min;
   s.d = d;
   s.beta = 1.0f;
@@ -116,19 +123,23 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
   const float u = s.ps.next(tab, hp);
   const float vv = s.ps.next(tab, hp);
   const V3 wi = hg_sample(wo, u, vv, D.g);
-  V3 p_light;
-  const float K = nee_sample(D, L, tab, hp, s.ps, pos, wo, &p_light);
-  if (K != 0.0f) {
-    const V3 diff = p_light - pos;
+  const NeeSample ns = nee_sample(D, L, tab, hp, s.ps, pos, wo);
+  if (ns.valid) {
+    const V3 diff = ns.p_light - pos;
     const float nrm = sqrtf(sqnorm(diff));
     const V3 dd = diff / nrm;
     const uint32_t w = (uint32_t)s.bounces;
     const uint32_t key = walk_key(s.path, w);
     const Track tr =
         track<ADJ>(D, s.path, w, s.mask, [&] { return box_track_measure(D, key, pos, dd, nrm - SHADOW_EPS); });
-    if (ADJ && track_live(tr) && s.delta * s.beta * K != 0.0f)
-      box_track_deposit(D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
-    s.acc += s.beta * K * track_value(tr);
+    if (ADJ) {
+      const float K = nee_k(ns, nee_li_w(ns, s.wp, s.we));
+      if (track_live(tr) && s.delta * s.beta * K != 0.0f)
+        box_track_deposit(D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
+      s.acc += s.beta * K * track_value(tr);
+    } else {
+      add_nee(s.rgb, L, ns, s.beta, track_value(tr));
+    }
   }
   if (s.bounces >= max_depth || s.beta == 0.0f) return true;
   s.o = pos;
@@ -141,9 +152,12 @@ FLG_D bool diff_step(const DiffDev& D, const LightsDev& L, const HaltonTables& t
   return false;
 }
 
-// Forward: the path's radiance and its tape bits.
+// Forward: the path's RGB radiance and its tape bits.
 FLG_D void diff_finish(const DiffDev& D, uint32_t p, const DiffPath& s, float* lpath) {
-  lpath[p] = s.acc;
+  float* o = lpath + 3 * (int64_t)p;
+  o[0] = s.rgb.x;
+  o[1] = s.rgb.y;
+  o[2] = s.rgb.z;
   if (D.tape) {
     D.mask_lo[p] = (uint32_t)s.mask;
     D.mask_hi[p] = (uint32_t)(s.mask >> 32);
@@ -215,18 +229,6 @@ int launch_diff(flg_ctx* ctx, const flg_render_cfg* cfg, const DiffDev& D, const
   k_diff_regen<ADJ><<<grid, 256, 0, ctx->stream>>>(D, L, tab, hp, cam, cfg->spp, cfg->max_depth, lpath, dl,
                                                    grad, galpha, counter);
   return FLG_OK;
-}
-
-__global__ __launch_bounds__(256) void k_mean(const float* lpath, int64_t npix, int32_t spp,
-                                              float* rgb) {
-  int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= npix) return;
-  float c = 0.0f;
-  for (int s = 0; s < spp; ++s) c += lpath[pix * spp + s];
-  c = c / (float)spp;
-  rgb[3 * pix] = c;
-  rgb[3 * pix + 1] = c;
-  rgb[3 * pix + 2] = c;
 }
 
 int64_t tape_bytes(const flg_render_cfg* cfg) {
@@ -395,13 +397,13 @@ int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_volume* pr
     const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
     float* lpath = d_lpath;
     if (!lpath) {
-      lpath = (float*)scratch(ctx, "diff_lpath", (size_t)npaths * 4, &st);
+      lpath = (float*)scratch(ctx, "diff_lpath", (size_t)npaths * 12, &st);
       if (!lpath) return st;
     }
     {
       ScopedTimer tm(ctx, "diff_forward");
       if ((st = launch_diff<false>(ctx, cfg, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr))) return st;
-      if (d_rgb) k_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+      if (d_rgb) pixel_mean(ctx->stream, lpath, npix, cfg->spp, d_rgb);
     }
     FLG_CHECK_LAUNCH(ctx);
     return FLG_OK;

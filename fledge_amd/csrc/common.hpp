@@ -228,9 +228,9 @@ int graph_launch(flg_ctx* ctx, const std::string& key, const std::function<void(
                  const char* what);
 void* scratch(flg_ctx* ctx, const char* key, size_t bytes, int* status);
 
-// The fixed per-GPU scratch of flg_volpath_render_noalloc: 64 MiB of path
-// radiance (a 512 x 512 frame at 64 spp) plus a page of counters.
-constexpr size_t kNoAllocScratchBytes = ((size_t)64 << 20) + 4096;
+// The fixed per-GPU scratch of flg_volpath_render_noalloc: 192 MiB of RGB
+// path radiance (a 512 x 512 frame at 64 spp) plus a page of counters.
+constexpr size_t kNoAllocScratchBytes = ((size_t)192 << 20) + 4096;
 constexpr int kMaxNoAllocDevices = 64;
 // The device address of that array on `device` (which must be current),
 // looked up once per device; FLG_ERR_HIP if the lookup fails.

@@ -173,34 +173,69 @@ FLG_D Track track(const DiffDev& D, uint32_t p, uint32_t w, uint64_t& mask, Meas
 }
 
 // Light choice and sample (UniformSampleOneLight / EstimateDirect,
-// integrator.cpp:69-120) without the shadow transmittance: returns
-// K = f Li / pdf / light_pdf, 0 if no contribution.
-FLG_D float nee_sample(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
-                       const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo, V3* p_light) {
+// integrator.cpp:69-120) without the shadow transmittance.  valid: pdf != 0,
+// Li != 0 in some channel and f != 0; then the estimate of one channel is
+// nee_k(n, Li) times the shadow transmittance, Li = light_li(L, n.d2) per
+// channel.  The adjoint prices the estimate in the loss's units instead:
+// nee_k with Li = sum_c dLoss/dL_c Li_c (nee_li_w), the linear combination of
+// the channels the gradient needs.
+struct NeeSample {
+  V3 p_light;
+  float f, d2, pdf, light_pdf;  // d2 < 0: the environment light
+  bool valid;
+};
+
+FLG_D NeeSample nee_sample(const DiffDev& D, const LightsDev& L, const HaltonTables& tab,
+                           const HaltonParams& hp, PathSampler& ps, V3 p, V3 wo) {
+  NeeSample n;
+  n.valid = false;
   const int n_lights = (L.has_point ? 1 : 0) + (L.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
+  if (n_lights == 0) return n;
   int light_num = (int)(ps.next(tab, hp) * (float)n_lights);
   if (light_num > n_lights - 1) light_num = n_lights - 1;
-  const float light_pdf = (float)(1.0 / (double)n_lights);
+  n.light_pdf = (float)(1.0 / (double)n_lights);
   const bool use_point = L.has_point && light_num == 0;
   float u0 = ps.next(tab, hp);
   float u1 = ps.next(tab, hp);
   V3 wi;
-  float pdf, Li;
   if (use_point) {
     V3 diff = L.point_pos - p;
-    float d2 = sqnorm(diff);
+    n.d2 = sqnorm(diff);
     wi = normalize(diff);
-    Li = L.point_intensity / d2;
-    pdf = 1.0f;
-    *p_light = L.point_pos;
+    n.pdf = 1.0f;
+    n.p_light = L.point_pos;
   } else {
-    Li = env_sample_li(L.env, p, u0, u1, &wi, &pdf, p_light);
+    env_sample_dir(L.env, p, u0, u1, &wi, &n.pdf, &n.p_light);
+    n.d2 = -1.0f;
   }
-  if (pdf == 0.0f || Li == 0.0f) return 0.0f;
-  float f = hg_p(wi, wo, D.g);
-  if (f == 0.0f) return 0.0f;
-  return f * Li / pdf / light_pdf;
+  const V3 Li = light_li(L, n.d2);
+  if (n.pdf == 0.0f || (Li.x == 0.0f && Li.y == 0.0f && Li.z == 0.0f)) return n;
+  n.f = hg_p(wi, wo, D.g);
+  n.valid = n.f != 0.0f;
+  return n;
+}
+
+// f Li / pdf / light_pdf for one channel's Li.
+FLG_D float nee_k(const NeeSample& n, float Li) { return n.f * Li / n.pdf / n.light_pdf; }
+
+// The sampled light's Li in the loss's units: wp = sum_c g_c I_c for the
+// point light (over d2), we = sum_c g_c E_c for the environment.
+FLG_D float nee_li_w(const NeeSample& n, float wp, float we) { return n.d2 < 0.0f ? we : wp / n.d2; }
+
+// The adjoint's per-pixel weights of the two lights: g = dLoss/dL of the
+// pixel (RGB).
+FLG_D void light_weights(const LightsDev& L, const float* g, float* wp, float* we) {
+  const V3 I = L.point_intensity, E = L.env.radiance;
+  *wp = L.has_point ? g[0] * I.x + g[1] * I.y + g[2] * I.z : 0.0f;
+  *we = L.env.present ? g[0] * E.x + g[1] * E.y + g[2] * E.z : 0.0f;
+}
+
+// acc += beta * K * tr per channel, K the channel's nee_k.
+FLG_D void add_nee(V3& acc, const LightsDev& L, const NeeSample& n, float beta, float tr) {
+  const V3 Li = light_li(L, n.d2);
+  acc.x += beta * nee_k(n, Li.x) * tr;
+  acc.y += beta * nee_k(n, Li.y) * tr;
+  acc.z += beta * nee_k(n, Li.z) * tr;
 }
 
 // Differential delta tracking along (o, d) within the medium and [0,

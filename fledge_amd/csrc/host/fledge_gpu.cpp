@@ -620,12 +620,12 @@ void GPUVolIntegrator::render(const GridVolume& volume, const Vector3f& origin, 
     sc.camera.look_at[k] = target[k];
     sc.camera.up[k] = up[k];
     sc.point_pos[k] = m_lights.point_pos[k];
+    sc.point_intensity[k] = m_lights.point_intensity[k];
+    sc.env_radiance[k] = m_lights.env_radiance[k];
   }
   sc.camera.fov_deg = fov;
   sc.has_point = m_lights.has_point ? 1 : 0;
-  sc.point_intensity = m_lights.point_intensity;
   sc.has_env = m_lights.has_env ? 1 : 0;
-  sc.env_radiance = m_lights.env_radiance;
   flg_render_cfg cfg;
   std::memset(&cfg, 0, sizeof(cfg));
   cfg.width = width;
@@ -1106,12 +1106,12 @@ bool HIPRender::render() {
     vs.camera.look_at[k] = sc.m_target[k];
     vs.camera.up[k] = sc.m_up[k];
     vs.point_pos[k] = m_lights.point_pos[k];
+    vs.point_intensity[k] = m_lights.point_intensity[k];
+    vs.env_radiance[k] = m_lights.env_radiance[k];
   }
   vs.camera.fov_deg = sc.m_FoV;
   vs.has_point = m_lights.has_point ? 1 : 0;
-  vs.point_intensity = m_lights.point_intensity;
   vs.has_env = m_lights.has_env ? 1 : 0;
-  vs.env_radiance = m_lights.env_radiance;
   flg_render_cfg cfg;
   std::memset(&cfg, 0, sizeof(cfg));
   cfg.width = sc.m_resX;

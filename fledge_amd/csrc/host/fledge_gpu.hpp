@@ -309,14 +309,15 @@ class GridVolume : public Volume {
 };
 
 // -------------------------------------------------------- Integrator ------
-// Lights of the volume-only integrator: a point light and a constant
-// environment (InfiniteAreaLight with a constant texture, light.cpp:93-159).
+// Lights of the volume integrators: a point light and a constant environment
+// (InfiniteAreaLight with a constant texture of colour env_radiance,
+// light.cpp:81-159), both RGB.
 struct GPUVolLights {
   bool has_point = false;
   Vector3f point_pos{0.0f};
-  Float point_intensity = 0;
+  Vector3f point_intensity{0.0f};
   bool has_env = false;
-  Float env_radiance = 0;
+  Vector3f env_radiance{0.0f};
 };
 
 // SVolIntegrator on the GPU (integrator.cpp:328-405): render() reads the

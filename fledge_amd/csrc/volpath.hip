@@ -105,7 +105,8 @@ FLG_D float shadow_tr(const MeshGeom& g, const VolDev& v, HaltonDraw& rng, V3 o,
 struct VPath {
   PathSampler ps;
   V3 o, d;
-  float L, beta;
+  V3 L;  // RGB
+  float beta;
   int bounces, crossings;
   bool inside;
   // film mode: the pixel, the film position of the splatted sample, and
@@ -118,7 +119,7 @@ struct VPath {
 // The camera ray at film position (fx, fy); the camera is outside the mesh.
 FLG_D void vp_start(const CamDevV& cam, float fx, float fy, VPath& s) {
   gen_ray_v(cam, fx, fy, &s.o, &s.d);
-  s.L = 0.0f;
+  s.L = v3(0.0f, 0.0f, 0.0f);
   s.beta = 1.0f;
   s.bounces = 0;
   s.crossings = 0;
@@ -179,7 +180,7 @@ FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const
   float t;
   V3 ng;
   const bool hit = mesh_hit<COMPAT, COUNT>(g, s.o, s.d, kInf, &t, &ng, cnt) >= 0;
-  if (s.bounces == 0 && !hit && L.env.present) s.L += s.beta * L.env.radiance;
+  if (s.bounces == 0 && !hit && L.env.present) add_scaled(s.L, s.beta, L.env.radiance);
   if (!hit || s.bounces >= max_depth) return true;
   bool success = false;
   V3 pos;
@@ -189,10 +190,10 @@ FLG_D bool vp_step(const MeshGeom& g, const VolDev& v, const LightsDev& L, const
   }
   if (success) {
     const V3 wo = -s.d;
-    s.L += s.beta * sample_one_light(L, v.g, tab, hp, s.ps, pos, wo, [&](V3 p, V3 d, float tmax) {
-      HaltonDraw rng{s.ps, tab, hp};
-      return shadow_tr<COMPAT, COUNT>(g, v, rng, p, d, tmax, true, cnt);
-    });
+    add_scaled(s.L, s.beta, sample_one_light(L, v.g, tab, hp, s.ps, pos, wo, [&](V3 p, V3 d, float tmax) {
+                 HaltonDraw rng{s.ps, tab, hp};
+                 return shadow_tr<COMPAT, COUNT>(g, v, rng, p, d, tmax, true, cnt);
+               }));
     const float u = s.ps.next(tab, hp);
     const float vv = s.ps.next(tab, hp);
     const V3 wi = hg_sample(wo, u, vv, v.g);
@@ -239,7 +240,10 @@ FLG_D void vp_task_begin(const HaltonTables& tab, const HaltonParams& hp, const 
 
 template <bool FILM>
 FLG_D void vp_task_finish(uint32_t p, const VPath& s, float* lpath, float2* fpos) {
-  lpath[p] = s.L;
+  float* o = lpath + 3 * (size_t)p;
+  o[0] = s.L.x;
+  o[1] = s.L.y;
+  o[2] = s.L.z;
   if (FILM) fpos[p] = make_float2(s.fx, s.fy);
 }
 
@@ -302,18 +306,6 @@ __global__ void k_vp_zero(unsigned long long* p) {
   if (threadIdx.x < 5) p[threadIdx.x] = 0ull;
 }
 
-__global__ __launch_bounds__(256) void k_vp_mean(const float* lpath, int64_t npix, int32_t spp, float* rgb) {
-  const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= npix) return;
-  float c = 0.0f;
-  const float* l = lpath + pix * spp;
-  for (int s = 0; s < spp; ++s) c += l[s];
-  c = c / (float)spp;
-  rgb[3 * pix] = c;
-  rgb[3 * pix + 1] = c;
-  rgb[3 * pix + 2] = c;
-}
-
 // ---------------------------------------------------------------------------
 // The differentiable estimator (diff.hpp) over the same scene: VolPath's path
 // structure and draw order (light sample, then phase sample; no roulette),
@@ -366,10 +358,12 @@ FLG_D void mesh_track_deposit(const MeshGeom& g, const DiffDev& D, uint32_t key,
   }
 }
 
+// The path state of the differentiable estimator over the mesh; rgb, acc,
+// delta, L_total, wp and we as in the volume-only estimator's DiffPath.
 struct DiffVPath {
   PathSampler ps;
-  V3 o, d;
-  float beta, acc, delta, L_total;
+  V3 o, d, rgb;
+  float beta, acc, delta, L_total, wp, we;
   int bounces, crossings;
   bool inside;
   uint32_t path;
@@ -379,19 +373,23 @@ struct DiffVPath {
 // Camera ray of path p; the adjoint (ADJ) also takes the path's radiance and
 // adjoint weight.  Returns true when the path has nothing to do.
 template <bool ADJ>
-FLG_D bool vpd_begin(const DiffDev& D, const HaltonTables& tab, const HaltonParams& hp, const CamDevV& cam,
-                     int32_t spp, uint32_t p, const float* lpath, const float* dl_drgb, DiffVPath& s) {
+FLG_D bool vpd_begin(const DiffDev& D, const LightsDev& L, const HaltonTables& tab, const HaltonParams& hp,
+                     const CamDevV& cam, int32_t spp, uint32_t p, const float* lpath, const float* dl_drgb,
+                     DiffVPath& s) {
   const uint32_t pix = p / (uint32_t)spp, smp = p - pix * (uint32_t)spp;
   const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
   s.acc = 0.0f;
+  s.rgb = v3(0.0f, 0.0f, 0.0f);
   s.path = p;
   s.mask = 0;
-  s.delta = s.L_total = 0.0f;
+  s.delta = s.L_total = s.wp = s.we = 0.0f;
   if (ADJ) {
     const float* gp = dl_drgb + 3 * (int64_t)pix;
-    s.delta = (gp[0] + gp[1] + gp[2]) / (float)spp;
-    if (s.delta == 0.0f) return true;
-    s.L_total = lpath[p];
+    light_weights(L, gp, &s.wp, &s.we);
+    if (s.wp == 0.0f && s.we == 0.0f) return true;
+    s.delta = 1.0f / (float)spp;
+    const float* l = lpath + 3 * (int64_t)p;
+    s.L_total = gp[0] * l[0] + gp[1] * l[1] + gp[2] * l[2];
     if (D.tape) s.mask = ((uint64_t)D.mask_hi[p] << 32) | D.mask_lo[p];
   }
   s.ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
@@ -413,16 +411,18 @@ FLG_D bool vpd_step(const MeshGeom& g, const DiffDev& D, const LightsDev& L, con
   float t;
   V3 ng;
   const bool hit = mesh_hit<COMPAT>(g, s.o, s.d, kInf, &t, &ng) >= 0;
-  if (s.bounces == 0 && !hit && L.env.present) s.acc += s.beta * L.env.radiance;
+  if (s.bounces == 0 && !hit && L.env.present) {
+    if (ADJ) s.acc += s.beta * s.we;
+    else add_scaled(s.rgb, s.beta, L.env.radiance);
+  }
   if (!hit || s.bounces >= max_depth) return true;
   V3 pos;
   if (s.inside && diff_delta_track<ADJ>(D, tab, hp, s.ps, s.o, s.d, t, s.beta, s.acc, s.delta, s.L_total, grad,
                                         galpha, &pos)) {
     const V3 wo = -s.d;
-    V3 p_light;
-    const float K = nee_sample(D, L, tab, hp, s.ps, pos, wo, &p_light);
-    if (K != 0.0f) {
-      const V3 diff = p_light - pos;
+    const NeeSample ns = nee_sample(D, L, tab, hp, s.ps, pos, wo);
+    if (ns.valid) {
+      const V3 diff = ns.p_light - pos;
       const float nrm = sqrtf(sqnorm(diff));
       const V3 dd = diff / nrm;
       const uint32_t w = (uint32_t)s.bounces + 1;
@@ -430,9 +430,14 @@ FLG_D bool vpd_step(const MeshGeom& g, const DiffDev& D, const LightsDev& L, con
       const Track tr = track<ADJ>(D, s.path, w, s.mask, [&] {
         return mesh_track_measure<COMPAT>(g, D, key, pos, dd, nrm - SHADOW_EPS);
       });
-      if (ADJ && track_live(tr) && s.delta * s.beta * K != 0.0f)
-        mesh_track_deposit<COMPAT>(g, D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
-      s.acc += s.beta * K * track_value(tr);
+      if (ADJ) {
+        const float K = nee_k(ns, nee_li_w(ns, s.wp, s.we));
+        if (track_live(tr) && s.delta * s.beta * K != 0.0f)
+          mesh_track_deposit<COMPAT>(g, D, key, pos, dd, nrm - SHADOW_EPS, tr, s.delta * s.beta * K, grad);
+        s.acc += s.beta * K * track_value(tr);
+      } else {
+        add_nee(s.rgb, L, ns, s.beta, track_value(tr));
+      }
     }
     const float u = s.ps.next(tab, hp);
     const float vv = s.ps.next(tab, hp);
@@ -452,9 +457,12 @@ FLG_D bool vpd_step(const MeshGeom& g, const DiffDev& D, const LightsDev& L, con
   return false;
 }
 
-// Forward: the path's radiance and its tape bits.
+// Forward: the path's RGB radiance and its tape bits.
 FLG_D void vpd_finish(const DiffDev& D, uint32_t p, const DiffVPath& s, float* lpath) {
-  lpath[p] = s.acc;
+  float* o = lpath + 3 * (int64_t)p;
+  o[0] = s.rgb.x;
+  o[1] = s.rgb.y;
+  o[2] = s.rgb.z;
   if (D.tape) {
     D.mask_lo[p] = (uint32_t)s.mask;
     D.mask_hi[p] = (uint32_t)(s.mask >> 32);
@@ -475,7 +483,7 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff(TraceGeom g, 
   float ga = 0.0f;
   if (p < npaths) {
     DiffVPath s;
-    bool done = vpd_begin<ADJ>(D, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, s);
+    bool done = vpd_begin<ADJ>(D, L, tab, hp, cam, spp, (uint32_t)p, lpath, dl_drgb, s);
     while (!done) done = vpd_step<ADJ, COMPAT>(mg, D, L, tab, hp, max_depth, grad, &ga, s);
     if (!ADJ) vpd_finish(D, (uint32_t)p, s, lpath);
   }
@@ -498,7 +506,7 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_vpdiff_regen(TraceGe
   regen_loop<DiffVPath>(
       npaths, next_path,
       [&](uint32_t q, DiffVPath& s) {
-        return vpd_begin<ADJ>(D, tab, hp, cam, spp, tile_order(q, spp, cam.w, cam.h), lpath, dl_drgb, s);
+        return vpd_begin<ADJ>(D, L, tab, hp, cam, spp, tile_order(q, spp, cam.w, cam.h), lpath, dl_drgb, s);
       },
       [&](DiffVPath& s) { return vpd_step<ADJ, COMPAT>(mg, D, L, tab, hp, max_depth, grad, &ga, s); },
       [&](uint32_t q, const DiffVPath& s) {
@@ -596,15 +604,15 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     // the pixel-centre path); with spp == 1 nothing reaches the film
     const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
     const int64_t ntasks = npix * tasks;
-    // [256 B counters][ntasks radiance floats][pad to 8 B][ntasks film positions]
-    char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 12 : 4) + 264, &st);
+    // [256 B counters][ntasks RGB radiances][pad to 8 B][ntasks film positions]
+    char* sc = (char*)scratch(ctx, "volpath_mesh", (size_t)ntasks * (film ? 20 : 12) + 264, &st);
     if (!sc) return st;
     // nl[0]: lookups, nl[1]: the path counter, nl[2..4]: rays, node visits,
     // triangle tests (cleared together)
     unsigned long long* nl = (unsigned long long*)sc;
     unsigned int* next_path = (unsigned int*)(nl + 1);
     float* lpath = (float*)(sc + 256);
-    float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 4 + ((ntasks & 1) ? 4 : 0)) : nullptr;
+    float2* fpos = film ? (float2*)(sc + 256 + (size_t)ntasks * 12 + ((ntasks & 1) ? 4 : 0)) : nullptr;
     HaltonTables tab;
     if ((st = get_halton_tables(ctx, &tab))) return st;
     LightsDev L;
@@ -637,7 +645,7 @@ int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* vol, c
     if (film)
       film_resolve(ctx->stream, ntasks > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
     else
-      k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+      pixel_mean(ctx->stream, lpath, npix, cfg->spp, d_rgb);
     FLG_CHECK_LAUNCH(ctx);
     if (h_stats) {
       unsigned long long c[5] = {0, 0, 0, 0, 0};
@@ -670,7 +678,7 @@ int flg_volpath_mesh_diff_forward(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* 
     const int64_t npix = (int64_t)cfg->width * cfg->height, npaths = npix * cfg->spp;
     float* lpath = d_lpath;
     if (!lpath) {
-      lpath = (float*)scratch(ctx, "vpdiff_lpath", (size_t)npaths * 4, &st);
+      lpath = (float*)scratch(ctx, "vpdiff_lpath", (size_t)npaths * 12, &st);
       if (!lpath) return st;
     }
     const TraceGeom g = geom_of(bvh);
@@ -680,7 +688,7 @@ int flg_volpath_mesh_diff_forward(flg_ctx* ctx, const flg_bvh* bvh, flg_volume* 
                ? launch_vpdiff<false, true>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr)
                : launch_vpdiff<false, false>(ctx, cfg, g, D, L, tab, hp, cam, lpath, nullptr, nullptr, nullptr);
       if (st) return st;
-      if (d_rgb) k_vp_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+      if (d_rgb) pixel_mean(ctx->stream, lpath, npix, cfg->spp, d_rgb);
     }
     FLG_CHECK_LAUNCH(ctx);
     return FLG_OK;

@@ -76,8 +76,11 @@ void env_sphere(const float* bounds, float* center, float* radius) {
 
 // InfiniteAreaLight(texture) with a ConstTexture (light.cpp:93-108) and
 // preprocess (light.cpp:110-117) over the scene bound (= the volume box).
-int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
-  if (vol->env && vol->env_radiance == radiance) return FLG_OK;
+int build_env(flg_ctx* ctx, flg_volume* vol, V3 radiance) {
+  // |texture| of a constant colour (Vector3f::norm: x*x + y*y + z*z, then
+  // the square root); the tables depend on the colour through it only
+  const float nrm = sqrtf(radiance.x * radiance.x + radiance.y * radiance.y + radiance.z * radiance.z);
+  if (vol->env && vol->env_norm == nrm) return FLG_OK;
   const int NU = kEnvN, NV = kEnvN;
   std::vector<float> host((size_t)513 + 512 + 512 * 513 + 512 * 512 + 512, 0.0f);
   float* marg_cdf = host.data();
@@ -85,7 +88,6 @@ int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
   float* row_cdf = marg_f + 512;
   float* row_f = row_cdf + 512 * 513;
   float* row_int = row_f + 512 * 512;
-  float nrm = sqrtf(radiance * radiance + radiance * radiance + radiance * radiance);
   for (int v = 0; v < NV; ++v) {
     float sin_theta = sinf(PI * (float)(v + 0.5f) / (float)NV);
     for (int u = 0; u < NU; ++u) {
@@ -101,7 +103,7 @@ int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
   if (!vol->env) FLG_CHECK_HIP(ctx, hipMalloc(&vol->env, host.size() * 4));
   FLG_CHECK_HIP(ctx, hipMemcpyAsync(vol->env, host.data(), host.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   FLG_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  vol->env_radiance = radiance;
+  vol->env_norm = nrm;
   vol->env_marg_int = marg_int;
   env_sphere(vol->bounds, vol->env_center, &vol->env_radius);
   return FLG_OK;
@@ -115,7 +117,8 @@ int build_env(flg_ctx* ctx, flg_volume* vol, float radiance) {
 struct SvolPath {
   PathSampler ps;
   V3 o, d;
-  float Lr, beta;
+  V3 Lr;  // RGB
+  float beta;
   int bounces;
   // film mode: the pixel, the film position of the splatted sample, and
   // whether the path in flight is the pixel-centre path (not splatted)
@@ -132,11 +135,11 @@ struct SvolPath {
 FLG_D bool svol_enter(const VolDev& v, const LightsDev& L, const CamDevV& cam, float fx, float fy, SvolPath& s) {
   V3 o, d;
   gen_ray_v(cam, fx, fy, &o, &d);
-  s.Lr = 0.0f;
+  s.Lr = v3(0.0f, 0.0f, 0.0f);
   s.beta = 1.0f;
   float t_min, t_max;
   if (!box_enter(v.bmin, v.bmax, o, d, &t_min, &t_max)) {
-    if (L.env.present) s.Lr += s.beta * L.env.radiance;
+    if (L.env.present) add_scaled(s.Lr, s.beta, L.env.radiance);
     return true;
   }
   s.o = o + d * t_min;
@@ -147,7 +150,7 @@ FLG_D bool svol_enter(const VolDev& v, const LightsDev& L, const CamDevV& cam, f
 
 // The primary transmittance tr's contribution.
 FLG_D void svol_primary(const LightsDev& L, float tr, SvolPath& s) {
-  if (L.env.present) s.Lr += tr * L.env.radiance;
+  if (L.env.present) add_scaled(s.Lr, tr, L.env.radiance);
 }
 
 // Starts a path at film position (fx, fy) with the sampler state in s.ps:
@@ -245,8 +248,8 @@ FLG_D bool svol_collide(const VolDev& v, const LightsDev& L, const HaltonTables&
 // The second half, given the light estimate ld: radiance, the depth and
 // throughput tests, Russian roulette after bounce 10.  Returns true when the
 // path is complete.
-FLG_D bool svol_continue(const HaltonTables& tab, const HaltonParams& hp, int max_depth, float ld, SvolPath& s) {
-  s.Lr += s.beta * ld;
+FLG_D bool svol_continue(const HaltonTables& tab, const HaltonParams& hp, int max_depth, V3 ld, SvolPath& s) {
+  add_scaled(s.Lr, s.beta, ld);
   if (s.bounces >= max_depth || s.beta * s.beta + s.beta * s.beta + s.beta * s.beta == 0.0f) return true;
   if (s.bounces > 10) {
     const float rr = 0.01f;
@@ -265,13 +268,13 @@ FLG_D bool svol_step(const VolDev& v, const LightsDev& L, const HaltonTables& ta
                      const HaltonParams& hp, int max_depth, SvolPath& s, uint32_t& nlook) {
   LightSample ls;
   if (!svol_collide<COUNT>(v, L, tab, hp, s, ls, nlook)) return true;
-  float ld;
+  V3 ld;
   if (ls.kind == 2) {
     V3 d;
     float tmax;
     light_ray(ls, s.o, &d, &tmax);
     HaltonDraw rng{s.ps, tab, hp};
-    ld = light_post(ls, vol_tr<COUNT>(v, rng, s.o, d, tmax, nlook));
+    ld = light_post(L, ls, vol_tr<COUNT>(v, rng, s.o, d, tmax, nlook));
   } else {
     ld = light_unshadowed(ls);
   }
@@ -297,10 +300,18 @@ FLG_D void count_lookups(uint32_t nlook, unsigned long long* nlookups) {
   }
 }
 
+// Radiance of path sample p: three floats at 3 p.
+FLG_D void put_rgb(float* lpath, uint32_t p, V3 L) {
+  float* o = lpath + 3 * (size_t)p;
+  o[0] = L.x;
+  o[1] = L.y;
+  o[2] = L.z;
+}
+
 // Per-sample outputs: radiance, and in film mode the film position.
 template <bool FILM>
 FLG_D void svol_finish(uint32_t p, const SvolPath& s, float* lpath, float2* fpos) {
-  lpath[p] = s.Lr;
+  put_rgb(lpath, p, s.Lr);
   if (FILM) fpos[p] = make_float2(s.fx, s.fy);
 }
 
@@ -382,23 +393,25 @@ __global__ __launch_bounds__(256, 5) void k_svol_regen(VolDev v, LightsDev L, Ha
 // and the whole sequence can be captured once as a hipGraph.
 enum : uint32_t { kWfCamera = 0, kWfShadow = 1, kWfUnshadowed = 2 };
 
-// A queue of path states, one 80-byte record per entry in five arrays, in
+// A queue of path states, one 88-byte record per entry in six arrays, in
 // queue order: a stage reads its input queue's records at consecutive
 // positions and writes its output records at the positions it claims, so
 // both are coalesced.
 struct WfQueue {
-  float4* od;   // o.xyz, Lr
+  float4* od;   // o.xyz, Lr.x
   float4* dir;  // d.xyz, beta
-  float4* lw;   // pending light: f, Li, pdf, light_pdf (kWfShadow); estimate in .x (kWfUnshadowed)
+  float4* lw;   // pending light: f, d2, pdf, light_pdf (kWfShadow); estimate in .x (kWfUnshadowed)
   float4* sh;   // shadow ray: direction, tmax (kWfShadow)
   uint4* meta;  // Halton index, next dimension, bounces | kind << 16 | centre << 24, task
+  float2* gb;   // Lr.y, Lr.z
 };
 
 FLG_D void wf_load(const WfQueue& Q, uint32_t i, SvolPath& s, uint32_t* kind, uint32_t* task) {
   const float4 a = Q.od[i], b = Q.dir[i];
   const uint4 m = Q.meta[i];
+  const float2 c = Q.gb[i];
   s.o = v3(a.x, a.y, a.z);
-  s.Lr = a.w;
+  s.Lr = v3(a.w, c.x, c.y);
   s.d = v3(b.x, b.y, b.z);
   s.beta = b.w;
   s.ps.index = m.x;
@@ -410,7 +423,8 @@ FLG_D void wf_load(const WfQueue& Q, uint32_t i, SvolPath& s, uint32_t* kind, ui
 }
 
 FLG_D void wf_store(const WfQueue& Q, uint32_t i, const SvolPath& s, uint32_t kind, uint32_t task) {
-  Q.od[i] = make_float4(s.o.x, s.o.y, s.o.z, s.Lr);
+  Q.od[i] = make_float4(s.o.x, s.o.y, s.o.z, s.Lr.x);
+  Q.gb[i] = make_float2(s.Lr.y, s.Lr.z);
   Q.dir[i] = make_float4(s.d.x, s.d.y, s.d.z, s.beta);
   Q.meta[i] = make_uint4(s.ps.index, s.ps.dim, (uint32_t)s.bounces | (kind << 16) | ((s.centre ? 1u : 0u) << 24),
                          task);
@@ -444,7 +458,7 @@ FLG_D bool wf_complete(const VolDev& v, const LightsDev& L, const HaltonTables& 
     fpos[p] = make_float2(s.fx, s.fy);
     if (!svol_enter(v, L, cam, s.fx, s.fy, s)) return true;
   }
-  lpath[p] = s.Lr;
+  put_rgb(lpath, p, s.Lr);
   return false;
 }
 
@@ -510,18 +524,19 @@ __global__ __launch_bounds__(256) void k_wf_walk(VolDev v, LightsDev L, HaltonTa
           svol_primary(L, vol_tr<COUNT>(v, rng, cam.pos, s.d, kInf, nlook), s);
           push = true;
         } else {
-          float ld;
+          V3 ld;
           if (kind == kWfShadow) {
             const float4 a = in.lw[i], b = in.sh[i];
             LightSample ls;
             ls.f = a.x;
-            ls.Li = a.y;
+            ls.d2 = a.y;
             ls.pdf = a.z;
             ls.light_pdf = a.w;
             HaltonDraw rng{s.ps, tab, hp};
-            ld = light_post(ls, vol_tr<COUNT>(v, rng, s.o, v3(b.x, b.y, b.z), b.w, nlook));
+            ld = light_post(L, ls, vol_tr<COUNT>(v, rng, s.o, v3(b.x, b.y, b.z), b.w, nlook));
           } else {
-            ld = in.lw[i].x;
+            const float z = in.lw[i].x;
+            ld = v3(z, z, z);
           }
           if (!svol_continue(tab, hp, max_depth, ld, s)) {
             push = true;
@@ -565,7 +580,7 @@ __global__ __launch_bounds__(256) void k_wf_collide(VolDev v, LightsDev L, Halto
           light_ray(ls, s.o, &shd, &shmax);
           kind = kWfShadow;
         } else {
-          ls.f = light_unshadowed(ls);
+          ls.f = light_unshadowed(ls).x;  // the same value in every channel
           kind = kWfUnshadowed;
         }
       } else if (wf_complete<FILM>(v, L, tab, hp, cam, tasks, p, s, lpath, fpos)) {
@@ -577,7 +592,7 @@ __global__ __launch_bounds__(256) void k_wf_collide(VolDev v, LightsDev L, Halto
     if (push) {
       wf_store(out, at, s, kind, p);
       if (kind == kWfShadow) {
-        out.lw[at] = make_float4(ls.f, ls.Li, ls.pdf, ls.light_pdf);
+        out.lw[at] = make_float4(ls.f, ls.d2, ls.pdf, ls.light_pdf);
         out.sh[at] = make_float4(shd.x, shd.y, shd.z, shmax);
       } else if (kind == kWfUnshadowed) {
         out.lw[at].x = ls.f;
@@ -591,17 +606,32 @@ __global__ void k_wf_zero(uint32_t* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
+// Pixel = the in-order float mean of its spp samples, per channel.  A block
+// takes kMeanPix pixels; their samples are staged through LDS kMeanTile at a
+// time, read as the contiguous rows they are in HBM (each pixel's samples are
+// 3 spp consecutive floats), and thread 3 q + c then adds channel c of pixel
+// q in sample order (the order of the per-sample schedule's mean).  The odd
+// row pitch spreads the summing threads of a wave over the LDS banks.
+constexpr int kMeanPix = 64, kMeanTile = 16, kMeanRow = 3 * kMeanTile + 1;
 __global__ __launch_bounds__(256) void k_pixel_mean(const float* lpath, int64_t npix, int32_t spp,
                                                     float* rgb) {
-  int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= npix) return;
-  float c = 0.0f;
-  const float* l = lpath + pix * spp;
-  for (int s = 0; s < spp; ++s) c += l[s];
-  c = c / (float)spp;
-  rgb[3 * pix] = c;
-  rgb[3 * pix + 1] = c;
-  rgb[3 * pix + 2] = c;
+  __shared__ float tile[kMeanPix * kMeanRow];
+  const int64_t pix0 = (int64_t)blockIdx.x * kMeanPix;
+  const int npx = (int)(npix - pix0 < kMeanPix ? npix - pix0 : kMeanPix);
+  const int t = threadIdx.x, q = t / 3, c = t - 3 * q;
+  float acc = 0.0f;
+  for (int s0 = 0; s0 < spp; s0 += kMeanTile) {
+    const int ns = spp - s0 < kMeanTile ? spp - s0 : kMeanTile;
+    __syncthreads();
+    for (int e = t; e < npx * 3 * ns; e += blockDim.x) {
+      const int r = e / (3 * ns), o = e - r * 3 * ns;
+      tile[r * kMeanRow + o] = lpath[((pix0 + r) * spp + s0) * 3 + o];
+    }
+    __syncthreads();
+    if (q < npx)
+      for (int k = 0; k < ns; ++k) acc += tile[q * kMeanRow + 3 * k + c];
+  }
+  if (q < npx) rgb[3 * (pix0 + q) + c] = acc / (float)spp;
 }
 
 // MitchellFilter::mitchell1D (common/filter.h:44-53) of the film's filter
@@ -630,7 +660,8 @@ __global__ __launch_bounds__(256) void k_film_resolve(const float* lpath, const 
   const int64_t pix = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (pix >= (int64_t)w * h) return;
   const int x = (int)(pix % w), y = (int)(pix / w);
-  float I = 0.0f, Ls = 0.0f;
+  float I = 0.0f;
+  V3 Ls = v3(0.0f, 0.0f, 0.0f);
   for (int qx = x - 1; qx <= x + 1; ++qx) {
     if (qx < 0 || qx >= w) continue;
     for (int qy = y - 1; qy <= y + 1; ++qy) {
@@ -640,14 +671,16 @@ __global__ __launch_bounds__(256) void k_film_resolve(const float* lpath, const 
         const float2 uv = fpos[base + t];
         const float wgt = film_mitchell1d((uv.x - 0.5f) - (float)x) * film_mitchell1d((uv.y - 0.5f) - (float)y);
         I = I + wgt;
-        Ls = Ls + lpath[base + t] * wgt;
+        const float* l = lpath + 3 * (base + t);
+        Ls.x = Ls.x + l[0] * wgt;
+        Ls.y = Ls.y + l[1] * wgt;
+        Ls.z = Ls.z + l[2] * wgt;
       }
     }
   }
-  const float c = I == 0.0f ? 0.0f : Ls / I;
-  rgb[3 * pix] = c;
-  rgb[3 * pix + 1] = c;
-  rgb[3 * pix + 2] = c;
+  rgb[3 * pix] = I == 0.0f ? 0.0f : Ls.x / I;
+  rgb[3 * pix + 1] = I == 0.0f ? 0.0f : Ls.y / I;
+  rgb[3 * pix + 2] = I == 0.0f ? 0.0f : Ls.z / I;
 }
 
 __global__ __launch_bounds__(256) void k_fill(float* p, int64_t n, float v) {
@@ -752,6 +785,10 @@ __global__ __launch_bounds__(256) void k_leaf_max(const int32_t* origin, const f
 
 }  // namespace
 
+void pixel_mean(hipStream_t st, const float* lpath, int64_t npix, int32_t spp, float* rgb) {
+  if (npix > 0) k_pixel_mean<<<div_up(npix, kMeanPix), 256, 0, st>>>(lpath, npix, spp, rgb);
+}
+
 void film_resolve(hipStream_t st, const float* lpath, const float2* fpos, int32_t w, int32_t h, int32_t tasks,
                   float* rgb) {
   const int64_t npix = (int64_t)w * h;
@@ -771,15 +808,16 @@ int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDe
   *L = LightsDev{};
   L->has_point = sc->has_point != 0;
   L->point_pos = v3(sc->point_pos[0], sc->point_pos[1], sc->point_pos[2]);
-  L->point_intensity = sc->point_intensity;
+  L->point_intensity = v3(sc->point_intensity[0], sc->point_intensity[1], sc->point_intensity[2]);
   L->env.present = sc->has_env != 0;
   if (L->env.present) {
-    if (!may_build && (!vol->env || vol->env_radiance != sc->env_radiance))
+    const V3 rad = v3(sc->env_radiance[0], sc->env_radiance[1], sc->env_radiance[2]);
+    const float nrm = sqrtf(rad.x * rad.x + rad.y * rad.y + rad.z * rad.z);
+    if (!may_build && (!vol->env || vol->env_norm != nrm))
       return set_error(ctx, FLG_ERR_INVALID,
-                       "the volume's environment-light table is not built for radiance " +
-                           std::to_string(sc->env_radiance) +
-                           ": render the scene once with flg_volpath_render first");
-    int st = build_env(ctx, vol, sc->env_radiance);
+                       "the volume's environment-light table is not built for a radiance of norm " +
+                           std::to_string(nrm) + ": render the scene once with flg_volpath_render first");
+    int st = build_env(ctx, vol, rad);
     if (st) return st;
     const float* e = vol->env;
     L->env.marg_cdf = e;
@@ -788,7 +826,7 @@ int make_lights(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc, LightsDe
     L->env.row_f = e + 513 + 512 + 512 * 513;
     L->env.row_int = e + 513 + 512 + 512 * 513 + 512 * 512;
     L->env.marg_int = vol->env_marg_int;
-    L->env.radiance = sc->env_radiance;
+    L->env.radiance = rad;
     L->env.center = v3(vol->env_center[0], vol->env_center[1], vol->env_center[2]);
     L->env.radius = vol->env_radius;
     if (scene_bounds) {
@@ -852,7 +890,7 @@ static bool sparse_covers_all(const flg_volume* vol, unsigned long long covered)
 
 // ------------------------------------------------------ frame render ---
 // The per-frame work of flg_volpath_render and flg_volpath_render_noalloc.
-// Scratch: [256 B counters][npaths radiance floats][pad to 8 B][npaths film
+// Scratch: [256 B counters][npaths RGB radiances][pad to 8 B][npaths film
 // positions], from the context's grow-only pool, or (noalloc) from the
 // library's fixed per-GPU array, in which case nothing is allocated: the
 // Halton tables come from flg_ctx_create and the environment table must
@@ -881,7 +919,7 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
   // the pixel-centre path); with spp == 1 nothing reaches the film
   const int32_t tasks = film ? cfg->spp - 1 : cfg->spp;
   const int64_t npaths = npix * tasks;
-  const size_t need = (size_t)npaths * (film ? 12 : 4) + 264;
+  const size_t need = (size_t)npaths * (film ? 20 : 12) + 264;
   int st = FLG_OK;
   char* sc = nullptr;
   if (noalloc) {
@@ -889,8 +927,9 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
       return set_error(ctx, FLG_ERR_RANGE,
                        std::string(what) + ": the frame needs " + std::to_string(need) +
                            " bytes of scratch, more than the fixed " + std::to_string(kNoAllocScratchBytes) +
-                           " (at most " + std::to_string((kNoAllocScratchBytes - 264) / 4) +
-                           " paths, or a third of that with film = 1)");
+                           " (at most " + std::to_string((kNoAllocScratchBytes - 264) / 12) +
+                           " paths, or " + std::to_string((kNoAllocScratchBytes - 264) / 20) +
+                           " tasks with film = 1)");
     sc = (char*)ctx->noalloc_scratch;
     if (!sc) return set_error(ctx, FLG_ERR_HIP, std::string(what) + ": no device address for the fixed scratch");
   } else {
@@ -900,7 +939,7 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
   unsigned long long* nl = (unsigned long long*)sc;
   unsigned int* next_path = (unsigned int*)(nl + 1);  // cleared with the lookup count
   float* lpath = (float*)(sc + 256);
-  float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 4 + ((npaths & 1) ? 4 : 0)) : nullptr;
+  float2* fpos = film ? (float2*)(sc + 256 + (size_t)npaths * 12 + ((npaths & 1) ? 4 : 0)) : nullptr;
   const bool count = h_stats != nullptr;
   HaltonTables tab;
   LightsDev L;
@@ -917,19 +956,19 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
   uint32_t* wf_counts = nullptr;
   int wf_ncounts = 0;
   if (npaths > 0 && sched == 3) {
-    // [two queues of npaths 80-byte records, five arrays each][stage counts]
+    // [two queues of npaths 88-byte records, six arrays each][stage counts]
     const int pairs = film ? 2 * cfg->max_depth + 1 : cfg->max_depth + 1;
     wf_ncounts = 2 * pairs + 2;
-    const size_t rec = (size_t)npaths * 16;
-    char* w = (char*)scratch(ctx, "vol_wavefront", 10 * rec + (size_t)wf_ncounts * 4, &st);
+    const size_t rec = (size_t)npaths * 16, qbytes = 5 * rec + (size_t)npaths * 8;
+    char* w = (char*)scratch(ctx, "vol_wavefront", 2 * qbytes + (size_t)wf_ncounts * 4, &st);
     if (!w) return st;
     auto queue = [&](int k) {
-      char* b = w + (size_t)k * 5 * rec;
+      char* b = w + (size_t)k * qbytes;
       return WfQueue{(float4*)b, (float4*)(b + rec), (float4*)(b + 2 * rec), (float4*)(b + 3 * rec),
-                     (uint4*)(b + 4 * rec)};
+                     (uint4*)(b + 4 * rec), (float2*)(b + 5 * rec)};
     };
     const WfQueue q0 = queue(0), q1 = queue(1);
-    wf_counts = (uint32_t*)(w + 10 * rec);
+    wf_counts = (uint32_t*)(w + 2 * qbytes);
     const VolDev v = make_voldev(vol);
     const CamDevV cam = make_cam_v(&scene->camera, cfg->width, cfg->height);
     const int grid = (int)std::min<int64_t>(div_up(npaths, 256), (int64_t)ctx->num_cus * 8);
@@ -1006,7 +1045,7 @@ static int volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* sc
     if (film)
       film_resolve(ctx->stream, npaths > 0 ? lpath : nullptr, fpos, cfg->width, cfg->height, tasks, d_rgb);
     else
-      k_pixel_mean<<<div_up(npix, 256), 256, 0, ctx->stream>>>(lpath, npix, cfg->spp, d_rgb);
+      pixel_mean(ctx->stream, lpath, npix, cfg->spp, d_rgb);
   }
   FLG_CHECK_LAUNCH(ctx);
   if (h_stats) {

@@ -17,9 +17,10 @@ struct flg_volume {
   int32_t* leaf_origin = nullptr;
   int32_t n_leaves = 0;
   bool sparse_covers_all = false;  // the leaves cover every voxel of the grid
-  // environment-light tables, rebuilt when the radiance changes
+  // environment-light tables, rebuilt when the norm of the radiance changes
+  // (the only way the colour enters them)
   float* env = nullptr;
-  float env_radiance = -1.0f;
+  float env_norm = -1.0f;
   float env_marg_int = 0;
   float env_center[3] = {0, 0, 0};
   float env_radius = 0;
@@ -409,7 +410,7 @@ struct EnvDev {
   const float* row_f;     // 512 x 512
   const float* row_int;   // 512
   float marg_int;
-  float radiance;
+  V3 radiance;  // RGB
   V3 center;
   float radius;
   int32_t present;
@@ -431,9 +432,9 @@ FLG_D float dist_sample(const float* cdf, const float* f, float integral, int n,
   return ((float)l + du) / (float)n;
 }
 
-// InfiniteAreaLight::sampleLi (light.cpp:119-145) with a constant texture.
-FLG_D float env_sample_li(const EnvDev& e, V3 ref, float u0, float u1, V3* wi, float* pdf,
-                          V3* p_light) {
+// InfiniteAreaLight::sampleLi (light.cpp:119-145) with a constant texture,
+// up to its radiance (the texture's colour, the same in every direction).
+FLG_D void env_sample_dir(const EnvDev& e, V3 ref, float u0, float u1, V3* wi, float* pdf, V3* p_light) {
   float p1, p0;
   int o1, o0;
   float d1 = dist_sample(e.marg_cdf, e.marg_f, e.marg_int, 512, u1, &p1, &o1);
@@ -450,29 +451,38 @@ FLG_D float env_sample_li(const EnvDev& e, V3 ref, float u0, float u1, V3* wi, f
   V3 p = e.center + (dir * 2.0f) * e.radius;
   *p_light = p;
   *wi = normalize(p - ref);
-  return e.radiance;
 }
 
 struct LightsDev {
   EnvDev env;
   int32_t has_point;
   V3 point_pos;
-  float point_intensity;
+  V3 point_intensity;  // RGB
 };
 
 // UniformSampleOneLight + EstimateDirect for a volume vertex
 // (integrator.cpp:69-120): lights in the order [point, environment], one
 // picked with p = 1/n, u_light drawn before sampleLi, the HG phase as f.
-// light_pre runs it up to the shadow ray: the light
-// choice and its sample.  kind 0: no light (the estimate is 0); 1: no shadow
-// ray (pdf, Li or the phase function is 0; the estimate is 0 / light_pdf);
-// 2: the estimate needs the transmittance along the shadow ray from p
-// towards p_light (light_post).
+// Radiance is RGB and every channel runs the reference's Vector3f arithmetic
+// (vector.inc: component-wise products and quotients).  light_pre runs it up
+// to the shadow ray: the light choice and its sample.  kind 0: no light (the
+// estimate is 0); 1: no shadow ray (pdf or the phase function is 0, or Li is
+// 0 in every channel; the estimate is 0 / light_pdf); 2: the estimate needs
+// the transmittance along the shadow ray from p towards p_light (light_post).
+// The sample keeps the point light's squared distance rather than Li: Li is
+// I / d2 per channel (or the environment's colour), recomputed where it is
+// used, so that a path carries one float for it instead of three.
 struct LightSample {
   V3 p_light;
-  float f, Li, pdf, light_pdf;
+  float f, d2, pdf, light_pdf;  // d2 < 0: the environment light
   int kind;
 };
+
+// Li of the sampled light, per channel.
+FLG_D V3 light_li(const LightsDev& L, float d2) {
+  if (d2 < 0.0f) return L.env.radiance;
+  return v3(L.point_intensity.x / d2, L.point_intensity.y / d2, L.point_intensity.z / d2);
+}
 
 FLG_D LightSample light_pre(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
                             PathSampler& ps, V3 p, V3 wo) {
@@ -489,23 +499,27 @@ FLG_D LightSample light_pre(const LightsDev& L, float g, const HaltonTables& tab
   V3 wi;
   if (use_point) {
     V3 diff = L.point_pos - p;
-    float d2 = sqnorm(diff);
+    ls.d2 = sqnorm(diff);
     wi = normalize(diff);
-    ls.Li = L.point_intensity / d2;
     ls.pdf = 1.0f;
     ls.p_light = L.point_pos;
   } else {
-    ls.Li = env_sample_li(L.env, p, u0, u1, &wi, &ls.pdf, &ls.p_light);
+    env_sample_dir(L.env, p, u0, u1, &wi, &ls.pdf, &ls.p_light);
+    ls.d2 = -1.0f;
   }
   ls.kind = 1;
-  if (ls.pdf == 0.0f || ls.Li == 0.0f) return ls;
+  const V3 Li = light_li(L, ls.d2);
+  if (ls.pdf == 0.0f || (Li.x == 0.0f && Li.y == 0.0f && Li.z == 0.0f)) return ls;
   ls.f = hg_p(wi, wo, g);
   if (ls.f != 0.0f) ls.kind = 2;
   return ls;
 }
 
 // The estimate without a shadow ray (kinds 0 and 1).
-FLG_D float light_unshadowed(const LightSample& ls) { return ls.kind == 0 ? 0.0f : 0.0f / ls.light_pdf; }
+FLG_D V3 light_unshadowed(const LightSample& ls) {
+  const float z = ls.kind == 0 ? 0.0f : 0.0f / ls.light_pdf;
+  return v3(z, z, z);
+}
 
 // The shadow ray of a kind-2 sample from p: direction and length.
 FLG_D void light_ray(const LightSample& ls, V3 p, V3* d, float* tmax) {
@@ -515,27 +529,40 @@ FLG_D void light_ray(const LightSample& ls, V3 p, V3* d, float* tmax) {
   *tmax = nrm - SHADOW_EPS;
 }
 
-// The estimate of a kind-2 sample given the shadow ray's transmittance.
-FLG_D float light_post(const LightSample& ls, float tr) {
-  float Li = ls.Li;
+// The estimate of a kind-2 sample given the shadow ray's transmittance:
+// Li *= tr; L += f * Li / pdf; L / light_pdf, per channel.
+FLG_D float light_post1(float Li, float f, float pdf, float light_pdf, float tr) {
   Li *= tr;
   float out = 0.0f;
-  out += ls.f * Li / ls.pdf;
-  return out / ls.light_pdf;
+  out += f * Li / pdf;
+  return out / light_pdf;
+}
+
+FLG_D V3 light_post(const LightsDev& L, const LightSample& ls, float tr) {
+  const V3 Li = light_li(L, ls.d2);
+  return v3(light_post1(Li.x, ls.f, ls.pdf, ls.light_pdf, tr), light_post1(Li.y, ls.f, ls.pdf, ls.light_pdf, tr),
+            light_post1(Li.z, ls.f, ls.pdf, ls.light_pdf, tr));
+}
+
+// L += beta * ld, per channel.
+FLG_D void add_scaled(V3& L, float beta, V3 ld) {
+  L.x += beta * ld.x;
+  L.y += beta * ld.y;
+  L.z += beta * ld.z;
 }
 
 // The whole estimate; `shadow(p, d, tmax)` is the transmittance along the
 // shadow ray: ratio tracking in the medium (SVolIntegrator) or VolEstimateTr
 // through interfaces (VolPathIntegrator).
 template <class Shadow>
-FLG_D float sample_one_light(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
-                             PathSampler& ps, V3 p, V3 wo, Shadow shadow) {
+FLG_D V3 sample_one_light(const LightsDev& L, float g, const HaltonTables& tab, const HaltonParams& hp,
+                          PathSampler& ps, V3 p, V3 wo, Shadow shadow) {
   const LightSample ls = light_pre(L, g, tab, hp, ps, p, wo);
   if (ls.kind != 2) return light_unshadowed(ls);
   V3 d;
   float tmax;
   light_ray(ls, p, &d, &tmax);
-  return light_post(ls, shadow(p, d, tmax));
+  return light_post(L, ls, shadow(p, d, tmax));
 }
 
 // Camera (src/common/camera.h:11-31): ctor on the host in float, per-ray
@@ -571,8 +598,12 @@ inline CamDevV make_cam_v(const flg_camera* c, int w, int h) {
 }
 
 VolDev make_voldev(const flg_volume* vol);
-// The film of the reference's render loop from the film tasks' radiance and
-// film positions (tasks per pixel, pixel-major), into rgb (w*h*3); lpath NULL
+// Pixel = the in-order float mean of its spp RGB path samples (three floats
+// per sample, pixel-major), per channel, into rgb (npix*3).
+void pixel_mean(hipStream_t st, const float* lpath, int64_t npix, int32_t spp, float* rgb);
+// The film of the reference's render loop from the film tasks' RGB radiance
+// (three floats per task) and film positions (tasks per pixel, pixel-major),
+// into rgb (w*h*3); lpath NULL
 // or tasks 0: nothing reached the film, rgb is 0.
 void film_resolve(hipStream_t st, const float* lpath, const float2* fpos, int32_t w, int32_t h, int32_t tasks,
                   float* rgb);

@@ -15,14 +15,14 @@ from .api import Volume, _ptr, check_buffer, check_device
 
 def _check_forward(ctx, w, h, spp, depth, out, lpath, tape):
     check_buffer(out, "out", torch.float32, None, min_numel=w * h * 3)
-    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp)
+    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp * 3)
     if tape is not None:
         check_buffer(tape, "tape", torch.uint8, None, min_numel=tape_bytes(ctx, w, h, spp, depth))
     check_device(ctx.device, out=out, lpath=lpath, tape=tape)
 
 
 def _check_adjoint(ctx, w, h, spp, depth, lpath, dl, grad, grad_numel, galpha, tape):
-    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp)
+    check_buffer(lpath, "lpath", torch.float32, None, min_numel=w * h * spp * 3)
     check_buffer(dl, "dl_drgb", torch.float32, None, numel=w * h * 3)
     check_buffer(grad, "grad", torch.float32, None, numel=grad_numel)
     check_buffer(galpha, "galpha", torch.float32, None, min_numel=1)
@@ -107,7 +107,7 @@ def new_tape(ctx, w, h, spp, depth):
 
 def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=None, schedule=0,
             tape=None, lookups=None):
-    """Image and per-path radiance; with `tape` (a buffer from new_tape) also
+    """Image and per-path RGB radiance ((w*h*spp, 3)); with `tape` (a buffer from new_tape) also
     the walk tape that the adjoint of this call can read; with `lookups` (an
     int64 CUDA tensor) the call adds its grid lookups to lookups[0]."""
     ctx = dv.ctx
@@ -115,7 +115,7 @@ def forward(dv, scene, w, h, spp, depth, s, albedo, majorant, out=None, lpath=No
     if out is None:
         out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
     if lpath is None:
-        lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
+        lpath = torch.empty((w * h * spp, 3), dtype=torch.float32, device=dev)
     _check_forward(ctx, w, h, spp, depth, out, lpath, tape)
     _check_counter(ctx, lookups)
     prop = dv.proposal.h if dv.proposal is not None else None
@@ -224,7 +224,7 @@ def mesh_forward(dv, bvh, scene, w, h, spp, depth, s, albedo, majorant, out=None
     if out is None:
         out = torch.empty((h, w, 3), dtype=torch.float32, device=dev)
     if lpath is None:
-        lpath = torch.empty(w * h * spp, dtype=torch.float32, device=dev)
+        lpath = torch.empty((w * h * spp, 3), dtype=torch.float32, device=dev)
     _check_forward(ctx, w, h, spp, depth, out, lpath, tape)
     prop = dv.proposal.h if dv.proposal is not None else None
     ctx.check(ctx.lib.flg_volpath_mesh_diff_forward(ctx.h, bvh.h, dv.vol.h, prop, ctypes.byref(scene),
@@ -315,7 +315,7 @@ def bench_adjoint(ctx, timed, sync, state="uniform", profile=False):
     mu = S * float(theta.max())
     plain = Volume(ctx, theta, (0, 0, 0, 1, 1, 1), 0.8 * S, 0.2 * S, 0.3)
     img_plain = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
-    lpath = torch.empty(W * H * SPP, dtype=torch.float32, device=dev)
+    lpath = torch.empty((W * H * SPP, 3), dtype=torch.float32, device=dev)
     img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
     grad = torch.zeros((128, 128, 128), dtype=torch.float32, device=dev)
     galpha = torch.zeros(1, dtype=torch.float32, device=dev)

@@ -350,13 +350,18 @@ FLG_API int flg_volume_create_sparse(flg_ctx* ctx, int32_t n_leaves, const int32
  * this call a dense one. */
 FLG_API int flg_volume_update_sparse(flg_ctx* ctx, flg_volume* vol, const float* d_leaf_values);
 
+/* Lights of the volume scenes, RGB: every channel runs the reference's
+ * Vector3f arithmetic (component-wise), so a grey light gives three equal
+ * channels.  The point light's Li is I / d^2 with pdf 1; the environment is
+ * InfiniteAreaLight over a ConstTexture of colour env_radiance
+ * (light.cpp:93-159), whose sampling tables depend on the colour's norm. */
 typedef struct flg_vol_scene {
   flg_camera camera;
   int32_t has_point;
   float point_pos[3];
-  float point_intensity;
+  float point_intensity[3]; /* RGB radiant intensity */
   int32_t has_env;
-  float env_radiance; /* constant environment */
+  float env_radiance[3]; /* constant environment, RGB */
 } flg_vol_scene;
 
 typedef struct flg_render_cfg {
@@ -396,14 +401,14 @@ FLG_API int flg_volpath_render(flg_ctx* ctx, flg_volume* vol, const flg_vol_scen
  * reference; for callers that cannot allocate: code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * results as flg_volpath_render (no statistics).  Its scratch (path counters
- * and per-path radiance) is a fixed array of 64 MiB + 4 KiB per GPU, held for
- * the life of the process: frames of more than 16,778,174 paths (w*h*spp;
- * film = 1: more than 5,592,724 tasks, w*h*(spp-1)) are refused with
- * FLG_ERR_RANGE.
+ * and per-path RGB radiance) is a fixed array of 192 MiB + 4 KiB per GPU,
+ * held for the life of the process: frames of more than 16,777,535 paths
+ * (w*h*spp; film = 1: more than 10,066,521 tasks, w*h*(spp-1)) are refused
+ * with FLG_ERR_RANGE.
  * The context's Halton tables come from flg_ctx_create; a scene with an
  * environment light needs the volume's environment table built for the
- * scene's radiance by an earlier flg_volpath_render of that volume
- * (FLG_ERR_INVALID otherwise).  Every call on one GPU shares the array, so
+ * norm of the scene's radiance by an earlier flg_volpath_render of that
+ * volume (FLG_ERR_INVALID otherwise).  Every call on one GPU shares the array, so
  * calls that overlap on one GPU race: serialise them (one stream). */
 FLG_API int flg_volpath_render_noalloc(flg_ctx* ctx, flg_volume* vol, const flg_vol_scene* scene,
                                        const flg_render_cfg* cfg, float* d_rgb);
@@ -436,8 +441,8 @@ FLG_API int flg_volpath_mesh_render(flg_ctx* ctx, const flg_bvh* bvh, flg_volume
  * walk draws from a counter-based stream of its own (path, walk), so it can
  * stop once its result is fixed.  `proposal` (NULL: `vol` itself) supplies
  * the collision decisions.  d_rgb: width*height RGB (may be NULL); d_lpath:
- * per-path radiance, width*height*spp floats, path = pixel * spp + s (may be
- * NULL; the adjoint needs it); d_tape: walk tape of
+ * per-path RGB radiance, width*height*spp*3 floats, path = pixel * spp + s
+ * (may be NULL; the adjoint needs it); d_tape: walk tape of
  * flg_volpath_diff_tape_bytes(cfg) bytes (may be NULL), the measurement of
  * every ratio-tracking walk, which spares the adjoint one of its two walks
  * per ratio track (no reference counterpart). */
@@ -464,7 +469,9 @@ FLG_API int flg_volpath_diff_forward(flg_ctx* ctx, flg_volume* vol, const flg_vo
  * and d_tape when not NULL, from that call; with d_tape NULL every ratio
  * track is walked twice): given dLoss/dImage (RGB, d_dl_drgb) adds
  * dLoss/dDensity into d_grad (nx*ny*nz floats, x fastest) and dLoss/dAlbedo
- * into d_grad_albedo (1 float).  Accumulation uses float atomics, so the
+ * into d_grad_albedo (1 float).  The replay prices each light in the loss's
+ * units (sum over channels of dLoss/dL_c times the light's channel c), so
+ * one replay covers all three channels.  Accumulation uses float atomics, so the
  * summation order (not the terms) varies between runs. */
 FLG_API int flg_volpath_adjoint(flg_ctx* ctx, flg_volume* vol, const flg_volume* proposal,
                                 const flg_vol_scene* scene, const flg_render_cfg* cfg,

@@ -224,18 +224,22 @@ void orc_camera_rays(const float* cam10, int32_t W, int32_t H, int32_t spp, int3
 }
 
 // ---------------------------------------------------------------- volume ---
-// medium: sigma_s, sigma_a, g ; lights: point_present, px, py, pz, intensity,
-// env_present, env_radiance ; cam10 as above.
+// medium: sigma_s, sigma_a, g ; lights11: point_present, px, py, pz,
+// intensity (RGB), env_present, env_radiance (RGB) ; cam10 as above.
+static void set_lights(VolScene& sc, const float* lights11) {
+  sc.point.present = lights11[0] != 0.0f;
+  sc.point.pos = v3(lights11[1], lights11[2], lights11[3]);
+  sc.point.intensity = v3(lights11[4], lights11[5], lights11[6]);
+  if (lights11[7] != 0.0f) sc.env.build(v3(lights11[8], lights11[9], lights11[10]), sc.grid.bmin, sc.grid.bmax);
+}
+
 void orc_volpath_render(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
-                        const float* medium3, const float* lights7, const float* cam10, int32_t W,
+                        const float* medium3, const float* lights11, const float* cam10, int32_t W,
                         int32_t H, int32_t spp, int32_t max_depth, float* rgb, int nthreads,
                         uint64_t* lookups, uint64_t* paths) {
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
@@ -245,15 +249,12 @@ void orc_volpath_render(const float* grid, int32_t nx, int32_t ny, int32_t nz, c
 // orc_volpath_render over the pixels 0, stride, 2 stride, ... only (the CPU
 // baseline's bounded sample); the other pixels of rgb are left untouched.
 void orc_volpath_render_strided(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
-                                const float* medium3, const float* lights7, const float* cam10, int32_t W,
+                                const float* medium3, const float* lights11, const float* cam10, int32_t W,
                                 int32_t H, int32_t spp, int32_t max_depth, int64_t stride, float* rgb,
                                 int nthreads, uint64_t* lookups, uint64_t* paths) {
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
@@ -263,35 +264,29 @@ void orc_volpath_render_strided(const float* grid, int32_t nx, int32_t ny, int32
 
 // The reference's render loop and film (film.cpp); arguments as
 // orc_volpath_render.  task_L / task_uv (may be NULL): per pixel the spp - 1
-// splatted samples (radiance; film position x, y).
+// splatted samples (RGB radiance; film position x, y).
 void orc_volpath_render_film(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
-                             const float* medium3, const float* lights7, const float* cam10, int32_t W,
+                             const float* medium3, const float* lights11, const float* cam10, int32_t W,
                              int32_t H, int32_t spp, int32_t max_depth, float* rgb, float* task_L,
                              float* task_uv, uint64_t* lookups, uint64_t* paths) {
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
   svol_render_film(sc, rgb, task_L, task_uv, lookups, paths);
 }
 
-// Per-sample radiance and film position of the per-sample schedule (sample s
-// of pixel p on Halton index offset(p) + s * stride, dimensions 0 and 1 for
-// the film position): L[W*H*spp], uv[2*W*H*spp], pixel-major.
+// Per-sample RGB radiance and film position of the per-sample schedule
+// (sample s of pixel p on Halton index offset(p) + s * stride, dimensions 0
+// and 1 for the film position): L[3*W*H*spp], uv[2*W*H*spp], pixel-major.
 void orc_volpath_samples(const float* grid, int32_t nx, int32_t ny, int32_t nz, const float* bounds6,
-                         const float* medium3, const float* lights7, const float* cam10, int32_t W, int32_t H,
+                         const float* medium3, const float* lights11, const float* cam10, int32_t W, int32_t H,
                          int32_t spp, int32_t max_depth, float* L, float* uv) {
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
   Halton hs((uint64_t)spp, W, H);
@@ -305,7 +300,10 @@ void orc_volpath_samples(const float* grid, int32_t nx, int32_t ny, int32_t nz, 
         V3 o, d;
         cam.generate(ux, uy, W, H, &o, &d);
         const size_t k = ((size_t)y * W + x) * (size_t)spp + (size_t)s;
-        L[k] = (float)svol_Li(sc, hs, o, d, &lk);
+        const V3 Lk = svol_Li(sc, hs, o, d, &lk);
+        L[3 * k] = Lk.x;
+        L[3 * k + 1] = Lk.y;
+        L[3 * k + 2] = Lk.z;
         uv[2 * k] = ux;
         uv[2 * k + 1] = uy;
       }
@@ -331,17 +329,14 @@ void orc_halton_stateful(int32_t spp, int32_t resx, int32_t resy, int32_t px, in
 // as orc_volpath_render.
 // film = 1: the reference's render loop and film (single-threaded)
 void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t nx, int32_t ny, int32_t nz,
-                             const float* medium3, const float* lights7, const float* cam10, int32_t W, int32_t H,
+                             const float* medium3, const float* lights11, const float* cam10, int32_t W, int32_t H,
                              int32_t spp, int32_t max_depth, float* rgb, int nthreads, uint64_t* lookups,
                              int32_t film) {
   const Lbvh& b = ((Handle*)p)->b;
   const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
@@ -354,17 +349,14 @@ void orc_volpath_mesh_render(void* p, int32_t compat, const float* grid, int32_t
 // orc_volpath_mesh_render (per-sample schedule) on every stride-th pixel
 // only: a bounded sample of a frame for timing; paths_out = path samples run.
 void orc_volpath_mesh_render_strided(void* p, int32_t compat, const float* grid, int32_t nx, int32_t ny,
-                                     int32_t nz, const float* medium3, const float* lights7, const float* cam10,
+                                     int32_t nz, const float* medium3, const float* lights11, const float* cam10,
                                      int32_t W, int32_t H, int32_t spp, int32_t max_depth, int64_t stride,
                                      float* rgb, int nthreads, uint64_t* lookups, uint64_t* paths_out) {
   const Lbvh& b = ((Handle*)p)->b;
   const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
   VolScene sc;
   init_vol_grid(sc, grid, nx, ny, nz, bounds6, medium3);
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.grid.bmin, sc.grid.bmax);
+  set_lights(sc, lights11);
   Camera cam = make_camera(cam10);
   sc.cam = &cam;
   sc.width = W; sc.height = H; sc.spp = spp; sc.max_depth = max_depth;
@@ -385,11 +377,12 @@ void orc_scene_trace(void* const* handles, int32_t n, int32_t compat, const floa
 // Differentiable estimator over this mesh as the interface (arguments as
 // orc_diff_render, the medium on the mesh's bounding box).
 void orc_diff_mesh_render(void* p, int32_t compat, const double* theta, const float* theta_p, int32_t nx,
-                          int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                          int32_t ny, int32_t nz, const float* params4, const float* lights11, const float* cam10,
                           int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
-                          const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups) {
-  diff_mesh_render(((Handle*)p)->b, compat != 0, theta, theta_p, nx, ny, nz, params4, lights7, cam10, W, H, spp,
-                   max_depth, lpath, img, dldpix, grad, galpha, nthreads, lookups);
+                          const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups,
+                          int32_t channel) {
+  diff_mesh_render(((Handle*)p)->b, compat != 0, theta, theta_p, nx, ny, nz, params4, lights11, cam10, W, H, spp,
+                   max_depth, lpath, img, dldpix, grad, galpha, nthreads, lookups, channel);
 }
 
 // ------------------------------------------------------------ path trace ---

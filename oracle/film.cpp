@@ -35,10 +35,11 @@ namespace {
 
 struct FilmAcc {
   int W, H;
-  std::vector<float> I, L;  // the colour channels are equal: one is kept
-  FilmAcc(int w, int h) : W(w), H(h), I((size_t)w * h, 0.0f), L((size_t)w * h, 0.0f) {}
+  std::vector<float> I;
+  std::vector<V3> L;
+  FilmAcc(int w, int h) : W(w), H(h), I((size_t)w * h, 0.0f), L((size_t)w * h, v3(0, 0, 0)) {}
   // Film::addSample (film.hpp:143-161)
-  void add(float ux, float uy, float Lv) {
+  void add(float ux, float uy, V3 Lv) {
     const float px = ux - 0.5f, py = uy - 0.5f;
     int lx = (int)std::ceil(px - 2.0f), ly = (int)std::ceil(py - 2.0f);
     int hx = (int)std::floor(px + 2.0f) + 1, hy = (int)std::floor(py + 2.0f) + 1;
@@ -51,14 +52,16 @@ struct FilmAcc {
         const float w = film_mitchell1d(px - (float)x) * film_mitchell1d(py - (float)y);
         const size_t i = (size_t)y * W + x;
         I[i] += w;
-        L[i] += Lv * w;
+        L[i] = L[i] + Lv * w;
       }
   }
   // Film::commitSamples (film.hpp:163-174) into a zero-initialised buffer
   void commit(float* rgb) const {
     for (size_t i = 0; i < I.size(); ++i) {
-      const float c = I[i] == 0.0f ? 0.0f : L[i] / I[i];
-      rgb[3 * i] = rgb[3 * i + 1] = rgb[3 * i + 2] = c;
+      const V3 c = I[i] == 0.0f ? v3(0, 0, 0) : L[i] / I[i];
+      rgb[3 * i] = c.x;
+      rgb[3 * i + 1] = c.y;
+      rgb[3 * i + 2] = c.z;
     }
   }
 };
@@ -67,8 +70,8 @@ struct FilmAcc {
 
 // The render loop over an integrator's Li(sampler, o, d, lookups).  rgb:
 // W*H*3.  task_L / task_uv (may be NULL): per pixel (x fastest) the spp - 1
-// splatted samples in order, radiance and film position.
-void film_render(const VolScene& sc, const std::function<double(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
+// splatted samples in order, RGB radiance (3 floats) and film position.
+void film_render(const VolScene& sc, const std::function<V3(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
                  float* task_L, float* task_uv, uint64_t* lookups_out, uint64_t* paths_out) {
   const int W = sc.width, H = sc.height, spp = sc.spp;
   const int T = spp - 1;
@@ -91,10 +94,14 @@ void film_render(const VolScene& sc, const std::function<double(Halton&, V3, V3,
             const float ux = (float)x + hs.get1D();
             const float uy = (float)y + hs.get1D();
             sc.cam->generate(ux, uy, W, H, &o, &d);
-            const float L = (float)Li(hs, o, d, &lk);
+            const V3 L = Li(hs, o, d, &lk);
             film.add(ux, uy, L);
             const size_t k = ((size_t)y * W + x) * (size_t)T + (size_t)(s - 1);
-            if (task_L) task_L[k] = L;
+            if (task_L) {
+              task_L[3 * k] = L.x;
+              task_L[3 * k + 1] = L.y;
+              task_L[3 * k + 2] = L.z;
+            }
             if (task_uv) {
               task_uv[2 * k] = ux;
               task_uv[2 * k + 1] = uy;

@@ -55,6 +55,7 @@ static inline V3 cross(V3 a, V3 b) {
 }
 // SquaredNorm<T,3> vector.h:311-317 ; Norm :320-322 ; Normalize :325-327 / :490-493
 static inline float sqnorm(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+static inline float channel(V3 a, int c) { return c == 0 ? a.x : c == 1 ? a.y : a.z; }
 static inline float norm(V3 a) { return std::sqrt(sqnorm(a)); }
 static inline V3 normalize(V3 a) { return a / std::sqrt(sqnorm(a)); }
 // Min / Max vector.h:435-447 (std::min / std::max per component)

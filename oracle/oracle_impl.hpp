@@ -99,17 +99,17 @@ struct Dist1D {  // src/distribution.hpp:12-62
 };
 struct EnvLight {  // InfiniteAreaLight with a ConstTexture, light.cpp:93-159
   bool present = false;
-  float radiance = 0;
+  V3 radiance{};  // the texture's colour, RGB
   V3 center{};
   float radius = 0;
   std::vector<Dist1D> rows;
   Dist1D marg;
-  void build(float radiance, V3 scene_lo, V3 scene_hi);
+  void build(V3 radiance, V3 scene_lo, V3 scene_hi);
 };
 struct PointLight {
   bool present = false;
   V3 pos{};
-  float intensity = 0;
+  V3 intensity{};  // RGB radiant intensity: Li = I / d^2 per channel
 };
 
 struct VolScene {
@@ -128,10 +128,10 @@ bool box_intersect_pbrt(V3 lo, V3 hi, V3 o, V3 d, float tmax, float* t_min, floa
 bool box_intersect(V3 lo, V3 hi, V3 o, V3 d, float* t_min, float* t_max);
 float hg_p(V3 wi, V3 wo, float g);
 V3 hg_sample(V3 wo, float u, float v, float g);
-float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf, V3* p_light);
+V3 env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf, V3* p_light);
 V3 hg_sample_pub(V3 wo, float u, float v, float g);
 float hg_p_pub(V3 wi, V3 wo, float g);
-double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups);
+V3 svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups);
 void init_vol_grid(VolScene& sc, const float* grid, int32_t nx, int32_t ny, int32_t nz,
                    const float* bounds6, const float* medium3);
 // OpenVDBVolume::tr / ::sample for one ray, drawing from hs
@@ -157,14 +157,16 @@ void scene_trace(const std::vector<const Lbvh*>& meshes, bool compat, const floa
                  int32_t* ids, float* ts, int nthreads);
 
 // The differentiable estimator over the interface mesh b (volume_diff.cpp):
-// forward (lpath, img) or, with dldpix, the adjoint (grad, galpha).
+// forward (lpath, img) or, with dldpix, the adjoint (grad, galpha), of one
+// colour channel of the lights.
 void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const float* theta_p, int32_t nx,
-                      int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                      int32_t ny, int32_t nz, const float* params4, const float* lights11, const float* cam10,
                       int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
-                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups);
+                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups,
+                      int32_t channel);
 // the reference's render loop and film (film.cpp)
 float film_mitchell1d(float d);
-void film_render(const VolScene& sc, const std::function<double(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
+void film_render(const VolScene& sc, const std::function<V3(Halton&, V3, V3, uint64_t*)>& Li, float* rgb,
                  float* task_L, float* task_uv, uint64_t* lookups_out, uint64_t* paths_out);
 void svol_render_film(const VolScene& sc, float* rgb, float* task_L, float* task_uv, uint64_t* lookups,
                       uint64_t* paths);

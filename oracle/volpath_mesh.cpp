@@ -50,18 +50,19 @@ float shadow_tr(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 o
   return tr;
 }
 
-// UniformSampleOneLight + EstimateDirect at a volume vertex.
-float one_light(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 p, V3 wo, uint64_t* lookups) {
+// UniformSampleOneLight + EstimateDirect at a volume vertex (Vector3f
+// arithmetic component by component).
+V3 one_light(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 p, V3 wo, uint64_t* lookups) {
   int n_lights = (sc.point.present ? 1 : 0) + (sc.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
+  if (n_lights == 0) return v3(0, 0, 0);
   int light_num = (int)(hs.get1D() * (float)n_lights);
   if (light_num > n_lights - 1) light_num = n_lights - 1;
   float light_pdf = (float)(1.0 / (double)n_lights);
   bool use_point = sc.point.present && light_num == 0;
   float u0 = hs.get1D();
   float u1 = hs.get1D();
-  V3 wi, p_light;
-  float pdf, Li;
+  V3 wi, p_light, Li;
+  float pdf;
   if (use_point) {
     V3 diff = sc.point.pos - p;
     float d2 = sqnorm(diff);
@@ -72,8 +73,8 @@ float one_light(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 p
   } else {
     Li = env_sample_li(sc.env, p, u0, u1, &wi, &pdf, &p_light);
   }
-  float L = 0.0f;
-  if (pdf == 0.0f || Li == 0.0f) return L / light_pdf;
+  V3 L = v3(0, 0, 0);
+  if (pdf == 0.0f || (Li.x == 0.0f && Li.y == 0.0f && Li.z == 0.0f)) return L / light_pdf;
   float f = hg_p(wi, wo, sc.med.g);
   if (f != 0.0f) {
     // VInteraction::SpawnRayTo: no normal, so no offset; tMax = norm - SHADOW_EPS
@@ -81,29 +82,30 @@ float one_light(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 p
     float nrm = norm(diff);
     V3 d = diff / nrm;
     float tr = shadow_tr(b, compat, sc, hs, p, d, nrm - SHADOW_EPS, true, lookups);
-    Li *= tr;
-    L += f * Li / pdf;
+    Li = Li * tr;
+    L = L + (Li * f) / pdf;
   }
   return L / light_pdf;
 }
 
 // VolPathIntegrator::Li for one camera ray.
-float volpath_Li(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
-  float L = 0.0f, beta = 1.0f;
+V3 volpath_Li(const Lbvh& b, bool compat, const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
+  V3 L = v3(0, 0, 0);
+  float beta = 1.0f;
   bool inside = false;  // the camera is outside the mesh
   int bounces = 0, crossings = 0;
   while (true) {
     float t = 0;
     V3 ng{};
     const bool hit = lbvh_trace(b, o, d, INFINITY, compat, &t, &ng, nullptr) >= 0;
-    if (bounces == 0 && !hit && sc.env.present) L += beta * sc.env.radiance;  // Le of the environment
+    if (bounces == 0 && !hit && sc.env.present) L = L + sc.env.radiance * beta;  // Le of the environment
     if (!hit || bounces >= sc.max_depth) break;
     bool success = false;
     V3 pos{};
     if (inside) beta *= vol_sample_lk(sc, hs, o, d, t, &success, &pos, lookups);
     if (success) {
       const V3 wo = -d;
-      L += beta * one_light(b, compat, sc, hs, pos, wo, lookups);
+      L = L + one_light(b, compat, sc, hs, pos, wo, lookups) * beta;
       const float u = hs.get1D();
       const float v = hs.get1D();
       const V3 wi = hg_sample(wo, u, v, sc.med.g);
@@ -141,17 +143,19 @@ void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* 
         const int64_t pix = next.fetch_add(1) * sc.pix_stride;
         if (pix >= npix) break;
         const int x = (int)(pix % W), y = (int)(pix / W);
-        float color = 0.0f;
+        V3 color = v3(0, 0, 0);
         for (int s = 0; s < sc.spp; ++s) {
           hs.start(x, y, (uint64_t)s);
           const float ux = hs.get1D();
           const float uy = hs.get1D();
           V3 o, d;
           sc.cam->generate((float)x + ux, (float)y + uy, W, H, &o, &d);
-          color += volpath_Li(b, compat, sc, hs, o, d, &lk[t]);
+          color = color + volpath_Li(b, compat, sc, hs, o, d, &lk[t]);
         }
         color = color / (float)sc.spp;
-        rgb[3 * pix] = rgb[3 * pix + 1] = rgb[3 * pix + 2] = color;
+        rgb[3 * pix] = color.x;
+        rgb[3 * pix + 1] = color.y;
+        rgb[3 * pix + 2] = color.z;
       }
     });
   for (auto& x : th) x.join();
@@ -164,7 +168,7 @@ void volpath_mesh_render(const Lbvh& b, bool compat, const VolScene& sc, float* 
 
 void volpath_mesh_render_film(const Lbvh& b, bool compat, const VolScene& sc, float* rgb, uint64_t* lookups) {
   film_render(
-      sc, [&](Halton& hs, V3 o, V3 d, uint64_t* lk) { return (double)volpath_Li(b, compat, sc, hs, o, d, lk); }, rgb,
+      sc, [&](Halton& hs, V3 o, V3 d, uint64_t* lk) { return volpath_Li(b, compat, sc, hs, o, d, lk); }, rgb,
       nullptr, nullptr, lookups, nullptr);
 }
 

@@ -256,12 +256,12 @@ float Dist1D::sample_c(float u, float* pdf, int* off) const {
 
 // InfiniteAreaLight(texture) light.cpp:93-108 with ConstTexture (texture.cpp:52-55)
 // and preprocess light.cpp:110-117 (scene bound = the volume box).
-void EnvLight::build(float rad, V3 lo, V3 hi) {
+void EnvLight::build(V3 rad, V3 lo, V3 hi) {
   present = true;
   radiance = rad;
   const int NU = 512, NV = 512;
   std::vector<float> fv(NU * NV, 0.0f);
-  float nrm = norm(v3(rad, rad, rad));
+  float nrm = norm(rad);  // m_tex->eval(u, v).norm()
   for (int v = 0; v < NV; ++v) {
     float sin_theta = std::sin(PI * (float)(v + 0.5f) / (float)NV);
     for (int u = 0; u < NU; ++u) {
@@ -283,7 +283,7 @@ void EnvLight::build(float rad, V3 lo, V3 hi) {
 }
 
 // Light::sampleLi for the environment, light.cpp:119-145
-float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf,
+V3 env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, float* pdf,
                            V3* p_light) {
   float pdfs[2];
   int offs[2];
@@ -306,10 +306,11 @@ float env_sample_li(const EnvLight& env, V3 ref, float u0, float u1, V3* wi, flo
 
 // EstimateDirect integrator.cpp:69-107 for a volume vertex, VolEstimateTr
 // reduced to the medium's transmittance (no surfaces in the scene);
-// UniformSampleOneLight integrator.cpp:109-120.
-static float sample_one_light(const VolScene& sc, Halton& hs, V3 p, V3 wo, uint64_t* lookups) {
+// UniformSampleOneLight integrator.cpp:109-120.  Vector3f arithmetic
+// (vector.inc): products and quotients component by component.
+static V3 sample_one_light(const VolScene& sc, Halton& hs, V3 p, V3 wo, uint64_t* lookups) {
   int n_lights = (sc.point.present ? 1 : 0) + (sc.env.present ? 1 : 0);
-  if (n_lights == 0) return 0.0f;
+  if (n_lights == 0) return v3(0, 0, 0);
   int light_num = (int)(hs.get1D() * (float)n_lights);
   if (light_num > n_lights - 1) light_num = n_lights - 1;
   float light_pdf = (float)(1.0 / (double)n_lights);
@@ -317,8 +318,8 @@ static float sample_one_light(const VolScene& sc, Halton& hs, V3 p, V3 wo, uint6
   // EstimateDirect: u_light drawn before sampleLi
   float u0 = hs.get1D();
   float u1 = hs.get1D();
-  V3 wi, p_light;
-  float pdf, Li;
+  V3 wi, p_light, Li;
+  float pdf;
   if (use_point) {
     V3 diff = sc.point.pos - p;
     float d2 = sqnorm(diff);
@@ -329,8 +330,8 @@ static float sample_one_light(const VolScene& sc, Halton& hs, V3 p, V3 wo, uint6
   } else {
     Li = env_sample_li(sc.env, p, u0, u1, &wi, &pdf, &p_light);
   }
-  float L = 0.0f;
-  if (pdf == 0.0f || Li == 0.0f) return L / light_pdf;
+  V3 L = v3(0, 0, 0);
+  if (pdf == 0.0f || (Li.x == 0.0f && Li.y == 0.0f && Li.z == 0.0f)) return L / light_pdf;
   float f = hg_p(wi, wo, sc.med.g);
   if (f != 0.0f) {
     // Interaction::SpawnRayTo interaction.hpp:33-39 (ng = 0: no offset)
@@ -338,15 +339,16 @@ static float sample_one_light(const VolScene& sc, Halton& hs, V3 p, V3 wo, uint6
     float nrm = norm(diff);
     V3 d = diff / nrm;
     float tr = vol_tr(sc, hs, p, d, nrm - SHADOW_EPS, lookups);
-    Li *= tr;
-    L += f * Li / pdf;
+    Li = Li * tr;
+    L = L + (Li * f) / pdf;
   }
   return L / light_pdf;
 }
 
 // SVolIntegrator::Li integrator.cpp:328-405 (m_rrThreshold = 0.01)
-double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
-  float L = 0.0f, beta = 1.0f;
+V3 svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
+  V3 L = v3(0, 0, 0);
+  float beta = 1.0f;
   bool in_volume = false;
   const float rr = 0.01f;
   for (int bounces = 0;; ++bounces) {
@@ -354,12 +356,12 @@ double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
       float t_min, t_max;
       if (box_intersect(sc.grid.bmin, sc.grid.bmax, o, d, &t_min, &t_max)) {
         float tr = vol_tr(sc, hs, o, d, INFINITY, lookups);
-        if (sc.env.present) L += tr * sc.env.radiance;
+        if (sc.env.present) L = L + sc.env.radiance * tr;
         in_volume = true;
         o = o + d * t_min;
         continue;
       } else {
-        if (sc.env.present) L += beta * sc.env.radiance;
+        if (sc.env.present) L = L + sc.env.radiance * beta;
         break;
       }
     }
@@ -373,7 +375,7 @@ double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
         float u = hs.get1D();
         float v = hs.get1D();
         V3 wi = hg_sample(wo, u, v, sc.med.g);
-        L += beta * sample_one_light(sc, hs, pos, wo, lookups);
+        L = L + sample_one_light(sc, hs, pos, wo, lookups) * beta;
         // beta.isZero(): SquaredNorm == 0 (vector.h:268-270)
         if (bounces >= sc.max_depth || beta * beta + beta * beta + beta * beta == 0.0f) break;
         o = pos;  // VInteraction::SpawnRay: ng = 0, origin unchanged
@@ -387,7 +389,7 @@ double svol_Li(const VolScene& sc, Halton& hs, V3 o, V3 d, uint64_t* lookups) {
       beta /= (1.0f - rr);
     }
   }
-  return (double)L;
+  return L;
 }
 
 void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups_out,
@@ -405,17 +407,19 @@ void svol_render(const VolScene& sc, float* rgb, int nthreads, uint64_t* lookups
         int64_t pix = next.fetch_add(1) * sc.pix_stride;
         if (pix >= npix) break;
         int x = (int)(pix % W), y = (int)(pix / W);
-        float color = 0.0f;
+        V3 color = v3(0, 0, 0);
         for (int s = 0; s < sc.spp; ++s) {
           hs.start(x, y, (uint64_t)s);
           float ux = hs.get1D();
           float uy = hs.get1D();
           V3 o, d;
           sc.cam->generate((float)x + ux, (float)y + uy, W, H, &o, &d);
-          color += (float)svol_Li(sc, hs, o, d, &lk[t]);
+          color = color + svol_Li(sc, hs, o, d, &lk[t]);
         }
         color = color / (float)sc.spp;
-        rgb[3 * pix] = rgb[3 * pix + 1] = rgb[3 * pix + 2] = color;
+        rgb[3 * pix] = color.x;
+        rgb[3 * pix + 1] = color.y;
+        rgb[3 * pix + 2] = color.z;
       }
     });
   }

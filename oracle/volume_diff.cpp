@@ -59,6 +59,7 @@ struct DiffScene {
   double s, alpha, mu;
   EnvLight env;
   PointLight point;
+  int ch = 0;  // the colour channel whose radiance this run computes
   Camera* cam;
   int W, H, spp, max_depth;
   int64_t pix_stride = 1;  // pixels 0, stride, 2 stride, ... (1: all)
@@ -177,8 +178,8 @@ NeeSample nee_sample(const DiffScene& sc, Halton& hs, V3 p, V3 wo) {
   bool use_point = sc.point.present && light_num == 0;
   float u0 = hs.get1D();
   float u1 = hs.get1D();
-  V3 wi, p_light;
-  float pdf, Li;
+  V3 wi, p_light, Li;
+  float pdf;
   if (use_point) {
     V3 diff = sc.point.pos - p;
     float d2 = sqnorm(diff);
@@ -189,10 +190,11 @@ NeeSample nee_sample(const DiffScene& sc, Halton& hs, V3 p, V3 wo) {
   } else {
     Li = env_sample_li(sc.env, p, u0, u1, &wi, &pdf, &p_light);
   }
-  if (pdf == 0.0f || Li == 0.0f) return r;
+  // a light sample counts when Li is non-zero in some channel
+  if (pdf == 0.0f || (Li.x == 0.0f && Li.y == 0.0f && Li.z == 0.0f)) return r;
   float f = hg_p(wi, wo, sc.g_f);
   if (f == 0.0f) return r;
-  r.K = (double)f * (double)Li / (double)pdf / light_pdf;
+  r.K = (double)f * (double)channel(Li, sc.ch) / (double)pdf / light_pdf;
   r.valid = true;
   r.p_light = p_light;
   return r;
@@ -245,7 +247,7 @@ void apply_events(const DiffScene& sc, const std::vector<Event>& ev, double L, d
 double diff_path(const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d, double delta,
                  double L_total, double* grad, double* galpha, uint64_t* lookups) {
   double L = 0.0;
-  const double Le = sc.env.present ? (double)sc.env.radiance : 0.0;
+  const double Le = sc.env.present ? (double)channel(sc.env.radiance, sc.ch) : 0.0;
   float t_min, t_max;
   if (!box_intersect(sc.gp.bmin, sc.gp.bmax, o, d, &t_min, &t_max)) return Le;
   std::vector<Tri> rec;
@@ -322,7 +324,7 @@ double mesh_ratio_track(const Lbvh& b, bool compat, const DiffScene& sc, uint32_
 double diff_path_mesh(const Lbvh& b, bool compat, const DiffScene& sc, Halton& hs, uint32_t path, V3 o, V3 d,
                       double delta, double L_total, double* grad, double* galpha, uint64_t* lookups) {
   double L = 0.0, beta = 1.0;
-  const double Le = sc.env.present ? (double)sc.env.radiance : 0.0;
+  const double Le = sc.env.present ? (double)channel(sc.env.radiance, sc.ch) : 0.0;
   bool inside = false;  // the camera is outside the mesh
   int bounces = 0, crossings = 0;
   std::vector<Tri> rec;
@@ -374,8 +376,8 @@ double diff_path_mesh(const Lbvh& b, bool compat, const DiffScene& sc, Halton& h
 }
 
 void setup_scene(DiffScene& sc, Camera& cam, const double* theta, const float* theta_p, int32_t nx, int32_t ny,
-                 int32_t nz, const float* bounds6, const float* params4, const float* lights7, int32_t W, int32_t H,
-                 int32_t spp, int32_t max_depth) {
+                 int32_t nz, const float* bounds6, const float* params4, const float* lights11, int32_t W, int32_t H,
+                 int32_t spp, int32_t max_depth, int32_t ch) {
   sc.gp.nx = nx; sc.gp.ny = ny; sc.gp.nz = nz;
   sc.gp.data = theta_p;
   sc.gp.bmin = v3(bounds6[0], bounds6[1], bounds6[2]);
@@ -390,10 +392,11 @@ void setup_scene(DiffScene& sc, Camera& cam, const double* theta, const float* t
   sc.s = sc.s_f;
   sc.alpha = sc.alpha_f;
   sc.mu = sc.mu_f;
-  sc.point.present = lights7[0] != 0.0f;
-  sc.point.pos = v3(lights7[1], lights7[2], lights7[3]);
-  sc.point.intensity = lights7[4];
-  if (lights7[5] != 0.0f) sc.env.build(lights7[6], sc.gp.bmin, sc.gp.bmax);
+  sc.point.present = lights11[0] != 0.0f;
+  sc.point.pos = v3(lights11[1], lights11[2], lights11[3]);
+  sc.point.intensity = v3(lights11[4], lights11[5], lights11[6]);
+  if (lights11[7] != 0.0f) sc.env.build(v3(lights11[8], lights11[9], lights11[10]), sc.gp.bmin, sc.gp.bmax);
+  sc.ch = ch < 0 ? 0 : ch > 2 ? 2 : ch;
   sc.cam = &cam;
   sc.W = W; sc.H = H; sc.spp = spp; sc.max_depth = max_depth;
 }
@@ -462,14 +465,15 @@ void run_paths(const DiffScene& sc, PathFn path_fn, double* lpath, double* img, 
 
 // The estimator over the interface mesh b (the medium on b's bounding box).
 void diff_mesh_render(const Lbvh& b, bool compat, const double* theta, const float* theta_p, int32_t nx,
-                      int32_t ny, int32_t nz, const float* params4, const float* lights7, const float* cam10,
+                      int32_t ny, int32_t nz, const float* params4, const float* lights11, const float* cam10,
                       int32_t W, int32_t H, int32_t spp, int32_t max_depth, double* lpath, double* img,
-                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups) {
+                      const double* dldpix, double* grad, double* galpha, int nthreads, uint64_t* lookups,
+                      int32_t channel) {
   const float bounds6[6] = {b.glo.x, b.glo.y, b.glo.z, b.ghi.x, b.ghi.y, b.ghi.z};
   DiffScene sc;
   Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
              v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
-  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights11, W, H, spp, max_depth, channel);
   run_paths(
       sc,
       [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {
@@ -485,21 +489,24 @@ using namespace orc;
 extern "C" {
 
 // theta: evaluated grid (double, nx*ny*nz, x fastest); theta_p: proposal
-// grid (float).  params: s, alpha, g, mu.  lights as orc_volpath_render.
+// grid (float).  params: s, alpha, g, mu.  lights11 as orc_volpath_render
+// (RGB); channel: the colour channel (0, 1, 2) whose radiance the call
+// computes (the estimator is linear in the lights' radiance and no decision
+// depends on it, so the three channels are three such calls).
 // Outputs: lpath[W*H*spp] (double, may be NULL), img[W*H] (double mean over
 // samples, may be NULL).  With dldpix != NULL the adjoint runs instead:
 // grad (nx*ny*nz doubles, overwritten) and galpha receive
 // sum_pixels dldpix[pixel] * dI[pixel]/dtheta (dI = mean over samples),
 // using lpath (from a previous forward call) as the paths' totals.
 void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int32_t ny, int32_t nz,
-                     const float* bounds6, const float* params4, const float* lights7,
+                     const float* bounds6, const float* params4, const float* lights11,
                      const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
                      double* lpath, double* img, const double* dldpix, double* grad,
-                     double* galpha, int nthreads, uint64_t* lookups) {
+                     double* galpha, int nthreads, uint64_t* lookups, int32_t channel) {
   DiffScene sc;
   Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
              v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
-  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights11, W, H, spp, max_depth, channel);
   run_paths(
       sc,
       [&](Halton& hs, uint32_t p, V3 o, V3 d, double delta, double Lt, double* g, double* ga, uint64_t* lk) {
@@ -512,14 +519,14 @@ void orc_diff_render(const double* theta, const float* theta_p, int32_t nx, int3
 // baseline's bounded sample): lpath and img entries of other pixels are left
 // untouched; the adjoint's gradient covers the sampled pixels.
 void orc_diff_render_strided(const double* theta, const float* theta_p, int32_t nx, int32_t ny, int32_t nz,
-                             const float* bounds6, const float* params4, const float* lights7,
+                             const float* bounds6, const float* params4, const float* lights11,
                              const float* cam10, int32_t W, int32_t H, int32_t spp, int32_t max_depth,
                              int64_t stride, double* lpath, double* img, const double* dldpix, double* grad,
-                             double* galpha, int nthreads, uint64_t* lookups) {
+                             double* galpha, int nthreads, uint64_t* lookups, int32_t channel) {
   DiffScene sc;
   Camera cam(v3(cam10[0], cam10[1], cam10[2]), v3(cam10[3], cam10[4], cam10[5]),
              v3(cam10[6], cam10[7], cam10[8]), cam10[9]);
-  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights7, W, H, spp, max_depth);
+  setup_scene(sc, cam, theta, theta_p, nx, ny, nz, bounds6, params4, lights11, W, H, spp, max_depth, channel);
   sc.pix_stride = stride < 1 ? 1 : stride;
   run_paths(
       sc,

@@ -598,9 +598,9 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
   fledge::GPUVolLights lights;
   lights.has_point = true;
   lights.point_pos = Vector3f(2.0f, 2.0f, -2.0f);
-  lights.point_intensity = 50.0f;
+  lights.point_intensity = Vector3f(50.0f, 30.0f, 12.5f);
   lights.has_env = true;
-  lights.env_radiance = 0.5f;
+  lights.env_radiance = Vector3f(0.5f, 0.35f, 0.8f);
   fledge::GPUVolIntegrator integ(depth, lights);
   fledge::Film film(W, H);
   fledge::Scene scene;
@@ -623,8 +623,8 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
   orc::init_vol_grid(sc, grid.data(), n, n, n, bounds, medium);
   sc.point.present = true;
   sc.point.pos = orc::v3(2, 2, -2);
-  sc.point.intensity = 50.0f;
-  sc.env.build(0.5f, sc.grid.bmin, sc.grid.bmax);
+  sc.point.intensity = orc::v3(50.0f, 30.0f, 12.5f);
+  sc.env.build(orc::v3(0.5f, 0.35f, 0.8f), sc.grid.bmin, sc.grid.bmax);
   orc::Camera cam(orc::v3(0.5f, 0.5f, -1.5f), orc::v3(0.5f, 0.5f, 0.0f), orc::v3(0, 1, 0), 45.0f);
   sc.cam = &cam;
   sc.width = W;
@@ -637,11 +637,12 @@ void test_vol_integrator(fledge::gpu::Context& ctx) {
   orc::svol_render_film(sc, rgb.data(), nullptr, nullptr, &lk, &paths);
   double worst = 0;
   for (int y = 0; y < H; ++y)
-    for (int x = 0; x < W; ++x) {
-      const Vector3f& c = film.getBuffer(x, y, fledge::EFilmBufferType::EColor);
-      const float r = rgb[(size_t(y) * W + x) * 3];
-      worst = std::max(worst, std::fabs(double(c[0]) - r) / std::max(std::fabs(double(r)), 1e-6));
-    }
+    for (int x = 0; x < W; ++x)
+      for (int k = 0; k < 3; ++k) {
+        const Vector3f& c = film.getBuffer(x, y, fledge::EFilmBufferType::EColor);
+        const float r = rgb[(size_t(y) * W + x) * 3 + k];
+        worst = std::max(worst, std::fabs(double(c[k]) - r) / std::max(std::fabs(double(r)), 1e-6));
+      }
   CHECK(worst <= 1e-4, "film vs oracle: worst relative error %.3g", worst);
   CHECK(integ.lastLookups() == lk, "lookup counts %llu vs oracle %llu",
         (unsigned long long)integ.lastLookups(), (unsigned long long)lk);
@@ -670,9 +671,9 @@ void test_hip_render(fledge::gpu::Context& ctx) {
   fledge::GPUVolLights lights;
   lights.has_point = true;
   lights.point_pos = Vector3f(2.0f, 2.0f, -2.0f);
-  lights.point_intensity = 50.0f;
+  lights.point_intensity = Vector3f(50.0f, 30.0f, 12.5f);
   lights.has_env = true;
-  lights.env_radiance = 0.5f;
+  lights.env_radiance = Vector3f(0.5f, 0.35f, 0.8f);
   fledge::Film film(W, H);
   fledge::Scene scene;
   scene.m_resX = W;
@@ -696,7 +697,8 @@ void test_hip_render(fledge::gpu::Context& ctx) {
     s2.m_film = &ref;
     integ.render(s2);
     for (int i = 0; i < W * H; ++i)
-      CHECK(bits(film.m_color[i][0]) == bits(ref.m_color[i][0]), "volume-only pixel %d", i);
+      for (int k = 0; k < 3; ++k)
+        CHECK(bits(film.m_color[i][k]) == bits(ref.m_color[i][k]), "volume-only pixel %d channel %d", i, k);
     CHECK(r.lastLookups() == integ.lastLookups(), "lookup counts");
   }
   // a closed interface mesh around the medium
@@ -721,8 +723,8 @@ void test_hip_render(fledge::gpu::Context& ctx) {
   orc::init_vol_grid(sc, grid.data(), n, n, n, mb, medium);
   sc.point.present = true;
   sc.point.pos = orc::v3(2, 2, -2);
-  sc.point.intensity = 50.0f;
-  sc.env.build(0.5f, sc.grid.bmin, sc.grid.bmax);
+  sc.point.intensity = orc::v3(50.0f, 30.0f, 12.5f);
+  sc.env.build(orc::v3(0.5f, 0.35f, 0.8f), sc.grid.bmin, sc.grid.bmax);
   orc::Camera cam(orc::v3(0.5f, 0.5f, -1.5f), orc::v3(0.5f, 0.5f, 0.0f), orc::v3(0, 1, 0), 45.0f);
   sc.cam = &cam;
   sc.width = W;
@@ -733,9 +735,9 @@ void test_hip_render(fledge::gpu::Context& ctx) {
   uint64_t lk = 0;
   orc::volpath_mesh_render_film(b, false, sc, rgb.data(), &lk);
   double worst = 0;  // the film sums in another order than the oracle's splats
-  for (int i = 0; i < W * H; ++i)
-    worst = std::max(worst, std::fabs(double(film.m_color[i][0]) - rgb[3 * i]) /
-                                std::max(std::fabs(double(rgb[3 * i])), 1e-6));
+  for (int i = 0; i < 3 * W * H; ++i)
+    worst = std::max(worst, std::fabs(double(film.m_color[i / 3][i % 3]) - rgb[i]) /
+                                std::max(std::fabs(double(rgb[i])), 1e-6));
   CHECK(worst <= 1e-5, "mesh + medium film vs oracle: worst relative error %.3g", worst);
   CHECK(r.lastLookups() == lk, "mesh + medium lookup counts");
   const std::string out = "/tmp/fledge_dropin_" + std::to_string((long)getpid()) + "_render.exr";

@@ -1,5 +1,6 @@
 """ctypes wrapper of the CPU oracle (oracle/build/liboracle.so) — test
 infrastructure only."""
+import collections
 import ctypes
 import os
 
@@ -44,7 +45,7 @@ _SIGS = {
     "orc_volpath_render_strided": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, i64, P,
                                           ctypes.c_int, P, P]),
     "orc_diff_render_strided": (None, [P, P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, i64, P, P, P, P, P,
-                                       ctypes.c_int, P]),
+                                       ctypes.c_int, P, i32]),
     "orc_diffuse_sample": (None, [P, P, P, i64, P, P]),
     "orc_shading_frame": (None, [P, i64, P]),
     "orc_volpath_render_film": (None, [P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P]),
@@ -57,7 +58,7 @@ _SIGS = {
     "orc_halton_stateful": (None, [i32, i32, i32, i32, i32, i32, P]),
     "orc_scene_trace": (None, [P, i32, i32, P, i64, P, P, P, ctypes.c_int]),
     "orc_diff_mesh_render": (None, [P, i32, P, P, i32, i32, i32, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
-                                    ctypes.c_int, P]),
+                                    ctypes.c_int, P, i32]),
 }
 
 _lib = None
@@ -80,8 +81,30 @@ def lib():
     return _lib
 
 
+# Small arrays made inside a call's argument list (bounds, medium, lights,
+# camera) would be freed as soon as their address is taken: the last few are
+# kept alive here until the call has run.
+_alive = collections.deque(maxlen=64)
+
+
 def _p(a):
-    return a.ctypes.data if a is not None else None
+    if a is None:
+        return None
+    if a.nbytes <= 4096:
+        _alive.append(a)
+    return a.ctypes.data
+
+
+def lights11(lights):
+    """The oracle's light array: point present, position (3), intensity (RGB),
+    environment present, radiance (RGB).  A 7-entry grey form (point present,
+    position, intensity, environment present, radiance) is widened."""
+    a = np.asarray(lights, np.float32).ravel()
+    if a.size == 7:
+        a = np.array([a[0], a[1], a[2], a[3], a[4], a[4], a[4], a[5], a[6], a[6], a[6]], np.float32)
+    if a.size != 11:
+        raise ValueError("lights: 7 (grey) or 11 (RGB) entries")
+    return a
 
 
 class Lbvh:
@@ -144,7 +167,7 @@ class Lbvh:
         rgb = np.zeros((H, W, 3), np.float32)
         lk = np.zeros(1, np.uint64)
         lib().orc_volpath_mesh_render(self.h, 1 if compat else 0, _p(grid), nx, ny, nz,
-                                      _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                                      _p(np.asarray(medium, np.float32)), _p(lights11(lights)),
                                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
                                       nthreads or NTHREADS, _p(lk), film)
         return rgb, int(lk[0])
@@ -158,16 +181,16 @@ class Lbvh:
         lk = np.zeros(1, np.uint64)
         pa = np.zeros(1, np.uint64)
         lib().orc_volpath_mesh_render_strided(self.h, 0, _p(grid), nx, ny, nz, _p(np.asarray(medium, np.float32)),
-                                              _p(np.asarray(lights, np.float32)), _p(np.asarray(cam10, np.float32)),
+                                              _p(lights11(lights)), _p(np.asarray(cam10, np.float32)),
                                               W, H, spp, max_depth, stride, _p(rgb), nthreads or NTHREADS, _p(lk),
                                               _p(pa))
         return rgb, int(lk[0]), int(pa[0])
 
     def diff_render(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, compat=False,
-                    nthreads=None):
+                    nthreads=None, channel=0):
         """FP64 differentiable forward over this mesh as the interface around
-        the medium (on the mesh's bounding box): (per-path radiance, per-pixel
-        mean, lookups)."""
+        the medium (on the mesh's bounding box), colour channel `channel` of
+        the lights: (per-path radiance, per-pixel mean, lookups)."""
         theta = np.ascontiguousarray(theta, np.float64)
         theta_p = np.ascontiguousarray(theta_p, np.float32)
         nz, ny, nx = theta_p.shape
@@ -175,15 +198,15 @@ class Lbvh:
         img = np.zeros(W * H, np.float64)
         lk = np.zeros(1, np.uint64)
         lib().orc_diff_mesh_render(self.h, 1 if compat else 0, _p(theta), _p(theta_p), nx, ny, nz,
-                                   _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                   _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                                    _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), _p(img),
-                                   None, None, None, nthreads or NTHREADS, _p(lk))
+                                   None, None, None, nthreads or NTHREADS, _p(lk), channel)
         return lp, img.reshape(H, W), int(lk[0])
 
     def diff_grad(self, theta, theta_p, params, lights, cam10, W, H, spp, max_depth, lpath, dldpix,
-                  compat=False, nthreads=None):
-        """FP64 analytic gradient of sum(dldpix * image) over this mesh: (dense
-        gradient (nz, ny, nx), d/dalbedo)."""
+                  compat=False, nthreads=None, channel=0):
+        """FP64 analytic gradient of sum(dldpix * image) over this mesh, image
+        = colour channel `channel`: (dense gradient (nz, ny, nx), d/dalbedo)."""
         theta = np.ascontiguousarray(theta, np.float64)
         theta_p = np.ascontiguousarray(theta_p, np.float32)
         nz, ny, nx = theta_p.shape
@@ -192,9 +215,9 @@ class Lbvh:
         dl = np.ascontiguousarray(dldpix, np.float64).ravel()
         lp = np.ascontiguousarray(lpath, np.float64)
         lib().orc_diff_mesh_render(self.h, 1 if compat else 0, _p(theta), _p(theta_p), nx, ny, nz,
-                                   _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                   _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                                    _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), None,
-                                   _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None)
+                                   _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None, channel)
         return grad.reshape(nz, ny, nx), float(ga[0])
 
     def __del__(self):
@@ -243,39 +266,40 @@ def volpath_render(grid, bounds, medium, lights, cam10, W, H, spp, max_depth, nt
     lk = np.zeros(1, np.uint64)
     pa = np.zeros(1, np.uint64)
     lib().orc_volpath_render(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                             _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                             _p(np.asarray(medium, np.float32)), _p(lights11(lights)),
                              _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb),
                              nthreads or NTHREADS, _p(lk), _p(pa))
     return rgb, int(lk[0]), int(pa[0])
 
 
 def volpath_render_film(grid, bounds, medium, lights, cam10, W, H, spp, max_depth):
-    """The reference's render loop and film: (rgb (H, W, 3), per-task radiance
-    (H*W, spp-1), per-task film position (H*W, spp-1, 2), lookups)."""
+    """The reference's render loop and film: (rgb (H, W, 3), per-task RGB
+    radiance (H*W, spp-1, 3), per-task film position (H*W, spp-1, 2),
+    lookups)."""
     grid = np.ascontiguousarray(grid, np.float32)
     nz, ny, nx = grid.shape
     rgb = np.zeros((H, W, 3), np.float32)
     T = max(spp - 1, 0)
-    tl = np.zeros((H * W, max(T, 1)), np.float32)
+    tl = np.zeros((H * W, max(T, 1), 3), np.float32)
     tu = np.zeros((H * W, max(T, 1), 2), np.float32)
     lk = np.zeros(1, np.uint64)
     pa = np.zeros(1, np.uint64)
     lib().orc_volpath_render_film(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                                  _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(medium, np.float32)), _p(lights11(lights)),
                                   _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(rgb), _p(tl),
                                   _p(tu), _p(lk), _p(pa))
     return rgb, tl[:, :T], tu[:, :T], int(lk[0])
 
 
 def volpath_samples(grid, bounds, medium, lights, cam10, W, H, spp, max_depth):
-    """Per-sample radiance (H*W, spp) and film position (H*W, spp, 2) of the
-    per-sample schedule."""
+    """Per-sample RGB radiance (H*W, spp, 3) and film position (H*W, spp, 2)
+    of the per-sample schedule."""
     grid = np.ascontiguousarray(grid, np.float32)
     nz, ny, nx = grid.shape
-    L = np.zeros((H * W, spp), np.float32)
+    L = np.zeros((H * W, spp, 3), np.float32)
     uv = np.zeros((H * W, spp, 2), np.float32)
     lib().orc_volpath_samples(_p(grid), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                              _p(np.asarray(medium, np.float32)), _p(np.asarray(lights, np.float32)),
+                              _p(np.asarray(medium, np.float32)), _p(lights11(lights)),
                               _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(L), _p(uv))
     return L, uv
 
@@ -287,8 +311,9 @@ def halton_stateful(spp, resx, resy, px, py, ndraw):
 
 
 def diff_render(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, bounds=(0, 0, 0, 1, 1, 1),
-                nthreads=None):
-    """FP64 differentiable forward: (per-path radiance, per-pixel mean)."""
+                nthreads=None, channel=0):
+    """FP64 differentiable forward of colour channel `channel` of the lights:
+    (per-path radiance, per-pixel mean)."""
     L = lib()
     _diff_sig(L)
     theta = np.ascontiguousarray(theta, np.float64)
@@ -297,15 +322,16 @@ def diff_render(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, bou
     lp = np.zeros(W * H * spp, np.float64)
     img = np.zeros(W * H, np.float64)
     L.orc_diff_render(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                      _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                      _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), _p(img), None,
-                      None, None, nthreads or NTHREADS, None)
+                      None, None, nthreads or NTHREADS, None, channel)
     return lp, img.reshape(H, W)
 
 
 def diff_grad(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, lpath, dldpix,
-              bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
-    """FP64 analytic gradient of sum(dldpix * image) w.r.t. theta and albedo."""
+              bounds=(0, 0, 0, 1, 1, 1), nthreads=None, channel=0):
+    """FP64 analytic gradient of sum(dldpix * image) w.r.t. theta and albedo,
+    image = colour channel `channel` (lpath: that channel's forward)."""
     L = lib()
     _diff_sig(L)
     theta = np.ascontiguousarray(theta, np.float64)
@@ -316,9 +342,9 @@ def diff_grad(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, lpath
     dl = np.ascontiguousarray(dldpix, np.float64).ravel()
     lp = np.ascontiguousarray(lpath, np.float64)
     L.orc_diff_render(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                      _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                      _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                       _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, _p(lp), None, _p(dl),
-                      _p(grad), _p(ga), nthreads or NTHREADS, None)
+                      _p(grad), _p(ga), nthreads or NTHREADS, None, channel)
     return grad.reshape(nz, ny, nx), float(ga[0])
 
 
@@ -327,7 +353,7 @@ def _diff_sig(L):
     if f.argtypes is None:
         f.restype = None
         f.argtypes = [P, P, i32, i32, i32, P, P, P, P, i32, i32, i32, i32, P, P, P, P, P,
-                      ctypes.c_int, P]
+                      ctypes.c_int, P, i32]
 
 
 def logf(x):
@@ -360,7 +386,7 @@ def halton_get(resx, resy, spp, px, py, sample, n):
 
 
 def diff_render_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, stride,
-                        bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
+                        bounds=(0, 0, 0, 1, 1, 1), nthreads=None, channel=0):
     """diff_render over the pixels 0, stride, 2 stride, ... only: (per-path
     radiance, per-pixel mean); entries of other pixels stay 0."""
     theta = np.ascontiguousarray(theta, np.float64)
@@ -369,14 +395,14 @@ def diff_render_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_de
     lp = np.zeros(W * H * spp, np.float64)
     img = np.zeros(W * H, np.float64)
     lib().orc_diff_render_strided(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                                  _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                                   _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, stride, _p(lp),
-                                  _p(img), None, None, None, nthreads or NTHREADS, None)
+                                  _p(img), None, None, None, nthreads or NTHREADS, None, channel)
     return lp, img.reshape(H, W)
 
 
 def diff_grad_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_depth, stride, lpath, dldpix,
-                      bounds=(0, 0, 0, 1, 1, 1), nthreads=None):
+                      bounds=(0, 0, 0, 1, 1, 1), nthreads=None, channel=0):
     """diff_grad over the pixels 0, stride, 2 stride, ... only."""
     theta = np.ascontiguousarray(theta, np.float64)
     theta_p = np.ascontiguousarray(theta_p, np.float32)
@@ -386,7 +412,7 @@ def diff_grad_strided(theta, theta_p, params, lights, cam10, W, H, spp, max_dept
     dl = np.ascontiguousarray(dldpix, np.float64).ravel()
     lp = np.ascontiguousarray(lpath, np.float64)
     lib().orc_diff_render_strided(_p(theta), _p(theta_p), nx, ny, nz, _p(np.asarray(bounds, np.float32)),
-                                  _p(np.asarray(params, np.float32)), _p(np.asarray(lights, np.float32)),
+                                  _p(np.asarray(params, np.float32)), _p(lights11(lights)),
                                   _p(np.asarray(cam10, np.float32)), W, H, spp, max_depth, stride, _p(lp), None,
-                                  _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None)
+                                  _p(dl), _p(grad), _p(ga), nthreads or NTHREADS, None, channel)
     return grad.reshape(nz, ny, nx), float(ga[0])

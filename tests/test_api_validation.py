@@ -106,7 +106,7 @@ def test_differentiable_wrappers_refuse_bad_buffers(monkeypatch):
     dv = _obj(diff.DiffVolume, ctx=ctx, vol=vol, proposal=None, g=0.3, bounds=(0, 0, 0, 1, 1, 1))
     W, H, spp = 4, 2, 3
     img = torch.zeros((H, W, 3))
-    lp = torch.zeros(W * H * spp)
+    lp = torch.zeros((W * H * spp, 3))
     _raises(lambda: diff.forward(dv, None, W, H, spp, 4, 8.0, 0.8, 4.0, out=img, lpath=lp[:-1]), "lpath")
     _raises(lambda: diff.forward(dv, None, W, H, spp, 4, 8.0, 0.8, 4.0, out=img[:, :2], lpath=lp), "out")
     _raises(lambda: diff.forward(dv, None, W, H, spp, 4, 8.0, 0.8, 4.0, out=img, lpath=lp,

@@ -16,6 +16,14 @@ LIGHTS = [1.0, 2.0, 2.0, -2.0, 50.0, 1.0, 0.5]
 S, ALBEDO, G = 8.0, 0.8, 0.3
 
 
+
+def _grey(lpath):
+    """Per-path radiance of grey lights as float64: one channel, after
+    checking that the three are equal."""
+    a = lpath.cpu().numpy()
+    assert np.array_equal(a[:, 0], a[:, 1]) and np.array_equal(a[:, 0], a[:, 2])
+    return a[:, 0].astype(np.float64)
+
 def _setup(ctx, grid, W, H, spp, depth, mu):
     dv = diff.DiffVolume(ctx, torch.from_numpy(grid).cuda(), g=G)
     cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
@@ -43,7 +51,7 @@ def test_diff_forward_matches_fp64_twin(ctx):
     # per path: float (GPU) against the FP64 twin, relative to the path's value
     # or, for paths far darker than average (products of near-zero ratio
     # tracking factors, where float and double round apart), to the mean path
-    lp = lpath.cpu().numpy().astype(np.float64)
+    lp = _grey(lpath)
     scale = max(float(np.abs(lp64).mean()), 1e-6)
     rel_p = np.abs(lp - lp64) / np.maximum(np.abs(lp64), scale)
     print("per-path: max %.3g at %d (gpu %.9g, fp64 %.9g), mean path %.4g" % (
@@ -229,3 +237,42 @@ def test_walk_tape_matches_replay(ctx, schedule):
         assert float(g_ref.abs().max()) > 0
         rel = float((g - g_ref).norm() / g_ref.norm())
         assert rel < 1e-5 and abs(float(ga) - float(ga_ref)) <= 1e-5 * abs(float(ga_ref)), rel
+
+
+LIGHTS_RGB = [1, 2, 2, -2, 50.0, 20.0, 5.0, 1, 0.3, 0.5, 0.7]
+
+
+def test_chromatic_forward_and_adjoint_match_fp64_twin(ctx):
+    """RGB lights: the forward's per-path radiance per channel against the
+    twin run on that channel (1e-4 of the path or the mean path), and the
+    adjoint, which prices each light in the loss's units in one replay,
+    against the sum over channels of the twin's gradients of
+    sum(dl_c * image_c) (1e-3)."""
+    grid = _grid(20, 19)
+    W = H = 40
+    spp, depth = 4, 32
+    mu = S * float(grid.max())
+    dv = diff.DiffVolume(ctx, torch.from_numpy(grid).cuda(), g=G)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    scene = api.vol_scene(cam, intensity=LIGHTS_RGB[4:7], env=LIGHTS_RGB[8:11])
+    img, lpath = diff.forward(dv, scene, W, H, spp, depth, S, ALBEDO, mu)
+    lp = lpath.cpu().numpy().astype(np.float64)
+    im = img.cpu().numpy().astype(np.float64)
+    dl = np.random.default_rng(4).normal(size=(H, W, 3)).astype(np.float32)
+    grad, galpha = diff.adjoint(dv, scene, W, H, spp, depth, S, ALBEDO, mu, lpath, torch.from_numpy(dl).cuda())
+    g64 = 0.0
+    ga64 = 0.0
+    th = grid.astype(np.float64)
+    for c in range(3):
+        lp64, img64 = orc.diff_render(th, grid, [S, ALBEDO, G, mu], LIGHTS_RGB, CAM0, W, H, spp, depth, channel=c)
+        scale = max(float(np.abs(lp64).mean()), 1e-6)
+        assert (np.abs(lp[:, c] - lp64) / np.maximum(np.abs(lp64), scale)).max() < 1e-4
+        assert (np.abs(im[..., c] - img64) / np.maximum(np.abs(img64), 1e-6)).max() < 1e-4
+        g, ga = orc.diff_grad(th, grid, [S, ALBEDO, G, mu], LIGHTS_RGB, CAM0, W, H, spp, depth, lp64,
+                              dl[..., c].astype(np.float64), channel=c)
+        g64 = g64 + g
+        ga64 += ga
+    got = grad.cpu().numpy().astype(np.float64)
+    assert np.abs(g64).max() > 0
+    assert np.linalg.norm(got - g64) / np.linalg.norm(g64) < 1e-3
+    assert abs(float(galpha.item()) - ga64) / abs(ga64) < 1e-3

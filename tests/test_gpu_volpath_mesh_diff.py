@@ -17,6 +17,14 @@ LIGHTS = [1.0, 2.0, 2.0, -2.0, 50.0, 1.0, 0.5]
 S, ALBEDO, G = 8.0, 0.8, 0.3
 
 
+
+def _grey(lpath):
+    """Per-path radiance of grey lights as float64: one channel, after
+    checking that the three are equal."""
+    a = lpath.cpu().numpy()
+    assert np.array_equal(a[:, 0], a[:, 1]) and np.array_equal(a[:, 0], a[:, 2])
+    return a[:, 0].astype(np.float64)
+
 def _sphere(level=4):
     v, ind = api.gen_octasphere(level, 0.05, seed=7, base_freq=2.0)
     return (v * 0.5 + 0.5).astype(np.float32), ind
@@ -57,7 +65,7 @@ def test_mesh_diff_forward_matches_fp64_twin(ctx):
     assert rel.max() < 1e-4
     # per path, relative to the path's value or, for paths far darker than
     # average, to the mean path (float and double products round apart there)
-    lp = lpath.cpu().numpy().astype(np.float64)
+    lp = _grey(lpath)
     scale = max(float(np.abs(lp64).mean()), 1e-6)
     rel_p = np.abs(lp - lp64) / np.maximum(np.abs(lp64), scale)
     assert rel_p.max() < 1e-4, rel_p.max()
@@ -205,7 +213,7 @@ def test_mesh_adjoint_ragged_bricks_and_depth_one(ctx):
     mu = r["mu"]
     m = orc.Lbvh(v, ind)
     lp64, _, _ = m.diff_render(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D)
-    lp = r["lpath"].cpu().numpy().astype(np.float64)
+    lp = _grey(r["lpath"])
     assert np.abs(lp - lp64).max() <= 1e-4 * max(float(np.abs(lp64).max()), 1e-6)
     g64, _ = m.diff_grad(grid.astype(np.float64), grid, [S, ALBEDO, G, mu], LIGHTS, CAM0, W, H, spp, D, lp64,
                          r["dl"].astype(np.float64).sum(axis=2))
@@ -253,3 +261,38 @@ def test_combined_scene_full_size(ctx):
     assert float((dense_b - gd)[act].abs().max()) <= 1e-4 * scale
     assert float(dense_b[~act].abs().max()) == 0.0
     assert abs(float(gab.item()) - float(gad.item())) <= 1e-4 * abs(float(gad.item()))
+
+
+LIGHTS_RGB = [1, 2, 2, -2, 50.0, 20.0, 5.0, 1, 0.3, 0.5, 0.7]
+
+
+def test_mesh_chromatic_forward_and_adjoint_match_fp64_twin(ctx):
+    """RGB lights over the interface mesh: per-path radiance per channel
+    against the twin run on that channel (1e-4), the one-replay adjoint
+    against the sum over channels of the twin's gradients (1e-3)."""
+    v, ind = _sphere(4)
+    grid = _grid(20, 23)
+    W, H, spp, D = 32, 24, 4, 16
+    mu = S * float(grid.max())
+    bvh, dv, _ = _setup(ctx, v, ind, grid)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    scene = api.vol_scene(cam, intensity=LIGHTS_RGB[4:7], env=LIGHTS_RGB[8:11])
+    img, lpath = diff.mesh_forward(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu)
+    dl = np.random.default_rng(5).normal(size=(H, W, 3)).astype(np.float32)
+    grad, ga = diff.mesh_adjoint(dv, bvh, scene, W, H, spp, D, S, ALBEDO, mu, lpath, torch.from_numpy(dl).cuda())
+    lp = lpath.cpu().numpy().astype(np.float64)
+    m = orc.Lbvh(v, ind)
+    th = grid.astype(np.float64)
+    g64, ga64 = 0.0, 0.0
+    for c in range(3):
+        lp64, _, _ = m.diff_render(th, grid, [S, ALBEDO, G, mu], LIGHTS_RGB, CAM0, W, H, spp, D, channel=c)
+        scale = max(float(np.abs(lp64).mean()), 1e-6)
+        assert (np.abs(lp[:, c] - lp64) / np.maximum(np.abs(lp64), scale)).max() < 1e-4
+        g, a = m.diff_grad(th, grid, [S, ALBEDO, G, mu], LIGHTS_RGB, CAM0, W, H, spp, D, lp64,
+                           dl[..., c].astype(np.float64), channel=c)
+        g64 = g64 + g
+        ga64 += a
+    got = grad.cpu().numpy().astype(np.float64)
+    assert np.abs(g64).max() > 0
+    assert np.linalg.norm(got - g64) / np.linalg.norm(g64) < 1e-3
+    assert abs(float(ga.item()) - ga64) / abs(ga64) < 1e-3

@@ -235,10 +235,52 @@ def test_volpath_halton_indices_beyond_2pow24(ctx):
     stride = 37
     L.orc_volpath_render_strided(orc._p(grid), 16, 16, 16, orc._p(np.array([0, 0, 0, 1, 1, 1], np.float32)),
                                  orc._p(np.array([6.4, 1.6, 0.3], np.float32)),
-                                 orc._p(np.array([1, 2, 2, -2, 50.0, 1, 0.5], np.float32)),
+                                 orc._p(orc.lights11([1, 2, 2, -2, 50.0, 1, 0.5])),
                                  orc._p(np.array(CAM0, np.float32)), W, H, spp, depth, stride, orc._p(rgb), 8,
                                  orc._p(lk), orc._p(pa))
     sel = np.arange(0, W * H, stride)
     assert int(pa[0]) == sel.size * spp
     ref = rgb.reshape(-1, 3)[sel]
     assert np.array_equal(out[sel].view(np.uint32), ref.view(np.uint32))
+
+
+LIGHTS_RGB = [1, 2, 2, -2, 50.0, 20.0, 5.0, 1, 0.3, 0.5, 0.7]
+
+
+@pytest.mark.parametrize("schedule,film", [(1, 0), (2, 0), (3, 0), (1, 1), (2, 1), (3, 1)])
+def test_volpath_chromatic_lights_match_oracle(ctx, schedule, film):
+    """RGB lights (point (50, 20, 5), environment (0.3, 0.5, 0.7)): every
+    channel equals the oracle's Vector3f restatement, bit for bit per sample
+    schedule, to the film's summation order in film mode; same lookups."""
+    print("volpath chromatic schedule %d film %d" % (schedule, film), flush=True)
+    grid = api.gen_fbm_grid(32, seed=14)
+    W, H, spp = 45, 37, 5
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    sc = api.vol_scene(cam, intensity=LIGHTS_RGB[4:7], env=LIGHTS_RGB[8:11])
+    out, st = vol.render(sc, W, H, spp, 32, stats=True, schedule=schedule, film=film)
+    got = out.cpu().numpy()
+    if film:
+        ref, _, _, lk = orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTS_RGB, CAM0, W, H,
+                                                spp, 32)
+        rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-6)
+        assert rel.max() <= 1e-5, rel.max()
+    else:
+        ref, lk, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTS_RGB, CAM0, W, H, spp, 32)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    assert st["lookups"] == lk
+    assert not np.array_equal(got[..., 0], got[..., 1]) and not np.array_equal(got[..., 1], got[..., 2])
+
+
+def test_volpath_channels_scale_exactly(ctx):
+    """Lights whose channels differ by powers of two give channels that differ
+    by exactly those powers (float operations commute with them)."""
+    grid = api.gen_fbm_grid(32, seed=15)
+    vol = api.Volume(ctx, torch.from_numpy(grid).cuda(), [0, 0, 0, 1, 1, 1], 6.4, 1.6, 0.3)
+    cam = N.Camera.make(CAM0[0:3], CAM0[3:6], CAM0[6:9], CAM0[9])
+    sc = api.vol_scene(cam, intensity=(64.0, 32.0, 16.0), env=(0.5, 0.25, 0.125))
+    for schedule in (1, 2, 3):
+        got = vol.render(sc, 40, 32, 4, 32, schedule=schedule).cpu().numpy()
+        assert got[..., 0].max() > 0
+        assert np.array_equal(got[..., 0], got[..., 1] * np.float32(2))
+        assert np.array_equal(got[..., 0], got[..., 2] * np.float32(4))

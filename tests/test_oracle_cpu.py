@@ -580,8 +580,9 @@ def test_film_filter_is_mitchell_with_b_c_zero():
     assert abs(mn(0.5, 1 / 3, 1 / 3) - 0.5) > 1e-2
 
 
-def _film_case(grid, W, H, spp, depth=16):
-    return orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTSF, CAMF, W, H, spp, depth)
+def _film_case(grid, W, H, spp, depth=16, lights=None):
+    return orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], LIGHTSF if lights is None else lights,
+                                   CAMF, W, H, spp, depth)
 
 
 def test_film_schedule_maps_to_per_sample_schedule():
@@ -610,18 +611,20 @@ def test_film_resolve_matches_float64_splat():
     normalisation of addSample / commitSamples)."""
     grid = (api.gen_fbm_grid(16, seed=4)).astype(np.float32)
     W, H, spp = 21, 19, 4
-    rgb, tl, tu, lk = _film_case(grid, W, H, spp)
+    # chromatic lights: every channel splats its own radiance
+    rgb, tl, tu, lk = _film_case(grid, W, H, spp, lights=[1, 2, 2, -2, 50.0, 20.0, 5.0, 1, 0.3, 0.5, 0.7])
     f = lambda d: np.where(np.abs(d) < 1, 2 * np.abs(d) ** 3 - 3 * d * d + 1, 0.0)
     uv = tu.reshape(-1, 2).astype(np.float64)
-    Ls = tl.reshape(-1).astype(np.float64)
-    out = np.zeros((H, W))
-    for y in range(H):
-        for x in range(W):
-            w = f(uv[:, 0] - 0.5 - x) * f(uv[:, 1] - 0.5 - y)
-            out[y, x] = (w * Ls).sum() / w.sum() if w.sum() > 0 else 0.0
-    got = rgb[:, :, 0].astype(np.float64)
-    assert np.allclose(got, out, rtol=2e-6, atol=1e-7)
-    assert np.array_equal(rgb[:, :, 0], rgb[:, :, 1]) and np.array_equal(rgb[:, :, 0], rgb[:, :, 2])
+    for c in range(3):
+        Ls = tl[..., c].reshape(-1).astype(np.float64)
+        out = np.zeros((H, W))
+        for y in range(H):
+            for x in range(W):
+                w = f(uv[:, 0] - 0.5 - x) * f(uv[:, 1] - 0.5 - y)
+                out[y, x] = (w * Ls).sum() / w.sum() if w.sum() > 0 else 0.0
+        got = rgb[:, :, c].astype(np.float64)
+        assert np.allclose(got, out, rtol=2e-6, atol=1e-7)
+    assert not np.array_equal(rgb[:, :, 0], rgb[:, :, 1])
 
 
 def test_film_constant_radiance_and_single_sample():

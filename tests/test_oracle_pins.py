@@ -9,6 +9,7 @@ light pick checked by additivity over the lights.  CPU only."""
 import math
 
 import numpy as np
+import pytest
 
 import oracle_lib as orc
 from fledge_amd import api
@@ -270,9 +271,11 @@ def test_constant_grid_density_model():
 
 
 def _per_sample(lights, W, H, spp, medium, n):
+    """Per-sample radiance of grey lights: one channel (the three are equal)."""
     grid = np.ones((n, n, n), np.float32)
     L, _ = orc.volpath_samples(grid, [0, 0, 0, 1, 1, 1], medium, lights, CAM0, W, H, spp, 1)
-    return L.astype(np.float64)
+    assert np.array_equal(L[..., 0], L[..., 1]) and np.array_equal(L[..., 0], L[..., 2])
+    return L[..., 0].astype(np.float64)
 
 
 def test_point_light_single_scatter_matches_quadrature():
@@ -316,3 +319,57 @@ def test_light_pick_is_additive_over_lights():
     assert abs(lhs - rhs) < 4 * sd, (lhs, rhs, sd)
     scattered = pt.mean() + env.mean() - env0.mean()
     assert scattered > 12 * sd  # a halved scattered part would be far outside
+
+
+# ------------------------------------------------------ RGB light colours --
+# The integrators carry RGB radiance and every channel runs the reference's
+# Vector3f arithmetic component by component (vector.inc).  Powers of two
+# commute with every float operation away from underflow and overflow, so
+# lights whose channels differ by powers of two give channels that differ by
+# exactly those powers; a grey render of one channel's colour equals that
+# channel; the environment's tables depend on the colour's norm only, so
+# colours that permute the same exact squares give permuted channels.  A
+# kernel or oracle that mixes channels, drops one, or prices every channel
+# with the first one's radiance fails these.
+def _rgb_render(lights, W=24, H=16, spp=4, depth=8, film=False):
+    grid = (api.gen_fbm_grid(16, seed=21) * np.float32(0.9)).astype(np.float32)
+    if film:
+        rgb, _, _, _ = orc.volpath_render_film(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], lights, CAM0, W, H,
+                                               spp, depth)
+    else:
+        rgb, _, _ = orc.volpath_render(grid, [0, 0, 0, 1, 1, 1], [6.4, 1.6, 0.3], lights, CAM0, W, H, spp, depth)
+    return rgb
+
+
+def _mesh_rgb_render(lights, W=20, H=16, spp=4, depth=8):
+    c = np.array([[x, y, z] for z in (0.0, 1.0) for y in (0.0, 1.0) for x in (0.0, 1.0)], np.float32)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    tris = [t for a, b, cc, d in quads for t in ((a, b, cc), (a, cc, d))]
+    m = orc.Lbvh(c.ravel(), np.array(tris, np.int32).ravel())
+    grid = (api.gen_fbm_grid(16, seed=22) * np.float32(0.9)).astype(np.float32)
+    rgb, _ = m.volpath_mesh(grid, [6.4, 1.6, 0.3], lights, CAM0, W, H, spp, depth)
+    return rgb
+
+
+@pytest.mark.parametrize("kind", ["svol", "svol_film", "mesh"])
+def test_rgb_channels_scale_exactly(kind):
+    lights = [1, 2, 2, -2, 64.0, 32.0, 16.0, 1, 0.5, 0.25, 0.125]
+    rgb = (_rgb_render(lights, film=kind == "svol_film") if kind != "mesh" else _mesh_rgb_render(lights))
+    assert rgb[..., 0].max() > 0
+    assert np.array_equal(rgb[..., 0], rgb[..., 1] * np.float32(2))
+    assert np.array_equal(rgb[..., 0], rgb[..., 2] * np.float32(4))
+
+
+def test_rgb_point_channel_equals_grey_render():
+    rgb = _rgb_render([1, 2, 2, -2, 50.0, 20.0, 5.0, 0, 0, 0, 0])
+    for c, i in enumerate((50.0, 20.0, 5.0)):
+        grey = _rgb_render([1, 2, 2, -2, i, 0, 0])
+        assert np.array_equal(rgb[..., c], grey[..., 0])
+
+
+def test_rgb_environment_channels_permute():
+    a = _rgb_render([0, 0, 0, 0, 0, 0, 0, 1, 0.5, 0.25, 0.125])
+    b = _rgb_render([0, 0, 0, 0, 0, 0, 0, 1, 0.125, 0.5, 0.25])
+    assert a[..., 0].max() > 0
+    for c in range(3):
+        assert np.array_equal(a[..., c], b[..., (c + 1) % 3])
