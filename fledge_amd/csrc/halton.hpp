@@ -42,31 +42,51 @@ void release_halton_tables(flg_ctx* ctx);
 HaltonParams make_halton_params(int resx, int resy);
 constexpr uint64_t kMaxHaltonIndex = 1ull << 32;
 
-// The digit loop on the double unit.  r = 1/b (1 + d) comes from the float
-// reciprocal r0 = (1 + e)/b, |e| <= 2^-24, by two Newton steps: 1 - b r0 = -e
-// exactly (b r0 has at most 42 significant bits), r1 = r0 (1 - e) rounded
-// has |d1| <= 2^-48 + 2^-53, and the second step leaves |d| <= 2^-53 plus
-// terms below 2^-95.  For a < 2^32 and a prime b < 2^18 the quotient is
-// q = floor(a r + 2^-19): a/b < 2^31 moves by at most 2^31 |d| <= 2^-22 and
-// the sum's rounding (below 2^31) by 2^-22, together below the bias, so an
-// exact multiple stays in (q, q + 1); otherwise a/b - q lies in
-// [1/b, 1 - 1/b] with 1/b > 2^-18, which the bias and errors cannot cross.
-// The remainder a - q b and the reversed digits (below b a < 2^50) are exact
-// in double fma.  The result is the reference's: (float) of the same
-// integer, times the same float product chain of 1 / base.
+// The digit loop on the double unit, two digits per trip.  r = 1/b (1 + d)
+// comes from the float reciprocal r0 = (1 + e)/b, |e| <= 2^-24, by two Newton
+// steps: 1 - b r0 = -e exactly (b r0 has at most 42 significant bits),
+// r1 = r0 (1 - e) rounded has |d1| <= 2^-48 + 2^-53, and the second step
+// leaves |d| <= 2^-53 plus terms below 2^-95.  For a < 2^32 and a prime
+// b < 2^18:
+//  * one digit: q = floor(a r + 2^-19) = floor(a / b).  a/b < 2^31 moves by
+//    at most 2^31 |d| <= 2^-22 and the sum's rounding by 2^-22, together
+//    below the bias, so an exact multiple stays in (q, q + 1); otherwise
+//    a/b - q lies in [1/b, 1 - 1/b] with 1/b > 2^-18, which the bias and the
+//    errors cannot cross;
+//  * two digits: with B = b^2 (exact) and rB = r r rounded = (1 + D)/B,
+//    |D| <= 3 2^-53, q = floor(a rB + 2^-17 rB) = floor(a / B): a/B < 2^32/B
+//    moves by at most 2^-19.4/B and the rounding by 2^-21/B, together below
+//    the bias 2^-17 rB > 2^-17.01/B, and bias plus errors stay below the
+//    1/B that separates a non-multiple from the next integer.
+// The remainders, the two digits of a mod B (one-digit step), the reversed
+// pair d0 b + d1 and the reversed digits (below b a < 2^50) are exact in
+// double fma.  The result is the reference's: (float) of the same integer,
+// times the same float product chain of 1 / base.  Two digits per trip halve
+// the trips of a wave whose lanes draw dimensions of different bases.
 FLG_HD float radical_inverse(int32_t base, float inv_base, uint32_t a) {
   const double bd = (double)base;
   const double r0 = (double)inv_base;
   const double r1 = __builtin_fma(r0, __builtin_fma(-bd, r0, 1.0), r0);
   const double r = __builtin_fma(r1, __builtin_fma(-bd, r1, 1.0), r1);
+  const double B = bd * bd;
+  const double rB = r * r;
+  const double beta = 0x1p-17 * rB;
   double ad = (double)a;
   double reversed = 0.0;
   float inv_base_n = 1.0f;
-  while (ad != 0.0) {
-    const double q = floor(__builtin_fma(ad, r, 0x1p-19));
-    reversed = __builtin_fma(reversed, bd, __builtin_fma(-q, bd, ad));
+  while (ad >= bd) {  // at least two digits: the lowest two
+    const double q = floor(__builtin_fma(ad, rB, beta));
+    const double lo = __builtin_fma(-q, B, ad);
+    const double d1 = floor(__builtin_fma(lo, r, 0x1p-19));
+    const double d0 = __builtin_fma(-d1, bd, lo);
+    reversed = __builtin_fma(reversed, B, __builtin_fma(d0, bd, d1));
+    inv_base_n *= inv_base;
     inv_base_n *= inv_base;
     ad = q;
+  }
+  if (ad != 0.0) {  // the last digit
+    reversed = __builtin_fma(reversed, bd, ad);
+    inv_base_n *= inv_base;
   }
   return (float)reversed * inv_base_n;
 }
@@ -82,6 +102,35 @@ FLG_D float halton_get1d(const HaltonTables& t, uint32_t index, uint32_t dim, in
   uint32_t a = dim == 0 ? index / (uint32_t)bs0 : (dim == 1 ? index / (uint32_t)bs1 : index);
   int32_t p;
   float ib;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // On the device no vector load from global memory feeds the entry: the
+  // staged entries are read as LDS, and a dimension beyond them (or every
+  // dimension when nothing is staged) comes from scalar loads, one distinct
+  // dimension of the wave at a time.  A vector global load here would share
+  // its registers with the LDS read at the join, and the compiler then waits
+  // for all of the lane's outstanding global loads (the density lookups in
+  // flight) before every draw.
+  if (dim < (uint32_t)t.nlocal) {
+    const int2 e = ((const __attribute__((address_space(3))) int2*)t.local)[dim];
+    p = e.x;
+    ib = u2f((uint32_t)e.y);
+  } else {
+    typedef const __attribute__((address_space(4))) int32_t ConstI32;
+    typedef const __attribute__((address_space(4))) float ConstF32;
+    p = 0;
+    ib = 0.0f;
+    bool done = false;
+    while (!done) {
+      const uint32_t d = __builtin_amdgcn_readfirstlane(dim);
+      const int32_t pd = __builtin_amdgcn_readfirstlane(((ConstI32*)t.prime)[d]);
+      const int32_t id = __builtin_amdgcn_readfirstlane((int32_t)f2u(((ConstF32*)t.inv)[d]));
+      const bool mine = dim == d;
+      p = mine ? pd : p;
+      ib = mine ? u2f((uint32_t)id) : ib;
+      done = done || mine;
+    }
+  }
+#else
   if (dim < (uint32_t)t.nlocal) {
     const int2 e = t.local[dim];
     p = e.x;
@@ -90,6 +139,7 @@ FLG_D float halton_get1d(const HaltonTables& t, uint32_t index, uint32_t dim, in
     p = t.prime[dim];
     ib = t.inv[dim];
   }
+#endif
   float v = radical_inverse(p, ib, a);
   return smin(v, ONE_MINUS_EPSILON);
 }

@@ -264,7 +264,14 @@ __global__ __launch_bounds__(kMeshBlock, kMeshWaves) void k_volpath(TraceGeom g,
   if (p < ntasks) {
     VPath s;
     vp_task_begin<FILM>(tab, hp, cam, tasks, (uint32_t)p, s);
-    while (!vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, cnt) || (FILM && !vp_film_next(tab, hp, cam, s))) {
+    // one trip per loop iteration; a film task's centre path goes on to
+    // sample 1 (written without a side effect inside the loop condition: the
+    // short-circuit form `!step() || !film_next()` was compiled wrongly for
+    // lanes of one wave that reach the end of the centre path on different
+    // trips, which then traced a path twice and kept the centre radiance)
+    bool more = true;
+    while (more) {
+      if (vp_step<COMPAT, COUNT>(mg, v, L, tab, hp, max_depth, s, cnt)) more = FILM && !vp_film_next(tab, hp, cam, s);
     }
     vp_task_finish<FILM>((uint32_t)p, s, lpath, fpos);
   }

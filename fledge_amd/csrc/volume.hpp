@@ -101,13 +101,22 @@ FLG_HD float trilerp(const float* g0, int sy, int sz, float fx, float fy, float 
   const float* g1 = g0 + sy;
   const float* g2 = g0 + sz;
   const float* g3 = g2 + sy;
+  // all eight loads first: under the kernels' register caps the scheduler
+  // would otherwise load a row, wait, blend it and only then load the next.
+  // The empty asm takes the eight values at once, so the four row loads are
+  // in flight together before the first blend.
+  const float a0 = g0[0], a1 = g0[1], b0 = g1[0], b1 = g1[1];
+  const float e0 = g2[0], e1 = g2[1], h0 = g3[0], h1 = g3[1];
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::"v"(a0), "v"(a1), "v"(b0), "v"(b1), "v"(e0), "v"(e1), "v"(h0), "v"(h1));
+#endif
   float wx0 = 1.0f - fx, wx1 = fx;
   float wy0 = 1.0f - fy, wy1 = fy;
   float wz0 = 1.0f - fz, wz1 = fz;
-  float c00 = g0[0] * wx0 + g0[1] * wx1;
-  float c10 = g1[0] * wx0 + g1[1] * wx1;
-  float c01 = g2[0] * wx0 + g2[1] * wx1;
-  float c11 = g3[0] * wx0 + g3[1] * wx1;
+  float c00 = a0 * wx0 + a1 * wx1;
+  float c10 = b0 * wx0 + b1 * wx1;
+  float c01 = e0 * wx0 + e1 * wx1;
+  float c11 = h0 * wx0 + h1 * wx1;
   float c0 = c00 * wy0 + c10 * wy1;
   float c1 = c01 * wy0 + c11 * wy1;
   return c0 * wz0 + c1 * wz1;

@@ -226,6 +226,16 @@ void check_radical_inverse() {
       }
     for (uint64_t pw = b; pw < top; pw *= b)
       for (int d = -1; d <= 1; ++d) one(p, pw + d);
+    // multiples of b^2 (the two-digit quotient) at the top, halfway and low
+    const uint64_t B = b * b;
+    if (B < top) {
+      const uint64_t kb = (top - 1) / B;
+      for (uint64_t k : {kb, kb - 1, kb / 2 + 1, (uint64_t)1, (uint64_t)2})
+        for (int64_t d : {(int64_t)-1, (int64_t)0, (int64_t)1, (int64_t)b - 1, (int64_t)b, (int64_t)B - 1}) {
+          const int64_t a = (int64_t)(k * B) + d;
+          if (a >= 0 && (uint64_t)a < top) one(p, (uint64_t)a);
+        }
+    }
     for (int i = 0; i < 32; ++i) one(p, U32(gen));
     for (int i = 0; i < 32; ++i) one(p, U24(gen));
   }

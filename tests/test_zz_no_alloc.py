@@ -53,6 +53,13 @@ def test_noalloc_render_matches_render(ctx):
     for (w, h, spp) in ((64, 48, 9), (16, 16, 3)):
         a, b = _both(ctx, vol, scene, w, h, spp, 16, 0, 1)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    # chromatic lights; the environment colour (0.6, 0.0, 0.8) has norm 1, as
+    # the grey 1/sqrt(3) would not, so its table is built by the first call
+    print("noalloc case: chromatic", flush=True)
+    rgb = api.vol_scene(cam, intensity=(50.0, 20.0, 5.0), env=(0.6, 0.0, 0.8))
+    a, b = _both(ctx, vol, rgb, 48, 32, 8, 16)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert not np.array_equal(a[..., 0], a[..., 1])
     vol.close()
 
 
