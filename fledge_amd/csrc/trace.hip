@@ -159,6 +159,7 @@ struct QRay {  // 32 bytes: origin + path id, direction + Halton index
 struct PtState {
   QRay* q[2];
   uint32_t* count;  // [max_depth + 2] queue sizes
+  uint32_t* taken;  // [max_depth + 1] chunks of each bounce's queue taken so far
   uint2* hits;      // split bounces: closest hit (sorted position, t bits) per queued ray
   int32_t* nseg;
   int32_t* seg_id;
@@ -228,6 +229,26 @@ FLG_D bool pt_finish(const TraceGeom& g, const PtState& s, const HaltonTables& t
   return spawn;
 }
 
+// The first bounce takes the camera paths tile by tile: bands of kTileH rows
+// cut into kTileW-wide tiles, row-major inside a tile.  The paths in flight at
+// any moment then cover a compact part of the image, whose rays cross the
+// same nodes, and the next bounce's queue, appended in that order, follows
+// it.  tile_pixel maps the k-th pixel of that order to (x, y), a bijection on
+// the w x h image (the last band and the last tile of a band may be smaller).
+constexpr uint32_t kTileW = 8, kTileH = 64;  // measured best on 1920 x 1080, 16 spp
+FLG_HD void tile_pixel(uint32_t w, uint32_t h, uint32_t k, uint32_t& x, uint32_t& y) {
+  const uint32_t band = k / w / kTileH;  // = k / (w kTileH), without the product's overflow
+  const uint32_t y0 = band * kTileH;
+  const uint32_t rem = k - y0 * w;
+  const uint32_t th = h - y0 < kTileH ? h - y0 : kTileH;
+  const uint32_t tx = rem / (kTileW * th);
+  const uint32_t x0 = tx * kTileW;
+  const uint32_t r2 = rem - x0 * th;
+  const uint32_t tw = w - x0 < kTileW ? w - x0 : kTileW;
+  y = y0 + r2 / tw;
+  x = x0 + r2 % tw;
+}
+
 // Wave-aggregated append of the lanes' new segments to the next queue.
 FLG_D void pt_append(const PtState& s, int bounce, bool spawn, const QRay& nr) {
   const uint64_t m = __ballot(spawn);
@@ -265,9 +286,11 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
       V3 o, d;
       uint32_t path, hidx;
       if (FIRST) {
-        path = i;
-        const uint32_t pix = i / (uint32_t)spp, smp = i - pix * (uint32_t)spp;
-        const int x = (int)(pix % (uint32_t)cam.w), y = (int)(pix / (uint32_t)cam.w);
+        const uint32_t k = i / (uint32_t)spp, smp = i - k * (uint32_t)spp;
+        uint32_t ux0, uy0;
+        tile_pixel((uint32_t)cam.w, (uint32_t)cam.h, k, ux0, uy0);
+        const int x = (int)ux0, y = (int)uy0;
+        path = (uy0 * (uint32_t)cam.w + ux0) * (uint32_t)spp + smp;  // pixel-major, as the outputs
         PathSampler ps;
         ps.index = halton_pixel_offset(hp, x, y) + smp * hp.stride;
         ps.dim = 0;
@@ -302,15 +325,29 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
 }
 
 // Bounces after the first, split in two launches (ray_fetch 2).  The
-// traversal kernel gives each wave a contiguous slice of the queue (the queue
-// length is read on the device); a lane whose ray has ended takes the next
-// entry of its wave's slice whenever enough of the wave's lanes are idle, so
-// a wave no longer idles behind its longest traversal, with no atomic per
-// fetch.  It only writes each ray's closest hit.  The shading kernel then
-// records the segments and spawns the next ones, one lane per ray, with the
-// code of the fused kernel: results are the same bits.
+// traversal kernel hands out the queue (its length is read on the device) in
+// chunks of kChunk rays, in queue order, through one counter per bounce: the
+// rays in flight at any moment then lie close together in the queue, which
+// follows the image, so they cross the same nodes and share the caches.
+// Within a chunk a lane whose ray has ended takes the next entry whenever
+// enough of the wave's lanes are idle, so a wave does not idle behind its
+// longest traversal.  It only writes each ray's closest hit.  The shading
+// kernel then records the segments and spawns the next ones, one lane per
+// ray, with the code of the fused kernel: results are the same bits.
 constexpr int kRefillIdle = 16;   // refill once this many lanes of a wave are idle
-constexpr uint32_t kMinSlice = 64;  // rays per wave at least: short queues keep full waves
+constexpr uint32_t kChunk = 128;  // rays per counter increment
+
+// Takes the wave's next chunk [next, end) of a queue of n rays; false when
+// none is left.  Called by every lane of the wave together.
+FLG_D bool take_chunk(uint32_t* taken, uint32_t n, uint32_t& next, uint32_t& end) {
+  uint32_t c = 0;
+  if ((threadIdx.x & 63) == 0) c = atomicAdd(taken, 1u);
+  c = __builtin_amdgcn_readfirstlane(c);  // a scalar: the chunk bounds stay in SGPRs
+  if (c >= (uint32_t)(((uint64_t)n + kChunk - 1) / kChunk)) return false;
+  next = c * kChunk;
+  end = next + kChunk < n ? next + kChunk : n;
+  return true;
+}
 
 template <bool COMPAT, int BS, int RING>
 __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtState s, int bounce) {
@@ -318,22 +355,19 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtSta
   const uint32_t n = s.count[bounce];
   const QRay* qin = s.q[bounce & 1];
   const int lane = threadIdx.x & 63;
-  const uint32_t nwaves = gridDim.x * (BS / 64);
-  const uint32_t wave = blockIdx.x * (BS / 64) + threadIdx.x / 64;
-  uint32_t per = (uint32_t)(((uint64_t)n + nwaves - 1) / nwaves);
-  per = per < kMinSlice ? kMinSlice : per;
-  const uint64_t lo = (uint64_t)wave * per;
-  const uint32_t end = (uint32_t)(lo + per < n ? lo + per : n);
-  uint32_t next = (uint32_t)(lo < n ? lo : n);  // the same in every lane of the wave
+  uint32_t next = 0, end = 0;  // the wave's current chunk; the same in every lane
+  bool more = true;            // chunks may be left
   uint2 priv[kStack];
   auto st = make_stack<RING>(ring + threadIdx.x, BS, priv);
   Counters c{0, 0};
   TraceRay r;
   uint32_t cur = 0;
   bool have = false, active = false;
-  // every wave leaves once its slice is taken and all its lanes are idle; each
-  // traversal ends (a node is visited at most once)
+  // the counter only grows, so a wave's takes end; every wave leaves once no
+  // chunk is left and all its lanes are idle; each traversal ends (a node is
+  // visited at most once)
   while (true) {
+    if (next >= end && more) more = take_chunk(s.taken + bounce, n, next, end);
     const uint64_t idle = __ballot(!have);
     const uint64_t live = ~idle & __ballot(true);
     if (next < end && idle != 0 && (live == 0 || __popcll(idle) >= kRefillIdle)) {
@@ -410,7 +444,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
 // Bounces from bounce0 on in one persistent launch (ray_fetch 3: from the
 // second segment of every path; ray_fetch 4: from the third, after the
 // second's traversal and shading kernels).  Each wave owns a contiguous slice
-// of bounce0's queue; a lane takes a queued
+// of bounce0's queue (at least kMinSlice rays: the tail's paths are long and
+// scattered, and many small slices spread over the blocks balance them better
+// than chunks taken in queue order); a lane takes a queued
 // ray and follows its path to the end: when its traversal ends it records the
 // segment and samples the diffuse bounce (pt_finish, the code of the other
 // schedules, so the same bits) and traces the next segment itself, with no
@@ -420,6 +456,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
 // slice as in k_pt_trace.  The segments traced after the queue's are counted
 // into count[bounce + 1] for the frame's statistics.
 constexpr int kFinishBatch = 16;
+constexpr uint32_t kMinSlice = 64;  // rays per wave at least: short queues keep full waves
 // The kernel holds a path's whole state besides the traversal: it asks for 4
 // waves per SIMD (128 VGPRs), which keeps the shading out of scratch.
 constexpr int kPathWaves = 4;
@@ -762,7 +799,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     CamDev cd = make_cam(cam, cfg->width, cfg->height);
     const int D = cfg->max_depth;
     size_t qbytes = (size_t)npaths * sizeof(QRay);
-    char* sc = (char*)scratch(ctx, "pt_queues", 2 * qbytes + (size_t)npaths * 8 + 4096, &st);
+    char* sc = (char*)scratch(ctx, "pt_queues", 2 * qbytes + (size_t)npaths * 8 + 4352, &st);
     if (!sc) return st;
     PtState s;
     s.q[0] = (QRay*)sc;
@@ -770,6 +807,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     s.hits = (uint2*)(sc + 2 * qbytes);
     s.count = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8);
     unsigned long long* visits = (unsigned long long*)(sc + 2 * qbytes + (size_t)npaths * 8 + 2048);
+    s.taken = (uint32_t*)(sc + 2 * qbytes + (size_t)npaths * 8 + 4096);  // 64 entries
     const bool count_visits = cfg->count_visits != 0;
     s.nseg = d_nseg;
     s.seg_id = d_seg_id;
@@ -782,6 +820,7 @@ int flg_pathtrace_diffuse(flg_ctx* ctx, const flg_bvh* bvh, const flg_camera* ca
     // s.count[0 .. D+1]: queue sizes
     auto enqueue = [&](hipStream_t stream, bool timed) {
       k_pt_zero<<<1, 256, 0, stream>>>(s.count, 256);
+      k_pt_zero<<<1, 256, 0, stream>>>(s.taken, 64);
       if (count_visits) k_pt_zero<<<1, 256, 0, stream>>>((uint32_t*)visits, 4);
       for (int b = 0; b <= D; ++b) {
         ScopedTimer tt(timed ? ctx : nullptr, "pathtrace_bounce");

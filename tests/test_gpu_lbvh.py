@@ -169,6 +169,25 @@ def test_pathtrace_parity_small(ctx):
     assert nrays > npaths  # some bounces happened
 
 
+@pytest.mark.parametrize("W,H,spp", [(37, 131, 3), (5, 3, 1), (1, 70, 2), (130, 1, 1)])
+def test_pathtrace_ragged_image(ctx, W, H, spp):
+    """Images that the first bounce's 8 x 64 tile order cuts into partial
+    tiles and a partial band (and one-pixel-wide or -high images): every path
+    is rendered once, equal to the oracle."""
+    v, i = api.gen_octasphere(6, 0.1)
+    g, o = _build_both(ctx, v, i)
+    cam10 = [0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0]
+    cam = N.Camera.make(cam10[0:3], cam10[3:6], cam10[6:9], cam10[9])
+    D = 5
+    r = g.pathtrace(cam, W, H, spp, D, record=True)
+    nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, W * H * spp)
+    assert r["segments"] == nrays
+    assert np.array_equal(r["nseg"].cpu().numpy(), nseg)
+    mask = np.arange(D + 1)[None, :] < nseg[:, None]
+    assert np.array_equal(r["seg_id"].cpu().numpy()[mask], ids[mask])
+    assert np.array_equal(r["seg_t"].cpu().numpy()[mask].view(np.uint32), ts[mask].view(np.uint32))
+
+
 def _octa_case(ctx):
     v, i = api.gen_octasphere(6, 0.1)
     g, o = _build_both(ctx, v, i)
