@@ -256,4 +256,36 @@ struct ScopedTimer {
 
 inline int div_up(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
+// Tile order of a w x h image: bands of TH rows cut into TW-wide tiles, left
+// to right (the band's last tile narrower and the last band shorter when w or
+// h is not a multiple), row-major inside a tile.  tile_pixel maps the k-th
+// pixel of that order to (x, y), a bijection on the image.  Kernels that start
+// paths in this order keep the paths in flight on a compact patch of the image
+// instead of a few scanlines; a path's result depends only on its own index,
+// so the order changes which paths run together and not what they compute.
+// All in 32 bits: the band is (k / w) / TH, never k / (w * TH), whose divisor
+// would wrap for large w (the callers' bound w * h * tasks < 2^31 allows
+// w = 2^31 - 1 at h = tasks = 1); band * TH * w <= k after it.
+template <uint32_t TW, uint32_t TH>
+FLG_HD void tile_pixel(uint32_t k, uint32_t w, uint32_t h, uint32_t& x, uint32_t& y) {
+  const uint32_t y0 = k / w / TH * TH;
+  const uint32_t o = k - y0 * w;                    // position within the band
+  const uint32_t rows = TH < h - y0 ? TH : h - y0;  // the band's height
+  const uint32_t x0 = o / (TW * rows) * TW;         // the tile's first column
+  const uint32_t in = o - x0 * rows;                // position within the tile
+  const uint32_t tw = TW < w - x0 ? TW : w - x0;    // the tile's width
+  x = x0 + in % tw;
+  y = y0 + in / tw;
+}
+
+// Path index of the q-th path in tile order, the tasks of one pixel
+// consecutive: a bijection on [0, w * h * tasks).
+template <uint32_t TW = 16, uint32_t TH = 16>
+FLG_HD uint32_t tile_order(uint32_t q, int32_t tasks, int32_t w, int32_t h) {
+  const uint32_t pix = q / (uint32_t)tasks, t = q - pix * (uint32_t)tasks;
+  uint32_t x, y;
+  tile_pixel<TW, TH>(pix, (uint32_t)w, (uint32_t)h, x, y);
+  return (y * (uint32_t)w + x) * (uint32_t)tasks + t;
+}
+
 }  // namespace flg

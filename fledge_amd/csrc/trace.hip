@@ -229,25 +229,13 @@ FLG_D bool pt_finish(const TraceGeom& g, const PtState& s, const HaltonTables& t
   return spawn;
 }
 
-// The first bounce takes the camera paths tile by tile: bands of kTileH rows
-// cut into kTileW-wide tiles, row-major inside a tile.  The paths in flight at
-// any moment then cover a compact part of the image, whose rays cross the
-// same nodes, and the next bounce's queue, appended in that order, follows
-// it.  tile_pixel maps the k-th pixel of that order to (x, y), a bijection on
-// the w x h image (the last band and the last tile of a band may be smaller).
-constexpr uint32_t kTileW = 8, kTileH = 64;  // measured best on 1920 x 1080, 16 spp
-FLG_HD void tile_pixel(uint32_t w, uint32_t h, uint32_t k, uint32_t& x, uint32_t& y) {
-  const uint32_t band = k / w / kTileH;  // = k / (w kTileH), without the product's overflow
-  const uint32_t y0 = band * kTileH;
-  const uint32_t rem = k - y0 * w;
-  const uint32_t th = h - y0 < kTileH ? h - y0 : kTileH;
-  const uint32_t tx = rem / (kTileW * th);
-  const uint32_t x0 = tx * kTileW;
-  const uint32_t r2 = rem - x0 * th;
-  const uint32_t tw = w - x0 < kTileW ? w - x0 : kTileW;
-  y = y0 + r2 / tw;
-  x = x0 + r2 % tw;
-}
+// The first bounce takes the camera paths in tile order (tile_pixel) of 8 x 64
+// tiles: the paths in flight then cover a compact part of the image, whose
+// rays cross the same nodes, and the next bounce's queue, appended in that
+// order, follows it.  Measured on 1920 x 1080 at 16 spp: 10.0 ms per frame
+// against 10.5 ms in scanline order (8 x 32, 16 x 64 and 16 x 256 tiles
+// 10.1-10.3 ms; tiles 128 rows high no faster than scanlines).
+constexpr uint32_t kTileW = 8, kTileH = 64;
 
 // Wave-aggregated append of the lanes' new segments to the next queue.
 FLG_D void pt_append(const PtState& s, int bounce, bool spawn, const QRay& nr) {
@@ -288,7 +276,7 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_bounce(TraceGeom g, PtSt
       if (FIRST) {
         const uint32_t k = i / (uint32_t)spp, smp = i - k * (uint32_t)spp;
         uint32_t ux0, uy0;
-        tile_pixel((uint32_t)cam.w, (uint32_t)cam.h, k, ux0, uy0);
+        tile_pixel<kTileW, kTileH>(k, (uint32_t)cam.w, (uint32_t)cam.h, ux0, uy0);
         const int x = (int)ux0, y = (int)uy0;
         path = (uy0 * (uint32_t)cam.w + ux0) * (uint32_t)spp + smp;  // pixel-major, as the outputs
         PathSampler ps;

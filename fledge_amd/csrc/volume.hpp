@@ -387,31 +387,6 @@ FLG_D void regen_loop(uint32_t npaths, unsigned int* next_path, Begin begin, Ste
   }
 }
 
-// Order in which the mesh kernels' regenerating lanes start paths: claimed
-// index q names path tile_order(q), whose pixels run through 16 x 16 tiles
-// (bands of 16 rows, tiles left to right, the band's last tile narrower and
-// the last band shorter when w or h is not a multiple of 16), the tasks of
-// one pixel consecutive.  A bijection on [0, w * h * tasks); a path's result
-// depends only on its own index, so the order changes which paths run
-// together and not what they compute.  Paths in flight then cover a compact
-// patch of the image instead of a few scanlines: 3.9% faster forward and
-// 3.4% faster adjoint on the 8M-triangle interface-mesh scene at 1920x1080.
-// All in 32 bits: the band is (pix / w) / 16, never pix / (w * 16), whose
-// divisor would wrap for w >= 2^28 (allowed by the callers' bound
-// w * h * tasks < 2^31 at h = tasks = 1); band * 16 * w <= pix after it.
-FLG_HD uint32_t tile_order(uint32_t q, int32_t tasks, int32_t w, int32_t h) {
-  constexpr uint32_t T = 16;
-  const uint32_t pix = q / (uint32_t)tasks, t = q - pix * (uint32_t)tasks;
-  const uint32_t band = pix / (uint32_t)w / T;
-  const uint32_t o = pix - band * T * (uint32_t)w;              // position within the band
-  const uint32_t rows = T < (uint32_t)h - band * T ? T : (uint32_t)h - band * T;  // the band's height
-  const uint32_t col = o / (T * rows);                          // tile within the band
-  const uint32_t in = o - col * T * rows;                       // position within the tile
-  const uint32_t tw = T < (uint32_t)w - col * T ? T : (uint32_t)w - col * T;      // the tile's width
-  const uint32_t x = col * T + in % tw, y = band * T + in / tw;
-  return (y * (uint32_t)w + x) * (uint32_t)tasks + t;
-}
-
 struct EnvDev {
   const float* marg_cdf;  // 513
   const float* marg_f;    // 512

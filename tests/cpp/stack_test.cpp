@@ -124,22 +124,26 @@ void run_case(const char* name, const std::vector<float>& verts, const std::vect
 
 }  // namespace
 
-// tile_order (the mesh kernels' path start order) is a bijection on
+// tile_order (the path start order of the mesh kernels, 16 x 16 tiles, and
+// of the path tracer's first bounce, 8 x 64) is a bijection on
 // [0, w h tasks): exhaustively on small ragged shapes; at widths of 2^27 and
-// above (h = 1: the identity; h = 8: the tile layout checked by hand).
-void check_tile_order() {
-  const int shapes[][3] = {{1, 1, 1}, {17, 33, 3}, {16, 16, 2}, {40, 7, 5}, {3, 100, 1}, {1920, 24, 2}};
+// above (h = 1: the identity; h = 8: the 16 x 16 layout checked by hand; the
+// 8 x 64 layout at 1920 x 1080 checked by hand).
+template <uint32_t TW, uint32_t TH>
+void check_tile_bijection() {
+  const int shapes[][3] = {{1, 1, 1},   {17, 33, 3}, {16, 16, 2}, {40, 7, 5},  {3, 100, 1},
+                           {1920, 24, 2}, {37, 131, 3}, {1, 70, 2}, {130, 1, 1}, {1920, 1080, 1}};
   for (const auto& s : shapes) {
     const uint32_t n = (uint32_t)s[0] * s[1] * s[2];
     std::vector<uint8_t> seen(n, 0);
     bool ok = true;
     for (uint32_t q = 0; q < n; ++q) {
-      const uint32_t p = flg::tile_order(q, s[2], s[0], s[1]);
+      const uint32_t p = flg::tile_order<TW, TH>(q, s[2], s[0], s[1]);
       if (p >= n || seen[p]) ok = false;
       else seen[p] = 1;
     }
     if (!ok) {
-      std::printf("FAIL tile_order not a bijection at %d x %d x %d\n", s[0], s[1], s[2]);
+      std::printf("FAIL tile_order<%u, %u> not a bijection at %d x %d x %d\n", TW, TH, s[0], s[1], s[2]);
       ++failures;
     }
   }
@@ -148,10 +152,31 @@ void check_tile_order() {
     std::uniform_int_distribution<uint32_t> U(0, (uint32_t)w - 1);
     for (int k = 0; k < 100000; ++k) {
       const uint32_t q = U(gen);
-      if (flg::tile_order(q, 1, w, 1) != q) {
-        std::printf("FAIL tile_order at width %d: %u -> %u\n", w, q, flg::tile_order(q, 1, w, 1));
+      if (flg::tile_order<TW, TH>(q, 1, w, 1) != q) {
+        std::printf("FAIL tile_order<%u, %u> at width %d: %u -> %u\n", TW, TH, w, q,
+                    flg::tile_order<TW, TH>(q, 1, w, 1));
         ++failures;
         break;
+      }
+    }
+  }
+}
+
+void check_tile_order() {
+  check_tile_bijection<16, 16>();
+  check_tile_bijection<8, 64>();
+  {  // 1920 x 1080 in 8 x 64 tiles: a band holds 1920 * 64 pixels, a tile 512;
+     // the last band is 1080 - 16 * 64 = 56 rows high, its tiles 448 pixels
+    const int32_t w = 1920, h = 1080;
+    const uint32_t cases[][3] = {{0, 0, 0},           {9, 1, 1},           {511, 7, 63},
+                                 {512 + 8, 8, 1},      {1920 * 64, 0, 64},  {1920 * 64 * 16 + 447, 7, 1079},
+                                 {1920 * 64 * 16 + 448 * 239 + 9, 239 * 8 + 1, 1025}};
+    for (const auto& c : cases) {
+      const uint32_t want = c[2] * (uint32_t)w + c[1];
+      if (flg::tile_order<8, 64>(c[0], 1, w, h) != want) {
+        std::printf("FAIL tile_order<8, 64>(%u) at %d x %d: %u, expected %u\n", c[0], w, h,
+                    flg::tile_order<8, 64>(c[0], 1, w, h), want);
+        ++failures;
       }
     }
   }
