@@ -201,10 +201,12 @@ void check_tile_order() {
 // radical_inverse divides in double with a biased refined reciprocal; its
 // result must equal RadicalInverse_'s integer digit loop (sampler.h:59-72)
 // for every prime below 2^18 and every 32-bit index.  Checked: the exact
-// multiples and their neighbours at the top of the range and halfway (where
-// the rounding argument is tightest), prime powers, random indices below
-// 2^32 and below 2^24 for every prime, and every index below 2^24 for bases
-// 2 and 3.
+// multiples of b, b^2 and b^3 and their neighbours at the top of the range
+// and halfway (where the rounding argument is tightest), with remainders at
+// the edges of the digit splits, prime powers, random indices below 2^32 and
+// below 2^24 for every prime, every index below 2^24 for bases 2, 3, 5 and 7,
+// and 200,000 random 32-bit indices for bases 2, 3 and 5, around 2^17 and at
+// the top of the table.
 float radical_inverse_int(uint32_t base, uint64_t a) {
   const float inv_base = 1.0f / (float)base;
   uint64_t reversed = 0;
@@ -261,11 +263,25 @@ void check_radical_inverse() {
           if (a >= 0 && (uint64_t)a < top) one(p, (uint64_t)a);
         }
     }
+    // multiples of b^3 (two two-digit steps), with remainders at the edges of
+    // the two- and one-digit splits of the low three digits
+    const uint64_t B3 = B * b;
+    if (B3 < top) {
+      const uint64_t kc = (top - 1) / B3;
+      for (uint64_t k : {kc, kc - 1, kc / 2 + 1, (uint64_t)1, (uint64_t)2, (uint64_t)b - 1, (uint64_t)b})
+        for (int64_t d : {(int64_t)-1, (int64_t)0, (int64_t)1, (int64_t)b - 1, (int64_t)b, (int64_t)B - 1,
+                          (int64_t)B, (int64_t)B + 1, (int64_t)(B3 - 1), (int64_t)(B3 - b)}) {
+          const int64_t a = (int64_t)(k * B3) + d;
+          if (a >= 0 && (uint64_t)a < top) one(p, (uint64_t)a);
+        }
+    }
     for (int i = 0; i < 32; ++i) one(p, U32(gen));
     for (int i = 0; i < 32; ++i) one(p, U24(gen));
   }
-  for (int32_t p : {2, 3})
+  for (int32_t p : {2, 3, 5, 7})
     for (uint64_t a = 0; a < (1u << 24); ++a) one(p, a);
+  for (int32_t p : {2, 3, 5, 131071, 131101, 262139})
+    for (int i = 0; i < 200000; ++i) one(p, U32(gen));
   std::printf("radical_inverse: %lld cases checked\n", (long long)checked);
 }
 
