@@ -444,7 +444,10 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
 // slice as in k_pt_trace.  The segments traced after the queue's are counted
 // into count[bounce + 1] for the frame's statistics.
 constexpr int kFinishBatch = 16;
-constexpr uint32_t kMinSlice = 64;  // rays per wave at least: short queues keep full waves
+// rays per wave at least: short queues keep full waves (measured on the 2M-
+// triangle frame's tail: 32 rays 6.53, 64 6.29-6.37, 128 6.19-6.22, 256
+// 6.35-6.36, 512 6.93 ms for the bounces after the first)
+constexpr uint32_t kMinSlice = 128;
 // The kernel holds a path's whole state besides the traversal: it asks for 4
 // waves per SIMD (128 VGPRs), which keeps the shading out of scratch.
 constexpr int kPathWaves = 4;
