@@ -58,7 +58,7 @@ template <bool COMPAT, bool COUNT>
 FLG_D int32_t mesh_hit(const MeshGeom& m, V3 o, V3 d, float tmax, float* t, V3* ng, VpCount& cnt) {
   float best = tmax;
   Counters c{0, 0};
-  const int32_t p = trace_lane<COMPAT, COUNT, kMeshBlock, kMeshRing>(m.g, o, d, best, c, m.ring);
+  const int32_t p = trace_lane<COMPAT, COUNT, kMeshBlock, kMeshRing, true>(m.g, o, d, best, c, m.ring);
   if (COUNT) {
     ++cnt.rays;
     cnt.nodes += c.nodes;
