@@ -337,8 +337,15 @@ FLG_D bool take_chunk(uint32_t* taken, uint32_t n, uint32_t& next, uint32_t& end
   return true;
 }
 
+// The split traversal fetches a node's two leaf triangles together
+// (trace_visit's PAIR), which needs more than 64 registers.  On the 2M-
+// triangle frame the bounces after the first took 5.77-5.83 ms at 7 waves per
+// SIMD (72 registers) against 6.16-6.22 ms at 8 waves without PAIR (6 waves
+// with it: 5.94-5.98 ms; 8 waves with it spilled: 7.67 ms).
+constexpr int kSplitTraceWaves = 7;
+
 template <bool COMPAT, int BS, int RING>
-__global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtState s, int bounce) {
+__global__ __launch_bounds__(BS, kSplitTraceWaves) void k_pt_trace(TraceGeom g, PtState s, int bounce) {
   __shared__ uint2 ring[RING > 0 ? BS * RING : 1];
   const uint32_t n = s.count[bounce];
   const QRay* qin = s.q[bounce & 1];
@@ -376,7 +383,7 @@ __global__ __launch_bounds__(BS, kTraceWaves) void k_pt_trace(TraceGeom g, PtSta
       break;
     }
     if (have) {
-      if (active) active = trace_visit<COMPAT, false>(g, r, st, c);
+      if (active) active = trace_visit<COMPAT, false, true>(g, r, st, c);
       if (!active) {
         s.hits[cur] = make_uint2((uint32_t)r.hit, f2u(r.best));
         have = false;
