@@ -33,9 +33,9 @@ FLG_D StackOf<RING> make_stack(uint2* ring, int stride, uint2* priv) {
   else return PrivStack(priv);
 }
 
-// Traversal is latency-bound: the kernels ask for 8 waves per SIMD (at most 64
-// VGPRs); the few values that no longer fit spill to scratch, which measured
-// faster than the 4-5 waves the unconstrained allocation allows.
+// Traversal is latency-bound: k_trace and k_pt_bounce ask for 8 waves per SIMD
+// (at most 64 VGPRs); the few values that no longer fit spill to scratch, which
+// measured faster than the 4-5 waves the unconstrained allocation allows.
 constexpr int kTraceWaves = 8;
 
 template <bool COMPAT, int BS, int RING>
