@@ -188,6 +188,31 @@ def test_pathtrace_ragged_image(ctx, W, H, spp):
     assert np.array_equal(r["seg_t"].cpu().numpy()[mask].view(np.uint32), ts[mask].view(np.uint32))
 
 
+@pytest.mark.parametrize("fetch", [1, 2, 3, 4])
+def test_pathtrace_empty_and_tiny_queues(ctx, fetch):
+    """Later bounces with no ray at all (the camera looks away from the mesh)
+    and with a handful (a 3 x 2 image): every ray fetch leaves its queues
+    drained and gives the oracle's segments."""
+    v, i = api.gen_octasphere(6, 0.1)
+    g, o = _build_both(ctx, v, i)
+    for cam10, W, H, spp in (([0.0, 0.0, -3.0, 0.0, 0.0, -6.0, 0.0, 1.0, 0.0, 45.0], 33, 17, 2),
+                             ([0.0, 0.0, -3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 45.0], 3, 2, 3)):
+        cam = N.Camera.make(cam10[0:3], cam10[3:6], cam10[6:9], cam10[9])
+        D = 6
+        r = g.pathtrace(cam, W, H, spp, D, record=True, ray_fetch=fetch)
+        nseg, ids, ts, nrays = o.diffuse_render(cam10, W, H, spp, D, 0, W * H * spp)
+        assert r["segments"] == nrays
+        assert np.array_equal(r["nseg"].cpu().numpy(), nseg)
+        mask = np.arange(D + 1)[None, :] < nseg[:, None]
+        assert np.array_equal(r["seg_id"].cpu().numpy()[mask], ids[mask])
+        assert np.array_equal(r["seg_t"].cpu().numpy()[mask].view(np.uint32), ts[mask].view(np.uint32))
+    # the first case: every camera ray misses, so no later bounce has a ray
+    r = g.pathtrace(N.Camera.make([0.0, 0.0, -3.0], [0.0, 0.0, -6.0], [0.0, 1.0, 0.0], 45.0), 33, 17, 2, D,
+                    record=True, ray_fetch=fetch)
+    assert r["segments"] == 33 * 17 * 2
+    assert (r["nseg"] == 1).all()
+
+
 def _octa_case(ctx):
     v, i = api.gen_octasphere(6, 0.1)
     g, o = _build_both(ctx, v, i)
